@@ -1,0 +1,195 @@
+"""Benchmark: FP64 assembly of A + b for vector-P2 linear elasticity on the
+64^3 x 6 unit-cube tetrahedral mesh (1,572,864 cells, 6,440,067 dofs) on one
+MI355X, through the cuDOLFINx-style API.
+
+One step = the timed regions of python/examples/poisson.py:91-114:
+  A region: assemble_matrix (zero + cell integral + Dirichlet rows/cols +
+            diagonal) + handoff (A.assemble(), a device synchronisation);
+  b region: assemble_vector (cells + traction facets) + apply_lifting + set_bc.
+Setup (mesh, dofmaps, sparsity, owner-computes schedule, hipRTC compile) is
+outside the timed region.  Inputs live in HBM before timing starts.
+
+Printed: one JSON line with DoF/s (value), the roofline of the dominant
+kernel (owner-computes matrix kernel, timed live with HIP events on the
+stream it runs on) and a CPU baseline (the oracle, one core, on a bounded
+sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path[:0] = [str(ROOT / "cuda-dolfinx_amd"), str(ROOT)]
+
+METRIC = "assembly DoF/s + % HBM roofline; mixed-prec vs FP64 speedup at rel err≤1e-6"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(P, A):
+    """Algorithmic HBM bytes of one matrix launch (owner-computes kernel).
+
+    values written once (8 nnz) + blocked CSR structure read once
+    (4 nnz_b + 4 (N_b + 1)) + geometry (24 Nv + 16 Nc) + packed Lame
+    coefficients (16 Nc) + blocked dofmap (4 nd Nc).
+    """
+    nc, nv = P.mesh.num_cells, P.mesh.num_vertices
+    nb = P.V.num_nodes
+    nnz = A.nnz
+    nnz_b = nnz // (P.V.bs * P.V.bs)
+    nd = P.V.element.ndofs
+    return 8 * nnz + 4 * nnz_b + 4 * (nb + 1) + 24 * nv + 16 * nc + 16 * nc + 4 * nd * nc
+
+
+def cpu_baseline(n_sample: int):
+    """Oracle (single core, FP64) on the same workload at n_sample^3 x 6 cells."""
+    from mifem import problems
+    from oracle import cpu
+    P = problems.elasticity_p2(n_sample)
+    Oa, OL = cpu.OracleAssembler(P.a), cpu.OracleAssembler(P.L)
+    rp, cols, _, _ = Oa.sparsity()
+    t0 = time.perf_counter()
+    Oa.assemble_matrix(rp, cols, P.bcs)
+    b = OL.assemble_vector()
+    Oa.apply_lifting(b, P.bcs)
+    cpu.set_bc(b, P.bcs, P.V)
+    dt = time.perf_counter() - t0
+    return {"value": P.V.size / dt, "unit": "DoF/s", "cores": 1, "kind": "port",
+            "sample": f"elasticity_p2 {n_sample}^3x6 tets ({P.mesh.num_cells} cells, "
+                      f"{P.V.size} dofs): A+b in {dt:.2f} s, oracle (C, serial, -O2)",
+            "cpu": platform.processor() or platform.machine()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=64, help="cubes per direction (64: 1.57M cells)")
+    ap.add_argument("--kernel", type=int, default=0, help="matrix kernel: 0 auto, 1 atomic, 2 owner")
+    ap.add_argument("--cpu-sample", type=int, default=32, help="oracle sample size (0: skip)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import cudolfinx as cufem
+    from cudolfinx import _lib
+    from mifem import problems
+
+    t0 = time.perf_counter()
+    P = problems.elasticity_p2(args.n)
+    log(f"problem: {P.mesh.num_cells} cells, {P.V.size} dofs ({time.perf_counter() - t0:.1f} s)")
+    a = cufem.form(P.a, options={"matrix_kernel": args.kernel})
+    L = cufem.form(P.L)
+    asm = cufem.CUDAAssembler()
+    A = asm.create_matrix(a)
+    b = asm.create_vector(L)
+    bcs = asm.pack_bcs(P.bcs)
+    log(f"setup done: nnz={A.nnz} ({time.perf_counter() - t0:.1f} s)")
+
+    def step():
+        asm.assemble_matrix(a, A, bcs=bcs)
+        A.assemble()
+        asm.assemble_vector(L, b)
+        asm.apply_lifting(b, [a], [bcs])
+        asm.set_bc(b, bcs, P.V)
+
+    for _ in range(args.warmup):
+        step()
+    asm.synchronize()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    asm.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    ms = wall / args.steps * 1e3
+    log(f"{args.steps} steps: {ms:.3f} ms/step")
+
+    # dominant kernel: the matrix assembly launch alone, timed with HIP events
+    # on the stream it runs on (the null stream, torch's default here)
+    lib = _lib.lib()
+    b0 = bcs._get_cpp_bcs(P.V)
+    stream = torch.cuda.current_stream()
+    reps = max(5, args.steps)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(reps):
+        _lib.check(lib.mf_assemble_matrix(a.handle, A.handle, b0.handle, b0.handle, 1.0, 1,
+                                          C.c_void_p(stream.cuda_stream)))
+    ev1.record(stream)
+    ev1.synchronize()
+    kms = ev0.elapsed_time(ev1) / reps
+    vb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    vb[0].record(stream)
+    for _ in range(reps):
+        asm.assemble_vector(L, b)
+        asm.apply_lifting(b, [a], [bcs])
+        asm.set_bc(b, bcs, P.V)
+    vb[1].record(stream)
+    vb[1].synchronize()
+    vms = vb[0].elapsed_time(vb[1]) / reps
+    alg = algorithmic_bytes(P, A)
+    achieved = alg / (kms * 1e-3) / 1e9
+    log(f"matrix kernel {kms:.3f} ms -> {achieved:.0f} GB/s algorithmic; b region {vms:.3f} ms")
+
+    # property check at full size: symmetric values on a sample of rows
+    rp, cols, vals = A.csr()
+    rng = np.random.default_rng(0)
+    rows = rng.choice(A.nrows, size=200, replace=False)
+    asym = 0.0
+    for r in rows:
+        for p in range(rp[r], rp[r + 1]):
+            c = cols[p]
+            q = rp[c] + np.searchsorted(cols[rp[c]:rp[c + 1]], r)
+            asym = max(asym, abs(vals[p] - vals[q]) / max(abs(vals[p]), 1e-300) if vals[p] else abs(vals[q]))
+    log(f"sampled symmetry defect {asym:.2e}")
+
+    out = {
+        "metric": METRIC,
+        "value": P.V.size / (ms * 1e-3),
+        "unit": "DoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Lame fields, generated mesh)",
+        "config": {"workload": f"elasticity_p2 {args.n}^3x6 tets A+b (matrix+vector+lifting+set_bc)",
+                   "cells": P.mesh.num_cells, "dofs": P.V.size, "nnz": A.nnz,
+                   "matrix_kernel": {0: "auto(owner-computes)", 1: "per-entity atomic",
+                                     2: "owner-computes"}[args.kernel],
+                   "matrix_ms": kms, "b_region_ms": vms},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "mf_owner_cell_0" if args.kernel != 1 else "mf_mat_cell_0",
+                     "algorithmic_bytes": alg},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu and args.cpu_sample > 0:
+        log("cpu baseline (oracle) ...")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

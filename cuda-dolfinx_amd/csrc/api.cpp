@@ -1,0 +1,588 @@
+// The C ABI (include/mifem.h).  Every entry point catches all C++
+// exceptions and turns them into a status code + thread-local message.
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+
+#include "mf_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+std::atomic<uint64_t> g_uid{1};
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return MF_OK;
+  } catch (const mf::Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return MF_ENOMEM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return MF_EINVAL;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return MF_EINVAL;
+  }
+}
+
+inline hipStream_t S(mf_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+void use_device(const mf_context* ctx) { MF_HIP(hipSetDevice(ctx->device)); }
+
+std::vector<std::string> rtc_options(const mf_context* ctx) {
+  return {"--offload-arch=" + ctx->arch, "-O3", "-munsafe-fp-atomics", "-std=c++17",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mf_last_error(void) { return g_last_error.c_str(); }
+int mf_version(void) { return 100; }
+
+int mf_context_create(int device, const char* cache_dir, mf_context** out) {
+  return guard([&] {
+    MF_CHECK(out, MF_EINVAL, "null output");
+    int n = 0;
+    MF_HIP(hipGetDeviceCount(&n));
+    MF_CHECK(device >= 0 && device < n, MF_EINVAL, "no such HIP device");
+    MF_HIP(hipSetDevice(device));
+    hipDeviceProp_t p;
+    MF_HIP(hipGetDeviceProperties(&p, device));
+    std::string arch = p.gcnArchName;
+    arch = arch.substr(0, arch.find(':'));
+    MF_CHECK(arch == "gfx950", MF_EUNSUPPORTED, "mifem targets gfx950 (MI355X); found " + arch);
+    auto* c = new mf_context;
+    c->device = device;
+    c->arch = arch;
+    c->cache_dir = cache_dir ? std::string(cache_dir) : mf::default_cache_dir();
+    c->num_cus = p.multiProcessorCount;
+    c->lds_per_block = p.sharedMemPerBlock;
+    *out = c;
+  });
+}
+
+int mf_context_destroy(mf_context* ctx) {
+  return guard([&] { delete ctx; });
+}
+
+int mf_context_arch(mf_context* ctx, char* buf, int len) {
+  return guard([&] {
+    MF_CHECK(ctx && buf && len > 0, MF_EINVAL, "bad arguments");
+    std::snprintf(buf, len, "%s", ctx->arch.c_str());
+  });
+}
+
+int mf_alloc(mf_context* ctx, int64_t bytes, void** d_ptr) {
+  return guard([&] {
+    MF_CHECK(ctx && d_ptr && bytes >= 0, MF_EINVAL, "bad arguments");
+    use_device(ctx);
+    *d_ptr = nullptr;
+    if (bytes > 0) {
+      hipError_t e = hipMalloc(d_ptr, bytes);
+      MF_CHECK(e == hipSuccess, MF_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+  });
+}
+
+int mf_free(mf_context* ctx, void* d_ptr) {
+  return guard([&] {
+    MF_CHECK(ctx, MF_EINVAL, "null context");
+    if (d_ptr) MF_HIP(hipFree(d_ptr));
+  });
+}
+
+int mf_memcpy_htod(mf_context* ctx, void* dst, const void* src, int64_t bytes, mf_stream s) {
+  return guard([&] {
+    MF_CHECK(ctx && (bytes == 0 || (dst && src)), MF_EINVAL, "bad arguments");
+    if (bytes) MF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(s)));
+  });
+}
+
+int mf_memcpy_dtoh(mf_context* ctx, void* dst, const void* src, int64_t bytes, mf_stream s) {
+  return guard([&] {
+    MF_CHECK(ctx && (bytes == 0 || (dst && src)), MF_EINVAL, "bad arguments");
+    if (bytes) {
+      MF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S(s)));
+      MF_HIP(hipStreamSynchronize(S(s)));
+    }
+  });
+}
+
+int mf_synchronize(mf_context* ctx, mf_stream s) {
+  return guard([&] {
+    MF_CHECK(ctx, MF_EINVAL, "null context");
+    MF_HIP(hipStreamSynchronize(S(s)));
+  });
+}
+
+// ------------------------------------------------------------------ mesh
+int mf_mesh_create(mf_context* ctx, const double* x, int64_t nvert, const int32_t* cells,
+                   int64_t ncells, int nv, const uint8_t* fperm, int nf, mf_mesh** out) {
+  return guard([&] {
+    MF_CHECK(ctx && out && x && cells && nvert > 0 && ncells >= 0 && nv > 0, MF_EINVAL,
+             "bad mesh arguments");
+    for (int64_t i = 0; i < ncells * nv; ++i)
+      MF_CHECK(cells[i] >= 0 && cells[i] < nvert, MF_EINVAL, "cell vertex index out of range");
+    use_device(ctx);
+    auto m = std::make_unique<mf_mesh>();
+    m->ctx = ctx;
+    m->num_vertices = nvert;
+    m->num_cells = ncells;
+    m->nv = nv;
+    m->nf = nf;
+    m->x.upload(x, nvert * 3);
+    m->cells.upload(cells, ncells * nv);
+    if (fperm && nf > 0) m->fperm.upload(fperm, ncells * nf);
+    *out = m.release();
+  });
+}
+
+int mf_mesh_destroy(mf_mesh* m) {
+  return guard([&] { delete m; });
+}
+
+// ------------------------------------------------------------------ dofmap
+int mf_dofmap_create(mf_context* ctx, const int32_t* dofs, int64_t ncells, int nd, int bs,
+                     int64_t nnodes, mf_dofmap** out) {
+  return guard([&] {
+    MF_CHECK(ctx && out && dofs && ncells >= 0 && nd > 0 && bs > 0 && nnodes > 0, MF_EINVAL,
+             "bad dofmap arguments");
+    MF_CHECK(nnodes * bs < INT32_MAX, MF_EUNSUPPORTED, "dof count exceeds int32 indexing");
+    for (int64_t i = 0; i < ncells * nd; ++i)
+      MF_CHECK(dofs[i] >= 0 && dofs[i] < nnodes, MF_EINVAL, "dofmap index out of range");
+    use_device(ctx);
+    auto d = std::make_unique<mf_dofmap>();
+    d->ctx = ctx;
+    d->host.assign(dofs, dofs + ncells * nd);
+    d->dev.upload(d->host);
+    d->num_cells = ncells;
+    d->num_nodes = nnodes;
+    d->nd = nd;
+    d->bs = bs;
+    *out = d.release();
+  });
+}
+
+int mf_dofmap_destroy(mf_dofmap* d) {
+  return guard([&] { delete d; });
+}
+
+// ------------------------------------------------------------------ form
+int mf_form_create(mf_context* ctx, mf_mesh* mesh, int rank, mf_dofmap* dm0, mf_dofmap* dm1,
+                   mf_form** out) {
+  return guard([&] {
+    MF_CHECK(ctx && mesh && out, MF_EINVAL, "null argument");
+    MF_CHECK(rank == 1 || rank == 2, MF_EUNSUPPORTED, "only rank 1 and 2 forms are supported");
+    MF_CHECK(dm0, MF_EINVAL, "rank >= 1 needs a test-space dofmap");
+    MF_CHECK(rank == 1 || dm1, MF_EINVAL, "rank 2 needs a trial-space dofmap");
+    MF_CHECK(dm0->num_cells == mesh->num_cells && (!dm1 || dm1->num_cells == mesh->num_cells),
+             MF_ESHAPE, "dofmap cell count does not match the mesh");
+    auto f = std::make_unique<mf_form>();
+    f->ctx = ctx;
+    f->mesh = mesh;
+    f->rank = rank;
+    f->dm[0] = dm0;
+    f->dm[1] = rank == 2 ? dm1 : nullptr;
+    f->uid = g_uid++;
+    *out = f.release();
+  });
+}
+
+int mf_form_add_integral(mf_form* f, int type, int id, const char* name, const char* src,
+                         const int32_t* ents, int64_t n, const char* staged) {
+  return guard([&] {
+    MF_CHECK(f && name && src, MF_EINVAL, "null argument");
+    MF_CHECK(!f->compiled, MF_ESTATE, "integrals must be added before compile");
+    MF_CHECK(type >= MF_CELL && type <= MF_INTERIOR_FACET, MF_EINVAL, "bad integral type");
+    MF_CHECK(n >= 0 && (n == 0 || ents), MF_EINVAL, "bad entity list");
+    const int s = mf::entity_stride(type);
+    for (int64_t e = 0; e < n; ++e)
+      for (int k = 0; k < s; k += 2) {
+        MF_CHECK(ents[e * s + k] >= 0 && ents[e * s + k] < f->mesh->num_cells, MF_EINVAL,
+                 "entity cell index out of range");
+        if (type != MF_CELL)
+          MF_CHECK(ents[e * s + k + 1] >= 0 && ents[e * s + k + 1] < std::max(1, f->mesh->nf),
+                   MF_EINVAL, "local facet index out of range");
+      }
+    auto I = std::make_unique<mf::Integral>();
+    I->type = type;
+    I->id = id;
+    I->tab_name = name;
+    I->tab_src = src;
+    if (staged) I->staged_src = staged;
+    I->h_ents.assign(ents, ents + n * s);
+    I->n = n;
+    I->index = static_cast<int>(f->ints.size());
+    use_device(f->ctx);
+    I->d_ents.upload(I->h_ents);
+    f->ints.push_back(std::move(I));
+    // keep cell, exterior, interior order (CUDAAssembler.h:696-739)
+    std::stable_sort(f->ints.begin(), f->ints.end(),
+                     [](auto& a, auto& b) { return a->type < b->type; });
+  });
+}
+
+int mf_form_add_coefficient(mf_form* f, mf_dofmap* dm, int64_t size, int* index) {
+  return guard([&] {
+    MF_CHECK(f && dm && index, MF_EINVAL, "null argument");
+    MF_CHECK(!f->compiled, MF_ESTATE, "coefficients must be added before compile");
+    MF_CHECK(dm->num_cells == f->mesh->num_cells, MF_ESHAPE, "coefficient dofmap/mesh mismatch");
+    MF_CHECK(size == dm->num_nodes * dm->bs, MF_ESHAPE, "coefficient size mismatch");
+    mf::Coef c;
+    c.dm = dm;
+    c.size = size;
+    c.offset = f->W;
+    c.width = dm->nd * dm->bs;
+    f->W += c.width;
+    *index = static_cast<int>(f->coefs.size());
+    f->coefs.push_back(std::move(c));
+  });
+}
+
+int mf_form_set_coefficient(mf_form* f, int idx, const double* values, int64_t n, int on_device) {
+  return guard([&] {
+    MF_CHECK(f && values, MF_EINVAL, "null argument");
+    MF_CHECK(idx >= 0 && idx < (int)f->coefs.size(), MF_EINVAL, "coefficient index out of range");
+    auto& c = f->coefs[idx];
+    MF_CHECK(n == c.size, MF_ESHAPE, "coefficient length mismatch");
+    use_device(f->ctx);
+    if (on_device) {
+      c.ext = values;
+      c.own.reset();
+    } else {
+      if (c.own.n != n) c.own.alloc(n);
+      MF_HIP(hipMemcpy(c.own.ptr, values, sizeof(double) * n, hipMemcpyHostToDevice));
+      c.ext = nullptr;
+    }
+  });
+}
+
+int mf_form_set_constants(mf_form* f, const double* v, int n) {
+  return guard([&] {
+    MF_CHECK(f && (n == 0 || v) && n >= 0, MF_EINVAL, "bad constants");
+    use_device(f->ctx);
+    f->h_consts.assign(v, v + n);
+    std::vector<double> padded = f->h_consts;
+    padded.push_back(0.0);
+    if (f->consts.n != (int64_t)padded.size()) f->consts.alloc(padded.size());
+    MF_HIP(hipMemcpy(f->consts.ptr, padded.data(), sizeof(double) * padded.size(),
+                     hipMemcpyHostToDevice));
+  });
+}
+
+int mf_form_set_option(mf_form* f, const char* key, int64_t value) {
+  return guard([&] {
+    MF_CHECK(f && key, MF_EINVAL, "null argument");
+    static const char* known[] = {"matrix_kernel", "chunk_values", "chunk_rows", "block",
+                                  "owner_block", "owner_nb"};
+    bool ok = false;
+    for (auto k : known) ok |= std::strcmp(k, key) == 0;
+    MF_CHECK(ok, MF_EINVAL, std::string("unknown option ") + key);
+    MF_CHECK(!f->compiled || std::strcmp(key, "matrix_kernel") == 0, MF_ESTATE,
+             "kernel shape options must be set before compile");
+    f->options[key] = value;
+  });
+}
+
+int mf_form_compile(mf_form* f, int verbose) {
+  return guard([&] {
+    MF_CHECK(f, MF_EINVAL, "null form");
+    MF_CHECK(!f->ints.empty(), MF_ESTATE, "form has no integrals");
+    use_device(f->ctx);
+    if (!f->consts.ptr) {
+      double z = 0.0;
+      f->consts.upload(&z, 1);
+    }
+    if (f->W > 0) f->packed.alloc(f->mesh->num_cells * (int64_t)f->W);
+    // owner-kernel shape: LDS budget (doubles) and cells per batch
+    int64_t cap = f->opt("chunk_values", 4096);
+    f->options["chunk_values"] = cap;
+    if (!f->options.count("owner_nb")) f->options["owner_nb"] = 32;
+    if (!f->options.count("chunk_rows")) f->options["chunk_rows"] = 256;
+    std::string src = mf::generate_form_source(*f);
+    {
+      // owner-kernel compile-time shape
+      auto rep = [&](const std::string& k, int64_t v) {
+        size_t p = 0;
+        std::string t = std::to_string(v);
+        while ((p = src.find(k, p)) != std::string::npos) {
+          src.replace(p, k.size(), t);
+          p += t.size();
+        }
+      };
+      rep("$OBLOCK", f->opt("owner_block", 256));
+      rep("$ONB", f->opt("owner_nb", 32));
+      rep("$OCAP", cap);
+      rep("$OMAXR", f->opt("chunk_rows", 256));
+    }
+    f->source = src;
+    bool cached = false;
+    auto code = mf::rtc_compile(src, "mifem_form", rtc_options(f->ctx), f->ctx->cache_dir, &cached);
+    f->mod = std::make_unique<mf::Module>();
+    f->mod->load(code);
+    for (auto& I : f->ints) {
+      if (f->rank == 2) {
+        I->k_mat = f->mod->get(mf::kernel_name("mat", I->type, I->index));
+        I->k_lift = f->mod->get(mf::kernel_name("lift", I->type, I->index));
+        if (I->type == MF_CELL && !I->staged_src.empty())
+          I->k_owner = f->mod->get(mf::kernel_name("owner", I->type, I->index));
+      } else {
+        I->k_vec = f->mod->get(mf::kernel_name("vec", I->type, I->index));
+      }
+    }
+    if (verbose)
+      std::fprintf(stderr, "[mifem] form %llu: %zu integrals, code object %zu bytes%s\n",
+                   (unsigned long long)f->uid, f->ints.size(), code.size(),
+                   cached ? " (cache)" : "");
+    f->compiled = true;
+  });
+}
+
+int mf_form_source(mf_form* f, char* buf, int64_t* len) {
+  return guard([&] {
+    MF_CHECK(f && len, MF_EINVAL, "null argument");
+    int64_t need = static_cast<int64_t>(f->source.size()) + 1;
+    if (buf && *len >= need) std::memcpy(buf, f->source.c_str(), need);
+    *len = need;
+  });
+}
+
+int mf_form_pack_coefficients(mf_form* f, const int* idx, int n, mf_stream s) {
+  return guard([&] {
+    MF_CHECK(f, MF_EINVAL, "null form");
+    use_device(f->ctx);
+    std::vector<int> sel;
+    if (idx) sel.assign(idx, idx + n);
+    else
+      for (int j = 0; j < (int)f->coefs.size(); ++j) sel.push_back(j);
+    for (int j : sel) {
+      MF_CHECK(j >= 0 && j < (int)f->coefs.size(), MF_EINVAL, "coefficient index out of range");
+      auto& c = f->coefs[j];
+      const double* v = c.ext ? c.ext : c.own.ptr;
+      MF_CHECK(v, MF_ESTATE, "coefficient " + std::to_string(j) + " has no values");
+      mf::k_pack(f->packed.ptr, f->mesh->num_cells, f->W, c.offset, v, c.dm->dev.ptr, c.dm->nd,
+                 c.dm->bs, S(s));
+    }
+  });
+}
+
+int mf_form_destroy(mf_form* f) {
+  return guard([&] { delete f; });
+}
+
+// ------------------------------------------------------------------ matrix
+int mf_matrix_create(mf_form* a, mf_matrix** out) {
+  return guard([&] {
+    MF_CHECK(a && out, MF_EINVAL, "null argument");
+    MF_CHECK(a->rank == 2, MF_EINVAL, "create_matrix needs a bilinear form");
+    use_device(a->ctx);
+    auto A = std::make_unique<mf_matrix>();
+    A->form = a;
+    A->bs0 = a->dm[0]->bs;
+    A->bs1 = a->dm[1]->bs;
+    A->nrows_b = a->dm[0]->num_nodes;
+    A->nrows = A->nrows_b * A->bs0;
+    A->ncols = a->dm[1]->num_nodes * A->bs1;
+    mf::build_pattern(*a, A->h_rpb, A->h_colsb);
+    A->nnz_b = A->h_rpb.back();
+    A->nnz = A->nnz_b * A->bs0 * A->bs1;
+    MF_CHECK(A->nnz < INT32_MAX, MF_EUNSUPPORTED, "nnz exceeds int32 CSR indexing");
+    std::vector<int32_t> rp(A->nrows + 1, 0);
+    for (int64_t r = 0; r < A->nrows_b; ++r) {
+      int64_t len = (int64_t)A->bs1 * (A->h_rpb[r + 1] - A->h_rpb[r]);
+      for (int a0 = 0; a0 < A->bs0; ++a0)
+        rp[r * A->bs0 + a0 + 1] = static_cast<int32_t>(rp[r * A->bs0 + a0] + len);
+    }
+    A->row_ptr.upload(rp);
+    A->rpb.upload(A->h_rpb);
+    A->colsb.upload(A->h_colsb);
+    A->cols.alloc(std::max<int64_t>(1, A->nnz));
+    mf::k_unroll_cols(A->cols.ptr, A->row_ptr.ptr, A->rpb.ptr, A->colsb.ptr, A->nrows_b, A->bs0,
+                      A->bs1, nullptr);
+    A->own_values.alloc(std::max<int64_t>(1, A->nnz));
+    A->values = A->own_values.ptr;
+    mf::k_fill(A->values, A->nnz, 0.0, nullptr);
+    MF_HIP(hipDeviceSynchronize());
+    *out = A.release();
+  });
+}
+
+int mf_matrix_info(mf_matrix* A, int64_t* nrows, int64_t* ncols, int64_t* nnz) {
+  return guard([&] {
+    MF_CHECK(A, MF_EINVAL, "null matrix");
+    if (nrows) *nrows = A->nrows;
+    if (ncols) *ncols = A->ncols;
+    if (nnz) *nnz = A->nnz;
+  });
+}
+
+int mf_matrix_device_arrays(mf_matrix* A, int32_t** rp, int32_t** cols, double** vals) {
+  return guard([&] {
+    MF_CHECK(A, MF_EINVAL, "null matrix");
+    if (rp) *rp = A->row_ptr.ptr;
+    if (cols) *cols = A->cols.ptr;
+    if (vals) *vals = A->values;
+  });
+}
+
+int mf_matrix_borrow_values(mf_matrix* A, double* d_values) {
+  return guard([&] {
+    MF_CHECK(A && d_values, MF_EINVAL, "null argument");
+    A->own_values.reset();
+    A->values = d_values;
+  });
+}
+
+int mf_matrix_copy_csr(mf_matrix* A, int32_t* rp, int32_t* cols, double* vals) {
+  return guard([&] {
+    MF_CHECK(A, MF_EINVAL, "null matrix");
+    use_device(A->form->ctx);
+    MF_HIP(hipDeviceSynchronize());
+    if (rp) MF_HIP(hipMemcpy(rp, A->row_ptr.ptr, sizeof(int32_t) * (A->nrows + 1), hipMemcpyDeviceToHost));
+    if (cols && A->nnz)
+      MF_HIP(hipMemcpy(cols, A->cols.ptr, sizeof(int32_t) * A->nnz, hipMemcpyDeviceToHost));
+    if (vals && A->nnz)
+      MF_HIP(hipMemcpy(vals, A->values, sizeof(double) * A->nnz, hipMemcpyDeviceToHost));
+  });
+}
+
+int mf_matrix_destroy(mf_matrix* A) {
+  return guard([&] { delete A; });
+}
+
+// ------------------------------------------------------------------ bcs
+int mf_bc_create(mf_context* ctx, int64_t ndofs, const int32_t* dofs, int64_t nbc, int64_t n_owned,
+                 const double* g, mf_bc** out) {
+  return guard([&] {
+    MF_CHECK(ctx && out && ndofs > 0 && nbc >= 0 && (nbc == 0 || dofs) && g, MF_EINVAL,
+             "bad bc arguments");
+    MF_CHECK(n_owned >= 0 && n_owned <= nbc, MF_EINVAL, "n_owned out of range");
+    for (int64_t i = 0; i < nbc; ++i)
+      MF_CHECK(dofs[i] >= 0 && dofs[i] < ndofs, MF_EINVAL, "bc dof out of range");
+    use_device(ctx);
+    auto b = std::make_unique<mf_bc>();
+    b->ctx = ctx;
+    b->ndofs = ndofs;
+    b->nbc = nbc;
+    b->n_owned = n_owned;
+    b->uid = g_uid++;
+    if (nbc) b->dofs.upload(dofs, nbc);
+    b->g.upload(g, ndofs);
+    b->markers.alloc(ndofs);
+    MF_HIP(hipMemset(b->markers.ptr, 0, ndofs));
+    mf::k_mark(b->markers.ptr, ndofs, b->dofs.ptr, nbc, nullptr);
+    MF_HIP(hipDeviceSynchronize());
+    *out = b.release();
+  });
+}
+
+int mf_bc_update(mf_bc* b, const double* g) {
+  return guard([&] {
+    MF_CHECK(b && g, MF_EINVAL, "null argument");
+    use_device(b->ctx);
+    MF_HIP(hipMemcpy(b->g.ptr, g, sizeof(double) * b->ndofs, hipMemcpyHostToDevice));
+  });
+}
+
+int mf_bc_destroy(mf_bc* b) {
+  return guard([&] { delete b; });
+}
+
+// ------------------------------------------------------------------ assembly
+int mf_zero(mf_context* ctx, double* x, int64_t n, mf_stream s) {
+  return guard([&] {
+    MF_CHECK(ctx && (n == 0 || x) && n >= 0, MF_EINVAL, "bad arguments");
+    mf::k_fill(x, n, 0.0, S(s));
+  });
+}
+
+int mf_assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc1, double diag,
+                       int zero, mf_stream s) {
+  return guard([&] {
+    MF_CHECK(a && A, MF_EINVAL, "null argument");
+    use_device(a->ctx);
+    mf::assemble_matrix(a, A, bc0, bc1, diag, zero, S(s));
+  });
+}
+
+int mf_assemble_vector(mf_form* L, double* b, int64_t n, int zero, mf_stream s) {
+  return guard([&] {
+    MF_CHECK(L && b, MF_EINVAL, "null argument");
+    MF_CHECK(L->rank == 1, MF_EINVAL, "assemble_vector needs a linear form");
+    MF_CHECK(L->compiled, MF_ESTATE, "form is not compiled");
+    MF_CHECK(n == L->dm[0]->num_nodes * L->dm[0]->bs, MF_ESHAPE, "vector length mismatch");
+    use_device(L->ctx);
+    if (zero) mf::k_fill(b, n, 0.0, S(s));
+    const uint32_t block = static_cast<uint32_t>(L->opt("block", 128));
+    for (auto& I : L->ints) {
+      if (I->n == 0) continue;
+      mf::launch(I->k_vec, static_cast<uint32_t>((I->n + block - 1) / block), block, 0, S(s),
+                 static_cast<int32_t>(I->n), I->d_ents.ptr, L->mesh->cells.ptr, L->mesh->x.ptr,
+                 L->packed.ptr, L->consts.ptr, L->dm[0]->dev.ptr, b, L->mesh->fperm.ptr);
+    }
+  });
+}
+
+int mf_apply_lifting(mf_form* const* a, int nforms, double* b, int64_t n,
+                     const mf_bc* const* bcs1, const double* const* x0, double scale,
+                     mf_stream s) {
+  return guard([&] {
+    MF_CHECK(nforms >= 0 && (nforms == 0 || (a && bcs1)) && b, MF_EINVAL, "bad arguments");
+    for (int k = 0; k < nforms; ++k) {
+      mf_form* f = a[k];
+      const mf_bc* bc = bcs1[k];
+      MF_CHECK(f && f->rank == 2 && f->compiled, MF_EINVAL, "lifting needs compiled bilinear forms");
+      MF_CHECK(n == f->dm[0]->num_nodes * f->dm[0]->bs, MF_ESHAPE, "vector length mismatch");
+      if (!bc || bc->nbc == 0) continue;
+      MF_CHECK(bc->ndofs == f->dm[1]->num_nodes * f->dm[1]->bs, MF_ESHAPE,
+               "bc size does not match the trial space");
+      use_device(f->ctx);
+      const double* xk = x0 ? x0[k] : nullptr;
+      const uint32_t block = static_cast<uint32_t>(f->opt("block", 128));
+      for (auto& I : f->ints) {
+        if (I->n == 0) continue;
+        auto it = I->lift_lists.find(bc->uid);
+        if (it == I->lift_lists.end()) {
+          mf::DevArray<int32_t> list, cnt;
+          list.alloc(std::max<int64_t>(1, I->n));
+          cnt.alloc(1);
+          mf::k_fill_i32(cnt.ptr, 1, 0, S(s));
+          mf::k_lift_list(list.ptr, cnt.ptr, I->d_ents.ptr, I->n, mf::entity_stride(I->type),
+                          mf::entity_ncell(I->type), f->dm[1]->dev.ptr, f->dm[1]->nd,
+                          f->dm[1]->bs, bc->markers.ptr, S(s));
+          int32_t h = 0;
+          MF_HIP(hipMemcpyAsync(&h, cnt.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, S(s)));
+          MF_HIP(hipStreamSynchronize(S(s)));
+          it = I->lift_lists.emplace(bc->uid, std::make_pair(std::move(list), (int64_t)h)).first;
+        }
+        const int64_t nl = it->second.second;
+        if (nl == 0) continue;
+        mf::launch(I->k_lift, static_cast<uint32_t>((nl + block - 1) / block), block, 0, S(s),
+                   static_cast<int32_t>(nl), it->second.first.ptr, I->d_ents.ptr,
+                   f->mesh->cells.ptr, f->mesh->x.ptr, f->packed.ptr, f->consts.ptr,
+                   f->dm[0]->dev.ptr, f->dm[1]->dev.ptr, bc->markers.ptr, bc->g.ptr, xk, scale,
+                   b, f->mesh->fperm.ptr);
+      }
+    }
+  });
+}
+
+int mf_set_bc(mf_context* ctx, double* b, int64_t n, const mf_bc* bc, const double* x0,
+              double scale, mf_stream s) {
+  return guard([&] {
+    MF_CHECK(ctx && b && bc, MF_EINVAL, "null argument");
+    MF_CHECK(bc->ndofs == n, MF_ESHAPE, "bc size does not match the vector");
+    use_device(ctx);
+    mf::k_set_bc(b, n, bc->dofs.ptr, bc->n_owned, bc->g.ptr, x0, scale, S(s));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,267 @@
+// Kernel generator: wraps each integral's UFCx tabulate_tensor body in
+// hand-written HIP templates and emits one hipRTC translation unit per form.
+//
+// Reference: the NVRTC string templates of
+// cpp/cudolfinx/fem/CUDAFormIntegral.cpp (vector :270-522, lifting :603-947,
+// matrix :1103-1226, :1667-1793, :1988-2103; interior-facet gathers
+// :1855-1986).  Differences by design:
+//  * the tabulate body becomes a __device__ __forceinline__ function
+//    (the reference declares it extern "C" __global__, quirk Q7);
+//  * the matrix scatter reads a precomputed, wave64-interleaved offset table
+//    (entity, i, j) -> CSR position instead of binary-searching every entry,
+//    and adds with native FP64 global atomics (-munsafe-fp-atomics);
+//  * lifting runs only over the precomputed list of entities that touch a
+//    constrained column, for every integral type (Q3 fixed);
+//  * cell integrals with a staged description also get the cooperative
+//    owner-computes kernel (owner_kernel.inc), which writes each CSR row
+//    segment exactly once from LDS.
+#include <sstream>
+
+#include "mf_internal.h"
+
+namespace mf {
+
+namespace {
+
+const char* kPrelude = R"(
+#define restrict __restrict__
+typedef unsigned char uint8_t;
+typedef signed char int8_t;
+typedef __hip_internal::int32_t int32_t;
+typedef __hip_internal::int64_t int64_t;
+typedef __hip_internal::uint32_t uint32_t;
+typedef __hip_internal::int16_t int16_t;
+)";
+
+const char* kGatherCell = R"(
+  const int32_t cl[1] = {ents[e]};
+  double xc[$NV * 3];
+  for (int v = 0; v < $NV; ++v) {
+    const int64_t gv = geo[(int64_t)cl[0] * $NV + v];
+    xc[3 * v] = xg[3 * gv]; xc[3 * v + 1] = xg[3 * gv + 1]; xc[3 * v + 2] = xg[3 * gv + 2];
+  }
+  const double* w = coeffs + (int64_t)cl[0] * $W;
+  const int* eli = nullptr;
+  const uint8_t* qperm = nullptr;
+)";
+
+const char* kGatherExt = R"(
+  const int32_t cl[1] = {ents[2 * e]};
+  const int eli[1] = {ents[2 * e + 1]};
+  double xc[$NV * 3];
+  for (int v = 0; v < $NV; ++v) {
+    const int64_t gv = geo[(int64_t)cl[0] * $NV + v];
+    xc[3 * v] = xg[3 * gv]; xc[3 * v + 1] = xg[3 * gv + 1]; xc[3 * v + 2] = xg[3 * gv + 2];
+  }
+  const double* w = coeffs + (int64_t)cl[0] * $W;
+  const uint8_t* qperm = nullptr;
+)";
+
+const char* kGatherInt = R"(
+  const int32_t cl[2] = {ents[4 * e], ents[4 * e + 2]};
+  const int eli[2] = {ents[4 * e + 1], ents[4 * e + 3]};
+  const uint8_t qperm[2] = {fperm ? fperm[(int64_t)cl[0] * $NF + eli[0]] : (uint8_t)0,
+                            fperm ? fperm[(int64_t)cl[1] * $NF + eli[1]] : (uint8_t)0};
+  double xc[2 * $NV * 3];
+  for (int s = 0; s < 2; ++s)
+    for (int v = 0; v < $NV; ++v) {
+      const int64_t gv = geo[(int64_t)cl[s] * $NV + v];
+      xc[(s * $NV + v) * 3] = xg[3 * gv];
+      xc[(s * $NV + v) * 3 + 1] = xg[3 * gv + 1];
+      xc[(s * $NV + v) * 3 + 2] = xg[3 * gv + 2];
+    }
+  double w[2 * $W + 1];
+  {
+    const double* w0 = coeffs + (int64_t)cl[0] * $W;
+    const double* w1 = coeffs + (int64_t)cl[1] * $W;
+$INTERLACE  }
+)";
+
+const char* kMatrix = R"(
+extern "C" __global__ void __launch_bounds__($BLOCK)
+$KNAME(const int32_t n, const int32_t* __restrict__ ents, const int32_t* __restrict__ geo,
+       const double* __restrict__ xg, const double* __restrict__ coeffs,
+       const double* __restrict__ consts, const int32_t* __restrict__ dm0,
+       const int32_t* __restrict__ dm1, const int8_t* __restrict__ bc0,
+       const int8_t* __restrict__ bc1, const int32_t* __restrict__ table,
+       const int32_t* __restrict__ row_ptr, double* __restrict__ values,
+       const uint8_t* __restrict__ fperm)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+$GATHER
+  double A[$NR * $NC];
+  for (int i = 0; i < $NR * $NC; ++i) A[i] = 0.0;
+  $TAB(A, w, consts, xc, eli, qperm);
+  const int32_t* tb = table + (int64_t)(e >> 6) * ($NI * $NJ * 64) + (e & 63);
+  for (int i = 0; i < $NI; ++i) {
+    const int32_t rnode = dm0[(int64_t)cl[i / $ND0] * $ND0 + (i % $ND0)];
+    for (int a = 0; a < $BS0; ++a) {
+      const int32_t row = $BS0 * rnode + a;
+      if (bc0 && bc0[row]) continue;
+      const int64_t rowlen = row_ptr[row + 1] - row_ptr[row];
+      for (int j = 0; j < $NJ; ++j) {
+        const int32_t base = tb[(i * $NJ + j) * 64];
+        const int32_t cnode = dm1[(int64_t)cl[j / $ND1] * $ND1 + (j % $ND1)];
+        for (int b = 0; b < $BS1; ++b) {
+          if (bc1 && bc1[$BS1 * cnode + b]) continue;
+          atomicAdd(&values[(int64_t)base + a * rowlen + b], A[(i * $BS0 + a) * $NC + j * $BS1 + b]);
+        }
+      }
+    }
+  }
+}
+)";
+
+const char* kVector = R"(
+extern "C" __global__ void __launch_bounds__($BLOCK)
+$KNAME(const int32_t n, const int32_t* __restrict__ ents, const int32_t* __restrict__ geo,
+       const double* __restrict__ xg, const double* __restrict__ coeffs,
+       const double* __restrict__ consts, const int32_t* __restrict__ dm0,
+       double* __restrict__ b, const uint8_t* __restrict__ fperm)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+$GATHER
+  double be[$NR];
+  for (int i = 0; i < $NR; ++i) be[i] = 0.0;
+  $TAB(be, w, consts, xc, eli, qperm);
+  for (int i = 0; i < $NI; ++i) {
+    const int64_t node = dm0[(int64_t)cl[i / $ND0] * $ND0 + (i % $ND0)];
+    for (int a = 0; a < $BS0; ++a) atomicAdd(&b[$BS0 * node + a], be[i * $BS0 + a]);
+  }
+}
+)";
+
+const char* kLift = R"(
+extern "C" __global__ void __launch_bounds__($BLOCK)
+$KNAME(const int32_t nlist, const int32_t* __restrict__ list, const int32_t* __restrict__ ents,
+       const int32_t* __restrict__ geo, const double* __restrict__ xg,
+       const double* __restrict__ coeffs, const double* __restrict__ consts,
+       const int32_t* __restrict__ dm0, const int32_t* __restrict__ dm1,
+       const int8_t* __restrict__ bc1, const double* __restrict__ g,
+       const double* __restrict__ x0, const double scale, double* __restrict__ b,
+       const uint8_t* __restrict__ fperm)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlist) return;
+  const int e = list[t];
+$GATHER
+  double A[$NR * $NC];
+  for (int i = 0; i < $NR * $NC; ++i) A[i] = 0.0;
+  $TAB(A, w, consts, xc, eli, qperm);
+  double be[$NR];
+  for (int i = 0; i < $NR; ++i) be[i] = 0.0;
+  for (int j = 0; j < $NJ; ++j) {
+    const int64_t cnode = dm1[(int64_t)cl[j / $ND1] * $ND1 + (j % $ND1)];
+    for (int bb = 0; bb < $BS1; ++bb) {
+      const int64_t col = $BS1 * cnode + bb;
+      if (!bc1[col]) continue;
+      const double v = scale * (g[col] - (x0 ? x0[col] : 0.0));
+      for (int r = 0; r < $NR; ++r) be[r] -= A[r * $NC + j * $BS1 + bb] * v;
+    }
+  }
+  for (int i = 0; i < $NI; ++i) {
+    const int64_t node = dm0[(int64_t)cl[i / $ND0] * $ND0 + (i % $ND0)];
+    for (int a = 0; a < $BS0; ++a) atomicAdd(&b[$BS0 * node + a], be[i * $BS0 + a]);
+  }
+}
+)";
+
+void replace_all(std::string& s, const std::string& from, const std::string& to) {
+  size_t p = 0;
+  while ((p = s.find(from, p)) != std::string::npos) {
+    s.replace(p, from.size(), to);
+    p += to.size();
+  }
+}
+
+std::string subst(std::string t, const std::map<std::string, std::string>& m) {
+  // longest keys first so $ND0 is not clobbered by $ND
+  std::vector<std::pair<std::string, std::string>> kv(m.begin(), m.end());
+  std::sort(kv.begin(), kv.end(),
+            [](auto& a, auto& b) { return a.first.size() > b.first.size(); });
+  for (auto& [k, v] : kv) replace_all(t, k, v);
+  return t;
+}
+
+}  // namespace
+
+std::string kernel_name(const char* what, int type, int index) {
+  static const char* tn[] = {"cell", "ext", "int"};
+  return std::string("mf_") + what + "_" + tn[type] + "_" + std::to_string(index);
+}
+
+extern const char* kOwnerTemplate;   // owner_kernel.cpp
+
+std::string generate_form_source(const mf_form& f) {
+  std::ostringstream os;
+  os << kPrelude;
+  std::map<std::string, bool> seen;
+  for (auto& I : f.ints) {
+    if (seen.count(I->tab_name)) continue;
+    seen[I->tab_name] = true;
+    std::string src = I->tab_src;
+    size_t p = src.find("void " + I->tab_name);
+    MF_CHECK(p != std::string::npos, MF_EINVAL,
+             "tabulate source does not define void " + I->tab_name);
+    src.insert(p, "static __device__ __forceinline__ ");
+    os << "\n" << src << "\n";
+  }
+  const mf_dofmap* d0 = f.dm[0];
+  const mf_dofmap* d1 = f.rank == 2 ? f.dm[1] : f.dm[0];
+  for (auto& I : f.ints) {
+    const int ncell = entity_ncell(I->type);
+    std::map<std::string, std::string> m;
+    m["$NV"] = std::to_string(f.mesh->nv);
+    m["$NF"] = std::to_string(std::max(1, f.mesh->nf));
+    m["$W"] = std::to_string(f.W);
+    m["$ND0"] = std::to_string(d0->nd);
+    m["$BS0"] = std::to_string(d0->bs);
+    m["$ND1"] = std::to_string(d1->nd);
+    m["$BS1"] = std::to_string(d1->bs);
+    m["$NI"] = std::to_string(ncell * d0->nd);
+    m["$NJ"] = std::to_string(ncell * d1->nd);
+    m["$NR"] = std::to_string(ncell * d0->nd * d0->bs);
+    m["$NC"] = std::to_string(ncell * d1->nd * d1->bs);
+    m["$TAB"] = I->tab_name;
+    m["$BLOCK"] = std::to_string(f.opt("block", 128));
+    std::string interlace;
+    for (auto& c : f.coefs) {
+      std::ostringstream il;
+      il << "    for (int k = 0; k < " << c.width << "; ++k) { w[" << 2 * c.offset << " + k] = w0["
+         << c.offset << " + k]; w[" << 2 * c.offset + c.width << " + k] = w1[" << c.offset
+         << " + k]; }\n";
+      interlace += il.str();
+    }
+    std::string gather = I->type == MF_CELL ? kGatherCell
+                         : I->type == MF_EXTERIOR_FACET ? kGatherExt
+                                                        : kGatherInt;
+    replace_all(gather, "$INTERLACE", interlace);
+    m["$GATHER"] = gather;
+    if (f.rank == 2) {
+      m["$KNAME"] = kernel_name("mat", I->type, I->index);
+      os << subst(subst(kMatrix, {{"$GATHER", gather}}), m);
+      m["$KNAME"] = kernel_name("lift", I->type, I->index);
+      os << subst(subst(kLift, {{"$GATHER", gather}}), m);
+      if (I->type == MF_CELL && !I->staged_src.empty()) {
+        std::string staged = I->staged_src;
+        // namespace the staged symbols per integral
+        std::string pfx = "I" + std::to_string(I->index) + "_";
+        for (const char* sym : {"SK_ND", "SK_BS", "SK_NQ", "SK_W", "SK_DPHI", "sk_combine"})
+          replace_all(staged, sym, pfx + sym);
+        std::string owner = kOwnerTemplate;
+        for (const char* sym : {"SK_ND", "SK_BS", "SK_NQ", "SK_W", "SK_DPHI", "sk_combine"})
+          replace_all(owner, sym, pfx + sym);
+        m["$KNAME"] = kernel_name("owner", I->type, I->index);
+        os << "\n" << staged << "\n" << subst(owner, m);
+      }
+    } else if (f.rank == 1) {
+      m["$KNAME"] = kernel_name("vec", I->type, I->index);
+      os << subst(subst(kVector, {{"$GATHER", gather}}), m);
+    }
+  }
+  return os.str();
+}
+
+}  // namespace mf

@@ -1,0 +1,185 @@
+// Matrix assembly dispatch: offset tables, owner-computes schedules and the
+// per-integral launches (CUDAAssembler::assemble_matrix,
+// cpp/cudolfinx/fem/CUDAAssembler.h:683-740, and set_diagonal :878-958).
+#include <algorithm>
+#include <cstdio>
+
+#include "mf_internal.h"
+
+namespace mf {
+
+struct OwnerPlan {
+  int64_t nchunks = 0;
+  DevArray<int32_t> chunk_rows, chunk_items, item_cell;
+  DevArray<uint32_t> item_mask;
+  int64_t nitems = 0;
+};
+
+namespace {
+
+constexpr int kTableLanes = 64;
+
+// Contiguous blocked-row chunks whose unrolled value segment fits `cap`
+// doubles (at most `maxr` rows), and the cells (with a mask of owned local
+// rows) each chunk must visit.
+std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I, const mf_matrix& A,
+                                            int64_t cap, int maxr) {
+  const mf_dofmap* d0 = a.dm[0];
+  const int nd = d0->nd;
+  const int bs2 = A.bs0 * A.bs1;
+  const int64_t nrows = A.nrows_b;
+  std::vector<int32_t> rows{0};
+  int64_t seg = 0;
+  int32_t r0 = 0;
+  for (int64_t r = 0; r < nrows; ++r) {
+    int64_t len = (int64_t)bs2 * (A.h_rpb[r + 1] - A.h_rpb[r]);
+    if (len > cap) return nullptr;   // a single row does not fit: caller falls back
+    if (seg + len > cap || r - r0 >= maxr) {
+      rows.push_back(static_cast<int32_t>(r));
+      r0 = static_cast<int32_t>(r);
+      seg = 0;
+    }
+    seg += len;
+  }
+  rows.push_back(static_cast<int32_t>(nrows));
+  const int64_t nch = static_cast<int64_t>(rows.size()) - 1;
+  std::vector<int32_t> row_chunk(nrows);
+  for (int64_t c = 0; c < nch; ++c)
+    for (int32_t r = rows[c]; r < rows[c + 1]; ++r) row_chunk[r] = static_cast<int32_t>(c);
+  // count (cell, chunk) pairs: bucket each cell by the distinct chunks of its rows
+  std::vector<int64_t> cnt(nch + 1, 0);
+  std::vector<int32_t> tmp(nd);
+  auto cell_chunks = [&](int32_t c, int32_t* out) {
+    int n = 0;
+    for (int l = 0; l < nd; ++l) {
+      int32_t ch = row_chunk[d0->host[(int64_t)c * nd + l]];
+      bool dup = false;
+      for (int q = 0; q < n; ++q) dup |= out[q] == ch;
+      if (!dup) out[n++] = ch;
+    }
+    return n;
+  };
+  for (int64_t e = 0; e < I.n; ++e) {
+    int n = cell_chunks(I.h_ents[e], tmp.data());
+    for (int q = 0; q < n; ++q) cnt[tmp[q] + 1]++;
+  }
+  for (int64_t c = 0; c < nch; ++c) cnt[c + 1] += cnt[c];
+  std::vector<int32_t> icell(cnt[nch]);
+  std::vector<uint32_t> imask(cnt[nch]);
+  std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+  for (int64_t e = 0; e < I.n; ++e) {
+    const int32_t c = I.h_ents[e];
+    int n = cell_chunks(c, tmp.data());
+    for (int q = 0; q < n; ++q) {
+      uint32_t m = 0;
+      for (int l = 0; l < nd; ++l)
+        if (row_chunk[d0->host[(int64_t)c * nd + l]] == tmp[q]) m |= 1u << l;
+      int64_t p = pos[tmp[q]]++;
+      icell[p] = c;
+      imask[p] = m;
+    }
+  }
+  auto plan = std::make_shared<OwnerPlan>();
+  plan->nchunks = nch;
+  plan->nitems = cnt[nch];
+  plan->chunk_rows.upload(rows);
+  std::vector<int32_t> ci(nch + 1);
+  for (int64_t c = 0; c <= nch; ++c) ci[c] = static_cast<int32_t>(cnt[c]);
+  plan->chunk_items.upload(ci);
+  plan->item_cell.upload(icell);
+  plan->item_mask.upload(imask);
+  return plan;
+}
+
+DevArray<int32_t>& get_table(mf_form* a, Integral& I, mf_matrix* A, hipStream_t s) {
+  auto key = std::make_pair(a->uid, I.index);
+  auto it = A->tables.find(key);
+  if (it != A->tables.end()) return it->second;
+  const int ncell = entity_ncell(I.type);
+  const int ni = ncell * a->dm[0]->nd, nj = ncell * a->dm[1]->nd;
+  const int64_t padded = (I.n + kTableLanes - 1) / kTableLanes * kTableLanes;
+  DevArray<int32_t> t;
+  t.alloc(std::max<int64_t>(1, padded * ni * nj));
+  DevArray<int32_t> missing;
+  missing.alloc(1);
+  k_fill_i32(missing.ptr, 1, 0, s);
+  k_build_table(t.ptr, I.d_ents.ptr, I.n, entity_stride(I.type), ncell, a->dm[0]->dev.ptr,
+                a->dm[0]->nd, a->dm[1]->dev.ptr, a->dm[1]->nd, A->bs0, A->bs1, A->row_ptr.ptr,
+                A->rpb.ptr, A->colsb.ptr, missing.ptr, s);
+  int32_t h_missing = 0;
+  MF_HIP(hipMemcpyAsync(&h_missing, missing.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  MF_HIP(hipStreamSynchronize(s));
+  MF_CHECK(h_missing == 0, MF_ENOTFOUND,
+           std::to_string(h_missing) + " element entries are missing from the sparsity pattern");
+  return A->tables.emplace(key, std::move(t)).first->second;
+}
+
+}  // namespace
+
+void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc1,
+                     double diagonal, int zero, hipStream_t s) {
+  MF_CHECK(a->rank == 2, MF_EINVAL, "assemble_matrix needs a bilinear form");
+  MF_CHECK(a->compiled, MF_ESTATE, "form is not compiled");
+  MF_CHECK(A->nrows == a->dm[0]->num_nodes * a->dm[0]->bs &&
+               A->ncols == a->dm[1]->num_nodes * a->dm[1]->bs,
+           MF_ESHAPE, "matrix shape does not match the form's spaces");
+  const bool square = a->dm[0] == a->dm[1];
+  if (bc0) MF_CHECK(bc0->ndofs == A->nrows, MF_ESHAPE, "bc0 size does not match the rows");
+  if (bc1) MF_CHECK(bc1->ndofs == A->ncols, MF_ESHAPE, "bc1 size does not match the columns");
+  const int8_t* m0 = bc0 && bc0->nbc ? bc0->markers.ptr : nullptr;
+  const int8_t* m1 = bc1 && bc1->nbc ? bc1->markers.ptr : nullptr;
+  const int64_t mode = a->opt("matrix_kernel", 0);
+
+  // the first cell integral with a staged description may run owner-computes
+  Integral* owner = nullptr;
+  if (mode != 1)
+    for (auto& I : a->ints)
+      if (I->type == MF_CELL && I->k_owner) {
+        owner = I.get();
+        break;
+      }
+  std::shared_ptr<OwnerPlan> plan;
+  if (owner) {
+    auto key = std::make_pair(a->uid, owner->index);
+    auto it = A->plans.find(key);
+    if (it == A->plans.end()) {
+      plan = build_owner_plan(*a, *owner, *A, a->opt("chunk_values", 4096),
+                              static_cast<int>(a->opt("chunk_rows", 256)));
+      A->plans[key] = plan;
+    } else {
+      plan = it->second;
+    }
+    if (!plan) owner = nullptr;
+  }
+  MF_CHECK(!(mode == 2 && !owner), MF_EUNSUPPORTED,
+           "owner-computes kernel requested but unavailable for this form/matrix");
+  bool diag_done = false;
+  if (owner) {
+    int flags = zero ? 1 : 0;
+    if (zero && square && m0) flags |= 2;
+    diag_done = (flags & 2) != 0;
+    const int64_t nx = 8, per = (plan->nchunks + nx - 1) / nx;
+    launch(owner->k_owner, static_cast<uint32_t>(per * nx),
+           static_cast<uint32_t>(a->opt("owner_block", 256)), 0, s,
+           static_cast<int32_t>(plan->nchunks), plan->chunk_rows.ptr, plan->chunk_items.ptr,
+           plan->item_cell.ptr, plan->item_mask.ptr, a->mesh->cells.ptr, a->mesh->x.ptr,
+           a->packed.ptr, a->consts.ptr, a->dm[0]->dev.ptr, a->dm[1]->dev.ptr, m0, m1,
+           A->row_ptr.ptr, A->rpb.ptr, A->colsb.ptr, A->values, diagonal, flags);
+  } else if (zero) {
+    k_fill(A->values, A->nnz, 0.0, s);
+  }
+  const uint32_t block = static_cast<uint32_t>(a->opt("block", 128));
+  for (auto& I : a->ints) {
+    if (I.get() == owner || I->n == 0) continue;
+    auto& table = get_table(a, *I, A, s);
+    launch(I->k_mat, static_cast<uint32_t>((I->n + block - 1) / block), block, 0, s,
+           static_cast<int32_t>(I->n), I->d_ents.ptr, a->mesh->cells.ptr, a->mesh->x.ptr,
+           a->packed.ptr, a->consts.ptr, a->dm[0]->dev.ptr, a->dm[1]->dev.ptr, m0, m1,
+           table.ptr, A->row_ptr.ptr, A->values, a->mesh->fperm.ptr);
+  }
+  if (square && m0 && !diag_done)
+    k_set_diagonal(A->values, A->row_ptr.ptr, A->cols.ptr, bc0->dofs.ptr, bc0->n_owned, A->nrows,
+                   diagonal, s);
+}
+
+}  // namespace mf

@@ -1,0 +1,235 @@
+// Internal declarations of the mifem engine (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mifem.h"
+
+namespace mf {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define MF_STR2(x) #x
+#define MF_STR(x) MF_STR2(x)
+#define MF_THROW(code, msg) \
+  throw ::mf::Error((code), std::string(msg) + " at " __FILE__ ":" MF_STR(__LINE__))
+#define MF_CHECK(cond, code, msg) \
+  do {                            \
+    if (!(cond)) MF_THROW(code, msg); \
+  } while (0)
+#define MF_HIP(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      MF_THROW(MF_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+// Owning device array.
+template <class T>
+struct DevArray {
+  T* ptr = nullptr;
+  int64_t n = 0;
+  DevArray() = default;
+  DevArray(const DevArray&) = delete;
+  DevArray& operator=(const DevArray&) = delete;
+  DevArray(DevArray&& o) noexcept : ptr(o.ptr), n(o.n) { o.ptr = nullptr; o.n = 0; }
+  DevArray& operator=(DevArray&& o) noexcept {
+    if (this != &o) { reset(); ptr = o.ptr; n = o.n; o.ptr = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevArray() { reset(); }
+  void reset() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    n = 0;
+  }
+  void alloc(int64_t count) {
+    reset();
+    if (count <= 0) return;
+    MF_HIP(hipMalloc(reinterpret_cast<void**>(&ptr), sizeof(T) * count));
+    n = count;
+  }
+  void upload(const T* host, int64_t count) {
+    alloc(count);
+    if (count > 0) MF_HIP(hipMemcpy(ptr, host, sizeof(T) * count, hipMemcpyHostToDevice));
+  }
+  void upload(const std::vector<T>& v) { upload(v.data(), static_cast<int64_t>(v.size())); }
+  std::vector<T> download() const {
+    std::vector<T> h(n);
+    if (n > 0) MF_HIP(hipMemcpy(h.data(), ptr, sizeof(T) * n, hipMemcpyDeviceToHost));
+    return h;
+  }
+};
+
+// hipRTC-built module (RAII).
+struct Module {
+  hipModule_t mod = nullptr;
+  Module() = default;
+  Module(const Module&) = delete;
+  Module& operator=(const Module&) = delete;
+  ~Module() { if (mod) (void)hipModuleUnload(mod); }
+  void load(const std::vector<char>& code);
+  hipFunction_t get(const std::string& name) const;
+};
+
+// Compile `src` with hipRTC for `arch`; code objects are cached under cache_dir
+// keyed by a 64-bit FNV-1a hash of (source, options, compiler version).
+std::vector<char> rtc_compile(const std::string& src, const std::string& name,
+                              const std::vector<std::string>& opts, const std::string& cache_dir,
+                              bool* from_cache);
+
+std::string default_cache_dir();
+uint64_t fnv1a(const std::string& s, uint64_t h = 1469598103934665603ULL);
+
+// Launch a hipRTC kernel with an argument pack.
+template <class... Args>
+void launch(hipFunction_t f, uint32_t grid, uint32_t block, uint32_t shmem, hipStream_t s,
+            Args... args) {
+  if (grid == 0) return;
+  void* ptrs[] = {static_cast<void*>(&args)...};
+  MF_HIP(hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, shmem, s, ptrs, nullptr));
+}
+
+// ------------------------------------------------------------ util kernels
+// (precompiled for gfx950 in util_kernels.hip; launchers run on `s`)
+void k_fill(double* x, int64_t n, double v, hipStream_t s);
+void k_fill_i32(int32_t* x, int64_t n, int32_t v, hipStream_t s);
+void k_mark(int8_t* markers, int64_t ndofs, const int32_t* dofs, int64_t nbc, hipStream_t s);
+void k_pack(double* packed, int64_t ncells, int W, int off, const double* values,
+            const int32_t* dofmap, int nd, int bs, hipStream_t s);
+void k_set_bc(double* b, int64_t n, const int32_t* dofs, int64_t nbc, const double* g,
+              const double* x0, double scale, hipStream_t s);
+void k_set_diagonal(double* values, const int32_t* row_ptr, const int32_t* cols,
+                    const int32_t* dofs, int64_t nbc, int64_t nrows, double diag, hipStream_t s);
+void k_unroll_cols(int32_t* cols, const int32_t* row_ptr, const int32_t* rpb,
+                   const int32_t* colsb, int64_t nrows_b, int bs0, int bs1, hipStream_t s);
+// Offset table of one integral: for every entity e and joint local nodes (i, j)
+// the CSR position of scalar entry (bs0*node_i, bs1*node_j), stored wave64
+// interleaved: table[(e/64)*(NI*NJ)*64 + (i*NJ+j)*64 + e%64].  Missing entries
+// increment *missing.
+void k_build_table(int32_t* table, const int32_t* ents, int64_t nents, int stride, int ncell,
+                   const int32_t* dm0, int nd0, const int32_t* dm1, int nd1, int bs0, int bs1,
+                   const int32_t* row_ptr, const int32_t* rpb, const int32_t* colsb,
+                   int32_t* missing, hipStream_t s);
+// Appends the index of every entity with a constrained column dof to list.
+void k_lift_list(int32_t* list, int32_t* count, const int32_t* ents, int64_t nents, int stride,
+                 int ncell, const int32_t* dm1, int nd1, int bs1, const int8_t* bc1,
+                 hipStream_t s);
+
+}  // namespace mf
+
+// ------------------------------------------------------------ handles
+struct mf_context {
+  int device = 0;
+  std::string arch;
+  std::string cache_dir;
+  int num_cus = 0;
+  size_t lds_per_block = 0;
+};
+
+struct mf_mesh {
+  mf_context* ctx = nullptr;
+  mf::DevArray<double> x;
+  mf::DevArray<int32_t> cells;
+  mf::DevArray<uint8_t> fperm;
+  int64_t num_vertices = 0, num_cells = 0;
+  int nv = 0, nf = 0;
+};
+
+struct mf_dofmap {
+  mf_context* ctx = nullptr;
+  std::vector<int32_t> host;
+  mf::DevArray<int32_t> dev;
+  int64_t num_cells = 0, num_nodes = 0;
+  int nd = 0, bs = 1;
+};
+
+struct mf_bc {
+  mf_context* ctx = nullptr;
+  int64_t ndofs = 0, nbc = 0, n_owned = 0;
+  mf::DevArray<int32_t> dofs;
+  mf::DevArray<int8_t> markers;
+  mf::DevArray<double> g;
+  uint64_t uid = 0;
+};
+
+namespace mf {
+struct Integral {
+  int type = 0, id = -1;
+  std::string tab_name, tab_src, staged_src;
+  std::vector<int32_t> h_ents;
+  DevArray<int32_t> d_ents;
+  int64_t n = 0;
+  int index = 0;                      // position in the form
+  hipFunction_t k_mat = nullptr, k_vec = nullptr, k_lift = nullptr, k_owner = nullptr;
+  // lifting entity lists per bc object uid
+  std::map<uint64_t, std::pair<DevArray<int32_t>, int64_t>> lift_lists;
+};
+struct Coef {
+  mf_dofmap* dm = nullptr;
+  int64_t size = 0;
+  DevArray<double> own;
+  const double* ext = nullptr;
+  int offset = 0, width = 0;
+};
+struct OwnerPlan;   // cooperative owner-computes schedule (matrix.cpp)
+}  // namespace mf
+
+struct mf_form {
+  mf_context* ctx = nullptr;
+  mf_mesh* mesh = nullptr;
+  int rank = 0;
+  mf_dofmap* dm[2] = {nullptr, nullptr};
+  std::vector<std::unique_ptr<mf::Integral>> ints;
+  std::vector<mf::Coef> coefs;
+  int W = 0;
+  mf::DevArray<double> packed;
+  mf::DevArray<double> consts;
+  std::vector<double> h_consts;
+  mf::DevArray<int32_t> d_coff;
+  bool compiled = false;
+  std::unique_ptr<mf::Module> mod;
+  std::string source;
+  std::map<std::string, int64_t> options;
+  uint64_t uid = 0;
+  int64_t opt(const std::string& k, int64_t dflt) const {
+    auto it = options.find(k);
+    return it == options.end() ? dflt : it->second;
+  }
+};
+
+struct mf_matrix {
+  mf_form* form = nullptr;
+  int64_t nrows = 0, ncols = 0, nnz = 0, nrows_b = 0, nnz_b = 0;
+  int bs0 = 1, bs1 = 1;
+  mf::DevArray<int32_t> row_ptr, cols, rpb, colsb;
+  std::vector<int32_t> h_rpb, h_colsb;
+  mf::DevArray<double> own_values;
+  double* values = nullptr;
+  // per (form uid, integral index) offset tables
+  std::map<std::pair<uint64_t, int>, mf::DevArray<int32_t>> tables;
+  std::map<std::pair<uint64_t, int>, std::shared_ptr<mf::OwnerPlan>> plans;
+};
+
+namespace mf {
+// code generation (codegen.cpp)
+std::string generate_form_source(const mf_form& f);
+std::string kernel_name(const char* what, int type, int index);
+// sparsity (sparsity.cpp): blocked pattern of all integrals
+void build_pattern(const mf_form& a, std::vector<int32_t>& rpb, std::vector<int32_t>& colsb);
+// matrix assembly paths (matrix.cpp)
+void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc1,
+                     double diagonal, int zero, hipStream_t s);
+int entity_stride(int type);
+int entity_ncell(int type);
+}  // namespace mf

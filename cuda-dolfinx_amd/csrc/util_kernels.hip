@@ -1,0 +1,217 @@
+// Precompiled utility kernels (gfx950).  They replace the util module the
+// reference builds with NVRTC in debug mode (CUDAAssembler.cpp:38-164,
+// compiled with --device-debug per fem.cpp:210): zero fill, coefficient
+// packing, set_bc, set_diagonal, plus the setup kernels of this engine
+// (bs-unrolling of the CSR columns, offset tables, lifting entity lists).
+#include "mf_internal.h"
+
+namespace mf {
+namespace {
+
+constexpr int kBlock = 256;
+
+inline uint32_t grid_for(int64_t n, int per_thread = 1) {
+  int64_t g = (n + (int64_t)kBlock * per_thread - 1) / ((int64_t)kBlock * per_thread);
+  if (g > 65535LL * 32) g = 65535LL * 32;
+  return static_cast<uint32_t>(g < 1 ? 1 : g);
+}
+
+__global__ void fill_kernel(double* __restrict__ x, int64_t n, double v) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = v;
+}
+
+__global__ void fill_i32_kernel(int32_t* __restrict__ x, int64_t n, int32_t v) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = v;
+}
+
+__global__ void mark_kernel(int8_t* __restrict__ m, int64_t ndofs, const int32_t* __restrict__ dofs,
+                            int64_t nbc) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nbc; i += stride) {
+    int32_t d = dofs[i];
+    if (d >= 0 && d < ndofs) m[d] = 1;
+  }
+}
+
+// packed[c*W + off + k*bs + j] = values[bs*dofmap[c*nd + k] + j]
+__global__ void pack_kernel(double* __restrict__ packed, int64_t ncells, int W, int off,
+                            const double* __restrict__ values, const int32_t* __restrict__ dofmap,
+                            int nd, int bs) {
+  const int64_t total = ncells * nd * bs;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < total; l += stride) {
+    int64_t c = l / (nd * bs);
+    int r = static_cast<int>(l - c * nd * bs);
+    int k = r / bs, j = r - k * bs;
+    packed[c * W + off + r] = values[(int64_t)bs * dofmap[c * nd + k] + j];
+  }
+}
+
+__global__ void set_bc_kernel(double* __restrict__ b, int64_t n, const int32_t* __restrict__ dofs,
+                              int64_t nbc, const double* __restrict__ g,
+                              const double* __restrict__ x0, double scale) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nbc; i += stride) {
+    int32_t d = dofs[i];
+    if (d < 0 || d >= n) continue;
+    b[d] = scale * (g[d] - (x0 ? x0[d] : 0.0));
+  }
+}
+
+__device__ __forceinline__ int64_t search(const int32_t* __restrict__ cols, int64_t lo, int64_t hi,
+                                          int32_t key) {
+  // first position in [lo, hi) with cols[p] >= key
+  while (lo < hi) {
+    int64_t mid = lo + ((hi - lo) >> 1);
+    if (cols[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void set_diag_kernel(double* __restrict__ values, const int32_t* __restrict__ row_ptr,
+                                const int32_t* __restrict__ cols, const int32_t* __restrict__ dofs,
+                                int64_t nbc, int64_t nrows, double diag) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nbc; i += stride) {
+    int32_t r = dofs[i];
+    if (r < 0 || r >= nrows) continue;
+    int64_t e = row_ptr[r + 1];
+    int64_t p = search(cols, row_ptr[r], e, r);
+    if (p < e && cols[p] == r) values[p] = diag;
+  }
+}
+
+// One thread per blocked entry (r, k): writes the bs0*bs1 scalar columns.
+__global__ void unroll_kernel(int32_t* __restrict__ cols, const int32_t* __restrict__ row_ptr,
+                              const int32_t* __restrict__ rpb, const int32_t* __restrict__ colsb,
+                              int64_t nrows_b, int bs0, int bs1) {
+  int64_t nnzb = rpb[nrows_b];
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nnzb; q += stride) {
+    // row of q by binary search in rpb
+    int64_t lo = 0, hi = nrows_b;
+    while (hi - lo > 1) {
+      int64_t mid = (lo + hi) >> 1;
+      if (rpb[mid] <= q) lo = mid;
+      else hi = mid;
+    }
+    int64_t r = lo, k = q - rpb[r];
+    int32_t c = colsb[q];
+    for (int a = 0; a < bs0; ++a) {
+      int64_t base = (int64_t)row_ptr[r * bs0 + a] + k * bs1;
+      for (int b = 0; b < bs1; ++b) cols[base + b] = bs1 * c + b;
+    }
+  }
+}
+
+__global__ void table_kernel(int32_t* __restrict__ table, const int32_t* __restrict__ ents,
+                             int64_t nents, int stride, int ncell, const int32_t* __restrict__ dm0,
+                             int nd0, const int32_t* __restrict__ dm1, int nd1, int bs0, int bs1,
+                             const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ rpb,
+                             const int32_t* __restrict__ colsb, int32_t* __restrict__ missing) {
+  const int NI = ncell * nd0, NJ = ncell * nd1;
+  const int64_t total = nents * NI * NJ;
+  int64_t gs = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < total; l += gs) {
+    int64_t e = l / (NI * NJ);
+    int ij = static_cast<int>(l - e * NI * NJ);
+    int i = ij / NJ, j = ij - i * NJ;
+    int32_t ci = ents[e * stride + 2 * (i / nd0)];
+    int32_t cj = ents[e * stride + 2 * (j / nd1)];
+    int32_t rn = dm0[(int64_t)ci * nd0 + (i % nd0)];
+    int32_t cn = dm1[(int64_t)cj * nd1 + (j % nd1)];
+    int64_t lo = rpb[rn], hi = rpb[rn + 1];
+    int64_t p = search(colsb, lo, hi, cn);
+    int32_t v = -1;
+    if (p < hi && colsb[p] == cn)
+      v = static_cast<int32_t>((int64_t)row_ptr[(int64_t)bs0 * rn] + (p - lo) * bs1);
+    else
+      atomicAdd(missing, 1);
+    table[(e >> 6) * (int64_t)NI * NJ * 64 + (int64_t)ij * 64 + (e & 63)] = v;
+  }
+}
+
+__global__ void lift_list_kernel(int32_t* __restrict__ list, int32_t* __restrict__ count,
+                                 const int32_t* __restrict__ ents, int64_t nents, int stride,
+                                 int ncell, const int32_t* __restrict__ dm1, int nd1, int bs1,
+                                 const int8_t* __restrict__ bc1) {
+  int64_t gs = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nents; e += gs) {
+    bool has = false;
+    for (int ci = 0; ci < ncell && !has; ++ci) {
+      int32_t c = ents[e * stride + 2 * ci];
+      for (int j = 0; j < nd1 && !has; ++j) {
+        int32_t node = dm1[(int64_t)c * nd1 + j];
+        for (int b = 0; b < bs1; ++b) has |= bc1[(int64_t)bs1 * node + b] != 0;
+      }
+    }
+    if (has) list[atomicAdd(count, 1)] = static_cast<int32_t>(e);
+  }
+}
+
+}  // namespace
+
+void k_fill(double* x, int64_t n, double v, hipStream_t s) {
+  if (n <= 0) return;
+  fill_kernel<<<grid_for(n, 4), kBlock, 0, s>>>(x, n, v);
+  MF_HIP(hipGetLastError());
+}
+void k_fill_i32(int32_t* x, int64_t n, int32_t v, hipStream_t s) {
+  if (n <= 0) return;
+  fill_i32_kernel<<<grid_for(n, 4), kBlock, 0, s>>>(x, n, v);
+  MF_HIP(hipGetLastError());
+}
+void k_mark(int8_t* m, int64_t ndofs, const int32_t* dofs, int64_t nbc, hipStream_t s) {
+  if (nbc <= 0) return;
+  mark_kernel<<<grid_for(nbc), kBlock, 0, s>>>(m, ndofs, dofs, nbc);
+  MF_HIP(hipGetLastError());
+}
+void k_pack(double* packed, int64_t ncells, int W, int off, const double* values,
+            const int32_t* dofmap, int nd, int bs, hipStream_t s) {
+  if (ncells <= 0) return;
+  pack_kernel<<<grid_for(ncells * nd * bs), kBlock, 0, s>>>(packed, ncells, W, off, values, dofmap,
+                                                             nd, bs);
+  MF_HIP(hipGetLastError());
+}
+void k_set_bc(double* b, int64_t n, const int32_t* dofs, int64_t nbc, const double* g,
+              const double* x0, double scale, hipStream_t s) {
+  if (nbc <= 0) return;
+  set_bc_kernel<<<grid_for(nbc), kBlock, 0, s>>>(b, n, dofs, nbc, g, x0, scale);
+  MF_HIP(hipGetLastError());
+}
+void k_set_diagonal(double* values, const int32_t* row_ptr, const int32_t* cols,
+                    const int32_t* dofs, int64_t nbc, int64_t nrows, double diag, hipStream_t s) {
+  if (nbc <= 0) return;
+  set_diag_kernel<<<grid_for(nbc), kBlock, 0, s>>>(values, row_ptr, cols, dofs, nbc, nrows, diag);
+  MF_HIP(hipGetLastError());
+}
+void k_unroll_cols(int32_t* cols, const int32_t* row_ptr, const int32_t* rpb, const int32_t* colsb,
+                   int64_t nrows_b, int bs0, int bs1, hipStream_t s) {
+  unroll_kernel<<<grid_for(nrows_b * 32), kBlock, 0, s>>>(cols, row_ptr, rpb, colsb, nrows_b, bs0,
+                                                          bs1);
+  MF_HIP(hipGetLastError());
+}
+void k_build_table(int32_t* table, const int32_t* ents, int64_t nents, int stride, int ncell,
+                   const int32_t* dm0, int nd0, const int32_t* dm1, int nd1, int bs0, int bs1,
+                   const int32_t* row_ptr, const int32_t* rpb, const int32_t* colsb,
+                   int32_t* missing, hipStream_t s) {
+  if (nents <= 0) return;
+  int64_t total = nents * ncell * nd0 * ncell * nd1;
+  table_kernel<<<grid_for(total, 4), kBlock, 0, s>>>(table, ents, nents, stride, ncell, dm0, nd0,
+                                                     dm1, nd1, bs0, bs1, row_ptr, rpb, colsb,
+                                                     missing);
+  MF_HIP(hipGetLastError());
+}
+void k_lift_list(int32_t* list, int32_t* count, const int32_t* ents, int64_t nents, int stride,
+                 int ncell, const int32_t* dm1, int nd1, int bs1, const int8_t* bc1,
+                 hipStream_t s) {
+  if (nents <= 0) return;
+  lift_list_kernel<<<grid_for(nents), kBlock, 0, s>>>(list, count, ents, nents, stride, ncell, dm1,
+                                                      nd1, bs1, bc1);
+  MF_HIP(hipGetLastError());
+}
+
+}  // namespace mf

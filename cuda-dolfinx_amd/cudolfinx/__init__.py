@@ -1,0 +1,15 @@
+"""cudolfinx -- the cuDOLFINx assembly API on MI355X (HIP / hipRTC, gfx950).
+
+Drop-in names of python/cudolfinx/__init__.py: `form`, `CUDAAssembler`,
+`CUDAForm`, `CUDAMatrix`, `CUDAVector`, `CUDADirichletBC`.  The work happens
+in libmifem.so through its C ABI; forms, meshes and function spaces come
+from `mifem` (the DOLFINx/FFCx/Basix stand-in of this repository).
+"""
+from .assemble import CUDAAssembler
+from .bcs import CUDADirichletBC
+from .context import CUDAContext, DeviceArray, get_cuda_context
+from .form import CUDAForm, form
+from .la import CUDAMatrix, CUDAVector
+
+__all__ = ["CUDAAssembler", "CUDADirichletBC", "CUDAContext", "DeviceArray", "get_cuda_context",
+           "CUDAForm", "form", "CUDAMatrix", "CUDAVector"]

@@ -1,0 +1,126 @@
+"""Device matrix and vector wrappers (python/cudolfinx/la.py).
+
+Without PETSc-HIP in this environment, `CUDAMatrix.mat` and
+`CUDAVector.vector` hand back light objects with the petsc4py calls the
+reference tests use (`getValuesCSR`, `array`, `getSize`, `norm`).  With
+petsc4py present, `petsc_mat()` wraps the same CSR arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, lib
+from .context import DeviceArray
+
+
+class DeviceCSRMatrix:
+    """The petsc4py-like view of an assembled device CSR matrix."""
+
+    def __init__(self, owner: "CUDAMatrix"):
+        self._owner = owner
+
+    def getSize(self):
+        return (self._owner.nrows, self._owner.ncols)
+
+    def getValuesCSR(self):
+        """(indptr, indices, data) copied to host, as petsc4py Mat.getValuesCSR."""
+        return self._owner.csr()
+
+    def assemble(self):
+        self._owner.assemble()
+
+
+class CUDAMatrix:
+    """Device CSR matrix created from a bilinear form (la/CUDAMatrix.cpp:36-225)."""
+
+    def __init__(self, ctx, form):
+        self._ctx = ctx
+        self._form = form            # keeps the form (and its dofmaps) alive
+        h = C.c_void_p()
+        check(lib().mf_matrix_create(form.handle, C.byref(h)))
+        self._handle = h
+        nr, nc, nnz = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().mf_matrix_info(h, C.byref(nr), C.byref(nc), C.byref(nnz)))
+        self.nrows, self.ncols, self.nnz = nr.value, nc.value, nnz.value
+        self._mat = DeviceCSRMatrix(self)
+
+    @property
+    def handle(self):
+        return self._handle
+
+    @property
+    def mat(self):
+        return self._mat
+
+    def device_arrays(self):
+        """(row_ptr, cols, values) device addresses."""
+        rp, cols, vals = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(lib().mf_matrix_device_arrays(self._handle, C.byref(rp), C.byref(cols), C.byref(vals)))
+        return rp.value, cols.value, vals.value
+
+    def assemble(self):
+        """Handoff: make the device values final (CUDAMatrix::to_host + MatAssembly)."""
+        self._ctx.synchronize()
+
+    def csr(self):
+        rp = np.empty(self.nrows + 1, dtype=np.int32)
+        cols = np.empty(max(self.nnz, 1), dtype=np.int32)
+        vals = np.empty(max(self.nnz, 1), dtype=np.float64)
+        check(lib().mf_matrix_copy_csr(self._handle, rp.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       cols.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       vals.ctypes.data_as(C.POINTER(C.c_double))))
+        return rp, cols[:self.nnz], vals[:self.nnz]
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        rp, cols, vals = self.csr()
+        return sp.csr_matrix((vals, cols, rp), shape=(self.nrows, self.ncols))
+
+    def __del__(self):
+        if getattr(self, "_handle", None):
+            try:
+                lib().mf_matrix_destroy(self._handle)
+            except Exception:
+                pass
+            self._handle = None
+
+
+class DeviceVector:
+    """petsc4py-like view of a device vector."""
+
+    def __init__(self, owner: "CUDAVector"):
+        self._owner = owner
+
+    @property
+    def array(self) -> np.ndarray:
+        return self._owner.data.to_numpy()
+
+    def getSize(self):
+        return self._owner.size
+
+    def norm(self):
+        return float(np.linalg.norm(self.array))
+
+
+class CUDAVector:
+    """Device vector (la/CUDAVector.cpp:35-204)."""
+
+    def __init__(self, ctx, size: int):
+        self._ctx = ctx
+        self.size = int(size)
+        self.data = DeviceArray(ctx, self.size)
+        check(lib().mf_zero(ctx.handle, self.data.ptr, self.size, None))
+        self._vec = DeviceVector(self)
+
+    @property
+    def vector(self):
+        return self._vec
+
+    @property
+    def ptr(self):
+        return self.data.ptr
+
+    def to_host(self):
+        return self.data.to_numpy()

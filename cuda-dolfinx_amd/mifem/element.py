@@ -1,0 +1,291 @@
+"""Reference elements and quadrature (the Basix stand-in).
+
+Elements
+--------
+* ``Lagrange(tetrahedron, k)``, k = 1..3, equispaced nodes.  Local order is
+  the Basix entity order: 4 vertices, then edges (e0=(2,3), e1=(1,3),
+  e2=(1,2), e3=(0,3), e4=(0,2), e5=(0,1); nodes ordered from the first to
+  the second vertex), then faces (face i opposite vertex i), then interior.
+* ``DG(tetrahedron, k)``: same nodes, no inter-cell sharing.
+* ``Lagrange(hexahedron, k)``, k = 1..3, Gauss-Lobatto-Legendre (GLL) nodes
+  in lexicographic tensor order (index a + (k+1) b + (k+1)^2 c).
+
+Each node also carries an *integer* position: barycentric multi-indices
+(tetrahedra, sum = k) or tensor indices (hexahedra).  The function-space
+layer turns those into global lattice keys, so shared nodes are identified
+exactly.
+
+The basis is built by inverting the Vandermonde matrix of the monomials
+(P_k or Q_k) at the nodes; tables are evaluated in float64.
+
+Quadrature
+----------
+* tetrahedra: centroid (degree 1), Hammer-Stroud 4 points (degree 2),
+  otherwise the collapsed (Stroud conical) Gauss-Jacobi product with
+  m = ceil((d+1)/2) points per direction;
+* triangles (tetrahedron facets): centroid, 3-point degree-2 rule,
+  otherwise collapsed Gauss-Jacobi;
+* hexahedra / quadrilaterals: Gauss-Legendre tensor products.
+"""
+from __future__ import annotations
+
+import itertools
+import math
+from dataclasses import dataclass
+from functools import lru_cache
+
+import numpy as np
+from scipy.special import roots_jacobi, roots_legendre
+
+TET_VERTICES = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=np.float64)
+TET_EDGES = [(2, 3), (1, 3), (1, 2), (0, 3), (0, 2), (0, 1)]
+TET_FACES = [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)]
+# Outward unit normals of the reference tetrahedron facets (facet i opposite vertex i).
+TET_FACET_NORMALS = np.array([[1, 1, 1] / np.sqrt(3.0), [-1, 0, 0], [0, -1, 0], [0, 0, -1]],
+                             dtype=np.float64)
+
+
+def gll_points(k: int) -> np.ndarray:
+    """Gauss-Lobatto-Legendre points of degree k on [0, 1] (k + 1 points)."""
+    if k == 1:
+        return np.array([0.0, 1.0])
+    # interior GLL points are the roots of P'_k, i.e. Jacobi(k-1, 1, 1)
+    r, _ = roots_jacobi(k - 1, 1.0, 1.0)
+    return np.concatenate([[0.0], np.sort((r + 1.0) / 2.0), [1.0]])
+
+
+@dataclass(frozen=True)
+class Element:
+    family: str          # "P" (continuous Lagrange) or "DG"
+    cell: str            # "tetrahedron" or "hexahedron"
+    degree: int
+
+    @property
+    def discontinuous(self) -> bool:
+        return self.family == "DG"
+
+    @property
+    def ndofs(self) -> int:
+        k = self.degree
+        if self.cell == "tetrahedron":
+            return (k + 1) * (k + 2) * (k + 3) // 6
+        return (k + 1) ** 3
+
+    # ------------------------------------------------------------------ nodes
+    @property
+    def node_indices(self) -> np.ndarray:
+        """Integer node positions.
+
+        tetrahedron: (nd, 4) barycentric multi-indices m (sum k);
+        node X = sum_v m_v / k * vertex_v.
+        hexahedron: (nd, 3) tensor indices (a, b, c) in 0..k.
+        """
+        return _node_indices(self.cell, self.degree)
+
+    @property
+    def nodes(self) -> np.ndarray:
+        """(nd, 3) reference coordinates of the nodes."""
+        idx = self.node_indices
+        k = self.degree
+        if self.cell == "tetrahedron":
+            if k == 0:
+                return np.array([[0.25, 0.25, 0.25]])
+            return (idx / k) @ TET_VERTICES
+        p = gll_points(k)
+        return p[idx]
+
+    # ------------------------------------------------------------------ basis
+    def _monomials(self):
+        k = self.degree
+        if self.cell == "tetrahedron":
+            return [e for e in itertools.product(range(k + 1), repeat=3) if sum(e) <= k]
+        return list(itertools.product(range(k + 1), repeat=3))
+
+    @property
+    def _coeffs(self) -> np.ndarray:
+        return _basis_coeffs(self.cell, self.degree)
+
+    def tabulate(self, points: np.ndarray, nderiv: int = 1) -> np.ndarray:
+        """Values and first derivatives at points.
+
+        Returns (1 + 3*nderiv, npts, nd): [phi, dphi/dX, dphi/dY, dphi/dZ].
+        """
+        pts = np.atleast_2d(np.asarray(points, dtype=np.float64))
+        if self.cell == "hexahedron":
+            return self._tabulate_tensor(pts, nderiv)
+        mons = self._monomials()
+        out = []
+        V = np.empty((pts.shape[0], len(mons)))
+        for j, e in enumerate(mons):
+            V[:, j] = pts[:, 0] ** e[0] * pts[:, 1] ** e[1] * pts[:, 2] ** e[2]
+        out.append(V @ self._coeffs)
+        if nderiv:
+            for d in range(3):
+                D = np.zeros_like(V)
+                for j, e in enumerate(mons):
+                    if e[d] == 0:
+                        continue
+                    ee = list(e)
+                    ee[d] -= 1
+                    D[:, j] = e[d] * pts[:, 0] ** ee[0] * pts[:, 1] ** ee[1] * pts[:, 2] ** ee[2]
+                out.append(D @ self._coeffs)
+        return np.stack(out)
+
+
+    def _tabulate_tensor(self, pts: np.ndarray, nderiv: int) -> np.ndarray:
+        """Q_k as a tensor product of 1D Lagrange polynomials on the GLL nodes."""
+        k = self.degree
+        C1 = np.linalg.inv(np.vander(gll_points(k), k + 1, increasing=True))   # (k+1, k+1)
+        val, der = [], []
+        for d in range(3):
+            t = pts[:, d]
+            V = np.vander(t, k + 1, increasing=True)
+            D = np.zeros_like(V)
+            for j in range(1, k + 1):
+                D[:, j] = j * t ** (j - 1)
+            val.append(V @ C1)
+            der.append(D @ C1)
+        idx = self.node_indices
+        a, b, c = idx[:, 0], idx[:, 1], idx[:, 2]
+        out = [val[0][:, a] * val[1][:, b] * val[2][:, c]]
+        if nderiv:
+            out.append(der[0][:, a] * val[1][:, b] * val[2][:, c])
+            out.append(val[0][:, a] * der[1][:, b] * val[2][:, c])
+            out.append(val[0][:, a] * val[1][:, b] * der[2][:, c])
+        return np.stack(out)
+
+
+@lru_cache(maxsize=None)
+def _node_indices(cell: str, k: int) -> np.ndarray:
+    if cell == "hexahedron":
+        return np.array([(a, b, c) for c in range(k + 1) for b in range(k + 1)
+                         for a in range(k + 1)], dtype=np.int64)
+    if k == 0:
+        return np.array([[1, 1, 1, 1]], dtype=np.int64)   # centroid, scaled by 1/4 below
+    nodes = []
+    for v in range(4):
+        m = [0, 0, 0, 0]
+        m[v] = k
+        nodes.append(m)
+    for a, b in TET_EDGES:
+        for s in range(1, k):
+            m = [0, 0, 0, 0]
+            m[a], m[b] = k - s, s
+            nodes.append(m)
+    for f in TET_FACES:
+        # interior nodes of the face, lexicographic in the face's own (s, t)
+        for t in range(1, k):
+            for s in range(1, k - t):
+                m = [0, 0, 0, 0]
+                m[f[0]], m[f[1]], m[f[2]] = k - s - t, s, t
+                nodes.append(m)
+    for t3 in range(1, k):
+        for t2 in range(1, k - t3):
+            for t1 in range(1, k - t2 - t3):
+                nodes.append([k - t1 - t2 - t3, t1, t2, t3])
+    return np.array(nodes, dtype=np.int64)
+
+
+@lru_cache(maxsize=None)
+def _basis_coeffs(cell: str, k: int) -> np.ndarray:
+    el = Element("P", cell, k)
+    X = el.nodes
+    mons = el._monomials()
+    V = np.empty((X.shape[0], len(mons)))
+    for j, e in enumerate(mons):
+        V[:, j] = X[:, 0] ** e[0] * X[:, 1] ** e[1] * X[:, 2] ** e[2]
+    return np.linalg.inv(V)
+
+
+# ---------------------------------------------------------------- quadrature
+@dataclass(frozen=True)
+class Quadrature:
+    points: np.ndarray   # (nq, dim)
+    weights: np.ndarray  # (nq,)
+
+
+def _gauss_jacobi01(m: int, alpha: float) -> tuple[np.ndarray, np.ndarray]:
+    """m-point Gauss-Jacobi rule for weight (1-t)^alpha on [0, 1]."""
+    x, w = roots_jacobi(m, alpha, 0.0)
+    return (x + 1.0) / 2.0, w / 2.0 ** (alpha + 1.0)
+
+
+def _gauss_legendre01(m: int) -> tuple[np.ndarray, np.ndarray]:
+    x, w = roots_legendre(m)
+    return (x + 1.0) / 2.0, w / 2.0
+
+
+@lru_cache(maxsize=None)
+def tet_collapsed(m: int) -> Quadrature:
+    """Stroud conical product: exact for degree 2m - 1 on the unit tetrahedron."""
+    a, wa = _gauss_jacobi01(m, 2.0)
+    b, wb = _gauss_jacobi01(m, 1.0)
+    c, wc = _gauss_legendre01(m)
+    pts, wts = [], []
+    for i in range(m):
+        for j in range(m):
+            for l in range(m):
+                x = a[i]
+                y = b[j] * (1.0 - a[i])
+                z = c[l] * (1.0 - a[i]) * (1.0 - b[j])
+                pts.append((x, y, z))
+                wts.append(wa[i] * wb[j] * wc[l])
+    return Quadrature(np.array(pts), np.array(wts))
+
+
+@lru_cache(maxsize=None)
+def tri_collapsed(m: int) -> Quadrature:
+    a, wa = _gauss_jacobi01(m, 1.0)
+    b, wb = _gauss_legendre01(m)
+    pts, wts = [], []
+    for i in range(m):
+        for j in range(m):
+            pts.append((a[i], b[j] * (1.0 - a[i])))
+            wts.append(wa[i] * wb[j])
+    return Quadrature(np.array(pts), np.array(wts))
+
+
+@lru_cache(maxsize=None)
+def tet_rule(degree: int) -> Quadrature:
+    if degree <= 1:
+        return Quadrature(np.array([[0.25, 0.25, 0.25]]), np.array([1.0 / 6.0]))
+    if degree == 2:
+        a = (5.0 - math.sqrt(5.0)) / 20.0
+        b = (5.0 + 3.0 * math.sqrt(5.0)) / 20.0
+        pts = np.array([[a, a, a], [b, a, a], [a, b, a], [a, a, b]])
+        return Quadrature(pts, np.full(4, 1.0 / 24.0))
+    return tet_collapsed((degree + 2) // 2)
+
+
+@lru_cache(maxsize=None)
+def tri_rule(degree: int) -> Quadrature:
+    if degree <= 1:
+        return Quadrature(np.array([[1.0 / 3.0, 1.0 / 3.0]]), np.array([0.5]))
+    if degree == 2:
+        pts = np.array([[1 / 6, 1 / 6], [2 / 3, 1 / 6], [1 / 6, 2 / 3]])
+        return Quadrature(pts, np.full(3, 1.0 / 6.0))
+    return tri_collapsed((degree + 2) // 2)
+
+
+@lru_cache(maxsize=None)
+def hex_gauss(m: int) -> Quadrature:
+    """m^3-point Gauss-Legendre rule, points ordered x fastest."""
+    x, w = _gauss_legendre01(m)
+    pts = np.array([(x[a], x[b], x[c]) for c in range(m) for b in range(m) for a in range(m)])
+    wts = np.array([w[a] * w[b] * w[c] for c in range(m) for b in range(m) for a in range(m)])
+    return Quadrature(pts, wts)
+
+
+@lru_cache(maxsize=None)
+def quad_gauss(m: int) -> Quadrature:
+    x, w = _gauss_legendre01(m)
+    pts = np.array([(x[a], x[b]) for b in range(m) for a in range(m)])
+    wts = np.array([w[a] * w[b] for b in range(m) for a in range(m)])
+    return Quadrature(pts, wts)
+
+
+def tet_facet_points(lf: int, pts2d: np.ndarray) -> np.ndarray:
+    """Map facet reference points (s, t) onto facet lf of the reference tetrahedron."""
+    f = TET_FACES[lf]
+    v0, v1, v2 = TET_VERTICES[f[0]], TET_VERTICES[f[1]], TET_VERTICES[f[2]]
+    return v0[None] + pts2d[:, :1] * (v1 - v0)[None] + pts2d[:, 1:2] * (v2 - v0)[None]

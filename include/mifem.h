@@ -1,0 +1,188 @@
+/*
+ * mifem.h -- C ABI of the MI355X finite element assembly engine.
+ *
+ * Plain C: opaque handles, pointers and sizes, int status codes; no C++
+ * exceptions or torch types cross this boundary.  Every entry point replaces
+ * one piece of cuDOLFINx's nanobind/C++ surface; the comment above each one
+ * cites the reference interface it stands in for (paths relative to the
+ * cuDOLFINx source tree).
+ *
+ * Conventions
+ *  - Device pointers (d_*) are HIP device memory; host pointers are copied.
+ *  - `stream` is a hipStream_t passed as void*; NULL means the null stream.
+ *    Nothing here synchronises implicitly except *_copy_* to host and
+ *    mf_synchronize.
+ *  - Dofmaps are *blocked*: [num_cells x nd] node indices; scalar dof of
+ *    component j of node n is bs*n + j (cuDOLFINx CUDADofMap unrolling).
+ *  - Matrices are bs-unrolled CSR with int32 row pointers/columns and
+ *    float64 values (la/petsc.cpp:14-153).
+ *  - Entities: cells (1 int each), exterior facets [cell, local facet],
+ *    interior facets [cell0, lf0, cell1, lf1].
+ *  - Errors: a non-zero MF_E* code; mf_last_error() gives the message
+ *    (thread-local, "what at file:line").
+ */
+#ifndef MIFEM_H
+#define MIFEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mf_context mf_context;
+typedef struct mf_mesh mf_mesh;
+typedef struct mf_dofmap mf_dofmap;
+typedef struct mf_form mf_form;
+typedef struct mf_matrix mf_matrix;
+typedef struct mf_bc mf_bc;
+typedef void* mf_stream;
+
+enum mf_status {
+  MF_OK = 0,
+  MF_EINVAL = 1,       /* bad argument (TypeError/ValueError in the Python shim) */
+  MF_ESHAPE = 2,       /* size or arity mismatch */
+  MF_EHIP = 3,         /* HIP runtime failure */
+  MF_ECOMPILE = 4,     /* hipRTC compile or module load failure */
+  MF_ENOMEM = 5,       /* allocation failure */
+  MF_ESTATE = 6,       /* call out of order (e.g. form not compiled) */
+  MF_ENOTFOUND = 7,    /* element entry missing from the sparsity pattern */
+  MF_EUNSUPPORTED = 8  /* feature not provided */
+};
+
+enum mf_integral_type { MF_CELL = 0, MF_EXTERIOR_FACET = 1, MF_INTERIOR_FACET = 2 };
+
+/* Last error message of the calling thread ("" if none). */
+const char* mf_last_error(void);
+/* ABI version (major*10000 + minor*100 + patch). */
+int mf_version(void);
+
+/* ---------------------------------------------------------------- context */
+/* Replaces CUDA::Context (cpp/cudolfinx/common/CUDA.cpp:28-52) and
+ * get_cuda_context() (python/cudolfinx/context.py:29-35): selects the HIP
+ * device, checks it is gfx950, and loads the precompiled utility kernels.
+ * cache_dir: code-object cache for hipRTC builds (NULL: $XDG_CACHE_HOME/mifem
+ * or ~/.cache/mifem; "" disables the disk cache). */
+int mf_context_create(int device, const char* cache_dir, mf_context** out);
+int mf_context_destroy(mf_context* ctx);
+/* Device architecture name (gcnArchName), NUL-terminated into buf. */
+int mf_context_arch(mf_context* ctx, char* buf, int len);
+
+/* Device memory helpers (the shim's plumbing when no other allocator is used). */
+int mf_alloc(mf_context* ctx, int64_t bytes, void** d_ptr);
+int mf_free(mf_context* ctx, void* d_ptr);
+int mf_memcpy_htod(mf_context* ctx, void* d_dst, const void* src, int64_t bytes, mf_stream stream);
+int mf_memcpy_dtoh(mf_context* ctx, void* dst, const void* d_src, int64_t bytes, mf_stream stream);
+/* Waits for all work on `stream`. */
+int mf_synchronize(mf_context* ctx, mf_stream stream);
+
+/* ---------------------------------------------------------------- mesh */
+/* Replaces CUDAMesh<T>(mesh) (cpp/cudolfinx/mesh/CUDAMesh.h:41-165):
+ * x [num_vertices x 3] float64, cells [num_cells x nv_per_cell] geometry
+ * dofmap, facet_perms [num_cells x nfacets_per_cell] (NULL: all zero). */
+int mf_mesh_create(mf_context* ctx, const double* x, int64_t num_vertices, const int32_t* cells,
+                   int64_t num_cells, int nv_per_cell, const uint8_t* facet_perms,
+                   int nfacets_per_cell, mf_mesh** out);
+int mf_mesh_destroy(mf_mesh* mesh);
+
+/* ---------------------------------------------------------------- dofmap */
+/* Replaces CUDADofMap(dofmap) (cpp/cudolfinx/fem/CUDADofMap.cpp:50-97,199-236). */
+int mf_dofmap_create(mf_context* ctx, const int32_t* dofs, int64_t num_cells, int nd, int bs,
+                     int64_t num_nodes, mf_dofmap** out);
+int mf_dofmap_destroy(mf_dofmap* dm);
+
+/* ---------------------------------------------------------------- form */
+/* Replaces CUDAForm<T,U>(ctx, form, ufcx_form, names, sources, indices)
+ * (python/cudolfinx/wrappers/fem.cpp:76-96, cpp/cudolfinx/fem/CUDAForm.h:40-87).
+ * rank 1: dm1 may be NULL; rank 2: dm0 = test (rows), dm1 = trial (columns). */
+int mf_form_create(mf_context* ctx, mf_mesh* mesh, int rank, mf_dofmap* dm0, mf_dofmap* dm1,
+                   mf_form** out);
+/* One integral: UFCx tabulate_tensor C source (function `tabulate_name`,
+ * the body python/cudolfinx/jit.py:15-114 extracts from FFCx), its entity
+ * list (CUDAFormIntegral.h:481-551), and optionally a staged description
+ * (#defines SK_*, tables and sk_combine) enabling the cooperative
+ * owner-computes matrix kernel.  Integrals run in cell, exterior-facet,
+ * interior-facet order (CUDAAssembler.h:683-740). */
+int mf_form_add_integral(mf_form* form, int type, int id, const char* tabulate_name,
+                         const char* tabulate_source, const int32_t* entities,
+                         int64_t num_entities, const char* staged_source);
+/* Appends a coefficient living on dofmap `dm` with `size` scalar values
+ * (CUDAFormCoefficients.h:61-212); returns its index in *index. */
+int mf_form_add_coefficient(mf_form* form, mf_dofmap* dm, int64_t size, int* index);
+/* Coefficient values: host array copied to the form's device copy
+ * (CUDACoefficient.h:50-53, the `to_device` step), or, with on_device != 0,
+ * a caller-owned device array that is read at pack time (no copy). */
+int mf_form_set_coefficient(mf_form* form, int index, const double* values, int64_t n,
+                            int on_device);
+/* Constants (CUDAFormConstants.h:123-147). */
+int mf_form_set_constants(mf_form* form, const double* values, int n);
+/* Generates the HIP source for every integral, compiles it with hipRTC for
+ * gfx950 (code-object cache keyed by a hash of source and options) and
+ * loads the module (CUDAForm::compile, CUDAForm.h:91-147;
+ * CUDAFormIntegral.cpp:2366-2558).  verbose != 0 prints kernel resource use. */
+int mf_form_compile(mf_form* form, int verbose);
+/* Gathers coefficient values into the per-cell packed array on the device
+ * (pack_coefficients, CUDAAssembler.h:115-283 and CUDAAssembler.cpp:59-96).
+ * indices NULL: all coefficients. */
+int mf_form_pack_coefficients(mf_form* form, const int* indices, int n, mf_stream stream);
+/* Tuning knobs: "matrix_kernel" (0 auto, 1 per-entity atomic scatter,
+ * 2 cooperative owner-computes), "chunk_values" (owner-computes LDS budget
+ * in doubles), "block" (threads per workgroup). */
+int mf_form_set_option(mf_form* form, const char* key, int64_t value);
+/* Generated HIP source of the form (after compile); len includes the NUL. */
+int mf_form_source(mf_form* form, char* buf, int64_t* len);
+int mf_form_destroy(mf_form* form);
+
+/* ---------------------------------------------------------------- matrix */
+/* Replaces fem::petsc::create_cuda_matrix + la::petsc::create_cuda_matrix
+ * (cpp/cudolfinx/fem/petsc.h:31-37, la/petsc.cpp:14-153) and CUDAMatrix
+ * (la/CUDAMatrix.cpp:36-148): builds the sparsity of all integrals of `a`
+ * (sorted unique columns per blocked row, bs-unrolled), keeps row_ptr /
+ * columns / values on the device. */
+int mf_matrix_create(mf_form* a, mf_matrix** out);
+int mf_matrix_info(mf_matrix* A, int64_t* nrows, int64_t* ncols, int64_t* nnz);
+/* Device pointers of the CSR arrays (values may be replaced by a borrowed
+ * buffer, e.g. a PETSc-HIP MatSeqAIJHIPSPARSEGetArray pointer). */
+int mf_matrix_device_arrays(mf_matrix* A, int32_t** d_row_ptr, int32_t** d_cols, double** d_values);
+int mf_matrix_borrow_values(mf_matrix* A, double* d_values);
+/* Copies CSR arrays to host (any pointer may be NULL); the handoff of
+ * CUDAMatrix::to_host (CUDASeqMatrix.cpp:459-533) when no PETSc-HIP exists. */
+int mf_matrix_copy_csr(mf_matrix* A, int32_t* row_ptr, int32_t* cols, double* values);
+int mf_matrix_destroy(mf_matrix* A);
+
+/* ---------------------------------------------------------------- bcs */
+/* Replaces CUDADirichletBC<T,U>(ctx, V, bcs) (CUDADirichletBC.h:56-194):
+ * dofs = constrained scalar dofs (owned first), g_dense = [ndofs] values
+ * (bc->set), markers derived on the device. */
+int mf_bc_create(mf_context* ctx, int64_t ndofs, const int32_t* dofs, int64_t nbc, int64_t n_owned,
+                 const double* g_dense, mf_bc** out);
+/* CUDADirichletBC::update (CUDADirichletBC.h:255-261): new dense values. */
+int mf_bc_update(mf_bc* bc, const double* g_dense);
+int mf_bc_destroy(mf_bc* bc);
+
+/* ---------------------------------------------------------------- assembly */
+/* zero_vector_entries / zero_matrix_entries (CUDAAssembler.cpp:292-454). */
+int mf_zero(mf_context* ctx, double* d_x, int64_t n, mf_stream stream);
+/* assemble_matrix_on_device + set_diagonal (python/cudolfinx/wrappers/fem.cpp:254-278,
+ * CUDAAssembler.h:683-740,878-958): rows in bc0 and columns in bc1 are
+ * skipped, then A[d,d] = diagonal for bc0 rows when the spaces match
+ * (DOLFINx semantics: `diagonal` is honoured).  zero != 0 zeroes first. */
+int mf_assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc1,
+                       double diagonal, int zero, mf_stream stream);
+/* assemble_vector_on_device (fem.cpp:291-309, CUDAAssembler.h:369-424). */
+int mf_assemble_vector(mf_form* L, double* d_b, int64_t n, int zero, mf_stream stream);
+/* apply_lifting_on_device (fem.cpp:327-355, CUDAAssembler.h:564-603):
+ * b -= scale * A_j[:, bc_j] (g_j - x0_j) over every integral of every form. */
+int mf_apply_lifting(mf_form* const* a, int nforms, double* d_b, int64_t n,
+                     const mf_bc* const* bcs1, const double* const* d_x0, double scale,
+                     mf_stream stream);
+/* set_bc_on_device (fem.cpp:357-383, CUDAAssembler.cpp:99-126):
+ * b[d] = scale * (g[d] - x0[d]) for the constrained dofs. */
+int mf_set_bc(mf_context* ctx, double* d_b, int64_t n, const mf_bc* bc, const double* d_x0,
+              double scale, mf_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MIFEM_H */

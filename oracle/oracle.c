@@ -1,0 +1,739 @@
+/*
+ * oracle.c -- FP64 CPU restatement of the cuDOLFINx / DOLFINx assembly path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP
+ * assembly engine: tests, the smoke check and the CPU-baseline leg of the
+ * benchmark call it; the product never links or calls it.
+ *
+ * It restates, in plain serial C:
+ *   - quadrature (Gauss-Jacobi / Gauss-Legendre by Newton on the Jacobi
+ *     recurrence, collapsed tetrahedron and triangle rules);
+ *   - Lagrange bases (P0-P3 tetrahedra by closed-form barycentric
+ *     polynomials, Q1-Q3 hexahedra as GLL tensor products) in the local dof
+ *     order the engine uses (vertices, edges e0=(2,3) e1=(1,3) e2=(1,2)
+ *     e3=(0,3) e4=(0,2) e5=(0,1), faces opposite vertex i; hexahedra
+ *     lexicographic);
+ *   - the element integrals the engine generates (stiffness / Helmholtz,
+ *     linear elasticity, volume and facet sources, SIPG + p-Laplacian DG),
+ *     evaluated directly at quadrature points, in the UFCx argument layout;
+ *   - the sparsity build (DOLFINx create_sparsity_pattern + the bs-unrolling
+ *     of la/petsc.cpp:14-65);
+ *   - the assembly loops of cpp/cudolfinx/fem/CUDAFormIntegral.cpp
+ *     (matrix cells :1103-1226, exterior facets :1667-1793, interior facets
+ *     :1988-2103 with the joint dofmaps :1948-1986 and coefficient
+ *     interlacing :1855-1874; vectors :270-522; lifting :603-947), the
+ *     set_bc / set_diagonal / pack_coefficients kernels of
+ *     cpp/cudolfinx/fem/CUDAAssembler.cpp:59-164, in the integral order of
+ *     CUDAAssembler.h:683-740, with the DOLFINx semantics of lifting over
+ *     every integral (cells, exterior and interior facets).
+ *
+ * Everything is FP64; entries are accumulated in entity order, so results are
+ * deterministic.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OR_MAXQ 4096
+#define OR_MAXND 64
+
+/* ------------------------------------------------------------------ */
+/* quadrature                                                          */
+/* ------------------------------------------------------------------ */
+
+/* P_n^{(a,b)}(x) by the three-term recurrence. */
+static double jacobi_p(int n, double a, double b, double x)
+{
+  if (n == 0) return 1.0;
+  double p0 = 1.0, p1 = 0.5 * ((a + b + 2.0) * x + (a - b));
+  for (int k = 1; k < n; ++k) {
+    double s = 2.0 * k + a + b;
+    double a1 = 2.0 * (k + 1) * (k + a + b + 1.0) * s;
+    double a2 = (s + 1.0) * (a * a - b * b);
+    double a3 = s * (s + 1.0) * (s + 2.0);
+    double a4 = 2.0 * (k + a) * (k + b) * (s + 2.0);
+    double p2 = ((a2 + a3 * x) * p1 - a4 * p0) / a1;
+    p0 = p1;
+    p1 = p2;
+  }
+  return p1;
+}
+
+static double jacobi_dp(int n, double a, double b, double x)
+{
+  if (n == 0) return 0.0;
+  return 0.5 * (n + a + b + 1.0) * jacobi_p(n - 1, a + 1.0, b + 1.0, x);
+}
+
+/* m-point Gauss-Jacobi rule on [0,1] for the weight (1-t)^alpha. */
+void or_gauss_jacobi(int m, double alpha, double* t, double* w)
+{
+  double x[OR_MAXND];
+  const double b = 0.0;
+  for (int k = 0; k < m; ++k) {
+    double r = -cos((2.0 * k + 1.0) * M_PI / (2.0 * m));
+    if (k > 0) r = 0.5 * (r + x[k - 1]);
+    for (int it = 0; it < 200; ++it) {
+      double s = 0.0;
+      for (int j = 0; j < k; ++j) s += 1.0 / (r - x[j]);
+      double p = jacobi_p(m, alpha, b, r), dp = jacobi_dp(m, alpha, b, r);
+      double delta = -p / (dp - s * p);
+      r += delta;
+      if (fabs(delta) < 1e-17) break;
+    }
+    x[k] = r;
+  }
+  double c = pow(2.0, alpha + b + 1.0) * tgamma(m + alpha + 1.0) * tgamma(m + b + 1.0) /
+             (tgamma(m + alpha + b + 1.0) * tgamma(m + 1.0));
+  for (int k = 0; k < m; ++k) {
+    double dp = jacobi_dp(m, alpha, b, x[k]);
+    double wk = c / ((1.0 - x[k] * x[k]) * dp * dp);
+    t[k] = 0.5 * (x[k] + 1.0);
+    w[k] = wk / pow(2.0, alpha + 1.0);
+  }
+}
+
+/* Collapsed (Stroud) m^3 rule on the unit tetrahedron; returns count. */
+int or_tet_rule(int m, double* pts, double* wts)
+{
+  double a[OR_MAXND], wa[OR_MAXND], b[OR_MAXND], wb[OR_MAXND], c[OR_MAXND], wc[OR_MAXND];
+  or_gauss_jacobi(m, 2.0, a, wa);
+  or_gauss_jacobi(m, 1.0, b, wb);
+  or_gauss_jacobi(m, 0.0, c, wc);
+  int n = 0;
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j)
+      for (int l = 0; l < m; ++l) {
+        pts[3 * n] = a[i];
+        pts[3 * n + 1] = b[j] * (1.0 - a[i]);
+        pts[3 * n + 2] = c[l] * (1.0 - a[i]) * (1.0 - b[j]);
+        wts[n] = wa[i] * wb[j] * wc[l];
+        ++n;
+      }
+  return n;
+}
+
+/* Collapsed m^2 rule on the unit triangle. */
+int or_tri_rule(int m, double* pts, double* wts)
+{
+  double a[OR_MAXND], wa[OR_MAXND], b[OR_MAXND], wb[OR_MAXND];
+  or_gauss_jacobi(m, 1.0, a, wa);
+  or_gauss_jacobi(m, 0.0, b, wb);
+  int n = 0;
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) {
+      pts[2 * n] = a[i];
+      pts[2 * n + 1] = b[j] * (1.0 - a[i]);
+      wts[n] = wa[i] * wb[j];
+      ++n;
+    }
+  return n;
+}
+
+/* Gauss-Legendre m^3 rule on the unit cube (x fastest). */
+int or_hex_rule(int m, double* pts, double* wts)
+{
+  double g[OR_MAXND], wg[OR_MAXND];
+  or_gauss_jacobi(m, 0.0, g, wg);
+  int n = 0;
+  for (int k = 0; k < m; ++k)
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i) {
+        pts[3 * n] = g[i];
+        pts[3 * n + 1] = g[j];
+        pts[3 * n + 2] = g[k];
+        wts[n] = wg[i] * wg[j] * wg[k];
+        ++n;
+      }
+  return n;
+}
+
+/* ------------------------------------------------------------------ */
+/* bases                                                               */
+/* ------------------------------------------------------------------ */
+
+static const int TET_E[6][2] = {{2, 3}, {1, 3}, {1, 2}, {0, 3}, {0, 2}, {0, 1}};
+static const int TET_F[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+static const double TET_V[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+static const double DLAM[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+
+int or_ndofs(int cell, int k)
+{
+  if (cell == 0) return (k + 1) * (k + 2) * (k + 3) / 6;
+  return (k + 1) * (k + 1) * (k + 1);
+}
+
+/* P_k on the reference tetrahedron: values phi[nd], gradients dphi[nd][3]. */
+static void tet_basis(int k, const double* X, double* phi, double* dphi)
+{
+  double l[4] = {1.0 - X[0] - X[1] - X[2], X[0], X[1], X[2]};
+  int n = 0;
+#define SETG(idx, c0, v0, c1, v1)                                   \
+  for (int d = 0; d < 3; ++d) dphi[3 * (idx) + d] = (c0) * DLAM[v0][d] + (c1) * DLAM[v1][d]
+  if (k == 0) {
+    phi[0] = 1.0;
+    dphi[0] = dphi[1] = dphi[2] = 0.0;
+    return;
+  }
+  if (k == 1) {
+    for (int v = 0; v < 4; ++v) {
+      phi[v] = l[v];
+      for (int d = 0; d < 3; ++d) dphi[3 * v + d] = DLAM[v][d];
+    }
+    return;
+  }
+  if (k == 2) {
+    for (int v = 0; v < 4; ++v, ++n) {
+      phi[n] = l[v] * (2.0 * l[v] - 1.0);
+      SETG(n, 4.0 * l[v] - 1.0, v, 0.0, v);
+    }
+    for (int e = 0; e < 6; ++e, ++n) {
+      int a = TET_E[e][0], b = TET_E[e][1];
+      phi[n] = 4.0 * l[a] * l[b];
+      SETG(n, 4.0 * l[b], a, 4.0 * l[a], b);
+    }
+    return;
+  }
+  /* k == 3, equispaced */
+  for (int v = 0; v < 4; ++v, ++n) {
+    double L = l[v];
+    phi[n] = 0.5 * L * (3.0 * L - 1.0) * (3.0 * L - 2.0);
+    SETG(n, 0.5 * (27.0 * L * L - 18.0 * L + 2.0), v, 0.0, v);
+  }
+  for (int e = 0; e < 6; ++e) {
+    int a = TET_E[e][0], b = TET_E[e][1];
+    double la = l[a], lb = l[b];
+    /* node nearer a: 9/2 la lb (3 la - 1) */
+    phi[n] = 4.5 * la * lb * (3.0 * la - 1.0);
+    SETG(n, 4.5 * lb * (6.0 * la - 1.0), a, 4.5 * la * (3.0 * la - 1.0), b);
+    ++n;
+    /* node nearer b: 9/2 la lb (3 lb - 1) */
+    phi[n] = 4.5 * la * lb * (3.0 * lb - 1.0);
+    SETG(n, 4.5 * lb * (3.0 * lb - 1.0), a, 4.5 * la * (6.0 * lb - 1.0), b);
+    ++n;
+  }
+  for (int f = 0; f < 4; ++f, ++n) {
+    int a = TET_F[f][0], b = TET_F[f][1], c = TET_F[f][2];
+    phi[n] = 27.0 * l[a] * l[b] * l[c];
+    for (int d = 0; d < 3; ++d)
+      dphi[3 * n + d] = 27.0 * (l[b] * l[c] * DLAM[a][d] + l[a] * l[c] * DLAM[b][d] +
+                                l[a] * l[b] * DLAM[c][d]);
+  }
+#undef SETG
+}
+
+static void gll_nodes(int k, double* t)
+{
+  if (k == 1) { t[0] = 0.0; t[1] = 1.0; }
+  else if (k == 2) { t[0] = 0.0; t[1] = 0.5; t[2] = 1.0; }
+  else { double s = 0.5 / sqrt(5.0); t[0] = 0.0; t[1] = 0.5 - s; t[2] = 0.5 + s; t[3] = 1.0; }
+}
+
+/* 1D Lagrange polynomials on the GLL nodes and their derivatives. */
+static void lagrange_1d(int k, double x, double* L, double* dL)
+{
+  double t[4];
+  gll_nodes(k, t);
+  for (int a = 0; a <= k; ++a) {
+    double v = 1.0, dv = 0.0;
+    for (int b = 0; b <= k; ++b) {
+      if (b == a) continue;
+      double den = t[a] - t[b];
+      dv = dv * (x - t[b]) / den + v / den;
+      v *= (x - t[b]) / den;
+    }
+    L[a] = v;
+    dL[a] = dv;
+  }
+}
+
+/* Q_k on the reference hexahedron, lexicographic nodes. */
+static void hex_basis(int k, const double* X, double* phi, double* dphi)
+{
+  double Lx[4], Ly[4], Lz[4], dx[4], dy[4], dz[4];
+  lagrange_1d(k, X[0], Lx, dx);
+  lagrange_1d(k, X[1], Ly, dy);
+  lagrange_1d(k, X[2], Lz, dz);
+  int n = 0;
+  for (int c = 0; c <= k; ++c)
+    for (int b = 0; b <= k; ++b)
+      for (int a = 0; a <= k; ++a, ++n) {
+        phi[n] = Lx[a] * Ly[b] * Lz[c];
+        dphi[3 * n] = dx[a] * Ly[b] * Lz[c];
+        dphi[3 * n + 1] = Lx[a] * dy[b] * Lz[c];
+        dphi[3 * n + 2] = Lx[a] * Ly[b] * dz[c];
+      }
+}
+
+void or_basis(int cell, int k, const double* X, double* phi, double* dphi)
+{
+  if (cell == 0) tet_basis(k, X, phi, dphi);
+  else hex_basis(k, X, phi, dphi);
+}
+
+/* ------------------------------------------------------------------ */
+/* geometry                                                            */
+/* ------------------------------------------------------------------ */
+
+/* Jacobian J[i][j] = dx_i/dX_j at X; returns det J, fills K = J^{-1}. */
+static double jacobian(int cell, const double* xc, const double* X, double K[3][3])
+{
+  double J[3][3] = {{0}};
+  if (cell == 0) {
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) J[i][j] = xc[3 * (j + 1) + i] - xc[i];
+  } else {
+    double p[8], dp[24];
+    hex_basis(1, X, p, dp);
+    for (int v = 0; v < 8; ++v)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) J[i][j] += xc[3 * v + i] * dp[3 * v + j];
+  }
+  double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+               J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+               J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  /* inverse by cofactors: K[r][c] = C[c][r] / det */
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      int r1 = (c + 1) % 3, r2 = (c + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
+      K[r][c] = (J[r1][c1] * J[r2][c2] - J[r1][c2] * J[r2][c1]) / det;
+    }
+  return det;
+}
+
+/* ------------------------------------------------------------------ */
+/* element kernels                                                     */
+/* ------------------------------------------------------------------ */
+
+enum {
+  OR_STIFFNESS = 0,   /* kappa grad u.grad v - k2 u v            */
+  OR_ELASTICITY = 1,  /* 2 mu eps:eps + lam div div               */
+  OR_SOURCE = 2,      /* f . v                                    */
+  OR_FACET_SOURCE = 3,/* g . v on exterior facets                 */
+  OR_DG_CELL = 4,     /* p-Laplacian cell Jacobian / residual      */
+  OR_DG_EXT = 5,      /* Robin exterior facet                     */
+  OR_DG_INT = 6       /* SIPG interior facet                      */
+};
+
+typedef struct {
+  int kind;
+  int cell;        /* 0 tetrahedron, 1 hexahedron */
+  int degree;
+  int bs;
+  int rank;        /* 1 vector, 2 matrix */
+  int qm;          /* quadrature points per direction */
+  int kappa_src;   /* 0 value, 1 w[off], 2 c[off] */
+  int kappa_off;
+  double kappa_val;
+  int mu_src, mu_off;
+  double mu_val;
+  int lam_src, lam_off;
+  double lam_val;
+  int f_src;       /* 2: constant c[f_off + a]; 1: coefficient w[f_off + l*bs + a] of degree f_degree */
+  int f_off;
+  int f_degree;
+  double k2;
+  /* DG / p-Laplacian parameters */
+  double p, eps, alpha, penalty, fval;
+  int u_off;       /* offset of the DG1 state u in w (per cell) */
+  int coeff_width; /* per-cell packed width (for interior-facet interlacing) */
+} or_kernel;
+
+static double param(int src, int off, double val, const double* w, const double* c)
+{
+  if (src == 1) return w[off];
+  if (src == 2) return c[off];
+  return val;
+}
+
+static int quad_cell(const or_kernel* K, double* pts, double* wts)
+{
+  return K->cell == 0 ? or_tet_rule(K->qm, pts, wts) : or_hex_rule(K->qm, pts, wts);
+}
+
+static void tab_cell(const or_kernel* K, double* A, const double* w, const double* c,
+                     const double* xc)
+{
+  static double pts[3 * OR_MAXQ], wts[OR_MAXQ];
+  const int nd = or_ndofs(K->cell, K->degree), bs = K->bs, n = nd * bs;
+  const int nq = quad_cell(K, pts, wts);
+  double phi[OR_MAXND], dref[3 * OR_MAXND], g[OR_MAXND][3], Kinv[3][3];
+  for (int q = 0; q < nq; ++q) {
+    const double* X = &pts[3 * q];
+    double det = jacobian(K->cell, xc, X, Kinv);
+    double dx = wts[q] * fabs(det);
+    or_basis(K->cell, K->degree, X, phi, dref);
+    for (int i = 0; i < nd; ++i)
+      for (int d = 0; d < 3; ++d)
+        g[i][d] = dref[3 * i] * Kinv[0][d] + dref[3 * i + 1] * Kinv[1][d] +
+                  dref[3 * i + 2] * Kinv[2][d];
+    if (K->kind == OR_STIFFNESS) {
+      double kap = param(K->kappa_src, K->kappa_off, K->kappa_val, w, c);
+      for (int i = 0; i < nd; ++i)
+        for (int j = 0; j < nd; ++j)
+          A[i * nd + j] += dx * (kap * (g[i][0] * g[j][0] + g[i][1] * g[j][1] + g[i][2] * g[j][2]) -
+                                 K->k2 * phi[i] * phi[j]);
+    } else if (K->kind == OR_ELASTICITY) {
+      double mu = param(K->mu_src, K->mu_off, K->mu_val, w, c);
+      double lam = param(K->lam_src, K->lam_off, K->lam_val, w, c);
+      for (int i = 0; i < nd; ++i)
+        for (int a = 0; a < 3; ++a)
+          for (int j = 0; j < nd; ++j)
+            for (int b = 0; b < 3; ++b) {
+              double v = lam * g[i][a] * g[j][b] + mu * g[i][b] * g[j][a];
+              if (a == b) v += mu * (g[i][0] * g[j][0] + g[i][1] * g[j][1] + g[i][2] * g[j][2]);
+              A[(3 * i + a) * n + 3 * j + b] += dx * v;
+            }
+    } else if (K->kind == OR_SOURCE) {
+      double f[3] = {0, 0, 0};
+      if (K->f_src == 2) {
+        for (int a = 0; a < bs; ++a) f[a] = c[K->f_off + a];
+      } else {
+        double psi[OR_MAXND], dpsi[3 * OR_MAXND];
+        or_basis(K->cell, K->f_degree, X, psi, dpsi);
+        int nf = or_ndofs(K->cell, K->f_degree);
+        for (int l = 0; l < nf; ++l)
+          for (int a = 0; a < bs; ++a) f[a] += psi[l] * w[K->f_off + l * bs + a];
+      }
+      for (int i = 0; i < nd; ++i)
+        for (int a = 0; a < bs; ++a) A[i * bs + a] += dx * phi[i] * f[a];
+    }
+  }
+}
+
+static void tab_ext_facet(const or_kernel* K, double* A, const double* w, const double* c,
+                          const double* xc, int lf)
+{
+  double pts[2 * OR_MAXND * OR_MAXND], wts[OR_MAXND * OR_MAXND];
+  const int nd = or_ndofs(K->cell, K->degree), bs = K->bs;
+  const int nq = or_tri_rule(K->qm, pts, wts);
+  const int* f = TET_F[lf];
+  double e1[3], e2[3];
+  for (int d = 0; d < 3; ++d) {
+    e1[d] = xc[3 * f[1] + d] - xc[3 * f[0] + d];
+    e2[d] = xc[3 * f[2] + d] - xc[3 * f[0] + d];
+  }
+  double nrm[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                   e1[0] * e2[1] - e1[1] * e2[0]};
+  double ds = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+  for (int q = 0; q < nq; ++q) {
+    double X[3], phi[OR_MAXND], dref[3 * OR_MAXND];
+    for (int d = 0; d < 3; ++d)
+      X[d] = TET_V[f[0]][d] + pts[2 * q] * (TET_V[f[1]][d] - TET_V[f[0]][d]) +
+             pts[2 * q + 1] * (TET_V[f[2]][d] - TET_V[f[0]][d]);
+    tet_basis(K->degree, X, phi, dref);
+    if (K->kind == OR_FACET_SOURCE)
+      for (int i = 0; i < nd; ++i)
+        for (int a = 0; a < bs; ++a) A[i * bs + a] += wts[q] * ds * phi[i] * c[K->f_off + a];
+  }
+}
+
+/* UFCx-shaped entry point: A += element tensor. */
+void or_tabulate(const or_kernel* K, double* A, const double* w, const double* c,
+                 const double* coordinate_dofs, const int* entity_local_index)
+{
+  switch (K->kind) {
+  case OR_STIFFNESS:
+  case OR_ELASTICITY:
+  case OR_SOURCE: tab_cell(K, A, w, c, coordinate_dofs); break;
+  case OR_FACET_SOURCE: tab_ext_facet(K, A, w, c, coordinate_dofs, entity_local_index[0]); break;
+  default: break;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* coefficient packing (CUDAAssembler.cpp:59-96)                       */
+/* ------------------------------------------------------------------ */
+
+/* packed[c*W + off_j + k] = values_j[dofs_j[c*n_j + k]] for the unrolled dofmaps. */
+void or_pack_coefficients(int ncoef, const int* n_per_cell, const int32_t* const* dofs,
+                          const double* const* values, const int* offsets, int64_t ncells, int W,
+                          double* packed)
+{
+  for (int j = 0; j < ncoef; ++j)
+    for (int64_t cc = 0; cc < ncells; ++cc)
+      for (int k = 0; k < n_per_cell[j]; ++k)
+        packed[cc * W + offsets[j] + k] = values[j][dofs[j][cc * n_per_cell[j] + k]];
+}
+
+/* ------------------------------------------------------------------ */
+/* sparsity                                                            */
+/* ------------------------------------------------------------------ */
+
+static int cmp_i32(const void* a, const void* b)
+{
+  int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+  return (x > y) - (x < y);
+}
+
+typedef struct {
+  int type;                /* 0 cell, 1 exterior facet, 2 interior facet */
+  const int32_t* entities; /* 1, 2 or 4 ints per entity */
+  int64_t n;
+} or_entities;
+
+static int stride_of(int type) { return type == 0 ? 1 : (type == 1 ? 2 : 4); }
+
+/*
+ * Blocked sparsity of the integrals (every (row node, col node) pair of the
+ * entity's cell(s); interior facets couple the union of both cells), sorted
+ * and unique per row, then unrolled to scalar rows bs0*r + a with columns
+ * bs1*c + b (la/petsc.cpp:38-65).  Outputs are malloc'd; free with or_free.
+ */
+int or_sparsity(int nint, const or_entities* ints, const int32_t* dofmap0, int nd0, int bs0,
+                const int32_t* dofmap1, int nd1, int bs1, int64_t nrows_b, int32_t** row_ptr_out,
+                int32_t** cols_out, int64_t* nnz_out, int32_t** rpb_out, int32_t** colsb_out)
+{
+  int64_t* cnt = calloc(nrows_b + 1, sizeof(int64_t));
+  for (int it = 0; it < nint; ++it) {
+    int s = stride_of(ints[it].type);
+    int ncell = ints[it].type == 2 ? 2 : 1;
+    for (int64_t e = 0; e < ints[it].n; ++e)
+      for (int ci = 0; ci < ncell; ++ci) {
+        int32_t cc = ints[it].entities[e * s + 2 * ci];
+        for (int i = 0; i < nd0; ++i) cnt[dofmap0[(int64_t)cc * nd0 + i] + 1] += (int64_t)ncell * nd1;
+      }
+  }
+  for (int64_t r = 0; r < nrows_b; ++r) cnt[r + 1] += cnt[r];
+  int32_t* buf = malloc((cnt[nrows_b] + 1) * sizeof(int32_t));
+  int64_t* pos = malloc(nrows_b * sizeof(int64_t));
+  memcpy(pos, cnt, nrows_b * sizeof(int64_t));
+  for (int it = 0; it < nint; ++it) {
+    int s = stride_of(ints[it].type);
+    int ncell = ints[it].type == 2 ? 2 : 1;
+    for (int64_t e = 0; e < ints[it].n; ++e)
+      for (int ci = 0; ci < ncell; ++ci) {
+        int32_t cr = ints[it].entities[e * s + 2 * ci];
+        for (int i = 0; i < nd0; ++i) {
+          int32_t row = dofmap0[(int64_t)cr * nd0 + i];
+          for (int cj = 0; cj < ncell; ++cj) {
+            int32_t ccol = ints[it].entities[e * s + 2 * cj];
+            for (int j = 0; j < nd1; ++j) buf[pos[row]++] = dofmap1[(int64_t)ccol * nd1 + j];
+          }
+        }
+      }
+  }
+  int32_t* rpb = malloc((nrows_b + 1) * sizeof(int32_t));
+  int64_t out = 0;
+  rpb[0] = 0;
+  for (int64_t r = 0; r < nrows_b; ++r) {
+    int64_t b0 = cnt[r], b1 = cnt[r + 1];
+    qsort(buf + b0, b1 - b0, sizeof(int32_t), cmp_i32);
+    int64_t start = out;
+    for (int64_t k = b0; k < b1; ++k)
+      if (k == b0 || buf[k] != buf[k - 1]) buf[out++] = buf[k];
+    (void)start;
+    rpb[r + 1] = (int32_t)out;
+  }
+  int64_t nnzb = out;
+  int64_t nrows = nrows_b * bs0;
+  int64_t nnz = nnzb * bs0 * bs1;
+  int32_t* rp = malloc((nrows + 1) * sizeof(int32_t));
+  int32_t* cols = malloc((nnz > 0 ? nnz : 1) * sizeof(int32_t));
+  rp[0] = 0;
+  int64_t w = 0;
+  for (int64_t r = 0; r < nrows_b; ++r) {
+    int64_t rn = rpb[r + 1] - rpb[r];
+    for (int a = 0; a < bs0; ++a) {
+      for (int64_t k = 0; k < rn; ++k)
+        for (int b = 0; b < bs1; ++b) cols[w++] = bs1 * buf[rpb[r] + k] + b;
+      rp[r * bs0 + a + 1] = (int32_t)w;
+    }
+  }
+  int32_t* colsb = malloc((nnzb > 0 ? nnzb : 1) * sizeof(int32_t));
+  memcpy(colsb, buf, nnzb * sizeof(int32_t));
+  free(buf);
+  free(pos);
+  free(cnt);
+  *row_ptr_out = rp;
+  *cols_out = cols;
+  *nnz_out = nnz;
+  *rpb_out = rpb;
+  *colsb_out = colsb;
+  return 0;
+}
+
+void or_free(void* p) { free(p); }
+
+/* ------------------------------------------------------------------ */
+/* assembly                                                            */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  const double* x;          /* [Nv x 3] */
+  const int32_t* cells;     /* [Nc x nv] geometry dofmap */
+  int nv;
+  const int32_t* dofmap0;   /* blocked [Nc x nd0] */
+  int nd0, bs0;
+  const int32_t* dofmap1;
+  int nd1, bs1;
+  const double* coeffs;     /* packed [Nc x W] */
+  int W;
+  const int* coeff_offsets; /* ncoef + 1 */
+  int ncoef;
+  const double* constants;
+} or_problem;
+
+/* binary search of key in a sorted row (CUDAFormIntegral.cpp:2572-2602) */
+static int64_t find(const int32_t* row_ptr, const int32_t* cols, int32_t row, int32_t key)
+{
+  int64_t lo = row_ptr[row], hi = row_ptr[row + 1] - 1;
+  while (lo <= hi) {
+    int64_t mid = lo + (hi - lo) / 2;
+    if (cols[mid] == key) return mid;
+    if (cols[mid] < key) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  return -1;
+}
+
+/* Gather the element data of one entity: coordinate dofs, coefficients
+ * (interlaced for interior facets), joint unrolled dofs, local indices. */
+static int gather(const or_problem* P, int type, const int32_t* ent, double* xc, double* w,
+                  int32_t* dofs0, int32_t* dofs1, int* eli)
+{
+  int ncell = type == 2 ? 2 : 1;
+  for (int ci = 0; ci < ncell; ++ci) {
+    int32_t cc = ent[2 * ci];
+    for (int v = 0; v < P->nv; ++v)
+      for (int d = 0; d < 3; ++d)
+        xc[(ci * P->nv + v) * 3 + d] = P->x[3 * (int64_t)P->cells[(int64_t)cc * P->nv + v] + d];
+    for (int i = 0; i < P->nd0; ++i)
+      for (int a = 0; a < P->bs0; ++a)
+        dofs0[(ci * P->nd0 + i) * P->bs0 + a] = P->bs0 * P->dofmap0[(int64_t)cc * P->nd0 + i] + a;
+    if (P->dofmap1)
+      for (int j = 0; j < P->nd1; ++j)
+        for (int b = 0; b < P->bs1; ++b)
+          dofs1[(ci * P->nd1 + j) * P->bs1 + b] = P->bs1 * P->dofmap1[(int64_t)cc * P->nd1 + j] + b;
+  }
+  if (type == 0) {
+    if (P->W) memcpy(w, &P->coeffs[(int64_t)ent[0] * P->W], P->W * sizeof(double));
+  } else if (type == 1) {
+    if (P->W) memcpy(w, &P->coeffs[(int64_t)ent[0] * P->W], P->W * sizeof(double));
+    eli[0] = ent[1];
+  } else {
+    const double* w0 = &P->coeffs[(int64_t)ent[0] * P->W];
+    const double* w1 = &P->coeffs[(int64_t)ent[2] * P->W];
+    for (int j = 0; j < P->ncoef; ++j) {
+      int off = P->coeff_offsets[j], sz = P->coeff_offsets[j + 1] - off;
+      for (int k = 0; k < sz; ++k) {
+        w[2 * off + k] = w0[off + k];
+        w[2 * off + sz + k] = w1[off + k];
+      }
+    }
+    eli[0] = ent[1];
+    eli[1] = ent[3];
+  }
+  return ncell;
+}
+
+#define OR_MAXT (2 * OR_MAXND * 3)
+
+/*
+ * values += A over every integral; rows with bc0 / columns with bc1 are
+ * skipped; afterwards, if insert_diag, values[d,d] = diagonal for bc0 rows.
+ */
+int or_assemble_matrix(const or_problem* P, int nint, const or_kernel* kernels,
+                       const or_entities* ints, const int8_t* bc0, const int8_t* bc1,
+                       const int32_t* row_ptr, const int32_t* cols, double* values,
+                       int insert_diag, double diagonal, int64_t nrows)
+{
+  static double A[OR_MAXT * OR_MAXT];
+  double xc[2 * 27 * 3], w[1024];
+  int32_t dofs0[OR_MAXT], dofs1[OR_MAXT];
+  int eli[2];
+  int missing = 0;
+  for (int it = 0; it < nint; ++it) {
+    const int s = stride_of(ints[it].type);
+    for (int64_t e = 0; e < ints[it].n; ++e) {
+      const int32_t* ent = &ints[it].entities[e * s];
+      int ncell = gather(P, ints[it].type, ent, xc, w, dofs0, dofs1, eli);
+      int nr = ncell * P->nd0 * P->bs0, nc = ncell * P->nd1 * P->bs1;
+      memset(A, 0, sizeof(double) * nr * nc);
+      or_tabulate(&kernels[it], A, w, P->constants, xc, eli);
+      for (int i = 0; i < nr; ++i) {
+        int32_t row = dofs0[i];
+        if (bc0 && bc0[row]) continue;
+        for (int j = 0; j < nc; ++j) {
+          int32_t col = dofs1[j];
+          if (bc1 && bc1[col]) continue;
+          int64_t p = find(row_ptr, cols, row, col);
+          if (p < 0) { ++missing; continue; }
+          values[p] += A[i * nc + j];
+        }
+      }
+    }
+  }
+  if (insert_diag && bc0)
+    for (int64_t r = 0; r < nrows; ++r)
+      if (bc0[r]) {
+        int64_t p = find(row_ptr, cols, (int32_t)r, (int32_t)r);
+        if (p >= 0) values[p] = diagonal;
+      }
+  return missing;
+}
+
+/* b += element vectors over every integral (no masking). */
+int or_assemble_vector(const or_problem* P, int nint, const or_kernel* kernels,
+                       const or_entities* ints, double* b)
+{
+  double be[OR_MAXT], xc[2 * 27 * 3], w[1024];
+  int32_t dofs0[OR_MAXT], dofs1[OR_MAXT];
+  int eli[2];
+  for (int it = 0; it < nint; ++it) {
+    const int s = stride_of(ints[it].type);
+    for (int64_t e = 0; e < ints[it].n; ++e) {
+      const int32_t* ent = &ints[it].entities[e * s];
+      int ncell = gather(P, ints[it].type, ent, xc, w, dofs0, dofs1, eli);
+      int nr = ncell * P->nd0 * P->bs0;
+      memset(be, 0, sizeof(double) * nr);
+      or_tabulate(&kernels[it], be, w, P->constants, xc, eli);
+      for (int i = 0; i < nr; ++i) b[dofs0[i]] += be[i];
+    }
+  }
+  return 0;
+}
+
+/* b -= scale * A[:, bc] (g - x0) over every integral of the bilinear form. */
+int or_apply_lifting(const or_problem* P, int nint, const or_kernel* kernels,
+                     const or_entities* ints, const int8_t* bc1, const double* g,
+                     const double* x0, double scale, double* b)
+{
+  static double A[OR_MAXT * OR_MAXT];
+  double be[OR_MAXT], xc[2 * 27 * 3], w[1024];
+  int32_t dofs0[OR_MAXT], dofs1[OR_MAXT];
+  int eli[2];
+  for (int it = 0; it < nint; ++it) {
+    const int s = stride_of(ints[it].type);
+    for (int64_t e = 0; e < ints[it].n; ++e) {
+      const int32_t* ent = &ints[it].entities[e * s];
+      int ncell = gather(P, ints[it].type, ent, xc, w, dofs0, dofs1, eli);
+      int nr = ncell * P->nd0 * P->bs0, nc = ncell * P->nd1 * P->bs1;
+      int has = 0;
+      for (int j = 0; j < nc; ++j) has |= bc1[dofs1[j]] != 0;
+      if (!has) continue;
+      memset(A, 0, sizeof(double) * nr * nc);
+      memset(be, 0, sizeof(double) * nr);
+      or_tabulate(&kernels[it], A, w, P->constants, xc, eli);
+      for (int j = 0; j < nc; ++j) {
+        int32_t col = dofs1[j];
+        if (!bc1[col]) continue;
+        double v = scale * (g[col] - (x0 ? x0[col] : 0.0));
+        for (int i = 0; i < nr; ++i) be[i] -= A[i * nc + j] * v;
+      }
+      for (int i = 0; i < nr; ++i) b[dofs0[i]] += be[i];
+    }
+  }
+  return 0;
+}
+
+/* b[d] = scale * (g[d] - x0[d]) for the constrained dofs (CUDAAssembler.cpp:99-126). */
+void or_set_bc(int64_t nbc, const int32_t* dofs, const double* g, const double* x0, double scale,
+               double* b)
+{
+  for (int64_t i = 0; i < nbc; ++i) {
+    int32_t d = dofs[i];
+    b[d] = scale * (g[d] - (x0 ? x0[d] : 0.0));
+  }
+}

@@ -1,0 +1,178 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bar: CSR row pointers and columns bit-exact; values within 1e-12 relative
+(Frobenius norm of the difference over the norm of the oracle matrix, l2
+for vectors) on the FP64 path.  Per-entry relative error is not used: P1
+rows hold structurally present entries that are numerically ~1e-17.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import cpu
+from mifem import forms, problems
+from mifem.fem import Constant, Function, dirichletbc, functionspace, locate_dofs_topological
+from mifem.mesh import create_unit_cube
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _rel(x, ref):
+    return np.linalg.norm(x - ref) / max(np.linalg.norm(ref), 1e-300)
+
+
+def _oracle(P, bcs=None, x0=None, scale=1.0, diagonal=1.0):
+    bcs = P.bcs if bcs is None else bcs
+    Oa, OL = cpu.OracleAssembler(P.a), cpu.OracleAssembler(P.L)
+    rp, cols, _, _ = Oa.sparsity()
+    vals = Oa.assemble_matrix(rp, cols, bcs, diagonal=diagonal)
+    b = OL.assemble_vector()
+    Oa.apply_lifting(b, bcs, x0=x0, scale=scale)
+    cpu.set_bc(b, bcs, P.V, x0=x0, scale=scale)
+    return rp, cols, vals, b
+
+
+def _gpu(P, options=None, bcs=None, x0=None, scale=1.0, diagonal=1.0):
+    import cudolfinx as cufem
+    bcs = P.bcs if bcs is None else bcs
+    a = cufem.form(P.a, options=options)
+    L = cufem.form(P.L)
+    asm = cufem.CUDAAssembler()
+    dbc = asm.pack_bcs(bcs)
+    A = asm.assemble_matrix(a, bcs=dbc, diagonal=diagonal)
+    A.assemble()
+    b = asm.assemble_vector(L)
+    xv = None
+    if x0 is not None:
+        xv = asm.create_vector(L)
+        xv.data.upload(x0)
+    asm.apply_lifting(b, [a], [dbc], x0=[xv] if xv is not None else None, scale=scale)
+    asm.set_bc(b, dbc, P.V, x0=xv, scale=scale)
+    asm.synchronize()
+    rp, cols, vals = A.csr()
+    return rp, cols, vals, b.to_host(), (a, L, asm, A, dbc)
+
+
+def _compare(P, gpu, ref):
+    rp, cols, vals, b = gpu[:4]
+    orp, ocols, ovals, ob = ref
+    assert np.array_equal(rp, orp), "row pointers differ"
+    assert np.array_equal(cols, ocols), "column indices differ"
+    ea, eb = _rel(vals, ovals), _rel(b, ob)
+    _say(f"{P.name}: nnz={cols.size} relerr A={ea:.3e} b={eb:.3e}")
+    assert ea <= TOL and eb <= TOL, (ea, eb)
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_poisson_p1(kernel):
+    _say(f"poisson_p1 n=8 matrix_kernel={kernel}")
+    P = problems.poisson_p1(8)
+    _compare(P, _gpu(P, {"matrix_kernel": kernel}), _oracle(P))
+
+
+@pytest.mark.parametrize("kernel", [0, 1, 2])
+def test_elasticity_p2(kernel):
+    _say(f"elasticity_p2 n=5 matrix_kernel={kernel}")
+    P = problems.elasticity_p2(5)
+    _compare(P, _gpu(P, {"matrix_kernel": kernel}), _oracle(P))
+
+
+def test_elasticity_p2_medium():
+    _say("elasticity_p2 n=16 owner-computes")
+    P = problems.elasticity_p2(16)
+    _compare(P, _gpu(P, {"matrix_kernel": 2}), _oracle(P))
+
+
+def test_poisson_p1_full_size_c1():
+    _say("poisson_p1 n=32 (196,608 cells)")
+    P = problems.poisson_p1(32)
+    _compare(P, _gpu(P), _oracle(P))
+
+
+@pytest.mark.parametrize("k", [2, 3])
+def test_pk_poisson_dg0_kappa(k):
+    _say(f"P{k} Poisson with DG0 kappa, jittered mesh")
+    mesh = create_unit_cube(4, jitter=0.2, seed=7)
+    V = functionspace(mesh, ("P", k))
+    D = functionspace(mesh, ("DG", 0))
+    kap = Function(D)
+    kap.set_cellwise(np.exp(np.sin(np.arange(mesh.num_cells))))
+    f = Function(V)
+    f.interpolate(lambda x: 1 + x[0] * x[1])
+    a, L = forms.stiffness(V, kap), forms.source(V, f)
+    bc = dirichletbc(0.0, locate_dofs_topological(V, 2, mesh.exterior_facets), V)
+    P = problems.Problem(f"p{k}_kappa", mesh, V, a, L, [bc])
+    for kernel in (1, 2):
+        _compare(P, _gpu(P, {"matrix_kernel": kernel}), _oracle(P))
+
+
+def test_lifting_with_values_x0_scale_and_diagonal():
+    _say("lifting g != 0, x0, scale, diagonal")
+    P = problems.poisson_p1(6)
+    rng = np.random.default_rng(11)
+    g = Function(P.V)
+    g.x[:] = rng.random(P.V.size)
+    bc = dirichletbc(g, locate_dofs_topological(P.V, 2, P.mesh.exterior_facets), P.V)
+    x0 = rng.random(P.V.size)
+    ref = _oracle(P, [bc], x0=x0, scale=-0.6, diagonal=2.5)
+    for kernel in (1, 2):
+        _compare(P, _gpu(P, {"matrix_kernel": kernel}, [bc], x0=x0, scale=-0.6, diagonal=2.5), ref)
+
+
+def test_reassembly_after_coefficient_change():
+    import cudolfinx as cufem
+    _say("reassembly after coefficient change")
+    P = problems.elasticity_p2(4)
+    gpu = _gpu(P)
+    a, L, asm, A, dbc = gpu[4]
+    P.extras["mu"].x[:] *= 1.7
+    P.extras["mu"].version += 1
+    asm.assemble_matrix(a, A, bcs=dbc)
+    asm.synchronize()
+    rp, cols, vals = A.csr()
+    orp, ocols, ovals, _ = _oracle(P)
+    assert _rel(vals, ovals) <= TOL
+
+
+def test_accumulate_without_zero():
+    _say("zero=False accumulates")
+    P = problems.elasticity_p2(3)
+    for kernel in (1, 2):
+        gpu = _gpu(P, {"matrix_kernel": kernel})
+        a, L, asm, A, dbc = gpu[4]
+        asm.assemble_matrix(a, A, bcs=dbc, zero=False)
+        asm.synchronize()
+        _, _, vals = A.csr()
+        orp, ocols, ovals, _ = _oracle(P)
+        diag = np.zeros(ovals.size, dtype=bool)
+        for r in P.bcs[0].dofs:
+            s, e = orp[r], orp[r + 1]
+            diag[s + np.nonzero(ocols[s:e] == r)[0]] = True
+        exp = 2 * ovals
+        exp[diag] = 1.0
+        assert _rel(vals, exp) <= TOL
+
+
+def test_helmholtz_q3_generic_kernel():
+    _say("helmholtz Q3 n=2 jittered (per-entity kernel)")
+    P = problems.helmholtz_q3(2)
+    _compare(P, _gpu(P), _oracle(P))
+
+
+def test_repeatability():
+    _say("two assemblies agree")
+    P = problems.elasticity_p2(6)
+    g1 = _gpu(P)
+    a, L, asm, A, dbc = g1[4]
+    asm.assemble_matrix(a, A, bcs=dbc)
+    asm.synchronize()
+    _, _, v2 = A.csr()
+    assert _rel(v2, g1[2]) <= 1e-14

@@ -1,0 +1,254 @@
+"""Pins of the CPU oracle against closed forms, library routines and brute force.
+
+Each check is chosen so that a plausible mistake in the oracle (wrong
+quadrature weight, basis index, Jacobian transpose, dropped elasticity
+term, bc masking order) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.special import roots_jacobi
+
+from oracle import cpu
+from mifem import element as E
+from mifem import forms
+from mifem import problems
+from mifem.fem import Constant, Function, dirichletbc, functionspace, locate_dofs_topological
+from mifem.mesh import create_unit_cube
+
+
+# ------------------------------------------------------------ quadrature
+@pytest.mark.parametrize("alpha", [0.0, 1.0, 2.0])
+@pytest.mark.parametrize("m", [1, 2, 3, 5, 8])
+def test_gauss_jacobi_matches_scipy(alpha, m):
+    t, w = cpu.gauss_jacobi(m, alpha)
+    x, wx = roots_jacobi(m, alpha, 0.0)
+    order = np.argsort(t)
+    np.testing.assert_allclose(t[order], (x + 1) / 2, atol=1e-15)
+    np.testing.assert_allclose(w[order], wx / 2 ** (alpha + 1), rtol=1e-13)
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 4])
+def test_tet_rule_exactness(m):
+    pts, wts = cpu.tet_rule(m)
+    for a in range(2 * m):
+        for b in range(2 * m - a):
+            for c in range(2 * m - a - b):
+                exact = math.factorial(a) * math.factorial(b) * math.factorial(c) / math.factorial(a + b + c + 3)
+                got = np.sum(wts * pts[:, 0] ** a * pts[:, 1] ** b * pts[:, 2] ** c)
+                assert abs(got - exact) < 1e-15, (a, b, c)
+    # one degree more is not integrated exactly (rule really has degree 2m-1)
+    d = 2 * m
+    exact = math.factorial(d) / math.factorial(d + 3)
+    assert abs(np.sum(wts * pts[:, 0] ** d) - exact) > 1e-12
+
+
+@pytest.mark.parametrize("m", [1, 2, 4])
+def test_hex_and_triangle_rules(m):
+    pts, wts = cpu.hex_rule(m)
+    for a in range(2 * m):
+        assert abs(np.sum(wts * pts[:, 1] ** a * pts[:, 2]) - 1 / ((a + 1) * 2)) < 1e-15
+    tp, tw = cpu.tri_rule(m)
+    for a in range(2 * m):
+        for b in range(2 * m - a):
+            exact = math.factorial(a) * math.factorial(b) / math.factorial(a + b + 2)
+            assert abs(np.sum(tw * tp[:, 0] ** a * tp[:, 1] ** b) - exact) < 1e-15
+
+
+# ------------------------------------------------------------ bases
+@pytest.mark.parametrize("cell,k", [(0, 1), (0, 2), (0, 3), (1, 1), (1, 2), (1, 3)])
+def test_basis_nodal_partition_and_gradients(cell, k):
+    el = E.Element("P", "tetrahedron" if cell == 0 else "hexahedron", k)
+    X = el.nodes
+    V = np.array([cpu.basis(cell, k, x)[0] for x in X])
+    np.testing.assert_allclose(V, np.eye(len(X)), atol=1e-14)
+    rng = np.random.default_rng(0)
+    P = rng.random((7, 3)) * (0.3 if cell == 0 else 1.0)
+    for p in P:
+        phi, dphi = cpu.basis(cell, k, p)
+        assert abs(phi.sum() - 1) < 1e-14
+        np.testing.assert_allclose(dphi.sum(axis=0), 0, atol=1e-12)
+        # gradient of the interpolant of a degree-k polynomial is exact
+        f = lambda x: x[..., 0] ** k + 2 * x[..., 1] * x[..., 2] ** (k - 1) - x[..., 2]
+        gf = np.array([k * p[0] ** (k - 1), 2 * p[2] ** (k - 1),
+                       2 * (k - 1) * p[1] * p[2] ** max(k - 2, 0) - 1])
+        vals = f(X)
+        np.testing.assert_allclose(vals @ dphi, gf, atol=1e-11)
+        assert abs(vals @ phi - f(p)) < 1e-12
+
+
+# ------------------------------------------------------------ element kernels
+def _tet(rng):
+    x = rng.random((4, 3))
+    while abs(np.linalg.det((x[1:] - x[0]).T)) < 0.05:
+        x = rng.random((4, 3))
+    return x
+
+
+def test_p1_stiffness_and_mass_closed_form():
+    rng = np.random.default_rng(1)
+    x = _tet(rng)
+    vol = abs(np.linalg.det((x[1:] - x[0]).T)) / 6
+    # barycentric gradients: rows of inv([[1, x]])
+    Gm = np.linalg.inv(np.hstack([np.ones((4, 1)), x]))[1:].T    # (4, 3)
+    spec = dict(kind="stiffness", cell=0, degree=1, bs=1, rank=2, qdeg=0, kappa=("val", 2.5), k2=0.0)
+    A = cpu.tabulate(spec, [], [], x)
+    np.testing.assert_allclose(A, 2.5 * vol * Gm @ Gm.T, rtol=1e-12, atol=1e-14)
+    spec = dict(kind="stiffness", cell=0, degree=1, bs=1, rank=2, qdeg=2, kappa=("val", 0.0), k2=-1.0)
+    M = cpu.tabulate(spec, [], [], x)
+    np.testing.assert_allclose(M, vol / 20 * (np.ones((4, 4)) + np.eye(4)), rtol=1e-12)
+
+
+def test_p2_elasticity_element_rigid_modes_and_psd():
+    rng = np.random.default_rng(2)
+    x = _tet(rng)
+    spec = dict(kind="elasticity", cell=0, degree=2, bs=3, rank=2, qdeg=2,
+                mu=("w", 0), lam=("w", 1))
+    A = cpu.tabulate(spec, [2.0, 3.0], [], x)
+    np.testing.assert_allclose(A, A.T, atol=1e-12 * abs(A).max())
+    ev = np.linalg.eigvalsh(A)
+    assert np.sum(np.abs(ev) < 1e-10 * ev.max()) == 6 and ev.min() > -1e-10 * ev.max()
+    # nodes of P2 in physical space
+    lam = np.concatenate([1 - E.Element("P", "tetrahedron", 2).nodes.sum(1, keepdims=True),
+                          E.Element("P", "tetrahedron", 2).nodes], axis=1)
+    X = lam @ x
+    for mode in (lambda p: np.array([1.0, 0, 0]), lambda p: np.array([-p[1], p[0], 0.0]),
+                 lambda p: np.array([0.0, -p[2], p[1]])):
+        u = np.concatenate([mode(p) for p in X])
+        assert np.abs(A @ u).max() < 1e-10 * abs(A).max()
+    # scaling: the stiffness of a tetrahedron scales like h
+    A2 = cpu.tabulate(spec, [2.0, 3.0], [], 2 * x)
+    np.testing.assert_allclose(A2, 2 * A, rtol=1e-12, atol=1e-13 * abs(A).max())
+
+
+def _gll1d_matrices(k, m):
+    t = E.gll_points(k)
+    V = np.vander(t, k + 1, increasing=True)
+    C = np.linalg.inv(V)
+    xg, wg = np.polynomial.legendre.leggauss(m)
+    xg, wg = (xg + 1) / 2, wg / 2
+    P = np.vander(xg, k + 1, increasing=True) @ C
+    dP = np.zeros_like(P)
+    for j in range(1, k + 1):
+        dP += j * np.outer(xg ** (j - 1), C[j])
+    return dP.T @ (wg[:, None] * dP), P.T @ (wg[:, None] * P)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_hex_kronecker_structure(k):
+    x = np.array([(a, b, c) for c in (0, 1) for b in (0, 1) for a in (0, 1)], dtype=float)
+    x = x * np.array([0.5, 0.25, 2.0])          # affine box
+    spec = dict(kind="stiffness", cell=1, degree=k, bs=1, rank=2, qm=k + 1, kappa=("val", 1.0), k2=4.0)
+    A = cpu.tabulate(spec, [], [], x)
+    Kx, Mx = _gll1d_matrices(k, k + 1)
+    hx, hy, hz = 0.5, 0.25, 2.0
+    K = (np.kron(np.kron(Mx * hz, Mx * hy), Kx / hx) + np.kron(np.kron(Mx * hz, Kx / hy), Mx * hx)
+         + np.kron(np.kron(Kx / hz, Mx * hy), Mx * hx))
+    M = np.kron(np.kron(Mx * hz, Mx * hy), Mx * hx)
+    np.testing.assert_allclose(A, K - 4.0 * M, rtol=1e-11, atol=1e-12 * abs(K).max())
+
+
+# ------------------------------------------------------------ global assembly
+def _dense(rp, cols, vals, n):
+    A = np.zeros((n, n))
+    for r in range(n):
+        A[r, cols[rp[r]:rp[r + 1]]] += vals[rp[r]:rp[r + 1]]
+    return A
+
+
+def test_c1_closed_forms_full_size():
+    """32^3 x 6 P1 Poisson: nnz closed form and the 7-point stencil times h."""
+    n = 32
+    P = problems.poisson_p1(n)
+    O = cpu.OracleAssembler(P.a)
+    rp, cols, _, _ = O.sparsity()
+    E_ = 3 * n * (n + 1) ** 2 + 3 * n ** 2 * (n + 1) + n ** 3
+    assert cols.size == (n + 1) ** 3 + 2 * E_ == 513313
+    vals = O.assemble_matrix(rp, cols)
+    X = P.V.node_coordinates
+    h = 1.0 / n
+    interior = np.nonzero(np.all((X > 1e-9) & (X < 1 - 1e-9), axis=1))[0]
+    for r in interior[::997]:
+        c, v = cols[rp[r]:rp[r + 1]], vals[rp[r]:rp[r + 1]] / h
+        assert c.size == 15
+        d = np.abs(X[c] - X[r]).sum(axis=1) * n
+        assert abs(v[c == r][0] - 6) < 1e-12
+        axis_nb = np.isclose(d, 1.0)
+        assert axis_nb.sum() == 6 and np.allclose(v[axis_nb], -1, atol=1e-12)
+        assert np.allclose(v[~axis_nb & (c != r)], 0, atol=1e-12)
+
+
+def test_sparsity_brute_force():
+    P = problems.elasticity_p2(2)
+    O = cpu.OracleAssembler(P.a)
+    rp, cols, rpb, colsb = O.sparsity()
+    dm = P.V.unrolled_dofmap()
+    pairs = set()
+    for c in range(dm.shape[0]):
+        for r in dm[c]:
+            for q in dm[c]:
+                pairs.add((int(r), int(q)))
+    got = {(r, int(q)) for r in range(P.V.size) for q in cols[rp[r]:rp[r + 1]]}
+    assert got == pairs
+    for r in range(P.V.size):
+        assert np.all(np.diff(cols[rp[r]:rp[r + 1]]) > 0)
+
+
+def test_bc_semantics_brute_force():
+    """A_bc = P A P + D, lifting b - s A[:, bc](g - x0) and set_bc, by dense algebra."""
+    P = problems.poisson_p1(3)
+    rng = np.random.default_rng(3)
+    g = Function(P.V)
+    g.x[:] = rng.random(P.V.size)
+    bc = dirichletbc(g, locate_dofs_topological(P.V, 2, P.mesh.exterior_facets), P.V)
+    O = cpu.OracleAssembler(P.a)
+    rp, cols, _, _ = O.sparsity()
+    n = P.V.size
+    A = _dense(rp, cols, O.assemble_matrix(rp, cols), n)
+    Abc = _dense(rp, cols, O.assemble_matrix(rp, cols, [bc], diagonal=3.0), n)
+    keep = np.ones(n)
+    keep[bc.dofs] = 0
+    ref = A * keep[:, None] * keep[None, :]
+    ref[bc.dofs, bc.dofs] = 3.0
+    np.testing.assert_allclose(Abc, ref, atol=1e-14)
+    OL = cpu.OracleAssembler(P.L)
+    b = OL.assemble_vector()
+    x0 = rng.random(n)
+    b2 = O.apply_lifting(b.copy(), [bc], x0=x0, scale=0.7)
+    gd = np.zeros(n)
+    gd[bc.dofs] = g.x[bc.dofs] - x0[bc.dofs]
+    np.testing.assert_allclose(b2, b - 0.7 * A @ gd, atol=1e-13)
+    b3 = cpu.set_bc(b2.copy(), [bc], P.V, x0=x0, scale=0.7)
+    exp = b2.copy()
+    exp[bc.dofs] = 0.7 * (g.x[bc.dofs] - x0[bc.dofs])
+    np.testing.assert_allclose(b3, exp)
+
+
+def test_elasticity_patch_and_load_totals():
+    mesh = create_unit_cube(3)
+    V = functionspace(mesh, ("P", 2), shape=(3,))
+    a = forms.elasticity(V, 2.0, 5.0)
+    O = cpu.OracleAssembler(a)
+    rp, cols, _, _ = O.sparsity()
+    A = _dense(rp, cols, O.assemble_matrix(rp, cols), V.size)
+    X = V.node_coordinates
+    u = (X @ np.array([[0.3, 0.1, -0.2], [0.0, 0.5, 0.4], [-0.1, 0.2, 0.7]])).reshape(-1)
+    r = (A @ u).reshape(-1, 3)
+    interior = np.all((X > 1e-9) & (X < 1 - 1e-9), axis=1)
+    assert np.abs(r[interior]).max() < 1e-11 * np.abs(A).max()
+    P = problems.elasticity_p2(3)
+    b = cpu.OracleAssembler(P.L).assemble_vector().reshape(-1, 3)
+    # body force (0,0,-1) over volume 1 plus traction (0,0,-1) over the unit x=1 face
+    np.testing.assert_allclose(b.sum(axis=0), [0.0, 0.0, -2.0], atol=1e-13)
+
+
+def test_jittered_hex_mass_sums_to_volume():
+    P = problems.helmholtz_q3(3)
+    spec = dict(P.a.integrals[0].spec)
+    spec.update(kappa=("val", 0.0), k2=-1.0)
+    mesh = P.mesh
+    total = 0.0
+    for c in range(mesh.num_cells):
+        total += cpu.tabulate(spec, [], [], mesh.x[mesh.cells[c]]).sum()
+    assert abs(total - 1.0) < 1e-13

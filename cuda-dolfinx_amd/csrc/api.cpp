@@ -282,11 +282,15 @@ int mf_form_set_option(mf_form* f, const char* key, int64_t value) {
   return guard([&] {
     MF_CHECK(f && key, MF_EINVAL, "null argument");
     static const char* known[] = {"matrix_kernel", "chunk_values", "chunk_rows", "block",
-                                  "owner_block", "owner_nb"};
+                                  "owner_block", "owner_nb", "owner_lds", "vector_kernel",
+                                  "vector_block", "vector_chunk_values", "lift_kernel"};
     bool ok = false;
     for (auto k : known) ok |= std::strcmp(k, key) == 0;
     MF_CHECK(ok, MF_EINVAL, std::string("unknown option ") + key);
-    MF_CHECK(!f->compiled || std::strcmp(key, "matrix_kernel") == 0, MF_ESTATE,
+    const bool runtime_key = std::strcmp(key, "matrix_kernel") == 0 ||
+                             std::strcmp(key, "vector_kernel") == 0 ||
+                             std::strcmp(key, "lift_kernel") == 0;
+    MF_CHECK(!f->compiled || runtime_key, MF_ESTATE,
              "kernel shape options must be set before compile");
     f->options[key] = value;
   });
@@ -319,7 +323,6 @@ int mf_form_compile(mf_form* f, int verbose) {
         }
       };
       rep("$OBLOCK", f->opt("owner_block", 256));
-      rep("$ONB", f->opt("owner_nb", 32));
       rep("$OCAP", cap);
       rep("$OMAXR", f->opt("chunk_rows", 256));
     }
@@ -332,10 +335,14 @@ int mf_form_compile(mf_form* f, int verbose) {
       if (f->rank == 2) {
         I->k_mat = f->mod->get(mf::kernel_name("mat", I->type, I->index));
         I->k_lift = f->mod->get(mf::kernel_name("lift", I->type, I->index));
-        if (I->type == MF_CELL && !I->staged_src.empty())
+        if (I->type == MF_CELL && !I->staged_src.empty()) {
           I->k_owner = f->mod->get(mf::kernel_name("owner", I->type, I->index));
+          I->k_slift = f->mod->get(mf::kernel_name("slift", I->type, I->index));
+        }
       } else {
         I->k_vec = f->mod->get(mf::kernel_name("vec", I->type, I->index));
+        if (I->type == MF_CELL && f->dm[0]->nd <= 32)
+          I->k_vown = f->mod->get(mf::kernel_name("vown", I->type, I->index));
       }
     }
     if (verbose)
@@ -516,18 +523,8 @@ int mf_assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* 
 int mf_assemble_vector(mf_form* L, double* b, int64_t n, int zero, mf_stream s) {
   return guard([&] {
     MF_CHECK(L && b, MF_EINVAL, "null argument");
-    MF_CHECK(L->rank == 1, MF_EINVAL, "assemble_vector needs a linear form");
-    MF_CHECK(L->compiled, MF_ESTATE, "form is not compiled");
-    MF_CHECK(n == L->dm[0]->num_nodes * L->dm[0]->bs, MF_ESHAPE, "vector length mismatch");
     use_device(L->ctx);
-    if (zero) mf::k_fill(b, n, 0.0, S(s));
-    const uint32_t block = static_cast<uint32_t>(L->opt("block", 128));
-    for (auto& I : L->ints) {
-      if (I->n == 0) continue;
-      mf::launch(I->k_vec, static_cast<uint32_t>((I->n + block - 1) / block), block, 0, S(s),
-                 static_cast<int32_t>(I->n), I->d_ents.ptr, L->mesh->cells.ptr, L->mesh->x.ptr,
-                 L->packed.ptr, L->consts.ptr, L->dm[0]->dev.ptr, b, L->mesh->fperm.ptr);
-    }
+    mf::assemble_vector(L, b, n, zero, S(s));
   });
 }
 
@@ -565,11 +562,20 @@ int mf_apply_lifting(mf_form* const* a, int nforms, double* b, int64_t n,
         }
         const int64_t nl = it->second.second;
         if (nl == 0) continue;
-        mf::launch(I->k_lift, static_cast<uint32_t>((nl + block - 1) / block), block, 0, S(s),
-                   static_cast<int32_t>(nl), it->second.first.ptr, I->d_ents.ptr,
-                   f->mesh->cells.ptr, f->mesh->x.ptr, f->packed.ptr, f->consts.ptr,
-                   f->dm[0]->dev.ptr, f->dm[1]->dev.ptr, bc->markers.ptr, bc->g.ptr, xk, scale,
-                   b, f->mesh->fperm.ptr);
+        if (I->k_slift && f->opt("lift_kernel", 0) != 1) {
+          const int64_t nt = nl * f->dm[0]->nd;
+          mf::launch(I->k_slift, static_cast<uint32_t>((nt + 127) / 128), 128, 0, S(s),
+                     static_cast<int32_t>(nl), it->second.first.ptr, I->d_ents.ptr,
+                     f->mesh->cells.ptr, f->mesh->x.ptr, f->packed.ptr, f->consts.ptr,
+                     f->dm[0]->dev.ptr, f->dm[1]->dev.ptr, bc->markers.ptr, bc->g.ptr, xk, scale,
+                     b);
+        } else {
+          mf::launch(I->k_lift, static_cast<uint32_t>((nl + block - 1) / block), block, 0, S(s),
+                     static_cast<int32_t>(nl), it->second.first.ptr, I->d_ents.ptr,
+                     f->mesh->cells.ptr, f->mesh->x.ptr, f->packed.ptr, f->consts.ptr,
+                     f->dm[0]->dev.ptr, f->dm[1]->dev.ptr, bc->markers.ptr, bc->g.ptr, xk, scale,
+                     b, f->mesh->fperm.ptr);
+        }
       }
     }
   });

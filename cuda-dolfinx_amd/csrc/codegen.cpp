@@ -187,12 +187,37 @@ std::string subst(std::string t, const std::map<std::string, std::string>& m) {
 
 }  // namespace
 
+int staged_define(const std::string& src, const std::string& name) {
+  const std::string key = "#define " + name + " ";
+  size_t p = src.find(key);
+  MF_CHECK(p != std::string::npos, MF_EINVAL, "staged source lacks " + name);
+  return std::stoi(src.substr(p + key.size()));
+}
+
+int owner_batch(const mf_form& f, const Integral& I) {
+  // Cells per batch so the kernel's static LDS fits the per-workgroup budget.
+  const int nd = staged_define(I.staged_src, "SK_ND");
+  const int nq = staged_define(I.staged_src, "SK_NQ");
+  const int bs = staged_define(I.staged_src, "SK_BS");
+  const int64_t cap = f.opt("chunk_values", 4096), maxr = f.opt("chunk_rows", 256);
+  const int64_t budget = f.opt("owner_lds", 96 * 1024);
+  const int64_t fixed = 8 * cap + 4 * (maxr + 1) + 4 * (cap / (bs * bs) + 1) + 16;
+  const int64_t per_cell = 24LL * nq * nd + 72 + 8 + 8 * std::max(1, f.W) + 8LL * nd + 4 + 2LL * nd;
+  int64_t nb = (budget - fixed) / per_cell;
+  nb = std::min<int64_t>(nb, f.opt("owner_nb", 32));
+  MF_CHECK(nb >= 1, MF_EUNSUPPORTED, "owner-computes kernel does not fit the LDS budget");
+  return static_cast<int>(nb);
+}
+
 std::string kernel_name(const char* what, int type, int index) {
   static const char* tn[] = {"cell", "ext", "int"};
   return std::string("mf_") + what + "_" + tn[type] + "_" + std::to_string(index);
 }
 
-extern const char* kOwnerTemplate;   // owner_kernel.cpp
+extern const char* kOwnerTemplate;         // owner_kernel.cpp
+extern const char* kOwnerCommon;
+extern const char* kOwnerVectorTemplate;
+extern const char* kStagedLiftTemplate;
 
 std::string generate_form_source(const mf_form& f) {
   std::ostringstream os;
@@ -210,6 +235,7 @@ std::string generate_form_source(const mf_form& f) {
   }
   const mf_dofmap* d0 = f.dm[0];
   const mf_dofmap* d1 = f.rank == 2 ? f.dm[1] : f.dm[0];
+  os << kOwnerCommon;
   for (auto& I : f.ints) {
     const int ncell = entity_ncell(I->type);
     std::map<std::string, std::string> m;
@@ -254,11 +280,23 @@ std::string generate_form_source(const mf_form& f) {
         for (const char* sym : {"SK_ND", "SK_BS", "SK_NQ", "SK_W", "SK_DPHI", "sk_combine"})
           replace_all(owner, sym, pfx + sym);
         m["$KNAME"] = kernel_name("owner", I->type, I->index);
+        m["$ONB"] = std::to_string(owner_batch(f, *I));
         os << "\n" << staged << "\n" << subst(owner, m);
+        std::string lift = kStagedLiftTemplate;
+        for (const char* sym : {"SK_ND", "SK_BS", "SK_NQ", "SK_W", "SK_DPHI", "sk_combine"})
+          replace_all(lift, sym, pfx + sym);
+        m["$KNAME"] = kernel_name("slift", I->type, I->index);
+        os << subst(lift, m);
       }
     } else if (f.rank == 1) {
       m["$KNAME"] = kernel_name("vec", I->type, I->index);
       os << subst(subst(kVector, {{"$GATHER", gather}}), m);
+      if (I->type == MF_CELL && d0->nd <= 32) {
+        m["$KNAME"] = kernel_name("vown", I->type, I->index);
+        m["$VBLOCK"] = std::to_string(f.opt("vector_block", 256));
+        m["$VCAP"] = std::to_string(f.opt("vector_chunk_values", 2048));
+        os << subst(kOwnerVectorTemplate, m);
+      }
     }
   }
   return os.str();

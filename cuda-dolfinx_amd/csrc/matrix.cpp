@@ -19,34 +19,15 @@ namespace {
 
 constexpr int kTableLanes = 64;
 
-// Contiguous blocked-row chunks whose unrolled value segment fits `cap`
-// doubles (at most `maxr` rows), and the cells (with a mask of owned local
-// rows) each chunk must visit.
-std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I, const mf_matrix& A,
-                                            int64_t cap, int maxr) {
-  const mf_dofmap* d0 = a.dm[0];
+// Items of a row partition: every entity of a cell integral that has a
+// row dof in a chunk, with the mask of its local rows the chunk owns.
+std::shared_ptr<OwnerPlan> build_items(const std::vector<int32_t>& rows, const Integral& I,
+                                       const mf_dofmap* d0) {
   const int nd = d0->nd;
-  const int bs2 = A.bs0 * A.bs1;
-  const int64_t nrows = A.nrows_b;
-  std::vector<int32_t> rows{0};
-  int64_t seg = 0;
-  int32_t r0 = 0;
-  for (int64_t r = 0; r < nrows; ++r) {
-    int64_t len = (int64_t)bs2 * (A.h_rpb[r + 1] - A.h_rpb[r]);
-    if (len > cap) return nullptr;   // a single row does not fit: caller falls back
-    if (seg + len > cap || r - r0 >= maxr) {
-      rows.push_back(static_cast<int32_t>(r));
-      r0 = static_cast<int32_t>(r);
-      seg = 0;
-    }
-    seg += len;
-  }
-  rows.push_back(static_cast<int32_t>(nrows));
   const int64_t nch = static_cast<int64_t>(rows.size()) - 1;
-  std::vector<int32_t> row_chunk(nrows);
+  std::vector<int32_t> row_chunk(d0->num_nodes);
   for (int64_t c = 0; c < nch; ++c)
     for (int32_t r = rows[c]; r < rows[c + 1]; ++r) row_chunk[r] = static_cast<int32_t>(c);
-  // count (cell, chunk) pairs: bucket each cell by the distinct chunks of its rows
   std::vector<int64_t> cnt(nch + 1, 0);
   std::vector<int32_t> tmp(nd);
   auto cell_chunks = [&](int32_t c, int32_t* out) {
@@ -64,19 +45,20 @@ std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I,
     for (int q = 0; q < n; ++q) cnt[tmp[q] + 1]++;
   }
   for (int64_t c = 0; c < nch; ++c) cnt[c + 1] += cnt[c];
-  std::vector<int32_t> icell(cnt[nch]);
-  std::vector<uint32_t> imask(cnt[nch]);
+  MF_CHECK(cnt[nch] < INT32_MAX, MF_EUNSUPPORTED, "too many owner items");
+  std::vector<int32_t> icell(std::max<int64_t>(1, cnt[nch]));
+  std::vector<uint32_t> imask(std::max<int64_t>(1, cnt[nch]));
   std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
   for (int64_t e = 0; e < I.n; ++e) {
     const int32_t c = I.h_ents[e];
     int n = cell_chunks(c, tmp.data());
     for (int q = 0; q < n; ++q) {
-      uint32_t m = 0;
+      uint32_t msk = 0;
       for (int l = 0; l < nd; ++l)
-        if (row_chunk[d0->host[(int64_t)c * nd + l]] == tmp[q]) m |= 1u << l;
+        if (row_chunk[d0->host[(int64_t)c * nd + l]] == tmp[q]) msk |= 1u << l;
       int64_t p = pos[tmp[q]]++;
       icell[p] = c;
-      imask[p] = m;
+      imask[p] = msk;
     }
   }
   auto plan = std::make_shared<OwnerPlan>();
@@ -90,6 +72,49 @@ std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I,
   plan->item_mask.upload(imask);
   return plan;
 }
+
+// Contiguous blocked-row chunks whose unrolled value segment fits `cap`
+// doubles (at most `maxr` rows).
+std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I, const mf_matrix& A,
+                                            int64_t cap, int maxr) {
+  if (a.dm[0]->nd > 32) return nullptr;
+  const int bs2 = A.bs0 * A.bs1;
+  std::vector<int32_t> rows{0};
+  int64_t seg = 0;
+  int32_t r0 = 0;
+  for (int64_t r = 0; r < A.nrows_b; ++r) {
+    int64_t len = (int64_t)bs2 * (A.h_rpb[r + 1] - A.h_rpb[r]);
+    if (len > cap) return nullptr;   // a single row does not fit: caller falls back
+    if (seg + len > cap || r - r0 >= maxr) {
+      rows.push_back(static_cast<int32_t>(r));
+      r0 = static_cast<int32_t>(r);
+      seg = 0;
+    }
+    seg += len;
+  }
+  rows.push_back(static_cast<int32_t>(A.nrows_b));
+  return build_items(rows, I, a.dm[0]);
+}
+
+}  // namespace
+
+std::shared_ptr<OwnerPlan> vector_plan(mf_form* L, Integral& I) {
+  auto it = L->vplans.find(I.index);
+  if (it != L->vplans.end()) return it->second;
+  const mf_dofmap* d0 = L->dm[0];
+  std::shared_ptr<OwnerPlan> plan;
+  if (d0->nd <= 32) {
+    const int64_t per = std::max<int64_t>(1, L->opt("vector_chunk_values", 2048) / d0->bs);
+    std::vector<int32_t> rows;
+    for (int64_t r = 0; r < d0->num_nodes; r += per) rows.push_back(static_cast<int32_t>(r));
+    rows.push_back(static_cast<int32_t>(d0->num_nodes));
+    plan = build_items(rows, I, d0);
+  }
+  L->vplans[I.index] = plan;
+  return plan;
+}
+
+namespace {
 
 DevArray<int32_t>& get_table(mf_form* a, Integral& I, mf_matrix* A, hipStream_t s) {
   auto key = std::make_pair(a->uid, I.index);
@@ -180,6 +205,44 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
   if (square && m0 && !diag_done)
     k_set_diagonal(A->values, A->row_ptr.ptr, A->cols.ptr, bc0->dofs.ptr, bc0->n_owned, A->nrows,
                    diagonal, s);
+}
+
+}  // namespace mf
+
+namespace mf {
+
+void assemble_vector(mf_form* L, double* b, int64_t n, int zero, hipStream_t s) {
+  MF_CHECK(L->rank == 1, MF_EINVAL, "assemble_vector needs a linear form");
+  MF_CHECK(L->compiled, MF_ESTATE, "form is not compiled");
+  MF_CHECK(n == L->dm[0]->num_nodes * L->dm[0]->bs, MF_ESHAPE, "vector length mismatch");
+  const int64_t mode = L->opt("vector_kernel", 0);
+  Integral* owner = nullptr;
+  std::shared_ptr<OwnerPlan> plan;
+  if (mode != 1)
+    for (auto& I : L->ints)
+      if (I->type == MF_CELL && I->k_vown) {
+        plan = vector_plan(L, *I);
+        if (plan) owner = I.get();
+        break;
+      }
+  MF_CHECK(!(mode == 2 && !owner), MF_EUNSUPPORTED, "owner-computes vector kernel unavailable");
+  if (owner) {
+    const int64_t nx = 8, per = (plan->nchunks + nx - 1) / nx;
+    launch(owner->k_vown, static_cast<uint32_t>(per * nx),
+           static_cast<uint32_t>(L->opt("vector_block", 256)), 0, s,
+           static_cast<int32_t>(plan->nchunks), plan->chunk_rows.ptr, plan->chunk_items.ptr,
+           plan->item_cell.ptr, plan->item_mask.ptr, L->mesh->cells.ptr, L->mesh->x.ptr,
+           L->packed.ptr, L->consts.ptr, L->dm[0]->dev.ptr, b, zero ? 1 : 0);
+  } else if (zero) {
+    k_fill(b, n, 0.0, s);
+  }
+  const uint32_t block = static_cast<uint32_t>(L->opt("block", 128));
+  for (auto& I : L->ints) {
+    if (I.get() == owner || I->n == 0) continue;
+    launch(I->k_vec, static_cast<uint32_t>((I->n + block - 1) / block), block, 0, s,
+           static_cast<int32_t>(I->n), I->d_ents.ptr, L->mesh->cells.ptr, L->mesh->x.ptr,
+           L->packed.ptr, L->consts.ptr, L->dm[0]->dev.ptr, b, L->mesh->fperm.ptr);
+  }
 }
 
 }  // namespace mf

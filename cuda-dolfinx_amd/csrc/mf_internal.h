@@ -172,6 +172,7 @@ struct Integral {
   int64_t n = 0;
   int index = 0;                      // position in the form
   hipFunction_t k_mat = nullptr, k_vec = nullptr, k_lift = nullptr, k_owner = nullptr;
+  hipFunction_t k_vown = nullptr, k_slift = nullptr;
   // lifting entity lists per bc object uid
   std::map<uint64_t, std::pair<DevArray<int32_t>, int64_t>> lift_lists;
 };
@@ -201,6 +202,7 @@ struct mf_form {
   std::unique_ptr<mf::Module> mod;
   std::string source;
   std::map<std::string, int64_t> options;
+  std::map<int, std::shared_ptr<mf::OwnerPlan>> vplans;   // owner-computes vector schedules
   uint64_t uid = 0;
   int64_t opt(const std::string& k, int64_t dflt) const {
     auto it = options.find(k);
@@ -230,6 +232,8 @@ void build_pattern(const mf_form& a, std::vector<int32_t>& rpb, std::vector<int3
 // matrix assembly paths (matrix.cpp)
 void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc1,
                      double diagonal, int zero, hipStream_t s);
+void assemble_vector(mf_form* L, double* b, int64_t n, int zero, hipStream_t s);
+std::shared_ptr<OwnerPlan> vector_plan(mf_form* L, Integral& I);
 int entity_stride(int type);
 int entity_ncell(int type);
 }  // namespace mf

@@ -1,0 +1,86 @@
+"""Autotuner for the cooperative owner-computes matrix kernel.
+
+Picks the kernel shape -- LDS value budget per workgroup ("chunk_values",
+i.e. how many CSR rows a workgroup owns), cells staged per batch
+("owner_nb"), threads per workgroup ("owner_block") -- by timing each
+candidate on the actual form, matrix and boundary conditions.  Every
+candidate is a separate hipRTC build of the same form; the matrix (sparsity)
+is shared.  Results are cached in a JSON table keyed by the form description
+and problem size, so a later run reuses the winner without timing.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import time
+from pathlib import Path
+
+from .form import CUDAForm
+
+DEFAULT_CANDIDATES = [
+    {"chunk_values": 4096, "owner_nb": 32, "owner_block": 256, "owner_lds": 98304},
+    {"chunk_values": 4096, "owner_nb": 16, "owner_block": 256, "owner_lds": 65536},
+    {"chunk_values": 2048, "owner_nb": 16, "owner_block": 256, "owner_lds": 49152},
+    {"chunk_values": 8192, "owner_nb": 32, "owner_block": 512, "owner_lds": 131072},
+    {"chunk_values": 8192, "owner_nb": 16, "owner_block": 256, "owner_lds": 98304},
+    {"chunk_values": 4096, "owner_nb": 32, "owner_block": 512, "owner_lds": 98304},
+]
+
+
+def _key(form, A) -> str:
+    desc = "|".join([form.description, str(A.nrows), str(A.nnz)] +
+                    [I.name for I in form.integrals])
+    return hashlib.sha1(desc.encode()).hexdigest()[:16]
+
+
+def _table_path() -> Path:
+    base = os.environ.get("MIFEM_CACHE_DIR") or str(Path.home() / ".cache" / "mifem")
+    return Path(base) / "autotune.json"
+
+
+def autotune_matrix(form, asm, A, bcs, candidates=None, reps: int = 5, verbose: bool = False):
+    """Return (best options, best CUDAForm, timings[list]) for assembling `form` into A."""
+    table_path = _table_path()
+    key = _key(form, A)
+    table = {}
+    if table_path.exists():
+        try:
+            table = json.loads(table_path.read_text())
+        except (OSError, ValueError):
+            table = {}
+    if candidates is None and key in table:
+        opts = dict(table[key]["options"])
+        return opts, CUDAForm(form, options=opts), table[key].get("timings", [])
+    timings = []
+    best = None
+    for cand in candidates or DEFAULT_CANDIDATES:
+        opts = dict(cand)
+        opts["matrix_kernel"] = 2
+        try:
+            a = CUDAForm(form, options=opts)
+            asm.assemble_matrix(a, A, bcs=bcs)         # plan build + warm-up
+            asm.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                asm.assemble_matrix(a, A, bcs=bcs)
+            asm.synchronize()
+            ms = (time.perf_counter() - t0) / reps * 1e3
+        except (RuntimeError, NotImplementedError, ValueError) as e:
+            if verbose:
+                print(f"[autotune] {cand}: unavailable ({e})", flush=True)
+            continue
+        timings.append({"options": cand, "ms": ms})
+        if verbose:
+            print(f"[autotune] {cand}: {ms:.3f} ms", flush=True)
+        if best is None or ms < best[0]:
+            best = (ms, opts, a)
+    if best is None:
+        raise RuntimeError("no owner-computes candidate could run")
+    table[key] = {"options": best[1], "ms": best[0], "timings": timings}
+    try:
+        table_path.parent.mkdir(parents=True, exist_ok=True)
+        table_path.write_text(json.dumps(table, indent=1))
+    except OSError:
+        pass
+    return best[1], best[2], timings

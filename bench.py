@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=32, help="oracle sample size (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the owner-kernel autotuner")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (PMC traffic calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -193,6 +195,14 @@ def main():
                      "algorithmic_bytes": alg},
         "cpu_baseline": None,
     }
+    if args.copy_calib:
+        nbytes = args.copy_calib * (1 << 20)
+        src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 x {args.copy_calib} MiB device-to-device")
     if not args.no_cpu and args.cpu_sample > 0:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)

@@ -197,14 +197,14 @@ int staged_define(const std::string& src, const std::string& name) {
 int owner_batch(const mf_form& f, const Integral& I) {
   // Cells per batch so the kernel's static LDS fits the per-workgroup budget.
   const int nd = staged_define(I.staged_src, "SK_ND");
-  const int nq = staged_define(I.staged_src, "SK_NQ");
   const int bs = staged_define(I.staged_src, "SK_BS");
   const int64_t cap = f.opt("chunk_values", 4096), maxr = f.opt("chunk_rows", 256);
-  const int64_t budget = f.opt("owner_lds", 96 * 1024);
+  const int64_t budget = f.opt("owner_lds", 64 * 1024);
   const int64_t fixed = 8 * cap + 4 * (maxr + 1) + 4 * (cap / (bs * bs) + 1) + 16;
-  const int64_t per_cell = 24LL * nq * nd + 72 + 8 + 8 * std::max(1, f.W) + 8LL * nd + 4 + 2LL * nd;
+  const int64_t per_cell = 8 * 11 + 8 * std::max(1, f.W) + 8LL * nd + 2LL * nd + 8 + 2LL * nd;
   int64_t nb = (budget - fixed) / per_cell;
-  nb = std::min<int64_t>(nb, f.opt("owner_nb", 32));
+  nb = std::min<int64_t>(nb, f.opt("owner_nb", 64));
+  nb = std::min<int64_t>(nb, 1023);
   MF_CHECK(nb >= 1, MF_EUNSUPPORTED, "owner-computes kernel does not fit the LDS budget");
   return static_cast<int>(nb);
 }

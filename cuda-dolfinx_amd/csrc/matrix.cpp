@@ -180,15 +180,19 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
            "owner-computes kernel requested but unavailable for this form/matrix");
   bool diag_done = false;
   if (owner) {
+    const int bs = a->dm[0]->bs;
+    const uint8_t* n0 = m0 ? bc0->bits_for(bs, s) : nullptr;
+    const uint8_t* n1 = m1 ? bc1->bits_for(a->dm[1]->bs, s) : nullptr;
     int flags = zero ? 1 : 0;
     if (zero && square && m0) flags |= 2;
+    if (square && n0 == n1) flags |= 4;
     diag_done = (flags & 2) != 0;
     const int64_t nx = 8, per = (plan->nchunks + nx - 1) / nx;
     launch(owner->k_owner, static_cast<uint32_t>(per * nx),
            static_cast<uint32_t>(a->opt("owner_block", 256)), 0, s,
            static_cast<int32_t>(plan->nchunks), plan->chunk_rows.ptr, plan->chunk_items.ptr,
            plan->item_cell.ptr, plan->item_mask.ptr, a->mesh->cells.ptr, a->mesh->x.ptr,
-           a->packed.ptr, a->consts.ptr, a->dm[0]->dev.ptr, a->dm[1]->dev.ptr, m0, m1,
+           a->packed.ptr, a->consts.ptr, a->dm[0]->dev.ptr, a->dm[1]->dev.ptr, n0, n1,
            A->row_ptr.ptr, A->rpb.ptr, A->colsb.ptr, A->values, diagonal, flags);
   } else if (zero) {
     k_fill(A->values, A->nnz, 0.0, s);
@@ -246,3 +250,13 @@ void assemble_vector(mf_form* L, double* b, int64_t n, int zero, hipStream_t s) 
 }
 
 }  // namespace mf
+
+const uint8_t* mf_bc::bits_for(int bs, hipStream_t s) const {
+  auto it = node_bits.find(bs);
+  if (it != node_bits.end()) return it->second.ptr;
+  MF_CHECK(ndofs % bs == 0, MF_ESHAPE, "bc size is not a multiple of the block size");
+  mf::DevArray<uint8_t> b;
+  b.alloc(ndofs / bs);
+  mf::k_node_bits(b.ptr, markers.ptr, ndofs / bs, bs, s);
+  return node_bits.emplace(bs, std::move(b)).first->second.ptr;
+}

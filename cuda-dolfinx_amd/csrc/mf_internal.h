@@ -105,6 +105,7 @@ void launch(hipFunction_t f, uint32_t grid, uint32_t block, uint32_t shmem, hipS
 void k_fill(double* x, int64_t n, double v, hipStream_t s);
 void k_fill_i32(int32_t* x, int64_t n, int32_t v, hipStream_t s);
 void k_mark(int8_t* markers, int64_t ndofs, const int32_t* dofs, int64_t nbc, hipStream_t s);
+void k_node_bits(uint8_t* bits, const int8_t* markers, int64_t nnodes, int bs, hipStream_t s);
 void k_pack(double* packed, int64_t ncells, int W, int off, const double* values,
             const int32_t* dofmap, int nd, int bs, hipStream_t s);
 void k_set_bc(double* b, int64_t n, const int32_t* dofs, int64_t nbc, const double* g,
@@ -161,6 +162,9 @@ struct mf_bc {
   mf::DevArray<int8_t> markers;
   mf::DevArray<double> g;
   uint64_t uid = 0;
+  // per-node component bits (bit a: scalar dof bs*node + a constrained), by block size
+  mutable std::map<int, mf::DevArray<uint8_t>> node_bits;
+  const uint8_t* bits_for(int bs, hipStream_t s) const;
 };
 
 namespace mf {

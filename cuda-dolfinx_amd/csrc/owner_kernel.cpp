@@ -61,28 +61,28 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_rows,
        const uint32_t* __restrict__ item_mask, const int32_t* __restrict__ geo,
        const double* __restrict__ xg, const double* __restrict__ coeffs,
        const double* __restrict__ consts, const int32_t* __restrict__ dm0,
-       const int32_t* __restrict__ dm1, const int8_t* __restrict__ bc0,
-       const int8_t* __restrict__ bc1, const int32_t* __restrict__ row_ptr,
+       const int32_t* __restrict__ dm1, const uint8_t* __restrict__ nb0,
+       const uint8_t* __restrict__ nb1, const int32_t* __restrict__ row_ptr,
        const int32_t* __restrict__ rpb, const int32_t* __restrict__ colsb,
        double* __restrict__ values, const double diag, const int flags)
 {
+  // nb0 / nb1: per-node Dirichlet bits (bit a = component a constrained), or null.
+  // flags: 1 overwrite (zero) instead of add, 2 set the diagonal of bc rows,
+  //        4 rows and columns share dofmap and bc (square form).
   constexpr int ND = SK_ND, BS = SK_BS, NQ = SK_NQ;
   constexpr int NB = $ONB, CAP = $OCAP, MAXR = $OMAXR, NV = $NV, W = $W;
   constexpr int WW = W > 0 ? W : 1;
   constexpr int CAPB = CAP / (BS * BS) + 1;
-  constexpr int NG = NQ * 3;                 // gradient values per (cell, node)
-  constexpr int NGP = (NG + 1) & ~1;         // padded to 16 bytes
+  constexpr int KS = 11;                     // per-cell record: K (9), |det|, pad
   __shared__ double acc[CAP];
   __shared__ int32_t lrp[MAXR + 1];
   __shared__ int32_t lcols[CAPB];
-  __shared__ __attribute__((aligned(16))) double sgrad[NB * ND * NGP];
-  __shared__ double sx[NB * NV * 3];
-  __shared__ double sdet[NB];
+  __shared__ double sK[NB * KS];
   __shared__ double sw[NB * WW];
   __shared__ int32_t sd0[NB * ND];
   __shared__ int32_t sd1[NB * ND];
-  __shared__ uint8_t sb0[NB * ND];           // bc0 component bits of each row node
-  __shared__ uint8_t sb1[NB * ND];           // bc1 component bits of each column node
+  __shared__ uint8_t sb0[NB * ND];
+  __shared__ uint8_t sb1[NB * ND];
   __shared__ uint32_t smask[NB];
   __shared__ int32_t scell[NB];
   __shared__ int16_t pairs[NB * ND];
@@ -90,8 +90,19 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_rows,
 
   const int k = mf_xcd_chunk(nchunks);
   if (k < 0) return;
+  const bool square = (flags & 4) != 0;
+  const int32_t* d1 = square ? sd0 : sd1;
+  const uint8_t* s1 = square ? sb0 : sb1;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int32_t r0 = chunk_rows[k], r1 = chunk_rows[k + 1];
+  const int i0 = chunk_items[k], i1 = chunk_items[k + 1];
+  // item loads of the first batch go out together with the chunk set-up loads
+  int32_t pre_cell = 0;
+  uint32_t pre_mask = 0;
+  if (tid < min(NB, i1 - i0)) {
+    pre_cell = item_cell[i0 + tid];
+    pre_mask = item_mask[i0 + tid];
+  }
   const int nr = r1 - r0;
   const int64_t base = row_ptr[(int64_t)BS * r0];
   const int seglen = (int)(row_ptr[(int64_t)BS * r1] - base);
@@ -101,46 +112,62 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_rows,
   for (int p = tid; p <= nr; p += nt) lrp[p] = rpb[r0 + p] - cb0;
   for (int p = tid; p < ncb; p += nt) lcols[p] = colsb[cb0 + p];
 
-  const int i0 = chunk_items[k], i1 = chunk_items[k + 1];
   for (int ib = i0; ib < i1; ib += NB) {
     const int nb = min(NB, i1 - ib);
-    __syncthreads();
-    if (tid == 0) npairs = 0;
-    for (int b = tid; b < nb; b += nt) {
-      scell[b] = item_cell[ib + b];
-      smask[b] = item_mask[ib + b];
-    }
-    __syncthreads();
-    // stage 1: all threads gather vertex coordinates, dofs, bc bits, coefficients
-    for (int u = tid; u < nb * (NV + 2 * ND + W); u += nt) {
-      const int b = u % nb, t = u / nb;
-      const int64_t c = scell[b];
-      if (t < NV) {
-        const int64_t gv = geo[c * NV + t];
-        double* d = &sx[(b * NV + t) * 3];
-        d[0] = xg[3 * gv]; d[1] = xg[3 * gv + 1]; d[2] = xg[3 * gv + 2];
-      } else if (t < NV + ND) {
-        const int l = t - NV;
-        const int32_t n0 = dm0[c * ND + l];
-        uint8_t bits = 0;
-        if (bc0)
-          for (int a = 0; a < BS; ++a) bits |= (bc0[(int64_t)BS * n0 + a] ? 1 : 0) << a;
-        sd0[b * ND + l] = n0;
-        sb0[b * ND + l] = bits;
-      } else if (t < NV + 2 * ND) {
-        const int l = t - NV - ND;
-        const int32_t n1 = dm1[c * ND + l];
-        uint8_t bits = 0;
-        if (bc1)
-          for (int a = 0; a < BS; ++a) bits |= (bc1[(int64_t)BS * n1 + a] ? 1 : 0) << a;
-        sd1[b * ND + l] = n1;
-        sb1[b * ND + l] = bits;
-      } else {
-        const int l = t - NV - 2 * ND;
-        sw[b * WW + l] = coeffs[c * W + l];
+    if (ib > i0) {
+      __syncthreads();                        // previous batch fully consumed
+      if (tid < nb) {
+        pre_cell = item_cell[ib + tid];
+        pre_mask = item_mask[ib + tid];
       }
     }
+    if (tid == 0) npairs = 0;
+    if (tid < nb) {
+      scell[tid] = pre_cell;
+      smask[tid] = pre_mask;
+    }
     __syncthreads();
+    // stage 1: one thread per (cell, task): geometry, row dofs + bits,
+    // column dofs + bits (non-square forms), coefficients
+    const int ntask = square ? 3 : 4;
+    for (int u = tid; u < nb * ntask; u += nt) {
+      const int b = u % nb, t = u / nb;
+      const int64_t c = scell[b];
+      if (t == 0) {
+        double x[NV][3];
+        for (int v = 0; v < NV; ++v) {
+          const int64_t gv = geo[c * NV + v];
+          x[v][0] = xg[3 * gv]; x[v][1] = xg[3 * gv + 1]; x[v][2] = xg[3 * gv + 2];
+        }
+        const double J00 = x[1][0] - x[0][0], J01 = x[2][0] - x[0][0], J02 = x[3][0] - x[0][0];
+        const double J10 = x[1][1] - x[0][1], J11 = x[2][1] - x[0][1], J12 = x[3][1] - x[0][1];
+        const double J20 = x[1][2] - x[0][2], J21 = x[2][2] - x[0][2], J22 = x[3][2] - x[0][2];
+        const double m00 = J11 * J22 - J12 * J21, m01 = J02 * J21 - J01 * J22, m02 = J01 * J12 - J02 * J11;
+        const double det = J00 * m00 + J10 * m01 + J20 * m02;
+        const double id = 1.0 / det;
+        double* K = &sK[b * KS];
+        K[0] = m00 * id; K[1] = m01 * id; K[2] = m02 * id;
+        K[3] = (J12 * J20 - J10 * J22) * id; K[4] = (J00 * J22 - J02 * J20) * id; K[5] = (J02 * J10 - J00 * J12) * id;
+        K[6] = (J10 * J21 - J11 * J20) * id; K[7] = (J01 * J20 - J00 * J21) * id; K[8] = (J00 * J11 - J01 * J10) * id;
+        K[9] = fabs(det);
+      } else if (t == 1) {
+        int32_t n[ND];
+        for (int l = 0; l < ND; ++l) n[l] = dm0[c * ND + l];
+        for (int l = 0; l < ND; ++l) {
+          sd0[b * ND + l] = n[l];
+          sb0[b * ND + l] = nb0 ? nb0[n[l]] : (uint8_t)0;
+        }
+      } else if (t == 2) {
+        for (int l = 0; l < W; ++l) sw[b * WW + l] = coeffs[c * W + l];
+      } else {
+        int32_t n[ND];
+        for (int l = 0; l < ND; ++l) n[l] = dm1[c * ND + l];
+        for (int l = 0; l < ND; ++l) {
+          sd1[b * ND + l] = n[l];
+          sb1[b * ND + l] = nb1 ? nb1[n[l]] : (uint8_t)0;
+        }
+      }
+    }
     // owned (cell, row) pairs
     for (int b = tid; b < nb; b += nt) {
       uint32_t m = smask[b];
@@ -151,80 +178,77 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_rows,
         pairs[off++] = (int16_t)((b << 5) | i);
       }
     }
-    // stage 2: physical gradients g_i(q) = K^T dphi_i(q), K = J^{-1} per cell
-    for (int u = tid; u < nb * ND; u += nt) {
-      const int b = u / ND, i = u - b * ND;
-      const double* x = &sx[b * NV * 3];
-      const double J00 = x[3] - x[0], J01 = x[6] - x[0], J02 = x[9] - x[0];
-      const double J10 = x[4] - x[1], J11 = x[7] - x[1], J12 = x[10] - x[1];
-      const double J20 = x[5] - x[2], J21 = x[8] - x[2], J22 = x[11] - x[2];
-      const double m00 = J11 * J22 - J12 * J21, m01 = J02 * J21 - J01 * J22, m02 = J01 * J12 - J02 * J11;
-      const double det = J00 * m00 + J10 * m01 + J20 * m02;
-      const double id = 1.0 / det;
-      const double K00 = m00 * id, K01 = m01 * id, K02 = m02 * id;
-      const double K10 = (J12 * J20 - J10 * J22) * id, K11 = (J00 * J22 - J02 * J20) * id, K12 = (J02 * J10 - J00 * J12) * id;
-      const double K20 = (J10 * J21 - J11 * J20) * id, K21 = (J01 * J20 - J00 * J21) * id, K22 = (J00 * J11 - J01 * J10) * id;
-      if (i == 0) sdet[b] = fabs(det);
-      double* g = &sgrad[u * NGP];
-      for (int q = 0; q < NQ; ++q) {
-        const double d0 = SK_DPHI[q][i][0], d1 = SK_DPHI[q][i][1], d2 = SK_DPHI[q][i][2];
-        g[3 * q] = K00 * d0 + K10 * d1 + K20 * d2;
-        g[3 * q + 1] = K01 * d0 + K11 * d1 + K21 * d2;
-        g[3 * q + 2] = K02 * d0 + K12 * d1 + K22 * d2;
-      }
-    }
     __syncthreads();
-    // stage 3: one thread per (cell, owned row i, column j) block
+    // stage 2: one thread per (cell, owned row i, column range); all lanes of a
+    // wave share the column range, so the column basis tables are wave-uniform.
     const int np = npairs;
-    for (int u = tid; u < np * ND; u += nt) {
-      const int pr = u / ND, j = u - pr * ND;
+    const int waves = nt >> 6;
+    int js = (waves * 64) / max(np, 1);
+    js = js < 1 ? 1 : (js > ND ? ND : js);
+    const int jw = (ND + js - 1) / js;
+    const int units = ((np + 63) >> 6) * 64 * js;
+    for (int u = tid; u < units; u += nt) {
+      const int lane = u & 63, wv = u >> 6;
+      const int part = __builtin_amdgcn_readfirstlane(wv % js);
+      const int pr = (wv / js) * 64 + lane;
+      if (pr >= np) continue;
       const int b = pairs[pr] >> 5, i = pairs[pr] & 31;
-      const double2* gi = reinterpret_cast<const double2*>(&sgrad[(b * ND + i) * NGP]);
-      const double2* gj = reinterpret_cast<const double2*>(&sgrad[(b * ND + j) * NGP]);
-      double vi[NGP], vj[NGP];
-      for (int h = 0; h < NGP / 2; ++h) {
-        const double2 p = gi[h], q2 = gj[h];
-        vi[2 * h] = p.x; vi[2 * h + 1] = p.y;
-        vj[2 * h] = q2.x; vj[2 * h + 1] = q2.y;
-      }
-      double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      const double* K = &sK[b * KS];
+      const double K0 = K[0], K1 = K[1], K2 = K[2], K3 = K[3], K4 = K[4], K5 = K[5];
+      const double K6 = K[6], K7 = K[7], K8 = K[8], dt = K[9];
+      double gi[NQ][3];
       for (int q = 0; q < NQ; ++q) {
-        const double wq = SK_W[q];
-        const double a0 = wq * vi[3 * q], a1 = wq * vi[3 * q + 1], a2 = wq * vi[3 * q + 2];
-        const double c0 = vj[3 * q], c1 = vj[3 * q + 1], c2 = vj[3 * q + 2];
-        G[0] += a0 * c0; G[1] += a0 * c1; G[2] += a0 * c2;
-        G[3] += a1 * c0; G[4] += a1 * c1; G[5] += a1 * c2;
-        G[6] += a2 * c0; G[7] += a2 * c1; G[8] += a2 * c2;
+        const double s = SK_W[q] * dt;
+        const double d0 = SK_DPHI[q][i][0], d1v = SK_DPHI[q][i][1], d2 = SK_DPHI[q][i][2];
+        gi[q][0] = s * (K0 * d0 + K3 * d1v + K6 * d2);
+        gi[q][1] = s * (K1 * d0 + K4 * d1v + K7 * d2);
+        gi[q][2] = s * (K2 * d0 + K5 * d1v + K8 * d2);
       }
-      const double dt = sdet[b];
-      for (int h = 0; h < 9; ++h) G[h] *= dt;
-      double blk[BS * BS];
-      sk_combine(G, &sw[b * WW], consts, blk);
-      const int32_t R = sd0[b * ND + i], Cn = sd1[b * ND + j];
-      const uint8_t rb0 = sb0[b * ND + i], cb1 = sb1[b * ND + j];
+      const int32_t R = sd0[b * ND + i];
+      const uint8_t rb0 = sb0[b * ND + i];
       const int lr = R - r0;
       const int lo = lrp[lr], hi = lrp[lr + 1];
-      const int kk = mf_lds_lower_bound(lcols, lo, hi, Cn) - lo;
       const int rowlen = BS * (hi - lo);
-      const int rb = BS * BS * lo + BS * kk;
-      for (int a = 0; a < BS; ++a) {
-        if (rb0 & (1 << a)) continue;
-        for (int bb = 0; bb < BS; ++bb) {
-          if (cb1 & (1 << bb)) continue;
-          atomicAdd(&acc[rb + a * rowlen + bb], blk[a * BS + bb]);
+      const double* wb = &sw[b * WW];
+      const int j0 = part * jw, j1 = min(ND, j0 + jw);
+      for (int j = j0; j < j1; ++j) {
+        double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int q = 0; q < NQ; ++q) {
+          const double d0 = SK_DPHI[q][j][0], d1v = SK_DPHI[q][j][1], d2 = SK_DPHI[q][j][2];
+          const double c0 = K0 * d0 + K3 * d1v + K6 * d2;
+          const double c1 = K1 * d0 + K4 * d1v + K7 * d2;
+          const double c2 = K2 * d0 + K5 * d1v + K8 * d2;
+          G[0] += gi[q][0] * c0; G[1] += gi[q][0] * c1; G[2] += gi[q][0] * c2;
+          G[3] += gi[q][1] * c0; G[4] += gi[q][1] * c1; G[5] += gi[q][1] * c2;
+          G[6] += gi[q][2] * c0; G[7] += gi[q][2] * c1; G[8] += gi[q][2] * c2;
+        }
+        double blk[BS * BS];
+        sk_combine(G, wb, consts, blk);
+        const int32_t Cn = d1[b * ND + j];
+        const uint8_t cb1 = s1[b * ND + j];
+        const int kk = mf_lds_lower_bound(lcols, lo, hi, Cn) - lo;
+        const int rb = BS * BS * lo + BS * kk;
+        for (int a = 0; a < BS; ++a) {
+          if (rb0 & (1 << a)) continue;
+          for (int bb = 0; bb < BS; ++bb) {
+            if (cb1 & (1 << bb)) continue;
+            atomicAdd(&acc[rb + a * rowlen + bb], blk[a * BS + bb]);
+          }
         }
       }
     }
   }
   __syncthreads();
-  if ((flags & 2) && bc0) {
+  if ((flags & 2) && nb0) {
     for (int lr = tid; lr < nr; lr += nt) {
       const int32_t R = r0 + lr;
+      const uint8_t bits = nb0[R];
+      if (!bits) continue;
       const int lo = lrp[lr], hi = lrp[lr + 1];
       const int p = mf_lds_lower_bound(lcols, lo, hi, R);
       if (p < hi && lcols[p] == R)
         for (int a = 0; a < BS; ++a)
-          if (bc0[(int64_t)BS * R + a]) acc[BS * BS * lo + a * BS * (hi - lo) + BS * (p - lo) + a] = diag;
+          if (bits & (1 << a)) acc[BS * BS * lo + a * BS * (hi - lo) + BS * (p - lo) + a] = diag;
     }
     __syncthreads();
   }

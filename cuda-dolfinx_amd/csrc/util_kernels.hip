@@ -35,6 +35,16 @@ __global__ void mark_kernel(int8_t* __restrict__ m, int64_t ndofs, const int32_t
   }
 }
 
+__global__ void node_bits_kernel(uint8_t* __restrict__ bits, const int8_t* __restrict__ m,
+                                 int64_t nnodes, int bs) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnodes; i += stride) {
+    uint8_t v = 0;
+    for (int a = 0; a < bs; ++a) v |= (m[i * bs + a] ? 1 : 0) << a;
+    bits[i] = v;
+  }
+}
+
 // packed[c*W + off + k*bs + j] = values[bs*dofmap[c*nd + k] + j]
 __global__ void pack_kernel(double* __restrict__ packed, int64_t ncells, int W, int off,
                             const double* __restrict__ values, const int32_t* __restrict__ dofmap,
@@ -167,6 +177,11 @@ void k_fill_i32(int32_t* x, int64_t n, int32_t v, hipStream_t s) {
 void k_mark(int8_t* m, int64_t ndofs, const int32_t* dofs, int64_t nbc, hipStream_t s) {
   if (nbc <= 0) return;
   mark_kernel<<<grid_for(nbc), kBlock, 0, s>>>(m, ndofs, dofs, nbc);
+  MF_HIP(hipGetLastError());
+}
+void k_node_bits(uint8_t* bits, const int8_t* m, int64_t nnodes, int bs, hipStream_t s) {
+  if (nnodes <= 0) return;
+  node_bits_kernel<<<grid_for(nnodes), kBlock, 0, s>>>(bits, m, nnodes, bs);
   MF_HIP(hipGetLastError());
 }
 void k_pack(double* packed, int64_t ncells, int W, int off, const double* values,

@@ -13,18 +13,19 @@ from __future__ import annotations
 import hashlib
 import json
 import os
+import sys
 import time
 from pathlib import Path
 
 from .form import CUDAForm
 
 DEFAULT_CANDIDATES = [
-    {"chunk_values": 4096, "owner_nb": 32, "owner_block": 256, "owner_lds": 98304},
-    {"chunk_values": 4096, "owner_nb": 16, "owner_block": 256, "owner_lds": 65536},
-    {"chunk_values": 2048, "owner_nb": 16, "owner_block": 256, "owner_lds": 49152},
-    {"chunk_values": 8192, "owner_nb": 32, "owner_block": 512, "owner_lds": 131072},
-    {"chunk_values": 8192, "owner_nb": 16, "owner_block": 256, "owner_lds": 98304},
-    {"chunk_values": 4096, "owner_nb": 32, "owner_block": 512, "owner_lds": 98304},
+    {"chunk_values": 4096, "owner_nb": 64, "owner_block": 256, "owner_lds": 65536},
+    {"chunk_values": 4096, "owner_nb": 64, "owner_block": 512, "owner_lds": 65536},
+    {"chunk_values": 4096, "owner_nb": 32, "owner_block": 512, "owner_lds": 53248},
+    {"chunk_values": 2048, "owner_nb": 48, "owner_block": 256, "owner_lds": 40960},
+    {"chunk_values": 6144, "owner_nb": 64, "owner_block": 512, "owner_lds": 81920},
+    {"chunk_values": 8192, "owner_nb": 128, "owner_block": 512, "owner_lds": 106496},
 ]
 
 
@@ -68,11 +69,11 @@ def autotune_matrix(form, asm, A, bcs, candidates=None, reps: int = 5, verbose: 
             ms = (time.perf_counter() - t0) / reps * 1e3
         except (RuntimeError, NotImplementedError, ValueError) as e:
             if verbose:
-                print(f"[autotune] {cand}: unavailable ({e})", flush=True)
+                print(f"[autotune] {cand}: unavailable ({e})", file=sys.stderr, flush=True)
             continue
         timings.append({"options": cand, "ms": ms})
         if verbose:
-            print(f"[autotune] {cand}: {ms:.3f} ms", flush=True)
+            print(f"[autotune] {cand}: {ms:.3f} ms", file=sys.stderr, flush=True)
         if best is None or ms < best[0]:
             best = (ms, opts, a)
     if best is None:

@@ -40,11 +40,11 @@ def _oracle(P, bcs=None, x0=None, scale=1.0, diagonal=1.0):
     return rp, cols, vals, b
 
 
-def _gpu(P, options=None, bcs=None, x0=None, scale=1.0, diagonal=1.0):
+def _gpu(P, options=None, bcs=None, x0=None, scale=1.0, diagonal=1.0, loptions=None):
     import cudolfinx as cufem
     bcs = P.bcs if bcs is None else bcs
     a = cufem.form(P.a, options=options)
-    L = cufem.form(P.L)
+    L = cufem.form(P.L, options=loptions)
     asm = cufem.CUDAAssembler()
     dbc = asm.pack_bcs(bcs)
     A = asm.assemble_matrix(a, bcs=dbc, diagonal=diagonal)
@@ -176,3 +176,18 @@ def test_repeatability():
     asm.synchronize()
     _, _, v2 = A.csr()
     assert _rel(v2, g1[2]) <= 1e-14
+
+
+@pytest.mark.parametrize("vk,lk", [(1, 1), (2, 0), (2, 1)])
+def test_vector_and_lifting_kernels(vk, lk):
+    _say(f"vector_kernel={vk} lift_kernel={lk} elasticity + poisson")
+    P = problems.elasticity_p2(5)
+    rng = np.random.default_rng(5)
+    g = Function(P.V)
+    g.x[:] = rng.random(P.V.size)
+    left = P.mesh.locate_boundary_facets(lambda x: np.isclose(x[0], 0.0))
+    bc = dirichletbc(g, locate_dofs_topological(P.V, 2, left), P.V)
+    ref = _oracle(P, [bc], scale=0.8)
+    _compare(P, _gpu(P, {"lift_kernel": lk}, [bc], scale=0.8, loptions={"vector_kernel": vk}), ref)
+    Q = problems.poisson_p1(7)
+    _compare(Q, _gpu(Q, {"lift_kernel": lk}, loptions={"vector_kernel": vk}), _oracle(Q))

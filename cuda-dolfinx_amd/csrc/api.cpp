@@ -336,8 +336,13 @@ int mf_form_compile(mf_form* f, int verbose) {
         I->k_mat = f->mod->get(mf::kernel_name("mat", I->type, I->index));
         I->k_lift = f->mod->get(mf::kernel_name("lift", I->type, I->index));
         if (I->type == MF_CELL && !I->staged_src.empty()) {
-          I->k_owner = f->mod->get(mf::kernel_name("owner", I->type, I->index));
           I->k_slift = f->mod->get(mf::kernel_name("slift", I->type, I->index));
+          if (mf::owner_supported(*f, *I)) {
+            I->k_owner = f->mod->get(mf::kernel_name("owner", I->type, I->index));
+            I->k_prep = f->mod->get(mf::kernel_name("prep", I->type, I->index));
+            I->record_bytes = mf::record_bytes(*f, *I);
+            I->records.alloc(f->mesh->num_cells * I->record_bytes / 8);
+          }
         }
       } else {
         I->k_vec = f->mod->get(mf::kernel_name("vec", I->type, I->index));

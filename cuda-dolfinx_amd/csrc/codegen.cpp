@@ -194,6 +194,19 @@ int staged_define(const std::string& src, const std::string& name) {
   return std::stoi(src.substr(p + key.size()));
 }
 
+int64_t record_bytes(const mf_form& f, const Integral& I) {
+  const int nd = staged_define(I.staged_src, "SK_ND");
+  const int64_t raw = 72 + 8 + 8LL * std::max(1, f.W) + 4LL * nd + 4;
+  return (raw + 15) / 16 * 16;
+}
+
+bool owner_supported(const mf_form& f, const Integral& I) {
+  if (f.rank != 2 || f.dm[0] != f.dm[1] || I.staged_src.empty() || f.mesh->nv != 4) return false;
+  const int nd = staged_define(I.staged_src, "SK_ND");
+  const int bs = staged_define(I.staged_src, "SK_BS");
+  return nd <= 32 && nd * bs <= 32;
+}
+
 int owner_batch(const mf_form& f, const Integral& I) {
   // Cells per batch so the kernel's static LDS fits the per-workgroup budget.
   const int nd = staged_define(I.staged_src, "SK_ND");
@@ -201,7 +214,7 @@ int owner_batch(const mf_form& f, const Integral& I) {
   const int64_t cap = f.opt("chunk_values", 4096), maxr = f.opt("chunk_rows", 256);
   const int64_t budget = f.opt("owner_lds", 64 * 1024);
   const int64_t fixed = 8 * cap + 4 * (maxr + 1) + 4 * (cap / (bs * bs) + 1) + 16;
-  const int64_t per_cell = 8 * 11 + 8 * std::max(1, f.W) + 8LL * nd + 2LL * nd + 8 + 2LL * nd;
+  const int64_t per_cell = record_bytes(f, I) + 4 + 4 + 2LL * nd;
   int64_t nb = (budget - fixed) / per_cell;
   nb = std::min<int64_t>(nb, f.opt("owner_nb", 64));
   nb = std::min<int64_t>(nb, 1023);
@@ -218,6 +231,8 @@ extern const char* kOwnerTemplate;         // owner_kernel.cpp
 extern const char* kOwnerCommon;
 extern const char* kOwnerVectorTemplate;
 extern const char* kStagedLiftTemplate;
+extern const char* kCellRecord;
+extern const char* kPrepTemplate;
 
 std::string generate_form_source(const mf_form& f) {
   std::ostringstream os;
@@ -274,19 +289,24 @@ std::string generate_form_source(const mf_form& f) {
         std::string staged = I->staged_src;
         // namespace the staged symbols per integral
         std::string pfx = "I" + std::to_string(I->index) + "_";
-        for (const char* sym : {"SK_ND", "SK_BS", "SK_NQ", "SK_W", "SK_DPHI", "sk_combine"})
-          replace_all(staged, sym, pfx + sym);
-        std::string owner = kOwnerTemplate;
-        for (const char* sym : {"SK_ND", "SK_BS", "SK_NQ", "SK_W", "SK_DPHI", "sk_combine"})
-          replace_all(owner, sym, pfx + sym);
-        m["$KNAME"] = kernel_name("owner", I->type, I->index);
-        m["$ONB"] = std::to_string(owner_batch(f, *I));
-        os << "\n" << staged << "\n" << subst(owner, m);
-        std::string lift = kStagedLiftTemplate;
-        for (const char* sym : {"SK_ND", "SK_BS", "SK_NQ", "SK_W", "SK_DPHI", "sk_combine"})
-          replace_all(lift, sym, pfx + sym);
+        auto rename = [&](std::string t) {
+          for (const char* sym : {"SK_ND", "SK_BS", "SK_NQ", "SK_W", "SK_DPHI", "sk_combine"})
+            replace_all(t, sym, pfx + sym);
+          return t;
+        };
+        os << "\n" << rename(staged) << "\n";
+        if (owner_supported(f, *I)) {
+          m["$TAG"] = "I" + std::to_string(I->index);
+          m["$WW"] = std::to_string(std::max(1, f.W));
+          os << subst(rename(kCellRecord), m);
+          m["$KNAME"] = kernel_name("prep", I->type, I->index);
+          os << subst(rename(kPrepTemplate), m);
+          m["$KNAME"] = kernel_name("owner", I->type, I->index);
+          m["$ONB"] = std::to_string(owner_batch(f, *I));
+          os << subst(rename(kOwnerTemplate), m);
+        }
         m["$KNAME"] = kernel_name("slift", I->type, I->index);
-        os << subst(lift, m);
+        os << subst(rename(kStagedLiftTemplate), m);
       }
     } else if (f.rank == 1) {
       m["$KNAME"] = kernel_name("vec", I->type, I->index);

@@ -10,7 +10,7 @@ namespace mf {
 
 struct OwnerPlan {
   int64_t nchunks = 0;
-  DevArray<int32_t> chunk_rows, chunk_items, item_cell;
+  DevArray<int32_t> chunk_rows, chunk_items, item_cell, chunk_meta;
   DevArray<uint32_t> item_mask;
   int64_t nitems = 0;
 };
@@ -93,7 +93,22 @@ std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I,
     seg += len;
   }
   rows.push_back(static_cast<int32_t>(A.nrows_b));
-  return build_items(rows, I, a.dm[0]);
+  auto plan = build_items(rows, I, a.dm[0]);
+  // per-chunk metadata: r0, r1, i0, i1, cb0, ncb, CSR base, segment length
+  std::vector<int32_t> ci = plan->chunk_items.download();
+  std::vector<int32_t> meta(8 * plan->nchunks);
+  for (int64_t c = 0; c < plan->nchunks; ++c) {
+    const int32_t q0 = rows[c], q1 = rows[c + 1];
+    const int64_t cb0 = A.h_rpb[q0], cb1 = A.h_rpb[q1];
+    const int64_t base = (int64_t)bs2 * cb0;
+    MF_CHECK(base + (int64_t)bs2 * (cb1 - cb0) < INT32_MAX, MF_EUNSUPPORTED, "CSR offset overflow");
+    int32_t* mc = &meta[8 * c];
+    mc[0] = q0; mc[1] = q1; mc[2] = ci[c]; mc[3] = ci[c + 1];
+    mc[4] = static_cast<int32_t>(cb0); mc[5] = static_cast<int32_t>(cb1 - cb0);
+    mc[6] = static_cast<int32_t>(base); mc[7] = static_cast<int32_t>(bs2 * (cb1 - cb0));
+  }
+  plan->chunk_meta.upload(meta);
+  return plan;
 }
 
 }  // namespace
@@ -179,21 +194,24 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
   MF_CHECK(!(mode == 2 && !owner), MF_EUNSUPPORTED,
            "owner-computes kernel requested but unavailable for this form/matrix");
   bool diag_done = false;
+  if (owner && square && bc0 != bc1 && (m0 || m1)) owner = nullptr;   // records hold one bc
+  MF_CHECK(!(mode == 2 && !owner), MF_EUNSUPPORTED,
+           "owner-computes kernel needs the same Dirichlet set on rows and columns");
   if (owner) {
     const int bs = a->dm[0]->bs;
     const uint8_t* n0 = m0 ? bc0->bits_for(bs, s) : nullptr;
-    const uint8_t* n1 = m1 ? bc1->bits_for(a->dm[1]->bs, s) : nullptr;
     int flags = zero ? 1 : 0;
-    if (zero && square && m0) flags |= 2;
-    if (square && n0 == n1) flags |= 4;
+    if (zero && m0) flags |= 2;
     diag_done = (flags & 2) != 0;
+    launch(owner->k_prep, static_cast<uint32_t>((owner->n + 255) / 256), 256, 0, s,
+           static_cast<int32_t>(owner->n), owner->d_ents.ptr, a->mesh->cells.ptr, a->mesh->x.ptr,
+           a->packed.ptr, a->dm[0]->dev.ptr, n0, owner->records.ptr);
     const int64_t nx = 8, per = (plan->nchunks + nx - 1) / nx;
     launch(owner->k_owner, static_cast<uint32_t>(per * nx),
            static_cast<uint32_t>(a->opt("owner_block", 256)), 0, s,
-           static_cast<int32_t>(plan->nchunks), plan->chunk_rows.ptr, plan->chunk_items.ptr,
-           plan->item_cell.ptr, plan->item_mask.ptr, a->mesh->cells.ptr, a->mesh->x.ptr,
-           a->packed.ptr, a->consts.ptr, a->dm[0]->dev.ptr, a->dm[1]->dev.ptr, n0, n1,
-           A->row_ptr.ptr, A->rpb.ptr, A->colsb.ptr, A->values, diagonal, flags);
+           static_cast<int32_t>(plan->nchunks), plan->chunk_meta.ptr, plan->item_cell.ptr,
+           plan->item_mask.ptr, owner->records.ptr, a->consts.ptr, A->rpb.ptr, A->colsb.ptr,
+           n0, A->values, diagonal, flags);
   } else if (zero) {
     k_fill(A->values, A->nnz, 0.0, s);
   }

@@ -176,7 +176,9 @@ struct Integral {
   int64_t n = 0;
   int index = 0;                      // position in the form
   hipFunction_t k_mat = nullptr, k_vec = nullptr, k_lift = nullptr, k_owner = nullptr;
-  hipFunction_t k_vown = nullptr, k_slift = nullptr;
+  hipFunction_t k_vown = nullptr, k_slift = nullptr, k_prep = nullptr;
+  DevArray<double> records;           // per-cell records of the owner-computes kernel
+  int64_t record_bytes = 0;
   // lifting entity lists per bc object uid
   std::map<uint64_t, std::pair<DevArray<int32_t>, int64_t>> lift_lists;
 };
@@ -230,6 +232,8 @@ struct mf_matrix {
 namespace mf {
 // code generation (codegen.cpp)
 std::string generate_form_source(const mf_form& f);
+bool owner_supported(const mf_form& f, const Integral& I);
+int64_t record_bytes(const mf_form& f, const Integral& I);
 std::string kernel_name(const char* what, int type, int index);
 // sparsity (sparsity.cpp): blocked pattern of all integrals
 void build_pattern(const mf_form& a, std::vector<int32_t>& rpb, std::vector<int32_t>& colsb);

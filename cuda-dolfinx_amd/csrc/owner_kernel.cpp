@@ -54,35 +54,81 @@ static __device__ __forceinline__ int mf_xcd_chunk(int nchunks)
 #endif
 )";
 
+// Per-cell record consumed by the owner kernel (square forms): geometry
+// factors, packed coefficients, row dofs and their Dirichlet bits, written
+// once per assembly by mf_prep_* in cell order.
+//   double K[9], |det J|, w[W]; int32 dofs[ND]; uint32 bits (BS*ND <= 32)
+const char* kCellRecord = R"(
+#ifndef MF_REC_$TAG
+#define MF_REC_$TAG
+struct __attribute__((aligned(16))) mf_rec_$TAG {
+  double K[9];
+  double det;
+  double w[$WW];
+  int32_t d[SK_ND];
+  uint32_t bits;
+};
+#endif
+)";
+
+const char* kPrepTemplate = R"(
+extern "C" __global__ void __launch_bounds__(256)
+$KNAME(const int32_t n, const int32_t* __restrict__ ents, const int32_t* __restrict__ geo,
+       const double* __restrict__ xg, const double* __restrict__ coeffs,
+       const int32_t* __restrict__ dm0, const uint8_t* __restrict__ nb0,
+       mf_rec_$TAG* __restrict__ rec)
+{
+  constexpr int ND = SK_ND, BS = SK_BS, NV = $NV, W = $W;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int64_t c = ents[e];
+  double x[NV][3];
+  for (int v = 0; v < NV; ++v) {
+    const int64_t gv = geo[c * NV + v];
+    x[v][0] = xg[3 * gv]; x[v][1] = xg[3 * gv + 1]; x[v][2] = xg[3 * gv + 2];
+  }
+  const double J00 = x[1][0] - x[0][0], J01 = x[2][0] - x[0][0], J02 = x[3][0] - x[0][0];
+  const double J10 = x[1][1] - x[0][1], J11 = x[2][1] - x[0][1], J12 = x[3][1] - x[0][1];
+  const double J20 = x[1][2] - x[0][2], J21 = x[2][2] - x[0][2], J22 = x[3][2] - x[0][2];
+  const double m00 = J11 * J22 - J12 * J21, m01 = J02 * J21 - J01 * J22, m02 = J01 * J12 - J02 * J11;
+  const double det = J00 * m00 + J10 * m01 + J20 * m02;
+  const double id = 1.0 / det;
+  mf_rec_$TAG r;
+  r.K[0] = m00 * id; r.K[1] = m01 * id; r.K[2] = m02 * id;
+  r.K[3] = (J12 * J20 - J10 * J22) * id; r.K[4] = (J00 * J22 - J02 * J20) * id; r.K[5] = (J02 * J10 - J00 * J12) * id;
+  r.K[6] = (J10 * J21 - J11 * J20) * id; r.K[7] = (J01 * J20 - J00 * J21) * id; r.K[8] = (J00 * J11 - J01 * J10) * id;
+  r.det = fabs(det);
+  for (int l = 0; l < $WW; ++l) r.w[l] = l < W ? coeffs[c * W + l] : 0.0;
+  uint32_t bits = 0;
+  for (int l = 0; l < ND; ++l) {
+    const int32_t node = dm0[c * ND + l];
+    r.d[l] = node;
+    if (nb0) bits |= (uint32_t)nb0[node] << (BS * l);
+  }
+  r.bits = bits;
+  rec[c] = r;
+}
+)";
+
 const char* kOwnerTemplate = R"(
 extern "C" __global__ void __launch_bounds__($OBLOCK)
-$KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_rows,
-       const int32_t* __restrict__ chunk_items, const int32_t* __restrict__ item_cell,
-       const uint32_t* __restrict__ item_mask, const int32_t* __restrict__ geo,
-       const double* __restrict__ xg, const double* __restrict__ coeffs,
-       const double* __restrict__ consts, const int32_t* __restrict__ dm0,
-       const int32_t* __restrict__ dm1, const uint8_t* __restrict__ nb0,
-       const uint8_t* __restrict__ nb1, const int32_t* __restrict__ row_ptr,
+$KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
+       const int32_t* __restrict__ item_cell, const uint32_t* __restrict__ item_mask,
+       const mf_rec_$TAG* __restrict__ rec, const double* __restrict__ consts,
        const int32_t* __restrict__ rpb, const int32_t* __restrict__ colsb,
-       double* __restrict__ values, const double diag, const int flags)
+       const uint8_t* __restrict__ nb0, double* __restrict__ values, const double diag,
+       const int flags)
 {
-  // nb0 / nb1: per-node Dirichlet bits (bit a = component a constrained), or null.
-  // flags: 1 overwrite (zero) instead of add, 2 set the diagonal of bc rows,
-  //        4 rows and columns share dofmap and bc (square form).
+  // chunk_meta[8k..8k+7] = r0, r1, i0, i1, cb0 (= rpb[r0]), ncb, base (CSR offset), seglen
+  // flags: 1 overwrite (zero) instead of add, 2 set the diagonal of Dirichlet rows.
   constexpr int ND = SK_ND, BS = SK_BS, NQ = SK_NQ;
-  constexpr int NB = $ONB, CAP = $OCAP, MAXR = $OMAXR, NV = $NV, W = $W;
-  constexpr int WW = W > 0 ? W : 1;
+  constexpr int NB = $ONB, CAP = $OCAP, MAXR = $OMAXR;
   constexpr int CAPB = CAP / (BS * BS) + 1;
-  constexpr int KS = 11;                     // per-cell record: K (9), |det|, pad
+  constexpr int RW = sizeof(mf_rec_$TAG) / 16;      // record size in 16-byte words
   __shared__ double acc[CAP];
   __shared__ int32_t lrp[MAXR + 1];
   __shared__ int32_t lcols[CAPB];
-  __shared__ double sK[NB * KS];
-  __shared__ double sw[NB * WW];
-  __shared__ int32_t sd0[NB * ND];
-  __shared__ int32_t sd1[NB * ND];
-  __shared__ uint8_t sb0[NB * ND];
-  __shared__ uint8_t sb1[NB * ND];
+  __shared__ mf_rec_$TAG srec[NB];
   __shared__ uint32_t smask[NB];
   __shared__ int32_t scell[NB];
   __shared__ int16_t pairs[NB * ND];
@@ -90,24 +136,19 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_rows,
 
   const int k = mf_xcd_chunk(nchunks);
   if (k < 0) return;
-  const bool square = (flags & 4) != 0;
-  const int32_t* d1 = square ? sd0 : sd1;
-  const uint8_t* s1 = square ? sb0 : sb1;
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int32_t r0 = chunk_rows[k], r1 = chunk_rows[k + 1];
-  const int i0 = chunk_items[k], i1 = chunk_items[k + 1];
-  // item loads of the first batch go out together with the chunk set-up loads
+  const int4 m0 = reinterpret_cast<const int4*>(chunk_meta)[2 * k];
+  const int4 m1 = reinterpret_cast<const int4*>(chunk_meta)[2 * k + 1];
+  const int32_t r0 = m0.x, r1 = m0.y, i0 = m0.z, i1 = m0.w;
+  const int32_t cb0 = m1.x, ncb = m1.y, base = m1.z, seglen = m1.w;
+  const int nr = r1 - r0;
+  // first batch of items: ids and masks go out with the chunk set-up loads
   int32_t pre_cell = 0;
   uint32_t pre_mask = 0;
   if (tid < min(NB, i1 - i0)) {
     pre_cell = item_cell[i0 + tid];
     pre_mask = item_mask[i0 + tid];
   }
-  const int nr = r1 - r0;
-  const int64_t base = row_ptr[(int64_t)BS * r0];
-  const int seglen = (int)(row_ptr[(int64_t)BS * r1] - base);
-  const int32_t cb0 = rpb[r0];
-  const int ncb = rpb[r1] - cb0;
   for (int p = tid; p < seglen; p += nt) acc[p] = 0.0;
   for (int p = tid; p <= nr; p += nt) lrp[p] = rpb[r0 + p] - cb0;
   for (int p = tid; p < ncb; p += nt) lcols[p] = colsb[cb0 + p];
@@ -123,49 +164,17 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_rows,
     }
     if (tid == 0) npairs = 0;
     if (tid < nb) {
-      scell[tid] = pre_cell;
       smask[tid] = pre_mask;
+      scell[tid] = pre_cell;
     }
     __syncthreads();
-    // stage 1: one thread per (cell, task): geometry, row dofs + bits,
-    // column dofs + bits (non-square forms), coefficients
-    const int ntask = square ? 3 : 4;
-    for (int u = tid; u < nb * ntask; u += nt) {
-      const int b = u % nb, t = u / nb;
-      const int64_t c = scell[b];
-      if (t == 0) {
-        double x[NV][3];
-        for (int v = 0; v < NV; ++v) {
-          const int64_t gv = geo[c * NV + v];
-          x[v][0] = xg[3 * gv]; x[v][1] = xg[3 * gv + 1]; x[v][2] = xg[3 * gv + 2];
-        }
-        const double J00 = x[1][0] - x[0][0], J01 = x[2][0] - x[0][0], J02 = x[3][0] - x[0][0];
-        const double J10 = x[1][1] - x[0][1], J11 = x[2][1] - x[0][1], J12 = x[3][1] - x[0][1];
-        const double J20 = x[1][2] - x[0][2], J21 = x[2][2] - x[0][2], J22 = x[3][2] - x[0][2];
-        const double m00 = J11 * J22 - J12 * J21, m01 = J02 * J21 - J01 * J22, m02 = J01 * J12 - J02 * J11;
-        const double det = J00 * m00 + J10 * m01 + J20 * m02;
-        const double id = 1.0 / det;
-        double* K = &sK[b * KS];
-        K[0] = m00 * id; K[1] = m01 * id; K[2] = m02 * id;
-        K[3] = (J12 * J20 - J10 * J22) * id; K[4] = (J00 * J22 - J02 * J20) * id; K[5] = (J02 * J10 - J00 * J12) * id;
-        K[6] = (J10 * J21 - J11 * J20) * id; K[7] = (J01 * J20 - J00 * J21) * id; K[8] = (J00 * J11 - J01 * J10) * id;
-        K[9] = fabs(det);
-      } else if (t == 1) {
-        int32_t n[ND];
-        for (int l = 0; l < ND; ++l) n[l] = dm0[c * ND + l];
-        for (int l = 0; l < ND; ++l) {
-          sd0[b * ND + l] = n[l];
-          sb0[b * ND + l] = nb0 ? nb0[n[l]] : (uint8_t)0;
-        }
-      } else if (t == 2) {
-        for (int l = 0; l < W; ++l) sw[b * WW + l] = coeffs[c * W + l];
-      } else {
-        int32_t n[ND];
-        for (int l = 0; l < ND; ++l) n[l] = dm1[c * ND + l];
-        for (int l = 0; l < ND; ++l) {
-          sd1[b * ND + l] = n[l];
-          sb1[b * ND + l] = nb1 ? nb1[n[l]] : (uint8_t)0;
-        }
+    // stage 1: copy the cell records, 16 bytes per thread
+    {
+      const int4* src = reinterpret_cast<const int4*>(rec);
+      int4* dst = reinterpret_cast<int4*>(srec);
+      for (int u = tid; u < nb * RW; u += nt) {
+        const int b = u / RW, w = u - b * RW;
+        dst[b * RW + w] = src[(int64_t)scell[b] * RW + w];
       }
     }
     // owned (cell, row) pairs
@@ -193,9 +202,9 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_rows,
       const int pr = (wv / js) * 64 + lane;
       if (pr >= np) continue;
       const int b = pairs[pr] >> 5, i = pairs[pr] & 31;
-      const double* K = &sK[b * KS];
-      const double K0 = K[0], K1 = K[1], K2 = K[2], K3 = K[3], K4 = K[4], K5 = K[5];
-      const double K6 = K[6], K7 = K[7], K8 = K[8], dt = K[9];
+      const mf_rec_$TAG& r = srec[b];
+      const double K0 = r.K[0], K1 = r.K[1], K2 = r.K[2], K3 = r.K[3], K4 = r.K[4], K5 = r.K[5];
+      const double K6 = r.K[6], K7 = r.K[7], K8 = r.K[8], dt = r.det;
       double gi[NQ][3];
       for (int q = 0; q < NQ; ++q) {
         const double s = SK_W[q] * dt;
@@ -204,12 +213,12 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_rows,
         gi[q][1] = s * (K1 * d0 + K4 * d1v + K7 * d2);
         gi[q][2] = s * (K2 * d0 + K5 * d1v + K8 * d2);
       }
-      const int32_t R = sd0[b * ND + i];
-      const uint8_t rb0 = sb0[b * ND + i];
+      const uint32_t bits = r.bits;
+      const int32_t R = r.d[i];
+      const uint32_t rb0 = (bits >> (BS * i)) & ((1u << BS) - 1);
       const int lr = R - r0;
       const int lo = lrp[lr], hi = lrp[lr + 1];
       const int rowlen = BS * (hi - lo);
-      const double* wb = &sw[b * WW];
       const int j0 = part * jw, j1 = min(ND, j0 + jw);
       for (int j = j0; j < j1; ++j) {
         double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -223,15 +232,15 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_rows,
           G[6] += gi[q][2] * c0; G[7] += gi[q][2] * c1; G[8] += gi[q][2] * c2;
         }
         double blk[BS * BS];
-        sk_combine(G, wb, consts, blk);
-        const int32_t Cn = d1[b * ND + j];
-        const uint8_t cb1 = s1[b * ND + j];
+        sk_combine(G, r.w, consts, blk);
+        const int32_t Cn = r.d[j];
+        const uint32_t cb1 = (bits >> (BS * j)) & ((1u << BS) - 1);
         const int kk = mf_lds_lower_bound(lcols, lo, hi, Cn) - lo;
         const int rb = BS * BS * lo + BS * kk;
         for (int a = 0; a < BS; ++a) {
-          if (rb0 & (1 << a)) continue;
+          if (rb0 & (1u << a)) continue;
           for (int bb = 0; bb < BS; ++bb) {
-            if (cb1 & (1 << bb)) continue;
+            if (cb1 & (1u << bb)) continue;
             atomicAdd(&acc[rb + a * rowlen + bb], blk[a * BS + bb]);
           }
         }
@@ -240,19 +249,20 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_rows,
   }
   __syncthreads();
   if ((flags & 2) && nb0) {
+    // Dirichlet rows: set the diagonal entries of the constrained components
     for (int lr = tid; lr < nr; lr += nt) {
       const int32_t R = r0 + lr;
-      const uint8_t bits = nb0[R];
-      if (!bits) continue;
+      const uint32_t rbits = nb0[R];
+      if (!rbits) continue;
       const int lo = lrp[lr], hi = lrp[lr + 1];
       const int p = mf_lds_lower_bound(lcols, lo, hi, R);
       if (p < hi && lcols[p] == R)
         for (int a = 0; a < BS; ++a)
-          if (bits & (1 << a)) acc[BS * BS * lo + a * BS * (hi - lo) + BS * (p - lo) + a] = diag;
+          if (rbits & (1u << a)) acc[BS * BS * lo + a * BS * (hi - lo) + BS * (p - lo) + a] = diag;
     }
     __syncthreads();
   }
-  double* out = values + base;
+  double* out = values + (int64_t)(uint32_t)base;
   if (flags & 1) {
     for (int p = tid; p < seglen; p += nt) __builtin_nontemporal_store(acc[p], &out[p]);
   } else {

@@ -197,12 +197,11 @@ def main():
     }
     if args.copy_calib:
         nbytes = args.copy_calib * (1 << 20)
-        src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
+        src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+        src.fill_(1.0)                    # writes exactly nbytes (WRITE_SIZE calibration)
+        tot = torch.sum(src)              # reads exactly nbytes (FETCH_SIZE calibration)
         torch.cuda.synchronize()
-        log(f"copy calibration: 3 x {args.copy_calib} MiB device-to-device")
+        log(f"traffic calibration: fill + sum of {args.copy_calib} MiB (sum={float(tot):.0f})")
     if not args.no_cpu and args.cpu_sample > 0:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)

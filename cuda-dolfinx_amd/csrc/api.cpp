@@ -283,7 +283,8 @@ int mf_form_set_option(mf_form* f, const char* key, int64_t value) {
     MF_CHECK(f && key, MF_EINVAL, "null argument");
     static const char* known[] = {"matrix_kernel", "chunk_values", "chunk_rows", "block",
                                   "owner_block", "owner_nb", "owner_lds", "vector_kernel",
-                                  "vector_block", "vector_chunk_values", "lift_kernel"};
+                                  "vector_block", "vector_chunk_values", "lift_kernel",
+                                  "owner_pairs"};
     bool ok = false;
     for (auto k : known) ok |= std::strcmp(k, key) == 0;
     MF_CHECK(ok, MF_EINVAL, std::string("unknown option ") + key);
@@ -310,7 +311,7 @@ int mf_form_compile(mf_form* f, int verbose) {
     int64_t cap = f->opt("chunk_values", 4096);
     f->options["chunk_values"] = cap;
     if (!f->options.count("owner_nb")) f->options["owner_nb"] = 32;
-    if (!f->options.count("chunk_rows")) f->options["chunk_rows"] = 256;
+    if (!f->options.count("chunk_rows")) f->options["chunk_rows"] = 64;
     std::string src = mf::generate_form_source(*f);
     {
       // owner-kernel compile-time shape
@@ -324,7 +325,7 @@ int mf_form_compile(mf_form* f, int verbose) {
       };
       rep("$OBLOCK", f->opt("owner_block", 256));
       rep("$OCAP", cap);
-      rep("$OMAXR", f->opt("chunk_rows", 256));
+      rep("$OMAXR", f->opt("chunk_rows", 64));
     }
     f->source = src;
     bool cached = false;

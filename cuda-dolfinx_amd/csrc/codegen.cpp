@@ -207,19 +207,22 @@ bool owner_supported(const mf_form& f, const Integral& I) {
   return nd <= 32 && nd * bs <= 32;
 }
 
-int owner_batch(const mf_form& f, const Integral& I) {
-  // Cells per batch so the kernel's static LDS fits the per-workgroup budget.
+OwnerShape owner_shape(const mf_form& f, const Integral& I) {
+  // Cells per batch (NB) and pairs per chunk (MAXP) so that the kernel's
+  // static LDS fits the per-workgroup budget.
   const int nd = staged_define(I.staged_src, "SK_ND");
   const int bs = staged_define(I.staged_src, "SK_BS");
-  const int64_t cap = f.opt("chunk_values", 4096), maxr = f.opt("chunk_rows", 256);
+  const int64_t cap = f.opt("chunk_values", 4096), maxr = f.opt("chunk_rows", 64);
   const int64_t budget = f.opt("owner_lds", 64 * 1024);
-  const int64_t fixed = 8 * cap + 4 * (maxr + 1) + 4 * (cap / (bs * bs) + 1) + 16;
-  const int64_t per_cell = record_bytes(f, I) + 4 + 4 + 2LL * nd;
-  int64_t nb = (budget - fixed) / per_cell;
-  nb = std::min<int64_t>(nb, f.opt("owner_nb", 64));
-  nb = std::min<int64_t>(nb, 1023);
-  MF_CHECK(nb >= 1, MF_EUNSUPPORTED, "owner-computes kernel does not fit the LDS budget");
-  return static_cast<int>(nb);
+  const int64_t fixed = 8 * cap + 4 * (maxr + 1) + 4 * (cap / (bs * bs) + 1) + 32;
+  OwnerShape sh;
+  sh.nb = static_cast<int>(std::min<int64_t>(f.opt("owner_nb", 64), 1023));
+  sh.maxp = static_cast<int>(f.opt("owner_pairs", 384));
+  const int64_t used = fixed + sh.nb * (record_bytes(f, I) + 4) + (int64_t)sh.maxp * (4 + nd);
+  MF_CHECK(used <= budget, MF_EUNSUPPORTED,
+           "owner-computes kernel shape exceeds the LDS budget (" + std::to_string(used) + " > " +
+               std::to_string(budget) + " bytes)");
+  return sh;
 }
 
 std::string kernel_name(const char* what, int type, int index) {
@@ -302,7 +305,9 @@ std::string generate_form_source(const mf_form& f) {
           m["$KNAME"] = kernel_name("prep", I->type, I->index);
           os << subst(rename(kPrepTemplate), m);
           m["$KNAME"] = kernel_name("owner", I->type, I->index);
-          m["$ONB"] = std::to_string(owner_batch(f, *I));
+          const OwnerShape sh = owner_shape(f, *I);
+          m["$ONB"] = std::to_string(sh.nb);
+          m["$OMAXP"] = std::to_string(sh.maxp);
           os << subst(rename(kOwnerTemplate), m);
         }
         m["$KNAME"] = kernel_name("slift", I->type, I->index);

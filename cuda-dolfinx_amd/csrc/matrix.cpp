@@ -3,6 +3,7 @@
 // cpp/cudolfinx/fem/CUDAAssembler.h:683-740, and set_diagonal :878-958).
 #include <algorithm>
 #include <cstdio>
+#include <thread>
 
 #include "mf_internal.h"
 
@@ -10,9 +11,13 @@ namespace mf {
 
 struct OwnerPlan {
   int64_t nchunks = 0;
-  DevArray<int32_t> chunk_rows, chunk_items, item_cell, chunk_meta;
+  DevArray<int32_t> chunk_rows, chunk_items, item_cell, chunk_meta, pair_list;
   DevArray<uint32_t> item_mask;
-  int64_t nitems = 0;
+  DevArray<uint8_t> pair_kk;
+  int64_t nitems = 0, npairs = 0;
+  // host copies used while building matrix schedules
+  std::vector<int32_t> h_items, h_cell;
+  std::vector<uint32_t> h_mask;
 };
 
 namespace {
@@ -70,44 +75,107 @@ std::shared_ptr<OwnerPlan> build_items(const std::vector<int32_t>& rows, const I
   plan->chunk_items.upload(ci);
   plan->item_cell.upload(icell);
   plan->item_mask.upload(imask);
+  plan->h_items = std::move(ci);
+  plan->h_cell = std::move(icell);
+  plan->h_mask = std::move(imask);
   return plan;
 }
 
 // Contiguous blocked-row chunks whose unrolled value segment fits `cap`
-// doubles (at most `maxr` rows).
+// doubles, with at most `maxr` rows and `maxp` (cell, owned row) pairs; then
+// the per-chunk pair lists and, for every pair and column j of the cell, the
+// position of the column node in the row's blocked column list.
 std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I, const mf_matrix& A,
-                                            int64_t cap, int maxr) {
-  if (a.dm[0]->nd > 32) return nullptr;
+                                            int64_t cap, int maxr, int maxp) {
+  const mf_dofmap* d0 = a.dm[0];
+  const mf_dofmap* d1 = a.dm[1];
+  const int nd = d0->nd;
+  if (nd > 32) return nullptr;
   const int bs2 = A.bs0 * A.bs1;
+  std::vector<int32_t> ncell(A.nrows_b, 0);
+  for (int64_t e = 0; e < I.n; ++e)
+    for (int l = 0; l < nd; ++l) ncell[d0->host[(int64_t)I.h_ents[e] * nd + l]]++;
   std::vector<int32_t> rows{0};
-  int64_t seg = 0;
+  int64_t seg = 0, pairs = 0;
   int32_t r0 = 0;
   for (int64_t r = 0; r < A.nrows_b; ++r) {
-    int64_t len = (int64_t)bs2 * (A.h_rpb[r + 1] - A.h_rpb[r]);
-    if (len > cap) return nullptr;   // a single row does not fit: caller falls back
-    if (seg + len > cap || r - r0 >= maxr) {
+    const int64_t len = (int64_t)bs2 * (A.h_rpb[r + 1] - A.h_rpb[r]);
+    if (len > cap || ncell[r] > maxp) return nullptr;   // a single row does not fit
+    if (seg + len > cap || r - r0 >= maxr || pairs + ncell[r] > maxp) {
       rows.push_back(static_cast<int32_t>(r));
       r0 = static_cast<int32_t>(r);
       seg = 0;
+      pairs = 0;
     }
     seg += len;
+    pairs += ncell[r];
   }
   rows.push_back(static_cast<int32_t>(A.nrows_b));
-  auto plan = build_items(rows, I, a.dm[0]);
-  // per-chunk metadata: r0, r1, i0, i1, cb0, ncb, CSR base, segment length
-  std::vector<int32_t> ci = plan->chunk_items.download();
-  std::vector<int32_t> meta(8 * plan->nchunks);
-  for (int64_t c = 0; c < plan->nchunks; ++c) {
+  auto plan = build_items(rows, I, d0);
+  const int64_t nch = plan->nchunks;
+  const auto& ci = plan->h_items;
+  // pair offsets per chunk
+  std::vector<int64_t> poff(nch + 1, 0);
+  for (int64_t c = 0; c < nch; ++c) {
+    int64_t n = 0;
+    for (int32_t it = ci[c]; it < ci[c + 1]; ++it) n += __builtin_popcount(plan->h_mask[it]);
+    poff[c + 1] = poff[c] + n;
+  }
+  const int64_t np = poff[nch];
+  std::vector<int32_t> plist(std::max<int64_t>(1, np));
+  std::vector<uint8_t> pkk(std::max<int64_t>(1, np * nd));
+  std::vector<int> bad(16, 0);
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  auto work = [&](unsigned t) {
+    for (int64_t c = t; c < nch; c += nt) {
+      int64_t p = poff[c];
+      for (int32_t it = ci[c]; it < ci[c + 1]; ++it) {
+        const int32_t cell = plan->h_cell[it];
+        uint32_t msk = plan->h_mask[it];
+        while (msk) {
+          const int i = __builtin_ctz(msk);
+          msk &= msk - 1;
+          plist[p] = ((it - ci[c]) << 8) | i;
+          const int32_t R = d0->host[(int64_t)cell * nd + i];
+          const int32_t* rb = A.h_colsb.data() + A.h_rpb[R];
+          const int32_t* re = A.h_colsb.data() + A.h_rpb[R + 1];
+          for (int j = 0; j < nd; ++j) {
+            const int32_t Cn = d1->host[(int64_t)cell * nd + j];
+            const int32_t* f = std::lower_bound(rb, re, Cn);
+            if (f == re || *f != Cn || f - rb > 255) {
+              bad[t % 16] = 1;
+              pkk[p * nd + j] = 0;
+            } else {
+              pkk[p * nd + j] = static_cast<uint8_t>(f - rb);
+            }
+          }
+          ++p;
+        }
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; ++t) th.emplace_back(work, t);
+  for (auto& t : th) t.join();
+  for (int v : bad)
+    if (v) return nullptr;   // entry missing from the pattern or row too long for 8-bit offsets
+  std::vector<int32_t> meta(12 * nch, 0);
+  for (int64_t c = 0; c < nch; ++c) {
     const int32_t q0 = rows[c], q1 = rows[c + 1];
     const int64_t cb0 = A.h_rpb[q0], cb1 = A.h_rpb[q1];
     const int64_t base = (int64_t)bs2 * cb0;
     MF_CHECK(base + (int64_t)bs2 * (cb1 - cb0) < INT32_MAX, MF_EUNSUPPORTED, "CSR offset overflow");
-    int32_t* mc = &meta[8 * c];
+    int32_t* mc = &meta[12 * c];
     mc[0] = q0; mc[1] = q1; mc[2] = ci[c]; mc[3] = ci[c + 1];
     mc[4] = static_cast<int32_t>(cb0); mc[5] = static_cast<int32_t>(cb1 - cb0);
     mc[6] = static_cast<int32_t>(base); mc[7] = static_cast<int32_t>(bs2 * (cb1 - cb0));
+    mc[8] = static_cast<int32_t>(poff[c]); mc[9] = static_cast<int32_t>(poff[c + 1]);
   }
   plan->chunk_meta.upload(meta);
+  plan->pair_list.upload(plist);
+  plan->pair_kk.upload(pkk);
+  plan->npairs = np;
+  plan->h_items.clear(); plan->h_cell.clear(); plan->h_mask.clear();
   return plan;
 }
 
@@ -184,7 +252,8 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
     auto it = A->plans.find(key);
     if (it == A->plans.end()) {
       plan = build_owner_plan(*a, *owner, *A, a->opt("chunk_values", 4096),
-                              static_cast<int>(a->opt("chunk_rows", 256)));
+                              static_cast<int>(a->opt("chunk_rows", 64)),
+                              owner_shape(*a, *owner).maxp);
       A->plans[key] = plan;
     } else {
       plan = it->second;
@@ -210,8 +279,8 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
     launch(owner->k_owner, static_cast<uint32_t>(per * nx),
            static_cast<uint32_t>(a->opt("owner_block", 256)), 0, s,
            static_cast<int32_t>(plan->nchunks), plan->chunk_meta.ptr, plan->item_cell.ptr,
-           plan->item_mask.ptr, owner->records.ptr, a->consts.ptr, A->rpb.ptr, A->colsb.ptr,
-           n0, A->values, diagonal, flags);
+           plan->pair_list.ptr, plan->pair_kk.ptr, owner->records.ptr, a->consts.ptr, A->rpb.ptr,
+           A->colsb.ptr, n0, A->values, diagonal, flags);
   } else if (zero) {
     k_fill(A->values, A->nnz, 0.0, s);
   }

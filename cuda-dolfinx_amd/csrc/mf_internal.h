@@ -233,6 +233,8 @@ namespace mf {
 // code generation (codegen.cpp)
 std::string generate_form_source(const mf_form& f);
 bool owner_supported(const mf_form& f, const Integral& I);
+struct OwnerShape { int nb = 0, maxp = 0; };
+OwnerShape owner_shape(const mf_form& f, const Integral& I);
 int64_t record_bytes(const mf_form& f, const Integral& I);
 std::string kernel_name(const char* what, int type, int index);
 // sparsity (sparsity.cpp): blocked pattern of all integrals

@@ -32,6 +32,7 @@ namespace mf {
 const char* kOwnerCommon = R"(
 #ifndef MF_OWNER_COMMON
 #define MF_OWNER_COMMON
+typedef double mf_d2 __attribute__((ext_vector_type(2)));
 static __device__ __forceinline__ int mf_lds_lower_bound(const int32_t* a, int lo, int hi, int32_t key)
 {
   while (lo < hi) {
@@ -113,59 +114,58 @@ $KNAME(const int32_t n, const int32_t* __restrict__ ents, const int32_t* __restr
 const char* kOwnerTemplate = R"(
 extern "C" __global__ void __launch_bounds__($OBLOCK)
 $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
-       const int32_t* __restrict__ item_cell, const uint32_t* __restrict__ item_mask,
-       const mf_rec_$TAG* __restrict__ rec, const double* __restrict__ consts,
-       const int32_t* __restrict__ rpb, const int32_t* __restrict__ colsb,
-       const uint8_t* __restrict__ nb0, double* __restrict__ values, const double diag,
-       const int flags)
+       const int32_t* __restrict__ item_cell, const int32_t* __restrict__ pair_list,
+       const uint8_t* __restrict__ pair_kk, const mf_rec_$TAG* __restrict__ rec,
+       const double* __restrict__ consts, const int32_t* __restrict__ rpb,
+       const int32_t* __restrict__ colsb, const uint8_t* __restrict__ nb0,
+       double* __restrict__ values, const double diag, const int flags)
 {
-  // chunk_meta[8k..8k+7] = r0, r1, i0, i1, cb0 (= rpb[r0]), ncb, base (CSR offset), seglen
+  // chunk_meta[12k ..]: r0, r1, i0, i1, cb0 (= rpb[r0]), ncb, base (CSR offset),
+  //   seglen, p0, p1 (pair range), 0, 0.
+  // pair_list[p] = (item offset in chunk) << 8 | local row i; pair_kk[p*ND + j] =
+  //   position of column node j within the blocked row of node i.
   // flags: 1 overwrite (zero) instead of add, 2 set the diagonal of Dirichlet rows.
   constexpr int ND = SK_ND, BS = SK_BS, NQ = SK_NQ;
-  constexpr int NB = $ONB, CAP = $OCAP, MAXR = $OMAXR;
+  constexpr int NB = $ONB, CAP = $OCAP, MAXR = $OMAXR, MAXP = $OMAXP;
   constexpr int CAPB = CAP / (BS * BS) + 1;
   constexpr int RW = sizeof(mf_rec_$TAG) / 16;      // record size in 16-byte words
-  __shared__ double acc[CAP];
+  __shared__ __attribute__((aligned(16))) double acc[CAP];
   __shared__ int32_t lrp[MAXR + 1];
   __shared__ int32_t lcols[CAPB];
   __shared__ mf_rec_$TAG srec[NB];
-  __shared__ uint32_t smask[NB];
   __shared__ int32_t scell[NB];
-  __shared__ int16_t pairs[NB * ND];
-  __shared__ int npairs;
+  __shared__ int32_t spair[MAXP];
+  __shared__ __attribute__((aligned(16))) uint8_t skk[MAXP * ND + 16];
 
   const int k = mf_xcd_chunk(nchunks);
   if (k < 0) return;
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int4 m0 = reinterpret_cast<const int4*>(chunk_meta)[2 * k];
-  const int4 m1 = reinterpret_cast<const int4*>(chunk_meta)[2 * k + 1];
+  const int4 m0 = reinterpret_cast<const int4*>(chunk_meta)[3 * k];
+  const int4 m1 = reinterpret_cast<const int4*>(chunk_meta)[3 * k + 1];
+  const int4 m2 = reinterpret_cast<const int4*>(chunk_meta)[3 * k + 2];
   const int32_t r0 = m0.x, r1 = m0.y, i0 = m0.z, i1 = m0.w;
   const int32_t cb0 = m1.x, ncb = m1.y, base = m1.z, seglen = m1.w;
+  const int32_t p0 = m2.x, p1 = m2.y;
   const int nr = r1 - r0;
-  // first batch of items: ids and masks go out with the chunk set-up loads
-  int32_t pre_cell = 0;
-  uint32_t pre_mask = 0;
-  if (tid < min(NB, i1 - i0)) {
-    pre_cell = item_cell[i0 + tid];
-    pre_mask = item_mask[i0 + tid];
+  const int npc = p1 - p0;
+  // chunk set-up: cell ids, pair list and column offsets, zero accumulator
+  for (int b = tid; b < i1 - i0 && b < NB; b += nt) scell[b] = item_cell[i0 + b];
+  for (int p = tid; p < npc; p += nt) spair[p] = pair_list[p0 + p];
+  for (int p = tid; p < npc * ND; p += nt) skk[p] = pair_kk[(int64_t)p0 * ND + p];
+  {
+    mf_d2* a2 = reinterpret_cast<mf_d2*>(acc);
+    const mf_d2 z = {0.0, 0.0};
+    for (int p = tid; p < (seglen + 1) / 2; p += nt) a2[p] = z;
   }
-  for (int p = tid; p < seglen; p += nt) acc[p] = 0.0;
   for (int p = tid; p <= nr; p += nt) lrp[p] = rpb[r0 + p] - cb0;
   for (int p = tid; p < ncb; p += nt) lcols[p] = colsb[cb0 + p];
 
+  int pb = 0;                                        // first pair of the batch
   for (int ib = i0; ib < i1; ib += NB) {
     const int nb = min(NB, i1 - ib);
     if (ib > i0) {
-      __syncthreads();                        // previous batch fully consumed
-      if (tid < nb) {
-        pre_cell = item_cell[ib + tid];
-        pre_mask = item_mask[ib + tid];
-      }
-    }
-    if (tid == 0) npairs = 0;
-    if (tid < nb) {
-      smask[tid] = pre_mask;
-      scell[tid] = pre_cell;
+      __syncthreads();                               // previous batch fully consumed
+      for (int b = tid; b < nb; b += nt) scell[b] = item_cell[ib + b];
     }
     __syncthreads();
     // stage 1: copy the cell records, 16 bytes per thread
@@ -177,20 +177,22 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
         dst[b * RW + w] = src[(int64_t)scell[b] * RW + w];
       }
     }
-    // owned (cell, row) pairs
-    for (int b = tid; b < nb; b += nt) {
-      uint32_t m = smask[b];
-      int off = atomicAdd(&npairs, __popc(m));
-      while (m) {
-        const int i = __ffs(m) - 1;
-        m &= m - 1;
-        pairs[off++] = (int16_t)((b << 5) | i);
+    // pairs of this batch: those of items [ib - i0, ib - i0 + nb) of the chunk
+    int pe;
+    {
+      int lo2 = pb, hi2 = npc;
+      const int lim = ib - i0 + nb;
+      while (lo2 < hi2) {
+        const int mid = (lo2 + hi2) >> 1;
+        if ((spair[mid] >> 8) < lim) lo2 = mid + 1;
+        else hi2 = mid;
       }
+      pe = lo2;
     }
     __syncthreads();
-    // stage 2: one thread per (cell, owned row i, column range); all lanes of a
-    // wave share the column range, so the column basis tables are wave-uniform.
-    const int np = npairs;
+    // stage 2: one thread per (pair, column range); lanes of a wave share the
+    // column range, so the column basis tables are wave-uniform.
+    const int np = pe - pb;
     const int waves = nt >> 6;
     int js = (waves * 64) / max(np, 1);
     js = js < 1 ? 1 : (js > ND ? ND : js);
@@ -201,7 +203,8 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
       const int part = __builtin_amdgcn_readfirstlane(wv % js);
       const int pr = (wv / js) * 64 + lane;
       if (pr >= np) continue;
-      const int b = pairs[pr] >> 5, i = pairs[pr] & 31;
+      const int pc = pb + pr;                        // pair index within the chunk
+      const int b = (spair[pc] >> 8) - (ib - i0), i = spair[pc] & 255;
       const mf_rec_$TAG& r = srec[b];
       const double K0 = r.K[0], K1 = r.K[1], K2 = r.K[2], K3 = r.K[3], K4 = r.K[4], K5 = r.K[5];
       const double K6 = r.K[6], K7 = r.K[7], K8 = r.K[8], dt = r.det;
@@ -214,11 +217,12 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
         gi[q][2] = s * (K2 * d0 + K5 * d1v + K8 * d2);
       }
       const uint32_t bits = r.bits;
-      const int32_t R = r.d[i];
       const uint32_t rb0 = (bits >> (BS * i)) & ((1u << BS) - 1);
-      const int lr = R - r0;
+      const int lr = r.d[i] - r0;
       const int lo = lrp[lr], hi = lrp[lr + 1];
       const int rowlen = BS * (hi - lo);
+      const int rbase = BS * BS * lo;
+      const uint8_t* kk = &skk[pc * ND];
       const int j0 = part * jw, j1 = min(ND, j0 + jw);
       for (int j = j0; j < j1; ++j) {
         double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -233,19 +237,23 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
         }
         double blk[BS * BS];
         sk_combine(G, r.w, consts, blk);
-        const int32_t Cn = r.d[j];
         const uint32_t cb1 = (bits >> (BS * j)) & ((1u << BS) - 1);
-        const int kk = mf_lds_lower_bound(lcols, lo, hi, Cn) - lo;
-        const int rb = BS * BS * lo + BS * kk;
-        for (int a = 0; a < BS; ++a) {
-          if (rb0 & (1u << a)) continue;
-          for (int bb = 0; bb < BS; ++bb) {
-            if (cb1 & (1u << bb)) continue;
-            atomicAdd(&acc[rb + a * rowlen + bb], blk[a * BS + bb]);
+        const int rb = rbase + BS * kk[j];
+        if ((rb0 | cb1) == 0) {
+          for (int a = 0; a < BS; ++a)
+            for (int bb = 0; bb < BS; ++bb) atomicAdd(&acc[rb + a * rowlen + bb], blk[a * BS + bb]);
+        } else {
+          for (int a = 0; a < BS; ++a) {
+            if (rb0 & (1u << a)) continue;
+            for (int bb = 0; bb < BS; ++bb) {
+              if (cb1 & (1u << bb)) continue;
+              atomicAdd(&acc[rb + a * rowlen + bb], blk[a * BS + bb]);
+            }
           }
         }
       }
     }
+    pb = pe;
   }
   __syncthreads();
   if ((flags & 2) && nb0) {
@@ -263,7 +271,12 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
     __syncthreads();
   }
   double* out = values + (int64_t)(uint32_t)base;
-  if (flags & 1) {
+  if ((flags & 1) && ((base & 1) == 0)) {
+    const mf_d2* a2 = reinterpret_cast<const mf_d2*>(acc);
+    mf_d2* o2 = reinterpret_cast<mf_d2*>(out);
+    for (int p = tid; p < seglen / 2; p += nt) __builtin_nontemporal_store(a2[p], &o2[p]);
+    if ((seglen & 1) && tid == 0) out[seglen - 1] = acc[seglen - 1];
+  } else if (flags & 1) {
     for (int p = tid; p < seglen; p += nt) __builtin_nontemporal_store(acc[p], &out[p]);
   } else {
     for (int p = tid; p < seglen; p += nt) out[p] += acc[p];

@@ -20,12 +20,12 @@ from pathlib import Path
 from .form import CUDAForm
 
 DEFAULT_CANDIDATES = [
-    {"chunk_values": 4096, "owner_nb": 64, "owner_block": 256, "owner_lds": 65536},
-    {"chunk_values": 4096, "owner_nb": 64, "owner_block": 512, "owner_lds": 65536},
-    {"chunk_values": 4096, "owner_nb": 32, "owner_block": 512, "owner_lds": 53248},
-    {"chunk_values": 2048, "owner_nb": 48, "owner_block": 256, "owner_lds": 40960},
-    {"chunk_values": 6144, "owner_nb": 64, "owner_block": 512, "owner_lds": 81920},
-    {"chunk_values": 8192, "owner_nb": 128, "owner_block": 512, "owner_lds": 106496},
+    {"chunk_values": 4096, "owner_nb": 64, "owner_pairs": 384, "owner_block": 256, "owner_lds": 65536},
+    {"chunk_values": 4096, "owner_nb": 48, "owner_pairs": 256, "owner_block": 256, "owner_lds": 65536},
+    {"chunk_values": 2048, "owner_nb": 48, "owner_pairs": 256, "owner_block": 256, "owner_lds": 40960},
+    {"chunk_values": 2048, "owner_nb": 32, "owner_pairs": 192, "owner_block": 128, "owner_lds": 40960},
+    {"chunk_values": 6144, "owner_nb": 64, "owner_pairs": 512, "owner_block": 256, "owner_lds": 98304},
+    {"chunk_values": 4096, "owner_nb": 64, "owner_pairs": 384, "owner_block": 512, "owner_lds": 65536},
 ]
 
 

@@ -51,6 +51,18 @@ def algorithmic_bytes(P, A):
     return 8 * nnz + 4 * nnz_b + 4 * (nb + 1) + 24 * nv + 16 * nc + 16 * nc + 4 * nd * nc
 
 
+def measured_traffic():
+    """HBM bytes per matrix launch from the committed rocprofv3 FETCH/WRITE summary."""
+    p = ROOT / "profiles" / "c2_elasticity_p2_64_matrix_traffic.md"
+    try:
+        for line in p.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(n_sample: int):
     """Oracle (single core, FP64) on the same workload at n_sample^3 x 6 cells."""
     from mifem import problems
@@ -190,8 +202,10 @@ def main():
                                      2: "owner-computes"}[args.kernel],
                    "matrix_ms": kms, "b_region_ms": vms, "autotune": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "mf_owner_cell_0" if args.kernel != 1 else "mf_mat_cell_0",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic() if args.n == 64 and args.kernel != 1 else None,
+                     "kernel": ("matrix launch: mf_prep_cell_0 + mf_owner_cell_0"
+                                if args.kernel != 1 else "mf_mat_cell_0"),
                      "algorithmic_bytes": alg},
         "cpu_baseline": None,
     }

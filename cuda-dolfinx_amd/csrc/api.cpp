@@ -284,11 +284,12 @@ int mf_form_set_option(mf_form* f, const char* key, int64_t value) {
     static const char* known[] = {"matrix_kernel", "chunk_values", "chunk_rows", "block",
                                   "owner_block", "owner_nb", "owner_lds", "vector_kernel",
                                   "vector_block", "vector_chunk_values", "lift_kernel",
-                                  "owner_pairs"};
+                                  "owner_pairs", "precision"};
     bool ok = false;
     for (auto k : known) ok |= std::strcmp(k, key) == 0;
     MF_CHECK(ok, MF_EINVAL, std::string("unknown option ") + key);
     const bool runtime_key = std::strcmp(key, "matrix_kernel") == 0 ||
+                             std::strcmp(key, "precision") == 0 ||
                              std::strcmp(key, "vector_kernel") == 0 ||
                              std::strcmp(key, "lift_kernel") == 0;
     MF_CHECK(!f->compiled || runtime_key, MF_ESTATE,
@@ -336,7 +337,16 @@ int mf_form_compile(mf_form* f, int verbose) {
       if (f->rank == 2) {
         I->k_mat = f->mod->get(mf::kernel_name("mat", I->type, I->index));
         I->k_lift = f->mod->get(mf::kernel_name("lift", I->type, I->index));
-        if (I->type == MF_CELL && !I->staged_src.empty()) {
+        if (I->type == MF_CELL && mf::is_hex_staged(*I)) {
+          if (mf::hex_owner_supported(*f, *I)) {
+            I->k_hprep = f->mod->get(mf::kernel_name("hprep", I->type, I->index));
+            const char* names[3] = {"howner64", "howner32", "howner16"};
+            for (int v = 0; v < 3; ++v)
+              I->k_howner[v] = f->mod->get(mf::kernel_name(names[v], I->type, I->index));
+            I->record_bytes = 464;
+            I->records.alloc(f->mesh->num_cells * I->record_bytes / 8);
+          }
+        } else if (I->type == MF_CELL && !I->staged_src.empty()) {
           I->k_slift = f->mod->get(mf::kernel_name("slift", I->type, I->index));
           if (mf::owner_supported(*f, *I)) {
             I->k_owner = f->mod->get(mf::kernel_name("owner", I->type, I->index));

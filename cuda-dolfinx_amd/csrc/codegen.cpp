@@ -200,7 +200,30 @@ int64_t record_bytes(const mf_form& f, const Integral& I) {
   return (raw + 15) / 16 * 16;
 }
 
+bool is_hex_staged(const Integral& I) {
+  return I.staged_src.find("#define HX_K ") != std::string::npos;
+}
+
+bool hex_owner_supported(const mf_form& f, const Integral& I) {
+  if (!is_hex_staged(I) || f.rank != 2 || f.dm[0] != f.dm[1] || f.mesh->nv != 8 || f.W != 0)
+    return false;
+  return staged_define(I.staged_src, "HX_K") == 3 && f.dm[0]->bs == 1 && f.dm[0]->nd == 64;
+}
+
+int hex_batch(const mf_form& f, const Integral& I, int prec) {
+  const int nq1 = staged_define(I.staged_src, "HX_NQ1");
+  const int64_t cap = f.opt("chunk_values", 4096), maxr = f.opt("chunk_rows", 64);
+  const int64_t budget = f.opt("owner_lds", 78 * 1024);
+  const int64_t fixed = 8 * cap + 4 * (maxr + 1) + 4 * (cap + 1) + 4 * f.opt("owner_pairs", 384) + 64;
+  const int64_t per_cell = 464 + 4 + (int64_t)nq1 * nq1 * nq1 * 8 * (prec == 64 ? 8 : 4);
+  int64_t nb = (budget - fixed) / per_cell;
+  nb = std::min<int64_t>(nb, f.opt("owner_nb", 16));
+  MF_CHECK(nb >= 1, MF_EUNSUPPORTED, "hexahedral owner kernel does not fit the LDS budget");
+  return static_cast<int>(nb);
+}
+
 bool owner_supported(const mf_form& f, const Integral& I) {
+  if (is_hex_staged(I)) return false;
   if (f.rank != 2 || f.dm[0] != f.dm[1] || I.staged_src.empty() || f.mesh->nv != 4) return false;
   const int nd = staged_define(I.staged_src, "SK_ND");
   const int bs = staged_define(I.staged_src, "SK_BS");
@@ -236,6 +259,9 @@ extern const char* kOwnerVectorTemplate;
 extern const char* kStagedLiftTemplate;
 extern const char* kCellRecord;
 extern const char* kPrepTemplate;
+extern const char* kHexRecord;
+extern const char* kHexPrepTemplate;
+extern const char* kHexOwnerTemplate;
 
 std::string generate_form_source(const mf_form& f) {
   std::ostringstream os;
@@ -288,7 +314,22 @@ std::string generate_form_source(const mf_form& f) {
       os << subst(subst(kMatrix, {{"$GATHER", gather}}), m);
       m["$KNAME"] = kernel_name("lift", I->type, I->index);
       os << subst(subst(kLift, {{"$GATHER", gather}}), m);
-      if (I->type == MF_CELL && !I->staged_src.empty()) {
+      if (I->type == MF_CELL && is_hex_staged(*I)) {
+        os << "\n" << I->staged_src << "\n";
+        if (hex_owner_supported(f, *I)) {
+          m["$TAG"] = "H" + std::to_string(I->index);
+          os << subst(kHexRecord, m);
+          m["$KNAME"] = kernel_name("hprep", I->type, I->index);
+          os << subst(kHexPrepTemplate, m);
+          for (int prec : {64, 32, 16}) {
+            m["$KNAME"] = kernel_name(("howner" + std::to_string(prec)).c_str(), I->type, I->index);
+            m["$HNB"] = std::to_string(hex_batch(f, *I, prec));
+            m["$OMAXP"] = std::to_string(f.opt("owner_pairs", 384));
+            os << "\n#define MF_HX_PREC " << prec << "\n" << subst(kHexOwnerTemplate, m)
+               << "\n#undef MF_HX_PREC\n";
+          }
+        }
+      } else if (I->type == MF_CELL && !I->staged_src.empty()) {
         std::string staged = I->staged_src;
         // namespace the staged symbols per integral
         std::string pfx = "I" + std::to_string(I->index) + "_";

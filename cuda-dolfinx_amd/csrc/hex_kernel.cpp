@@ -1,0 +1,310 @@
+// Matrix-core owner-computes kernel for kappa grad.grad - k^2 mass on Q_k
+// hexahedra with trilinear (non-affine) geometry (HIP template, hipRTC).
+//
+// Per cell and quadrature point q the operator is a 4 x 4 block
+//   D_q = [[kappa w_q |J_q| K_q K_q^T, 0], [0, -k^2 w_q |J_q|]]
+// so A_ij = sum_{q, a, b} B_a(q, i) D_q[a][b] B_b(q, j), B_a the reference
+// gradient components (a = x, y, z) and values (a = 0).  Written as
+//   A[(cell, i), j] = sum_k F[(cell, i), k] * Bt[k, j],  k = (q, b),
+//   F[(cell, i), (q, b)] = sum_a B_a(q, i) D_q[a][b],    Bt[(q, b), j] = B_b(q, j),
+// the right factor Bt is the same for every cell, so one 16-row MFMA tile can
+// hold rows of different cells: the workgroup packs all (cell, owned row)
+// pairs of its CSR row chunk densely into tiles (no padding), contracts them
+// against Bt with v_mfma_*_16x16x* (K loop over the quadrature points) and
+// accumulates the 16 x (k+1)^3 result rows in the LDS copy of the chunk's CSR
+// values (FP64, ds_add_f64), which is written out once.
+//
+// Precision variants (compile-time MF_HX_PREC): 64 = FP64 operands and
+// accumulation (v_mfma_f64_16x16x4f64), 32 = FP32 operands and accumulation
+// (v_mfma_f32_16x16x4f32), 16 = BF16 operands with FP32 accumulation
+// (v_mfma_f32_16x16x16bf16_1k).  Geometry (J, det, K), D_q, and the CSR
+// accumulation are FP64 in all three.
+//
+// Operand layouts (16x16xK, lane l): A[row l%16][k-slot l/16], B[k-slot
+// l/16][col l%16], result rows 4*(l/16)+r, column l%16.  The K index is
+// (q, b) with b fastest, so one K-slot block covers one quadrature point.
+#include "mf_internal.h"
+
+namespace mf {
+
+const char* kHexRecord = R"(
+#ifndef MF_HREC_$TAG
+#define MF_HREC_$TAG
+struct __attribute__((aligned(16))) mf_hrec_$TAG {
+  double x[8][3];
+  int32_t d[HX_ND];
+  uint32_t bits[(HX_ND + 31) / 32];
+};
+#endif
+)";
+
+const char* kHexPrepTemplate = R"(
+extern "C" __global__ void __launch_bounds__(256)
+$KNAME(const int32_t n, const int32_t* __restrict__ ents, const int32_t* __restrict__ geo,
+       const double* __restrict__ xg, const int32_t* __restrict__ dm0,
+       const uint8_t* __restrict__ nb0, mf_hrec_$TAG* __restrict__ rec)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int64_t c = ents[e];
+  mf_hrec_$TAG r;
+  for (int v = 0; v < 8; ++v) {
+    const int64_t gv = geo[c * 8 + v];
+    r.x[v][0] = xg[3 * gv]; r.x[v][1] = xg[3 * gv + 1]; r.x[v][2] = xg[3 * gv + 2];
+  }
+  for (int w = 0; w < (HX_ND + 31) / 32; ++w) r.bits[w] = 0;
+  for (int l = 0; l < HX_ND; ++l) {
+    const int32_t node = dm0[c * HX_ND + l];
+    r.d[l] = node;
+    if (nb0 && nb0[node]) r.bits[l >> 5] |= 1u << (l & 31);
+  }
+  rec[c] = r;
+}
+)";
+
+const char* kHexOwnerTemplate = R"(
+#if MF_HX_PREC == 16
+typedef short mf_bf4 __attribute__((ext_vector_type(4)));
+static __device__ __forceinline__ short mf_to_bf16(float f)
+{
+  // round to nearest even
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+#endif
+
+extern "C" __global__ void __launch_bounds__(256)
+$KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
+       const int32_t* __restrict__ item_cell, const int32_t* __restrict__ pair_list,
+       const mf_hrec_$TAG* __restrict__ rec, const int32_t* __restrict__ rpb,
+       const int32_t* __restrict__ colsb, const uint8_t* __restrict__ nb0,
+       double* __restrict__ values, const double diag, const int flags)
+{
+  constexpr int K1 = HX_K + 1, ND = HX_ND, M1 = HX_NQ1, NQ = M1 * M1 * M1;
+  constexpr int NB = $HNB, CAP = $OCAP, MAXR = $OMAXR, MAXP = $OMAXP;
+  constexpr int RW = sizeof(mf_hrec_$TAG) / 16;
+  constexpr int DS = 8;                              // D_q record: G00 G01 G02 G11 G12 G22 M pad
+#if MF_HX_PREC == 64
+  typedef double real;
+#else
+  typedef float real;
+#endif
+  __shared__ __attribute__((aligned(16))) double acc[CAP];
+  __shared__ int32_t lrp[MAXR + 1];
+  __shared__ int32_t lcols[CAP + 1];
+  __shared__ mf_hrec_$TAG srec[NB];
+  __shared__ real sD[NB * NQ * DS];
+  __shared__ int32_t scell[NB];
+  __shared__ int32_t spair[MAXP];
+
+  const int k = mf_xcd_chunk(nchunks);
+  if (k < 0) return;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
+  const int4 m0 = reinterpret_cast<const int4*>(chunk_meta)[3 * k];
+  const int4 m1 = reinterpret_cast<const int4*>(chunk_meta)[3 * k + 1];
+  const int4 m2 = reinterpret_cast<const int4*>(chunk_meta)[3 * k + 2];
+  const int32_t r0 = m0.x, r1 = m0.y, i0 = m0.z, i1 = m0.w;
+  const int32_t cb0 = m1.x, ncb = m1.y, base = m1.z, seglen = m1.w;
+  const int32_t p0 = m2.x, p1 = m2.y;
+  const int nr = r1 - r0, npc = p1 - p0;
+  for (int p = tid; p < npc; p += nt) spair[p] = pair_list[p0 + p];
+  {
+    mf_d2* a2 = reinterpret_cast<mf_d2*>(acc);
+    const mf_d2 z = {0.0, 0.0};
+    for (int p = tid; p < (seglen + 1) / 2; p += nt) a2[p] = z;
+  }
+  for (int p = tid; p <= nr; p += nt) lrp[p] = rpb[r0 + p] - cb0;
+  for (int p = tid; p < ncb; p += nt) lcols[p] = colsb[cb0 + p];
+
+  int pb = 0;
+  for (int ib = i0; ib < i1; ib += NB) {
+    const int nb = min(NB, i1 - ib);
+    __syncthreads();
+    for (int b = tid; b < nb; b += nt) scell[b] = item_cell[ib + b];
+    __syncthreads();
+    {
+      const int4* src = reinterpret_cast<const int4*>(rec);
+      int4* dst = reinterpret_cast<int4*>(srec);
+      for (int u = tid; u < nb * RW; u += nt) {
+        const int b = u / RW, w = u - b * RW;
+        dst[b * RW + w] = src[(int64_t)scell[b] * RW + w];
+      }
+    }
+    int pe;
+    {
+      int lo2 = pb, hi2 = npc;
+      const int lim = ib - i0 + nb;
+      while (lo2 < hi2) {
+        const int mid = (lo2 + hi2) >> 1;
+        if ((spair[mid] >> 8) < lim) lo2 = mid + 1;
+        else hi2 = mid;
+      }
+      pe = lo2;
+    }
+    __syncthreads();
+    // geometry: D_q for every (cell, quadrature point), FP64
+    for (int u = tid; u < nb * NQ; u += nt) {
+      const int b = u / NQ, q = u - b * NQ;
+      const int q1 = q % M1, q2 = (q / M1) % M1, q3 = q / (M1 * M1);
+      const double X = HX_X1[q1], Y = HX_X1[q2], Z = HX_X1[q3];
+      double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int v = 0; v < 8; ++v) {
+        const int a = v & 1, bb = (v >> 1) & 1, c = v >> 2;
+        const double fx = a ? X : 1 - X, fy = bb ? Y : 1 - Y, fz = c ? Z : 1 - Z;
+        const double gx = a ? 1.0 : -1.0, gy = bb ? 1.0 : -1.0, gz = c ? 1.0 : -1.0;
+        const double d0 = gx * fy * fz, d1 = fx * gy * fz, d2 = fx * fy * gz;
+        for (int i = 0; i < 3; ++i) {
+          const double xv = srec[b].x[v][i];
+          J[i][0] += xv * d0; J[i][1] += xv * d1; J[i][2] += xv * d2;
+        }
+      }
+      const double m00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+      const double m01 = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+      const double m02 = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+      const double det = J[0][0] * m00 + J[1][0] * m01 + J[2][0] * m02;
+      const double id = 1.0 / det;
+      double K[3][3];
+      K[0][0] = m00 * id; K[0][1] = m01 * id; K[0][2] = m02 * id;
+      K[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+      K[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+      K[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+      K[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+      K[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+      K[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+      const double wq = HX_W1[q1] * HX_W1[q2] * HX_W1[q3] * fabs(det);
+      const double s = HX_KAPPA * wq;
+      real* D = &sD[(b * NQ + q) * DS];
+      D[0] = (real)(s * (K[0][0] * K[0][0] + K[0][1] * K[0][1] + K[0][2] * K[0][2]));
+      D[1] = (real)(s * (K[0][0] * K[1][0] + K[0][1] * K[1][1] + K[0][2] * K[1][2]));
+      D[2] = (real)(s * (K[0][0] * K[2][0] + K[0][1] * K[2][1] + K[0][2] * K[2][2]));
+      D[3] = (real)(s * (K[1][0] * K[1][0] + K[1][1] * K[1][1] + K[1][2] * K[1][2]));
+      D[4] = (real)(s * (K[1][0] * K[2][0] + K[1][1] * K[2][1] + K[1][2] * K[2][2]));
+      D[5] = (real)(s * (K[2][0] * K[2][0] + K[2][1] * K[2][1] + K[2][2] * K[2][2]));
+      D[6] = (real)(-HX_K2 * wq);
+      D[7] = (real)0;
+    }
+    __syncthreads();
+    // contraction: one wave per 16-pair tile, all 64 columns (4 tiles of 16)
+    const int np = pe - pb;
+    const int ntile = (np + 15) / 16;
+    const int r = lane & 15, slot = lane >> 4;      // A row / K slot of this lane
+    for (int t = wave; t < ntile; t += nwave) {
+      const int pr = t * 16 + r;
+      const bool valid = pr < np;
+      const int pc = pb + (valid ? pr : 0);
+      const int b = (spair[pc] >> 8) - (ib - i0), i = spair[pc] & 255;
+      const int ia = i % K1, ib2 = (i / K1) % K1, ic = i / (K1 * K1);
+      const real* Db = &sD[b * NQ * DS];
+      // column factors of this lane: j = 16 * tile + (lane % 16), tile = jc
+      const int ja = (lane & 15) % K1, jb = (lane & 15) / K1;
+#if MF_HX_PREC == 64
+      typedef double acc4 __attribute__((ext_vector_type(4)));
+#else
+      typedef float acc4 __attribute__((ext_vector_type(4)));
+#endif
+      acc4 C[4];
+      for (int jt = 0; jt < 4; ++jt) C[jt] = (acc4){0, 0, 0, 0};
+#if MF_HX_PREC == 16
+      // K slot block of 16 = 4 quadrature points; lane holds points q0 + slot, b = 0..3
+      for (int qb = 0; qb < NQ; qb += 4) {
+        const int q = qb + slot;
+        const int q1 = q % M1, q2 = (q / M1) % M1, q3 = q / (M1 * M1);
+        const real* D = &Db[q * DS];
+        const double Lx = HX_L[q1][ia], Ly = HX_L[q2][ib2], Lz = HX_L[q3][ic];
+        const double Gx = HX_DL[q1][ia], Gy = HX_DL[q2][ib2], Gz = HX_DL[q3][ic];
+        const double B0 = Lx * Ly * Lz, Bx = Gx * Ly * Lz, By = Lx * Gy * Lz, Bz = Lx * Ly * Gz;
+        float f[4];
+        f[0] = (float)(Bx * D[0] + By * D[1] + Bz * D[2]);
+        f[1] = (float)(Bx * D[1] + By * D[3] + Bz * D[4]);
+        f[2] = (float)(Bx * D[2] + By * D[4] + Bz * D[5]);
+        f[3] = (float)(B0 * D[6]);
+        mf_bf4 av;
+        for (int h = 0; h < 4; ++h) av[h] = valid ? mf_to_bf16(f[h]) : (short)0;
+        const double cLx = HX_L[q1][ja], cLy = HX_L[q2][jb];
+        const double cGx = HX_DL[q1][ja], cGy = HX_DL[q2][jb];
+        for (int jt = 0; jt < 4; ++jt) {
+          const double cLz = HX_L[q3][jt], cGz = HX_DL[q3][jt];
+          mf_bf4 bv;
+          bv[0] = mf_to_bf16((float)(cGx * cLy * cLz));
+          bv[1] = mf_to_bf16((float)(cLx * cGy * cLz));
+          bv[2] = mf_to_bf16((float)(cLx * cLy * cGz));
+          bv[3] = mf_to_bf16((float)(cLx * cLy * cLz));
+          C[jt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, C[jt], 0, 0, 0);
+        }
+      }
+#else
+      for (int q = 0; q < NQ; ++q) {
+        const int q1 = q % M1, q2 = (q / M1) % M1, q3 = q / (M1 * M1);
+        const real* D = &Db[q * DS];
+        const double Lx = HX_L[q1][ia], Ly = HX_L[q2][ib2], Lz = HX_L[q3][ic];
+        const double Gx = HX_DL[q1][ia], Gy = HX_DL[q2][ib2], Gz = HX_DL[q3][ic];
+        // this lane's K slot is component b = slot of point q
+        real fa;
+        if (slot == 3) {
+          fa = (real)(Lx * Ly * Lz) * D[6];
+        } else {
+          const real bx = (real)(Gx * Ly * Lz), by = (real)(Lx * Gy * Lz), bz = (real)(Lx * Ly * Gz);
+          const int c0 = slot == 0 ? 0 : (slot == 1 ? 1 : 2);
+          const int c1 = slot == 0 ? 1 : (slot == 1 ? 3 : 4);
+          const int c2 = slot == 0 ? 2 : (slot == 1 ? 4 : 5);
+          fa = bx * D[c0] + by * D[c1] + bz * D[c2];
+        }
+        if (!valid) fa = (real)0;
+        // B operand: component `slot` of column basis function j at point q
+        const double cLx = HX_L[q1][ja], cLy = HX_L[q2][jb];
+        const double cGx = HX_DL[q1][ja], cGy = HX_DL[q2][jb];
+        const double fxy = slot == 0 ? cGx * cLy : (slot == 1 ? cLx * cGy : cLx * cLy);
+        for (int jt = 0; jt < 4; ++jt) {
+          const double fz = slot == 2 ? HX_DL[q3][jt] : HX_L[q3][jt];
+          const real bv = (real)(fxy * fz);
+#if MF_HX_PREC == 64
+          C[jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa, bv, C[jt], 0, 0, 0);
+#else
+          C[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa, bv, C[jt], 0, 0, 0);
+#endif
+        }
+      }
+#endif
+      // scatter the 16 x 64 tile into the LDS segment
+      for (int rr = 0; rr < 4; ++rr) {
+        const int prr = t * 16 + slot * 4 + rr;
+        if (prr >= np) continue;
+        const int pcc = pb + prr;
+        const int bq = (spair[pcc] >> 8) - (ib - i0), iq = spair[pcc] & 255;
+        const mf_hrec_$TAG& R = srec[bq];
+        if ((R.bits[iq >> 5] >> (iq & 31)) & 1u) continue;            // Dirichlet row
+        const int lr = R.d[iq] - r0;
+        const int lo = lrp[lr], hi = lrp[lr + 1];
+        for (int jt = 0; jt < 4; ++jt) {
+          const int j = jt * 16 + (lane & 15);
+          if ((R.bits[j >> 5] >> (j & 31)) & 1u) continue;             // Dirichlet column
+          const int kk = mf_lds_lower_bound(lcols, lo, hi, R.d[j]);
+          atomicAdd(&acc[kk], (double)C[jt][rr]);
+        }
+      }
+    }
+    pb = pe;
+  }
+  __syncthreads();
+  if ((flags & 2) && nb0) {
+    for (int lr = tid; lr < nr; lr += nt) {
+      const int32_t Rr = r0 + lr;
+      if (!nb0[Rr]) continue;
+      const int lo = lrp[lr], hi = lrp[lr + 1];
+      const int p = mf_lds_lower_bound(lcols, lo, hi, Rr);
+      if (p < hi && lcols[p] == Rr) acc[p] = diag;
+    }
+    __syncthreads();
+  }
+  double* out = values + (int64_t)(uint32_t)base;
+  if (flags & 1) {
+    for (int p = tid; p < seglen; p += nt) __builtin_nontemporal_store(acc[p], &out[p]);
+  } else {
+    for (int p = tid; p < seglen; p += nt) out[p] += acc[p];
+  }
+}
+)";
+
+}  // namespace mf

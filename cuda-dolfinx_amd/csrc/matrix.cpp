@@ -17,7 +17,7 @@ struct OwnerPlan {
   int64_t nitems = 0, npairs = 0;
   // host copies used while building matrix schedules
   std::vector<int32_t> h_items, h_cell;
-  std::vector<uint32_t> h_mask;
+  std::vector<uint64_t> h_mask;
 };
 
 namespace {
@@ -29,6 +29,7 @@ constexpr int kTableLanes = 64;
 std::shared_ptr<OwnerPlan> build_items(const std::vector<int32_t>& rows, const Integral& I,
                                        const mf_dofmap* d0) {
   const int nd = d0->nd;
+  MF_CHECK(nd <= 64, MF_EUNSUPPORTED, "owner schedules need at most 64 dofs per cell");
   const int64_t nch = static_cast<int64_t>(rows.size()) - 1;
   std::vector<int32_t> row_chunk(d0->num_nodes);
   for (int64_t c = 0; c < nch; ++c)
@@ -52,15 +53,15 @@ std::shared_ptr<OwnerPlan> build_items(const std::vector<int32_t>& rows, const I
   for (int64_t c = 0; c < nch; ++c) cnt[c + 1] += cnt[c];
   MF_CHECK(cnt[nch] < INT32_MAX, MF_EUNSUPPORTED, "too many owner items");
   std::vector<int32_t> icell(std::max<int64_t>(1, cnt[nch]));
-  std::vector<uint32_t> imask(std::max<int64_t>(1, cnt[nch]));
+  std::vector<uint64_t> imask(std::max<int64_t>(1, cnt[nch]));
   std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
   for (int64_t e = 0; e < I.n; ++e) {
     const int32_t c = I.h_ents[e];
     int n = cell_chunks(c, tmp.data());
     for (int q = 0; q < n; ++q) {
-      uint32_t msk = 0;
+      uint64_t msk = 0;
       for (int l = 0; l < nd; ++l)
-        if (row_chunk[d0->host[(int64_t)c * nd + l]] == tmp[q]) msk |= 1u << l;
+        if (row_chunk[d0->host[(int64_t)c * nd + l]] == tmp[q]) msk |= 1ull << l;
       int64_t p = pos[tmp[q]]++;
       icell[p] = c;
       imask[p] = msk;
@@ -74,7 +75,11 @@ std::shared_ptr<OwnerPlan> build_items(const std::vector<int32_t>& rows, const I
   for (int64_t c = 0; c <= nch; ++c) ci[c] = static_cast<int32_t>(cnt[c]);
   plan->chunk_items.upload(ci);
   plan->item_cell.upload(icell);
-  plan->item_mask.upload(imask);
+  if (nd <= 32) {
+    std::vector<uint32_t> m32(imask.size());
+    for (size_t q = 0; q < imask.size(); ++q) m32[q] = static_cast<uint32_t>(imask[q]);
+    plan->item_mask.upload(m32);
+  }
   plan->h_items = std::move(ci);
   plan->h_cell = std::move(icell);
   plan->h_mask = std::move(imask);
@@ -86,11 +91,11 @@ std::shared_ptr<OwnerPlan> build_items(const std::vector<int32_t>& rows, const I
 // the per-chunk pair lists and, for every pair and column j of the cell, the
 // position of the column node in the row's blocked column list.
 std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I, const mf_matrix& A,
-                                            int64_t cap, int maxr, int maxp) {
+                                            int64_t cap, int maxr, int maxp, bool with_kk) {
   const mf_dofmap* d0 = a.dm[0];
   const mf_dofmap* d1 = a.dm[1];
   const int nd = d0->nd;
-  if (nd > 32) return nullptr;
+  if (nd > (with_kk ? 32 : 64)) return nullptr;
   const int bs2 = A.bs0 * A.bs1;
   std::vector<int32_t> ncell(A.nrows_b, 0);
   for (int64_t e = 0; e < I.n; ++e)
@@ -118,12 +123,12 @@ std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I,
   std::vector<int64_t> poff(nch + 1, 0);
   for (int64_t c = 0; c < nch; ++c) {
     int64_t n = 0;
-    for (int32_t it = ci[c]; it < ci[c + 1]; ++it) n += __builtin_popcount(plan->h_mask[it]);
+    for (int32_t it = ci[c]; it < ci[c + 1]; ++it) n += __builtin_popcountll(plan->h_mask[it]);
     poff[c + 1] = poff[c] + n;
   }
   const int64_t np = poff[nch];
   std::vector<int32_t> plist(std::max<int64_t>(1, np));
-  std::vector<uint8_t> pkk(std::max<int64_t>(1, np * nd));
+  std::vector<uint8_t> pkk(with_kk ? std::max<int64_t>(1, np * nd) : 1);
   std::vector<int> bad(16, 0);
   unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   auto work = [&](unsigned t) {
@@ -131,11 +136,15 @@ std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I,
       int64_t p = poff[c];
       for (int32_t it = ci[c]; it < ci[c + 1]; ++it) {
         const int32_t cell = plan->h_cell[it];
-        uint32_t msk = plan->h_mask[it];
+        uint64_t msk = plan->h_mask[it];
         while (msk) {
-          const int i = __builtin_ctz(msk);
+          const int i = __builtin_ctzll(msk);
           msk &= msk - 1;
           plist[p] = ((it - ci[c]) << 8) | i;
+          if (!with_kk) {
+            ++p;
+            continue;
+          }
           const int32_t R = d0->host[(int64_t)cell * nd + i];
           const int32_t* rb = A.h_colsb.data() + A.h_rpb[R];
           const int32_t* re = A.h_colsb.data() + A.h_rpb[R + 1];
@@ -242,7 +251,7 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
   Integral* owner = nullptr;
   if (mode != 1)
     for (auto& I : a->ints)
-      if (I->type == MF_CELL && I->k_owner) {
+      if (I->type == MF_CELL && (I->k_owner || I->k_hprep)) {
         owner = I.get();
         break;
       }
@@ -251,9 +260,12 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
     auto key = std::make_pair(a->uid, owner->index);
     auto it = A->plans.find(key);
     if (it == A->plans.end()) {
+      const bool hex = owner->k_hprep != nullptr;
       plan = build_owner_plan(*a, *owner, *A, a->opt("chunk_values", 4096),
                               static_cast<int>(a->opt("chunk_rows", 64)),
-                              owner_shape(*a, *owner).maxp);
+                              hex ? static_cast<int>(a->opt("owner_pairs", 384))
+                                  : owner_shape(*a, *owner).maxp,
+                              !hex);
       A->plans[key] = plan;
     } else {
       plan = it->second;
@@ -272,15 +284,28 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
     int flags = zero ? 1 : 0;
     if (zero && m0) flags |= 2;
     diag_done = (flags & 2) != 0;
+    const int64_t nx = 8, per = (plan->nchunks + nx - 1) / nx;
+    if (owner->k_hprep) {
+      const int64_t prec = a->opt("precision", 64);
+      const int v = prec == 64 ? 0 : (prec == 32 ? 1 : 2);
+      MF_CHECK(prec == 64 || prec == 32 || prec == 16, MF_EINVAL, "precision must be 64, 32 or 16");
+      launch(owner->k_hprep, static_cast<uint32_t>((owner->n + 255) / 256), 256, 0, s,
+             static_cast<int32_t>(owner->n), owner->d_ents.ptr, a->mesh->cells.ptr,
+             a->mesh->x.ptr, a->dm[0]->dev.ptr, n0, owner->records.ptr);
+      launch(owner->k_howner[v], static_cast<uint32_t>(per * nx), 256, 0, s,
+             static_cast<int32_t>(plan->nchunks), plan->chunk_meta.ptr, plan->item_cell.ptr,
+             plan->pair_list.ptr, owner->records.ptr, A->rpb.ptr, A->colsb.ptr, n0, A->values,
+             diagonal, flags);
+    } else {
     launch(owner->k_prep, static_cast<uint32_t>((owner->n + 255) / 256), 256, 0, s,
            static_cast<int32_t>(owner->n), owner->d_ents.ptr, a->mesh->cells.ptr, a->mesh->x.ptr,
            a->packed.ptr, a->dm[0]->dev.ptr, n0, owner->records.ptr);
-    const int64_t nx = 8, per = (plan->nchunks + nx - 1) / nx;
     launch(owner->k_owner, static_cast<uint32_t>(per * nx),
            static_cast<uint32_t>(a->opt("owner_block", 256)), 0, s,
            static_cast<int32_t>(plan->nchunks), plan->chunk_meta.ptr, plan->item_cell.ptr,
            plan->pair_list.ptr, plan->pair_kk.ptr, owner->records.ptr, a->consts.ptr, A->rpb.ptr,
            A->colsb.ptr, n0, A->values, diagonal, flags);
+    }
   } else if (zero) {
     k_fill(A->values, A->nnz, 0.0, s);
   }

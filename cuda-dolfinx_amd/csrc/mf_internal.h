@@ -177,6 +177,7 @@ struct Integral {
   int index = 0;                      // position in the form
   hipFunction_t k_mat = nullptr, k_vec = nullptr, k_lift = nullptr, k_owner = nullptr;
   hipFunction_t k_vown = nullptr, k_slift = nullptr, k_prep = nullptr;
+  hipFunction_t k_hprep = nullptr, k_howner[3] = {nullptr, nullptr, nullptr};  // FP64, FP32, BF16
   DevArray<double> records;           // per-cell records of the owner-computes kernel
   int64_t record_bytes = 0;
   // lifting entity lists per bc object uid
@@ -233,6 +234,9 @@ namespace mf {
 // code generation (codegen.cpp)
 std::string generate_form_source(const mf_form& f);
 bool owner_supported(const mf_form& f, const Integral& I);
+bool is_hex_staged(const Integral& I);
+bool hex_owner_supported(const mf_form& f, const Integral& I);
+int hex_batch(const mf_form& f, const Integral& I, int prec);
 struct OwnerShape { int nb = 0, maxp = 0; };
 OwnerShape owner_shape(const mf_form& f, const Integral& I);
 int64_t record_bytes(const mf_form& f, const Integral& I);

@@ -60,9 +60,18 @@ class FunctionSpace:
         lat = mesh.lattice[mesh.cells]                          # (Nc, nv, 3)
         if mesh.cell_type == "tetrahedron":
             keys = np.einsum("dv,cvx->cdx", idx, lat)           # (Nc, nd, 3), refined by k
+            code = morton3(keys[..., 0], keys[..., 1], keys[..., 2])
         else:
+            # hexahedra: nodes grouped by owning cube (cube Morton order), each
+            # cube owning its k x k x k block of the refined lattice (the last
+            # layer of cubes also owns the upper boundary nodes)
             keys = k * lat[:, 0, None, :] + idx[None, :, :]
-        code = morton3(keys[..., 0], keys[..., 1], keys[..., 2])
+            cube = np.minimum(keys // k, mesh.n - 1)
+            loc = keys - k * cube
+            k1 = np.uint64(k + 1)
+            code = (morton3(cube[..., 0], cube[..., 1], cube[..., 2]) * k1 ** np.uint64(3)
+                    + (loc[..., 2].astype(np.uint64) * k1 + loc[..., 1].astype(np.uint64)) * k1
+                    + loc[..., 0].astype(np.uint64))
         uniq, inv = np.unique(code.reshape(-1), return_inverse=True)
         self._dofmap = inv.reshape(nc, nd).astype(np.int32)
         self._num_nodes = int(uniq.shape[0])

@@ -76,6 +76,37 @@ class HexContraction:
     kappa: float
 
 
+@dataclass
+class HexStaged:
+    """Staged description of kappa grad.grad - k^2 mass on Q_k hexahedra.
+
+    Consumed by the matrix-core owner kernel: 1D GLL Lagrange values and
+    derivatives at the 1D Gauss points (the Q_k tables are their tensor
+    products, node index a + (k+1) b + (k+1)^2 c, point index q1 + m q2 + m^2 q3),
+    1D Gauss points and weights, k^2 and kappa.
+    """
+    degree: int
+    m: int
+    k2: float
+    kappa: float
+
+    def source(self) -> str:
+        k, m = self.degree, self.m
+        x, w = E._gauss_legendre01(m)
+        el1 = E.gll_points(k)
+        C1 = np.linalg.inv(np.vander(el1, k + 1, increasing=True))
+        V = np.vander(x, k + 1, increasing=True) @ C1               # (m, k+1)
+        D = np.zeros((m, k + 1))
+        for j in range(1, k + 1):
+            D += j * np.outer(x ** (j - 1), C1[j])
+        return (f"#define HX_K {k}\n#define HX_NQ1 {m}\n#define HX_ND {(k + 1) ** 3}\n"
+                f"#define HX_K2 {self.k2!r}\n#define HX_KAPPA {self.kappa!r}\n"
+                + _carr("HX_L", V, "__constant__ double")
+                + _carr("HX_DL", D, "__constant__ double")
+                + _carr("HX_X1", x, "__constant__ double")
+                + _carr("HX_W1", w, "__constant__ double"))
+
+
 def helmholtz(V: FunctionSpace, k: float, kappa=None, qpoints_per_dir: Optional[int] = None) -> Form:
     """kappa grad u . grad v - k^2 u v on a scalar Q_k hexahedral space."""
     if V.mesh.cell_type != "hexahedron":
@@ -122,7 +153,10 @@ def helmholtz(V: FunctionSpace, k: float, kappa=None, qpoints_per_dir: Optional[
     cells = np.arange(V.mesh.num_cells, dtype=np.int32)
     spec = dict(kind="stiffness", cell=1, degree=el.degree, bs=1, rank=2, qm=m,
                 kappa=_src_of(kexpr), k2=k2)
-    form = Form(2, [V, V], [Integral(CELL, -1, cells, name, src, None, spec)], coeffs, consts,
+    staged = None
+    if not coeffs and not consts:
+        staged = HexStaged(el.degree, m, k2, float(kappa) if isinstance(kappa, (int, float)) else 1.0)
+    form = Form(2, [V, V], [Integral(CELL, -1, cells, name, src, staged, spec)], coeffs, consts,
                 "helmholtz")
     form.hex_contraction = HexContraction(nd, nq, q.weights, phi, dphi, dpsi, k2,
                                           float(kappa) if isinstance(kappa, (int, float)) else 1.0)

@@ -191,3 +191,16 @@ def test_vector_and_lifting_kernels(vk, lk):
     _compare(P, _gpu(P, {"lift_kernel": lk}, [bc], scale=0.8, loptions={"vector_kernel": vk}), ref)
     Q = problems.poisson_p1(7)
     _compare(Q, _gpu(Q, {"lift_kernel": lk}, loptions={"vector_kernel": vk}), _oracle(Q))
+
+
+@pytest.mark.parametrize("prec,tol", [(64, 1e-12), (32, 1e-6), (16, 2e-2)])
+def test_helmholtz_q3_matrix_core(prec, tol):
+    _say(f"helmholtz Q3 n=4 jittered, matrix-core owner kernel precision={prec}")
+    P = problems.helmholtz_q3(4)
+    g = _gpu(P, {"matrix_kernel": 2, "precision": prec})
+    ref = _oracle(P)
+    rp, cols, vals, b = g[:4]
+    assert np.array_equal(rp, ref[0]) and np.array_equal(cols, ref[1])
+    ea, eb = _rel(vals, ref[2]), _rel(b, ref[3])
+    _say(f"  precision {prec}: relerr A={ea:.3e} b={eb:.3e}")
+    assert ea <= tol and eb <= 1e-12, (ea, eb)

@@ -1,0 +1,198 @@
+"""Discontinuous Galerkin forms: symmetric interior penalty (SIPG) discretisation
+of a p-Laplacian with a Robin boundary, on affine tetrahedra.
+
+Residual F(u; v) and its Jacobian J(u; du, v) for a DG_k state u:
+
+  F = sum_K int_K mu(grad u) grad u . grad v - f v dx
+    + sum_{F interior} int_F -{grad u}.n [v] - {grad v}.n [u] + (sigma / h_F) [u][v] ds
+    + sum_{F boundary} int_F alpha u v ds
+  J = sum_K int_K mu grad du . grad v + mu' (grad u . grad du)(grad u . grad v) dx
+    + (the facet terms with du in place of u)
+
+with mu = (eps^2 + |grad u|^2)^((p-2)/2), mu' = (p-2)(eps^2 + |grad u|^2)^((p-4)/2),
+n the unit normal pointing out of the first cell of the facet, [w] = w+ - w-,
+{q} = (q+ + q-)/2, sigma = penalty (10 k^2 by default), h_F the mean of the
+two cells' diameters (largest vertex distance).  The facet terms are those
+of the linear Laplacian, so J is exactly the derivative of F.
+
+Interior-facet tensors follow the UFCx layout the reference assembles
+(cpp/cudolfinx/fem/CUDAFormIntegral.cpp:1988-2103): rows and columns
+[cell0 dofs, cell1 dofs], coefficients interlaced (w_cell0, w_cell1)
+(:1855-1874), coordinate_dofs [2][4][3], entity_local_index {lf0, lf1}.
+Facet quadrature points of the two cells coincide because tetrahedron
+vertices are stored in ascending global order (no quadrature permutation).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import element as E
+from .fem import Constant, Function, FunctionSpace
+from .forms import (CELL, EXTERIOR_FACET, INTERIOR_FACET, Form, Integral, _SIG, _carr, _uid)
+
+
+def _geom(s: str, xd: str) -> str:
+    """Affine map of one tetrahedron: K = J^-1 entries K{r}{c}{s}, |det J| adet{s}, diameter h{s}."""
+    return f"""
+  const double J00{s} = {xd}[3] - {xd}[0], J01{s} = {xd}[6] - {xd}[0], J02{s} = {xd}[9] - {xd}[0];
+  const double J10{s} = {xd}[4] - {xd}[1], J11{s} = {xd}[7] - {xd}[1], J12{s} = {xd}[10] - {xd}[1];
+  const double J20{s} = {xd}[5] - {xd}[2], J21{s} = {xd}[8] - {xd}[2], J22{s} = {xd}[11] - {xd}[2];
+  const double m00{s} = J11{s} * J22{s} - J12{s} * J21{s};
+  const double m01{s} = J02{s} * J21{s} - J01{s} * J22{s};
+  const double m02{s} = J01{s} * J12{s} - J02{s} * J11{s};
+  const double det{s} = J00{s} * m00{s} + J10{s} * m01{s} + J20{s} * m02{s};
+  const double id{s} = 1.0 / det{s};
+  const double K{s}[3][3] = {{{{m00{s} * id{s}, m01{s} * id{s}, m02{s} * id{s}}},
+    {{(J12{s} * J20{s} - J10{s} * J22{s}) * id{s}, (J00{s} * J22{s} - J02{s} * J20{s}) * id{s}, (J02{s} * J10{s} - J00{s} * J12{s}) * id{s}}},
+    {{(J10{s} * J21{s} - J11{s} * J20{s}) * id{s}, (J01{s} * J20{s} - J00{s} * J21{s}) * id{s}, (J00{s} * J11{s} - J01{s} * J10{s}) * id{s}}}}};
+  const double adet{s} = fabs(det{s});
+  double h{s} = 0.0;
+  for (int va = 0; va < 4; ++va)
+    for (int vb = va + 1; vb < 4; ++vb) {{
+      const double dx = {xd}[3 * va] - {xd}[3 * vb], dy = {xd}[3 * va + 1] - {xd}[3 * vb + 1],
+                   dz = {xd}[3 * va + 2] - {xd}[3 * vb + 2];
+      const double dd = sqrt(dx * dx + dy * dy + dz * dz);
+      h{s} = dd > h{s} ? dd : h{s};
+    }}
+"""
+
+
+_FACET = """
+  static const int FV[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+  const int lf0 = entity_local_index[0];
+  const double* xa = coordinate_dofs;
+  const int a0 = FV[lf0][0], a1 = FV[lf0][1], a2 = FV[lf0][2];
+  const double e1x = xa[3 * a1] - xa[3 * a0], e1y = xa[3 * a1 + 1] - xa[3 * a0 + 1], e1z = xa[3 * a1 + 2] - xa[3 * a0 + 2];
+  const double e2x = xa[3 * a2] - xa[3 * a0], e2y = xa[3 * a2 + 1] - xa[3 * a0 + 1], e2z = xa[3 * a2 + 2] - xa[3 * a0 + 2];
+  double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+  const double ds = sqrt(nx * nx + ny * ny + nz * nz);
+  nx /= ds; ny /= ds; nz /= ds;
+  {
+    /* outward from the first cell: away from the vertex opposite the facet */
+    const double ox = xa[3 * lf0] - xa[3 * a0], oy = xa[3 * lf0 + 1] - xa[3 * a0 + 1], oz = xa[3 * lf0 + 2] - xa[3 * a0 + 2];
+    if (nx * ox + ny * oy + nz * oz > 0.0) { nx = -nx; ny = -ny; nz = -nz; }
+  }
+"""
+
+
+def _grad(s: str, tab: str, nd: int) -> str:
+    """g{s}[i][d] = sum_r tab[i][r] K{s}[r][d] (physical gradients of the basis)."""
+    return f"""    double g{s}[{nd}][3];
+    for (int i = 0; i < {nd}; ++i)
+      for (int d = 0; d < 3; ++d)
+        g{s}[i][d] = {tab}[i][0] * K{s}[0][d] + {tab}[i][1] * K{s}[1][d] + {tab}[i][2] * K{s}[2][d];
+"""
+
+
+def sipg_plaplace(V: FunctionSpace, u: Function, f: float = 1.0, alpha: float = 2.0,
+                  penalty: float | None = None, p: float = 3.0, eps: float = 1e-3):
+    """(J, F): Jacobian and residual forms of the SIPG p-Laplacian (module doc)."""
+    mesh = V.mesh
+    el = V.element
+    if mesh.cell_type != "tetrahedron" or not el.discontinuous or V.bs != 1:
+        raise NotImplementedError("SIPG forms need a scalar DG space on tetrahedra")
+    if u.function_space is not V:
+        raise ValueError("the state u must live on V")
+    k, nd = el.degree, el.ndofs
+    if penalty is None:
+        penalty = 10.0 * k * k
+    mc = max(1, k)                       # cell rule: exact for degree 2k - 1 (grad.grad and f v)
+    mf = k + 1                           # facet rule: exact for [u][v] (degree 2k)
+    qc, qf = E.tet_collapsed(mc), E.tri_collapsed(mf)
+    tc = el.tabulate(qc.points)
+    phic = tc[0]
+    dphic = np.stack([tc[1], tc[2], tc[3]], axis=-1)                 # (nqc, nd, 3)
+    phif, dphif = [], []
+    for lf in range(4):
+        t = el.tabulate(E.tet_facet_points(lf, qf.points))
+        phif.append(t[0])
+        dphif.append(np.stack([t[1], t[2], t[3]], axis=-1))
+    phif, dphif = np.stack(phif), np.stack(dphif)                   # (4, nqf, nd[, 3])
+    nqc, nqf = qc.weights.size, qf.weights.size
+    e2 = repr(float(eps) ** 2)
+    e_mu, e_dmu = repr((p - 2.0) / 2.0), repr((p - 4.0) / 2.0)
+    common = dict(cell=0, degree=k, bs=1, p=float(p), eps=float(eps), alpha=float(alpha),
+                  penalty=float(penalty), fval=float(f), u_off=0)
+
+    def cell(rank):
+        name = f"tabulate_tensor_integral_{_uid('dg_cell', rank, k, p, eps, f)}"
+        out = (f"        A[i * {nd} + j] += fw * (mu * (g[i][0] * g[j][0] + g[i][1] * g[j][1] + g[i][2] * g[j][2])\n"
+               f"                               + dmu * gui[i] * gui[j]);\n" if rank == 2 else "")
+        loop = (f"    for (int i = 0; i < {nd}; ++i)\n      for (int j = 0; j < {nd}; ++j)\n{out}"
+                if rank == 2 else
+                f"    for (int i = 0; i < {nd}; ++i)\n"
+                f"      A[i] += fw * (mu * gui[i] - {float(f)!r} * FE_phi[iq][i]);\n")
+        body = (_carr("FE_w", qc.weights) + _carr("FE_phi", phic) + _carr("FE_dphi", dphic)
+                + _geom("", "coordinate_dofs") + f"""
+  (void)h;
+  for (int iq = 0; iq < {nqc}; ++iq) {{
+{_grad("", "FE_dphi[iq]", nd)}    double gu[3] = {{0.0, 0.0, 0.0}};
+    for (int i = 0; i < {nd}; ++i)
+      for (int d = 0; d < 3; ++d) gu[d] += w[i] * g[i][d];
+    double gui[{nd}];
+    for (int i = 0; i < {nd}; ++i) gui[i] = gu[0] * g[i][0] + gu[1] * g[i][1] + gu[2] * g[i][2];
+    const double s2 = {e2} + gu[0] * gu[0] + gu[1] * gu[1] + gu[2] * gu[2];
+    const double mu = pow(s2, {e_mu});
+    const double dmu = {float(p - 2.0)!r} * pow(s2, {e_dmu});
+    (void)dmu;
+    const double fw = FE_w[iq] * adet;
+{loop}  }}
+""")
+        src = _SIG.format(name=name) + "\n{\n" + body + "}\n"
+        spec = dict(kind="dg_cell", rank=rank, qm=mc, **common)
+        return Integral(CELL, -1, np.arange(mesh.num_cells, dtype=np.int32), name, src, None, spec)
+
+    def ext(rank):
+        name = f"tabulate_tensor_integral_{_uid('dg_ext', rank, k, alpha)}"
+        if rank == 2:
+            loop = (f"    for (int i = 0; i < {nd}; ++i)\n      for (int j = 0; j < {nd}; ++j)\n"
+                    f"        A[i * {nd} + j] += fw * ph[i] * ph[j];\n")
+        else:
+            loop = (f"    double uq = 0.0;\n    for (int j = 0; j < {nd}; ++j) uq += w[j] * ph[j];\n"
+                    f"    for (int i = 0; i < {nd}; ++i) A[i] += fw * uq * ph[i];\n")
+        body = (_carr("FE_w", qf.weights) + _carr("FE_phi", phif) + _FACET + f"""
+  for (int iq = 0; iq < {nqf}; ++iq) {{
+    const double* ph = FE_phi[lf0][iq];
+    const double fw = FE_w[iq] * ds * {float(alpha)!r};
+{loop}  }}
+""")
+        src = _SIG.format(name=name) + "\n{\n" + body + "}\n"
+        spec = dict(kind="dg_ext", rank=rank, qm=mf, **common)
+        ent = np.ascontiguousarray(mesh.exterior_facets.reshape(-1), dtype=np.int32)
+        return Integral(EXTERIOR_FACET, -1, ent, name, src, None, spec)
+
+    def inter(rank):
+        name = f"tabulate_tensor_integral_{_uid('dg_int', rank, k, penalty)}"
+        n2 = 2 * nd
+        if rank == 2:
+            loop = (f"    for (int I = 0; I < {n2}; ++I)\n      for (int J = 0; J < {n2}; ++J)\n"
+                    f"        A[I * {n2} + J] += fw * (-dn[J] * jmp[I] - dn[I] * jmp[J] + sig * jmp[I] * jmp[J]);\n")
+        else:
+            loop = (f"    double ju = 0.0, du = 0.0;\n"
+                    f"    for (int J = 0; J < {n2}; ++J) {{ ju += jmp[J] * w[J]; du += dn[J] * w[J]; }}\n"
+                    f"    for (int I = 0; I < {n2}; ++I)\n"
+                    f"      A[I] += fw * (-du * jmp[I] - dn[I] * ju + sig * ju * jmp[I]);\n")
+        body = (_carr("FE_w", qf.weights) + _carr("FE_phi", phif) + _carr("FE_dphi", dphif)
+                + _FACET + "  const int lf1 = entity_local_index[1];\n"
+                + _geom("0", "coordinate_dofs") + _geom("1", "(coordinate_dofs + 12)") + f"""
+  (void)adet0; (void)adet1;
+  const double sig = {float(penalty)!r} / (0.5 * (h0 + h1));
+  for (int iq = 0; iq < {nqf}; ++iq) {{
+{_grad("0", "FE_dphi[lf0][iq]", nd)}{_grad("1", "FE_dphi[lf1][iq]", nd)}    double jmp[{n2}], dn[{n2}];
+    for (int i = 0; i < {nd}; ++i) {{
+      jmp[i] = FE_phi[lf0][iq][i];
+      jmp[{nd} + i] = -FE_phi[lf1][iq][i];
+      dn[i] = 0.5 * (g0[i][0] * nx + g0[i][1] * ny + g0[i][2] * nz);
+      dn[{nd} + i] = 0.5 * (g1[i][0] * nx + g1[i][1] * ny + g1[i][2] * nz);
+    }}
+    const double fw = FE_w[iq] * ds;
+{loop}  }}
+""")
+        src = _SIG.format(name=name) + "\n{\n" + body + "}\n"
+        spec = dict(kind="dg_int", rank=rank, qm=mf, **common)
+        ent = np.ascontiguousarray(mesh.interior_facets.reshape(-1), dtype=np.int32)
+        return Integral(INTERIOR_FACET, -1, ent, name, src, None, spec)
+
+    J = Form(2, [V, V], [cell(2), ext(2), inter(2)], [u], [], "sipg_plaplace_J")
+    F = Form(1, [V], [cell(1), ext(1), inter(1)], [u], [], "sipg_plaplace_F")
+    return J, F

@@ -11,6 +11,10 @@ builds its forms: mesh, space, bilinear and linear forms, Dirichlet BCs.
 * `helmholtz_q3(n)`: grad u . grad v - k^2 u v, k = 5, Q3 on n^3 hexahedra
   with interior vertices jittered by +-0.2 h, f = 1, u = 0 on the boundary,
   4 x 4 x 4 Gauss points.
+* `dg_plaplace(n)`: SIPG DG1 p-Laplacian (p = 3, eps = 1e-3) with Robin
+  alpha = 2 and f = 1 on n^3 x 6 tets with interior vertices jittered by
+  +-0.2 h; `a` is the Newton Jacobian J(u), `L` the residual F(u) at the
+  state u = sin(pi x) sin(pi y) sin(pi z) interpolated into DG1.
 """
 from __future__ import annotations
 
@@ -77,3 +81,13 @@ def helmholtz_q3(n: int = 48, seed: int = 1, k: float = 5.0, degree: int = 3) ->
     a = forms.helmholtz(V, k, qpoints_per_dir=degree + 1)
     L = forms.hex_source(V, Constant(1.0), qpoints_per_dir=degree + 1)
     return Problem("helmholtz_q3", mesh, V, a, L, [_boundary_bc(mesh, V)], {"k": k})
+
+
+def dg_plaplace(n: int = 48, seed: int = 2, degree: int = 1, p: float = 3.0) -> Problem:
+    from .dgforms import sipg_plaplace
+    mesh = create_unit_cube(n, "tetrahedron", jitter=0.2, seed=seed)
+    V = functionspace(mesh, ("DG", degree))
+    u = Function(V, "u")
+    u.interpolate(lambda x: np.sin(np.pi * x[0]) * np.sin(np.pi * x[1]) * np.sin(np.pi * x[2]))
+    J, F = sipg_plaplace(V, u, f=1.0, alpha=2.0, p=p, eps=1e-3)
+    return Problem("dg_plaplace", mesh, V, J, F, [], {"u": u, "p": p})

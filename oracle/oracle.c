@@ -429,6 +429,147 @@ static void tab_ext_facet(const or_kernel* K, double* A, const double* w, const 
   }
 }
 
+/* ------------------------------------------------------------------ */
+/* SIPG p-Laplacian with Robin boundary (DG_k on affine tetrahedra)     */
+/* ------------------------------------------------------------------ */
+
+static double diameter(const double* xc)
+{
+  double h = 0.0;
+  for (int a = 0; a < 4; ++a)
+    for (int b = a + 1; b < 4; ++b) {
+      double s = 0.0;
+      for (int d = 0; d < 3; ++d) s += (xc[3 * a + d] - xc[3 * b + d]) * (xc[3 * a + d] - xc[3 * b + d]);
+      if (sqrt(s) > h) h = sqrt(s);
+    }
+  return h;
+}
+
+/* 2 x area of facet lf and its unit normal pointing away from the opposite vertex. */
+static double facet_normal(const double* xc, int lf, double* n)
+{
+  const int* f = TET_F[lf];
+  double e1[3], e2[3], o[3];
+  for (int d = 0; d < 3; ++d) {
+    e1[d] = xc[3 * f[1] + d] - xc[3 * f[0] + d];
+    e2[d] = xc[3 * f[2] + d] - xc[3 * f[0] + d];
+    o[d] = xc[3 * lf + d] - xc[3 * f[0] + d];
+  }
+  n[0] = e1[1] * e2[2] - e1[2] * e2[1];
+  n[1] = e1[2] * e2[0] - e1[0] * e2[2];
+  n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+  double ds = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  double sgn = (n[0] * o[0] + n[1] * o[1] + n[2] * o[2]) > 0.0 ? -1.0 : 1.0;
+  for (int d = 0; d < 3; ++d) n[d] *= sgn / ds;
+  return ds;
+}
+
+/* reference point of facet lf at triangle coordinates (s, t) */
+static void facet_point(int lf, double s, double t, double* X)
+{
+  const int* f = TET_F[lf];
+  for (int d = 0; d < 3; ++d)
+    X[d] = TET_V[f[0]][d] + s * (TET_V[f[1]][d] - TET_V[f[0]][d]) + t * (TET_V[f[2]][d] - TET_V[f[0]][d]);
+}
+
+/* physical basis gradients of a P_k tetrahedron at X */
+static void phys_grad(const or_kernel* K, const double* xc, const double* X, double* phi,
+                      double g[][3], double* adet)
+{
+  double dref[3 * OR_MAXND], Kinv[3][3];
+  double det = jacobian(0, xc, X, Kinv);
+  if (adet) *adet = fabs(det);
+  tet_basis(K->degree, X, phi, dref);
+  int nd = or_ndofs(0, K->degree);
+  for (int i = 0; i < nd; ++i)
+    for (int d = 0; d < 3; ++d)
+      g[i][d] = dref[3 * i] * Kinv[0][d] + dref[3 * i + 1] * Kinv[1][d] + dref[3 * i + 2] * Kinv[2][d];
+}
+
+static void tab_dg_cell(const or_kernel* K, double* A, const double* w, const double* xc)
+{
+  static double pts[3 * OR_MAXQ], wts[OR_MAXQ];
+  const int nd = or_ndofs(0, K->degree);
+  const int nq = or_tet_rule(K->qm, pts, wts);
+  double phi[OR_MAXND], g[OR_MAXND][3], adet;
+  for (int q = 0; q < nq; ++q) {
+    phys_grad(K, xc, &pts[3 * q], phi, g, &adet);
+    double dx = wts[q] * adet, gu[3] = {0, 0, 0};
+    for (int i = 0; i < nd; ++i)
+      for (int d = 0; d < 3; ++d) gu[d] += w[K->u_off + i] * g[i][d];
+    double s = K->eps * K->eps + gu[0] * gu[0] + gu[1] * gu[1] + gu[2] * gu[2];
+    double mu = pow(s, 0.5 * (K->p - 2.0)), dmu = (K->p - 2.0) * pow(s, 0.5 * (K->p - 4.0));
+    for (int i = 0; i < nd; ++i) {
+      double gi = gu[0] * g[i][0] + gu[1] * g[i][1] + gu[2] * g[i][2];
+      if (K->rank == 1) {
+        A[i] += dx * (mu * gi - K->fval * phi[i]);
+        continue;
+      }
+      for (int j = 0; j < nd; ++j) {
+        double gj = gu[0] * g[j][0] + gu[1] * g[j][1] + gu[2] * g[j][2];
+        A[i * nd + j] += dx * (mu * (g[i][0] * g[j][0] + g[i][1] * g[j][1] + g[i][2] * g[j][2]) +
+                               dmu * gi * gj);
+      }
+    }
+  }
+}
+
+static void tab_dg_ext(const or_kernel* K, double* A, const double* w, const double* xc, int lf)
+{
+  double pts[2 * OR_MAXND * OR_MAXND], wts[OR_MAXND * OR_MAXND], nrm[3];
+  const int nd = or_ndofs(0, K->degree);
+  const int nq = or_tri_rule(K->qm, pts, wts);
+  double ds = facet_normal(xc, lf, nrm);
+  for (int q = 0; q < nq; ++q) {
+    double X[3], phi[OR_MAXND], dref[3 * OR_MAXND];
+    facet_point(lf, pts[2 * q], pts[2 * q + 1], X);
+    tet_basis(K->degree, X, phi, dref);
+    double fw = wts[q] * ds * K->alpha, uq = 0.0;
+    for (int j = 0; j < nd; ++j) uq += w[K->u_off + j] * phi[j];
+    for (int i = 0; i < nd; ++i) {
+      if (K->rank == 1) { A[i] += fw * uq * phi[i]; continue; }
+      for (int j = 0; j < nd; ++j) A[i * nd + j] += fw * phi[i] * phi[j];
+    }
+  }
+}
+
+/* interior facet: rows/columns [cell 0 dofs, cell 1 dofs]; w interlaced (u0, u1) */
+static void tab_dg_int(const or_kernel* K, double* A, const double* w, const double* xc,
+                       const int* eli)
+{
+  double pts[2 * OR_MAXND * OR_MAXND], wts[OR_MAXND * OR_MAXND], nrm[3];
+  const int nd = or_ndofs(0, K->degree), n2 = 2 * nd;
+  const int nq = or_tri_rule(K->qm, pts, wts);
+  const double* xc1 = xc + 12;
+  double ds = facet_normal(xc, eli[0], nrm);
+  double sig = K->penalty / (0.5 * (diameter(xc) + diameter(xc1)));
+  for (int q = 0; q < nq; ++q) {
+    double X0[3], X1[3], phi0[OR_MAXND], phi1[OR_MAXND], g0[OR_MAXND][3], g1[OR_MAXND][3];
+    double jmp[2 * OR_MAXND], dn[2 * OR_MAXND];
+    facet_point(eli[0], pts[2 * q], pts[2 * q + 1], X0);
+    facet_point(eli[1], pts[2 * q], pts[2 * q + 1], X1);
+    phys_grad(K, xc, X0, phi0, g0, NULL);
+    phys_grad(K, xc1, X1, phi1, g1, NULL);
+    for (int i = 0; i < nd; ++i) {
+      jmp[i] = phi0[i];
+      jmp[nd + i] = -phi1[i];
+      dn[i] = 0.5 * (g0[i][0] * nrm[0] + g0[i][1] * nrm[1] + g0[i][2] * nrm[2]);
+      dn[nd + i] = 0.5 * (g1[i][0] * nrm[0] + g1[i][1] * nrm[1] + g1[i][2] * nrm[2]);
+    }
+    double fw = wts[q] * ds;
+    if (K->rank == 1) {
+      const double* u = w + 2 * K->u_off;
+      double ju = 0.0, du = 0.0;
+      for (int j = 0; j < n2; ++j) { ju += jmp[j] * u[j]; du += dn[j] * u[j]; }
+      for (int i = 0; i < n2; ++i) A[i] += fw * (-du * jmp[i] - dn[i] * ju + sig * ju * jmp[i]);
+      continue;
+    }
+    for (int i = 0; i < n2; ++i)
+      for (int j = 0; j < n2; ++j)
+        A[i * n2 + j] += fw * (-dn[j] * jmp[i] - dn[i] * jmp[j] + sig * jmp[i] * jmp[j]);
+  }
+}
+
 /* UFCx-shaped entry point: A += element tensor. */
 void or_tabulate(const or_kernel* K, double* A, const double* w, const double* c,
                  const double* coordinate_dofs, const int* entity_local_index)
@@ -438,6 +579,9 @@ void or_tabulate(const or_kernel* K, double* A, const double* w, const double* c
   case OR_ELASTICITY:
   case OR_SOURCE: tab_cell(K, A, w, c, coordinate_dofs); break;
   case OR_FACET_SOURCE: tab_ext_facet(K, A, w, c, coordinate_dofs, entity_local_index[0]); break;
+  case OR_DG_CELL: tab_dg_cell(K, A, w, coordinate_dofs); break;
+  case OR_DG_EXT: tab_dg_ext(K, A, w, coordinate_dofs, entity_local_index[0]); break;
+  case OR_DG_INT: tab_dg_int(K, A, w, coordinate_dofs, entity_local_index); break;
   default: break;
   }
 }

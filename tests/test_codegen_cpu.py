@@ -59,10 +59,21 @@ def _check_form(form, tmp_path, ncells=5, seed=0):
             ent = I.entities[e * stride:(e + 1) * stride]
             c = ent[0]
             xc = mesh.x[mesh.cells[c]]
+            w = packed[c]
             eli = [int(ent[1])] if I.type == "exterior_facet" else None
-            nA = nd * nd if form.rank == 2 else nd
-            got = _call(lib, I.name, nd, packed[c], consts, xc, eli, nA)
-            ref = cpu.tabulate(I.spec, packed[c], consts, xc, eli).reshape(-1)
+            n = nd
+            if I.type == "interior_facet":
+                c1 = ent[2]
+                xc = np.concatenate([xc, mesh.x[mesh.cells[c1]]])
+                off = form.coefficient_offsets
+                w = np.concatenate([np.concatenate([packed[c][off[j]:off[j + 1]],
+                                                    packed[c1][off[j]:off[j + 1]]])
+                                    for j in range(len(off) - 1)]) if len(off) > 1 else w
+                eli = [int(ent[1]), int(ent[3])]
+                n = 2 * nd
+            nA = n * n if form.rank == 2 else n
+            got = _call(lib, I.name, nd, w, consts, xc, eli, nA)
+            ref = cpu.tabulate(I.spec, w, consts, xc, eli).reshape(-1)
             scale = max(np.abs(ref).max(), 1e-300)
             assert np.abs(got - ref).max() <= 1e-13 * scale, (I.name, e)
 
@@ -97,6 +108,15 @@ def test_generated_hex_helmholtz(tmp_path, k):
     P = problems.helmholtz_q3(2, degree=k)
     _check_form(P.a, tmp_path, ncells=3)
     _check_form(P.L, tmp_path, ncells=3)
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_generated_sipg_plaplace(tmp_path, k):
+    P = problems.dg_plaplace(2, degree=k)
+    rng = np.random.default_rng(1)
+    P.extras["u"].x[:] = rng.uniform(-1, 1, P.V.size)
+    _check_form(P.a, tmp_path, ncells=4)
+    _check_form(P.L, tmp_path, ncells=4)
 
 
 def test_staged_description_matches_oracle():

@@ -204,3 +204,12 @@ def test_helmholtz_q3_matrix_core(prec, tol):
     ea, eb = _rel(vals, ref[2]), _rel(b, ref[3])
     _say(f"  precision {prec}: relerr A={ea:.3e} b={eb:.3e}")
     assert ea <= tol and eb <= 1e-12, (ea, eb)
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_sipg_plaplace_interior_facets(k):
+    _say(f"SIPG p-Laplacian DG{k} n=3 jittered: cells + Robin facets + interior facets")
+    P = problems.dg_plaplace(3, degree=k)
+    rng = np.random.default_rng(11)
+    P.extras["u"].x[:] = rng.uniform(-1, 1, P.V.size)
+    _compare(P, _gpu(P), _oracle(P))

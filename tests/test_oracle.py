@@ -252,3 +252,126 @@ def test_jittered_hex_mass_sums_to_volume():
     for c in range(mesh.num_cells):
         total += cpu.tabulate(spec, [], [], mesh.x[mesh.cells[c]]).sum()
     assert abs(total - 1.0) < 1e-13
+
+
+# ------------------------------------------------------------ SIPG p-Laplacian (DG1)
+_TF = [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)]
+
+
+def _dg1_energy(mesh, u, p, eps, f, alpha, penalty):
+    """E(u) whose derivative is the SIPG residual, from vertex values only.
+
+    Cell gradients come from solving the vertex-difference system, facet
+    integrals of products of linear functions from the exact triangle rule
+    int_T a b = |T| / 12 (sum a_k b_k + sum a_k sum b_k); nothing here shares
+    code or tables with the oracle.
+    """
+    cells, x = mesh.cells, mesh.x
+    U = u.reshape(-1, 4)
+    E_ = 0.0
+    grads, diam = [], []
+    for c in range(mesh.num_cells):
+        xc = x[cells[c]]
+        D = xc[1:] - xc[0]
+        g = np.linalg.solve(D, U[c, 1:] - U[c, 0])
+        vol = abs(np.linalg.det(D)) / 6.0
+        s = eps * eps + g @ g
+        E_ += vol * (s ** (p / 2) / p - f * U[c].mean())
+        grads.append(g)
+        diam.append(max(np.linalg.norm(xc[a] - xc[b]) for a in range(4) for b in range(a + 1, 4)))
+
+    def tri(c, lf):
+        v = [cells[c][k] for k in _TF[lf]]
+        P = x[v]
+        n = np.cross(P[1] - P[0], P[2] - P[0])
+        area = 0.5 * np.linalg.norm(n)
+        n = n / np.linalg.norm(n)
+        if n @ (x[cells[c][lf]] - P[0]) > 0:
+            n = -n
+        return v, area, n
+
+    def int2(a, b, area):
+        return area / 12.0 * (a @ b + a.sum() * b.sum())
+
+    for c0, l0, c1, l1 in mesh.interior_facets:
+        v, area, n = tri(c0, l0)
+        u0 = np.array([U[c0][list(cells[c0]).index(g)] for g in v])
+        u1 = np.array([U[c1][list(cells[c1]).index(g)] for g in v])
+        j = u0 - u1
+        hF = 0.5 * (diam[c0] + diam[c1])
+        avg_dn = 0.5 * (grads[c0] + grads[c1]) @ n
+        E_ += 0.5 * penalty / hF * int2(j, j, area) - avg_dn * area * j.mean()
+    for c, lf in mesh.exterior_facets:
+        v, area, n = tri(c, lf)
+        uv = np.array([U[c][list(cells[c]).index(g)] for g in v])
+        E_ += 0.5 * alpha * int2(uv, uv, area)
+    return E_
+
+
+def _dg_problem(n=2, p=3.0):
+    P = problems.dg_plaplace(n, p=p)
+    rng = np.random.default_rng(7)
+    P.extras["u"].x[:] = rng.uniform(-1, 1, P.V.size)
+    return P
+
+
+def _dg_residual(P, u):
+    P.extras["u"].x[:] = u
+    return cpu.OracleAssembler(P.L).assemble_vector()
+
+
+@pytest.mark.parametrize("p", [2.0, 3.0])
+def test_sipg_residual_is_energy_gradient(p):
+    P = _dg_problem(2, p)
+    u = P.extras["u"].x.copy()
+    F = _dg_residual(P, u)
+    spec = P.L.integrals[0].spec
+    args = (P.mesh, None, spec["p"], spec["eps"], spec["fval"], spec["alpha"], spec["penalty"])
+    rng = np.random.default_rng(3)
+    for _ in range(4):
+        d = rng.standard_normal(u.size)
+        h = 1e-5
+        dE = (_dg1_energy(P.mesh, u + h * d, *args[2:]) - _dg1_energy(P.mesh, u - h * d, *args[2:])) / (2 * h)
+        assert abs(dE - F @ d) <= 1e-7 * max(1.0, abs(dE)), (dE, F @ d)
+
+
+def test_sipg_jacobian_is_residual_derivative():
+    P = _dg_problem(2)
+    u = P.extras["u"].x.copy()
+    oa = cpu.OracleAssembler(P.a)
+    rp, cols, _, _ = oa.sparsity()
+    vals = oa.assemble_matrix(rp, cols)
+    import scipy.sparse as sp
+    J = sp.csr_matrix((vals, cols, rp), shape=(P.V.size, P.V.size))
+    assert abs(J - J.T).max() <= 1e-13 * abs(vals).max()
+    rng = np.random.default_rng(4)
+    d = rng.standard_normal(u.size)
+    h = 1e-6
+    fd = (_dg_residual(P, u + h * d) - _dg_residual(P, u - h * d)) / (2 * h)
+    Jd = J @ d
+    assert np.linalg.norm(fd - Jd) <= 1e-7 * np.linalg.norm(Jd)
+
+
+def test_sipg_consistency_for_linear_fields():
+    """p = 2, f = 0, alpha = 0: for a continuous linear u the cell and interior
+    facet terms sum to the boundary flux int_dOmega (grad u . n) v."""
+    from mifem.dgforms import sipg_plaplace
+    mesh = create_unit_cube(2, jitter=0.2, seed=2)
+    V = functionspace(mesh, ("DG", 1))
+    u = Function(V)
+    gvec = np.array([0.3, -1.1, 0.7])
+    u.interpolate(lambda x: 0.5 + gvec @ x)
+    _, F = sipg_plaplace(V, u, f=0.0, alpha=0.0, p=2.0, eps=0.0)
+    r = cpu.OracleAssembler(F).assemble_vector().reshape(-1, 4)
+    ref = np.zeros_like(r)
+    for c, lf in mesh.exterior_facets:
+        v = mesh.cells[c][list(_TF[lf])]
+        P = mesh.x[v]
+        nrm = np.cross(P[1] - P[0], P[2] - P[0])
+        area = 0.5 * np.linalg.norm(nrm)
+        nrm /= np.linalg.norm(nrm)
+        if nrm @ (mesh.x[mesh.cells[c][lf]] - P[0]) > 0:
+            nrm = -nrm
+        for k in _TF[lf]:
+            ref[c, k] += (gvec @ nrm) * area / 3.0
+    np.testing.assert_allclose(r, ref, atol=1e-13)

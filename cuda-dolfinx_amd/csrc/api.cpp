@@ -284,7 +284,7 @@ int mf_form_set_option(mf_form* f, const char* key, int64_t value) {
     static const char* known[] = {"matrix_kernel", "chunk_values", "chunk_rows", "block",
                                   "owner_block", "owner_nb", "owner_lds", "vector_kernel",
                                   "vector_block", "vector_chunk_values", "lift_kernel",
-                                  "owner_pairs", "precision"};
+                                  "owner_pairs", "precision", "dg_block"};
     bool ok = false;
     for (auto k : known) ok |= std::strcmp(k, key) == 0;
     MF_CHECK(ok, MF_EINVAL, std::string("unknown option ") + key);
@@ -333,6 +333,7 @@ int mf_form_compile(mf_form* f, int verbose) {
     auto code = mf::rtc_compile(src, "mifem_form", rtc_options(f->ctx), f->ctx->cache_dir, &cached);
     f->mod = std::make_unique<mf::Module>();
     f->mod->load(code);
+    if (mf::dg_owner_emitted(*f)) f->k_dgown = f->mod->get(f->rank == 2 ? "mf_dgown" : "mf_dgvown");
     for (auto& I : f->ints) {
       if (f->rank == 2) {
         I->k_mat = f->mod->get(mf::kernel_name("mat", I->type, I->index));
@@ -605,6 +606,38 @@ int mf_set_bc(mf_context* ctx, double* b, int64_t n, const mf_bc* bc, const doub
     use_device(ctx);
     mf::k_set_bc(b, n, bc->dofs.ptr, bc->n_owned, bc->g.ptr, x0, scale, S(s));
   });
+}
+
+// ------------------------------------------------------------------ graphs
+int mf_graph_begin(mf_context* ctx, mf_stream s) {
+  return guard([&] {
+    MF_CHECK(ctx && s, MF_EINVAL, "graph capture needs a context and a non-default stream");
+    use_device(ctx);
+    MF_HIP(hipStreamBeginCapture(S(s), hipStreamCaptureModeThreadLocal));
+  });
+}
+
+int mf_graph_end(mf_context* ctx, mf_stream s, mf_graph** out) {
+  return guard([&] {
+    MF_CHECK(ctx && s && out, MF_EINVAL, "null argument");
+    use_device(ctx);
+    auto g = std::make_unique<mf_graph>();
+    MF_HIP(hipStreamEndCapture(S(s), &g->graph));
+    MF_CHECK(g->graph, MF_ESTATE, "stream capture produced no graph");
+    MF_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    *out = g.release();
+  });
+}
+
+int mf_graph_launch(mf_graph* g, mf_stream s) {
+  return guard([&] {
+    MF_CHECK(g && g->exec, MF_EINVAL, "null graph");
+    MF_HIP(hipGraphLaunch(g->exec, S(s)));
+  });
+}
+
+int mf_graph_destroy(mf_graph* g) {
+  return guard([&] { delete g; });
 }
 
 }  // extern "C"

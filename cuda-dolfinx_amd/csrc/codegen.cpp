@@ -262,6 +262,22 @@ extern const char* kPrepTemplate;
 extern const char* kHexRecord;
 extern const char* kHexPrepTemplate;
 extern const char* kHexOwnerTemplate;
+extern const char* kDGOwnerMatrix;        // dg_kernel.cpp
+extern const char* kDGOwnerVector;
+extern const char* kDGCaseCellMat;
+extern const char* kDGCaseExtMat;
+extern const char* kDGCaseIntSide;
+extern const char* kDGIntMatBody;
+extern const char* kDGCaseCellVec;
+extern const char* kDGCaseExtVec;
+extern const char* kDGIntVecBody;
+
+bool dg_owner_emitted(const mf_form& f) {
+  // one thread per cell keeps the interior-facet tensor (2 nd bs)^2 in registers
+  if (f.rank == 2 && f.dm[0] != f.dm[1]) return false;
+  const int ndb = f.dm[0]->nd * f.dm[0]->bs;
+  return (f.rank == 1 || f.rank == 2) && ndb <= 10;
+}
 
 std::string generate_form_source(const mf_form& f) {
   std::ostringstream os;
@@ -280,6 +296,42 @@ std::string generate_form_source(const mf_form& f) {
   const mf_dofmap* d0 = f.dm[0];
   const mf_dofmap* d1 = f.rank == 2 ? f.dm[1] : f.dm[0];
   os << kOwnerCommon;
+  std::string interlace;
+  for (auto& c : f.coefs) {
+    std::ostringstream il;
+    il << "    for (int k = 0; k < " << c.width << "; ++k) { w[" << 2 * c.offset << " + k] = w0["
+       << c.offset << " + k]; w[" << 2 * c.offset + c.width << " + k] = w1[" << c.offset
+       << " + k]; }\n";
+    interlace += il.str();
+  }
+  if (dg_owner_emitted(f)) {
+    std::map<std::string, std::string> m;
+    m["$NV"] = std::to_string(f.mesh->nv);
+    m["$NF"] = std::to_string(std::max(1, f.mesh->nf));
+    m["$W"] = std::to_string(f.W);
+    m["$ND0"] = std::to_string(d0->nd);
+    m["$BS0"] = std::to_string(d0->bs);
+    m["$DBLOCK"] = std::to_string(f.opt("dg_block", 64));
+    std::string cases;
+    for (auto& I : f.ints) {
+      std::map<std::string, std::string> c{{"$K", std::to_string(I->index)}, {"$TAB", I->tab_name}};
+      if (I->type == MF_CELL) {
+        cases += subst(f.rank == 2 ? kDGCaseCellMat : kDGCaseCellVec, c);
+      } else if (I->type == MF_EXTERIOR_FACET) {
+        cases += subst(f.rank == 2 ? kDGCaseExtMat : kDGCaseExtVec, c);
+      } else {
+        std::string body = "      case " + std::to_string(I->index) + ": {\n";
+        for (int side : {0, 1}) {
+          std::string t = subst(kDGCaseIntSide, {{"$BODY", f.rank == 2 ? kDGIntMatBody : kDGIntVecBody},
+                                                 {"$INTERLACE", interlace}});
+          body += subst(t, {{"$SIDE", std::to_string(side)}, {"$TAB", I->tab_name}});
+        }
+        cases += body + "      } break;\n";
+      }
+    }
+    m["$KNAME"] = f.rank == 2 ? "mf_dgown" : "mf_dgvown";
+    os << subst(subst(f.rank == 2 ? kDGOwnerMatrix : kDGOwnerVector, {{"$CASES", cases}}), m);
+  }
   for (auto& I : f.ints) {
     const int ncell = entity_ncell(I->type);
     std::map<std::string, std::string> m;
@@ -296,14 +348,6 @@ std::string generate_form_source(const mf_form& f) {
     m["$NC"] = std::to_string(ncell * d1->nd * d1->bs);
     m["$TAB"] = I->tab_name;
     m["$BLOCK"] = std::to_string(f.opt("block", 128));
-    std::string interlace;
-    for (auto& c : f.coefs) {
-      std::ostringstream il;
-      il << "    for (int k = 0; k < " << c.width << "; ++k) { w[" << 2 * c.offset << " + k] = w0["
-         << c.offset << " + k]; w[" << 2 * c.offset + c.width << " + k] = w1[" << c.offset
-         << " + k]; }\n";
-      interlace += il.str();
-    }
     std::string gather = I->type == MF_CELL ? kGatherCell
                          : I->type == MF_EXTERIOR_FACET ? kGatherExt
                                                         : kGatherInt;

@@ -1,0 +1,233 @@
+// Cell-owner assembly kernels for forms on discontinuous spaces (HIP
+// templates, hipRTC).
+//
+// When the test (and trial) space is discontinuous with cell-major dofs
+// (dof(c, i) = c * nd + i), the CSR rows of cell c are exactly its own dofs,
+// and every column block of those rows comes from one source: the self
+// block (c, c) from the cell integral, the exterior facets and the c-side of
+// every interior facet; the block (c, c') of a face neighbour c' only from
+// the interior facet the two cells share.  One thread per cell therefore
+// owns its rows outright: it walks its items (cell / facet integrals that
+// touch it), tabulates each element tensor with the form's UFCx body,
+// accumulates the self block in registers, stores each neighbour block as
+// soon as its facet is tabulated, and finally stores the self block.  Every
+// value is written exactly once, with no zero pass, no atomics and no
+// sparsity search (block positions come precomputed in the items).  An
+// interior facet is tabulated by both of its cells (each keeps its own rows):
+// twice the facet arithmetic in exchange for a conflict-free write pattern.
+//
+// Reference semantics: cell, exterior-facet and interior-facet matrix
+// assembly of cpp/cudolfinx/fem/CUDAFormIntegral.cpp:1103-1226, 1667-1793,
+// 1988-2103 (interlaced coefficients :1855-1874), Dirichlet rows/columns
+// skipped and the diagonal inserted as in CUDAAssembler.h:878-958; vectors
+// :270-522.
+//
+// Item layout (int4, sorted by owning cell): x = integral index in the form,
+// y = neighbour cell (interior facets) or -1, z = local facet of the owner |
+// local facet of the neighbour << 8, w = side (0: owner is the facet's first
+// cell) | self block position << 4 | neighbour block position << 16.
+#include "mf_internal.h"
+
+namespace mf {
+
+const char* kDGOwnerMatrix = R"(
+extern "C" __global__ void __launch_bounds__($DBLOCK)
+$KNAME(const int32_t ncells, const int32_t* __restrict__ cell_ptr, const int4* __restrict__ items,
+       const int32_t* __restrict__ geo, const double* __restrict__ xg,
+       const double* __restrict__ coeffs, const double* __restrict__ consts,
+       const int32_t* __restrict__ row_ptr, const int8_t* __restrict__ bc0,
+       const int8_t* __restrict__ bc1, double* __restrict__ values, const double diag,
+       const int flags, const uint8_t* __restrict__ fperm)
+{
+  constexpr int NDB = $ND0 * $BS0, N2 = 2 * NDB;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const int i0 = cell_ptr[c], i1 = cell_ptr[c + 1];
+  if (i0 == i1) return;
+  double xs[$NV * 3];
+  for (int v = 0; v < $NV; ++v) {
+    const int64_t gv = geo[(int64_t)c * $NV + v];
+    xs[3 * v] = xg[3 * gv]; xs[3 * v + 1] = xg[3 * gv + 1]; xs[3 * v + 2] = xg[3 * gv + 2];
+  }
+  const double* ws = coeffs + (int64_t)c * $W;
+  const int64_t r0 = (int64_t)c * NDB;
+  const int64_t rp0 = row_ptr[r0];
+  const int64_t L = row_ptr[r0 + 1] - rp0;
+  const bool zero = flags & 1;
+  double S[NDB * NDB];
+#pragma unroll
+  for (int k = 0; k < NDB * NDB; ++k) S[k] = 0.0;
+  int pself = 0;
+  // store one neighbour block (rows of this cell, columns of cell co)
+  auto put_block = [&](const double* blk, int64_t co, int pos, int stride) {
+    for (int i = 0; i < NDB; ++i) {
+      const int64_t row = r0 + i;
+      const bool rmask = bc0 && bc0[row];
+      double* out = values + rp0 + i * L + (int64_t)pos * NDB;
+      for (int j = 0; j < NDB; ++j) {
+        const bool m = rmask || (bc1 && bc1[co * NDB + j]);
+        const double v = m ? 0.0 : blk[i * stride + j];
+        if (zero) out[j] = v;
+        else if (!m) out[j] += v;
+      }
+    }
+  };
+  for (int t = i0; t < i1; ++t) {
+    const int4 it = items[t];
+    pself = (it.w >> 4) & 0xFFF;
+    switch (it.x) {
+$CASES
+      default: break;
+    }
+  }
+  // self block, with the Dirichlet diagonal
+  for (int i = 0; i < NDB; ++i) {
+    const int64_t row = r0 + i;
+    const bool rmask = bc0 && bc0[row];
+    double* out = values + rp0 + i * L + (int64_t)pself * NDB;
+    for (int j = 0; j < NDB; ++j) {
+      const bool m = rmask || (bc1 && bc1[r0 + j]);
+      double v = m ? 0.0 : S[i * NDB + j];
+      if ((flags & 2) && rmask && j == i) {
+        out[j] = diag;
+        continue;
+      }
+      if (zero) out[j] = v;
+      else if (!m) out[j] += v;
+    }
+  }
+}
+)";
+
+const char* kDGOwnerVector = R"(
+extern "C" __global__ void __launch_bounds__($DBLOCK)
+$KNAME(const int32_t ncells, const int32_t* __restrict__ cell_ptr, const int4* __restrict__ items,
+       const int32_t* __restrict__ geo, const double* __restrict__ xg,
+       const double* __restrict__ coeffs, const double* __restrict__ consts,
+       double* __restrict__ b, const int flags, const uint8_t* __restrict__ fperm)
+{
+  constexpr int NDB = $ND0 * $BS0, N2 = 2 * NDB;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const int i0 = cell_ptr[c], i1 = cell_ptr[c + 1];
+  double xs[$NV * 3];
+  for (int v = 0; v < $NV; ++v) {
+    const int64_t gv = geo[(int64_t)c * $NV + v];
+    xs[3 * v] = xg[3 * gv]; xs[3 * v + 1] = xg[3 * gv + 1]; xs[3 * v + 2] = xg[3 * gv + 2];
+  }
+  const double* ws = coeffs + (int64_t)c * $W;
+  double S[NDB];
+#pragma unroll
+  for (int k = 0; k < NDB; ++k) S[k] = 0.0;
+  for (int t = i0; t < i1; ++t) {
+    const int4 it = items[t];
+    switch (it.x) {
+$CASES
+      default: break;
+    }
+  }
+  double* out = b + (int64_t)c * NDB;
+  if (flags & 1) {
+    for (int i = 0; i < NDB; ++i) out[i] = S[i];
+  } else {
+    for (int i = 0; i < NDB; ++i) out[i] += S[i];
+  }
+}
+)";
+
+// Case bodies: $K integral index, $TAB tabulate function.  In the interior
+// facet case $SIDE is substituted with 0 and 1 (constant indices keep the
+// element tensor in registers).
+const char* kDGCaseCellMat = R"(
+      case $K: {
+        double A[NDB * NDB];
+#pragma unroll
+        for (int k = 0; k < NDB * NDB; ++k) A[k] = 0.0;
+        $TAB(A, ws, consts, xs, nullptr, nullptr);
+#pragma unroll
+        for (int k = 0; k < NDB * NDB; ++k) S[k] += A[k];
+      } break;
+)";
+
+const char* kDGCaseExtMat = R"(
+      case $K: {
+        const int eli[1] = {it.z & 0xFF};
+        const uint8_t qp[1] = {fperm ? fperm[(int64_t)c * $NF + eli[0]] : (uint8_t)0};
+        double A[NDB * NDB];
+#pragma unroll
+        for (int k = 0; k < NDB * NDB; ++k) A[k] = 0.0;
+        $TAB(A, ws, consts, xs, eli, qp);
+#pragma unroll
+        for (int k = 0; k < NDB * NDB; ++k) S[k] += A[k];
+      } break;
+)";
+
+const char* kDGCaseIntSide = R"(
+        if ((it.w & 1) == $SIDE) {
+          const int64_t co = it.y;
+          const int lfs = it.z & 0xFF, lfo = (it.z >> 8) & 0xFF;
+          double xc[2 * $NV * 3];
+          for (int k = 0; k < $NV * 3; ++k) xc[$SIDE * $NV * 3 + k] = xs[k];
+          for (int v = 0; v < $NV; ++v) {
+            const int64_t gv = geo[co * $NV + v];
+            xc[(1 - $SIDE) * $NV * 3 + 3 * v] = xg[3 * gv];
+            xc[(1 - $SIDE) * $NV * 3 + 3 * v + 1] = xg[3 * gv + 1];
+            xc[(1 - $SIDE) * $NV * 3 + 3 * v + 2] = xg[3 * gv + 2];
+          }
+          const double* w0 = $SIDE == 0 ? ws : coeffs + co * $W;
+          const double* w1 = $SIDE == 0 ? coeffs + co * $W : ws;
+          double w[2 * $W + 1];
+$INTERLACE          const int eli[2] = {$SIDE == 0 ? lfs : lfo, $SIDE == 0 ? lfo : lfs};
+          const int64_t c0 = $SIDE == 0 ? (int64_t)c : co, c1 = $SIDE == 0 ? co : (int64_t)c;
+          const uint8_t qp[2] = {fperm ? fperm[c0 * $NF + eli[0]] : (uint8_t)0,
+                                 fperm ? fperm[c1 * $NF + eli[1]] : (uint8_t)0};
+          (void)w0; (void)w1;
+$BODY        }
+)";
+
+const char* kDGIntMatBody = R"(
+          double A[N2 * N2];
+#pragma unroll
+          for (int k = 0; k < N2 * N2; ++k) A[k] = 0.0;
+          $TAB(A, w, consts, xc, eli, qp);
+#pragma unroll
+          for (int i = 0; i < NDB; ++i)
+#pragma unroll
+            for (int j = 0; j < NDB; ++j) S[i * NDB + j] += A[($SIDE * NDB + i) * N2 + $SIDE * NDB + j];
+          put_block(&A[$SIDE * NDB * N2 + (1 - $SIDE) * NDB], co, (it.w >> 16) & 0xFFF, N2);
+)";
+
+const char* kDGCaseCellVec = R"(
+      case $K: {
+        double A[NDB];
+#pragma unroll
+        for (int k = 0; k < NDB; ++k) A[k] = 0.0;
+        $TAB(A, ws, consts, xs, nullptr, nullptr);
+#pragma unroll
+        for (int k = 0; k < NDB; ++k) S[k] += A[k];
+      } break;
+)";
+
+const char* kDGCaseExtVec = R"(
+      case $K: {
+        const int eli[1] = {it.z & 0xFF};
+        const uint8_t qp[1] = {fperm ? fperm[(int64_t)c * $NF + eli[0]] : (uint8_t)0};
+        double A[NDB];
+#pragma unroll
+        for (int k = 0; k < NDB; ++k) A[k] = 0.0;
+        $TAB(A, ws, consts, xs, eli, qp);
+#pragma unroll
+        for (int k = 0; k < NDB; ++k) S[k] += A[k];
+      } break;
+)";
+
+const char* kDGIntVecBody = R"(
+          double A[N2];
+#pragma unroll
+          for (int k = 0; k < N2; ++k) A[k] = 0.0;
+          $TAB(A, w, consts, xc, eli, qp);
+#pragma unroll
+          for (int i = 0; i < NDB; ++i) S[i] += A[$SIDE * NDB + i];
+)";
+
+}  // namespace mf

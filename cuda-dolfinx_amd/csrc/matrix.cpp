@@ -233,6 +233,96 @@ DevArray<int32_t>& get_table(mf_form* a, Integral& I, mf_matrix* A, hipStream_t 
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Cell-owner schedule for discontinuous spaces (kernel: dg_kernel.cpp)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// The test space must number dofs cell by cell (node(c, i) = c * nd + i) so a
+// cell's CSR rows are its own.  With a matrix, each item also carries the
+// block positions of the owner and the neighbour inside the owner's rows.
+std::shared_ptr<DGPlan> build_dg_plan(const mf_form& f, const mf_matrix* A) {
+  const mf_dofmap* d = f.dm[0];
+  const int nd = d->nd;
+  const int64_t nc = d->num_cells;
+  if (f.rank == 2 && f.dm[1] != d) return nullptr;
+  if (d->num_nodes != nc * nd) return nullptr;
+  for (int64_t k = 0; k < nc * nd; ++k)
+    if (d->host[k] != k) return nullptr;
+  std::vector<int64_t> count(nc + 1, 0);
+  for (auto& I : f.ints) {
+    const int st = entity_stride(I->type);
+    for (int64_t e = 0; e < I->n; ++e) {
+      count[I->h_ents[e * st] + 1]++;
+      if (I->type == MF_INTERIOR_FACET) count[I->h_ents[e * st + 2] + 1]++;
+    }
+  }
+  for (int64_t c = 0; c < nc; ++c) count[c + 1] += count[c];
+  const int64_t nitems = count[nc];
+  MF_CHECK(nitems < (int64_t(1) << 31), MF_EUNSUPPORTED, "too many cell items");
+  std::vector<int32_t> items(4 * std::max<int64_t>(nitems, 1), 0);
+  std::vector<int64_t> pos(count.begin(), count.end() - 1);
+  auto push = [&](int64_t c, int32_t k, int32_t other, int32_t lfs, int32_t lfo, int32_t side) {
+    int32_t* it = &items[4 * pos[c]++];
+    it[0] = k; it[1] = other; it[2] = lfs | (lfo << 8); it[3] = side;
+  };
+  for (auto& I : f.ints) {
+    const int st = entity_stride(I->type);
+    for (int64_t e = 0; e < I->n; ++e) {
+      const int32_t* en = &I->h_ents[e * st];
+      if (I->type == MF_CELL) push(en[0], I->index, -1, 0, 0, 0);
+      else if (I->type == MF_EXTERIOR_FACET) push(en[0], I->index, -1, en[1], 0, 0);
+      else {
+        push(en[0], I->index, en[2], en[1], en[3], 0);
+        push(en[2], I->index, en[0], en[3], en[1], 1);
+      }
+    }
+  }
+  if (A) {
+    // block positions in the owner's rows; each neighbour block must have one source
+    std::vector<int32_t> seen;
+    for (int64_t c = 0; c < nc; ++c) {
+      if (count[c] == count[c + 1]) continue;
+      const int64_t R = c * nd;
+      const int32_t* cb = A->h_colsb.data() + A->h_rpb[R];
+      const int64_t len = A->h_rpb[R + 1] - A->h_rpb[R];
+      if (len % nd) return nullptr;
+      auto block_of = [&](int64_t cell) -> int {
+        const int32_t key = static_cast<int32_t>(cell * nd);
+        const int32_t* p = std::lower_bound(cb, cb + len, key);
+        if (p == cb + len || *p != key || (p - cb) % nd) return -1;
+        return static_cast<int>((p - cb) / nd);
+      };
+      const int ps = block_of(c);
+      if (ps < 0 || ps > 0xFFF) return nullptr;
+      seen.clear();
+      for (int64_t t = count[c]; t < count[c + 1]; ++t) {
+        int32_t* it = &items[4 * t];
+        int po = 0;
+        if (it[1] >= 0) {
+          po = block_of(it[1]);
+          if (po < 0 || po > 0xFFF || po == ps) return nullptr;
+          if (std::find(seen.begin(), seen.end(), po) != seen.end()) return nullptr;
+          seen.push_back(po);
+        }
+        it[3] |= (ps << 4) | (po << 16);
+      }
+      if (len / nd != 1 + static_cast<int64_t>(seen.size())) return nullptr;   // unwritten blocks
+    }
+  }
+  auto plan = std::make_shared<DGPlan>();
+  std::vector<int32_t> cp(nc + 1);
+  for (int64_t c = 0; c <= nc; ++c) cp[c] = static_cast<int32_t>(count[c]);
+  plan->cell_ptr.upload(cp.data(), nc + 1);
+  plan->items.upload(items.data(), items.size());
+  plan->ncells = nc;
+  plan->nitems = nitems;
+  return plan;
+}
+
+}  // namespace
+
 void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc1,
                      double diagonal, int zero, hipStream_t s) {
   MF_CHECK(a->rank == 2, MF_EINVAL, "assemble_matrix needs a bilinear form");
@@ -246,6 +336,30 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
   const int8_t* m0 = bc0 && bc0->nbc ? bc0->markers.ptr : nullptr;
   const int8_t* m1 = bc1 && bc1->nbc ? bc1->markers.ptr : nullptr;
   const int64_t mode = a->opt("matrix_kernel", 0);
+
+  // discontinuous spaces: one cell-owner launch over every integral
+  if (a->k_dgown && (mode == 0 || mode == 3)) {
+    auto it = A->dgplans.find(a->uid);
+    std::shared_ptr<DGPlan> dp;
+    if (it == A->dgplans.end()) {
+      dp = build_dg_plan(*a, A);
+      A->dgplans[a->uid] = dp;
+    } else {
+      dp = it->second;
+    }
+    if (dp) {
+      int flags = zero ? 1 : 0;
+      if (m0 && square) flags |= 2;
+      const uint32_t blk = static_cast<uint32_t>(a->opt("dg_block", 64));
+      launch(a->k_dgown, static_cast<uint32_t>((dp->ncells + blk - 1) / blk), blk, 0, s,
+             static_cast<int32_t>(dp->ncells), dp->cell_ptr.ptr,
+             reinterpret_cast<const int4*>(dp->items.ptr), a->mesh->cells.ptr, a->mesh->x.ptr,
+             a->packed.ptr, a->consts.ptr, A->row_ptr.ptr, m0, m1, A->values, diagonal, flags,
+             a->mesh->fperm.ptr);
+      return;
+    }
+  }
+  MF_CHECK(mode != 3, MF_EUNSUPPORTED, "cell-owner DG kernel unavailable for this form/matrix");
 
   // the first cell integral with a staged description may run owner-computes
   Integral* owner = nullptr;
@@ -332,6 +446,21 @@ void assemble_vector(mf_form* L, double* b, int64_t n, int zero, hipStream_t s) 
   MF_CHECK(L->compiled, MF_ESTATE, "form is not compiled");
   MF_CHECK(n == L->dm[0]->num_nodes * L->dm[0]->bs, MF_ESHAPE, "vector length mismatch");
   const int64_t mode = L->opt("vector_kernel", 0);
+  if (L->k_dgown && (mode == 0 || mode == 3)) {
+    if (!L->dgvplan_tried) {
+      L->dgvplan = build_dg_plan(*L, nullptr);
+      L->dgvplan_tried = true;
+    }
+    if (auto& dp = L->dgvplan) {
+      const uint32_t blk = static_cast<uint32_t>(L->opt("dg_block", 64));
+      launch(L->k_dgown, static_cast<uint32_t>((dp->ncells + blk - 1) / blk), blk, 0, s,
+             static_cast<int32_t>(dp->ncells), dp->cell_ptr.ptr,
+             reinterpret_cast<const int4*>(dp->items.ptr), L->mesh->cells.ptr, L->mesh->x.ptr,
+             L->packed.ptr, L->consts.ptr, b, zero ? 1 : 0, L->mesh->fperm.ptr);
+      return;
+    }
+  }
+  MF_CHECK(mode != 3, MF_EUNSUPPORTED, "cell-owner DG vector kernel unavailable for this form");
   Integral* owner = nullptr;
   std::shared_ptr<OwnerPlan> plan;
   if (mode != 1)

@@ -191,6 +191,12 @@ struct Coef {
   int offset = 0, width = 0;
 };
 struct OwnerPlan;   // cooperative owner-computes schedule (matrix.cpp)
+// Cell-owner schedule for forms on discontinuous spaces (matrix.cpp)
+struct DGPlan {
+  DevArray<int32_t> cell_ptr;         // [ncells + 1] item ranges
+  DevArray<int32_t> items;            // int4 per item (layout in dg_kernel.cpp)
+  int64_t ncells = 0, nitems = 0;
+};
 }  // namespace mf
 
 struct mf_form {
@@ -210,10 +216,22 @@ struct mf_form {
   std::string source;
   std::map<std::string, int64_t> options;
   std::map<int, std::shared_ptr<mf::OwnerPlan>> vplans;   // owner-computes vector schedules
+  hipFunction_t k_dgown = nullptr;    // cell-owner kernel over all integrals (DG spaces)
+  std::shared_ptr<mf::DGPlan> dgvplan;                    // its vector schedule
+  bool dgvplan_tried = false;
   uint64_t uid = 0;
   int64_t opt(const std::string& k, int64_t dflt) const {
     auto it = options.find(k);
     return it == options.end() ? dflt : it->second;
+  }
+};
+
+struct mf_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~mf_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
   }
 };
 
@@ -228,6 +246,7 @@ struct mf_matrix {
   // per (form uid, integral index) offset tables
   std::map<std::pair<uint64_t, int>, mf::DevArray<int32_t>> tables;
   std::map<std::pair<uint64_t, int>, std::shared_ptr<mf::OwnerPlan>> plans;
+  std::map<uint64_t, std::shared_ptr<mf::DGPlan>> dgplans;   // per form uid (null: unsupported)
 };
 
 namespace mf {
@@ -241,6 +260,7 @@ struct OwnerShape { int nb = 0, maxp = 0; };
 OwnerShape owner_shape(const mf_form& f, const Integral& I);
 int64_t record_bytes(const mf_form& f, const Integral& I);
 std::string kernel_name(const char* what, int type, int index);
+bool dg_owner_emitted(const mf_form& f);
 // sparsity (sparsity.cpp): blocked pattern of all integrals
 void build_pattern(const mf_form& a, std::vector<int32_t>& rpb, std::vector<int32_t>& colsb);
 // matrix assembly paths (matrix.cpp)

@@ -9,7 +9,8 @@ from .assemble import CUDAAssembler
 from .bcs import CUDADirichletBC
 from .context import CUDAContext, DeviceArray, get_cuda_context
 from .form import CUDAForm, form
+from .graph import CUDAGraph
 from .la import CUDAMatrix, CUDAVector
 
 __all__ = ["CUDAAssembler", "CUDADirichletBC", "CUDAContext", "DeviceArray", "get_cuda_context",
-           "CUDAForm", "form", "CUDAMatrix", "CUDAVector"]
+           "CUDAForm", "form", "CUDAMatrix", "CUDAVector", "CUDAGraph"]

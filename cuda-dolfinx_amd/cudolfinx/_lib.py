@@ -60,6 +60,10 @@ SIGNATURES = {
     "mf_apply_lifting": (C.c_int, [C.POINTER(_vp), C.c_int, _vp, _i64, C.POINTER(_vp),
                                    C.POINTER(_vp), C.c_double, _vp]),
     "mf_set_bc": (C.c_int, [_vp, _vp, _i64, _vp, _vp, C.c_double, _vp]),
+    "mf_graph_begin": (C.c_int, [_vp, _vp]),
+    "mf_graph_end": (C.c_int, [_vp, _vp, _vp]),
+    "mf_graph_launch": (C.c_int, [_vp, _vp]),
+    "mf_graph_destroy": (C.c_int, [_vp]),
 }
 
 _lib = None

@@ -33,6 +33,7 @@ class CUDAAssembler:
     def __init__(self, stream=None):
         self._ctx = get_cuda_context()
         self._stream = _stream(stream)
+        self._no_bcs = None
 
     # ------------------------------------------------------------ creation
     def create_matrix(self, a: CUDAForm) -> CUDAMatrix:
@@ -52,6 +53,11 @@ class CUDAAssembler:
         if isinstance(bcs, CUDADirichletBC):
             return bcs
         if isinstance(bcs, (list, tuple)):
+            if not bcs:
+                # one reusable empty collection: no device allocation per call
+                if self._no_bcs is None:
+                    self._no_bcs = self.pack_bcs([])
+                return self._no_bcs
             return self.pack_bcs(list(bcs))
         raise TypeError(f"Expected either a list of DirichletBC's or a CUDADirichletBC, got '{type(bcs)}'")
 

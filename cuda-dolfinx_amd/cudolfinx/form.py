@@ -108,6 +108,7 @@ class CUDAForm:
                                              I.num_entities, staged))
         self._coef_dofmaps = []
         self._coef_versions = []
+        self._consts = None
         for f in form.coefficients:
             d = _create_dofmap_on_device(f.function_space)
             self._coef_dofmaps.append(d)
@@ -163,14 +164,18 @@ class CUDAForm:
         the last upload (Function.version), keeping them device-resident.
         """
         for j, f in enumerate(self._form.coefficients):
+            if not force and self._coef_versions[j] is None:
+                continue                      # bound to caller device memory
             if force or f.version != self._coef_versions[j]:
                 x = np.ascontiguousarray(f.x, dtype=np.float64)
                 check(lib().mf_form_set_coefficient(self._handle, j, x.ctypes.data_as(C.c_void_p),
                                                     x.size, 0))
                 self._coef_versions[j] = f.version
         c = np.ascontiguousarray(self._form.constant_values(), dtype=np.float64)
-        check(lib().mf_form_set_constants(self._handle, c.ctypes.data_as(C.POINTER(C.c_double)),
-                                          c.size))
+        if force or self._consts is None or not np.array_equal(c, self._consts):
+            check(lib().mf_form_set_constants(self._handle,
+                                              c.ctypes.data_as(C.POINTER(C.c_double)), c.size))
+            self._consts = c.copy()
 
     def set_coefficient_device(self, j: int, address: int, n: int) -> None:
         """Point coefficient j at caller-owned device memory (no copy)."""

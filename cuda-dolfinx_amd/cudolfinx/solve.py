@@ -11,14 +11,21 @@ import numpy as np
 
 
 def csr_tensor(A, values=None):
-    """torch sparse CSR view of a CUDAMatrix (values: torch tensor or None = copy)."""
+    """torch sparse CSR view of a CUDAMatrix (values: torch tensor or None = copy).
+
+    The index arrays are copied once per matrix and reused by later calls.
+    """
     import torch
-    rp, cols, vals = A.csr()
     dev = torch.device("cuda")
-    crow = torch.from_numpy(rp.astype(np.int64)).to(dev)
-    col = torch.from_numpy(cols.astype(np.int64)).to(dev)
-    v = values if values is not None else torch.from_numpy(vals).to(dev)
-    return torch.sparse_csr_tensor(crow, col, v, size=(A.nrows, A.ncols))
+    st = getattr(A, "_torch_structure", None)
+    if st is None or values is None:
+        rp, cols, vals = A.csr()
+        st = (torch.from_numpy(rp.astype(np.int64)).to(dev),
+              torch.from_numpy(cols.astype(np.int64)).to(dev))
+        A._torch_structure = st
+        if values is None:
+            values = torch.from_numpy(vals).to(dev)
+    return torch.sparse_csr_tensor(st[0], st[1], values, size=(A.nrows, A.ncols))
 
 
 def pcg(M, b, rtol: float = 1e-10, maxiter: int = 5000):

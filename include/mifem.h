@@ -36,6 +36,7 @@ typedef struct mf_dofmap mf_dofmap;
 typedef struct mf_form mf_form;
 typedef struct mf_matrix mf_matrix;
 typedef struct mf_bc mf_bc;
+typedef struct mf_graph mf_graph;
 typedef void* mf_stream;
 
 enum mf_status {
@@ -180,6 +181,20 @@ int mf_apply_lifting(mf_form* const* a, int nforms, double* d_b, int64_t n,
  * b[d] = scale * (g[d] - x0[d]) for the constrained dofs. */
 int mf_set_bc(mf_context* ctx, double* d_b, int64_t n, const mf_bc* bc, const double* d_x0,
               double scale, mf_stream stream);
+
+/* ------------------------------------------------------------ HIP graphs */
+/* No counterpart in the reference (it synchronises after every launch,
+ * CUDAFormIntegral.cpp:2354-2361).  For callers that re-assemble in a loop
+ * (Newton, time stepping): capture a sequence of the calls above on a
+ * non-default stream and replay it with one launch.  Between begin and end
+ * only device work may be enqueued on that stream: the first assembly of a
+ * form into a matrix builds its schedule on the host and must run before
+ * capture.  Coefficients bound with mf_form_set_coefficient(..., on_device=1)
+ * are re-read at every replay; host-side value changes are not seen. */
+int mf_graph_begin(mf_context* ctx, mf_stream stream);
+int mf_graph_end(mf_context* ctx, mf_stream stream, mf_graph** out);
+int mf_graph_launch(mf_graph* graph, mf_stream stream);
+int mf_graph_destroy(mf_graph* graph);
 
 #ifdef __cplusplus
 }

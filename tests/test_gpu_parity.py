@@ -206,10 +206,79 @@ def test_helmholtz_q3_matrix_core(prec, tol):
     assert ea <= tol and eb <= 1e-12, (ea, eb)
 
 
-@pytest.mark.parametrize("k", [1, 2])
-def test_sipg_plaplace_interior_facets(k):
-    _say(f"SIPG p-Laplacian DG{k} n=3 jittered: cells + Robin facets + interior facets")
+@pytest.mark.parametrize("k,kernel", [(1, 1), (1, 3), (2, 1), (2, 3)])
+def test_sipg_plaplace_interior_facets(k, kernel):
+    _say(f"SIPG p-Laplacian DG{k} n=3 jittered: cells + Robin facets + interior facets, "
+         f"kernel={'cell-owner' if kernel == 3 else 'per-entity'}")
     P = problems.dg_plaplace(3, degree=k)
     rng = np.random.default_rng(11)
     P.extras["u"].x[:] = rng.uniform(-1, 1, P.V.size)
-    _compare(P, _gpu(P), _oracle(P))
+    _compare(P, _gpu(P, {"matrix_kernel": kernel}, loptions={"vector_kernel": kernel}), _oracle(P))
+
+
+def test_sipg_cell_owner_with_dirichlet_and_accumulate():
+    _say("SIPG DG1 cell-owner kernel: Dirichlet rows/cols, diagonal 2.5, accumulate without zero")
+    P = problems.dg_plaplace(3)
+    rng = np.random.default_rng(12)
+    P.extras["u"].x[:] = rng.uniform(-1, 1, P.V.size)
+    dofs = np.arange(0, P.V.size, 7, dtype=np.int32)
+    g = Function(P.V)
+    g.x[:] = rng.random(P.V.size)
+    bc = dirichletbc(g, dofs, P.V)
+    ref = _oracle(P, [bc], diagonal=2.5)
+    got = _gpu(P, {"matrix_kernel": 3}, [bc], diagonal=2.5, loptions={"vector_kernel": 3})
+    _compare(P, got, ref)
+    a, L, asm, A, dbc = got[4]
+    asm.assemble_matrix(a, A, bcs=dbc, diagonal=2.5, zero=False)
+    asm.synchronize()
+    _, _, v2 = A.csr()
+    exp = 2.0 * ref[2]
+    rp, cols = ref[0], ref[1]
+    rows = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+    on = np.isin(rows, dofs) & (rows == cols)
+    exp[on] = 2.5
+    assert _rel(v2, exp) <= TOL
+
+
+def test_graph_replay_matches_eager_and_oracle():
+    _say("HIP graph: capture SIPG J + F (device-bound state u), 3 replays vs eager and oracle")
+    import torch
+    import cudolfinx as cufem
+    P = problems.dg_plaplace(3)
+    n = P.V.size
+    rng = np.random.default_rng(21)
+    s = torch.cuda.Stream()
+    J, F = cufem.form(P.a), cufem.form(P.L)
+    asm = cufem.CUDAAssembler(stream=s)
+    ud = torch.zeros(n, dtype=torch.float64, device="cuda")
+    J.set_coefficient_device(0, ud.data_ptr(), n)
+    F.set_coefficient_device(0, ud.data_ptr(), n)
+    A, b = asm.create_matrix(J), asm.create_vector(F)
+
+    def seq():
+        asm.assemble_matrix(J, A)
+        asm.assemble_vector(F, b)
+
+    seq()                                   # builds the schedules outside capture
+    asm.synchronize()
+    g = cufem.CUDAGraph(s)
+    with g:
+        seq()
+    oa, ol = cpu.OracleAssembler(P.a), cpu.OracleAssembler(P.L)
+    rp, cols, _, _ = oa.sparsity()
+    for rep in range(3):
+        u = rng.uniform(-1, 1, n)
+        ud.copy_(torch.from_numpy(u).cuda())
+        torch.cuda.synchronize()
+        g.launch()
+        asm.synchronize()
+        vg, bg = A.csr()[2].copy(), b.to_host().copy()
+        seq()
+        asm.synchronize()
+        ve, be = A.csr()[2], b.to_host()
+        assert np.array_equal(vg, ve) and np.array_equal(bg, be), f"replay {rep} differs from eager"
+        P.extras["u"].x[:] = u
+        oa, ol = cpu.OracleAssembler(P.a), cpu.OracleAssembler(P.L)
+        ea, eb = _rel(vg, oa.assemble_matrix(rp, cols)), _rel(bg, ol.assemble_vector())
+        _say(f"  replay {rep}: relerr A={ea:.3e} b={eb:.3e}")
+        assert ea <= TOL and eb <= TOL

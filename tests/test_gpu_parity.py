@@ -282,3 +282,23 @@ def test_graph_replay_matches_eager_and_oracle():
         ea, eb = _rel(vg, oa.assemble_matrix(rp, cols)), _rel(bg, ol.assemble_vector())
         _say(f"  replay {rep}: relerr A={ea:.3e} b={eb:.3e}")
         assert ea <= TOL and eb <= TOL
+
+
+def test_many_instances_block_separate():
+    _say("batch of 6 Poisson instances (P1-P3, n=2..4) merged per degree, owner-computes")
+    from mifem import batch
+    import cudolfinx as cufem
+    B = batch.many_poisson(6, seed=3, nmin=2, nmax=4)
+    for g in B.groups:
+        class _P:
+            pass
+        P = _P()
+        P.name, P.a, P.L, P.bcs, P.V = f"batch P{g.degree}", g.a, g.L, g.bcs, g.V
+        ref = _oracle(P)
+        got = _gpu(P)
+        _compare(P, got, ref)
+        for j in range(len(g.instances)):
+            r1, c1, v1 = g.block_csr(got[0], got[1], got[2], j)
+            r2, c2, v2 = g.block_csr(ref[0], ref[1], ref[2], j)
+            assert np.array_equal(r1, r2) and np.array_equal(c1, c2)
+            assert _rel(v1, v2) <= TOL

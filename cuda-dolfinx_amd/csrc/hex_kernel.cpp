@@ -14,10 +14,12 @@
 // accumulates the 16 x (k+1)^3 result rows in the LDS copy of the chunk's CSR
 // values (FP64, ds_add_f64), which is written out once.
 //
-// Precision variants (compile-time MF_HX_PREC): 64 = FP64 operands and
-// accumulation (v_mfma_f64_16x16x4f64), 32 = FP32 operands and accumulation
-// (v_mfma_f32_16x16x4f32), 16 = BF16 operands with FP32 accumulation
-// (v_mfma_f32_16x16x16bf16_1k).  Geometry (J, det, K), D_q, and the CSR
+// Precision variants (compile-time MF_HX_PREC): 32 = FP32 operands and
+// accumulation (v_mfma_f32_16x16x4f32), 16 = BF16 operands with FP32
+// accumulation (v_mfma_f32_16x16x16bf16_1k), 64 = FP64 on the vector ALUs
+// (same tiles and pair packing; on MI355X the FP64 vector and matrix peaks
+// are equal, and chained v_mfma_f64_16x16x4f64 accumulations gave wrong
+// results with this toolchain).  Geometry (J, det, K), D_q, and the CSR
 // accumulation are FP64 in all three.
 //
 // Operand layouts (16x16xK, lane l): A[row l%16][k-slot l/16], B[k-slot
@@ -197,13 +199,66 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
       const int b = (spair[pc] >> 8) - (ib - i0), i = spair[pc] & 255;
       const int ia = i % K1, ib2 = (i / K1) % K1, ic = i / (K1 * K1);
       const real* Db = &sD[b * NQ * DS];
+#if MF_HX_PREC == 64
+      // FP64 on the vector ALUs: lane = (tile row r, column block jc = lane / 16),
+      // 16 accumulators for the columns ja + 4 jb + 16 jc of that row.  Column
+      // factors B_b(q, j) = X_b(q1, ja) Y_b(q2, jb) Z_b(q3, jc) are folded in
+      // the order z, y, x so each quadrature point costs 2 FMAs per column.
+      const int jc = lane >> 4;
+      double cacc[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) cacc[k] = 0.0;
+      for (int q3 = 0; q3 < M1; ++q3) {
+        const double Lz = HX_L[q3][ic], Gz = HX_DL[q3][ic];
+        const double zl = HX_L[q3][jc], zd = HX_DL[q3][jc];
+        for (int q2 = 0; q2 < M1; ++q2) {
+          const double Ly = HX_L[q2][ib2], Gy = HX_DL[q2][ib2];
+          for (int q1 = 0; q1 < M1; ++q1) {
+            const double* D = &Db[(q1 + M1 * (q2 + M1 * q3)) * DS];
+            const double Lx = HX_L[q1][ia], Gx = HX_DL[q1][ia];
+            const double B0 = Lx * Ly * Lz, Bx = Gx * Ly * Lz, By = Lx * Gy * Lz, Bz = Lx * Ly * Gz;
+            double f0 = Bx * D[0] + By * D[1] + Bz * D[2];
+            double f1 = Bx * D[1] + By * D[3] + Bz * D[4];
+            double f2 = Bx * D[2] + By * D[4] + Bz * D[5];
+            double f3 = B0 * D[6];
+            if (!valid) f0 = f1 = f2 = f3 = 0.0;
+            // b = 0: X' Y Z, b = 1: X Y' Z, b = 2: X Y Z', b = 3: X Y Z
+            const double g0 = f0 * zl, g1 = f1 * zl, g2 = f2 * zd, g3 = f3 * zl;
+#pragma unroll
+            for (int jb = 0; jb < 4; ++jb) {
+              const double yl = HX_L[q2][jb], yd = HX_DL[q2][jb];
+              const double hd = g0 * yl;                       // multiplies X'
+              const double hl = g1 * yd + (g2 + g3) * yl;      // multiplies X
+#pragma unroll
+              for (int ja = 0; ja < 4; ++ja)
+                cacc[ja + 4 * jb] += hd * HX_DL[q1][ja] + hl * HX_L[q1][ja];
+            }
+          }
+        }
+      }
+      {
+        const int prr = t * 16 + r;
+        if (prr < np) {
+          const int pcc = pb + prr;
+          const int bq = (spair[pcc] >> 8) - (ib - i0), iq = spair[pcc] & 255;
+          const mf_hrec_$TAG& R = srec[bq];
+          if (!((R.bits[iq >> 5] >> (iq & 31)) & 1u)) {
+            const int lr = R.d[iq] - r0;
+            const int lo = lrp[lr], hi = lrp[lr + 1];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+              const int j = 16 * jc + jj;
+              if ((R.bits[j >> 5] >> (j & 31)) & 1u) continue;
+              const int kk = mf_lds_lower_bound(lcols, lo, hi, R.d[j]);
+              atomicAdd(&acc[kk], cacc[jj]);
+            }
+          }
+        }
+      }
+#else
       // column factors of this lane: j = 16 * tile + (lane % 16), tile = jc
       const int ja = (lane & 15) % K1, jb = (lane & 15) / K1;
-#if MF_HX_PREC == 64
-      typedef double acc4 __attribute__((ext_vector_type(4)));
-#else
       typedef float acc4 __attribute__((ext_vector_type(4)));
-#endif
       acc4 C[4];
       for (int jt = 0; jt < 4; ++jt) C[jt] = (acc4){0, 0, 0, 0};
 #if MF_HX_PREC == 16
@@ -259,11 +314,7 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
         for (int jt = 0; jt < 4; ++jt) {
           const double fz = slot == 2 ? HX_DL[q3][jt] : HX_L[q3][jt];
           const real bv = (real)(fxy * fz);
-#if MF_HX_PREC == 64
-          C[jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa, bv, C[jt], 0, 0, 0);
-#else
           C[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa, bv, C[jt], 0, 0, 0);
-#endif
         }
       }
 #endif
@@ -284,6 +335,7 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
           atomicAdd(&acc[kk], (double)C[jt][rr]);
         }
       }
+#endif
     }
     pb = pe;
   }

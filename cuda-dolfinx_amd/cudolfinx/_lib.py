@@ -80,6 +80,13 @@ def lib():
         if not LIB_PATH.exists():
             raise ImportError(f"libmifem.so not found at {LIB_PATH}; run __graft_entry__.build() "
                               f"or `make -C cuda-dolfinx_amd`")
+        try:
+            # torch's ROCm wheel ships its own HIP runtime; loading torch first
+            # makes libmifem bind to that same runtime instead of a second copy
+            # (two runtimes in one process cannot share the device)
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

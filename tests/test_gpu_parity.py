@@ -164,7 +164,7 @@ def test_accumulate_without_zero():
 def test_helmholtz_q3_generic_kernel():
     _say("helmholtz Q3 n=2 jittered (per-entity kernel)")
     P = problems.helmholtz_q3(2)
-    _compare(P, _gpu(P), _oracle(P))
+    _compare(P, _gpu(P, {"matrix_kernel": 1}), _oracle(P))
 
 
 def test_repeatability():

@@ -284,14 +284,17 @@ int mf_form_set_option(mf_form* f, const char* key, int64_t value) {
     static const char* known[] = {"matrix_kernel", "chunk_values", "chunk_rows", "block",
                                   "owner_block", "owner_nb", "owner_lds", "vector_kernel",
                                   "vector_block", "vector_chunk_values", "lift_kernel",
-                                  "owner_pairs", "precision", "dg_block"};
+                                  "owner_pairs", "precision", "dg_block", "owner_persistent",
+                                  "owner_waves"};
     bool ok = false;
     for (auto k : known) ok |= std::strcmp(k, key) == 0;
     MF_CHECK(ok, MF_EINVAL, std::string("unknown option ") + key);
     const bool runtime_key = std::strcmp(key, "matrix_kernel") == 0 ||
                              std::strcmp(key, "precision") == 0 ||
                              std::strcmp(key, "vector_kernel") == 0 ||
-                             std::strcmp(key, "lift_kernel") == 0;
+                             std::strcmp(key, "lift_kernel") == 0 ||
+                             std::strcmp(key, "owner_persistent") == 0 ||
+                             std::strcmp(key, "owner_waves") == 0;
     MF_CHECK(!f->compiled || runtime_key, MF_ESTATE,
              "kernel shape options must be set before compile");
     f->options[key] = value;

@@ -128,12 +128,18 @@ std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I,
   }
   const int64_t np = poff[nch];
   std::vector<int32_t> plist(std::max<int64_t>(1, np));
-  std::vector<uint8_t> pkk(with_kk ? std::max<int64_t>(1, np * nd) : 1);
+  // column offsets: one 16-byte aligned block of npairs * nd bytes per chunk
+  std::vector<int64_t> kkoff(nch + 1, 0);
+  for (int64_t c = 0; c < nch; ++c)
+    kkoff[c + 1] = kkoff[c] + ((poff[c + 1] - poff[c]) * nd + 15) / 16 * 16;
+  MF_CHECK(kkoff[nch] < INT32_MAX, MF_EUNSUPPORTED, "column offset table exceeds int32 indexing");
+  std::vector<uint8_t> pkk(with_kk ? std::max<int64_t>(16, kkoff[nch]) : 16, 0);
   std::vector<int> bad(16, 0);
   unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   auto work = [&](unsigned t) {
     for (int64_t c = t; c < nch; c += nt) {
       int64_t p = poff[c];
+      uint8_t* kkc = pkk.data() + kkoff[c] - poff[c] * nd;   // pair p of chunk c at kkc[p * nd]
       for (int32_t it = ci[c]; it < ci[c + 1]; ++it) {
         const int32_t cell = plan->h_cell[it];
         uint64_t msk = plan->h_mask[it];
@@ -153,9 +159,9 @@ std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I,
             const int32_t* f = std::lower_bound(rb, re, Cn);
             if (f == re || *f != Cn || f - rb > 255) {
               bad[t % 16] = 1;
-              pkk[p * nd + j] = 0;
+              kkc[p * nd + j] = 0;
             } else {
-              pkk[p * nd + j] = static_cast<uint8_t>(f - rb);
+              kkc[p * nd + j] = static_cast<uint8_t>(f - rb);
             }
           }
           ++p;
@@ -179,6 +185,7 @@ std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I,
     mc[4] = static_cast<int32_t>(cb0); mc[5] = static_cast<int32_t>(cb1 - cb0);
     mc[6] = static_cast<int32_t>(base); mc[7] = static_cast<int32_t>(bs2 * (cb1 - cb0));
     mc[8] = static_cast<int32_t>(poff[c]); mc[9] = static_cast<int32_t>(poff[c + 1]);
+    mc[10] = static_cast<int32_t>(kkoff[c]);
   }
   plan->chunk_meta.upload(meta);
   plan->pair_list.upload(plist);
@@ -414,8 +421,20 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
     launch(owner->k_prep, static_cast<uint32_t>((owner->n + 255) / 256), 256, 0, s,
            static_cast<int32_t>(owner->n), owner->d_ents.ptr, a->mesh->cells.ptr, a->mesh->x.ptr,
            a->packed.ptr, a->dm[0]->dev.ptr, n0, owner->records.ptr);
-    launch(owner->k_owner, static_cast<uint32_t>(per * nx),
-           static_cast<uint32_t>(a->opt("owner_block", 256)), 0, s,
+    // persistent grid: as many workgroups as fit on the device at once
+    // (multiple of 8, one group of chunks per XCD); owner_persistent=0 gives
+    // one workgroup per chunk
+    const uint32_t oblock = static_cast<uint32_t>(a->opt("owner_block", 256));
+    int64_t grid = per * nx;
+    if (a->opt("owner_persistent", 1)) {
+      if (owner->owner_grid == 0) {
+        int nblk = 0;
+        MF_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, owner->k_owner, oblock, 0));
+        owner->owner_grid = std::max<int64_t>(nx, (int64_t)std::max(1, nblk) * a->ctx->num_cus / nx * nx);
+      }
+      grid = std::min<int64_t>(grid, owner->owner_grid * a->opt("owner_waves", 1));
+    }
+    launch(owner->k_owner, static_cast<uint32_t>(grid), oblock, 0, s,
            static_cast<int32_t>(plan->nchunks), plan->chunk_meta.ptr, plan->item_cell.ptr,
            plan->pair_list.ptr, plan->pair_kk.ptr, owner->records.ptr, a->consts.ptr, A->rpb.ptr,
            A->colsb.ptr, n0, A->values, diagonal, flags);

@@ -179,6 +179,7 @@ struct Integral {
   hipFunction_t k_vown = nullptr, k_slift = nullptr, k_prep = nullptr;
   hipFunction_t k_hprep = nullptr, k_howner[3] = {nullptr, nullptr, nullptr};  // FP64, FP32, BF16
   DevArray<double> records;           // per-cell records of the owner-computes kernel
+  int64_t owner_grid = 0;             // resident workgroups of the owner kernel (0: not queried)
   int64_t record_bytes = 0;
   // lifting entity lists per bc object uid
   std::map<uint64_t, std::pair<DevArray<int32_t>, int64_t>> lift_lists;

@@ -121,165 +121,277 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
        double* __restrict__ values, const double diag, const int flags)
 {
   // chunk_meta[12k ..]: r0, r1, i0, i1, cb0 (= rpb[r0]), ncb, base (CSR offset),
-  //   seglen, p0, p1 (pair range), 0, 0.
-  // pair_list[p] = (item offset in chunk) << 8 | local row i; pair_kk[p*ND + j] =
-  //   position of column node j within the blocked row of node i.
+  //   seglen, p0, p1 (pair range), kko (byte offset of the chunk's column
+  //   offsets, 16-byte aligned), 0.
+  // pair_list[p] = (item offset in chunk) << 8 | local row i; pair_kk[kko + (p-p0)*ND + j]
+  //   = position of column node j within the blocked row of node i.
   // flags: 1 overwrite (zero) instead of add, 2 set the diagonal of Dirichlet rows.
+  //
+  // Persistent schedule: the grid holds G workgroups (a multiple of 8); the
+  // workgroups of XCD x = blockIdx % 8 walk the x-th contiguous eighth of the
+  // chunks with stride G / 8, so neighbouring chunks (which share cells) run
+  // at the same time on one XCD's L2.  While a chunk's values stream out, the
+  // next chunk's metadata, pair list, column offsets, row pointers, columns
+  // and first cell ids are already being loaded into registers.  With G = 8 *
+  // ceil(nchunks / 8) every workgroup owns exactly one chunk.
   constexpr int ND = SK_ND, BS = SK_BS, NQ = SK_NQ;
-  constexpr int NB = $ONB, CAP = $OCAP, MAXR = $OMAXR, MAXP = $OMAXP;
+  constexpr int NB = $ONB, CAP = $OCAP, MAXR = $OMAXR, MAXP = $OMAXP, NT = $OBLOCK;
   constexpr int CAPB = CAP / (BS * BS) + 1;
   constexpr int RW = sizeof(mf_rec_$TAG) / 16;      // record size in 16-byte words
+  constexpr int KK4 = (MAXP * ND + 15) / 16;         // column-offset words (16 bytes)
+  constexpr int PF_P = (MAXP + NT - 1) / NT, PF_K = (KK4 + NT - 1) / NT;
+  constexpr int PF_R = (MAXR + 1 + NT - 1) / NT, PF_C = (CAPB + NT - 1) / NT;
+  constexpr int PF_S = (NB + NT - 1) / NT;
   __shared__ __attribute__((aligned(16))) double acc[CAP];
   __shared__ int32_t lrp[MAXR + 1];
   __shared__ int32_t lcols[CAPB];
   __shared__ mf_rec_$TAG srec[NB];
   __shared__ int32_t scell[NB];
   __shared__ int32_t spair[MAXP];
-  __shared__ __attribute__((aligned(16))) uint8_t skk[MAXP * ND + 16];
+  __shared__ __attribute__((aligned(16))) uint8_t skk[KK4 * 16];
 
-  const int k = mf_xcd_chunk(nchunks);
-  if (k < 0) return;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int4 m0 = reinterpret_cast<const int4*>(chunk_meta)[3 * k];
-  const int4 m1 = reinterpret_cast<const int4*>(chunk_meta)[3 * k + 1];
-  const int4 m2 = reinterpret_cast<const int4*>(chunk_meta)[3 * k + 2];
-  const int32_t r0 = m0.x, r1 = m0.y, i0 = m0.z, i1 = m0.w;
-  const int32_t cb0 = m1.x, ncb = m1.y, base = m1.z, seglen = m1.w;
-  const int32_t p0 = m2.x, p1 = m2.y;
-  const int nr = r1 - r0;
-  const int npc = p1 - p0;
-  // chunk set-up: cell ids, pair list and column offsets, zero accumulator
-  for (int b = tid; b < i1 - i0 && b < NB; b += nt) scell[b] = item_cell[i0 + b];
-  for (int p = tid; p < npc; p += nt) spair[p] = pair_list[p0 + p];
-  for (int p = tid; p < npc * ND; p += nt) skk[p] = pair_kk[(int64_t)p0 * ND + p];
+  const int tid = threadIdx.x, nt = NT;
+  const int nx = 8, gx = gridDim.x / nx;
+  const int per = (nchunks + nx - 1) / nx;
+  const int xcd = blockIdx.x % nx;
+  int jj = blockIdx.x / nx;
+  int k = xcd * per + jj;
+  if (jj >= per || k >= nchunks) return;
+
+  int4 m0 = reinterpret_cast<const int4*>(chunk_meta)[3 * k];
+  int4 m1 = reinterpret_cast<const int4*>(chunk_meta)[3 * k + 1];
+  int4 m2 = reinterpret_cast<const int4*>(chunk_meta)[3 * k + 2];
+  // first chunk: stage its lists, zero the whole accumulator once (each
+  // write-out re-zeroes what it read, so acc is zero at every chunk start)
+  for (int b = tid; b < m0.w - m0.z && b < NB; b += nt) scell[b] = item_cell[m0.z + b];
+  for (int p = tid; p < m2.y - m2.x; p += nt) spair[p] = pair_list[m2.x + p];
+  {
+    const int n4 = ((m2.y - m2.x) * ND + 15) / 16;
+    const int4* src = reinterpret_cast<const int4*>(pair_kk + (uint32_t)m2.z);
+    for (int p = tid; p < n4; p += nt) reinterpret_cast<int4*>(skk)[p] = src[p];
+  }
   {
     mf_d2* a2 = reinterpret_cast<mf_d2*>(acc);
     const mf_d2 z = {0.0, 0.0};
-    for (int p = tid; p < (seglen + 1) / 2; p += nt) a2[p] = z;
+    for (int p = tid; p < CAP / 2; p += nt) a2[p] = z;
   }
-  for (int p = tid; p <= nr; p += nt) lrp[p] = rpb[r0 + p] - cb0;
-  for (int p = tid; p < ncb; p += nt) lcols[p] = colsb[cb0 + p];
+  for (int p = tid; p <= m0.y - m0.x; p += nt) lrp[p] = rpb[m0.x + p] - m1.x;
+  for (int p = tid; p < m1.y; p += nt) lcols[p] = colsb[m1.x + p];
 
-  int pb = 0;                                        // first pair of the batch
-  for (int ib = i0; ib < i1; ib += NB) {
-    const int nb = min(NB, i1 - ib);
-    if (ib > i0) {
-      __syncthreads();                               // previous batch fully consumed
-      for (int b = tid; b < nb; b += nt) scell[b] = item_cell[ib + b];
-    }
-    __syncthreads();
-    // stage 1: copy the cell records, 16 bytes per thread
-    {
-      const int4* src = reinterpret_cast<const int4*>(rec);
-      int4* dst = reinterpret_cast<int4*>(srec);
-      for (int u = tid; u < nb * RW; u += nt) {
-        const int b = u / RW, w = u - b * RW;
-        dst[b * RW + w] = src[(int64_t)scell[b] * RW + w];
+  while (true) {
+    const int32_t r0 = m0.x, r1 = m0.y, i0 = m0.z, i1 = m0.w;
+    const int32_t base = m1.z, seglen = m1.w;
+    const int nr = r1 - r0;
+    const int npc = m2.y - m2.x;
+
+    int pb = 0;                                        // first pair of the batch
+    for (int ib = i0; ib < i1; ib += NB) {
+      const int nb = min(NB, i1 - ib);
+      if (ib > i0) {
+        __syncthreads();                               // previous batch fully consumed
+        for (int b = tid; b < nb; b += nt) scell[b] = item_cell[ib + b];
       }
-    }
-    // pairs of this batch: those of items [ib - i0, ib - i0 + nb) of the chunk
-    int pe;
-    {
-      int lo2 = pb, hi2 = npc;
-      const int lim = ib - i0 + nb;
-      while (lo2 < hi2) {
-        const int mid = (lo2 + hi2) >> 1;
-        if ((spair[mid] >> 8) < lim) lo2 = mid + 1;
-        else hi2 = mid;
-      }
-      pe = lo2;
-    }
-    __syncthreads();
-    // stage 2: one thread per (pair, column range); lanes of a wave share the
-    // column range, so the column basis tables are wave-uniform.
-    const int np = pe - pb;
-    const int waves = nt >> 6;
-    int js = (waves * 64) / max(np, 1);
-    js = js < 1 ? 1 : (js > ND ? ND : js);
-    const int jw = (ND + js - 1) / js;
-    const int units = ((np + 63) >> 6) * 64 * js;
-    for (int u = tid; u < units; u += nt) {
-      const int lane = u & 63, wv = u >> 6;
-      const int part = __builtin_amdgcn_readfirstlane(wv % js);
-      const int pr = (wv / js) * 64 + lane;
-      if (pr >= np) continue;
-      const int pc = pb + pr;                        // pair index within the chunk
-      const int b = (spair[pc] >> 8) - (ib - i0), i = spair[pc] & 255;
-      const mf_rec_$TAG& r = srec[b];
-      const double K0 = r.K[0], K1 = r.K[1], K2 = r.K[2], K3 = r.K[3], K4 = r.K[4], K5 = r.K[5];
-      const double K6 = r.K[6], K7 = r.K[7], K8 = r.K[8], dt = r.det;
-      double gi[NQ][3];
-      for (int q = 0; q < NQ; ++q) {
-        const double s = SK_W[q] * dt;
-        const double d0 = SK_DPHI[q][i][0], d1v = SK_DPHI[q][i][1], d2 = SK_DPHI[q][i][2];
-        gi[q][0] = s * (K0 * d0 + K3 * d1v + K6 * d2);
-        gi[q][1] = s * (K1 * d0 + K4 * d1v + K7 * d2);
-        gi[q][2] = s * (K2 * d0 + K5 * d1v + K8 * d2);
-      }
-      const uint32_t bits = r.bits;
-      const uint32_t rb0 = (bits >> (BS * i)) & ((1u << BS) - 1);
-      const int lr = r.d[i] - r0;
-      const int lo = lrp[lr], hi = lrp[lr + 1];
-      const int rowlen = BS * (hi - lo);
-      const int rbase = BS * BS * lo;
-      const uint8_t* kk = &skk[pc * ND];
-      const int j0 = part * jw, j1 = min(ND, j0 + jw);
-      for (int j = j0; j < j1; ++j) {
-        double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int q = 0; q < NQ; ++q) {
-          const double d0 = SK_DPHI[q][j][0], d1v = SK_DPHI[q][j][1], d2 = SK_DPHI[q][j][2];
-          const double c0 = K0 * d0 + K3 * d1v + K6 * d2;
-          const double c1 = K1 * d0 + K4 * d1v + K7 * d2;
-          const double c2 = K2 * d0 + K5 * d1v + K8 * d2;
-          G[0] += gi[q][0] * c0; G[1] += gi[q][0] * c1; G[2] += gi[q][0] * c2;
-          G[3] += gi[q][1] * c0; G[4] += gi[q][1] * c1; G[5] += gi[q][1] * c2;
-          G[6] += gi[q][2] * c0; G[7] += gi[q][2] * c1; G[8] += gi[q][2] * c2;
+      __syncthreads();
+      // stage 1: copy the cell records, 16 bytes per thread
+      {
+        const int4* src = reinterpret_cast<const int4*>(rec);
+        int4* dst = reinterpret_cast<int4*>(srec);
+        for (int u = tid; u < nb * RW; u += nt) {
+          const int b = u / RW, w = u - b * RW;
+          dst[b * RW + w] = src[(int64_t)scell[b] * RW + w];
         }
-        double blk[BS * BS];
-        sk_combine(G, r.w, consts, blk);
-        const uint32_t cb1 = (bits >> (BS * j)) & ((1u << BS) - 1);
-        const int rb = rbase + BS * kk[j];
-        if ((rb0 | cb1) == 0) {
-          for (int a = 0; a < BS; ++a)
-            for (int bb = 0; bb < BS; ++bb) atomicAdd(&acc[rb + a * rowlen + bb], blk[a * BS + bb]);
-        } else {
-          for (int a = 0; a < BS; ++a) {
-            if (rb0 & (1u << a)) continue;
-            for (int bb = 0; bb < BS; ++bb) {
-              if (cb1 & (1u << bb)) continue;
-              atomicAdd(&acc[rb + a * rowlen + bb], blk[a * BS + bb]);
+      }
+      // pairs of this batch: those of items [ib - i0, ib - i0 + nb) of the chunk
+      int pe;
+      {
+        int lo2 = pb, hi2 = npc;
+        const int lim = ib - i0 + nb;
+        while (lo2 < hi2) {
+          const int mid = (lo2 + hi2) >> 1;
+          if ((spair[mid] >> 8) < lim) lo2 = mid + 1;
+          else hi2 = mid;
+        }
+        pe = lo2;
+      }
+      __syncthreads();
+      // stage 2: one thread per (pair, column range); lanes of a wave share the
+      // column range, so the column basis tables are wave-uniform.
+      const int np = pe - pb;
+      const int waves = nt >> 6;
+      int js = (waves * 64) / max(np, 1);
+      js = js < 1 ? 1 : (js > ND ? ND : js);
+      const int jw = (ND + js - 1) / js;
+      const int units = ((np + 63) >> 6) * 64 * js;
+      for (int u = tid; u < units; u += nt) {
+        const int lane = u & 63, wv = u >> 6;
+        const int part = __builtin_amdgcn_readfirstlane(wv % js);
+        const int pr = (wv / js) * 64 + lane;
+        if (pr >= np) continue;
+        const int pc = pb + pr;                        // pair index within the chunk
+        const int b = (spair[pc] >> 8) - (ib - i0), i = spair[pc] & 255;
+        const mf_rec_$TAG& r = srec[b];
+        const double K0 = r.K[0], K1 = r.K[1], K2 = r.K[2], K3 = r.K[3], K4 = r.K[4], K5 = r.K[5];
+        const double K6 = r.K[6], K7 = r.K[7], K8 = r.K[8], dt = r.det;
+        double gi[NQ][3];
+        for (int q = 0; q < NQ; ++q) {
+          const double s = SK_W[q] * dt;
+          const double d0 = SK_DPHI[q][i][0], d1v = SK_DPHI[q][i][1], d2 = SK_DPHI[q][i][2];
+          gi[q][0] = s * (K0 * d0 + K3 * d1v + K6 * d2);
+          gi[q][1] = s * (K1 * d0 + K4 * d1v + K7 * d2);
+          gi[q][2] = s * (K2 * d0 + K5 * d1v + K8 * d2);
+        }
+        const uint32_t bits = r.bits;
+        const uint32_t rb0 = (bits >> (BS * i)) & ((1u << BS) - 1);
+        const int lr = r.d[i] - r0;
+        const int lo = lrp[lr], hi = lrp[lr + 1];
+        const int rowlen = BS * (hi - lo);
+        const int rbase = BS * BS * lo;
+        const uint8_t* kk = &skk[pc * ND];
+        const int j0 = part * jw, j1 = min(ND, j0 + jw);
+        for (int j = j0; j < j1; ++j) {
+          double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+          for (int q = 0; q < NQ; ++q) {
+            const double d0 = SK_DPHI[q][j][0], d1v = SK_DPHI[q][j][1], d2 = SK_DPHI[q][j][2];
+            const double c0 = K0 * d0 + K3 * d1v + K6 * d2;
+            const double c1 = K1 * d0 + K4 * d1v + K7 * d2;
+            const double c2 = K2 * d0 + K5 * d1v + K8 * d2;
+            G[0] += gi[q][0] * c0; G[1] += gi[q][0] * c1; G[2] += gi[q][0] * c2;
+            G[3] += gi[q][1] * c0; G[4] += gi[q][1] * c1; G[5] += gi[q][1] * c2;
+            G[6] += gi[q][2] * c0; G[7] += gi[q][2] * c1; G[8] += gi[q][2] * c2;
+          }
+          double blk[BS * BS];
+          sk_combine(G, r.w, consts, blk);
+          const uint32_t cb1 = (bits >> (BS * j)) & ((1u << BS) - 1);
+          const int rb = rbase + BS * kk[j];
+          if ((rb0 | cb1) == 0) {
+            for (int a = 0; a < BS; ++a)
+              for (int bb = 0; bb < BS; ++bb) atomicAdd(&acc[rb + a * rowlen + bb], blk[a * BS + bb]);
+          } else {
+            for (int a = 0; a < BS; ++a) {
+              if (rb0 & (1u << a)) continue;
+              for (int bb = 0; bb < BS; ++bb) {
+                if (cb1 & (1u << bb)) continue;
+                atomicAdd(&acc[rb + a * rowlen + bb], blk[a * BS + bb]);
+              }
             }
           }
         }
       }
-    }
-    pb = pe;
-  }
-  __syncthreads();
-  if ((flags & 2) && nb0) {
-    // Dirichlet rows: set the diagonal entries of the constrained components
-    for (int lr = tid; lr < nr; lr += nt) {
-      const int32_t R = r0 + lr;
-      const uint32_t rbits = nb0[R];
-      if (!rbits) continue;
-      const int lo = lrp[lr], hi = lrp[lr + 1];
-      const int p = mf_lds_lower_bound(lcols, lo, hi, R);
-      if (p < hi && lcols[p] == R)
-        for (int a = 0; a < BS; ++a)
-          if (rbits & (1u << a)) acc[BS * BS * lo + a * BS * (hi - lo) + BS * (p - lo) + a] = diag;
+      pb = pe;
     }
     __syncthreads();
-  }
-  double* out = values + (int64_t)(uint32_t)base;
-  if ((flags & 1) && ((base & 1) == 0)) {
-    const mf_d2* a2 = reinterpret_cast<const mf_d2*>(acc);
-    mf_d2* o2 = reinterpret_cast<mf_d2*>(out);
-    for (int p = tid; p < seglen / 2; p += nt) __builtin_nontemporal_store(a2[p], &o2[p]);
-    if ((seglen & 1) && tid == 0) out[seglen - 1] = acc[seglen - 1];
-  } else if (flags & 1) {
-    for (int p = tid; p < seglen; p += nt) __builtin_nontemporal_store(acc[p], &out[p]);
-  } else {
-    for (int p = tid; p < seglen; p += nt) out[p] += acc[p];
+    if ((flags & 2) && nb0) {
+      // Dirichlet rows: set the diagonal entries of the constrained components
+      for (int lr = tid; lr < nr; lr += nt) {
+        const int32_t R = r0 + lr;
+        const uint32_t rbits = nb0[R];
+        if (!rbits) continue;
+        const int lo = lrp[lr], hi = lrp[lr + 1];
+        const int p = mf_lds_lower_bound(lcols, lo, hi, R);
+        if (p < hi && lcols[p] == R)
+          for (int a = 0; a < BS; ++a)
+            if (rbits & (1u << a)) acc[BS * BS * lo + a * BS * (hi - lo) + BS * (p - lo) + a] = diag;
+      }
+      __syncthreads();
+    }
+
+    // prefetch the next chunk's lists into registers
+    const int jn = jj + gx, kn = xcd * per + jn;
+    const bool more = jn < per && kn < nchunks;
+    int4 n0 = {0, 0, 0, 0}, n1 = {0, 0, 0, 0}, n2 = {0, 0, 0, 0};
+    int pf_p[PF_P], pf_r[PF_R], pf_c[PF_C], pf_s[PF_S];
+    int4 pf_k[PF_K];
+    if (more) {
+      n0 = reinterpret_cast<const int4*>(chunk_meta)[3 * kn];
+      n1 = reinterpret_cast<const int4*>(chunk_meta)[3 * kn + 1];
+      n2 = reinterpret_cast<const int4*>(chunk_meta)[3 * kn + 2];
+      const int nnp = n2.y - n2.x, nk4 = (nnp * ND + 15) / 16;
+      const int4* ksrc = reinterpret_cast<const int4*>(pair_kk + (uint32_t)n2.z);
+#pragma unroll
+      for (int t = 0; t < PF_P; ++t) {
+        const int p = tid + t * NT;
+        pf_p[t] = p < nnp ? pair_list[n2.x + p] : 0;
+      }
+#pragma unroll
+      for (int t = 0; t < PF_K; ++t) {
+        const int p = tid + t * NT;
+        pf_k[t] = p < nk4 ? ksrc[p] : make_int4(0, 0, 0, 0);
+      }
+#pragma unroll
+      for (int t = 0; t < PF_R; ++t) {
+        const int p = tid + t * NT;
+        pf_r[t] = p <= n0.y - n0.x ? rpb[n0.x + p] - n1.x : 0;
+      }
+#pragma unroll
+      for (int t = 0; t < PF_C; ++t) {
+        const int p = tid + t * NT;
+        pf_c[t] = p < n1.y ? colsb[n1.x + p] : 0;
+      }
+#pragma unroll
+      for (int t = 0; t < PF_S; ++t) {
+        const int p = tid + t * NT;
+        pf_s[t] = p < n0.w - n0.z && p < NB ? item_cell[n0.z + p] : 0;
+      }
+    }
+
+    // write the segment once and re-zero what was read
+    double* out = values + (int64_t)(uint32_t)base;
+    mf_d2* a2 = reinterpret_cast<mf_d2*>(acc);
+    const mf_d2 z = {0.0, 0.0};
+    if ((flags & 1) && ((base & 1) == 0)) {
+      mf_d2* o2 = reinterpret_cast<mf_d2*>(out);
+      for (int p = tid; p < seglen / 2; p += nt) {
+        __builtin_nontemporal_store(a2[p], &o2[p]);
+        a2[p] = z;
+      }
+      if ((seglen & 1) && tid == 0) {
+        out[seglen - 1] = acc[seglen - 1];
+        acc[seglen - 1] = 0.0;
+      }
+    } else if (flags & 1) {
+      for (int p = tid; p < seglen; p += nt) {
+        __builtin_nontemporal_store(acc[p], &out[p]);
+        acc[p] = 0.0;
+      }
+    } else {
+      for (int p = tid; p < seglen; p += nt) {
+        out[p] += acc[p];
+        acc[p] = 0.0;
+      }
+    }
+    if (!more) break;
+    __syncthreads();                                   // lists of this chunk no longer read
+    {
+      const int nnp = n2.y - n2.x, nk4 = (nnp * ND + 15) / 16;
+#pragma unroll
+      for (int t = 0; t < PF_P; ++t) {
+        const int p = tid + t * NT;
+        if (p < nnp) spair[p] = pf_p[t];
+      }
+#pragma unroll
+      for (int t = 0; t < PF_K; ++t) {
+        const int p = tid + t * NT;
+        if (p < nk4) reinterpret_cast<int4*>(skk)[p] = pf_k[t];
+      }
+#pragma unroll
+      for (int t = 0; t < PF_R; ++t) {
+        const int p = tid + t * NT;
+        if (p <= n0.y - n0.x) lrp[p] = pf_r[t];
+      }
+#pragma unroll
+      for (int t = 0; t < PF_C; ++t) {
+        const int p = tid + t * NT;
+        if (p < n1.y) lcols[p] = pf_c[t];
+      }
+#pragma unroll
+      for (int t = 0; t < PF_S; ++t) {
+        const int p = tid + t * NT;
+        if (p < n0.w - n0.z && p < NB) scell[p] = pf_s[t];
+      }
+    }
+    m0 = n0; m1 = n1; m2 = n2;
+    jj = jn;
+    k = kn;
   }
 }
 )";

@@ -312,7 +312,11 @@ int mf_form_compile(mf_form* f, int verbose) {
     }
     if (f->W > 0) f->packed.alloc(f->mesh->num_cells * (int64_t)f->W);
     // owner-kernel shape: LDS budget (doubles) and cells per batch
-    int64_t cap = f->opt("chunk_values", 4096);
+    // scalar forms on the reference-tensor path keep a (nd x nd x 6) table in
+    // LDS: give them a 2048-value chunk unless the caller chose one
+    bool trace = false;
+    for (auto& I : f->ints) trace |= mf::staged_trace(*I);
+    int64_t cap = f->opt("chunk_values", trace ? 2048 : 4096);
     f->options["chunk_values"] = cap;
     if (!f->options.count("owner_nb")) f->options["owner_nb"] = 32;
     if (!f->options.count("chunk_rows")) f->options["chunk_rows"] = 64;

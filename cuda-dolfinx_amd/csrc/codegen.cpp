@@ -200,6 +200,10 @@ int64_t record_bytes(const mf_form& f, const Integral& I) {
   return (raw + 15) / 16 * 16;
 }
 
+bool staged_trace(const Integral& I) {
+  return I.staged_src.find("#define SK_TRACE ") != std::string::npos;
+}
+
 bool is_hex_staged(const Integral& I) {
   return I.staged_src.find("#define HX_K ") != std::string::npos;
 }
@@ -241,7 +245,8 @@ OwnerShape owner_shape(const mf_form& f, const Integral& I) {
   OwnerShape sh;
   sh.nb = static_cast<int>(std::min<int64_t>(f.opt("owner_nb", 64), 1023));
   sh.maxp = static_cast<int>(f.opt("owner_pairs", 384));
-  const int64_t used = fixed + sh.nb * (record_bytes(f, I) + 4) + (int64_t)sh.maxp * (4 + nd);
+  const int64_t trace = staged_trace(I) ? 8LL * nd * nd * 6 : 0;
+  const int64_t used = fixed + trace + sh.nb * (record_bytes(f, I) + 4) + (int64_t)sh.maxp * (4 + nd);
   MF_CHECK(used <= budget, MF_EUNSUPPORTED,
            "owner-computes kernel shape exceeds the LDS budget (" + std::to_string(used) + " > " +
                std::to_string(budget) + " bytes)");
@@ -378,7 +383,8 @@ std::string generate_form_source(const mf_form& f) {
         // namespace the staged symbols per integral
         std::string pfx = "I" + std::to_string(I->index) + "_";
         auto rename = [&](std::string t) {
-          for (const char* sym : {"SK_ND", "SK_BS", "SK_NQ", "SK_W", "SK_DPHI", "sk_combine"})
+          for (const char* sym : {"SK_ND", "SK_BS", "SK_NQ", "SK_W", "SK_DPHI", "SK_TRACE", "SK_RS",
+                                  "sk_combine"})
             replace_all(t, sym, pfx + sym);
           return t;
         };

@@ -255,6 +255,7 @@ namespace mf {
 std::string generate_form_source(const mf_form& f);
 bool owner_supported(const mf_form& f, const Integral& I);
 bool is_hex_staged(const Integral& I);
+bool staged_trace(const Integral& I);
 bool hex_owner_supported(const mf_form& f, const Integral& I);
 int hex_batch(const mf_form& f, const Integral& I, int prec);
 struct OwnerShape { int nb = 0, maxp = 0; };

@@ -149,6 +149,11 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
   __shared__ int32_t scell[NB];
   __shared__ int32_t spair[MAXP];
   __shared__ __attribute__((aligned(16))) uint8_t skk[KK4 * 16];
+#ifdef SK_TRACE
+  // reference tensors R_ij (symmetrised, 6 per entry), loaded once per workgroup
+  __shared__ double sRS[ND * ND * 6];
+  for (int p = threadIdx.x; p < ND * ND * 6; p += NT) sRS[p] = (&SK_RS[0][0][0])[p];
+#endif
 
   const int tid = threadIdx.x, nt = NT;
   const int nx = 8, gx = gridDim.x / nx;
@@ -232,6 +237,12 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
         const mf_rec_$TAG& r = srec[b];
         const double K0 = r.K[0], K1 = r.K[1], K2 = r.K[2], K3 = r.K[3], K4 = r.K[4], K5 = r.K[5];
         const double K6 = r.K[6], K7 = r.K[7], K8 = r.K[8], dt = r.det;
+#ifdef SK_TRACE
+        // metric M = |det J| K K^T (K[3 r + c] = dX_r / dx_c): tr G_ij = sum_rs M_rs R_ij[r][s]
+        const double M00 = dt * (K0 * K0 + K1 * K1 + K2 * K2), M11 = dt * (K3 * K3 + K4 * K4 + K5 * K5);
+        const double M22 = dt * (K6 * K6 + K7 * K7 + K8 * K8), M01 = dt * (K0 * K3 + K1 * K4 + K2 * K5);
+        const double M02 = dt * (K0 * K6 + K1 * K7 + K2 * K8), M12 = dt * (K3 * K6 + K4 * K7 + K5 * K8);
+#else
         double gi[NQ][3];
         for (int q = 0; q < NQ; ++q) {
           const double s = SK_W[q] * dt;
@@ -240,6 +251,7 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
           gi[q][1] = s * (K1 * d0 + K4 * d1v + K7 * d2);
           gi[q][2] = s * (K2 * d0 + K5 * d1v + K8 * d2);
         }
+#endif
         const uint32_t bits = r.bits;
         const uint32_t rb0 = (bits >> (BS * i)) & ((1u << BS) - 1);
         const int lr = r.d[i] - r0;
@@ -250,6 +262,12 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
         const int j0 = part * jw, j1 = min(ND, j0 + jw);
         for (int j = j0; j < j1; ++j) {
           double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#ifdef SK_TRACE
+          {
+            const double* R = &sRS[(i * ND + j) * 6];
+            G[0] = M00 * R[0] + M11 * R[1] + M22 * R[2] + M01 * R[3] + M02 * R[4] + M12 * R[5];
+          }
+#else
           for (int q = 0; q < NQ; ++q) {
             const double d0 = SK_DPHI[q][j][0], d1v = SK_DPHI[q][j][1], d2 = SK_DPHI[q][j][2];
             const double c0 = K0 * d0 + K3 * d1v + K6 * d2;
@@ -259,6 +277,7 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
             G[3] += gi[q][1] * c0; G[4] += gi[q][1] * c1; G[5] += gi[q][1] * c2;
             G[6] += gi[q][2] * c0; G[7] += gi[q][2] * c1; G[8] += gi[q][2] * c2;
           }
+#endif
           double blk[BS * BS];
           sk_combine(G, r.w, consts, blk);
           const uint32_t cb1 = (bits >> (BS * j)) & ((1u << BS) - 1);

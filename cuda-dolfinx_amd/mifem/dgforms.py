@@ -9,11 +9,15 @@ Residual F(u; v) and its Jacobian J(u; du, v) for a DG_k state u:
   J = sum_K int_K mu grad du . grad v + mu' (grad u . grad du)(grad u . grad v) dx
     + (the facet terms with du in place of u)
 
-with mu = (eps^2 + |grad u|^2)^((p-2)/2), mu' = (p-2)(eps^2 + |grad u|^2)^((p-4)/2),
+with mu = mu0 + (eps^2 + |grad u|^2)^((p-2)/2), mu' = (p-2)(eps^2 + |grad u|^2)^((p-4)/2),
 n the unit normal pointing out of the first cell of the facet, [w] = w+ - w-,
 {q} = (q+ + q-)/2, sigma = penalty (10 k^2 by default), h_F the mean of the
 two cells' diameters (largest vertex distance).  The facet terms are those
-of the linear Laplacian, so J is exactly the derivative of F.
+of the linear Laplacian, so J is exactly the derivative of F; the constant
+part mu0 (default 1) of the coefficient keeps the cell term at least as
+strong as the Laplacian consistency terms, so the SIPG form stays coercive
+where |grad u| vanishes (with mu0 = 0 and a degenerate coefficient the
+linear facet terms make the Jacobian indefinite there).
 
 Interior-facet tensors follow the UFCx layout the reference assembles
 (cpp/cudolfinx/fem/CUDAFormIntegral.cpp:1988-2103): rows and columns
@@ -85,7 +89,8 @@ def _grad(s: str, tab: str, nd: int) -> str:
 
 
 def sipg_plaplace(V: FunctionSpace, u: Function, f: float = 1.0, alpha: float = 2.0,
-                  penalty: float | None = None, p: float = 3.0, eps: float = 1e-3):
+                  penalty: float | None = None, p: float = 3.0, eps: float = 1e-3,
+                  mu0: float = 1.0):
     """(J, F): Jacobian and residual forms of the SIPG p-Laplacian (module doc)."""
     mesh = V.mesh
     el = V.element
@@ -112,10 +117,10 @@ def sipg_plaplace(V: FunctionSpace, u: Function, f: float = 1.0, alpha: float = 
     e2 = repr(float(eps) ** 2)
     e_mu, e_dmu = repr((p - 2.0) / 2.0), repr((p - 4.0) / 2.0)
     common = dict(cell=0, degree=k, bs=1, p=float(p), eps=float(eps), alpha=float(alpha),
-                  penalty=float(penalty), fval=float(f), u_off=0)
+                  penalty=float(penalty), fval=float(f), u_off=0, mu0=float(mu0))
 
     def cell(rank):
-        name = f"tabulate_tensor_integral_{_uid('dg_cell', rank, k, p, eps, f)}"
+        name = f"tabulate_tensor_integral_{_uid('dg_cell', rank, k, p, eps, f, mu0)}"
         out = (f"        A[i * {nd} + j] += fw * (mu * (g[i][0] * g[j][0] + g[i][1] * g[j][1] + g[i][2] * g[j][2])\n"
                f"                               + dmu * gui[i] * gui[j]);\n" if rank == 2 else "")
         loop = (f"    for (int i = 0; i < {nd}; ++i)\n      for (int j = 0; j < {nd}; ++j)\n{out}"
@@ -132,7 +137,7 @@ def sipg_plaplace(V: FunctionSpace, u: Function, f: float = 1.0, alpha: float = 
     double gui[{nd}];
     for (int i = 0; i < {nd}; ++i) gui[i] = gu[0] * g[i][0] + gu[1] * g[i][1] + gu[2] * g[i][2];
     const double s2 = {e2} + gu[0] * gu[0] + gu[1] * gu[1] + gu[2] * gu[2];
-    const double mu = pow(s2, {e_mu});
+    const double mu = {float(mu0)!r} + pow(s2, {e_mu});
     const double dmu = {float(p - 2.0)!r} * pow(s2, {e_dmu});
     (void)dmu;
     const double fw = FE_w[iq] * adet;

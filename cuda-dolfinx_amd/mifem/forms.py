@@ -84,9 +84,25 @@ class StagedKernel:
     combine: str                 # body of combine(G, w, c, out)
     coeff_width: int             # packed coefficient values per cell
     symmetric: bool = True
+    trace_only: bool = False     # combine reads only G[0] + G[4] + G[8]
+
+    def reference_tensor(self) -> np.ndarray:
+        """(nd, nd, 6) R_ij = sum_q w_q dphi_i (x) dphi_j in reference coordinates,
+        symmetrised for a symmetric metric M: components (00, 11, 22, 01+10, 02+20, 12+21).
+
+        For affine cells tr G_ij = |det J| sum_rs M_rs R_ij[r][s] with M = K K^T,
+        K = J^-1, so the trace needs 6 multiply-adds per entry at any degree.
+        """
+        R = np.einsum("q,qia,qjb->ijab", self.weights, self.dphi, self.dphi)
+        return np.stack([R[..., 0, 0], R[..., 1, 1], R[..., 2, 2],
+                         R[..., 0, 1] + R[..., 1, 0], R[..., 0, 2] + R[..., 2, 0],
+                         R[..., 1, 2] + R[..., 2, 1]], axis=-1)
 
     def source(self) -> str:
+        trace = ("#define SK_TRACE 1\n" + _carr("SK_RS", self.reference_tensor(), "__constant__ double")
+                 if self.trace_only else "")
         return (f"#define SK_ND {self.nd}\n#define SK_BS {self.bs}\n#define SK_NQ {self.nq}\n"
+                + trace
                 + _carr("SK_W", self.weights, "__constant__ double")
                 + _carr("SK_DPHI", self.dphi, "__constant__ double")
                 + "static __device__ __forceinline__ void sk_combine(const double* G, const double* "
@@ -225,7 +241,7 @@ def stiffness(V: FunctionSpace, kappa=None, qdegree: Optional[int] = None) -> Fo
     src = _SIG.format(name=name) + "\n{\n" + body + "}\n"
     staged = StagedKernel(nd=nd, bs=1, nq=q.weights.size, weights=q.weights, dphi=dphi,
                           combine=f"  out[0] = ({kexpr}) * (G[0] + G[4] + G[8]);\n",
-                          coeff_width=len(coeffs))
+                          coeff_width=len(coeffs), trace_only=True)
     cells = np.arange(V.mesh.num_cells, dtype=np.int32)
     spec = dict(kind="stiffness", cell=0, degree=el.degree, bs=1, rank=2,
                 qdeg=2 * (el.degree - 1) if qdegree is None else qdegree,

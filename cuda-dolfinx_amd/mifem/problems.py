@@ -11,7 +11,8 @@ builds its forms: mesh, space, bilinear and linear forms, Dirichlet BCs.
 * `helmholtz_q3(n)`: grad u . grad v - k^2 u v, k = 5, Q3 on n^3 hexahedra
   with interior vertices jittered by +-0.2 h, f = 1, u = 0 on the boundary,
   4 x 4 x 4 Gauss points.
-* `dg_plaplace(n)`: SIPG DG1 p-Laplacian (p = 3, eps = 1e-3) with Robin
+* `dg_plaplace(n)`: SIPG DG1 p-Laplacian (coefficient 1 + (eps^2 + |grad u|^2)^((p-2)/2),
+  p = 3, eps = 1e-3) with Robin
   alpha = 2 and f = 1 on n^3 x 6 tets with interior vertices jittered by
   +-0.2 h; `a` is the Newton Jacobian J(u), `L` the residual F(u) at the
   state u = sin(pi x) sin(pi y) sin(pi z) interpolated into DG1.

@@ -49,7 +49,7 @@ class Kernel(C.Structure):
                 ("k2", C.c_double),
                 ("p", C.c_double), ("eps", C.c_double), ("alpha", C.c_double),
                 ("penalty", C.c_double), ("fval", C.c_double),
-                ("u_off", C.c_int), ("coeff_width", C.c_int)]
+                ("u_off", C.c_int), ("coeff_width", C.c_int), ("mu0", C.c_double)]
 
 
 class Entities(C.Structure):
@@ -169,7 +169,7 @@ def kernel_from_spec(spec: dict, coeff_width: int = 0) -> Kernel:
         k.f_src, k.f_off = s, o
         k.f_degree = spec.get("f_degree", 0)
     k.k2 = spec.get("k2", 0.0)
-    for name in ("p", "eps", "alpha", "penalty", "fval"):
+    for name in ("p", "eps", "alpha", "penalty", "fval", "mu0"):
         setattr(k, name, float(spec.get(name, 0.0)))
     k.u_off = int(spec.get("u_off", 0))
     k.coeff_width = coeff_width

@@ -338,6 +338,7 @@ typedef struct {
   double p, eps, alpha, penalty, fval;
   int u_off;       /* offset of the DG1 state u in w (per cell) */
   int coeff_width; /* per-cell packed width (for interior-facet interlacing) */
+  double mu0;      /* constant part of the p-Laplacian coefficient */
 } or_kernel;
 
 static double param(int src, int off, double val, const double* w, const double* c)
@@ -498,7 +499,7 @@ static void tab_dg_cell(const or_kernel* K, double* A, const double* w, const do
     for (int i = 0; i < nd; ++i)
       for (int d = 0; d < 3; ++d) gu[d] += w[K->u_off + i] * g[i][d];
     double s = K->eps * K->eps + gu[0] * gu[0] + gu[1] * gu[1] + gu[2] * gu[2];
-    double mu = pow(s, 0.5 * (K->p - 2.0)), dmu = (K->p - 2.0) * pow(s, 0.5 * (K->p - 4.0));
+    double mu = K->mu0 + pow(s, 0.5 * (K->p - 2.0)), dmu = (K->p - 2.0) * pow(s, 0.5 * (K->p - 4.0));
     for (int i = 0; i < nd; ++i) {
       double gi = gu[0] * g[i][0] + gu[1] * g[i][1] + gu[2] * g[i][2];
       if (K->rank == 1) {

@@ -141,6 +141,32 @@ def test_staged_description_matches_oracle():
     np.testing.assert_allclose(A, ref, atol=1e-13 * abs(ref).max())
 
 
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_reference_tensor_trace_matches_oracle(k):
+    """Scalar stiffness on the reference-tensor path: kappa |det J| sum_rs M_rs R_ij[rs]
+    with M = K K^T reproduces the oracle's element matrix."""
+    mesh = create_unit_cube(2, jitter=0.25, seed=9)
+    V = functionspace(mesh, ("P", k))
+    D = functionspace(mesh, ("DG", 0))
+    kap = Function(D)
+    kap.set_cellwise(np.linspace(0.5, 2.0, mesh.num_cells))
+    form = forms.stiffness(V, kap)
+    I = form.integrals[0]
+    S = I.staged
+    assert S.trace_only
+    RS = S.reference_tensor()
+    packed = form.pack_coefficients()
+    for c in (0, 7, 31):
+        xc = mesh.x[mesh.cells[c]]
+        J = (xc[1:] - xc[0]).T
+        K = np.linalg.inv(J)
+        M = abs(np.linalg.det(J)) * K @ K.T
+        m = np.array([M[0, 0], M[1, 1], M[2, 2], M[0, 1], M[0, 2], M[1, 2]])
+        A = packed[c][0] * RS @ m
+        ref = cpu.tabulate(I.spec, packed[c], [], xc)
+        np.testing.assert_allclose(A, ref, atol=1e-13 * abs(ref).max())
+
+
 def test_library_exports_every_declared_symbol():
     header = (ROOT / "include" / "mifem.h").read_text()
     declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(mf_\w+)\s*\(", header, re.M))

@@ -258,7 +258,7 @@ def test_jittered_hex_mass_sums_to_volume():
 _TF = [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)]
 
 
-def _dg1_energy(mesh, u, p, eps, f, alpha, penalty):
+def _dg1_energy(mesh, u, p, eps, f, alpha, penalty, mu0):
     """E(u) whose derivative is the SIPG residual, from vertex values only.
 
     Cell gradients come from solving the vertex-difference system, facet
@@ -276,7 +276,7 @@ def _dg1_energy(mesh, u, p, eps, f, alpha, penalty):
         g = np.linalg.solve(D, U[c, 1:] - U[c, 0])
         vol = abs(np.linalg.det(D)) / 6.0
         s = eps * eps + g @ g
-        E_ += vol * (s ** (p / 2) / p - f * U[c].mean())
+        E_ += vol * (s ** (p / 2) / p + 0.5 * mu0 * (g @ g) - f * U[c].mean())
         grads.append(g)
         diam.append(max(np.linalg.norm(xc[a] - xc[b]) for a in range(4) for b in range(a + 1, 4)))
 
@@ -326,7 +326,8 @@ def test_sipg_residual_is_energy_gradient(p):
     u = P.extras["u"].x.copy()
     F = _dg_residual(P, u)
     spec = P.L.integrals[0].spec
-    args = (P.mesh, None, spec["p"], spec["eps"], spec["fval"], spec["alpha"], spec["penalty"])
+    args = (P.mesh, None, spec["p"], spec["eps"], spec["fval"], spec["alpha"], spec["penalty"],
+            spec["mu0"])
     rng = np.random.default_rng(3)
     for _ in range(4):
         d = rng.standard_normal(u.size)
@@ -361,7 +362,7 @@ def test_sipg_consistency_for_linear_fields():
     u = Function(V)
     gvec = np.array([0.3, -1.1, 0.7])
     u.interpolate(lambda x: 0.5 + gvec @ x)
-    _, F = sipg_plaplace(V, u, f=0.0, alpha=0.0, p=2.0, eps=0.0)
+    _, F = sipg_plaplace(V, u, f=0.0, alpha=0.0, p=2.0, eps=0.0, mu0=0.0)
     r = cpu.OracleAssembler(F).assemble_vector().reshape(-1, 4)
     ref = np.zeros_like(r)
     for c, lf in mesh.exterior_facets:

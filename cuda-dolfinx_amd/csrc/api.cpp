@@ -507,6 +507,8 @@ int mf_bc_create(mf_context* ctx, int64_t ndofs, const int32_t* dofs, int64_t nb
     b->uid = g_uid++;
     if (nbc) b->dofs.upload(dofs, nbc);
     b->g.upload(g, ndofs);
+    b->g_zero = true;
+    for (int64_t i = 0; i < nbc; ++i) b->g_zero &= g[dofs[i]] == 0.0;
     b->markers.alloc(ndofs);
     MF_HIP(hipMemset(b->markers.ptr, 0, ndofs));
     mf::k_mark(b->markers.ptr, ndofs, b->dofs.ptr, nbc, nullptr);
@@ -520,6 +522,11 @@ int mf_bc_update(mf_bc* b, const double* g) {
     MF_CHECK(b && g, MF_EINVAL, "null argument");
     use_device(b->ctx);
     MF_HIP(hipMemcpy(b->g.ptr, g, sizeof(double) * b->ndofs, hipMemcpyHostToDevice));
+    std::vector<int32_t> d(b->nbc);
+    if (b->nbc)
+      MF_HIP(hipMemcpy(d.data(), b->dofs.ptr, sizeof(int32_t) * b->nbc, hipMemcpyDeviceToHost));
+    b->g_zero = true;
+    for (int32_t k : d) b->g_zero &= g[k] == 0.0;
   });
 }
 
@@ -567,6 +574,7 @@ int mf_apply_lifting(mf_form* const* a, int nforms, double* b, int64_t n,
                "bc size does not match the trial space");
       use_device(f->ctx);
       const double* xk = x0 ? x0[k] : nullptr;
+      if (!xk && bc->g_zero) continue;   // b -= scale A[:, bc] (g - x0) with g - x0 = 0
       const uint32_t block = static_cast<uint32_t>(f->opt("block", 128));
       for (auto& I : f->ints) {
         if (I->n == 0) continue;

@@ -211,7 +211,8 @@ bool is_hex_staged(const Integral& I) {
 bool hex_owner_supported(const mf_form& f, const Integral& I) {
   if (!is_hex_staged(I) || f.rank != 2 || f.dm[0] != f.dm[1] || f.mesh->nv != 8 || f.W != 0)
     return false;
-  return staged_define(I.staged_src, "HX_K") == 3 && f.dm[0]->bs == 1 && f.dm[0]->nd == 64;
+  return staged_define(I.staged_src, "HX_K") == 3 && staged_define(I.staged_src, "HX_NQ1") == 4 &&
+         f.dm[0]->bs == 1 && f.dm[0]->nd == 64;
 }
 
 int hex_batch(const mf_form& f, const Integral& I, int prec) {

@@ -76,6 +76,25 @@ static __device__ __forceinline__ short mf_to_bf16(float f)
 }
 #endif
 
+#ifndef MF_HEX_SEARCH
+#define MF_HEX_SEARCH
+// First position p in a[lo, lo + n) with a[p] >= key, n < 512: nine fixed
+// halving steps with no data-dependent branch, so the searches for the
+// columns a lane owns are independent and their LDS reads overlap.
+static __device__ __forceinline__ int mf_hex_lower_bound(const int32_t* a, int lo, int n, int32_t key)
+{
+  int pos = lo;
+  const int end = lo + n;
+#pragma unroll
+  for (int step = 256; step > 0; step >>= 1) {
+    const int t = pos + step;
+    const int v = a[min(t, end) - 1];
+    pos = (t <= end && v < key) ? t : pos;
+  }
+  return pos;
+}
+#endif
+
 extern "C" __global__ void __launch_bounds__(256)
 $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
        const int32_t* __restrict__ item_cell, const int32_t* __restrict__ pair_list,
@@ -244,14 +263,14 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
           const mf_hrec_$TAG& R = srec[bq];
           if (!((R.bits[iq >> 5] >> (iq & 31)) & 1u)) {
             const int lr = R.d[iq] - r0;
-            const int lo = lrp[lr], hi = lrp[lr + 1];
+            const int lo = lrp[lr], n = lrp[lr + 1] - lo;
+            const uint32_t cbits = (R.bits[jc >> 1] >> (16 * (jc & 1))) & 0xFFFFu;
+            int pos[16];
 #pragma unroll
-            for (int jj = 0; jj < 16; ++jj) {
-              const int j = 16 * jc + jj;
-              if ((R.bits[j >> 5] >> (j & 31)) & 1u) continue;
-              const int kk = mf_lds_lower_bound(lcols, lo, hi, R.d[j]);
-              atomicAdd(&acc[kk], cacc[jj]);
-            }
+            for (int jj = 0; jj < 16; ++jj) pos[jj] = mf_hex_lower_bound(lcols, lo, n, R.d[16 * jc + jj]);
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj)
+              if (!((cbits >> jj) & 1u)) atomicAdd(&acc[pos[jj]], cacc[jj]);
           }
         }
       }
@@ -290,50 +309,71 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
         }
       }
 #else
-      for (int q = 0; q < NQ; ++q) {
-        const int q1 = q % M1, q2 = (q / M1) % M1, q3 = q / (M1 * M1);
-        const real* D = &Db[q * DS];
-        const double Lx = HX_L[q1][ia], Ly = HX_L[q2][ib2], Lz = HX_L[q3][ic];
-        const double Gx = HX_DL[q1][ia], Gy = HX_DL[q2][ib2], Gz = HX_DL[q3][ic];
-        // this lane's K slot is component b = slot of point q
-        real fa;
-        if (slot == 3) {
-          fa = (real)(Lx * Ly * Lz) * D[6];
-        } else {
-          const real bx = (real)(Gx * Ly * Lz), by = (real)(Lx * Gy * Lz), bz = (real)(Lx * Ly * Gz);
-          const int c0 = slot == 0 ? 0 : (slot == 1 ? 1 : 2);
-          const int c1 = slot == 0 ? 1 : (slot == 1 ? 3 : 4);
-          const int c2 = slot == 0 ? 2 : (slot == 1 ? 4 : 5);
-          fa = bx * D[c0] + by * D[c1] + bz * D[c2];
-        }
-        if (!valid) fa = (real)0;
-        // B operand: component `slot` of column basis function j at point q
-        const double cLx = HX_L[q1][ja], cLy = HX_L[q2][jb];
-        const double cGx = HX_DL[q1][ja], cGy = HX_DL[q2][jb];
-        const double fxy = slot == 0 ? cGx * cLy : (slot == 1 ? cLx * cGy : cLx * cLy);
-        for (int jt = 0; jt < 4; ++jt) {
-          const double fz = slot == 2 ? HX_DL[q3][jt] : HX_L[q3][jt];
-          const real bv = (real)(fxy * fz);
-          C[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa, bv, C[jt], 0, 0, 0);
+      // FP32: operands built in FP32.  Column factors of this lane (column
+      // ja + 4 jb + 16 jt, component `slot`) are lane constants; row factors
+      // of the tile row are loaded once per tile.
+      float cx[4], cy[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        cx[m] = slot == 0 ? HX_DLF[m][ja] : HX_LF[m][ja];
+        cy[m] = slot == 1 ? HX_DLF[m][jb] : HX_LF[m][jb];
+      }
+      // which D entries combine into this lane's K slot (slot 3: the mass entry)
+      const int c0 = slot == 0 ? 0 : (slot == 1 ? 1 : (slot == 2 ? 2 : 6));
+      const int c1 = slot == 0 ? 1 : (slot == 1 ? 3 : 4);
+      const int c2 = slot == 0 ? 2 : (slot == 1 ? 4 : 5);
+      const float vmask = valid ? 1.0f : 0.0f;
+#pragma unroll 1
+      for (int q3 = 0; q3 < M1; ++q3) {
+        const float Lz = HX_LF[q3][ic], Gz = HX_DLF[q3][ic];
+        float cz[4];
+#pragma unroll
+        for (int jt = 0; jt < 4; ++jt) cz[jt] = slot == 2 ? HX_DLF[q3][jt] : HX_LF[q3][jt];
+#pragma unroll 1
+        for (int q2 = 0; q2 < M1; ++q2) {
+          const float Ly = HX_LF[q2][ib2], Gy = HX_DLF[q2][ib2];
+          const float LyLz = Ly * Lz * vmask, GyLz = Gy * Lz * vmask, LyGz = Ly * Gz * vmask;
+#pragma unroll
+          for (int q1 = 0; q1 < 4; ++q1) {
+            const float* D = &Db[(q1 + 4 * (q2 + 4 * q3)) * DS];
+            const float Lx = HX_LF[q1][ia], Gx = HX_DLF[q1][ia];
+            const float bx = Gx * LyLz, by = Lx * GyLz, bz = Lx * LyGz, b0 = Lx * LyLz;
+            const float fa = slot == 3 ? b0 * D[6] : bx * D[c0] + by * D[c1] + bz * D[c2];
+            const float cxy = cx[q1] * cy[q2];
+#pragma unroll
+            for (int jt = 0; jt < 4; ++jt)
+              C[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa, cxy * cz[jt], C[jt], 0, 0, 0);
+          }
         }
       }
 #endif
-      // scatter the 16 x 64 tile into the LDS segment
-      for (int rr = 0; rr < 4; ++rr) {
-        const int prr = t * 16 + slot * 4 + rr;
-        if (prr >= np) continue;
-        const int pcc = pb + prr;
-        const int bq = (spair[pcc] >> 8) - (ib - i0), iq = spair[pcc] & 255;
-        const mf_hrec_$TAG& R = srec[bq];
-        if ((R.bits[iq >> 5] >> (iq & 31)) & 1u) continue;            // Dirichlet row
-        const int lr = R.d[iq] - r0;
-        const int lo = lrp[lr], hi = lrp[lr + 1];
-        for (int jt = 0; jt < 4; ++jt) {
-          const int j = jt * 16 + (lane & 15);
-          if ((R.bits[j >> 5] >> (j & 31)) & 1u) continue;             // Dirichlet column
-          const int kk = mf_lds_lower_bound(lcols, lo, hi, R.d[j]);
-          atomicAdd(&acc[kk], (double)C[jt][rr]);
+      // scatter the 16 x 64 tile into the LDS segment: all 16 positions of
+      // the lane first (independent searches), then the adds
+      {
+        int pos[4][4];
+        bool ok[4][4];
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int prr = t * 16 + slot * 4 + rr;
+          const bool rv = prr < np;
+          const int pcc = pb + (rv ? prr : 0);
+          const int bq = (spair[pcc] >> 8) - (ib - i0), iq = spair[pcc] & 255;
+          const mf_hrec_$TAG& R = srec[bq];
+          const bool rowok = rv && !((R.bits[iq >> 5] >> (iq & 31)) & 1u);   // Dirichlet row
+          const int lr = rv ? R.d[iq] - r0 : 0;
+          const int lo = lrp[lr], n = lrp[lr + 1] - lo;
+#pragma unroll
+          for (int jt = 0; jt < 4; ++jt) {
+            const int j = jt * 16 + (lane & 15);
+            ok[rr][jt] = rowok && !((R.bits[j >> 5] >> (j & 31)) & 1u);     // Dirichlet column
+            pos[rr][jt] = mf_hex_lower_bound(lcols, lo, n, R.d[j]);
+          }
         }
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr)
+#pragma unroll
+          for (int jt = 0; jt < 4; ++jt)
+            if (ok[rr][jt]) atomicAdd(&acc[pos[rr][jt]], (double)C[jt][rr]);
       }
 #endif
     }

@@ -162,6 +162,7 @@ struct mf_bc {
   mf::DevArray<int8_t> markers;
   mf::DevArray<double> g;
   uint64_t uid = 0;
+  bool g_zero = false;                // every constrained value is 0 (lifting adds nothing)
   // per-node component bits (bit a: scalar dof bs*node + a constrained), by block size
   mutable std::map<int, mf::DevArray<uint8_t>> node_bits;
   const uint8_t* bits_for(int bs, hipStream_t s) const;

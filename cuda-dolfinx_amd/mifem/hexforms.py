@@ -103,6 +103,8 @@ class HexStaged:
                 f"#define HX_K2 {self.k2!r}\n#define HX_KAPPA {self.kappa!r}\n"
                 + _carr("HX_L", V, "__constant__ double")
                 + _carr("HX_DL", D, "__constant__ double")
+                + _carr("HX_LF", V, "__constant__ float")
+                + _carr("HX_DLF", D, "__constant__ float")
                 + _carr("HX_X1", x, "__constant__ double")
                 + _carr("HX_W1", w, "__constant__ double"))
 

@@ -173,7 +173,9 @@ int mf_assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* 
 /* assemble_vector_on_device (fem.cpp:291-309, CUDAAssembler.h:369-424). */
 int mf_assemble_vector(mf_form* L, double* d_b, int64_t n, int zero, mf_stream stream);
 /* apply_lifting_on_device (fem.cpp:327-355, CUDAAssembler.h:564-603):
- * b -= scale * A_j[:, bc_j] (g_j - x0_j) over every integral of every form. */
+ * b -= scale * A_j[:, bc_j] (g_j - x0_j) over every integral of every form.
+ * A form whose constrained values are all zero and has no x0 adds exactly
+ * nothing and launches no kernel. */
 int mf_apply_lifting(mf_form* const* a, int nforms, double* d_b, int64_t n,
                      const mf_bc* const* bcs1, const double* const* d_x0, double scale,
                      mf_stream stream);

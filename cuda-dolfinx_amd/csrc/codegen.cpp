@@ -247,7 +247,7 @@ OwnerShape owner_shape(const mf_form& f, const Integral& I) {
   sh.nb = static_cast<int>(std::min<int64_t>(f.opt("owner_nb", 64), 1023));
   sh.maxp = static_cast<int>(f.opt("owner_pairs", 384));
   const int64_t trace = staged_trace(I) ? 8LL * nd * nd * 6 : 0;
-  const int64_t used = fixed + trace + sh.nb * (record_bytes(f, I) + 4) + (int64_t)sh.maxp * (4 + nd);
+  const int64_t used = fixed + trace + (int64_t)sh.maxp * (4 + nd);
   MF_CHECK(used <= budget, MF_EUNSUPPORTED,
            "owner-computes kernel shape exceeds the LDS budget (" + std::to_string(used) + " > " +
                std::to_string(budget) + " bytes)");

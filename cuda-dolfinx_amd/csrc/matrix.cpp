@@ -96,6 +96,7 @@ std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I,
   const mf_dofmap* d1 = a.dm[1];
   const int nd = d0->nd;
   if (nd > (with_kk ? 32 : 64)) return nullptr;
+  if (with_kk && d0->num_cells >= (int64_t(1) << 26)) return nullptr;   // cell << 6 | i in 32 bits
   const int bs2 = A.bs0 * A.bs1;
   std::vector<int32_t> ncell(A.nrows_b, 0);
   for (int64_t e = 0; e < I.n; ++e)
@@ -146,7 +147,10 @@ std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I,
         while (msk) {
           const int i = __builtin_ctzll(msk);
           msk &= msk - 1;
-          plist[p] = ((it - ci[c]) << 8) | i;
+          // simplex kernel (column offsets precomputed): cell id << 6 | local row;
+          // hexahedral kernel: item offset in the chunk << 8 | local row
+          plist[p] = with_kk ? static_cast<int32_t>((static_cast<uint32_t>(cell) << 6) | i)
+                             : ((it - ci[c]) << 8) | i;
           if (!with_kk) {
             ++p;
             continue;

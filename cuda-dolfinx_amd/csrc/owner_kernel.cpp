@@ -123,30 +123,29 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
   // chunk_meta[12k ..]: r0, r1, i0, i1, cb0 (= rpb[r0]), ncb, base (CSR offset),
   //   seglen, p0, p1 (pair range), kko (byte offset of the chunk's column
   //   offsets, 16-byte aligned), 0.
-  // pair_list[p] = (item offset in chunk) << 8 | local row i; pair_kk[kko + (p-p0)*ND + j]
-  //   = position of column node j within the blocked row of node i.
+  // pair_list[p] = cell << 6 | local row i; pair_kk[kko + (p-p0)*ND + j] = position of
+  //   column node j within the blocked row of node i.  Each (pair, column range)
+  //   thread reads its cell's record straight from memory (neighbouring chunks
+  //   share cells, so these reads mostly hit L2): no staging batches, no barriers
+  //   inside a chunk, and no LDS spent on records.
   // flags: 1 overwrite (zero) instead of add, 2 set the diagonal of Dirichlet rows.
   //
   // Persistent schedule: the grid holds G workgroups (a multiple of 8); the
   // workgroups of XCD x = blockIdx % 8 walk the x-th contiguous eighth of the
   // chunks with stride G / 8, so neighbouring chunks (which share cells) run
   // at the same time on one XCD's L2.  While a chunk's values stream out, the
-  // next chunk's metadata, pair list, column offsets, row pointers, columns
-  // and first cell ids are already being loaded into registers.  With G = 8 *
+  // next chunk's metadata, pair list, column offsets, row pointers and
+  // columns are already being loaded into registers.  With G = 8 *
   // ceil(nchunks / 8) every workgroup owns exactly one chunk.
   constexpr int ND = SK_ND, BS = SK_BS, NQ = SK_NQ;
-  constexpr int NB = $ONB, CAP = $OCAP, MAXR = $OMAXR, MAXP = $OMAXP, NT = $OBLOCK;
+  constexpr int CAP = $OCAP, MAXR = $OMAXR, MAXP = $OMAXP, NT = $OBLOCK;
   constexpr int CAPB = CAP / (BS * BS) + 1;
-  constexpr int RW = sizeof(mf_rec_$TAG) / 16;      // record size in 16-byte words
   constexpr int KK4 = (MAXP * ND + 15) / 16;         // column-offset words (16 bytes)
   constexpr int PF_P = (MAXP + NT - 1) / NT, PF_K = (KK4 + NT - 1) / NT;
   constexpr int PF_R = (MAXR + 1 + NT - 1) / NT, PF_C = (CAPB + NT - 1) / NT;
-  constexpr int PF_S = (NB + NT - 1) / NT;
   __shared__ __attribute__((aligned(16))) double acc[CAP];
   __shared__ int32_t lrp[MAXR + 1];
   __shared__ int32_t lcols[CAPB];
-  __shared__ mf_rec_$TAG srec[NB];
-  __shared__ int32_t scell[NB];
   __shared__ int32_t spair[MAXP];
   __shared__ __attribute__((aligned(16))) uint8_t skk[KK4 * 16];
 #ifdef SK_TRACE
@@ -168,7 +167,6 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
   int4 m2 = reinterpret_cast<const int4*>(chunk_meta)[3 * k + 2];
   // first chunk: stage its lists, zero the whole accumulator once (each
   // write-out re-zeroes what it read, so acc is zero at every chunk start)
-  for (int b = tid; b < m0.w - m0.z && b < NB; b += nt) scell[b] = item_cell[m0.z + b];
   for (int p = tid; p < m2.y - m2.x; p += nt) spair[p] = pair_list[m2.x + p];
   {
     const int n4 = ((m2.y - m2.x) * ND + 15) / 16;
@@ -184,44 +182,16 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
   for (int p = tid; p < m1.y; p += nt) lcols[p] = colsb[m1.x + p];
 
   while (true) {
-    const int32_t r0 = m0.x, r1 = m0.y, i0 = m0.z, i1 = m0.w;
+    const int32_t r0 = m0.x, r1 = m0.y;
     const int32_t base = m1.z, seglen = m1.w;
     const int nr = r1 - r0;
     const int npc = m2.y - m2.x;
 
-    int pb = 0;                                        // first pair of the batch
-    for (int ib = i0; ib < i1; ib += NB) {
-      const int nb = min(NB, i1 - ib);
-      if (ib > i0) {
-        __syncthreads();                               // previous batch fully consumed
-        for (int b = tid; b < nb; b += nt) scell[b] = item_cell[ib + b];
-      }
-      __syncthreads();
-      // stage 1: copy the cell records, 16 bytes per thread
-      {
-        const int4* src = reinterpret_cast<const int4*>(rec);
-        int4* dst = reinterpret_cast<int4*>(srec);
-        for (int u = tid; u < nb * RW; u += nt) {
-          const int b = u / RW, w = u - b * RW;
-          dst[b * RW + w] = src[(int64_t)scell[b] * RW + w];
-        }
-      }
-      // pairs of this batch: those of items [ib - i0, ib - i0 + nb) of the chunk
-      int pe;
-      {
-        int lo2 = pb, hi2 = npc;
-        const int lim = ib - i0 + nb;
-        while (lo2 < hi2) {
-          const int mid = (lo2 + hi2) >> 1;
-          if ((spair[mid] >> 8) < lim) lo2 = mid + 1;
-          else hi2 = mid;
-        }
-        pe = lo2;
-      }
-      __syncthreads();
-      // stage 2: one thread per (pair, column range); lanes of a wave share the
+    __syncthreads();                                   // lists and zeroed acc visible
+    {
+      // one thread per (pair, column range); lanes of a wave share the
       // column range, so the column basis tables are wave-uniform.
-      const int np = pe - pb;
+      const int np = npc;
       const int waves = nt >> 6;
       int js = (waves * 64) / max(np, 1);
       js = js < 1 ? 1 : (js > ND ? ND : js);
@@ -232,9 +202,10 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
         const int part = __builtin_amdgcn_readfirstlane(wv % js);
         const int pr = (wv / js) * 64 + lane;
         if (pr >= np) continue;
-        const int pc = pb + pr;                        // pair index within the chunk
-        const int b = (spair[pc] >> 8) - (ib - i0), i = spair[pc] & 255;
-        const mf_rec_$TAG& r = srec[b];
+        const int pc = pr;                             // pair index within the chunk
+        const uint32_t pv = (uint32_t)spair[pc];
+        const int i = pv & 63;
+        const mf_rec_$TAG& r = rec[pv >> 6];
         const double K0 = r.K[0], K1 = r.K[1], K2 = r.K[2], K3 = r.K[3], K4 = r.K[4], K5 = r.K[5];
         const double K6 = r.K[6], K7 = r.K[7], K8 = r.K[8], dt = r.det;
 #ifdef SK_TRACE
@@ -296,7 +267,6 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
           }
         }
       }
-      pb = pe;
     }
     __syncthreads();
     if ((flags & 2) && nb0) {
@@ -318,7 +288,7 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
     const int jn = jj + gx, kn = xcd * per + jn;
     const bool more = jn < per && kn < nchunks;
     int4 n0 = {0, 0, 0, 0}, n1 = {0, 0, 0, 0}, n2 = {0, 0, 0, 0};
-    int pf_p[PF_P], pf_r[PF_R], pf_c[PF_C], pf_s[PF_S];
+    int pf_p[PF_P], pf_r[PF_R], pf_c[PF_C];
     int4 pf_k[PF_K];
     if (more) {
       n0 = reinterpret_cast<const int4*>(chunk_meta)[3 * kn];
@@ -345,11 +315,6 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
       for (int t = 0; t < PF_C; ++t) {
         const int p = tid + t * NT;
         pf_c[t] = p < n1.y ? colsb[n1.x + p] : 0;
-      }
-#pragma unroll
-      for (int t = 0; t < PF_S; ++t) {
-        const int p = tid + t * NT;
-        pf_s[t] = p < n0.w - n0.z && p < NB ? item_cell[n0.z + p] : 0;
       }
     }
 
@@ -402,16 +367,12 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
         const int p = tid + t * NT;
         if (p < n1.y) lcols[p] = pf_c[t];
       }
-#pragma unroll
-      for (int t = 0; t < PF_S; ++t) {
-        const int p = tid + t * NT;
-        if (p < n0.w - n0.z && p < NB) scell[p] = pf_s[t];
-      }
     }
     m0 = n0; m1 = n1; m2 = n2;
     jj = jn;
     k = kn;
   }
+  (void)item_cell;
 }
 )";
 

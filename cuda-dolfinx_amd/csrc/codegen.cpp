@@ -400,6 +400,9 @@ std::string generate_form_source(const mf_form& f) {
           const OwnerShape sh = owner_shape(f, *I);
           m["$ONB"] = std::to_string(sh.nb);
           m["$OMAXP"] = std::to_string(sh.maxp);
+          // optional register cap: at least owner_wpe waves per SIMD
+          const int64_t wpe = f.opt("owner_wpe", 0);
+          m["$OATTR"] = wpe > 0 ? "__attribute__((amdgpu_waves_per_eu(" + std::to_string(wpe) + ")))" : "";
           os << subst(rename(kOwnerTemplate), m);
         }
         m["$KNAME"] = kernel_name("slift", I->type, I->index);

@@ -112,7 +112,7 @@ $KNAME(const int32_t n, const int32_t* __restrict__ ents, const int32_t* __restr
 )";
 
 const char* kOwnerTemplate = R"(
-extern "C" __global__ void __launch_bounds__($OBLOCK)
+extern "C" __global__ void __launch_bounds__($OBLOCK) $OATTR
 $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
        const int32_t* __restrict__ item_cell, const int32_t* __restrict__ pair_list,
        const uint8_t* __restrict__ pair_kk, const mf_rec_$TAG* __restrict__ rec,
@@ -188,6 +188,20 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
     const int npc = m2.y - m2.x;
 
     __syncthreads();                                   // lists and zeroed acc visible
+    if ((flags & 2) && nb0) {
+      // Dirichlet rows: the diagonal entries of constrained components receive
+      // no contributions (their rows are masked), so they are set up front
+      for (int lr = tid; lr < nr; lr += nt) {
+        const int32_t R = r0 + lr;
+        const uint32_t rbits = nb0[R];
+        if (!rbits) continue;
+        const int lo = lrp[lr], hi = lrp[lr + 1];
+        const int p = mf_lds_lower_bound(lcols, lo, hi, R);
+        if (p < hi && lcols[p] == R)
+          for (int a = 0; a < BS; ++a)
+            if (rbits & (1u << a)) acc[BS * BS * lo + a * BS * (hi - lo) + BS * (p - lo) + a] = diag;
+      }
+    }
     {
       // one thread per (pair, column range); lanes of a wave share the
       // column range, so the column basis tables are wave-uniform.
@@ -268,21 +282,7 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
         }
       }
     }
-    __syncthreads();
-    if ((flags & 2) && nb0) {
-      // Dirichlet rows: set the diagonal entries of the constrained components
-      for (int lr = tid; lr < nr; lr += nt) {
-        const int32_t R = r0 + lr;
-        const uint32_t rbits = nb0[R];
-        if (!rbits) continue;
-        const int lo = lrp[lr], hi = lrp[lr + 1];
-        const int p = mf_lds_lower_bound(lcols, lo, hi, R);
-        if (p < hi && lcols[p] == R)
-          for (int a = 0; a < BS; ++a)
-            if (rbits & (1u << a)) acc[BS * BS * lo + a * BS * (hi - lo) + BS * (p - lo) + a] = diag;
-      }
-      __syncthreads();
-    }
+    __syncthreads();                                   // all contributions in acc
 
     // prefetch the next chunk's lists into registers
     const int jn = jj + gx, kn = xcd * per + jn;
@@ -344,7 +344,8 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
       }
     }
     if (!more) break;
-    __syncthreads();                                   // lists of this chunk no longer read
+    // the lists of this chunk were last read before the barrier above, so the
+    // next chunk's lists can be stored without another barrier
     {
       const int nnp = n2.y - n2.x, nk4 = (nnp * ND + 15) / 16;
 #pragma unroll

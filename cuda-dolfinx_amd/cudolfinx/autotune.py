@@ -1,8 +1,9 @@
 """Autotuner for the cooperative owner-computes matrix kernel.
 
 Picks the kernel shape -- LDS value budget per workgroup ("chunk_values",
-i.e. how many CSR rows a workgroup owns), cells staged per batch
-("owner_nb"), threads per workgroup ("owner_block") -- by timing each
+i.e. how many CSR rows a workgroup owns), pairs per chunk ("owner_pairs"),
+threads per workgroup ("owner_block"), an optional register cap
+("owner_wpe": waves per SIMD) -- by timing each
 candidate on the actual form, matrix and boundary conditions.  Every
 candidate is a separate hipRTC build of the same form; the matrix (sparsity)
 is shared.  Results are cached in a JSON table keyed by the form description
@@ -20,12 +21,12 @@ from pathlib import Path
 from .form import CUDAForm
 
 DEFAULT_CANDIDATES = [
-    {"chunk_values": 4096, "owner_nb": 64, "owner_pairs": 384, "owner_block": 256, "owner_lds": 65536},
-    {"chunk_values": 4096, "owner_nb": 48, "owner_pairs": 256, "owner_block": 256, "owner_lds": 65536},
-    {"chunk_values": 2048, "owner_nb": 48, "owner_pairs": 256, "owner_block": 256, "owner_lds": 40960},
-    {"chunk_values": 2048, "owner_nb": 32, "owner_pairs": 192, "owner_block": 128, "owner_lds": 40960},
-    {"chunk_values": 6144, "owner_nb": 64, "owner_pairs": 512, "owner_block": 256, "owner_lds": 98304},
-    {"chunk_values": 4096, "owner_nb": 64, "owner_pairs": 384, "owner_block": 512, "owner_lds": 65536},
+    {"chunk_values": 4096, "owner_pairs": 256, "owner_block": 256},
+    {"chunk_values": 4096, "owner_pairs": 384, "owner_block": 256},
+    {"chunk_values": 3072, "owner_pairs": 256, "owner_block": 256, "owner_wpe": 5},
+    {"chunk_values": 3072, "owner_pairs": 256, "owner_block": 256},
+    {"chunk_values": 2048, "owner_pairs": 192, "owner_block": 128},
+    {"chunk_values": 6144, "owner_pairs": 512, "owner_block": 256, "owner_lds": 98304},
 ]
 
 

@@ -143,8 +143,10 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
   constexpr int KK4 = (MAXP * ND + 15) / 16;         // column-offset words (16 bytes)
   constexpr int PF_P = (MAXP + NT - 1) / NT, PF_K = (KK4 + NT - 1) / NT;
   constexpr int PF_R = (MAXR + 1 + NT - 1) / NT, PF_C = (CAPB + NT - 1) / NT;
+  constexpr int PF_B = (MAXR + NT - 1) / NT;
   __shared__ __attribute__((aligned(16))) double acc[CAP];
   __shared__ int32_t lrp[MAXR + 1];
+  __shared__ uint8_t snb[MAXR];                      // Dirichlet bits of the chunk's rows
   __shared__ int32_t lcols[CAPB];
   __shared__ int32_t spair[MAXP];
   __shared__ __attribute__((aligned(16))) uint8_t skk[KK4 * 16];
@@ -179,7 +181,39 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
     for (int p = tid; p < CAP / 2; p += nt) a2[p] = z;
   }
   for (int p = tid; p <= m0.y - m0.x; p += nt) lrp[p] = rpb[m0.x + p] - m1.x;
+  for (int p = tid; p < m0.y - m0.x; p += nt) snb[p] = nb0 ? nb0[m0.x + p] : 0;
   for (int p = tid; p < m1.y; p += nt) lcols[p] = colsb[m1.x + p];
+
+  // pair index of this thread's first (pair, column range) unit in a chunk
+  // of np pairs (same mapping as the unit loop below), -1 if it has none
+  auto mf_first_pair = [&](int np) -> int {
+    int js = ((NT >> 6) * 64) / max(np, 1);
+    js = js < 1 ? 1 : (js > ND ? ND : js);
+    const int pr = ((tid >> 6) / js) * 64 + (tid & 63);
+    return pr < np ? pr : -1;
+  };
+  // this thread's first unit: pair value and the record fields it needs,
+  // loaded one chunk ahead so the compute phase starts with them in registers
+  uint32_t u_pv = 0, u_bits = 0;
+  int32_t u_di = 0;
+  double u_K[9], u_det = 0.0, u_w[$WW];
+  for (int x = 0; x < 9; ++x) u_K[x] = 0.0;
+  for (int x = 0; x < $WW; ++x) u_w[x] = 0.0;
+  auto mf_load_unit = [&](uint32_t pv) {
+    const mf_rec_$TAG& r = rec[pv >> 6];
+    for (int x = 0; x < 9; ++x) u_K[x] = r.K[x];
+    u_det = r.det;
+    for (int x = 0; x < $WW; ++x) u_w[x] = r.w[x];
+    u_di = r.d[pv & 63];
+    u_bits = r.bits;
+  };
+  {
+    const int pr0 = mf_first_pair(m2.y - m2.x);
+    if (pr0 >= 0) {
+      u_pv = (uint32_t)pair_list[m2.x + pr0];
+      mf_load_unit(u_pv);
+    }
+  }
 
   while (true) {
     const int32_t r0 = m0.x, r1 = m0.y;
@@ -193,7 +227,7 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
       // no contributions (their rows are masked), so they are set up front
       for (int lr = tid; lr < nr; lr += nt) {
         const int32_t R = r0 + lr;
-        const uint32_t rbits = nb0[R];
+        const uint32_t rbits = snb[lr];
         if (!rbits) continue;
         const int lo = lrp[lr], hi = lrp[lr + 1];
         const int p = mf_lds_lower_bound(lcols, lo, hi, R);
@@ -202,93 +236,13 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
             if (rbits & (1u << a)) acc[BS * BS * lo + a * BS * (hi - lo) + BS * (p - lo) + a] = diag;
       }
     }
-    {
-      // one thread per (pair, column range); lanes of a wave share the
-      // column range, so the column basis tables are wave-uniform.
-      const int np = npc;
-      const int waves = nt >> 6;
-      int js = (waves * 64) / max(np, 1);
-      js = js < 1 ? 1 : (js > ND ? ND : js);
-      const int jw = (ND + js - 1) / js;
-      const int units = ((np + 63) >> 6) * 64 * js;
-      for (int u = tid; u < units; u += nt) {
-        const int lane = u & 63, wv = u >> 6;
-        const int part = __builtin_amdgcn_readfirstlane(wv % js);
-        const int pr = (wv / js) * 64 + lane;
-        if (pr >= np) continue;
-        const int pc = pr;                             // pair index within the chunk
-        const uint32_t pv = (uint32_t)spair[pc];
-        const int i = pv & 63;
-        const mf_rec_$TAG& r = rec[pv >> 6];
-        const double K0 = r.K[0], K1 = r.K[1], K2 = r.K[2], K3 = r.K[3], K4 = r.K[4], K5 = r.K[5];
-        const double K6 = r.K[6], K7 = r.K[7], K8 = r.K[8], dt = r.det;
-#ifdef SK_TRACE
-        // metric M = |det J| K K^T (K[3 r + c] = dX_r / dx_c): tr G_ij = sum_rs M_rs R_ij[r][s]
-        const double M00 = dt * (K0 * K0 + K1 * K1 + K2 * K2), M11 = dt * (K3 * K3 + K4 * K4 + K5 * K5);
-        const double M22 = dt * (K6 * K6 + K7 * K7 + K8 * K8), M01 = dt * (K0 * K3 + K1 * K4 + K2 * K5);
-        const double M02 = dt * (K0 * K6 + K1 * K7 + K2 * K8), M12 = dt * (K3 * K6 + K4 * K7 + K5 * K8);
-#else
-        double gi[NQ][3];
-        for (int q = 0; q < NQ; ++q) {
-          const double s = SK_W[q] * dt;
-          const double d0 = SK_DPHI[q][i][0], d1v = SK_DPHI[q][i][1], d2 = SK_DPHI[q][i][2];
-          gi[q][0] = s * (K0 * d0 + K3 * d1v + K6 * d2);
-          gi[q][1] = s * (K1 * d0 + K4 * d1v + K7 * d2);
-          gi[q][2] = s * (K2 * d0 + K5 * d1v + K8 * d2);
-        }
-#endif
-        const uint32_t bits = r.bits;
-        const uint32_t rb0 = (bits >> (BS * i)) & ((1u << BS) - 1);
-        const int lr = r.d[i] - r0;
-        const int lo = lrp[lr], hi = lrp[lr + 1];
-        const int rowlen = BS * (hi - lo);
-        const int rbase = BS * BS * lo;
-        const uint8_t* kk = &skk[pc * ND];
-        const int j0 = part * jw, j1 = min(ND, j0 + jw);
-        for (int j = j0; j < j1; ++j) {
-          double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-#ifdef SK_TRACE
-          {
-            const double* R = &sRS[(i * ND + j) * 6];
-            G[0] = M00 * R[0] + M11 * R[1] + M22 * R[2] + M01 * R[3] + M02 * R[4] + M12 * R[5];
-          }
-#else
-          for (int q = 0; q < NQ; ++q) {
-            const double d0 = SK_DPHI[q][j][0], d1v = SK_DPHI[q][j][1], d2 = SK_DPHI[q][j][2];
-            const double c0 = K0 * d0 + K3 * d1v + K6 * d2;
-            const double c1 = K1 * d0 + K4 * d1v + K7 * d2;
-            const double c2 = K2 * d0 + K5 * d1v + K8 * d2;
-            G[0] += gi[q][0] * c0; G[1] += gi[q][0] * c1; G[2] += gi[q][0] * c2;
-            G[3] += gi[q][1] * c0; G[4] += gi[q][1] * c1; G[5] += gi[q][1] * c2;
-            G[6] += gi[q][2] * c0; G[7] += gi[q][2] * c1; G[8] += gi[q][2] * c2;
-          }
-#endif
-          double blk[BS * BS];
-          sk_combine(G, r.w, consts, blk);
-          const uint32_t cb1 = (bits >> (BS * j)) & ((1u << BS) - 1);
-          const int rb = rbase + BS * kk[j];
-          if ((rb0 | cb1) == 0) {
-            for (int a = 0; a < BS; ++a)
-              for (int bb = 0; bb < BS; ++bb) atomicAdd(&acc[rb + a * rowlen + bb], blk[a * BS + bb]);
-          } else {
-            for (int a = 0; a < BS; ++a) {
-              if (rb0 & (1u << a)) continue;
-              for (int bb = 0; bb < BS; ++bb) {
-                if (cb1 & (1u << bb)) continue;
-                atomicAdd(&acc[rb + a * rowlen + bb], blk[a * BS + bb]);
-              }
-            }
-          }
-        }
-      }
-    }
-    __syncthreads();                                   // all contributions in acc
-
-    // prefetch the next chunk's lists into registers
+    // prefetch the next chunk's lists into registers (issued before the
+    // compute phase so their latency hides behind it)
     const int jn = jj + gx, kn = xcd * per + jn;
     const bool more = jn < per && kn < nchunks;
     int4 n0 = {0, 0, 0, 0}, n1 = {0, 0, 0, 0}, n2 = {0, 0, 0, 0};
-    int pf_p[PF_P], pf_r[PF_R], pf_c[PF_C];
+    uint32_t pvn = 0xFFFFFFFFu;
+    int pf_p[PF_P], pf_r[PF_R], pf_c[PF_C], pf_b[PF_B];
     int4 pf_k[PF_K];
     if (more) {
       n0 = reinterpret_cast<const int4*>(chunk_meta)[3 * kn];
@@ -316,6 +270,103 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
         const int p = tid + t * NT;
         pf_c[t] = p < n1.y ? colsb[n1.x + p] : 0;
       }
+#pragma unroll
+      for (int t = 0; t < PF_B; ++t) {
+        const int p = tid + t * NT;
+        pf_b[t] = p < n0.y - n0.x && nb0 ? nb0[n0.x + p] : 0;
+      }
+      // pair of this thread's first unit in the next chunk (its record is
+      // loaded after the compute phase, while this chunk is written out)
+      const int prn = mf_first_pair(n2.y - n2.x);
+      if (prn >= 0) pvn = (uint32_t)pair_list[n2.x + prn];
+    }
+    {
+      // one thread per (pair, column range); lanes of a wave share the
+      // column range, so the column basis tables are wave-uniform.
+      const int np = npc;
+      const int waves = nt >> 6;
+      int js = (waves * 64) / max(np, 1);
+      js = js < 1 ? 1 : (js > ND ? ND : js);
+      const int jw = (ND + js - 1) / js;
+      const int units = ((np + 63) >> 6) * 64 * js;
+      for (int u = tid; u < units; u += nt) {
+        const int lane = u & 63, wv = u >> 6;
+        const int part = __builtin_amdgcn_readfirstlane(wv % js);
+        const int pr = (wv / js) * 64 + lane;
+        if (pr >= np) continue;
+        const int pc = pr;                             // pair index within the chunk
+        if (u != tid) {                                // later units: load directly
+          u_pv = (uint32_t)spair[pc];
+          mf_load_unit(u_pv);
+        }
+        const uint32_t pv = u_pv;
+        const int i = pv & 63;
+        const double K0 = u_K[0], K1 = u_K[1], K2 = u_K[2], K3 = u_K[3], K4 = u_K[4], K5 = u_K[5];
+        const double K6 = u_K[6], K7 = u_K[7], K8 = u_K[8], dt = u_det;
+#ifdef SK_TRACE
+        // metric M = |det J| K K^T (K[3 r + c] = dX_r / dx_c): tr G_ij = sum_rs M_rs R_ij[r][s]
+        const double M00 = dt * (K0 * K0 + K1 * K1 + K2 * K2), M11 = dt * (K3 * K3 + K4 * K4 + K5 * K5);
+        const double M22 = dt * (K6 * K6 + K7 * K7 + K8 * K8), M01 = dt * (K0 * K3 + K1 * K4 + K2 * K5);
+        const double M02 = dt * (K0 * K6 + K1 * K7 + K2 * K8), M12 = dt * (K3 * K6 + K4 * K7 + K5 * K8);
+#else
+        double gi[NQ][3];
+        for (int q = 0; q < NQ; ++q) {
+          const double s = SK_W[q] * dt;
+          const double d0 = SK_DPHI[q][i][0], d1v = SK_DPHI[q][i][1], d2 = SK_DPHI[q][i][2];
+          gi[q][0] = s * (K0 * d0 + K3 * d1v + K6 * d2);
+          gi[q][1] = s * (K1 * d0 + K4 * d1v + K7 * d2);
+          gi[q][2] = s * (K2 * d0 + K5 * d1v + K8 * d2);
+        }
+#endif
+        const uint32_t bits = u_bits;
+        const uint32_t rb0 = (bits >> (BS * i)) & ((1u << BS) - 1);
+        const int lr = u_di - r0;
+        const int lo = lrp[lr], hi = lrp[lr + 1];
+        const int rowlen = BS * (hi - lo);
+        const int rbase = BS * BS * lo;
+        const uint8_t* kk = &skk[pc * ND];
+        const int j0 = part * jw, j1 = min(ND, j0 + jw);
+        for (int j = j0; j < j1; ++j) {
+          double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#ifdef SK_TRACE
+          {
+            const double* R = &sRS[(i * ND + j) * 6];
+            G[0] = M00 * R[0] + M11 * R[1] + M22 * R[2] + M01 * R[3] + M02 * R[4] + M12 * R[5];
+          }
+#else
+          for (int q = 0; q < NQ; ++q) {
+            const double d0 = SK_DPHI[q][j][0], d1v = SK_DPHI[q][j][1], d2 = SK_DPHI[q][j][2];
+            const double c0 = K0 * d0 + K3 * d1v + K6 * d2;
+            const double c1 = K1 * d0 + K4 * d1v + K7 * d2;
+            const double c2 = K2 * d0 + K5 * d1v + K8 * d2;
+            G[0] += gi[q][0] * c0; G[1] += gi[q][0] * c1; G[2] += gi[q][0] * c2;
+            G[3] += gi[q][1] * c0; G[4] += gi[q][1] * c1; G[5] += gi[q][1] * c2;
+            G[6] += gi[q][2] * c0; G[7] += gi[q][2] * c1; G[8] += gi[q][2] * c2;
+          }
+#endif
+          double blk[BS * BS];
+          sk_combine(G, u_w, consts, blk);
+          const uint32_t cb1 = (bits >> (BS * j)) & ((1u << BS) - 1);
+          const int rb = rbase + BS * kk[j];
+          if ((rb0 | cb1) == 0) {
+            for (int a = 0; a < BS; ++a)
+              for (int bb = 0; bb < BS; ++bb) atomicAdd(&acc[rb + a * rowlen + bb], blk[a * BS + bb]);
+          } else {
+            for (int a = 0; a < BS; ++a) {
+              if (rb0 & (1u << a)) continue;
+              for (int bb = 0; bb < BS; ++bb) {
+                if (cb1 & (1u << bb)) continue;
+                atomicAdd(&acc[rb + a * rowlen + bb], blk[a * BS + bb]);
+              }
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();                                   // all contributions in acc
+    if (pvn != 0xFFFFFFFFu) {
+      u_pv = pvn;
+      mf_load_unit(pvn);
     }
 
     // write the segment once and re-zero what was read
@@ -367,6 +418,11 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
       for (int t = 0; t < PF_C; ++t) {
         const int p = tid + t * NT;
         if (p < n1.y) lcols[p] = pf_c[t];
+      }
+#pragma unroll
+      for (int t = 0; t < PF_B; ++t) {
+        const int p = tid + t * NT;
+        if (p < n0.y - n0.x) snb[p] = (uint8_t)pf_b[t];
       }
     }
     m0 = n0; m1 = n1; m2 = n2;

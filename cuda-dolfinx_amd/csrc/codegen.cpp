@@ -31,6 +31,7 @@ typedef __hip_internal::int32_t int32_t;
 typedef __hip_internal::int64_t int64_t;
 typedef __hip_internal::uint32_t uint32_t;
 typedef __hip_internal::int16_t int16_t;
+typedef unsigned short uint16_t;
 )";
 
 const char* kGatherCell = R"(
@@ -219,7 +220,7 @@ int hex_batch(const mf_form& f, const Integral& I, int prec) {
   const int nq1 = staged_define(I.staged_src, "HX_NQ1");
   const int64_t cap = f.opt("chunk_values", 4096), maxr = f.opt("chunk_rows", 64);
   const int64_t budget = f.opt("owner_lds", 78 * 1024);
-  const int64_t fixed = 8 * cap + 4 * (maxr + 1) + 4 * (cap + 1) + 4 * f.opt("owner_pairs", 384) + 64;
+  const int64_t fixed = 8 * cap + 4 * (maxr + 1) + 4 * f.opt("owner_pairs", 384) + 64;
   const int64_t per_cell = 464 + 4 + (int64_t)nq1 * nq1 * nq1 * 8 * (prec == 64 ? 8 : 4);
   int64_t nb = (budget - fixed) / per_cell;
   nb = std::min<int64_t>(nb, f.opt("owner_nb", 16));
@@ -374,6 +375,7 @@ std::string generate_form_source(const mf_form& f) {
           for (int prec : {64, 32, 16}) {
             m["$KNAME"] = kernel_name(("howner" + std::to_string(prec)).c_str(), I->type, I->index);
             m["$HNB"] = std::to_string(hex_batch(f, *I, prec));
+            m["$HBLOCK"] = std::to_string(f.opt("hex_block", 256));
             m["$OMAXP"] = std::to_string(f.opt("owner_pairs", 384));
             os << "\n#define MF_HX_PREC " << prec << "\n" << subst(kHexOwnerTemplate, m)
                << "\n#undef MF_HX_PREC\n";

@@ -76,30 +76,13 @@ static __device__ __forceinline__ short mf_to_bf16(float f)
 }
 #endif
 
-#ifndef MF_HEX_SEARCH
-#define MF_HEX_SEARCH
-// First position p in a[lo, lo + n) with a[p] >= key, n < 512: nine fixed
-// halving steps with no data-dependent branch, so the searches for the
-// columns a lane owns are independent and their LDS reads overlap.
-static __device__ __forceinline__ int mf_hex_lower_bound(const int32_t* a, int lo, int n, int32_t key)
-{
-  int pos = lo;
-  const int end = lo + n;
-#pragma unroll
-  for (int step = 256; step > 0; step >>= 1) {
-    const int t = pos + step;
-    const int v = a[min(t, end) - 1];
-    pos = (t <= end && v < key) ? t : pos;
-  }
-  return pos;
-}
-#endif
 
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__($HBLOCK)
 $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
        const int32_t* __restrict__ item_cell, const int32_t* __restrict__ pair_list,
-       const mf_hrec_$TAG* __restrict__ rec, const int32_t* __restrict__ rpb,
-       const int32_t* __restrict__ colsb, const uint8_t* __restrict__ nb0,
+       const uint16_t* __restrict__ pair_pos, const mf_hrec_$TAG* __restrict__ rec,
+       const int32_t* __restrict__ rpb,
+       const int32_t* __restrict__ diag_pos, const uint8_t* __restrict__ nb0,
        double* __restrict__ values, const double diag, const int flags)
 {
   constexpr int K1 = HX_K + 1, ND = HX_ND, M1 = HX_NQ1, NQ = M1 * M1 * M1;
@@ -113,10 +96,8 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
 #endif
   __shared__ __attribute__((aligned(16))) double acc[CAP];
   __shared__ int32_t lrp[MAXR + 1];
-  __shared__ int32_t lcols[CAP + 1];
   __shared__ mf_hrec_$TAG srec[NB];
   __shared__ real sD[NB * NQ * DS];
-  __shared__ int32_t scell[NB];
   __shared__ int32_t spair[MAXP];
 
   const int k = mf_xcd_chunk(nchunks);
@@ -137,20 +118,31 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
     for (int p = tid; p < (seglen + 1) / 2; p += nt) a2[p] = z;
   }
   for (int p = tid; p <= nr; p += nt) lrp[p] = rpb[r0 + p] - cb0;
-  for (int p = tid; p < ncb; p += nt) lcols[p] = colsb[cb0 + p];
+
+  // cell records of a batch, fetched into registers one batch ahead (the
+  // next batch's loads are in flight while this batch's tiles run)
+  constexpr int NTH = $HBLOCK, PFH = (NB * RW + NTH - 1) / NTH;
+  int4 pfr[PFH];
+  {
+#pragma clang loop unroll(full)
+    for (int t = 0; t < PFH; ++t) {
+      // clamped, unconditional loads keep pfr in registers (every chunk has >= 1 item)
+      const int u = min(tid + t * NTH, min(NB, i1 - i0) * RW - 1);
+      const int b = u / RW, w = u - b * RW;
+      pfr[t] = reinterpret_cast<const int4*>(rec)[(int64_t)item_cell[i0 + b] * RW + w];
+    }
+  }
 
   int pb = 0;
   for (int ib = i0; ib < i1; ib += NB) {
     const int nb = min(NB, i1 - ib);
-    __syncthreads();
-    for (int b = tid; b < nb; b += nt) scell[b] = item_cell[ib + b];
-    __syncthreads();
+    __syncthreads();                                 // previous batch done with srec / sD
     {
-      const int4* src = reinterpret_cast<const int4*>(rec);
       int4* dst = reinterpret_cast<int4*>(srec);
-      for (int u = tid; u < nb * RW; u += nt) {
-        const int b = u / RW, w = u - b * RW;
-        dst[b * RW + w] = src[(int64_t)scell[b] * RW + w];
+#pragma clang loop unroll(full)
+      for (int t = 0; t < PFH; ++t) {
+        const int u = tid + t * NTH;
+        if (u < nb * RW) dst[u] = pfr[t];
       }
     }
     int pe;
@@ -207,6 +199,15 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
       D[7] = (real)0;
     }
     __syncthreads();
+    if (ib + NB < i1) {
+      const int ibx = ib + NB, nbx = min(NB, i1 - ib - NB);
+#pragma clang loop unroll(full)
+      for (int t = 0; t < PFH; ++t) {
+        const int u = min(tid + t * NTH, nbx * RW - 1);
+        const int b = u / RW, w = u - b * RW;
+        pfr[t] = reinterpret_cast<const int4*>(rec)[(int64_t)item_cell[ibx + b] * RW + w];
+      }
+    }
     // contraction: one wave per 16-pair tile, all 64 columns (4 tiles of 16)
     const int np = pe - pb;
     const int ntile = (np + 15) / 16;
@@ -263,11 +264,15 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
           const mf_hrec_$TAG& R = srec[bq];
           if (!((R.bits[iq >> 5] >> (iq & 31)) & 1u)) {
             const int lr = R.d[iq] - r0;
-            const int lo = lrp[lr], n = lrp[lr + 1] - lo;
+            const int lo = lrp[lr];
             const uint32_t cbits = (R.bits[jc >> 1] >> (16 * (jc & 1))) & 0xFFFFu;
+            // row-relative column positions of this pair (precomputed), 32 bytes
+            const uint4* pp = reinterpret_cast<const uint4*>(pair_pos + ((int64_t)p0 + pcc) * ND + 16 * jc);
+            const uint4 w0 = pp[0], w1 = pp[1];
+            const uint32_t pw[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
             int pos[16];
 #pragma unroll
-            for (int jj = 0; jj < 16; ++jj) pos[jj] = mf_hex_lower_bound(lcols, lo, n, R.d[16 * jc + jj]);
+            for (int jj = 0; jj < 16; ++jj) pos[jj] = lo + (int)((pw[jj >> 1] >> (16 * (jj & 1))) & 0xFFFFu);
 #pragma unroll
             for (int jj = 0; jj < 16; ++jj)
               if (!((cbits >> jj) & 1u)) atomicAdd(&acc[pos[jj]], cacc[jj]);
@@ -361,12 +366,13 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
           const mf_hrec_$TAG& R = srec[bq];
           const bool rowok = rv && !((R.bits[iq >> 5] >> (iq & 31)) & 1u);   // Dirichlet row
           const int lr = rv ? R.d[iq] - r0 : 0;
-          const int lo = lrp[lr], n = lrp[lr + 1] - lo;
+          const int lo = lrp[lr];
+          const uint16_t* pp = pair_pos + ((int64_t)p0 + pcc) * ND;   // precomputed positions
 #pragma unroll
           for (int jt = 0; jt < 4; ++jt) {
             const int j = jt * 16 + (lane & 15);
             ok[rr][jt] = rowok && !((R.bits[j >> 5] >> (j & 31)) & 1u);     // Dirichlet column
-            pos[rr][jt] = mf_hex_lower_bound(lcols, lo, n, R.d[j]);
+            pos[rr][jt] = lo + (int)pp[j];
           }
         }
 #pragma unroll
@@ -384,9 +390,8 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
     for (int lr = tid; lr < nr; lr += nt) {
       const int32_t Rr = r0 + lr;
       if (!nb0[Rr]) continue;
-      const int lo = lrp[lr], hi = lrp[lr + 1];
-      const int p = mf_lds_lower_bound(lcols, lo, hi, Rr);
-      if (p < hi && lcols[p] == Rr) acc[p] = diag;
+      const int pd = diag_pos[Rr];                   // position of (Rr, Rr) in the row
+      if (pd >= 0) acc[lrp[lr] + pd] = diag;
     }
     __syncthreads();
   }

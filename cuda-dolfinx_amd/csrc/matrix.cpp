@@ -14,6 +14,8 @@ struct OwnerPlan {
   DevArray<int32_t> chunk_rows, chunk_items, item_cell, chunk_meta, pair_list;
   DevArray<uint32_t> item_mask;
   DevArray<uint8_t> pair_kk;
+  DevArray<uint16_t> pair_pos;        // hexahedral kernel: row-relative position of each column
+  DevArray<int32_t> diag_pos;         // hexahedral kernel: row-relative position of the diagonal
   int64_t nitems = 0, npairs = 0;
   // host copies used while building matrix schedules
   std::vector<int32_t> h_items, h_cell;
@@ -135,6 +137,9 @@ std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I,
     kkoff[c + 1] = kkoff[c] + ((poff[c + 1] - poff[c]) * nd + 15) / 16 * 16;
   MF_CHECK(kkoff[nch] < INT32_MAX, MF_EUNSUPPORTED, "column offset table exceeds int32 indexing");
   std::vector<uint8_t> pkk(with_kk ? std::max<int64_t>(16, kkoff[nch]) : 16, 0);
+  // hexahedral kernel: position of each of the nd column nodes within the
+  // pair's (scalar) row, relative to the row start, one uint16 per column
+  std::vector<uint16_t> ppos(with_kk ? 8 : std::max<int64_t>(8, np * nd), 0);
   std::vector<int> bad(16, 0);
   unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   auto work = [&](unsigned t) {
@@ -151,13 +156,23 @@ std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I,
           // hexahedral kernel: item offset in the chunk << 8 | local row
           plist[p] = with_kk ? static_cast<int32_t>((static_cast<uint32_t>(cell) << 6) | i)
                              : ((it - ci[c]) << 8) | i;
-          if (!with_kk) {
-            ++p;
-            continue;
-          }
           const int32_t R = d0->host[(int64_t)cell * nd + i];
           const int32_t* rb = A.h_colsb.data() + A.h_rpb[R];
           const int32_t* re = A.h_colsb.data() + A.h_rpb[R + 1];
+          if (!with_kk) {
+            for (int j = 0; j < nd; ++j) {
+              const int32_t Cn = d1->host[(int64_t)cell * nd + j];
+              const int32_t* f = std::lower_bound(rb, re, Cn);
+              if (f == re || *f != Cn || f - rb > 65535) {
+                bad[t % 16] = 1;
+                ppos[p * nd + j] = 0;
+              } else {
+                ppos[p * nd + j] = static_cast<uint16_t>(f - rb);
+              }
+            }
+            ++p;
+            continue;
+          }
           for (int j = 0; j < nd; ++j) {
             const int32_t Cn = d1->host[(int64_t)cell * nd + j];
             const int32_t* f = std::lower_bound(rb, re, Cn);
@@ -194,6 +209,17 @@ std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I,
   plan->chunk_meta.upload(meta);
   plan->pair_list.upload(plist);
   plan->pair_kk.upload(pkk);
+  plan->pair_pos.upload(ppos);
+  if (!with_kk) {
+    std::vector<int32_t> dp(std::max<int64_t>(1, A.nrows_b), -1);
+    for (int64_t R = 0; R < A.nrows_b; ++R) {
+      const int32_t* rb = A.h_colsb.data() + A.h_rpb[R];
+      const int32_t* re = A.h_colsb.data() + A.h_rpb[R + 1];
+      const int32_t* f = std::lower_bound(rb, re, static_cast<int32_t>(R));
+      if (f != re && *f == R) dp[R] = static_cast<int32_t>(f - rb);
+    }
+    plan->diag_pos.upload(dp);
+  }
   plan->npairs = np;
   plan->h_items.clear(); plan->h_cell.clear(); plan->h_mask.clear();
   return plan;
@@ -417,10 +443,11 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
       launch(owner->k_hprep, static_cast<uint32_t>((owner->n + 255) / 256), 256, 0, s,
              static_cast<int32_t>(owner->n), owner->d_ents.ptr, a->mesh->cells.ptr,
              a->mesh->x.ptr, a->dm[0]->dev.ptr, n0, owner->records.ptr);
-      launch(owner->k_howner[v], static_cast<uint32_t>(per * nx), 256, 0, s,
+      launch(owner->k_howner[v], static_cast<uint32_t>(per * nx),
+             static_cast<uint32_t>(a->opt("hex_block", 256)), 0, s,
              static_cast<int32_t>(plan->nchunks), plan->chunk_meta.ptr, plan->item_cell.ptr,
-             plan->pair_list.ptr, owner->records.ptr, A->rpb.ptr, A->colsb.ptr, n0, A->values,
-             diagonal, flags);
+             plan->pair_list.ptr, plan->pair_pos.ptr, owner->records.ptr, A->rpb.ptr,
+             plan->diag_pos.ptr, n0, A->values, diagonal, flags);
     } else {
     launch(owner->k_prep, static_cast<uint32_t>((owner->n + 255) / 256), 256, 0, s,
            static_cast<int32_t>(owner->n), owner->d_ents.ptr, a->mesh->cells.ptr, a->mesh->x.ptr,

@@ -180,16 +180,17 @@ int mf_form_create(mf_context* ctx, mf_mesh* mesh, int rank, mf_dofmap* dm0, mf_
                    mf_form** out) {
   return guard([&] {
     MF_CHECK(ctx && mesh && out, MF_EINVAL, "null argument");
-    MF_CHECK(rank == 1 || rank == 2, MF_EUNSUPPORTED, "only rank 1 and 2 forms are supported");
-    MF_CHECK(dm0, MF_EINVAL, "rank >= 1 needs a test-space dofmap");
-    MF_CHECK(rank == 1 || dm1, MF_EINVAL, "rank 2 needs a trial-space dofmap");
-    MF_CHECK(dm0->num_cells == mesh->num_cells && (!dm1 || dm1->num_cells == mesh->num_cells),
+    MF_CHECK(rank >= 0 && rank <= 2, MF_EUNSUPPORTED, "only rank 0, 1 and 2 forms are supported");
+    MF_CHECK(rank == 0 || dm0, MF_EINVAL, "rank >= 1 needs a test-space dofmap");
+    MF_CHECK(rank != 2 || dm1, MF_EINVAL, "rank 2 needs a trial-space dofmap");
+    MF_CHECK((!dm0 || dm0->num_cells == mesh->num_cells) &&
+                 (!dm1 || dm1->num_cells == mesh->num_cells),
              MF_ESHAPE, "dofmap cell count does not match the mesh");
     auto f = std::make_unique<mf_form>();
     f->ctx = ctx;
     f->mesh = mesh;
     f->rank = rank;
-    f->dm[0] = dm0;
+    f->dm[0] = rank >= 1 ? dm0 : nullptr;
     f->dm[1] = rank == 2 ? dm1 : nullptr;
     f->uid = g_uid++;
     *out = f.release();
@@ -363,11 +364,24 @@ int mf_form_compile(mf_form* f, int verbose) {
             I->records.alloc(f->mesh->num_cells * I->record_bytes / 8);
           }
         }
+      } else if (f->rank == 0) {
+        I->k_scl = f->mod->get(mf::kernel_name("scl", I->type, I->index));
       } else {
         I->k_vec = f->mod->get(mf::kernel_name("vec", I->type, I->index));
         if (I->type == MF_CELL && f->dm[0]->nd <= 32)
           I->k_vown = f->mod->get(mf::kernel_name("vown", I->type, I->index));
       }
+    }
+    if (f->rank == 0) {
+      // one partial per workgroup; at most 1024 workgroups per integral
+      int64_t total = 0;
+      for (auto& I : f->ints) {
+        I->scl_grid = static_cast<uint32_t>(std::max<int64_t>(1, std::min<int64_t>(1024, (I->n + 255) / 256)));
+        I->scl_off = total;
+        total += I->scl_grid;
+      }
+      f->partials.alloc(total);
+      f->scalar_out.alloc(1);
     }
     if (verbose)
       std::fprintf(stderr, "[mifem] form %llu: %zu integrals, code object %zu bytes%s\n",
@@ -548,6 +562,24 @@ int mf_assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* 
     MF_CHECK(a && A, MF_EINVAL, "null argument");
     use_device(a->ctx);
     mf::assemble_matrix(a, A, bc0, bc1, diag, zero, S(s));
+  });
+}
+
+int mf_assemble_scalar(mf_form* M, double* value, mf_stream s) {
+  return guard([&] {
+    MF_CHECK(M && value, MF_EINVAL, "null argument");
+    use_device(M->ctx);
+    mf::assemble_scalar(M, M->scalar_out.ptr, 1, S(s));
+    MF_HIP(hipMemcpyAsync(value, M->scalar_out.ptr, sizeof(double), hipMemcpyDeviceToHost, S(s)));
+    MF_HIP(hipStreamSynchronize(S(s)));
+  });
+}
+
+int mf_assemble_scalar_device(mf_form* M, double* d_value, int zero, mf_stream s) {
+  return guard([&] {
+    MF_CHECK(M && d_value, MF_EINVAL, "null argument");
+    use_device(M->ctx);
+    mf::assemble_scalar(M, d_value, zero, S(s));
   });
 }
 

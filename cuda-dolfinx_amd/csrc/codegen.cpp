@@ -134,6 +134,32 @@ $GATHER
 }
 )";
 
+// Rank 0: each workgroup sums its entities' values (grid-stride loop, fixed
+// entity-to-thread assignment), reduces through a wave64 shuffle tree and
+// an LDS step, and writes one partial; a fixed-order sum of the partials
+// follows (no floating-point atomics, so results repeat bit for bit).
+const char* kScalar = R"(
+extern "C" __global__ void __launch_bounds__(256)
+$KNAME(const int32_t n, const int32_t* __restrict__ ents, const int32_t* __restrict__ geo,
+       const double* __restrict__ xg, const double* __restrict__ coeffs,
+       const double* __restrict__ consts, double* __restrict__ partial,
+       const uint8_t* __restrict__ fperm)
+{
+  double acc = 0.0;
+  for (int e = blockIdx.x * 256 + threadIdx.x; e < n; e += gridDim.x * 256) {
+$GATHER
+    double v[1] = {0.0};
+    $TAB(v, w, consts, xc, eli, qperm);
+    acc += v[0];
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+)";
+
 const char* kLift = R"(
 extern "C" __global__ void __launch_bounds__($BLOCK)
 $KNAME(const int32_t nlist, const int32_t* __restrict__ list, const int32_t* __restrict__ ents,
@@ -280,6 +306,7 @@ extern const char* kDGCaseExtVec;
 extern const char* kDGIntVecBody;
 
 bool dg_owner_emitted(const mf_form& f) {
+  if (f.rank == 0) return false;
   // one thread per cell keeps the interior-facet tensor (2 nd bs)^2 in registers
   if (f.rank == 2 && f.dm[0] != f.dm[1]) return false;
   const int ndb = f.dm[0]->nd * f.dm[0]->bs;
@@ -338,6 +365,22 @@ std::string generate_form_source(const mf_form& f) {
     }
     m["$KNAME"] = f.rank == 2 ? "mf_dgown" : "mf_dgvown";
     os << subst(subst(f.rank == 2 ? kDGOwnerMatrix : kDGOwnerVector, {{"$CASES", cases}}), m);
+  }
+  if (f.rank == 0) {
+    for (auto& I : f.ints) {
+      std::map<std::string, std::string> m;
+      m["$NV"] = std::to_string(f.mesh->nv);
+      m["$NF"] = std::to_string(std::max(1, f.mesh->nf));
+      m["$W"] = std::to_string(f.W);
+      m["$TAB"] = I->tab_name;
+      std::string gather = I->type == MF_CELL ? kGatherCell
+                           : I->type == MF_EXTERIOR_FACET ? kGatherExt
+                                                          : kGatherInt;
+      replace_all(gather, "$INTERLACE", interlace);
+      m["$KNAME"] = kernel_name("scl", I->type, I->index);
+      os << subst(subst(kScalar, {{"$GATHER", gather}}), m);
+    }
+    return os.str();
   }
   for (auto& I : f.ints) {
     const int ncell = entity_ncell(I->type);

@@ -540,6 +540,19 @@ void assemble_vector(mf_form* L, double* b, int64_t n, int zero, hipStream_t s) 
   }
 }
 
+// Rank-0 assembly (CUDAAssembler.h:296-351, CUDAFormIntegral.cpp:68-266):
+// per integral, in cell / exterior / interior order, one partial per
+// workgroup; then one fixed-order sum of all partials into d_value.
+void assemble_scalar(mf_form* M, double* d_value, int zero, hipStream_t s) {
+  MF_CHECK(M->rank == 0, MF_EINVAL, "assemble_scalar needs a functional (rank-0 form)");
+  MF_CHECK(M->compiled, MF_ESTATE, "form is not compiled");
+  for (auto& I : M->ints)
+    launch(I->k_scl, I->scl_grid, 256, 0, s, static_cast<int32_t>(I->n), I->d_ents.ptr,
+           M->mesh->cells.ptr, M->mesh->x.ptr, M->packed.ptr, M->consts.ptr,
+           M->partials.ptr + I->scl_off, M->mesh->fperm.ptr);
+  k_sum_partials(M->partials.ptr, M->partials.n, d_value, zero ? 0 : 1, s);
+}
+
 }  // namespace mf
 
 const uint8_t* mf_bc::bits_for(int bs, hipStream_t s) const {

@@ -104,6 +104,8 @@ void launch(hipFunction_t f, uint32_t grid, uint32_t block, uint32_t shmem, hipS
 // (precompiled for gfx950 in util_kernels.hip; launchers run on `s`)
 void k_fill(double* x, int64_t n, double v, hipStream_t s);
 void k_fill_i32(int32_t* x, int64_t n, int32_t v, hipStream_t s);
+// out[0] (+)= sum of partials[0..n) in a fixed order (one workgroup)
+void k_sum_partials(const double* partials, int64_t n, double* out, int accumulate, hipStream_t s);
 void k_mark(int8_t* markers, int64_t ndofs, const int32_t* dofs, int64_t nbc, hipStream_t s);
 void k_node_bits(uint8_t* bits, const int8_t* markers, int64_t nnodes, int bs, hipStream_t s);
 void k_pack(double* packed, int64_t ncells, int W, int off, const double* values,
@@ -179,6 +181,9 @@ struct Integral {
   hipFunction_t k_mat = nullptr, k_vec = nullptr, k_lift = nullptr, k_owner = nullptr;
   hipFunction_t k_vown = nullptr, k_slift = nullptr, k_prep = nullptr;
   hipFunction_t k_hprep = nullptr, k_howner[3] = {nullptr, nullptr, nullptr};  // FP64, FP32, BF16
+  hipFunction_t k_scl = nullptr;      // rank 0: per-block partial sums
+  int64_t scl_off = 0;                // rank 0: first partial of this integral
+  uint32_t scl_grid = 1;              // rank 0: workgroups (one partial each)
   DevArray<double> records;           // per-cell records of the owner-computes kernel
   int64_t owner_grid = 0;             // resident workgroups of the owner kernel (0: not queried)
   int64_t record_bytes = 0;
@@ -221,6 +226,8 @@ struct mf_form {
   hipFunction_t k_dgown = nullptr;    // cell-owner kernel over all integrals (DG spaces)
   std::shared_ptr<mf::DGPlan> dgvplan;                    // its vector schedule
   bool dgvplan_tried = false;
+  mf::DevArray<double> partials;      // rank 0: per-workgroup partial sums of every integral
+  mf::DevArray<double> scalar_out;    // rank 0: device result of the host-returning entry
   uint64_t uid = 0;
   int64_t opt(const std::string& k, int64_t dflt) const {
     auto it = options.find(k);
@@ -270,6 +277,7 @@ void build_pattern(const mf_form& a, std::vector<int32_t>& rpb, std::vector<int3
 void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc1,
                      double diagonal, int zero, hipStream_t s);
 void assemble_vector(mf_form* L, double* b, int64_t n, int zero, hipStream_t s);
+void assemble_scalar(mf_form* M, double* d_value, int zero, hipStream_t s);
 std::shared_ptr<OwnerPlan> vector_plan(mf_form* L, Integral& I);
 int entity_stride(int type);
 int entity_ncell(int type);

@@ -26,6 +26,25 @@ __global__ void fill_i32_kernel(int32_t* __restrict__ x, int64_t n, int32_t v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = v;
 }
 
+// Sum of the per-block partial sums of a scalar assembly, in a fixed order
+// (one workgroup: strided per-thread sums, then a wave64 shuffle tree and a
+// cross-wave LDS step), so repeated assemblies give bit-identical values.
+__global__ void __launch_bounds__(kBlock) sum_partials_kernel(const double* __restrict__ p,
+                                                              int64_t n, double* __restrict__ out,
+                                                              int accumulate) {
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += kBlock) acc += p[i];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  __shared__ double red[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < kBlock / 64; ++k) t += red[k];
+    out[0] = accumulate ? out[0] + t : t;
+  }
+}
+
 __global__ void mark_kernel(int8_t* __restrict__ m, int64_t ndofs, const int32_t* __restrict__ dofs,
                             int64_t nbc) {
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -172,6 +191,10 @@ void k_fill(double* x, int64_t n, double v, hipStream_t s) {
 void k_fill_i32(int32_t* x, int64_t n, int32_t v, hipStream_t s) {
   if (n <= 0) return;
   fill_i32_kernel<<<grid_for(n, 4), kBlock, 0, s>>>(x, n, v);
+  MF_HIP(hipGetLastError());
+}
+void k_sum_partials(const double* partials, int64_t n, double* out, int accumulate, hipStream_t s) {
+  sum_partials_kernel<<<1, kBlock, 0, s>>>(partials, n, out, accumulate);
   MF_HIP(hipGetLastError());
 }
 void k_mark(int8_t* m, int64_t ndofs, const int32_t* dofs, int64_t nbc, hipStream_t s) {

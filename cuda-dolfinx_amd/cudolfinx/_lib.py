@@ -57,6 +57,8 @@ SIGNATURES = {
     "mf_zero": (C.c_int, [_vp, _vp, _i64, _vp]),
     "mf_assemble_matrix": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp]),
     "mf_assemble_vector": (C.c_int, [_vp, _vp, _i64, C.c_int, _vp]),
+    "mf_assemble_scalar": (C.c_int, [_vp, _vp, _vp]),
+    "mf_assemble_scalar_device": (C.c_int, [_vp, _vp, C.c_int, _vp]),
     "mf_apply_lifting": (C.c_int, [C.POINTER(_vp), C.c_int, _vp, _i64, C.POINTER(_vp),
                                    C.POINTER(_vp), C.c_double, _vp]),
     "mf_set_bc": (C.c_int, [_vp, _vp, _i64, _vp, _vp, C.c_double, _vp]),

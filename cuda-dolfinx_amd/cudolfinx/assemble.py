@@ -102,6 +102,39 @@ class CUDAAssembler:
         check(lib().mf_assemble_vector(b.handle, vec.ptr, vec.size, int(bool(zero)), self._stream))
         return vec
 
+    # ------------------------------------------------------------ functionals
+    def assemble_scalar(self, b: CUDAForm, constants=None, coeffs=None) -> float:
+        """Value of a functional (rank-0 form) (assemble.py:204-226).
+
+        Synchronizes the assembler's stream and returns a Python float; the
+        per-workgroup partial sums are added in a fixed order, so the value
+        repeats exactly from call to call.
+        """
+        if not isinstance(b, CUDAForm):
+            raise TypeError(f"Expected CUDAForm, got '{type(b)}'")
+        if b.rank != 0:
+            raise ValueError(f"assemble_scalar needs a rank-0 form, got rank {b.rank}")
+        b.to_device(force=False)
+        self.pack_coefficients(b, coeffs)
+        out = C.c_double(0.0)
+        check(lib().mf_assemble_scalar(b.handle, C.byref(out), self._stream))
+        return out.value
+
+    def assemble_scalar_device(self, b: CUDAForm, out, zero: bool = True, coeffs=None):
+        """out[0] = (or +=) the functional's value, on the device, without a host
+        synchronization; `out` is a device buffer (an object with `data_ptr()`
+        such as a float64 torch tensor, or a raw address)."""
+        if not isinstance(b, CUDAForm):
+            raise TypeError(f"Expected CUDAForm, got '{type(b)}'")
+        if b.rank != 0:
+            raise ValueError(f"assemble_scalar needs a rank-0 form, got rank {b.rank}")
+        b.to_device(force=False)
+        self.pack_coefficients(b, coeffs)
+        ptr = out.data_ptr() if hasattr(out, "data_ptr") else int(out)
+        check(lib().mf_assemble_scalar_device(b.handle, C.c_void_p(ptr), int(bool(zero)),
+                                              self._stream))
+        return out
+
     def apply_lifting(self, b: CUDAVector, a: typing.List[CUDAForm], bcs: typing.List,
                       x0: typing.Optional[typing.List[CUDAVector]] = None, scale: float = 1.0,
                       coeffs=None):

@@ -93,9 +93,10 @@ class CUDAForm:
         args = dict(DEFAULT_CUDA_JIT_ARGS)
         args.update(jit_args or {})
         h = C.c_void_p()
+        dm0 = self._dofmaps[0].handle if form.rank >= 1 else None
         dm1 = self._dofmaps[1].handle if form.rank == 2 else None
         check(lib().mf_form_create(self._ctx.handle, self._cuda_mesh.handle, form.rank,
-                                   self._dofmaps[0].handle, dm1, C.byref(h)))
+                                   dm0, dm1, C.byref(h)))
         self._handle = h
         self._entities = []
         for I in form.integrals:

@@ -92,6 +92,30 @@ def sipg_plaplace(V: FunctionSpace, u: Function, f: float = 1.0, alpha: float = 
                   penalty: float | None = None, p: float = 3.0, eps: float = 1e-3,
                   mu0: float = 1.0):
     """(J, F): Jacobian and residual forms of the SIPG p-Laplacian (module doc)."""
+    cell, ext, inter = _sipg_parts(V, u, f, alpha, penalty, p, eps, mu0)
+    J = Form(2, [V, V], [cell(2), ext(2), inter(2)], [u], [], "sipg_plaplace_J")
+    F = Form(1, [V], [cell(1), ext(1), inter(1)], [u], [], "sipg_plaplace_F")
+    return J, F
+
+
+def sipg_energy(V: FunctionSpace, u: Function, f: float = 1.0, alpha: float = 2.0,
+                penalty: float | None = None, p: float = 3.0, eps: float = 1e-3,
+                mu0: float = 1.0) -> Form:
+    """E(u), the functional whose first variation is the residual F of
+    `sipg_plaplace` (same parameters):
+
+      E = sum_K int_K (eps^2 + |grad u|^2)^(p/2) / p + mu0 |grad u|^2 / 2 - f u dx
+        + sum_{F interior} int_F (sigma / h_F) [u]^2 / 2 - {grad u}.n [u] ds
+        + sum_{F boundary} int_F alpha u^2 / 2 ds
+
+    Newton with a line search on E uses it as the merit function.
+    """
+    cell, ext, inter = _sipg_parts(V, u, f, alpha, penalty, p, eps, mu0)
+    return Form(0, [], [cell(0), ext(0), inter(0)], [u], [], "sipg_energy", domain=V.mesh)
+
+
+def _sipg_parts(V, u, f, alpha, penalty, p, eps, mu0):
+    """Builders cell(rank), ext(rank), inter(rank) of the SIPG integrals."""
     mesh = V.mesh
     el = V.element
     if mesh.cell_type != "tetrahedron" or not el.discontinuous or V.bs != 1:
@@ -123,10 +147,16 @@ def sipg_plaplace(V: FunctionSpace, u: Function, f: float = 1.0, alpha: float = 
         name = f"tabulate_tensor_integral_{_uid('dg_cell', rank, k, p, eps, f, mu0)}"
         out = (f"        A[i * {nd} + j] += fw * (mu * (g[i][0] * g[j][0] + g[i][1] * g[j][1] + g[i][2] * g[j][2])\n"
                f"                               + dmu * gui[i] * gui[j]);\n" if rank == 2 else "")
-        loop = (f"    for (int i = 0; i < {nd}; ++i)\n      for (int j = 0; j < {nd}; ++j)\n{out}"
-                if rank == 2 else
-                f"    for (int i = 0; i < {nd}; ++i)\n"
-                f"      A[i] += fw * (mu * gui[i] - {float(f)!r} * FE_phi[iq][i]);\n")
+        if rank == 2:
+            loop = f"    for (int i = 0; i < {nd}; ++i)\n      for (int j = 0; j < {nd}; ++j)\n{out}"
+        elif rank == 1:
+            loop = (f"    for (int i = 0; i < {nd}; ++i)\n"
+                    f"      A[i] += fw * (mu * gui[i] - {float(f)!r} * FE_phi[iq][i]);\n")
+        else:
+            loop = (f"    double uq = 0.0;\n    for (int i = 0; i < {nd}; ++i) uq += w[i] * FE_phi[iq][i];\n"
+                    f"    A[0] += fw * (pow(s2, {float(p) / 2.0!r}) / {float(p)!r}"
+                    f" + {0.5 * float(mu0)!r} * (gu[0] * gu[0] + gu[1] * gu[1] + gu[2] * gu[2])"
+                    f" - {float(f)!r} * uq);\n")
         body = (_carr("FE_w", qc.weights) + _carr("FE_phi", phic) + _carr("FE_dphi", dphic)
                 + _geom("", "coordinate_dofs") + f"""
   (void)h;
@@ -152,9 +182,12 @@ def sipg_plaplace(V: FunctionSpace, u: Function, f: float = 1.0, alpha: float = 
         if rank == 2:
             loop = (f"    for (int i = 0; i < {nd}; ++i)\n      for (int j = 0; j < {nd}; ++j)\n"
                     f"        A[i * {nd} + j] += fw * ph[i] * ph[j];\n")
-        else:
+        elif rank == 1:
             loop = (f"    double uq = 0.0;\n    for (int j = 0; j < {nd}; ++j) uq += w[j] * ph[j];\n"
                     f"    for (int i = 0; i < {nd}; ++i) A[i] += fw * uq * ph[i];\n")
+        else:
+            loop = (f"    double uq = 0.0;\n    for (int j = 0; j < {nd}; ++j) uq += w[j] * ph[j];\n"
+                    f"    A[0] += 0.5 * fw * uq * uq;\n")
         body = (_carr("FE_w", qf.weights) + _carr("FE_phi", phif) + _FACET + f"""
   for (int iq = 0; iq < {nqf}; ++iq) {{
     const double* ph = FE_phi[lf0][iq];
@@ -174,9 +207,12 @@ def sipg_plaplace(V: FunctionSpace, u: Function, f: float = 1.0, alpha: float = 
                     f"        A[I * {n2} + J] += fw * (-dn[J] * jmp[I] - dn[I] * jmp[J] + sig * jmp[I] * jmp[J]);\n")
         else:
             loop = (f"    double ju = 0.0, du = 0.0;\n"
-                    f"    for (int J = 0; J < {n2}; ++J) {{ ju += jmp[J] * w[J]; du += dn[J] * w[J]; }}\n"
-                    f"    for (int I = 0; I < {n2}; ++I)\n"
-                    f"      A[I] += fw * (-du * jmp[I] - dn[I] * ju + sig * ju * jmp[I]);\n")
+                    f"    for (int J = 0; J < {n2}; ++J) {{ ju += jmp[J] * w[J]; du += dn[J] * w[J]; }}\n")
+            if rank == 1:
+                loop += (f"    for (int I = 0; I < {n2}; ++I)\n"
+                         f"      A[I] += fw * (-du * jmp[I] - dn[I] * ju + sig * ju * jmp[I]);\n")
+            else:
+                loop += "    A[0] += fw * (0.5 * sig * ju * ju - du * ju);\n"
         body = (_carr("FE_w", qf.weights) + _carr("FE_phi", phif) + _carr("FE_dphi", dphif)
                 + _FACET + "  const int lf1 = entity_local_index[1];\n"
                 + _geom("0", "coordinate_dofs") + _geom("1", "(coordinate_dofs + 12)") + f"""
@@ -198,6 +234,4 @@ def sipg_plaplace(V: FunctionSpace, u: Function, f: float = 1.0, alpha: float = 
         ent = np.ascontiguousarray(mesh.interior_facets.reshape(-1), dtype=np.int32)
         return Integral(INTERIOR_FACET, -1, ent, name, src, None, spec)
 
-    J = Form(2, [V, V], [cell(2), ext(2), inter(2)], [u], [], "sipg_plaplace_J")
-    F = Form(1, [V], [cell(1), ext(1), inter(1)], [u], [], "sipg_plaplace_F")
-    return J, F
+    return cell, ext, inter

@@ -134,10 +134,15 @@ class Form:
     coefficients: List[Function] = field(default_factory=list)
     constants: List[Constant] = field(default_factory=list)
     description: str = ""
+    domain: Optional[Mesh] = None      # integration domain (needed by rank-0 forms)
 
     @property
     def mesh(self) -> Mesh:
-        return self.function_spaces[0].mesh
+        if self.domain is not None:
+            return self.domain
+        if self.function_spaces:
+            return self.function_spaces[0].mesh
+        return self.coefficients[0].function_space.mesh
 
     @property
     def coefficient_offsets(self) -> List[int]:

@@ -95,7 +95,8 @@ int mf_dofmap_destroy(mf_dofmap* dm);
 /* ---------------------------------------------------------------- form */
 /* Replaces CUDAForm<T,U>(ctx, form, ufcx_form, names, sources, indices)
  * (python/cudolfinx/wrappers/fem.cpp:76-96, cpp/cudolfinx/fem/CUDAForm.h:40-87).
- * rank 1: dm1 may be NULL; rank 2: dm0 = test (rows), dm1 = trial (columns). */
+ * rank 0 (functional): dm0 and dm1 may be NULL; rank 1: dm1 may be NULL;
+ * rank 2: dm0 = test (rows), dm1 = trial (columns). */
 int mf_form_create(mf_context* ctx, mf_mesh* mesh, int rank, mf_dofmap* dm0, mf_dofmap* dm1,
                    mf_form** out);
 /* One integral: UFCx tabulate_tensor C source (function `tabulate_name`,
@@ -172,6 +173,17 @@ int mf_assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* 
                        double diagonal, int zero, mf_stream stream);
 /* assemble_vector_on_device (fem.cpp:291-309, CUDAAssembler.h:369-424). */
 int mf_assemble_vector(mf_form* L, double* d_b, int64_t n, int zero, mf_stream stream);
+/* assemble_scalar_on_device (python/cudolfinx/wrappers/fem.cpp:311-323,
+ * CUDAAssembler.h:296-351): the value of a functional (rank-0 form), summed
+ * over its cell, exterior-facet and interior-facet integrals.  Returns the
+ * value on the host in *value after synchronizing `stream`.  Per-workgroup
+ * partial sums are added in a fixed order (no atomics), so repeated calls
+ * give identical values. */
+int mf_assemble_scalar(mf_form* M, double* value, mf_stream stream);
+/* The same, asynchronous: d_value[0] = value (zero != 0) or d_value[0] +=
+ * value, on `stream`, with no host synchronization (usable under graph
+ * capture).  d_value is device memory. */
+int mf_assemble_scalar_device(mf_form* M, double* d_value, int zero, mf_stream stream);
 /* apply_lifting_on_device (fem.cpp:327-355, CUDAAssembler.h:564-603):
  * b -= scale * A_j[:, bc_j] (g_j - x0_j) over every integral of every form.
  * A form whose constrained values are all zero and has no x0 adds exactly

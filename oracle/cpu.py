@@ -90,6 +90,9 @@ def _declare(L):
                                      C.POINTER(Entities), _dp]
     L.or_apply_lifting.argtypes = [C.POINTER(Problem), C.c_int, C.POINTER(Kernel),
                                    C.POINTER(Entities), _i8, _dp, _dp, C.c_double, _dp]
+    L.or_assemble_scalar.argtypes = [C.POINTER(Problem), C.c_int, C.POINTER(Kernel),
+                                     C.POINTER(Entities)]
+    L.or_assemble_scalar.restype = C.c_double
     L.or_set_bc.argtypes = [C.c_int64, _ip, _dp, _dp, C.c_double, _dp]
     L.or_pack_coefficients.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(_ip),
                                        C.POINTER(_dp), C.POINTER(C.c_int), C.c_int64, C.c_int, _dp]
@@ -134,7 +137,7 @@ def basis(cell: int, k: int, X):
 
 # ------------------------------------------------------------- kernels
 _KINDS = {"stiffness": 0, "elasticity": 1, "source": 2, "facet_source": 3,
-          "dg_cell": 4, "dg_ext": 5, "dg_int": 6}
+          "dg_cell": 4, "dg_ext": 5, "dg_int": 6, "fn_cell": 7, "fn_ext": 8, "fn_int": 9}
 _TYPES = {"cell": 0, "exterior_facet": 1, "interior_facet": 2}
 
 
@@ -172,6 +175,10 @@ def kernel_from_spec(spec: dict, coeff_width: int = 0) -> Kernel:
     for name in ("p", "eps", "alpha", "penalty", "fval", "mu0"):
         setattr(k, name, float(spec.get(name, 0.0)))
     k.u_off = int(spec.get("u_off", 0))
+    if "kappa_off" in spec:
+        k.kappa_off = int(spec["kappa_off"])
+    if "v_off" in spec:
+        k.f_off, k.f_degree = int(spec["v_off"]), int(spec["v_degree"])
     k.coeff_width = coeff_width
     return k
 
@@ -182,14 +189,14 @@ def tabulate(spec: dict, w, c, coordinate_dofs, entity_local_index=None):
     nd = lib().or_ndofs(spec["cell"], spec["degree"]) * spec["bs"]
     mult = 2 if spec["kind"] == "dg_int" else 1
     n = nd * mult
-    A = np.zeros(n * n if spec["rank"] == 2 else n)
+    A = np.zeros(n * n if spec["rank"] == 2 else (n if spec["rank"] == 1 else 1))
     w = np.ascontiguousarray(w, dtype=np.float64)
     c = np.ascontiguousarray(c if len(c) else np.zeros(1), dtype=np.float64)
     xc = np.ascontiguousarray(coordinate_dofs, dtype=np.float64)
     eli = (C.c_int * 2)(*(list(entity_local_index or [0, 0]) + [0, 0])[:2])
     lib().or_tabulate(C.byref(k), _p(A, C.c_double), _p(w if w.size else np.zeros(1), C.c_double),
                       _p(c, C.c_double), _p(xc, C.c_double), eli)
-    return A.reshape(n, n) if spec["rank"] == 2 else A
+    return A.reshape(n, n) if spec["rank"] == 2 else (A if spec["rank"] == 1 else A[0])
 
 
 # ------------------------------------------------------------- assembly
@@ -198,21 +205,22 @@ class OracleAssembler:
 
     def __init__(self, form):
         self.form = form
-        V0 = form.function_spaces[0]
-        V1 = form.function_spaces[-1]
-        mesh = V0.mesh
+        V0 = form.function_spaces[0] if form.rank >= 1 else None
+        V1 = form.function_spaces[-1] if form.rank >= 1 else None
+        mesh = form.mesh
         self._keep = []
         self.x = np.ascontiguousarray(mesh.x)
         self.cells = np.ascontiguousarray(mesh.cells, dtype=np.int32)
-        self.dm0 = np.ascontiguousarray(V0.dofmap, dtype=np.int32)
-        self.dm1 = np.ascontiguousarray(V1.dofmap, dtype=np.int32)
+        self.dm0 = np.ascontiguousarray(V0.dofmap if V0 else np.zeros(1), dtype=np.int32)
+        self.dm1 = np.ascontiguousarray(V1.dofmap if V1 else np.zeros(1), dtype=np.int32)
         self.V0, self.V1 = V0, V1
         self.coeffs = np.ascontiguousarray(self.pack_coefficients())
         self.coff = np.ascontiguousarray(np.array(form.coefficient_offsets, dtype=np.int32))
         self.consts = np.ascontiguousarray(np.concatenate([form.constant_values(), np.zeros(1)]))
         P = Problem()
         P.x, P.cells, P.nv = _p(self.x, C.c_double), _p(self.cells, C.c_int32), mesh.nv_per_cell
-        P.dofmap0, P.nd0, P.bs0 = _p(self.dm0, C.c_int32), V0.element.ndofs, V0.bs
+        if V0 is not None:
+            P.dofmap0, P.nd0, P.bs0 = _p(self.dm0, C.c_int32), V0.element.ndofs, V0.bs
         if form.rank == 2:
             P.dofmap1, P.nd1, P.bs1 = _p(self.dm1, C.c_int32), V1.element.ndofs, V1.bs
         P.coeffs = _p(self.coeffs, C.c_double) if self.coeffs.size else None
@@ -289,6 +297,13 @@ class OracleAssembler:
         lib().or_assemble_vector(C.byref(self.P), len(self.form.integrals), self.kernels,
                                  self.ents, _p(b, C.c_double))
         return b
+
+    def assemble_scalar(self) -> float:
+        """Value of a functional: every integral's entity values summed in order."""
+        if self.form.rank != 0:
+            raise ValueError("assemble_scalar needs a rank-0 form")
+        return float(lib().or_assemble_scalar(C.byref(self.P), len(self.form.integrals),
+                                              self.kernels, self.ents))
 
     def apply_lifting(self, b, bcs, x0=None, scale=1.0):
         bc1 = _markers(bcs, self.V1)

@@ -14,7 +14,9 @@
  *     e3=(0,3) e4=(0,2) e5=(0,1), faces opposite vertex i; hexahedra
  *     lexicographic);
  *   - the element integrals the engine generates (stiffness / Helmholtz,
- *     linear elasticity, volume and facet sources, SIPG + p-Laplacian DG),
+ *     linear elasticity, volume and facet sources, SIPG + p-Laplacian DG
+ *     with its energy functional, a functional over cells, exterior and
+ *     interior facets),
  *     evaluated directly at quadrature points, in the UFCx argument layout;
  *   - the sparsity build (DOLFINx create_sparsity_pattern + the bs-unrolling
  *     of la/petsc.cpp:14-65);
@@ -313,7 +315,10 @@ enum {
   OR_FACET_SOURCE = 3,/* g . v on exterior facets                 */
   OR_DG_CELL = 4,     /* p-Laplacian cell Jacobian / residual      */
   OR_DG_EXT = 5,      /* Robin exterior facet                     */
-  OR_DG_INT = 6       /* SIPG interior facet                      */
+  OR_DG_INT = 6,      /* SIPG interior facet                      */
+  OR_FN_CELL = 7,     /* functional: exp(u) kappa dx              */
+  OR_FN_EXT = 8,      /* functional: u kappa ds                   */
+  OR_FN_INT = 9       /* functional: avg(v^2) avg(kappa) dS       */
 };
 
 typedef struct {
@@ -321,7 +326,7 @@ typedef struct {
   int cell;        /* 0 tetrahedron, 1 hexahedron */
   int degree;
   int bs;
-  int rank;        /* 1 vector, 2 matrix */
+  int rank;        /* 0 functional, 1 vector, 2 matrix */
   int qm;          /* quadrature points per direction */
   int kappa_src;   /* 0 value, 1 w[off], 2 c[off] */
   int kappa_off;
@@ -500,6 +505,13 @@ static void tab_dg_cell(const or_kernel* K, double* A, const double* w, const do
       for (int d = 0; d < 3; ++d) gu[d] += w[K->u_off + i] * g[i][d];
     double s = K->eps * K->eps + gu[0] * gu[0] + gu[1] * gu[1] + gu[2] * gu[2];
     double mu = K->mu0 + pow(s, 0.5 * (K->p - 2.0)), dmu = (K->p - 2.0) * pow(s, 0.5 * (K->p - 4.0));
+    if (K->rank == 0) {
+      /* energy density whose first variation is the residual below */
+      double uq = 0.0, g2 = gu[0] * gu[0] + gu[1] * gu[1] + gu[2] * gu[2];
+      for (int i = 0; i < nd; ++i) uq += w[K->u_off + i] * phi[i];
+      A[0] += dx * (pow(s, 0.5 * K->p) / K->p + 0.5 * K->mu0 * g2 - K->fval * uq);
+      continue;
+    }
     for (int i = 0; i < nd; ++i) {
       double gi = gu[0] * g[i][0] + gu[1] * g[i][1] + gu[2] * g[i][2];
       if (K->rank == 1) {
@@ -527,6 +539,7 @@ static void tab_dg_ext(const or_kernel* K, double* A, const double* w, const dou
     tet_basis(K->degree, X, phi, dref);
     double fw = wts[q] * ds * K->alpha, uq = 0.0;
     for (int j = 0; j < nd; ++j) uq += w[K->u_off + j] * phi[j];
+    if (K->rank == 0) { A[0] += 0.5 * fw * uq * uq; continue; }
     for (int i = 0; i < nd; ++i) {
       if (K->rank == 1) { A[i] += fw * uq * phi[i]; continue; }
       for (int j = 0; j < nd; ++j) A[i * nd + j] += fw * phi[i] * phi[j];
@@ -558,6 +571,13 @@ static void tab_dg_int(const or_kernel* K, double* A, const double* w, const dou
       dn[nd + i] = 0.5 * (g1[i][0] * nrm[0] + g1[i][1] * nrm[1] + g1[i][2] * nrm[2]);
     }
     double fw = wts[q] * ds;
+    if (K->rank == 0) {
+      const double* u = w + 2 * K->u_off;
+      double ju = 0.0, du = 0.0;
+      for (int j = 0; j < n2; ++j) { ju += jmp[j] * u[j]; du += dn[j] * u[j]; }
+      A[0] += fw * (0.5 * sig * ju * ju - du * ju);
+      continue;
+    }
     if (K->rank == 1) {
       const double* u = w + 2 * K->u_off;
       double ju = 0.0, du = 0.0;
@@ -568,6 +588,65 @@ static void tab_dg_int(const or_kernel* K, double* A, const double* w, const dou
     for (int i = 0; i < n2; ++i)
       for (int j = 0; j < n2; ++j)
         A[i * n2 + j] += fw * (-dn[j] * jmp[i] - dn[i] * jmp[j] + sig * jmp[i] * jmp[j]);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* a functional over all three integral types (the mixed functional of  */
+/* python/test/test_cuda_assembly.py:66-71): u, kappa in P_degree       */
+/* (offsets u_off, kappa_off), v in DG_f_degree (offset f_off)          */
+/* ------------------------------------------------------------------ */
+
+static double eval_at(int k, const double* X, const double* coef)
+{
+  double phi[OR_MAXND], dref[3 * OR_MAXND];
+  tet_basis(k, X, phi, dref);
+  double v = 0.0;
+  for (int i = 0; i < or_ndofs(0, k); ++i) v += coef[i] * phi[i];
+  return v;
+}
+
+static void tab_fn_cell(const or_kernel* K, double* A, const double* w, const double* xc)
+{
+  static double pts[3 * OR_MAXQ], wts[OR_MAXQ];
+  const int nq = or_tet_rule(K->qm, pts, wts);
+  double Kinv[3][3];
+  for (int q = 0; q < nq; ++q) {
+    const double* X = &pts[3 * q];
+    double dx = wts[q] * fabs(jacobian(0, xc, X, Kinv));
+    A[0] += dx * exp(eval_at(K->degree, X, w + K->u_off)) * eval_at(K->degree, X, w + K->kappa_off);
+  }
+}
+
+static void tab_fn_ext(const or_kernel* K, double* A, const double* w, const double* xc, int lf)
+{
+  double pts[2 * OR_MAXND * OR_MAXND], wts[OR_MAXND * OR_MAXND], nrm[3];
+  const int nq = or_tri_rule(K->qm, pts, wts);
+  double ds = facet_normal(xc, lf, nrm);
+  for (int q = 0; q < nq; ++q) {
+    double X[3];
+    facet_point(lf, pts[2 * q], pts[2 * q + 1], X);
+    A[0] += wts[q] * ds * eval_at(K->degree, X, w + K->u_off) * eval_at(K->degree, X, w + K->kappa_off);
+  }
+}
+
+/* w interlaced per coefficient: (cell 0 values, cell 1 values) at twice the offset */
+static void tab_fn_int(const or_kernel* K, double* A, const double* w, const double* xc,
+                       const int* eli)
+{
+  double pts[2 * OR_MAXND * OR_MAXND], wts[OR_MAXND * OR_MAXND], nrm[3];
+  const int nq = or_tri_rule(K->qm, pts, wts);
+  const int np = or_ndofs(0, K->degree), nv = or_ndofs(0, K->f_degree);
+  double ds = facet_normal(xc, eli[0], nrm);
+  const double* k0 = w + 2 * K->kappa_off;
+  const double* v0 = w + 2 * K->f_off;
+  for (int q = 0; q < nq; ++q) {
+    double X0[3], X1[3];
+    facet_point(eli[0], pts[2 * q], pts[2 * q + 1], X0);
+    facet_point(eli[1], pts[2 * q], pts[2 * q + 1], X1);
+    double va = eval_at(K->f_degree, X0, v0), vb = eval_at(K->f_degree, X1, v0 + nv);
+    double ka = eval_at(K->degree, X0, k0), kb = eval_at(K->degree, X1, k0 + np);
+    A[0] += wts[q] * ds * 0.5 * (va * va + vb * vb) * 0.5 * (ka + kb);
   }
 }
 
@@ -583,6 +662,9 @@ void or_tabulate(const or_kernel* K, double* A, const double* w, const double* c
   case OR_DG_CELL: tab_dg_cell(K, A, w, coordinate_dofs); break;
   case OR_DG_EXT: tab_dg_ext(K, A, w, coordinate_dofs, entity_local_index[0]); break;
   case OR_DG_INT: tab_dg_int(K, A, w, coordinate_dofs, entity_local_index); break;
+  case OR_FN_CELL: tab_fn_cell(K, A, w, coordinate_dofs); break;
+  case OR_FN_EXT: tab_fn_ext(K, A, w, coordinate_dofs, entity_local_index[0]); break;
+  case OR_FN_INT: tab_fn_int(K, A, w, coordinate_dofs, entity_local_index); break;
   default: break;
   }
 }
@@ -838,6 +920,27 @@ int or_assemble_vector(const or_problem* P, int nint, const or_kernel* kernels,
     }
   }
   return 0;
+}
+
+/* Value of a functional: the sum over every integral's entities, in entity
+ * order (CUDAAssembler.h:296-351 sums cells, exterior, interior facets). */
+double or_assemble_scalar(const or_problem* P, int nint, const or_kernel* kernels,
+                          const or_entities* ints)
+{
+  double total = 0.0, xc[2 * 27 * 3], w[1024];
+  int32_t dofs0[OR_MAXT], dofs1[OR_MAXT];
+  int eli[2];
+  for (int it = 0; it < nint; ++it) {
+    const int s = stride_of(ints[it].type);
+    for (int64_t e = 0; e < ints[it].n; ++e) {
+      const int32_t* ent = &ints[it].entities[e * s];
+      gather(P, ints[it].type, ent, xc, w, dofs0, dofs1, eli);
+      double a = 0.0;
+      or_tabulate(&kernels[it], &a, w, P->constants, xc, eli);
+      total += a;
+    }
+  }
+  return total;
 }
 
 /* b -= scale * A[:, bc] (g - x0) over every integral of the bilinear form. */

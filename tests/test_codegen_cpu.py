@@ -51,8 +51,8 @@ def _check_form(form, tmp_path, ncells=5, seed=0):
     consts = form.constant_values()
     for k, I in enumerate(form.integrals):
         lib = _compile(I.source, tmp_path, f"f{k}_{I.name}")
-        V0 = form.function_spaces[0]
-        nd = V0.element.ndofs * V0.bs
+        V0 = form.function_spaces[0] if form.rank else None
+        nd = V0.element.ndofs * V0.bs if V0 else 1
         ne = I.num_entities
         stride = {"cell": 1, "exterior_facet": 2, "interior_facet": 4}[I.type]
         for e in rng.choice(ne, size=min(ncells, ne), replace=False):
@@ -70,7 +70,7 @@ def _check_form(form, tmp_path, ncells=5, seed=0):
                                                     packed[c1][off[j]:off[j + 1]]])
                                     for j in range(len(off) - 1)]) if len(off) > 1 else w
                 eli = [int(ent[1]), int(ent[3])]
-                n = 2 * nd
+                n = 2 * nd if form.rank else 1
             nA = n * n if form.rank == 2 else n
             got = _call(lib, I.name, nd, w, consts, xc, eli, nA)
             ref = cpu.tabulate(I.spec, w, consts, xc, eli).reshape(-1)
@@ -117,6 +117,29 @@ def test_generated_sipg_plaplace(tmp_path, k):
     P.extras["u"].x[:] = rng.uniform(-1, 1, P.V.size)
     _check_form(P.a, tmp_path, ncells=4)
     _check_form(P.L, tmp_path, ncells=4)
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_generated_sipg_energy(tmp_path, k):
+    from mifem.dgforms import sipg_energy
+    P = problems.dg_plaplace(2, degree=k)
+    rng = np.random.default_rng(2)
+    u = P.extras["u"]
+    u.x[:] = rng.uniform(-1, 1, P.V.size)
+    _check_form(sipg_energy(P.V, u), tmp_path, ncells=4)
+
+
+@pytest.mark.parametrize("k,kv", [(1, 1), (2, 1), (1, 2)])
+def test_generated_mixed_functional(tmp_path, k, kv):
+    from mifem.functionals import mixed_functional
+    mesh = create_unit_cube(2, jitter=0.2, seed=4)
+    V, D = functionspace(mesh, ("P", k)), functionspace(mesh, ("DG", kv))
+    u, kap, v = Function(V), Function(V), Function(D)
+    rng = np.random.default_rng(5)
+    u.x[:] = rng.uniform(-1, 1, V.size)
+    kap.x[:] = rng.uniform(1, 2, V.size)
+    v.x[:] = rng.uniform(-1, 1, D.size)
+    _check_form(mixed_functional(u, kap, v), tmp_path, ncells=4)
 
 
 def test_staged_description_matches_oracle():

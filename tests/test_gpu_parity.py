@@ -284,6 +284,105 @@ def test_graph_replay_matches_eager_and_oracle():
         assert ea <= TOL and eb <= TOL
 
 
+@pytest.mark.parametrize("k,kv", [(1, 1), (2, 1), (1, 2)])
+def test_mixed_functional_three_integral_types(k, kv):
+    _say(f"functional exp(u) kappa dx + u kappa ds + avg(v^2) avg(kappa) dS, P{k}/DG{kv}, n=4")
+    import cudolfinx as cufem
+    from mifem.functionals import mixed_functional
+    mesh = create_unit_cube(4, jitter=0.2, seed=4)
+    V, D = functionspace(mesh, ("P", k)), functionspace(mesh, ("DG", kv))
+    u, kap, v = Function(V), Function(V), Function(D)
+    rng = np.random.default_rng(31)
+    u.x[:] = rng.uniform(-1, 1, V.size)
+    kap.x[:] = rng.uniform(1, 2, V.size)
+    v.x[:] = rng.uniform(-1, 1, D.size)
+    M = mixed_functional(u, kap, v)
+    asm = cufem.CUDAAssembler()
+    Mg = cufem.form(M)
+    got = asm.assemble_scalar(Mg)
+    again = asm.assemble_scalar(Mg)
+    ref = cpu.OracleAssembler(M).assemble_scalar()
+    _say(f"  value {got:.15e} oracle {ref:.15e}")
+    assert abs(got - ref) <= 1e-13 * abs(ref), (got, ref)
+    assert got == again, "repeated assembly differs"
+    # a host-side coefficient change reaches the device on the next call
+    u.x[:] *= 0.5
+    u.version += 1
+    ref2 = cpu.OracleAssembler(M).assemble_scalar()
+    got2 = asm.assemble_scalar(Mg)
+    assert abs(got2 - ref2) <= 1e-13 * abs(ref2), (got2, ref2)
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_sipg_energy_functional(k):
+    _say(f"SIPG p-Laplacian energy functional DG{k}, n=6 jittered; device output, accumulate")
+    import torch
+    import cudolfinx as cufem
+    from mifem.dgforms import sipg_energy
+    P = problems.dg_plaplace(6, degree=k)
+    u = P.extras["u"]
+    rng = np.random.default_rng(41)
+    u.x[:] = rng.uniform(-1, 1, P.V.size)
+    E_ = sipg_energy(P.V, u)
+    ref = cpu.OracleAssembler(E_).assemble_scalar()
+    asm = cufem.CUDAAssembler()
+    Eg = cufem.form(E_)
+    got = asm.assemble_scalar(Eg)
+    _say(f"  value {got:.15e} oracle {ref:.15e}")
+    assert abs(got - ref) <= 1e-12 * abs(ref), (got, ref)
+    out = torch.full((1,), 1.5, dtype=torch.float64, device="cuda")
+    asm.assemble_scalar_device(Eg, out, zero=False)
+    asm.synchronize()
+    assert abs(out.item() - (1.5 + got)) <= 1e-12 * abs(1.5 + got)
+    asm.assemble_scalar_device(Eg, out)
+    asm.synchronize()
+    assert out.item() == got
+
+
+def test_graph_newton_step_with_energy():
+    _say("HIP graph: capture SIPG J + F + energy E (device-bound u), 3 replays vs eager and oracle")
+    import torch
+    import cudolfinx as cufem
+    from mifem.dgforms import sipg_energy
+    P = problems.dg_plaplace(3)
+    n = P.V.size
+    s = torch.cuda.Stream()
+    E_ = sipg_energy(P.V, P.extras["u"])
+    J, F, Eg = cufem.form(P.a), cufem.form(P.L), cufem.form(E_)
+    asm = cufem.CUDAAssembler(stream=s)
+    ud = torch.zeros(n, dtype=torch.float64, device="cuda")
+    for f in (J, F, Eg):
+        f.set_coefficient_device(0, ud.data_ptr(), n)
+    A, b = asm.create_matrix(J), asm.create_vector(F)
+    ev = torch.zeros(1, dtype=torch.float64, device="cuda")
+
+    def seq():
+        asm.assemble_matrix(J, A)
+        asm.assemble_vector(F, b)
+        asm.assemble_scalar_device(Eg, ev)
+
+    seq()
+    asm.synchronize()
+    g = cufem.CUDAGraph(s)
+    with g:
+        seq()
+    rng = np.random.default_rng(22)
+    for rep in range(3):
+        u = rng.uniform(-1, 1, n)
+        ud.copy_(torch.from_numpy(u).cuda())
+        torch.cuda.synchronize()
+        g.launch()
+        asm.synchronize()
+        eg = ev.item()
+        seq()
+        asm.synchronize()
+        assert ev.item() == eg, f"replay {rep}: energy differs from eager"
+        P.extras["u"].x[:] = u
+        ref = cpu.OracleAssembler(E_).assemble_scalar()
+        _say(f"  replay {rep}: E={eg:.15e} oracle {ref:.15e}")
+        assert abs(eg - ref) <= 1e-12 * abs(ref)
+
+
 def test_many_instances_block_separate():
     _say("batch of 6 Poisson instances (P1-P3, n=2..4) merged per degree, owner-computes")
     from mifem import batch

@@ -406,3 +406,125 @@ def test_batch_blocks_equal_independent_instances():
             b2 = cpu.OracleAssembler(forms.source(V, Constant([1.0]))).assemble_vector()
             cpu.set_bc(b2, [bc], V)
             assert np.array_equal(b[g.dof_offsets[j]:g.dof_offsets[j + 1]], b2)
+
+
+# ------------------------------------------------------------ functionals (rank 0)
+def _sub(form, k):
+    """The functional restricted to its k-th integral."""
+    return forms.Form(0, [], [form.integrals[k]], form.coefficients, [], domain=form.mesh)
+
+
+def _fn_setup(n, jitter, k=1, kv=1, seed=4):
+    from mifem.functionals import mixed_functional
+    mesh = create_unit_cube(n, jitter=jitter, seed=seed)
+    V, D = functionspace(mesh, ("P", k)), functionspace(mesh, ("DG", kv))
+    u, kap, v = Function(V), Function(V), Function(D)
+    return mesh, u, kap, v, mixed_functional(u, kap, v)
+
+
+def _tri_area(P):
+    return 0.5 * np.linalg.norm(np.cross(P[1] - P[0], P[2] - P[0]))
+
+
+def _int_tri_product(P, fs):
+    """Exact integral over triangle P of a product of linear functions given by
+    their vertex values fs[m][0..2]: int_T lambda^a = 2|T| a! / (|a| + 2)!."""
+    area = _tri_area(P)
+    tot = 0.0
+    for idx in np.ndindex(*(3,) * len(fs)):
+        a = np.bincount(idx, minlength=3)
+        mono = 2 * area * np.prod([math.factorial(t) for t in a]) / math.factorial(len(fs) + 2)
+        tot += np.prod([fs[m][idx[m]] for m in range(len(fs))]) * mono
+    return tot
+
+
+def test_functional_cell_term_closed_form():
+    """int exp(u) kappa dx with u linear, kappa = 1: per tetrahedron
+    int_T e^l = 6 |T| sum_i e^{l_i} / prod_{j != i} (l_i - l_j)."""
+    mesh, u, kap, v, M = _fn_setup(3, 0.0)
+    a = np.array([0.3, -0.5, 0.7])
+    u.interpolate(lambda x: 0.1 + a @ x)
+    kap.x[:] = 1.0
+    got = cpu.OracleAssembler(_sub(M, 0)).assemble_scalar()
+    ref = 0.0
+    for c in mesh.cells:
+        X = mesh.x[c]
+        vol = abs(np.linalg.det(X[1:] - X[0])) / 6.0
+        l = 0.1 + X @ a
+        ref += 6 * vol * sum(np.exp(l[i]) / np.prod([l[i] - l[j] for j in range(4) if j != i])
+                             for i in range(4))
+    assert abs(got - ref) <= 1e-7 * abs(ref), (got, ref)
+    # constant u: exp(c) times the volume of the (jittered) unit cube
+    mesh, u, kap, v, M = _fn_setup(3, 0.2)
+    u.x[:] = 0.4
+    kap.x[:] = 1.0
+    assert abs(cpu.OracleAssembler(_sub(M, 0)).assemble_scalar() - math.exp(0.4)) <= 1e-13
+
+
+def test_functional_exterior_term_closed_form():
+    """int_dOmega u kappa ds with u = y, kappa = z: 1/4 + 1/4 (x faces) + 1/2 (y = 1)
+    + 1/2 (z = 1) = 3/2."""
+    mesh, u, kap, v, M = _fn_setup(3, 0.2)
+    u.interpolate(lambda x: x[1])
+    kap.interpolate(lambda x: x[2])
+    assert abs(cpu.OracleAssembler(_sub(M, 1)).assemble_scalar() - 1.5) <= 1e-13
+
+
+@pytest.mark.parametrize("kv", [1, 2])
+def test_functional_interior_term_exact(kv):
+    """int_F avg(v^2) avg(kappa) over interior facets: kappa = x and a
+    continuous linear v give a cubic, integrated exactly from barycentric
+    monomials; v = 1, kappa = 1 gives the total interior facet area."""
+    mesh, u, kap, v, M = _fn_setup(3, 0.2, kv=kv)
+    b = np.array([0.6, -0.2, 0.9])
+    kap.interpolate(lambda x: x[0])
+    v.interpolate(lambda x: 0.3 + b @ x)
+    got = cpu.OracleAssembler(_sub(M, 2)).assemble_scalar()
+    ref = 0.0
+    for c0, l0, _, _ in mesh.interior_facets:
+        P = mesh.x[mesh.cells[c0][list(_TF[l0])]]
+        vv = 0.3 + P @ b
+        ref += _int_tri_product(P, [P[:, 0], vv, vv])
+    assert abs(got - ref) <= 1e-12 * abs(ref), (got, ref)
+    kap.x[:] = 1.0
+    v.x[:] = 1.0
+    area = sum(_tri_area(mesh.x[mesh.cells[c0][list(_TF[l0])]])
+               for c0, l0, _, _ in mesh.interior_facets)
+    assert abs(cpu.OracleAssembler(_sub(M, 2)).assemble_scalar() - area) <= 1e-12 * area
+
+
+def test_sipg_energy_matches_independent_energy():
+    """DG1: the oracle's energy functional equals the vertex-value energy
+    computed without the oracle (exact for DG1: constant gradients, linear
+    traces)."""
+    from mifem.dgforms import sipg_energy
+    P = _dg_problem(2)
+    spec = P.L.integrals[0].spec
+    E_ = sipg_energy(P.V, P.extras["u"], f=spec["fval"], alpha=spec["alpha"],
+                     penalty=spec["penalty"], p=spec["p"], eps=spec["eps"], mu0=spec["mu0"])
+    got = cpu.OracleAssembler(E_).assemble_scalar()
+    ref = _dg1_energy(P.mesh, P.extras["u"].x, spec["p"], spec["eps"], spec["fval"], spec["alpha"],
+                      spec["penalty"], spec["mu0"])
+    assert abs(got - ref) <= 1e-12 * abs(ref), (got, ref)
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_sipg_energy_gradient_is_residual(k):
+    """d/dh E(u + h d) at h = 0 equals F(u) . d (central differences)."""
+    from mifem.dgforms import sipg_energy
+    P = problems.dg_plaplace(2, degree=k)
+    u = P.extras["u"]
+    rng = np.random.default_rng(11)
+    u.x[:] = rng.uniform(-1, 1, P.V.size)
+    E_ = sipg_energy(P.V, u)
+    u0 = u.x.copy()
+    F = cpu.OracleAssembler(P.L).assemble_vector()
+    d = rng.standard_normal(u0.size)
+    h = 1e-5
+    u.x[:] = u0 + h * d
+    ep = cpu.OracleAssembler(E_).assemble_scalar()
+    u.x[:] = u0 - h * d
+    em = cpu.OracleAssembler(E_).assemble_scalar()
+    u.x[:] = u0
+    dE = (ep - em) / (2 * h)
+    assert abs(dE - F @ d) <= 1e-7 * max(1.0, abs(dE)), (dE, F @ d)

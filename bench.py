@@ -203,11 +203,13 @@ def bench_c3(args):
 
 def bench_c4(args):
     """SIPG DG1 p-Laplacian Newton on 48^3 x 6 jittered tets: the reassembly of the
-    Jacobian J(u) and residual F(u) per Newton step, captured once as a HIP graph.
+    three forms of a Newton step -- Jacobian J(u), residual F(u) and the energy
+    functional E(u) whose gradient is F -- captured once as a HIP graph.
 
-    value = ms per graph replay (pack u for both forms + J + F, all integral
-    types); the eager time of the same sequence, the J kernel alone (HIP events),
-    and the 10-step Newton residual history are reported beside it.
+    value = ms per graph replay (pack u for the three forms + J + F + E, all
+    integral types); the eager time of the same sequence, each form's kernels
+    alone (HIP events), and the 10-step Newton history (|F| and E; line search
+    on E) are reported beside it.
     """
     import numpy as np
     import torch
@@ -215,6 +217,7 @@ def bench_c4(args):
     from cudolfinx import _lib
     from cudolfinx.solve import csr_tensor, pcg
     from mifem import problems
+    from mifem.dgforms import sipg_energy
 
     t0 = time.perf_counter()
     P = problems.dg_plaplace(args.n_dg)
@@ -226,11 +229,13 @@ def bench_c4(args):
     lib = _lib.lib()
     s = torch.cuda.Stream()
     J, F = cufem.form(P.a), cufem.form(P.L)
+    Ef = cufem.form(sipg_energy(P.V, P.extras["u"]))
     asm = cufem.CUDAAssembler(stream=s)
     A = asm.create_matrix(J)
     ud = torch.from_numpy(P.extras["u"].x.copy()).cuda()
-    J.set_coefficient_device(0, ud.data_ptr(), n)
-    F.set_coefficient_device(0, ud.data_ptr(), n)
+    for f in (J, F, Ef):
+        f.set_coefficient_device(0, ud.data_ptr(), n)
+    ev = torch.zeros(1, dtype=torch.float64, device="cuda")
     vals = torch.empty(A.nnz, dtype=torch.float64, device="cuda")
     _lib.check(lib.mf_matrix_borrow_values(A.handle, C.c_void_p(vals.data_ptr())))
     bt = torch.empty(n, dtype=torch.float64, device="cuda")
@@ -243,6 +248,14 @@ def bench_c4(args):
         _lib.check(lib.mf_assemble_matrix(J.handle, A.handle, nob, nob, 1.0, 1, sh))
         _lib.check(lib.mf_form_pack_coefficients(F.handle, None, 0, sh))
         _lib.check(lib.mf_assemble_vector(F.handle, C.c_void_p(bt.data_ptr()), n, 1, sh))
+        _lib.check(lib.mf_form_pack_coefficients(Ef.handle, None, 0, sh))
+        _lib.check(lib.mf_assemble_scalar_device(Ef.handle, C.c_void_p(ev.data_ptr()), 1, sh))
+
+    def energy_and_residual():
+        _lib.check(lib.mf_form_pack_coefficients(F.handle, None, 0, sh))
+        _lib.check(lib.mf_assemble_vector(F.handle, C.c_void_p(bt.data_ptr()), n, 1, sh))
+        _lib.check(lib.mf_form_pack_coefficients(Ef.handle, None, 0, sh))
+        _lib.check(lib.mf_assemble_scalar_device(Ef.handle, C.c_void_p(ev.data_ptr()), 1, sh))
 
     seq()
     s.synchronize()
@@ -272,46 +285,52 @@ def bench_c4(args):
     jk_ms = timed(lambda: lib.mf_assemble_matrix(J.handle, A.handle, nob, nob, 1.0, 1, sh), args.steps)
     fk_ms = timed(lambda: lib.mf_assemble_vector(F.handle, C.c_void_p(bt.data_ptr()), n, 1, sh),
                   args.steps)
+    ek_ms = timed(lambda: lib.mf_assemble_scalar_device(Ef.handle, C.c_void_p(ev.data_ptr()), 1, sh),
+                  args.steps)
     log(f"step: eager {eager_ms:.3f} ms, graph {graph_ms:.3f} ms; J kernel {jk_ms:.3f} ms, "
-        f"F kernel {fk_ms:.3f} ms")
+        f"F kernel {fk_ms:.3f} ms, E kernels {ek_ms:.3f} ms")
 
-    # Newton (damped by residual backtracking), each step = one graph replay + a CG solve
-    history = []
-    M = csr_tensor(A, vals)
+    # Newton with an Armijo line search on the energy (F = grad E, J = Hessian,
+    # J is SPD, so the Newton direction descends); one graph replay per step
+    history, energies = [], []
     for it in range(args.newton):
         g.launch()
         s.synchronize()
-        r0 = float(torch.linalg.norm(bt))
+        r0, e0 = float(torch.linalg.norm(bt)), float(ev.item())
         history.append(r0)
+        energies.append(e0)
         M = csr_tensor(A, vals)
-        du, cg_it, cg_res = pcg(M, -bt.clone(), rtol=1e-8, maxiter=args.cg_maxiter)
+        du, cg_it, cg_res = pcg(M, -bt.clone(), rtol=1e-10, maxiter=args.cg_maxiter)
+        slope = float(torch.dot(bt, du))
         u_old = ud.clone()
         alpha = 1.0
-        for _ in range(6):
+        for _ in range(12):
             ud.copy_(u_old + alpha * du)
-            seq()
+            energy_and_residual()
             s.synchronize()
-            if float(torch.linalg.norm(bt)) < r0:
+            if float(ev.item()) <= e0 + 1e-4 * alpha * slope:
                 break
             alpha *= 0.5
-        log(f"Newton {it}: |F| = {r0:.3e}, CG {cg_it} its (res {cg_res:.1e}), step {alpha}")
+        log(f"Newton {it}: |F| = {r0:.3e}, E = {e0:.12e}, CG {cg_it} its (res {cg_res:.1e}), "
+            f"step {alpha}")
     g.launch()
     s.synchronize()
     history.append(float(torch.linalg.norm(bt)))
+    energies.append(float(ev.item()))
 
     nc = mesh.num_cells
     alg = (8 * A.nnz + 8 * nc + 16 * (nc + 2 * nint + next_) + 24 * mesh.num_vertices + 16 * nc
            + 8 * 4 * nc)
     out = {
-        "metric": "hipGraph-captured Newton reassembly step time (SIPG DG1 J + F)",
+        "metric": "hipGraph-captured Newton reassembly step time (SIPG DG1 J + F + energy)",
         "value": graph_ms, "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": graph_ms, "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded jitter, interpolated state)",
         "config": {"workload": f"sipg_plaplace DG1 {args.n_dg}^3x6 jittered tets, p=3, Robin alpha=2",
                    "cells": nc, "dofs": n, "nnz": A.nnz, "interior_facets": nint,
                    "exterior_facets": next_, "eager_ms": eager_ms, "graph_wall_ms": wall_ms,
-                   "J_kernel_ms": jk_ms, "F_kernel_ms": fk_ms,
-                   "newton_residuals": history},
+                   "J_kernel_ms": jk_ms, "F_kernel_ms": fk_ms, "E_kernels_ms": ek_ms,
+                   "newton_residuals": history, "newton_energies": energies},
         "roofline": {"bound": "hbm", "achieved": alg / (jk_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": alg / (jk_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "traffic": None, "kernel": "mf_dgown (cell-owner J over cells + facets)",

@@ -346,6 +346,9 @@ std::string generate_form_source(const mf_form& f) {
     m["$ND0"] = std::to_string(d0->nd);
     m["$BS0"] = std::to_string(d0->bs);
     m["$DBLOCK"] = std::to_string(f.opt("dg_block", 64));
+    // optional register cap: at least dg_wpe waves per SIMD
+    const int64_t dwpe = f.opt("dg_wpe", 0);
+    m["$DATTR"] = dwpe > 0 ? "__attribute__((amdgpu_waves_per_eu(" + std::to_string(dwpe) + ")))" : "";
     std::string cases;
     for (auto& I : f.ints) {
       std::map<std::string, std::string> c{{"$K", std::to_string(I->index)}, {"$TAB", I->tab_name}};

@@ -31,7 +31,7 @@
 namespace mf {
 
 const char* kDGOwnerMatrix = R"(
-extern "C" __global__ void __launch_bounds__($DBLOCK)
+extern "C" __global__ void __launch_bounds__($DBLOCK) $DATTR
 $KNAME(const int32_t ncells, const int32_t* __restrict__ cell_ptr, const int4* __restrict__ items,
        const int32_t* __restrict__ geo, const double* __restrict__ xg,
        const double* __restrict__ coeffs, const double* __restrict__ consts,
@@ -100,7 +100,7 @@ $CASES
 )";
 
 const char* kDGOwnerVector = R"(
-extern "C" __global__ void __launch_bounds__($DBLOCK)
+extern "C" __global__ void __launch_bounds__($DBLOCK) $DATTR
 $KNAME(const int32_t ncells, const int32_t* __restrict__ cell_ptr, const int4* __restrict__ items,
        const int32_t* __restrict__ geo, const double* __restrict__ xg,
        const double* __restrict__ coeffs, const double* __restrict__ consts,

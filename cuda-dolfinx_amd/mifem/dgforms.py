@@ -51,29 +51,36 @@ def _geom(s: str, xd: str) -> str:
     {{(J10{s} * J21{s} - J11{s} * J20{s}) * id{s}, (J01{s} * J20{s} - J00{s} * J21{s}) * id{s}, (J00{s} * J11{s} - J01{s} * J10{s}) * id{s}}}}};
   const double adet{s} = fabs(det{s});
   double h{s} = 0.0;
-  for (int va = 0; va < 4; ++va)
-    for (int vb = va + 1; vb < 4; ++vb) {{
-      const double dx = {xd}[3 * va] - {xd}[3 * vb], dy = {xd}[3 * va + 1] - {xd}[3 * vb + 1],
-                   dz = {xd}[3 * va + 2] - {xd}[3 * vb + 2];
-      const double dd = sqrt(dx * dx + dy * dy + dz * dz);
-      h{s} = dd > h{s} ? dd : h{s};
-    }}
-"""
+""" + "".join(f"""  {{
+    const double dx = {xd}[{3 * a}] - {xd}[{3 * b}], dy = {xd}[{3 * a + 1}] - {xd}[{3 * b + 1}], dz = {xd}[{3 * a + 2}] - {xd}[{3 * b + 2}];
+    const double dd = sqrt(dx * dx + dy * dy + dz * dz);
+    h{s} = dd > h{s} ? dd : h{s};
+  }}
+""" for a in range(4) for b in range(a + 1, 4))
 
 
 _FACET = """
-  static const int FV[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+  /* facet lf0 of the first cell has vertices (1,2,3), (0,2,3), (0,1,3) or
+     (0,1,2): picked by selects, so coordinate_dofs is only read at constant
+     offsets and stays in registers */
   const int lf0 = entity_local_index[0];
   const double* xa = coordinate_dofs;
-  const int a0 = FV[lf0][0], a1 = FV[lf0][1], a2 = FV[lf0][2];
-  const double e1x = xa[3 * a1] - xa[3 * a0], e1y = xa[3 * a1 + 1] - xa[3 * a0 + 1], e1z = xa[3 * a1 + 2] - xa[3 * a0 + 2];
-  const double e2x = xa[3 * a2] - xa[3 * a0], e2y = xa[3 * a2 + 1] - xa[3 * a0 + 1], e2z = xa[3 * a2 + 2] - xa[3 * a0 + 2];
+  double P0[3], P1[3], P2[3], PO[3];
+  for (int d = 0; d < 3; ++d) {
+    const double v0 = xa[d], v1 = xa[3 + d], v2 = xa[6 + d], v3 = xa[9 + d];
+    P0[d] = lf0 == 0 ? v1 : v0;
+    P1[d] = lf0 <= 1 ? v2 : v1;
+    P2[d] = lf0 <= 2 ? v3 : v2;
+    PO[d] = lf0 == 0 ? v0 : (lf0 == 1 ? v1 : (lf0 == 2 ? v2 : v3));
+  }
+  const double e1x = P1[0] - P0[0], e1y = P1[1] - P0[1], e1z = P1[2] - P0[2];
+  const double e2x = P2[0] - P0[0], e2y = P2[1] - P0[1], e2z = P2[2] - P0[2];
   double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
   const double ds = sqrt(nx * nx + ny * ny + nz * nz);
   nx /= ds; ny /= ds; nz /= ds;
   {
     /* outward from the first cell: away from the vertex opposite the facet */
-    const double ox = xa[3 * lf0] - xa[3 * a0], oy = xa[3 * lf0 + 1] - xa[3 * a0 + 1], oz = xa[3 * lf0 + 2] - xa[3 * a0 + 2];
+    const double ox = PO[0] - P0[0], oy = PO[1] - P0[1], oz = PO[2] - P0[2];
     if (nx * ox + ny * oy + nz * oz > 0.0) { nx = -nx; ny = -ny; nz = -nz; }
   }
 """

@@ -565,6 +565,45 @@ int mf_assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* 
   });
 }
 
+// ------------------------------------------------------------------ interpolation
+int mf_interpolator_create(mf_context* ctx, mf_dofmap* dm1, mf_dofmap* dm0, const double* im,
+                           mf_interpolator** out) {
+  return guard([&] {
+    MF_CHECK(ctx && dm1 && dm0 && im && out, MF_EINVAL, "null argument");
+    MF_CHECK(dm1->num_cells == dm0->num_cells, MF_ESHAPE, "dofmaps of different meshes");
+    MF_CHECK(dm1->bs == dm0->bs, MF_ESHAPE, "block sizes differ");
+    MF_CHECK((int64_t)dm1->num_cells * dm1->nd < INT32_MAX, MF_EUNSUPPORTED,
+             "cell count x dofs per cell exceeds int32 indexing");
+    use_device(ctx);
+    auto ip = std::make_unique<mf_interpolator>();
+    ip->ctx = ctx;
+    ip->dm1 = dm1;
+    ip->dm0 = dm0;
+    ip->im.upload(im, (int64_t)dm1->nd * dm0->nd);
+    // owner of each target node: the last (highest-numbered) cell holding it
+    std::vector<int32_t> own(dm1->num_nodes, -1);
+    for (int64_t c = 0; c < dm1->num_cells; ++c)
+      for (int i = 0; i < dm1->nd; ++i)
+        own[dm1->host[c * dm1->nd + i]] = static_cast<int32_t>(c * dm1->nd + i);
+    ip->own.upload(own);
+    *out = ip.release();
+  });
+}
+
+int mf_interpolate(mf_interpolator* ip, double* d_u1, const double* d_u0, mf_stream s) {
+  return guard([&] {
+    MF_CHECK(ip && d_u1 && d_u0, MF_EINVAL, "null argument");
+    MF_CHECK(d_u1 != d_u0, MF_EINVAL, "interpolation in place is not supported");
+    use_device(ip->ctx);
+    mf::k_interpolate(d_u1, d_u0, ip->own.ptr, ip->dm0->dev.ptr, ip->im.ptr, ip->dm1->num_nodes,
+                      ip->dm1->nd, ip->dm0->nd, ip->dm1->bs, S(s));
+  });
+}
+
+int mf_interpolator_destroy(mf_interpolator* ip) {
+  return guard([&] { delete ip; });
+}
+
 int mf_assemble_scalar(mf_form* M, double* value, mf_stream s) {
   return guard([&] {
     MF_CHECK(M && value, MF_EINVAL, "null argument");

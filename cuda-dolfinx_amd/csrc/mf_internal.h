@@ -106,6 +106,10 @@ void k_fill(double* x, int64_t n, double v, hipStream_t s);
 void k_fill_i32(int32_t* x, int64_t n, int32_t v, hipStream_t s);
 // out[0] (+)= sum of partials[0..n) in a fixed order (one workgroup)
 void k_sum_partials(const double* partials, int64_t n, double* out, int accumulate, hipStream_t s);
+// u1[node * bs + comp] = sum_k im[i * n0 + k] u0[bs * dm0[c * n0 + k] + comp] with
+// own[node] = c * n1 + i (or -1: node untouched)
+void k_interpolate(double* u1, const double* u0, const int32_t* own, const int32_t* dm0,
+                   const double* im, int64_t nnodes1, int n1, int n0, int bs, hipStream_t s);
 void k_mark(int8_t* markers, int64_t ndofs, const int32_t* dofs, int64_t nbc, hipStream_t s);
 void k_node_bits(uint8_t* bits, const int8_t* markers, int64_t nnodes, int bs, hipStream_t s);
 void k_pack(double* packed, int64_t ncells, int W, int off, const double* values,
@@ -233,6 +237,14 @@ struct mf_form {
     auto it = options.find(k);
     return it == options.end() ? dflt : it->second;
   }
+};
+
+struct mf_interpolator {
+  mf_context* ctx = nullptr;
+  mf_dofmap* dm1 = nullptr;           // target
+  mf_dofmap* dm0 = nullptr;           // source
+  mf::DevArray<double> im;            // n1 x n0
+  mf::DevArray<int32_t> own;          // per target node: c * n1 + i, or -1
 };
 
 struct mf_graph {

@@ -45,6 +45,26 @@ __global__ void __launch_bounds__(kBlock) sum_partials_kernel(const double* __re
   }
 }
 
+// Same-map interpolation, one thread per target dof (node, component): the
+// target node's value comes from its owning cell (c, local i),
+// u1 = sum_k im[i][k] u0(dm0(c, k)), k ascending.
+__global__ void interp_kernel(double* __restrict__ u1, const double* __restrict__ u0,
+                              const int32_t* __restrict__ own, const int32_t* __restrict__ dm0,
+                              const double* __restrict__ im, int64_t n, int n1, int n0, int bs) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t node = t / bs;
+  const int comp = static_cast<int>(t - node * bs);
+  const int32_t ci = own[node];
+  if (ci < 0) return;
+  const int32_t c = ci / n1, i = ci - c * n1;
+  const int32_t* d0 = dm0 + (int64_t)c * n0;
+  const double* row = im + (int64_t)i * n0;
+  double acc = 0.0;
+  for (int k = 0; k < n0; ++k) acc += row[k] * u0[(int64_t)bs * d0[k] + comp];
+  u1[t] = acc;
+}
+
 __global__ void mark_kernel(int8_t* __restrict__ m, int64_t ndofs, const int32_t* __restrict__ dofs,
                             int64_t nbc) {
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -195,6 +215,13 @@ void k_fill_i32(int32_t* x, int64_t n, int32_t v, hipStream_t s) {
 }
 void k_sum_partials(const double* partials, int64_t n, double* out, int accumulate, hipStream_t s) {
   sum_partials_kernel<<<1, kBlock, 0, s>>>(partials, n, out, accumulate);
+  MF_HIP(hipGetLastError());
+}
+void k_interpolate(double* u1, const double* u0, const int32_t* own, const int32_t* dm0,
+                   const double* im, int64_t nnodes1, int n1, int n0, int bs, hipStream_t s) {
+  const int64_t n = nnodes1 * bs;
+  if (n <= 0) return;
+  interp_kernel<<<grid_for(n), kBlock, 0, s>>>(u1, u0, own, dm0, im, n, n1, n0, bs);
   MF_HIP(hipGetLastError());
 }
 void k_mark(int8_t* m, int64_t ndofs, const int32_t* dofs, int64_t nbc, hipStream_t s) {

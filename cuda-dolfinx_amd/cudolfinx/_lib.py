@@ -58,6 +58,9 @@ SIGNATURES = {
     "mf_assemble_matrix": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp]),
     "mf_assemble_vector": (C.c_int, [_vp, _vp, _i64, C.c_int, _vp]),
     "mf_assemble_scalar": (C.c_int, [_vp, _vp, _vp]),
+    "mf_interpolator_create": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "mf_interpolate": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mf_interpolator_destroy": (C.c_int, [_vp]),
     "mf_assemble_scalar_device": (C.c_int, [_vp, _vp, C.c_int, _vp]),
     "mf_apply_lifting": (C.c_int, [C.POINTER(_vp), C.c_int, _vp, _i64, C.POINTER(_vp),
                                    C.POINTER(_vp), C.c_double, _vp]),

@@ -37,6 +37,7 @@ typedef struct mf_form mf_form;
 typedef struct mf_matrix mf_matrix;
 typedef struct mf_bc mf_bc;
 typedef struct mf_graph mf_graph;
+typedef struct mf_interpolator mf_interpolator;
 typedef void* mf_stream;
 
 enum mf_status {
@@ -184,6 +185,24 @@ int mf_assemble_scalar(mf_form* M, double* value, mf_stream stream);
  * value, on `stream`, with no host synchronization (usable under graph
  * capture).  d_value is device memory. */
 int mf_assemble_scalar_device(mf_form* M, double* d_value, int zero, mf_stream stream);
+/* ---------------------------------------------------------------- interpolation */
+/* Same-map interpolation between two spaces on one mesh: the device half of
+ * CUDACoefficient::interpolate (cpp/cudolfinx/fem/CUDACoefficient.h:50-66,
+ * interpolate_same_map in CUDAInterpolate.h:24-43 / CUDAInterpolate.cu:48-74).
+ * dm1: target space, dm0: source space (same cells, same block size);
+ * im: host memory, the dm1->nd x dm0->nd element interpolation operator
+ * (row-major; the scalar operator, applied to every block component) that
+ * DOLFINx's create_interpolation_operator returns.  The operator and each
+ * target node's owning cell (the highest-numbered cell holding it, i.e. the
+ * last writer of a sequential cell loop) are uploaded once here. */
+int mf_interpolator_create(mf_context* ctx, mf_dofmap* dm1, mf_dofmap* dm0, const double* im,
+                           mf_interpolator** out);
+/* d_u1[node * bs + j] = sum_k im[i][k] d_u0[dm0(c, k) * bs + j] for each
+ * target node with owner (c, i): one launch on `stream`, no allocation, no
+ * synchronization (graph-capturable).  d_u1 and d_u0 must not alias. */
+int mf_interpolate(mf_interpolator* ip, double* d_u1, const double* d_u0, mf_stream stream);
+int mf_interpolator_destroy(mf_interpolator* ip);
+
 /* apply_lifting_on_device (fem.cpp:327-355, CUDAAssembler.h:564-603):
  * b -= scale * A_j[:, bc_j] (g_j - x0_j) over every integral of every form.
  * A form whose constrained values are all zero and has no x0 adds exactly

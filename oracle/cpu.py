@@ -94,6 +94,8 @@ def _declare(L):
                                      C.POINTER(Entities)]
     L.or_assemble_scalar.restype = C.c_double
     L.or_set_bc.argtypes = [C.c_int64, _ip, _dp, _dp, C.c_double, _dp]
+    L.or_interpolate_same_map.argtypes = [C.c_int64, _ip, C.c_int, _ip, C.c_int, C.c_int, _dp,
+                                          _dp, _dp]
     L.or_pack_coefficients.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(_ip),
                                        C.POINTER(_dp), C.POINTER(C.c_int), C.c_int64, C.c_int, _dp]
 
@@ -316,6 +318,19 @@ class OracleAssembler:
                                _p(x0, C.c_double) if x0 is not None else None, scale,
                                _p(b, C.c_double))
         return b
+
+
+def interpolate_same_map(V1, V0, im, u0, u1=None):
+    """u1 on V1 from u0 on V0 (same mesh) with the (n1, n0) element operator im."""
+    dm1 = np.ascontiguousarray(V1.dofmap, dtype=np.int32)
+    dm0 = np.ascontiguousarray(V0.dofmap, dtype=np.int32)
+    im = np.ascontiguousarray(im, dtype=np.float64)
+    u0 = np.ascontiguousarray(u0, dtype=np.float64)
+    u1 = np.zeros(V1.size) if u1 is None else u1
+    lib().or_interpolate_same_map(dm1.shape[0], _p(dm1, C.c_int32), dm1.shape[1],
+                                  _p(dm0, C.c_int32), dm0.shape[1], V1.bs, _p(im, C.c_double),
+                                  _p(u0, C.c_double), _p(u1, C.c_double))
+    return u1
 
 
 def set_bc(b, bcs, V, x0=None, scale=1.0):

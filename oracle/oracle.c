@@ -18,6 +18,7 @@
  *     with its energy functional, a functional over cells, exterior and
  *     interior facets),
  *     evaluated directly at quadrature points, in the UFCx argument layout;
+ *   - same-map interpolation between two spaces on one mesh;
  *   - the sparsity build (DOLFINx create_sparsity_pattern + the bs-unrolling
  *     of la/petsc.cpp:14-65);
  *   - the assembly loops of cpp/cudolfinx/fem/CUDAFormIntegral.cpp
@@ -974,6 +975,21 @@ int or_apply_lifting(const or_problem* P, int nint, const or_kernel* kernels,
     }
   }
   return 0;
+}
+
+/* Same-map interpolation, cell by cell in order (CUDAInterpolate.cu:48-74:
+ * gather u0's cell values, multiply by the n1 x n0 operator im (k ascending),
+ * scatter; a node shared by several cells keeps the last cell's value). */
+void or_interpolate_same_map(int64_t ncells, const int32_t* dm1, int n1, const int32_t* dm0,
+                             int n0, int bs, const double* im, const double* u0, double* u1)
+{
+  for (int64_t c = 0; c < ncells; ++c)
+    for (int i = 0; i < n1; ++i)
+      for (int j = 0; j < bs; ++j) {
+        double acc = 0.0;
+        for (int k = 0; k < n0; ++k) acc += im[i * n0 + k] * u0[(int64_t)bs * dm0[c * n0 + k] + j];
+        u1[(int64_t)bs * dm1[c * n1 + i] + j] = acc;
+      }
 }
 
 /* b[d] = scale * (g[d] - x0[d]) for the constrained dofs (CUDAAssembler.cpp:99-126). */

@@ -383,6 +383,34 @@ def test_graph_newton_step_with_energy():
         assert abs(eg - ref) <= 1e-12 * abs(ref)
 
 
+@pytest.mark.parametrize("cell,fam0,k0,fam1,k1,bs", [
+    ("tetrahedron", "P", 1, "P", 2, 1), ("tetrahedron", "P", 2, "P", 3, 3),
+    ("tetrahedron", "DG", 1, "P", 1, 1), ("tetrahedron", "P", 3, "DG", 1, 1),
+    ("hexahedron", "P", 1, "P", 3, 1)])
+def test_same_map_interpolation(cell, fam0, k0, fam1, k1, bs):
+    _say(f"interpolation {fam0}{k0} -> {fam1}{k1} bs={bs} on {cell} n=6 (random source)")
+    import cudolfinx as cufem
+    from cudolfinx.coefficient import interpolation_operator
+    mesh = create_unit_cube(6, cell, jitter=0.15, seed=6)
+    shape = (bs,) if bs > 1 else None
+    V0 = functionspace(mesh, (fam0, k0), shape=shape)
+    V1 = functionspace(mesh, (fam1, k1), shape=shape)
+    f0, f1 = Function(V0), Function(V1)
+    rng = np.random.default_rng(17)
+    f0.x[:] = rng.uniform(-1, 1, V0.size)
+    c0, c1 = cufem.Coefficient(f0), cufem.Coefficient(f1)
+    im = interpolation_operator(V1.element, V0.element)
+    for rep in range(2):
+        c1.interpolate(c0)
+        got = c1.values()
+        ref = cpu.interpolate_same_map(V1, V0, im, f0.x)
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        _say(f"  call {rep}: max rel diff {err:.2e}")
+        assert err <= 1e-14, err
+        f0.x[:] = rng.uniform(-1, 1, V0.size)
+        c0.copy_host_values_to_device()
+
+
 def test_many_instances_block_separate():
     _say("batch of 6 Poisson instances (P1-P3, n=2..4) merged per degree, owner-computes")
     from mifem import batch

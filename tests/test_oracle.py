@@ -528,3 +528,67 @@ def test_sipg_energy_gradient_is_residual(k):
     u.x[:] = u0
     dE = (ep - em) / (2 * h)
     assert abs(dE - F @ d) <= 1e-7 * max(1.0, abs(dE)), (dE, F @ d)
+
+
+# ------------------------------------------------------------ same-map interpolation
+def _poly(deg):
+    c = np.array([0.7, -0.3, 0.5, 1.1])
+
+    def f(x):
+        v = c[0] + c[1] * x[0] + c[2] * x[1] + c[3] * x[2]
+        if deg >= 2:
+            v = v + 0.4 * x[0] * x[1] - 0.9 * x[2] ** 2
+        if deg >= 3:
+            v = v + 0.6 * x[0] * x[1] * x[2] - 0.2 * x[1] ** 3
+        return v
+    return f
+
+
+@pytest.mark.parametrize("cell,fam0,k0,fam1,k1,bs", [
+    ("tetrahedron", "P", 1, "P", 2, 1), ("tetrahedron", "P", 2, "P", 3, 1),
+    ("tetrahedron", "P", 1, "P", 3, 1), ("tetrahedron", "P", 2, "DG", 2, 1),
+    ("tetrahedron", "DG", 1, "DG", 3, 1), ("tetrahedron", "P", 2, "P", 3, 3),
+    ("tetrahedron", "P", 2, "P", 1, 1), ("hexahedron", "P", 1, "P", 3, 1),
+    ("hexahedron", "P", 2, "P", 3, 1)])
+def test_interpolation_exact_for_polynomials(cell, fam0, k0, fam1, k1, bs):
+    """Interpolating the degree-k0 interpolant of a polynomial of degree <= k0
+    into a space that contains it (or onto a subset of its nodes) reproduces the
+    target's own nodal interpolant of that polynomial."""
+    from cudolfinx.coefficient import interpolation_operator
+    mesh = create_unit_cube(2, cell, jitter=0.15, seed=6)
+    shape = (bs,) if bs > 1 else None
+    V0 = functionspace(mesh, (fam0, k0), shape=shape)
+    V1 = functionspace(mesh, (fam1, k1), shape=shape)
+    deg = min(k0, k1) if cell == "tetrahedron" else 1
+    p = _poly(deg)
+    f0, f1 = Function(V0), Function(V1)
+    g = (lambda x: np.stack([p(x), 2 * p(x), -p(x)])) if bs > 1 else p
+    f0.interpolate(g)
+    f1.interpolate(g)
+    u1 = cpu.interpolate_same_map(V1, V0, interpolation_operator(V1.element, V0.element), f0.x)
+    np.testing.assert_allclose(u1, f1.x, atol=1e-13 * np.abs(f1.x).max())
+
+
+def test_interpolation_to_dg0_is_vertex_mean_of_p1():
+    from cudolfinx.coefficient import interpolation_operator
+    mesh = create_unit_cube(3, jitter=0.2, seed=2)
+    V0, V1 = functionspace(mesh, ("P", 1)), functionspace(mesh, ("DG", 0))
+    u0 = np.random.default_rng(3).uniform(-1, 1, V0.size)
+    u1 = cpu.interpolate_same_map(V1, V0, interpolation_operator(V1.element, V0.element), u0)
+    ref = u0[V0.dofmap].mean(axis=1)
+    np.testing.assert_allclose(u1[V1.dofmap[:, 0]], ref, atol=1e-15)
+
+
+def test_interpolation_shared_nodes_take_last_cell():
+    """From a discontinuous field into P1: each vertex takes the value of the
+    highest-numbered cell containing it (brute force over the cell list)."""
+    from cudolfinx.coefficient import interpolation_operator
+    mesh = create_unit_cube(2, jitter=0.1, seed=1)
+    V0, V1 = functionspace(mesh, ("DG", 1)), functionspace(mesh, ("P", 1))
+    u0 = np.random.default_rng(5).uniform(-1, 1, V0.size)
+    u1 = cpu.interpolate_same_map(V1, V0, interpolation_operator(V1.element, V0.element), u0)
+    ref = np.full(V1.size, np.nan)
+    for c in range(mesh.num_cells):
+        for i in range(4):
+            ref[V1.dofmap[c, i]] = u0[V0.dofmap[c, i]]
+    np.testing.assert_allclose(u1, ref, atol=1e-15)

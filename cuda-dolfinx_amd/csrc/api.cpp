@@ -302,6 +302,36 @@ int mf_form_set_option(mf_form* f, const char* key, int64_t value) {
   });
 }
 
+int mf_form_set_restriction(mf_form* f, int axis, const int32_t* node_map, int64_t n,
+                            int64_t offset) {
+  return guard([&] {
+    MF_CHECK(f, MF_EINVAL, "null form");
+    MF_CHECK(f->rank >= 1 && axis >= 0 && axis < f->rank, MF_EINVAL, "axis out of range");
+    const mf_dofmap* dm = f->dm[axis];
+    MF_CHECK(!node_map || n == dm->num_nodes, MF_ESHAPE, "node map length != number of nodes");
+    MF_CHECK(offset >= 0, MF_EINVAL, "negative offset");
+    int64_t kept = dm->num_nodes;
+    f->h_rmap[axis].clear();
+    if (node_map) {
+      f->h_rmap[axis].assign(node_map, node_map + n);
+      int32_t mx = -1;
+      for (int32_t v : f->h_rmap[axis]) {
+        MF_CHECK(v >= -1 && v < n, MF_EINVAL, "node map value out of range");
+        mx = std::max(mx, v);
+      }
+      kept = mx + 1;
+    }
+    MF_CHECK(offset + kept * dm->bs < INT32_MAX, MF_EUNSUPPORTED, "block offset exceeds int32");
+    use_device(f->ctx);
+    if (node_map) f->rmap[axis].upload(f->h_rmap[axis]);
+    else f->rmap[axis].reset();
+    f->roff[axis] = offset;
+    f->rcount[axis] = kept;
+    f->restricted = true;
+    f->block_diag_rows.clear();
+  });
+}
+
 int mf_form_compile(mf_form* f, int verbose) {
   return guard([&] {
     MF_CHECK(f, MF_EINVAL, "null form");
@@ -460,6 +490,45 @@ int mf_matrix_create(mf_form* a, mf_matrix** out) {
   });
 }
 
+int mf_matrix_create_block(mf_form* const* forms, int nr, int nc, mf_matrix** out) {
+  return guard([&] {
+    MF_CHECK(forms && out && nr > 0 && nc > 0, MF_EINVAL, "bad arguments");
+    std::vector<mf_form*> fs(forms, forms + (int64_t)nr * nc);
+    mf_form* first = nullptr;
+    int64_t nrows = 0, ncols = 0;
+    for (auto* f : fs) {
+      if (!f) continue;
+      MF_CHECK(f->rank == 2, MF_EINVAL, "block matrices need bilinear forms");
+      if (!first) first = f;
+      MF_CHECK(f->ctx == first->ctx, MF_EINVAL, "forms of different contexts");
+      const int64_t k0 = f->h_rmap[0].empty() ? f->dm[0]->num_nodes : f->rcount[0];
+      const int64_t k1 = f->h_rmap[1].empty() ? f->dm[1]->num_nodes : f->rcount[1];
+      nrows = std::max(nrows, f->roff[0] + k0 * f->dm[0]->bs);
+      ncols = std::max(ncols, f->roff[1] + k1 * f->dm[1]->bs);
+    }
+    MF_CHECK(first, MF_EINVAL, "every block is empty");
+    use_device(first->ctx);
+    auto A = std::make_unique<mf_matrix>();
+    A->form = first;
+    A->block = true;
+    A->nrows = A->nrows_b = nrows;
+    A->ncols = ncols;
+    std::vector<int32_t> rp, cols;
+    mf::build_block_pattern(fs, nr, nc, nrows, ncols, rp, cols);
+    A->nnz = A->nnz_b = rp.back();
+    A->row_ptr.upload(rp);
+    A->cols.upload(cols.empty() ? std::vector<int32_t>{0} : cols);
+    A->rpb.upload(rp);
+    A->colsb.upload(cols.empty() ? std::vector<int32_t>{0} : cols);
+    A->h_rpb = rp;
+    A->own_values.alloc(std::max<int64_t>(1, A->nnz));
+    A->values = A->own_values.ptr;
+    mf::k_fill(A->values, A->nnz, 0.0, nullptr);
+    MF_HIP(hipDeviceSynchronize());
+    *out = A.release();
+  });
+}
+
 int mf_matrix_info(mf_matrix* A, int64_t* nrows, int64_t* ncols, int64_t* nnz) {
   return guard([&] {
     MF_CHECK(A, MF_EINVAL, "null matrix");
@@ -521,6 +590,7 @@ int mf_bc_create(mf_context* ctx, int64_t ndofs, const int32_t* dofs, int64_t nb
     b->uid = g_uid++;
     if (nbc) b->dofs.upload(dofs, nbc);
     b->g.upload(g, ndofs);
+    b->h_dofs.assign(dofs, dofs + nbc);
     b->g_zero = true;
     for (int64_t i = 0; i < nbc; ++i) b->g_zero &= g[dofs[i]] == 0.0;
     b->markers.alloc(ndofs);
@@ -639,6 +709,7 @@ int mf_apply_lifting(mf_form* const* a, int nforms, double* b, int64_t n,
       mf_form* f = a[k];
       const mf_bc* bc = bcs1[k];
       MF_CHECK(f && f->rank == 2 && f->compiled, MF_EINVAL, "lifting needs compiled bilinear forms");
+      MF_CHECK(!f->restricted, MF_EUNSUPPORTED, "lifting of block/restricted forms is not supported");
       MF_CHECK(n == f->dm[0]->num_nodes * f->dm[0]->bs, MF_ESHAPE, "vector length mismatch");
       if (!bc || bc->nbc == 0) continue;
       MF_CHECK(bc->ndofs == f->dm[1]->num_nodes * f->dm[1]->bs, MF_ESHAPE,

@@ -86,7 +86,7 @@ $KNAME(const int32_t n, const int32_t* __restrict__ ents, const int32_t* __restr
        const int32_t* __restrict__ dm1, const int8_t* __restrict__ bc0,
        const int8_t* __restrict__ bc1, const int32_t* __restrict__ table,
        const int32_t* __restrict__ row_ptr, double* __restrict__ values,
-       const uint8_t* __restrict__ fperm)
+       const uint8_t* __restrict__ fperm, const int32_t* __restrict__ rmap0, const int64_t off0)
 {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
@@ -97,12 +97,17 @@ $GATHER
   const int32_t* tb = table + (int64_t)(e >> 6) * ($NI * $NJ * 64) + (e & 63);
   for (int i = 0; i < $NI; ++i) {
     const int32_t rnode = dm0[(int64_t)cl[i / $ND0] * $ND0 + (i % $ND0)];
+    // block member forms: global row node (dropped rows: -1)
+    const int32_t gnode = rmap0 ? rmap0[rnode] : rnode;
+    if (gnode < 0) continue;
     for (int a = 0; a < $BS0; ++a) {
       const int32_t row = $BS0 * rnode + a;
       if (bc0 && bc0[row]) continue;
-      const int64_t rowlen = row_ptr[row + 1] - row_ptr[row];
+      const int64_t grow = off0 + (int64_t)$BS0 * gnode + a;
+      const int64_t rowlen = row_ptr[grow + 1] - row_ptr[grow];
       for (int j = 0; j < $NJ; ++j) {
         const int32_t base = tb[(i * $NJ + j) * 64];
+        if (base < 0) continue;
         const int32_t cnode = dm1[(int64_t)cl[j / $ND1] * $ND1 + (j % $ND1)];
         for (int b = 0; b < $BS1; ++b) {
           if (bc1 && bc1[$BS1 * cnode + b]) continue;
@@ -119,7 +124,8 @@ extern "C" __global__ void __launch_bounds__($BLOCK)
 $KNAME(const int32_t n, const int32_t* __restrict__ ents, const int32_t* __restrict__ geo,
        const double* __restrict__ xg, const double* __restrict__ coeffs,
        const double* __restrict__ consts, const int32_t* __restrict__ dm0,
-       double* __restrict__ b, const uint8_t* __restrict__ fperm)
+       double* __restrict__ b, const uint8_t* __restrict__ fperm,
+       const int32_t* __restrict__ rmap0, const int64_t off0)
 {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
@@ -128,8 +134,10 @@ $GATHER
   for (int i = 0; i < $NR; ++i) be[i] = 0.0;
   $TAB(be, w, consts, xc, eli, qperm);
   for (int i = 0; i < $NI; ++i) {
-    const int64_t node = dm0[(int64_t)cl[i / $ND0] * $ND0 + (i % $ND0)];
-    for (int a = 0; a < $BS0; ++a) atomicAdd(&b[$BS0 * node + a], be[i * $BS0 + a]);
+    int64_t node = dm0[(int64_t)cl[i / $ND0] * $ND0 + (i % $ND0)];
+    if (rmap0) node = rmap0[node];
+    if (node < 0) continue;
+    for (int a = 0; a < $BS0; ++a) atomicAdd(&b[off0 + $BS0 * node + a], be[i * $BS0 + a]);
   }
 }
 )";

@@ -360,10 +360,95 @@ std::shared_ptr<DGPlan> build_dg_plan(const mf_form& f, const mf_matrix* A) {
 
 }  // namespace
 
+namespace {
+
+DevArray<int32_t>& get_block_table(mf_form* a, Integral& I, mf_matrix* A, hipStream_t s) {
+  auto key = std::make_pair(a->uid, I.index);
+  auto it = A->tables.find(key);
+  if (it != A->tables.end()) return it->second;
+  const int ncell = entity_ncell(I.type);
+  const int ni = ncell * a->dm[0]->nd, nj = ncell * a->dm[1]->nd;
+  const int64_t padded = (I.n + kTableLanes - 1) / kTableLanes * kTableLanes;
+  DevArray<int32_t> t;
+  t.alloc(std::max<int64_t>(1, padded * ni * nj));
+  DevArray<int32_t> missing;
+  missing.alloc(1);
+  k_fill_i32(missing.ptr, 1, 0, s);
+  k_build_block_table(t.ptr, I.d_ents.ptr, I.n, entity_stride(I.type), ncell, a->dm[0]->dev.ptr,
+                      a->dm[0]->nd, a->rmap[0].ptr, a->roff[0], a->dm[0]->bs, a->dm[1]->dev.ptr,
+                      a->dm[1]->nd, a->rmap[1].ptr, a->roff[1], a->dm[1]->bs, A->row_ptr.ptr,
+                      A->cols.ptr, missing.ptr, s);
+  int32_t h_missing = 0;
+  MF_HIP(hipMemcpyAsync(&h_missing, missing.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  MF_HIP(hipStreamSynchronize(s));
+  MF_CHECK(h_missing == 0, MF_ENOTFOUND,
+           std::to_string(h_missing) + " element entries are missing from the block pattern");
+  return A->tables.emplace(key, std::move(t)).first->second;
+}
+
+// Block member form into a monolithic block matrix (CUDAAssembler's
+// assemble_matrix on a restricted CUDAForm; python/cudolfinx/assemble.py:112-134
+// calls it per block with zero=False): per-entity kernels with the row node
+// map and offset, offset tables against the scalar CSR; for a diagonal block
+// the Dirichlet rows get `diagonal` at their global positions.
+void assemble_block_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc1,
+                           double diagonal, int zero, hipStream_t s) {
+  MF_CHECK(A->block, MF_EINVAL, "a block/restricted form assembles into a block matrix");
+  for (int ax = 0; ax < 2; ++ax) {
+    const int64_t kept = a->h_rmap[ax].empty() ? a->dm[ax]->num_nodes : a->rcount[ax];
+    MF_CHECK(a->roff[ax] + kept * a->dm[ax]->bs <= (ax == 0 ? A->nrows : A->ncols), MF_ESHAPE,
+             "block form does not fit the matrix");
+  }
+  if (bc0) MF_CHECK(bc0->ndofs == a->dm[0]->num_nodes * a->dm[0]->bs, MF_ESHAPE,
+                    "bc0 size does not match the block's test space");
+  if (bc1) MF_CHECK(bc1->ndofs == a->dm[1]->num_nodes * a->dm[1]->bs, MF_ESHAPE,
+                    "bc1 size does not match the block's trial space");
+  const int8_t* m0 = bc0 && bc0->nbc ? bc0->markers.ptr : nullptr;
+  const int8_t* m1 = bc1 && bc1->nbc ? bc1->markers.ptr : nullptr;
+  if (zero) k_fill(A->values, A->nnz, 0.0, s);
+  const uint32_t block = static_cast<uint32_t>(a->opt("block", 128));
+  for (auto& I : a->ints) {
+    if (I->n == 0) continue;
+    auto& table = get_block_table(a, *I, A, s);
+    launch(I->k_mat, static_cast<uint32_t>((I->n + block - 1) / block), block, 0, s,
+           static_cast<int32_t>(I->n), I->d_ents.ptr, a->mesh->cells.ptr, a->mesh->x.ptr,
+           a->packed.ptr, a->consts.ptr, a->dm[0]->dev.ptr, a->dm[1]->dev.ptr, m0, m1,
+           table.ptr, A->row_ptr.ptr, A->values, a->mesh->fperm.ptr,
+           static_cast<const int32_t*>(a->rmap[0].ptr), static_cast<int64_t>(a->roff[0]));
+  }
+  const bool diag_block = a->dm[0] == a->dm[1] && a->h_rmap[0] == a->h_rmap[1] &&
+                          a->roff[0] == a->roff[1];
+  if (diag_block && m0 && bc0->n_owned) {
+    auto it = a->block_diag_rows.find(bc0->uid);
+    if (it == a->block_diag_rows.end()) {
+      const int bs = a->dm[0]->bs;
+      std::vector<int32_t> rows;
+      for (int64_t k = 0; k < bc0->n_owned; ++k) {
+        const int32_t d = bc0->h_dofs[k], node = d / bs;
+        const int32_t m = a->h_rmap[0].empty() ? node : a->h_rmap[0][node];
+        if (m >= 0) rows.push_back(static_cast<int32_t>(a->roff[0] + (int64_t)bs * m + d % bs));
+      }
+      DevArray<int32_t> dr;
+      dr.upload(rows.empty() ? std::vector<int32_t>{0} : rows);
+      dr.n = static_cast<int64_t>(rows.size());
+      it = a->block_diag_rows.emplace(bc0->uid, std::move(dr)).first;
+    }
+    if (it->second.n)
+      k_set_diagonal(A->values, A->row_ptr.ptr, A->cols.ptr, it->second.ptr, it->second.n,
+                     A->nrows, diagonal, s);
+  }
+}
+
+}  // namespace
+
 void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc1,
                      double diagonal, int zero, hipStream_t s) {
   MF_CHECK(a->rank == 2, MF_EINVAL, "assemble_matrix needs a bilinear form");
   MF_CHECK(a->compiled, MF_ESTATE, "form is not compiled");
+  if (A->block || a->restricted) {
+    assemble_block_matrix(a, A, bc0, bc1, diagonal, zero, s);
+    return;
+  }
   MF_CHECK(A->nrows == a->dm[0]->num_nodes * a->dm[0]->bs &&
                A->ncols == a->dm[1]->num_nodes * a->dm[1]->bs,
            MF_ESHAPE, "matrix shape does not match the form's spaces");
@@ -480,7 +565,8 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
     launch(I->k_mat, static_cast<uint32_t>((I->n + block - 1) / block), block, 0, s,
            static_cast<int32_t>(I->n), I->d_ents.ptr, a->mesh->cells.ptr, a->mesh->x.ptr,
            a->packed.ptr, a->consts.ptr, a->dm[0]->dev.ptr, a->dm[1]->dev.ptr, m0, m1,
-           table.ptr, A->row_ptr.ptr, A->values, a->mesh->fperm.ptr);
+           table.ptr, A->row_ptr.ptr, A->values, a->mesh->fperm.ptr,
+           static_cast<const int32_t*>(nullptr), static_cast<int64_t>(0));
   }
   if (square && m0 && !diag_done)
     k_set_diagonal(A->values, A->row_ptr.ptr, A->cols.ptr, bc0->dofs.ptr, bc0->n_owned, A->nrows,
@@ -494,6 +580,21 @@ namespace mf {
 void assemble_vector(mf_form* L, double* b, int64_t n, int zero, hipStream_t s) {
   MF_CHECK(L->rank == 1, MF_EINVAL, "assemble_vector needs a linear form");
   MF_CHECK(L->compiled, MF_ESTATE, "form is not compiled");
+  if (L->restricted) {
+    // block vector member: per-entity kernels with the node map and offset
+    const int64_t kept = L->h_rmap[0].empty() ? L->dm[0]->num_nodes : L->rcount[0];
+    MF_CHECK(L->roff[0] + kept * L->dm[0]->bs <= n, MF_ESHAPE, "block form does not fit the vector");
+    if (zero) k_fill(b, n, 0.0, s);
+    const uint32_t block = static_cast<uint32_t>(L->opt("block", 128));
+    for (auto& I : L->ints) {
+      if (I->n == 0) continue;
+      launch(I->k_vec, static_cast<uint32_t>((I->n + block - 1) / block), block, 0, s,
+             static_cast<int32_t>(I->n), I->d_ents.ptr, L->mesh->cells.ptr, L->mesh->x.ptr,
+             L->packed.ptr, L->consts.ptr, L->dm[0]->dev.ptr, b, L->mesh->fperm.ptr,
+             static_cast<const int32_t*>(L->rmap[0].ptr), static_cast<int64_t>(L->roff[0]));
+    }
+    return;
+  }
   MF_CHECK(n == L->dm[0]->num_nodes * L->dm[0]->bs, MF_ESHAPE, "vector length mismatch");
   const int64_t mode = L->opt("vector_kernel", 0);
   if (L->k_dgown && (mode == 0 || mode == 3)) {
@@ -536,7 +637,8 @@ void assemble_vector(mf_form* L, double* b, int64_t n, int zero, hipStream_t s) 
     if (I.get() == owner || I->n == 0) continue;
     launch(I->k_vec, static_cast<uint32_t>((I->n + block - 1) / block), block, 0, s,
            static_cast<int32_t>(I->n), I->d_ents.ptr, L->mesh->cells.ptr, L->mesh->x.ptr,
-           L->packed.ptr, L->consts.ptr, L->dm[0]->dev.ptr, b, L->mesh->fperm.ptr);
+           L->packed.ptr, L->consts.ptr, L->dm[0]->dev.ptr, b, L->mesh->fperm.ptr,
+           static_cast<const int32_t*>(nullptr), static_cast<int64_t>(0));
   }
 }
 

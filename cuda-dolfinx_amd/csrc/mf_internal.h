@@ -110,6 +110,14 @@ void k_sum_partials(const double* partials, int64_t n, double* out, int accumula
 // own[node] = c * n1 + i (or -1: node untouched)
 void k_interpolate(double* u1, const double* u0, const int32_t* own, const int32_t* dm0,
                    const double* im, int64_t nnodes1, int n1, int n0, int bs, hipStream_t s);
+// Offset table against a scalar CSR for a block member form: rows
+// off0 + bs0 * rmap0[node] (+ a), columns off1 + bs1 * rmap1[node] (+ b); a
+// dropped node gives -1 (entry skipped).  Same layout as k_build_table.
+void k_build_block_table(int32_t* table, const int32_t* ents, int64_t nents, int stride, int ncell,
+                         const int32_t* dm0, int nd0, const int32_t* rmap0, int64_t off0, int bs0,
+                         const int32_t* dm1, int nd1, const int32_t* rmap1, int64_t off1, int bs1,
+                         const int32_t* row_ptr, const int32_t* cols, int32_t* missing,
+                         hipStream_t s);
 void k_mark(int8_t* markers, int64_t ndofs, const int32_t* dofs, int64_t nbc, hipStream_t s);
 void k_node_bits(uint8_t* bits, const int8_t* markers, int64_t nnodes, int bs, hipStream_t s);
 void k_pack(double* packed, int64_t ncells, int W, int off, const double* values,
@@ -164,6 +172,7 @@ struct mf_dofmap {
 struct mf_bc {
   mf_context* ctx = nullptr;
   int64_t ndofs = 0, nbc = 0, n_owned = 0;
+  std::vector<int32_t> h_dofs;
   mf::DevArray<int32_t> dofs;
   mf::DevArray<int8_t> markers;
   mf::DevArray<double> g;
@@ -230,6 +239,14 @@ struct mf_form {
   hipFunction_t k_dgown = nullptr;    // cell-owner kernel over all integrals (DG spaces)
   std::shared_ptr<mf::DGPlan> dgvplan;                    // its vector schedule
   bool dgvplan_tried = false;
+  // block / restricted assembly, per argument: node -> kept node (-1: dropped;
+  // empty: identity) and the scalar offset of this block in the global numbering
+  std::vector<int32_t> h_rmap[2];
+  mf::DevArray<int32_t> rmap[2];
+  int64_t roff[2] = {0, 0};
+  int64_t rcount[2] = {0, 0};         // nodes kept
+  bool restricted = false;
+  std::map<uint64_t, mf::DevArray<int32_t>> block_diag_rows;   // bc uid -> global owned bc rows
   mf::DevArray<double> partials;      // rank 0: per-workgroup partial sums of every integral
   mf::DevArray<double> scalar_out;    // rank 0: device result of the host-returning entry
   uint64_t uid = 0;
@@ -258,6 +275,7 @@ struct mf_graph {
 
 struct mf_matrix {
   mf_form* form = nullptr;
+  bool block = false;                 // monolithic matrix of a block form (scalar CSR)
   int64_t nrows = 0, ncols = 0, nnz = 0, nrows_b = 0, nnz_b = 0;
   int bs0 = 1, bs1 = 1;
   mf::DevArray<int32_t> row_ptr, cols, rpb, colsb;
@@ -290,6 +308,9 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
                      double diagonal, int zero, hipStream_t s);
 void assemble_vector(mf_form* L, double* b, int64_t n, int zero, hipStream_t s);
 void assemble_scalar(mf_form* M, double* d_value, int zero, hipStream_t s);
+// scalar-level CSR of a block of forms (row-major nr x nc, null: empty block)
+void build_block_pattern(const std::vector<mf_form*>& forms, int nr, int nc, int64_t nrows,
+                         int64_t ncols, std::vector<int32_t>& row_ptr, std::vector<int32_t>& cols);
 std::shared_ptr<OwnerPlan> vector_plan(mf_form* L, Integral& I);
 int entity_stride(int type);
 int entity_ncell(int type);

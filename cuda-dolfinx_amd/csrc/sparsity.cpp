@@ -90,4 +90,55 @@ void build_pattern(const mf_form& a, std::vector<int32_t>& rpb, std::vector<int3
   for (unsigned t = 0; t < nt; ++t) colsb.insert(colsb.end(), part_cols[t].begin(), part_cols[t].end());
 }
 
+// Monolithic scalar CSR of a block of bilinear forms (fem/petsc.h:39-107
+// create_cuda_matrix_block with the restricted patterns of
+// fem/utils.h:13-90): block (i, j) adds, for every node pair (r, c) of its own
+// pattern whose nodes are kept by the restrictions, the bs0 x bs1 scalar
+// entries at rows roff0 + bs0 r' + a and columns roff1 + bs1 c' + b (r', c'
+// the kept node numbers).  Rows sorted and unique.
+void build_block_pattern(const std::vector<mf_form*>& forms, int nr, int nc, int64_t nrows,
+                         int64_t ncols, std::vector<int32_t>& row_ptr, std::vector<int32_t>& cols) {
+  std::vector<std::vector<int32_t>> rows(nrows);
+  for (int bi = 0; bi < nr; ++bi)
+    for (int bj = 0; bj < nc; ++bj) {
+      const mf_form* f = forms[bi * nc + bj];
+      if (!f) continue;
+      std::vector<int32_t> rpb, colsb;
+      build_pattern(*f, rpb, colsb);
+      const int bs0 = f->dm[0]->bs, bs1 = f->dm[1]->bs;
+      const auto& m0 = f->h_rmap[0];
+      const auto& m1 = f->h_rmap[1];
+      for (int64_t r = 0; r + 1 < (int64_t)rpb.size(); ++r) {
+        const int32_t rr = m0.empty() ? static_cast<int32_t>(r) : m0[r];
+        if (rr < 0) continue;
+        for (int a = 0; a < bs0; ++a) {
+          const int64_t g = f->roff[0] + (int64_t)bs0 * rr + a;
+          MF_CHECK(g < nrows, MF_ESHAPE, "block row outside the matrix");
+          auto& row = rows[g];
+          for (int64_t k = rpb[r]; k < rpb[r + 1]; ++k) {
+            const int32_t cc = m1.empty() ? colsb[k] : m1[colsb[k]];
+            if (cc < 0) continue;
+            for (int b = 0; b < bs1; ++b) {
+              const int64_t col = f->roff[1] + (int64_t)bs1 * cc + b;
+              MF_CHECK(col < ncols, MF_ESHAPE, "block column outside the matrix");
+              row.push_back(static_cast<int32_t>(col));
+            }
+          }
+        }
+      }
+    }
+  row_ptr.assign(nrows + 1, 0);
+  cols.clear();
+  for (int64_t r = 0; r < nrows; ++r) {
+    auto& row = rows[r];
+    std::sort(row.begin(), row.end());
+    row.erase(std::unique(row.begin(), row.end()), row.end());
+    MF_CHECK((int64_t)cols.size() + (int64_t)row.size() < INT32_MAX, MF_EUNSUPPORTED,
+             "block pattern exceeds int32 indexing");
+    cols.insert(cols.end(), row.begin(), row.end());
+    row_ptr[r + 1] = static_cast<int32_t>(cols.size());
+    std::vector<int32_t>().swap(row);
+  }
+}
+
 }  // namespace mf

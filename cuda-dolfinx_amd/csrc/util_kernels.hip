@@ -183,6 +183,40 @@ __global__ void table_kernel(int32_t* __restrict__ table, const int32_t* __restr
   }
 }
 
+__global__ void block_table_kernel(int32_t* __restrict__ table, const int32_t* __restrict__ ents,
+                                   int64_t nents, int stride, int ncell,
+                                   const int32_t* __restrict__ dm0, int nd0,
+                                   const int32_t* __restrict__ rmap0, int64_t off0, int bs0,
+                                   const int32_t* __restrict__ dm1, int nd1,
+                                   const int32_t* __restrict__ rmap1, int64_t off1, int bs1,
+                                   const int32_t* __restrict__ row_ptr,
+                                   const int32_t* __restrict__ cols, int32_t* __restrict__ missing) {
+  const int NI = ncell * nd0, NJ = ncell * nd1;
+  const int64_t total = nents * NI * NJ;
+  int64_t gs = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < total; l += gs) {
+    int64_t e = l / (NI * NJ);
+    int ij = static_cast<int>(l - e * NI * NJ);
+    int i = ij / NJ, j = ij - i * NJ;
+    int32_t ci = ents[e * stride + 2 * (i / nd0)];
+    int32_t cj = ents[e * stride + 2 * (j / nd1)];
+    int32_t rn = dm0[(int64_t)ci * nd0 + (i % nd0)];
+    int32_t cn = dm1[(int64_t)cj * nd1 + (j % nd1)];
+    if (rmap0) rn = rmap0[rn];
+    if (rmap1) cn = rmap1[cn];
+    int32_t v = -1;
+    if (rn >= 0 && cn >= 0) {
+      const int64_t g = off0 + (int64_t)bs0 * rn;
+      const int32_t key = static_cast<int32_t>(off1 + (int64_t)bs1 * cn);
+      const int64_t lo = row_ptr[g], hi = row_ptr[g + 1];
+      const int64_t p = search(cols, lo, hi, key);
+      if (p < hi && cols[p] == key) v = static_cast<int32_t>(p);
+      else atomicAdd(missing, 1);
+    }
+    table[(e >> 6) * (int64_t)NI * NJ * 64 + (int64_t)ij * 64 + (e & 63)] = v;
+  }
+}
+
 __global__ void lift_list_kernel(int32_t* __restrict__ list, int32_t* __restrict__ count,
                                  const int32_t* __restrict__ ents, int64_t nents, int stride,
                                  int ncell, const int32_t* __restrict__ dm1, int nd1, int bs1,
@@ -222,6 +256,18 @@ void k_interpolate(double* u1, const double* u0, const int32_t* own, const int32
   const int64_t n = nnodes1 * bs;
   if (n <= 0) return;
   interp_kernel<<<grid_for(n), kBlock, 0, s>>>(u1, u0, own, dm0, im, n, n1, n0, bs);
+  MF_HIP(hipGetLastError());
+}
+void k_build_block_table(int32_t* table, const int32_t* ents, int64_t nents, int stride, int ncell,
+                         const int32_t* dm0, int nd0, const int32_t* rmap0, int64_t off0, int bs0,
+                         const int32_t* dm1, int nd1, const int32_t* rmap1, int64_t off1, int bs1,
+                         const int32_t* row_ptr, const int32_t* cols, int32_t* missing,
+                         hipStream_t s) {
+  const int64_t total = nents * (int64_t)ncell * nd0 * ncell * nd1;
+  if (total <= 0) return;
+  block_table_kernel<<<grid_for(total, 4), kBlock, 0, s>>>(table, ents, nents, stride, ncell, dm0,
+                                                           nd0, rmap0, off0, bs0, dm1, nd1, rmap1,
+                                                           off1, bs1, row_ptr, cols, missing);
   MF_HIP(hipGetLastError());
 }
 void k_mark(int8_t* m, int64_t ndofs, const int32_t* dofs, int64_t nbc, hipStream_t s) {

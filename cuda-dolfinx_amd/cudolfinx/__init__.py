@@ -9,9 +9,9 @@ from .assemble import CUDAAssembler
 from .bcs import CUDADirichletBC
 from .coefficient import Coefficient
 from .context import CUDAContext, DeviceArray, get_cuda_context
-from .form import CUDAForm, form
+from .form import BlockCUDAForm, CUDAForm, form
 from .graph import CUDAGraph
 from .la import CUDAMatrix, CUDAVector
 
 __all__ = ["CUDAAssembler", "CUDADirichletBC", "CUDAContext", "DeviceArray", "get_cuda_context",
-           "CUDAForm", "form", "CUDAMatrix", "CUDAVector", "CUDAGraph", "Coefficient"]
+           "CUDAForm", "BlockCUDAForm", "form", "CUDAMatrix", "CUDAVector", "CUDAGraph", "Coefficient"]

@@ -58,6 +58,8 @@ SIGNATURES = {
     "mf_assemble_matrix": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp]),
     "mf_assemble_vector": (C.c_int, [_vp, _vp, _i64, C.c_int, _vp]),
     "mf_assemble_scalar": (C.c_int, [_vp, _vp, _vp]),
+    "mf_form_set_restriction": (C.c_int, [_vp, C.c_int, _vp, _i64, _i64]),
+    "mf_matrix_create_block": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
     "mf_interpolator_create": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "mf_interpolate": (C.c_int, [_vp, _vp, _vp, _vp]),
     "mf_interpolator_destroy": (C.c_int, [_vp]),

@@ -15,7 +15,7 @@ import typing
 from ._lib import check, lib
 from .bcs import CUDADirichletBC
 from .context import get_cuda_context
-from .form import CUDAForm
+from .form import BlockCUDAForm, CUDAForm
 from .la import CUDAMatrix, CUDAVector
 
 
@@ -45,6 +45,18 @@ class CUDAAssembler:
         if not isinstance(b, CUDAForm):
             raise TypeError(f"Expected CUDAForm, got type '{type(b)}'.")
         return CUDAVector(self._ctx, b.function_spaces[0].size)
+
+    def create_matrix_block(self, a: BlockCUDAForm) -> CUDAMatrix:
+        """Monolithic block matrix (assemble.py:236-246)."""
+        if not isinstance(a, BlockCUDAForm):
+            raise TypeError(f"Expected BlockCUDAForm, got type '{type(a)}'")
+        return CUDAMatrix.block(self._ctx, a)
+
+    def create_vector_block(self, b: BlockCUDAForm) -> CUDAVector:
+        """Block vector of size b.local_size (assemble.py:255-263)."""
+        if not isinstance(b, BlockCUDAForm):
+            raise TypeError(f"Expected BlockCUDAForm, got type '{type(b)}')")
+        return CUDAVector(self._ctx, b.local_size)
 
     def pack_bcs(self, bcs) -> CUDADirichletBC:
         return CUDADirichletBC(self._ctx, bcs)
@@ -90,6 +102,26 @@ class CUDAAssembler:
                                        int(bool(zero)), self._stream))
         return mat
 
+    def assemble_matrix_block(self, a: BlockCUDAForm, mat: typing.Optional[CUDAMatrix] = None,
+                              bcs=None, coeffs=None, constants=None, zero: bool = True):
+        """Every block into one matrix (assemble.py:112-134): one zeroing, then
+        assemble_matrix(form_ij, mat, bcs[i][j], zero=False) per block."""
+        if not isinstance(a, BlockCUDAForm):
+            raise TypeError(f"Expected BlockCUDAForm, got '{type(a)}'")
+        if mat is None:
+            mat = self.create_matrix_block(a)
+        if zero:
+            _, _, vals = mat.device_arrays()
+            check(lib().mf_zero(self._ctx.handle, C.c_void_p(vals), mat.nnz, self._stream))
+        for i, row in enumerate(a.forms):
+            for j, form in enumerate(row):
+                if form is None:
+                    continue
+                _bcs = bcs[i][j] if bcs is not None else []
+                _coeffs = coeffs[i][j] if coeffs is not None else None
+                self.assemble_matrix(form, mat, bcs=_bcs, coeffs=_coeffs, zero=False)
+        return mat
+
     # ------------------------------------------------------------ vectors
     def assemble_vector(self, b: CUDAForm, vec: typing.Optional[CUDAVector] = None, constants=None,
                         coeffs=None, zero: bool = True):
@@ -100,6 +132,24 @@ class CUDAAssembler:
         b.to_device(force=False)
         self.pack_coefficients(b, coeffs)
         check(lib().mf_assemble_vector(b.handle, vec.ptr, vec.size, int(bool(zero)), self._stream))
+        return vec
+
+    def assemble_vector_block(self, b: BlockCUDAForm, vec: typing.Optional[CUDAVector] = None,
+                              constants=None, coeffs=None, zero: bool = True):
+        """Every block form into one vector at its offset (assemble.py:168-202)."""
+        if not isinstance(b, BlockCUDAForm):
+            raise TypeError(f"Expected BlockCUDAForm, got '{type(b)}'")
+        if vec is None:
+            vec = self.create_vector_block(b)
+        num_forms = len(b.forms)
+        _coeffs = [None] * num_forms if coeffs is None else coeffs
+        if len(_coeffs) != num_forms:
+            raise ValueError(f"Expected coeffs to be None or a list of length: '{num_forms}', "
+                             f"got '{len(_coeffs)}' instead!")
+        if zero:
+            check(lib().mf_zero(self._ctx.handle, vec.ptr, vec.size, self._stream))
+        for form, fc in zip(b.forms, _coeffs):
+            self.assemble_vector(form, vec=vec, coeffs=fc, zero=False)
         return vec
 
     # ------------------------------------------------------------ functionals

@@ -147,6 +147,18 @@ class CUDAForm:
     def rank(self):
         return self._form.rank
 
+    def set_restriction(self, axis: int, node_map, offset: int) -> None:
+        """Number argument `axis` in a block space: node r -> scalar dofs
+        offset + bs * node_map[r] + j, node_map[r] == -1 drops the node
+        (node_map None: identity).  CUDAForm.set_restriction, CUDAForm.h:201-231."""
+        V = self.function_spaces[axis]
+        if node_map is None:
+            check(lib().mf_form_set_restriction(self._handle, axis, None, V.num_nodes, int(offset)))
+        else:
+            m = np.ascontiguousarray(node_map, dtype=np.int32)
+            check(lib().mf_form_set_restriction(self._handle, axis, m.ctypes.data_as(C.c_void_p),
+                                                m.size, int(offset)))
+
     def set_option(self, key: str, value: int) -> None:
         check(lib().mf_form_set_option(self._handle, key.encode(), int(value)))
 
@@ -199,9 +211,116 @@ class CUDAForm:
             self._handle = None
 
 
-def form(form, cuda_jit_args: typing.Optional[dict] = None, options: typing.Optional[dict] = None,
-         **kwargs):
-    """Create a CUDAForm (or nested lists of them) from mifem forms (form.py:227-249)."""
-    if isinstance(form, collections.abc.Iterable) and not hasattr(form, "integrals"):
-        return [globals()["form"](f, cuda_jit_args, options, **kwargs) for f in form]
-    return CUDAForm(form, jit_args=cuda_jit_args, options=options)
+class BlockCUDAForm:
+    """Several forms assembled as one block system (form.py:107-225).
+
+    forms: a list of linear forms (block vector) or a list of lists of
+    bilinear forms (block matrix; None marks an empty block).  restrictions:
+    for a vector, one array per block of the kept nodes of its space (the
+    position in the array is the node's new number); for a matrix, a pair
+    (row restrictions, column restrictions); None keeps every node.  Block i
+    starts at the sum of the kept sizes (nodes x block size) of blocks < i.
+    """
+
+    def __init__(self, forms, restrictions=None):
+        self._forms = forms
+        self._restrictions = restrictions
+        if not len(forms):
+            raise ValueError("Must provide at least one form!")
+        if isinstance(forms[0], CUDAForm):
+            self._init_vector()
+        else:
+            self._init_matrix()
+
+    @staticmethod
+    def _maps_offsets(spaces, restrictions):
+        maps, offsets = [], [0]
+        for i, V in enumerate(spaces):
+            r = None if restrictions is None else restrictions[i]
+            if r is None:
+                maps.append(None)
+                kept = V.num_nodes
+            else:
+                r = np.asarray(r, dtype=np.int64)
+                if r.size != np.unique(r).size or (r.size and (r.min() < 0 or r.max() >= V.num_nodes)):
+                    raise ValueError(f"restriction {i}: node indices must be unique and in range")
+                m = np.full(V.num_nodes, -1, dtype=np.int32)
+                m[r] = np.arange(r.size, dtype=np.int32)
+                maps.append(m)
+                kept = r.size
+            offsets.append(offsets[-1] + kept * V.bs)
+        return maps, offsets
+
+    def _init_vector(self):
+        spaces = [f.function_spaces[0] for f in self._forms]
+        maps, self._offsets = self._maps_offsets(spaces, self._restrictions)
+        for f, m, off in zip(self._forms, maps, self._offsets):
+            f.set_restriction(0, m, off)
+        self._col_offsets = None
+        self._global_size = self._offsets[-1]
+
+    def _init_matrix(self):
+        nr, nc = len(self._forms), len(self._forms[0])
+        if any(len(row) != nc for row in self._forms):
+            raise ValueError("block matrix rows must have equal length")
+
+        def space(i, axis):
+            cells = self._forms[i] if axis == 0 else [row[i] for row in self._forms]
+            for f in cells:
+                if f is not None:
+                    return f.function_spaces[axis]
+            raise ValueError(f"{'row' if axis == 0 else 'column'} {i} has no form")
+
+        rr = self._restrictions[0] if self._restrictions is not None else None
+        cr = self._restrictions[1] if self._restrictions is not None else None
+        rmaps, self._offsets = self._maps_offsets([space(i, 0) for i in range(nr)], rr)
+        cmaps, self._col_offsets = self._maps_offsets([space(j, 1) for j in range(nc)], cr)
+        for i, row in enumerate(self._forms):
+            for j, f in enumerate(row):
+                if f is None:
+                    continue
+                f.set_restriction(0, rmaps[i], self._offsets[i])
+                f.set_restriction(1, cmaps[j], self._col_offsets[j])
+        self._global_size = self._offsets[-1]
+
+    @property
+    def forms(self):
+        return self._forms
+
+    @property
+    def dolfinx_forms(self):
+        return [f.dolfinx_form if isinstance(f, CUDAForm) else [g.dolfinx_form if g else None for g in f]
+                for f in self._forms]
+
+    @property
+    def offsets(self):
+        return self._offsets
+
+    @property
+    def col_offsets(self):
+        return self._col_offsets
+
+    @property
+    def local_size(self):
+        return self._offsets[-1]
+
+    @property
+    def global_size(self):
+        return self._global_size
+
+
+def form(form, restriction=None, cuda_jit_args: typing.Optional[dict] = None,
+         options: typing.Optional[dict] = None, **kwargs):
+    """CUDAForm of a mifem form; a list (of lists) of forms gives a BlockCUDAForm
+    with the given restrictions (form.py:227-249)."""
+    def _create(f):
+        if f is None:
+            return None
+        if isinstance(f, collections.abc.Iterable) and not hasattr(f, "integrals"):
+            return [_create(g) for g in f]
+        return CUDAForm(f, jit_args=cuda_jit_args, options=options)
+
+    cuda_form = _create(form)
+    if isinstance(cuda_form, list):
+        return BlockCUDAForm(cuda_form, restriction)
+    return cuda_form

@@ -46,6 +46,25 @@ class CUDAMatrix:
         self.nrows, self.ncols, self.nnz = nr.value, nc.value, nnz.value
         self._mat = DeviceCSRMatrix(self)
 
+    @classmethod
+    def block(cls, ctx, block_form):
+        """Monolithic matrix of a BlockCUDAForm (create_cuda_matrix_block, fem/petsc.h:39-107)."""
+        self = cls.__new__(cls)
+        self._ctx = ctx
+        self._form = block_form
+        rows = block_form.forms
+        nr, nc = len(rows), len(rows[0])
+        arr = (C.c_void_p * (nr * nc))(*[f.handle.value if f is not None else None
+                                         for row in rows for f in row])
+        h = C.c_void_p()
+        check(lib().mf_matrix_create_block(arr, nr, nc, C.byref(h)))
+        self._handle = h
+        nrw, ncl, nnz = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().mf_matrix_info(h, C.byref(nrw), C.byref(ncl), C.byref(nnz)))
+        self.nrows, self.ncols, self.nnz = nrw.value, ncl.value, nnz.value
+        self._mat = DeviceCSRMatrix(self)
+        return self
+
     @property
     def handle(self):
         return self._handle

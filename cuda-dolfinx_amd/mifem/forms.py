@@ -317,6 +317,43 @@ def elasticity(V: FunctionSpace, mu, lam, qdegree: Optional[int] = None) -> Form
                 "elasticity")
 
 
+def mass(V: FunctionSpace, W: Optional[FunctionSpace] = None, kappa=None,
+         qdegree: Optional[int] = None) -> Form:
+    """a(u, v) = integral kappa u v dx, v in V (rows), u in W (columns, default V);
+    scalar spaces on affine tetrahedra; kappa None, a number, Constant or DG0 Function."""
+    W = V if W is None else W
+    _check_affine(V)
+    if V.mesh is not W.mesh or V.bs != 1 or W.bs != 1:
+        raise NotImplementedError("mass() couples scalar spaces on one mesh")
+    ev, ew = V.element, W.element
+    nv, nw = ev.ndofs, ew.ndofs
+    qd = ev.degree + ew.degree if qdegree is None else qdegree
+    q = E.tet_rule(qd)
+    coeffs, consts = [], []
+    kexpr = _coef_expr(kappa, 0, "kappa", [], coeffs, consts, [0])
+    if kexpr is None:
+        kexpr = "w[0]"
+    uid = _uid("mass", ev, ew, q.points.tobytes(), kexpr)
+    name = f"tabulate_tensor_integral_{uid}"
+    body = (_carr("FE_w", q.weights) + _carr("FE_phi", ev.tabulate(q.points, 0)[0])
+            + _carr("FE_psi", ew.tabulate(q.points, 0)[0]) + _AFFINE_GEOM + f"""
+  (void)K00; (void)K01; (void)K02; (void)K10; (void)K11; (void)K12; (void)K20; (void)K21; (void)K22;
+  const double kap = {kexpr};
+  for (int iq = 0; iq < {q.weights.size}; ++iq) {{
+    const double fw = FE_w[iq] * adet * kap;
+    for (int i = 0; i < {nv}; ++i)
+      for (int j = 0; j < {nw}; ++j)
+        A[i * {nw} + j] += fw * FE_phi[iq][i] * FE_psi[iq][j];
+  }}
+""")
+    src = _SIG.format(name=name) + "\n{\n" + body + "}\n"
+    cells = np.arange(V.mesh.num_cells, dtype=np.int32)
+    spec = dict(kind="mass", cell=0, degree=ev.degree, trial_degree=ew.degree, bs=1, rank=2,
+                qdeg=qd, kappa=_src_of(kexpr))
+    return Form(2, [V, W], [Integral(CELL, -1, cells, name, src, None, spec)], coeffs, consts,
+                "mass")
+
+
 def source(V: FunctionSpace, f, qdegree: Optional[int] = None) -> Form:
     """L(v) = integral f . v dx; f a Constant (bs values) or a Function on a scalar/bs space."""
     _check_affine(V)

@@ -132,6 +132,18 @@ int mf_form_pack_coefficients(mf_form* form, const int* indices, int n, mf_strea
  * 2 cooperative owner-computes), "chunk_values" (owner-computes LDS budget
  * in doubles), "block" (threads per workgroup). */
 int mf_form_set_option(mf_form* form, const char* key, int64_t value);
+/* Block / restricted assembly (CUDAForm::set_restriction, CUDAForm.h:201-231,
+ * and the offset + restriction unrolling of CUDADofMap::update,
+ * CUDADofMap.cpp:199-236): argument `axis` (0 rows, 1 columns) of the form
+ * is numbered in a global block space: node r becomes scalar dofs
+ * offset + bs * node_map[r] + j, and nodes with node_map[r] == -1 are
+ * dropped (their rows / columns are skipped).  node_map NULL keeps every node
+ * (identity) and only applies the offset; n = nodes of that space.  A form
+ * so marked assembles through the per-entity kernels into a block matrix
+ * (mf_matrix_create_block) or a block vector; lifting of such forms is not
+ * supported.  May be called after compile. */
+int mf_form_set_restriction(mf_form* form, int axis, const int32_t* node_map, int64_t n,
+                            int64_t offset);
 /* Generated HIP source of the form (after compile); len includes the NUL. */
 int mf_form_source(mf_form* form, char* buf, int64_t* len);
 int mf_form_destroy(mf_form* form);
@@ -143,6 +155,13 @@ int mf_form_destroy(mf_form* form);
  * (sorted unique columns per blocked row, bs-unrolled), keeps row_ptr /
  * columns / values on the device. */
 int mf_matrix_create(mf_form* a, mf_matrix** out);
+/* Monolithic matrix of a block form (fem::petsc::create_cuda_matrix_block,
+ * cpp/cudolfinx/fem/petsc.h:39-107, with the restricted sparsity of
+ * fem/utils.h:13-90): forms is row-major nr x nc (NULL entries are empty
+ * blocks), each placed by its restriction maps and offsets; the pattern is
+ * built per scalar row (sorted, unique).  Assemble each block with
+ * mf_assemble_matrix(form_ij, A, ..., zero = 0) after one zeroing. */
+int mf_matrix_create_block(mf_form* const* forms, int nr, int nc, mf_matrix** out);
 int mf_matrix_info(mf_matrix* A, int64_t* nrows, int64_t* ncols, int64_t* nnz);
 /* Device pointers of the CSR arrays (values may be replaced by a borrowed
  * buffer, e.g. a PETSc-HIP MatSeqAIJHIPSPARSEGetArray pointer). */

@@ -139,7 +139,8 @@ def basis(cell: int, k: int, X):
 
 # ------------------------------------------------------------- kernels
 _KINDS = {"stiffness": 0, "elasticity": 1, "source": 2, "facet_source": 3,
-          "dg_cell": 4, "dg_ext": 5, "dg_int": 6, "fn_cell": 7, "fn_ext": 8, "fn_int": 9}
+          "dg_cell": 4, "dg_ext": 5, "dg_int": 6, "fn_cell": 7, "fn_ext": 8, "fn_int": 9,
+          "mass": 10}
 _TYPES = {"cell": 0, "exterior_facet": 1, "interior_facet": 2}
 
 
@@ -181,6 +182,8 @@ def kernel_from_spec(spec: dict, coeff_width: int = 0) -> Kernel:
         k.kappa_off = int(spec["kappa_off"])
     if "v_off" in spec:
         k.f_off, k.f_degree = int(spec["v_off"]), int(spec["v_degree"])
+    if "trial_degree" in spec:
+        k.f_degree = int(spec["trial_degree"])
     k.coeff_width = coeff_width
     return k
 
@@ -191,14 +194,16 @@ def tabulate(spec: dict, w, c, coordinate_dofs, entity_local_index=None):
     nd = lib().or_ndofs(spec["cell"], spec["degree"]) * spec["bs"]
     mult = 2 if spec["kind"] == "dg_int" else 1
     n = nd * mult
-    A = np.zeros(n * n if spec["rank"] == 2 else (n if spec["rank"] == 1 else 1))
+    m = (lib().or_ndofs(spec["cell"], spec["trial_degree"]) * spec["bs"] * mult
+         if "trial_degree" in spec else n)
+    A = np.zeros(n * m if spec["rank"] == 2 else (n if spec["rank"] == 1 else 1))
     w = np.ascontiguousarray(w, dtype=np.float64)
     c = np.ascontiguousarray(c if len(c) else np.zeros(1), dtype=np.float64)
     xc = np.ascontiguousarray(coordinate_dofs, dtype=np.float64)
     eli = (C.c_int * 2)(*(list(entity_local_index or [0, 0]) + [0, 0])[:2])
     lib().or_tabulate(C.byref(k), _p(A, C.c_double), _p(w if w.size else np.zeros(1), C.c_double),
                       _p(c, C.c_double), _p(xc, C.c_double), eli)
-    return A.reshape(n, n) if spec["rank"] == 2 else (A if spec["rank"] == 1 else A[0])
+    return A.reshape(n, m) if spec["rank"] == 2 else (A if spec["rank"] == 1 else A[0])
 
 
 # ------------------------------------------------------------- assembly

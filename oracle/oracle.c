@@ -14,7 +14,7 @@
  *     e3=(0,3) e4=(0,2) e5=(0,1), faces opposite vertex i; hexahedra
  *     lexicographic);
  *   - the element integrals the engine generates (stiffness / Helmholtz,
- *     linear elasticity, volume and facet sources, SIPG + p-Laplacian DG
+ *     mass between two Lagrange spaces, linear elasticity, volume and facet sources, SIPG + p-Laplacian DG
  *     with its energy functional, a functional over cells, exterior and
  *     interior facets),
  *     evaluated directly at quadrature points, in the UFCx argument layout;
@@ -319,7 +319,8 @@ enum {
   OR_DG_INT = 6,      /* SIPG interior facet                      */
   OR_FN_CELL = 7,     /* functional: exp(u) kappa dx              */
   OR_FN_EXT = 8,      /* functional: u kappa ds                   */
-  OR_FN_INT = 9       /* functional: avg(v^2) avg(kappa) dS       */
+  OR_FN_INT = 9,      /* functional: avg(v^2) avg(kappa) dS       */
+  OR_MASS = 10        /* kappa u v, P_degree test x P_f_degree trial */
 };
 
 typedef struct {
@@ -651,6 +652,25 @@ static void tab_fn_int(const or_kernel* K, double* A, const double* w, const dou
   }
 }
 
+/* kappa u v with v in P_degree (rows) and u in P_f_degree (columns) */
+static void tab_mass(const or_kernel* K, double* A, const double* w, const double* c,
+                     const double* xc)
+{
+  static double pts[3 * OR_MAXQ], wts[OR_MAXQ];
+  const int nq = or_tet_rule(K->qm, pts, wts);
+  const int nv = or_ndofs(0, K->degree), nw = or_ndofs(0, K->f_degree);
+  double Kinv[3][3], phi[OR_MAXND], psi[OR_MAXND], d[3 * OR_MAXND];
+  const double kap = param(K->kappa_src, K->kappa_off, K->kappa_val, w, c);
+  for (int q = 0; q < nq; ++q) {
+    const double* X = &pts[3 * q];
+    double dx = wts[q] * fabs(jacobian(0, xc, X, Kinv)) * kap;
+    tet_basis(K->degree, X, phi, d);
+    tet_basis(K->f_degree, X, psi, d);
+    for (int i = 0; i < nv; ++i)
+      for (int j = 0; j < nw; ++j) A[i * nw + j] += dx * phi[i] * psi[j];
+  }
+}
+
 /* UFCx-shaped entry point: A += element tensor. */
 void or_tabulate(const or_kernel* K, double* A, const double* w, const double* c,
                  const double* coordinate_dofs, const int* entity_local_index)
@@ -664,6 +684,7 @@ void or_tabulate(const or_kernel* K, double* A, const double* w, const double* c
   case OR_DG_EXT: tab_dg_ext(K, A, w, coordinate_dofs, entity_local_index[0]); break;
   case OR_DG_INT: tab_dg_int(K, A, w, coordinate_dofs, entity_local_index); break;
   case OR_FN_CELL: tab_fn_cell(K, A, w, coordinate_dofs); break;
+  case OR_MASS: tab_mass(K, A, w, c, coordinate_dofs); break;
   case OR_FN_EXT: tab_fn_ext(K, A, w, coordinate_dofs, entity_local_index[0]); break;
   case OR_FN_INT: tab_fn_int(K, A, w, coordinate_dofs, entity_local_index); break;
   default: break;

@@ -53,6 +53,8 @@ def _check_form(form, tmp_path, ncells=5, seed=0):
         lib = _compile(I.source, tmp_path, f"f{k}_{I.name}")
         V0 = form.function_spaces[0] if form.rank else None
         nd = V0.element.ndofs * V0.bs if V0 else 1
+        V1 = form.function_spaces[-1] if form.rank == 2 else V0
+        nd1 = V1.element.ndofs * V1.bs if V1 else 1
         ne = I.num_entities
         stride = {"cell": 1, "exterior_facet": 2, "interior_facet": 4}[I.type]
         for e in rng.choice(ne, size=min(ncells, ne), replace=False):
@@ -61,7 +63,7 @@ def _check_form(form, tmp_path, ncells=5, seed=0):
             xc = mesh.x[mesh.cells[c]]
             w = packed[c]
             eli = [int(ent[1])] if I.type == "exterior_facet" else None
-            n = nd
+            n, n1 = nd, nd1
             if I.type == "interior_facet":
                 c1 = ent[2]
                 xc = np.concatenate([xc, mesh.x[mesh.cells[c1]]])
@@ -71,7 +73,8 @@ def _check_form(form, tmp_path, ncells=5, seed=0):
                                     for j in range(len(off) - 1)]) if len(off) > 1 else w
                 eli = [int(ent[1]), int(ent[3])]
                 n = 2 * nd if form.rank else 1
-            nA = n * n if form.rank == 2 else n
+                n1 = 2 * nd1
+            nA = n * n1 if form.rank == 2 else n
             got = _call(lib, I.name, nd, w, consts, xc, eli, nA)
             ref = cpu.tabulate(I.spec, w, consts, xc, eli).reshape(-1)
             scale = max(np.abs(ref).max(), 1e-300)
@@ -117,6 +120,16 @@ def test_generated_sipg_plaplace(tmp_path, k):
     P.extras["u"].x[:] = rng.uniform(-1, 1, P.V.size)
     _check_form(P.a, tmp_path, ncells=4)
     _check_form(P.L, tmp_path, ncells=4)
+
+
+@pytest.mark.parametrize("kv,kw", [(1, 1), (2, 1), (1, 3)])
+def test_generated_mixed_mass(tmp_path, kv, kw):
+    mesh = create_unit_cube(2, jitter=0.2, seed=3)
+    V, W = functionspace(mesh, ("P", kv)), functionspace(mesh, ("P", kw))
+    D = functionspace(mesh, ("DG", 0))
+    kap = Function(D)
+    kap.set_cellwise(np.linspace(1, 2, mesh.num_cells))
+    _check_form(forms.mass(V, W, kap), tmp_path)
 
 
 @pytest.mark.parametrize("k", [1, 2])

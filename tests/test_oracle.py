@@ -592,3 +592,21 @@ def test_interpolation_shared_nodes_take_last_cell():
         for i in range(4):
             ref[V1.dofmap[c, i]] = u0[V0.dofmap[c, i]]
     np.testing.assert_allclose(u1, ref, atol=1e-15)
+
+
+# ------------------------------------------------------------ mass between two spaces
+@pytest.mark.parametrize("kv,kw", [(1, 1), (1, 2), (3, 2)])
+def test_mixed_mass_element_sums(kv, kw):
+    """Partition of unity in both spaces: the entries sum to |T|, row i sums to
+    int phi_i (= |T|/4 for P1 rows); P1 x P1 is |T|/20 (1 + delta_ij)."""
+    x = np.array([[0.1, 0.0, 0.2], [1.1, 0.3, 0.0], [0.2, 0.9, 0.1], [0.3, 0.2, 1.3]])
+    vol = abs(np.linalg.det(x[1:] - x[0])) / 6
+    spec = dict(kind="mass", cell=0, degree=kv, trial_degree=kw, bs=1, rank=2,
+                qm=max(1, (kv + kw + 2) // 2), kappa=("val", 1.0))
+    A = cpu.tabulate(spec, [], [], x)
+    assert A.shape == (cpu.lib().or_ndofs(0, kv), cpu.lib().or_ndofs(0, kw))
+    assert abs(A.sum() - vol) <= 1e-14
+    if kv == 1:
+        np.testing.assert_allclose(A.sum(axis=1), vol / 4, rtol=1e-13)
+    if kv == 1 and kw == 1:
+        np.testing.assert_allclose(A, vol / 20 * (np.ones((4, 4)) + np.eye(4)), rtol=1e-13)

@@ -250,6 +250,13 @@ bool hex_owner_supported(const mf_form& f, const Integral& I) {
          f.dm[0]->bs == 1 && f.dm[0]->nd == 64;
 }
 
+// Threads per workgroup of the hex matrix-core kernel: 512 for the FP32 / BF16
+// operand paths (more waves to overlap the MFMA chains with the operand
+// loads), 256 for FP64; option hex_block overrides both.
+int hex_block(const mf_form& f, int prec) {
+  return static_cast<int>(f.opt("hex_block", prec == 64 ? 256 : 512));
+}
+
 int hex_batch(const mf_form& f, const Integral& I, int prec) {
   const int nq1 = staged_define(I.staged_src, "HX_NQ1");
   const int64_t cap = f.opt("chunk_values", 4096), maxr = f.opt("chunk_rows", 64);
@@ -429,7 +436,7 @@ std::string generate_form_source(const mf_form& f) {
           for (int prec : {64, 32, 16}) {
             m["$KNAME"] = kernel_name(("howner" + std::to_string(prec)).c_str(), I->type, I->index);
             m["$HNB"] = std::to_string(hex_batch(f, *I, prec));
-            m["$HBLOCK"] = std::to_string(f.opt("hex_block", 256));
+            m["$HBLOCK"] = std::to_string(hex_block(f, prec));
             m["$OMAXP"] = std::to_string(f.opt("owner_pairs", 384));
             os << "\n#define MF_HX_PREC " << prec << "\n" << subst(kHexOwnerTemplate, m)
                << "\n#undef MF_HX_PREC\n";

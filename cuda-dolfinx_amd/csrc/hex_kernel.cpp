@@ -99,6 +99,10 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
   __shared__ mf_hrec_$TAG srec[NB];
   __shared__ real sD[NB * NQ * DS];
   __shared__ int32_t spair[MAXP];
+  // 1D basis values / derivatives at the 1D points, [q][n], in LDS: the
+  // contraction reads them with lane-dependent n, which from constant memory
+  // would be a global load (and a wait) per quadrature line
+  __shared__ real sTL[M1 * K1], sTD[M1 * K1];
 
   const int k = mf_xcd_chunk(nchunks);
   if (k < 0) return;
@@ -112,6 +116,15 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
   const int32_t p0 = m2.x, p1 = m2.y;
   const int nr = r1 - r0, npc = p1 - p0;
   for (int p = tid; p < npc; p += nt) spair[p] = pair_list[p0 + p];
+  if (tid < M1 * K1) {
+#if MF_HX_PREC == 64
+    sTL[tid] = HX_L[tid / K1][tid % K1];
+    sTD[tid] = HX_DL[tid / K1][tid % K1];
+#else
+    sTL[tid] = HX_LF[tid / K1][tid % K1];
+    sTD[tid] = HX_DLF[tid / K1][tid % K1];
+#endif
+  }
   {
     mf_d2* a2 = reinterpret_cast<mf_d2*>(acc);
     const mf_d2 z = {0.0, 0.0};
@@ -229,13 +242,13 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
 #pragma unroll
       for (int k = 0; k < 16; ++k) cacc[k] = 0.0;
       for (int q3 = 0; q3 < M1; ++q3) {
-        const double Lz = HX_L[q3][ic], Gz = HX_DL[q3][ic];
-        const double zl = HX_L[q3][jc], zd = HX_DL[q3][jc];
+        const double Lz = sTL[(q3) * K1 + (ic)], Gz = sTD[(q3) * K1 + (ic)];
+        const double zl = sTL[(q3) * K1 + (jc)], zd = sTD[(q3) * K1 + (jc)];
         for (int q2 = 0; q2 < M1; ++q2) {
-          const double Ly = HX_L[q2][ib2], Gy = HX_DL[q2][ib2];
+          const double Ly = sTL[(q2) * K1 + (ib2)], Gy = sTD[(q2) * K1 + (ib2)];
           for (int q1 = 0; q1 < M1; ++q1) {
             const double* D = &Db[(q1 + M1 * (q2 + M1 * q3)) * DS];
-            const double Lx = HX_L[q1][ia], Gx = HX_DL[q1][ia];
+            const double Lx = sTL[(q1) * K1 + (ia)], Gx = sTD[(q1) * K1 + (ia)];
             const double B0 = Lx * Ly * Lz, Bx = Gx * Ly * Lz, By = Lx * Gy * Lz, Bz = Lx * Ly * Gz;
             double f0 = Bx * D[0] + By * D[1] + Bz * D[2];
             double f1 = Bx * D[1] + By * D[3] + Bz * D[4];
@@ -246,12 +259,12 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
             const double g0 = f0 * zl, g1 = f1 * zl, g2 = f2 * zd, g3 = f3 * zl;
 #pragma unroll
             for (int jb = 0; jb < 4; ++jb) {
-              const double yl = HX_L[q2][jb], yd = HX_DL[q2][jb];
+              const double yl = sTL[(q2) * K1 + (jb)], yd = sTD[(q2) * K1 + (jb)];
               const double hd = g0 * yl;                       // multiplies X'
               const double hl = g1 * yd + (g2 + g3) * yl;      // multiplies X
 #pragma unroll
               for (int ja = 0; ja < 4; ++ja)
-                cacc[ja + 4 * jb] += hd * HX_DL[q1][ja] + hl * HX_L[q1][ja];
+                cacc[ja + 4 * jb] += hd * sTD[(q1) * K1 + (ja)] + hl * sTL[(q1) * K1 + (ja)];
             }
           }
         }
@@ -291,8 +304,8 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
         const int q = qb + slot;
         const int q1 = q % M1, q2 = (q / M1) % M1, q3 = q / (M1 * M1);
         const real* D = &Db[q * DS];
-        const double Lx = HX_L[q1][ia], Ly = HX_L[q2][ib2], Lz = HX_L[q3][ic];
-        const double Gx = HX_DL[q1][ia], Gy = HX_DL[q2][ib2], Gz = HX_DL[q3][ic];
+        const double Lx = sTL[(q1) * K1 + (ia)], Ly = sTL[(q2) * K1 + (ib2)], Lz = sTL[(q3) * K1 + (ic)];
+        const double Gx = sTD[(q1) * K1 + (ia)], Gy = sTD[(q2) * K1 + (ib2)], Gz = sTD[(q3) * K1 + (ic)];
         const double B0 = Lx * Ly * Lz, Bx = Gx * Ly * Lz, By = Lx * Gy * Lz, Bz = Lx * Ly * Gz;
         float f[4];
         f[0] = (float)(Bx * D[0] + By * D[1] + Bz * D[2]);
@@ -301,10 +314,10 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
         f[3] = (float)(B0 * D[6]);
         mf_bf4 av;
         for (int h = 0; h < 4; ++h) av[h] = valid ? mf_to_bf16(f[h]) : (short)0;
-        const double cLx = HX_L[q1][ja], cLy = HX_L[q2][jb];
-        const double cGx = HX_DL[q1][ja], cGy = HX_DL[q2][jb];
+        const double cLx = sTL[(q1) * K1 + (ja)], cLy = sTL[(q2) * K1 + (jb)];
+        const double cGx = sTD[(q1) * K1 + (ja)], cGy = sTD[(q2) * K1 + (jb)];
         for (int jt = 0; jt < 4; ++jt) {
-          const double cLz = HX_L[q3][jt], cGz = HX_DL[q3][jt];
+          const double cLz = sTL[(q3) * K1 + (jt)], cGz = sTD[(q3) * K1 + (jt)];
           mf_bf4 bv;
           bv[0] = mf_to_bf16((float)(cGx * cLy * cLz));
           bv[1] = mf_to_bf16((float)(cLx * cGy * cLz));
@@ -320,30 +333,46 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
       float cx[4], cy[4];
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
-        cx[m] = slot == 0 ? HX_DLF[m][ja] : HX_LF[m][ja];
-        cy[m] = slot == 1 ? HX_DLF[m][jb] : HX_LF[m][jb];
+        cx[m] = slot == 0 ? sTD[(m) * K1 + (ja)] : sTL[(m) * K1 + (ja)];
+        cy[m] = slot == 1 ? sTD[(m) * K1 + (jb)] : sTL[(m) * K1 + (jb)];
       }
       // which D entries combine into this lane's K slot (slot 3: the mass entry)
       const int c0 = slot == 0 ? 0 : (slot == 1 ? 1 : (slot == 2 ? 2 : 6));
       const int c1 = slot == 0 ? 1 : (slot == 1 ? 3 : 4);
       const int c2 = slot == 0 ? 2 : (slot == 1 ? 4 : 5);
+      // one branch-free formula for every slot:
+      //   fa = P(q1) LyLz D[c0] + Q(q1) (GyLz D[c1] + LyGz D[c2])
+      // slots 0-2: P = Gx, Q = Lx (the gradient row); slot 3: P = Lx, Q = 0
+      // (the mass entry b0 D[6]); invalid rows are zeroed through P and Q
       const float vmask = valid ? 1.0f : 0.0f;
+      float P[4], Q[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const float Lx = sTL[m * K1 + ia], Gx = sTD[m * K1 + ia];
+        P[m] = (slot == 3 ? Lx : Gx) * vmask;
+        Q[m] = (slot == 3 ? 0.0f : Lx) * vmask;
+      }
 #pragma unroll 1
       for (int q3 = 0; q3 < M1; ++q3) {
-        const float Lz = HX_LF[q3][ic], Gz = HX_DLF[q3][ic];
+        const float Lz = sTL[(q3) * K1 + (ic)], Gz = sTD[(q3) * K1 + (ic)];
         float cz[4];
 #pragma unroll
-        for (int jt = 0; jt < 4; ++jt) cz[jt] = slot == 2 ? HX_DLF[q3][jt] : HX_LF[q3][jt];
-#pragma unroll 1
-        for (int q2 = 0; q2 < M1; ++q2) {
-          const float Ly = HX_LF[q2][ib2], Gy = HX_DLF[q2][ib2];
-          const float LyLz = Ly * Lz * vmask, GyLz = Gy * Lz * vmask, LyGz = Ly * Gz * vmask;
+        for (int jt = 0; jt < 4; ++jt) cz[jt] = slot == 2 ? sTD[(q3) * K1 + (jt)] : sTL[(q3) * K1 + (jt)];
+#pragma unroll
+        for (int q2 = 0; q2 < 4; ++q2) {
+          const float Ly = sTL[(q2) * K1 + (ib2)], Gy = sTD[(q2) * K1 + (ib2)];
+          const float LyLz = Ly * Lz, GyLz = Gy * Lz, LyGz = Ly * Gz;
+          const float* Dq = &Db[(4 * (q2 + 4 * q3)) * DS];
+          float d0[4], d1[4], d2[4];
 #pragma unroll
           for (int q1 = 0; q1 < 4; ++q1) {
-            const float* D = &Db[(q1 + 4 * (q2 + 4 * q3)) * DS];
-            const float Lx = HX_LF[q1][ia], Gx = HX_DLF[q1][ia];
-            const float bx = Gx * LyLz, by = Lx * GyLz, bz = Lx * LyGz, b0 = Lx * LyLz;
-            const float fa = slot == 3 ? b0 * D[6] : bx * D[c0] + by * D[c1] + bz * D[c2];
+            d0[q1] = Dq[q1 * DS + c0];
+            d1[q1] = Dq[q1 * DS + c1];
+            d2[q1] = Dq[q1 * DS + c2];
+          }
+#pragma unroll
+          for (int q1 = 0; q1 < 4; ++q1) {
+            const float fa = P[q1] * LyLz * d0[q1] + Q[q1] * (GyLz * d1[q1] + LyGz * d2[q1]);
             const float cxy = cx[q1] * cy[q2];
 #pragma unroll
             for (int jt = 0; jt < 4; ++jt)

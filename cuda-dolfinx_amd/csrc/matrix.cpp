@@ -529,7 +529,7 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
              static_cast<int32_t>(owner->n), owner->d_ents.ptr, a->mesh->cells.ptr,
              a->mesh->x.ptr, a->dm[0]->dev.ptr, n0, owner->records.ptr);
       launch(owner->k_howner[v], static_cast<uint32_t>(per * nx),
-             static_cast<uint32_t>(a->opt("hex_block", 256)), 0, s,
+             static_cast<uint32_t>(hex_block(*a, static_cast<int>(prec))), 0, s,
              static_cast<int32_t>(plan->nchunks), plan->chunk_meta.ptr, plan->item_cell.ptr,
              plan->pair_list.ptr, plan->pair_pos.ptr, owner->records.ptr, A->rpb.ptr,
              plan->diag_pos.ptr, n0, A->values, diagonal, flags);

@@ -296,6 +296,7 @@ bool is_hex_staged(const Integral& I);
 bool staged_trace(const Integral& I);
 bool hex_owner_supported(const mf_form& f, const Integral& I);
 int hex_batch(const mf_form& f, const Integral& I, int prec);
+int hex_block(const mf_form& f, int prec);
 struct OwnerShape { int nb = 0, maxp = 0; };
 OwnerShape owner_shape(const mf_form& f, const Integral& I);
 int64_t record_bytes(const mf_form& f, const Integral& I);

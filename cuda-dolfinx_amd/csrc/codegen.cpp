@@ -235,6 +235,10 @@ int64_t record_bytes(const mf_form& f, const Integral& I) {
   return (raw + 15) / 16 * 16;
 }
 
+bool staged_fullrt(const Integral& I) {
+  return I.staged_src.find("#define SK_FULLRT ") != std::string::npos;
+}
+
 bool staged_trace(const Integral& I) {
   return I.staged_src.find("#define SK_TRACE ") != std::string::npos;
 }
@@ -288,7 +292,7 @@ OwnerShape owner_shape(const mf_form& f, const Integral& I) {
   OwnerShape sh;
   sh.nb = static_cast<int>(std::min<int64_t>(f.opt("owner_nb", 64), 1023));
   sh.maxp = static_cast<int>(f.opt("owner_pairs", 384));
-  const int64_t trace = staged_trace(I) ? 8LL * nd * nd * 6 : 0;
+  const int64_t trace = staged_trace(I) ? 8LL * nd * nd * 6 : (staged_fullrt(I) ? 8LL * nd * nd * 9 : 0);
   const int64_t used = fixed + trace + (int64_t)sh.maxp * (4 + nd);
   MF_CHECK(used <= budget, MF_EUNSUPPORTED,
            "owner-computes kernel shape exceeds the LDS budget (" + std::to_string(used) + " > " +
@@ -447,7 +451,7 @@ std::string generate_form_source(const mf_form& f) {
         // namespace the staged symbols per integral
         std::string pfx = "I" + std::to_string(I->index) + "_";
         auto rename = [&](std::string t) {
-          for (const char* sym : {"SK_ND", "SK_BS", "SK_NQ", "SK_W", "SK_DPHI", "SK_TRACE", "SK_RS",
+          for (const char* sym : {"SK_ND", "SK_BS", "SK_NQ", "SK_W", "SK_DPHI", "SK_TRACE", "SK_RS", "SK_FULLRT", "SK_RT",
                                   "sk_combine"})
             replace_all(t, sym, pfx + sym);
           return t;

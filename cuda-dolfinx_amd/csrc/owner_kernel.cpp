@@ -154,6 +154,10 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
   // reference tensors R_ij (symmetrised, 6 per entry), loaded once per workgroup
   __shared__ double sRS[ND * ND * 6];
   for (int p = threadIdx.x; p < ND * ND * 6; p += NT) sRS[p] = (&SK_RS[0][0][0])[p];
+#elif defined(SK_FULLRT)
+  // full reference tensors R_ij[3 r + s] (9 per entry), loaded once per workgroup
+  __shared__ double sRT[ND * ND * 9];
+  for (int p = threadIdx.x; p < ND * ND * 9; p += NT) sRT[p] = (&SK_RT[0][0][0])[p];
 #endif
 
   const int tid = threadIdx.x, nt = NT;
@@ -308,6 +312,10 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
         const double M00 = dt * (K0 * K0 + K1 * K1 + K2 * K2), M11 = dt * (K3 * K3 + K4 * K4 + K5 * K5);
         const double M22 = dt * (K6 * K6 + K7 * K7 + K8 * K8), M01 = dt * (K0 * K3 + K1 * K4 + K2 * K5);
         const double M02 = dt * (K0 * K6 + K1 * K7 + K2 * K8), M12 = dt * (K3 * K6 + K4 * K7 + K5 * K8);
+#elif defined(SK_FULLRT)
+        // G_ij = |det J| K^T R_ij K; the left factor carries |det J|
+        const double L0 = dt * K0, L1 = dt * K1, L2 = dt * K2, L3 = dt * K3, L4 = dt * K4;
+        const double L5 = dt * K5, L6 = dt * K6, L7 = dt * K7, L8 = dt * K8;
 #else
         double gi[NQ][3];
         for (int q = 0; q < NQ; ++q) {
@@ -332,6 +340,25 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
           {
             const double* R = &sRS[(i * ND + j) * 6];
             G[0] = M00 * R[0] + M11 * R[1] + M22 * R[2] + M01 * R[3] + M02 * R[4] + M12 * R[5];
+          }
+#elif defined(SK_FULLRT)
+          {
+            const double* R = &sRT[(i * ND + j) * 9];
+            // T = R K (T[3 r + c] = sum_s R[3 r + s] K[3 s + c]), G = L^T T
+            double T[9];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+              const double a0 = R[3 * r], a1 = R[3 * r + 1], a2 = R[3 * r + 2];
+              T[3 * r] = a0 * K0 + a1 * K3 + a2 * K6;
+              T[3 * r + 1] = a0 * K1 + a1 * K4 + a2 * K7;
+              T[3 * r + 2] = a0 * K2 + a1 * K5 + a2 * K8;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              G[c] = L0 * T[c] + L3 * T[3 + c] + L6 * T[6 + c];
+              G[3 + c] = L1 * T[c] + L4 * T[3 + c] + L7 * T[6 + c];
+              G[6 + c] = L2 * T[c] + L5 * T[3 + c] + L8 * T[6 + c];
+            }
           }
 #else
           for (int q = 0; q < NQ; ++q) {

@@ -85,6 +85,7 @@ class StagedKernel:
     coeff_width: int             # packed coefficient values per cell
     symmetric: bool = True
     trace_only: bool = False     # combine reads only G[0] + G[4] + G[8]
+    full_tensor: bool = False    # emit the (nd, nd, 3, 3) reference tensor (G = |det J| K^T R K)
 
     def reference_tensor(self) -> np.ndarray:
         """(nd, nd, 6) R_ij = sum_q w_q dphi_i (x) dphi_j in reference coordinates,
@@ -98,9 +99,21 @@ class StagedKernel:
                          R[..., 0, 1] + R[..., 1, 0], R[..., 0, 2] + R[..., 2, 0],
                          R[..., 1, 2] + R[..., 2, 1]], axis=-1)
 
+    def reference_tensor_full(self) -> np.ndarray:
+        """(nd, nd, 9) R_ij[3 r + s] = sum_q w_q dphi_i/dX_r dphi_j/dX_s on the reference cell.
+
+        For affine cells G_ij[p][c] = |det J| sum_rs K_rp R_ij[r][s] K_sc with
+        K = J^-1 (K_rp = dX_r / dx_p): 54 multiply-adds per entry at any degree.
+        """
+        R = np.einsum("q,qia,qjb->ijab", self.weights, self.dphi, self.dphi)
+        return R.reshape(self.nd, self.nd, 9)
+
     def source(self) -> str:
         trace = ("#define SK_TRACE 1\n" + _carr("SK_RS", self.reference_tensor(), "__constant__ double")
                  if self.trace_only else "")
+        if self.full_tensor and not self.trace_only:
+            trace += ("#define SK_FULLRT 1\n"
+                      + _carr("SK_RT", self.reference_tensor_full(), "__constant__ double"))
         return (f"#define SK_ND {self.nd}\n#define SK_BS {self.bs}\n#define SK_NQ {self.nq}\n"
                 + trace
                 + _carr("SK_W", self.weights, "__constant__ double")
@@ -255,11 +268,17 @@ def stiffness(V: FunctionSpace, kappa=None, qdegree: Optional[int] = None) -> Fo
                 "stiffness")
 
 
-def elasticity(V: FunctionSpace, mu, lam, qdegree: Optional[int] = None) -> Form:
+def elasticity(V: FunctionSpace, mu, lam, qdegree: Optional[int] = None,
+               reference_tensor: bool = False) -> Form:
     """a(u, v) = integral 2 mu eps(u):eps(v) + lam div u div v dx on a vector Lagrange space.
 
     Entry ((i, a), (j, b)) = mu (delta_ab G_ij:I + G_ij[b][a]) + lam G_ij[a][b] with
-    G_ij[p][q] = integral d_p phi_i d_q phi_j.
+    G_ij[p][q] = integral d_p phi_i d_q phi_j.  reference_tensor=True also emits
+    the (nd, nd, 9) reference tensors so the owner-computes kernel forms
+    G_ij = |det J| K^T R_ij K (54 FMAs) instead of summing over quadrature
+    points; the table costs nd^2 * 72 bytes of LDS per workgroup (at P2 it
+    lowers the resident workgroups per CU from 4 to 3, which costs more than
+    the arithmetic saves).
     """
     _check_affine(V)
     if V.bs != 3:
@@ -308,7 +327,7 @@ def elasticity(V: FunctionSpace, mu, lam, qdegree: Optional[int] = None) -> Form
       out[3 * a + b] = mu * ((a == b ? tr : 0.0) + G[3 * b + a]) + lam * G[3 * a + b];
 """
     staged = StagedKernel(nd=nd, bs=3, nq=q.weights.size, weights=q.weights, dphi=dphi,
-                          combine=combine, coeff_width=len(coeffs))
+                          combine=combine, coeff_width=len(coeffs), full_tensor=reference_tensor)
     cells = np.arange(V.mesh.num_cells, dtype=np.int32)
     spec = dict(kind="elasticity", cell=0, degree=el.degree, bs=3, rank=2,
                 qdeg=2 * (el.degree - 1) if qdegree is None else qdegree,

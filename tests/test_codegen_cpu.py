@@ -203,6 +203,40 @@ def test_reference_tensor_trace_matches_oracle(k):
         np.testing.assert_allclose(A, ref, atol=1e-13 * abs(ref).max())
 
 
+@pytest.mark.parametrize("k", [1, 2])
+def test_full_reference_tensor_elasticity_matches_oracle(k):
+    """Elasticity on the full reference-tensor path: G_ij = |det J| K^T R_ij K
+    (K = J^-1), combined as the kernels do (mu (delta_ab tr G + G^T) + lam G),
+    reproduces the oracle's element matrix."""
+    mesh = create_unit_cube(2, jitter=0.25, seed=5)
+    V = functionspace(mesh, ("P", k), shape=(3,))
+    D = functionspace(mesh, ("DG", 0))
+    mu, lam = Function(D), Function(D)
+    mu.set_cellwise(np.linspace(1.0, 3.0, mesh.num_cells))
+    lam.set_cellwise(np.linspace(5.0, 2.0, mesh.num_cells))
+    form = forms.elasticity(V, mu, lam, reference_tensor=True)
+    I = form.integrals[0]
+    S = I.staged
+    assert S.full_tensor and "SK_FULLRT" in S.source()
+    RT = S.reference_tensor_full().reshape(S.nd, S.nd, 3, 3)
+    packed = form.pack_coefficients()
+    nd = S.nd
+    for c in (0, 9, 40):
+        xc = mesh.x[mesh.cells[c]]
+        J = (xc[1:] - xc[0]).T
+        K = np.linalg.inv(J)                      # K[r, p] = dX_r / dx_p
+        det = abs(np.linalg.det(J))
+        m, l = packed[c][0], packed[c][1]
+        A = np.zeros((3 * nd, 3 * nd))
+        for i in range(nd):
+            for j in range(nd):
+                G = det * K.T @ RT[i, j] @ K
+                blk = m * (np.trace(G) * np.eye(3) + G.T) + l * G
+                A[3 * i:3 * i + 3, 3 * j:3 * j + 3] = blk
+        ref = cpu.tabulate(I.spec, packed[c], [], xc)
+        np.testing.assert_allclose(A, ref, atol=1e-13 * abs(ref).max())
+
+
 def test_library_exports_every_declared_symbol():
     header = (ROOT / "include" / "mifem.h").read_text()
     declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(mf_\w+)\s*\(", header, re.M))

@@ -85,6 +85,13 @@ def test_elasticity_p2(kernel):
     _compare(P, _gpu(P, {"matrix_kernel": kernel}), _oracle(P))
 
 
+def test_elasticity_p2_reference_tensor_owner():
+    _say("elasticity P2 n=6, owner kernel on the full reference-tensor path (G = |det J| K^T R K)")
+    P = problems.elasticity_p2(6)
+    P.a = forms.elasticity(P.V, P.extras["mu"], P.extras["lambda"], reference_tensor=True)
+    _compare(P, _gpu(P, {"matrix_kernel": 2}), _oracle(P))
+
+
 def test_elasticity_p2_medium():
     _say("elasticity_p2 n=16 owner-computes")
     P = problems.elasticity_p2(16)

@@ -535,3 +535,78 @@ def test_block_matrix_with_restriction_and_bcs():
     mask[bcrows] = False
     d = (sp.csr_matrix((v2, cols, rp), shape=ref.shape) - exp)[mask]
     assert abs(d).max() <= 1e-13 * abs(exp).max()
+
+
+# ------------------------------------------------------------ error behaviour and edge cases
+def test_error_paths_raise_like_the_reference():
+    """Argument checks raise TypeError / ValueError as python/cudolfinx/assemble.py
+    does; C-ABI status codes surface as ValueError (bad shape / argument) or
+    NotImplementedError (unsupported); the library keeps working afterwards."""
+    _say("error behaviour: wrong types, length and shape mismatches, unsupported calls")
+    import cudolfinx as cufem
+    from cudolfinx.la import CUDAVector
+    P = problems.poisson_p1(2)
+    a, L = cufem.form(P.a), cufem.form(P.L)
+    asm = cufem.CUDAAssembler()
+    with pytest.raises(TypeError):
+        asm.assemble_matrix(P.a)
+    with pytest.raises(TypeError):
+        asm.assemble_vector("L")
+    with pytest.raises(TypeError):
+        asm.create_matrix_block(a)
+    b = asm.create_vector(L)
+    with pytest.raises(ValueError):
+        asm.apply_lifting(b, [a], [])
+    with pytest.raises(ValueError):
+        asm.assemble_scalar(L)
+    with pytest.raises(ValueError):
+        asm.assemble_vector(L, CUDAVector(a._ctx, P.V.size + 1))
+    P3 = problems.poisson_p1(3)
+    with pytest.raises(ValueError):
+        asm.assemble_matrix(cufem.form(P3.a), asm.create_matrix(a))
+    mesh2 = create_unit_cube(2)
+    f_other = Function(functionspace(mesh2, ("P", 1)))
+    with pytest.raises(ValueError):
+        cufem.Coefficient(Function(P.V)).interpolate(cufem.Coefficient(f_other))
+    # lifting with a block member form is rejected
+    blk = cufem.form([[forms.stiffness(P.V)]])
+    with pytest.raises(NotImplementedError):
+        asm.apply_lifting(b, [blk.forms[0][0]], [P.bcs])
+    # still healthy: a normal assembly matches the oracle
+    _compare(P, _gpu(P), _oracle(P))
+
+
+def test_edge_cases_one_cube_and_empty_integrals():
+    _say("edge cases: one-cube meshes (P1, P2, vector P2), an empty facet integral, an empty form")
+    import dataclasses
+    import cudolfinx as cufem
+    for k in (1, 2):
+        mesh = create_unit_cube(1)
+        V = functionspace(mesh, ("P", k))
+        f = Function(V)
+        f.interpolate(lambda x: 1.0 + x[0])
+        bc = dirichletbc(0.0, locate_dofs_topological(V, 2, mesh.exterior_facets[:3]), V)
+
+        class _P:
+            pass
+        P = _P()
+        P.name, P.V, P.bcs = f"one cube P{k}", V, [bc]
+        P.a, P.L = forms.stiffness(V), forms.source(V, f)
+        _compare(P, _gpu(P), _oracle(P))
+    Pe = problems.elasticity_p2(1)
+    _compare(Pe, _gpu(Pe), _oracle(Pe))
+    # a facet integral over no facets adds nothing
+    P = problems.poisson_p1(3)
+    empty = forms.facet_source(P.V, Constant([1.0]), np.zeros((0, 2), dtype=np.int32))
+    P.L = forms.linear_sum(P.L, empty)
+    assert P.L.integrals[-1].num_entities == 0
+    _compare(P, _gpu(P), _oracle(P))
+    # a bilinear form whose only integral has no cells: empty pattern, nothing assembled
+    a0 = forms.stiffness(P.V)
+    a0.integrals[0] = dataclasses.replace(a0.integrals[0], entities=np.zeros(0, dtype=np.int32))
+    ag = cufem.form(a0)
+    asm = cufem.CUDAAssembler()
+    A = asm.assemble_matrix(ag, bcs=[])
+    asm.synchronize()
+    rp, cols, vals = A.csr()
+    assert A.nnz == 0 and cols.size == 0 and np.all(rp == 0)

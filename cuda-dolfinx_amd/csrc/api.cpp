@@ -635,6 +635,122 @@ int mf_assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* 
   });
 }
 
+// ------------------------------------------------------------------ fused Newton system
+namespace {
+bool same_integrals(const mf_form& x, const mf_form& y) {
+  if (x.ints.size() != y.ints.size() || x.mesh != y.mesh) return false;
+  for (size_t k = 0; k < x.ints.size(); ++k) {
+    const auto& a = *x.ints[k];
+    const auto& b = *y.ints[k];
+    if (a.type != b.type || a.index != b.index || a.h_ents != b.h_ents) return false;
+  }
+  return true;
+}
+bool same_coefficients(const mf_form& x, const mf_form& y) {
+  if (x.W != y.W || x.coefs.size() != y.coefs.size()) return false;
+  for (size_t k = 0; k < x.coefs.size(); ++k)
+    if (x.coefs[k].dm != y.coefs[k].dm || x.coefs[k].offset != y.coefs[k].offset ||
+        x.coefs[k].width != y.coefs[k].width)
+      return false;
+  return true;
+}
+}  // namespace
+
+int mf_system_create(mf_form* a, mf_form* L, mf_form* M, int block, mf_system** out) {
+  return guard([&] {
+    MF_CHECK(a && L && out, MF_EINVAL, "null argument");
+    MF_CHECK(a->rank == 2 && L->rank == 1 && (!M || M->rank == 0), MF_EINVAL,
+             "a system is a bilinear, a linear and an optional rank-0 form");
+    MF_CHECK(a->dm[0] == a->dm[1] && L->dm[0] == a->dm[0], MF_EINVAL,
+             "the system's forms must share one test/trial space");
+    MF_CHECK(same_integrals(*a, *L) && (!M || same_integrals(*a, *M)), MF_EINVAL,
+             "the system's forms must have the same integrals over the same entities");
+    MF_CHECK(same_coefficients(*a, *L) && (!M || same_coefficients(*a, *M)), MF_EINVAL,
+             "the system's forms must share their coefficient layout");
+    MF_CHECK(!a->restricted && !L->restricted, MF_EUNSUPPORTED, "block member forms");
+    const int ndb = a->dm[0]->nd * a->dm[0]->bs;
+    MF_CHECK(ndb <= 10, MF_EUNSUPPORTED, "the fused system kernel keeps 2 nd x 2 nd facet tensors in registers (nd bs <= 10)");
+    if (block <= 0) block = 64;
+    MF_CHECK(block % 64 == 0 && block <= 1024, MF_EINVAL, "block must be a multiple of 64, <= 1024");
+    use_device(a->ctx);
+    auto sys = std::make_unique<mf_system>();
+    sys->ctx = a->ctx;
+    sys->a = a;
+    sys->L = L;
+    sys->M = M;
+    sys->block = block;
+    sys->source = mf::generate_system_source(*a, *L, M, block, 0);
+    auto code = mf::rtc_compile(sys->source, "mifem_system", rtc_options(a->ctx), a->ctx->cache_dir, nullptr);
+    sys->mod = std::make_unique<mf::Module>();
+    sys->mod->load(code);
+    sys->k = sys->mod->get("mf_dgsys");
+    // persistent grid: the workgroups resident at once
+    int nblk = 0;
+    MF_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, sys->k, block, 0));
+    const int64_t ncells = a->mesh->num_cells;
+    sys->grid = std::max<int64_t>(1, std::min<int64_t>((ncells + block - 1) / block,
+                                                       (int64_t)std::max(1, nblk) * a->ctx->num_cus));
+    sys->partials.alloc(sys->grid);
+    sys->counter.alloc(1);
+    MF_HIP(hipMemset(sys->counter.ptr, 0, sizeof(uint32_t)));
+    MF_HIP(hipDeviceSynchronize());
+    *out = sys.release();
+  });
+}
+
+int mf_system_assemble(mf_system* sys, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc1,
+                       double diagonal, double* d_b, int64_t n, double* d_value, int flags,
+                       mf_stream s) {
+  return guard([&] {
+    MF_CHECK(sys && A && d_b, MF_EINVAL, "null argument");
+    mf_form* a = sys->a;
+    MF_CHECK(a->compiled && sys->L->compiled && (!sys->M || sys->M->compiled), MF_ESTATE,
+             "forms are not compiled");
+    MF_CHECK(!A->block && A->nrows == a->dm[0]->num_nodes * a->dm[0]->bs, MF_ESHAPE,
+             "matrix does not match the system's space");
+    MF_CHECK(n == A->nrows, MF_ESHAPE, "vector length mismatch");
+    MF_CHECK(!sys->M || d_value, MF_EINVAL, "the system has a functional: d_value is required");
+    if (bc0) MF_CHECK(bc0->ndofs == A->nrows, MF_ESHAPE, "bc0 size does not match the rows");
+    if (bc1) MF_CHECK(bc1->ndofs == A->ncols, MF_ESHAPE, "bc1 size does not match the columns");
+    use_device(sys->ctx);
+    auto it = A->dgplans.find(a->uid);
+    std::shared_ptr<mf::DGPlan> dp;
+    if (it == A->dgplans.end()) {
+      dp = mf::build_dg_plan(*a, A);
+      A->dgplans[a->uid] = dp;
+    } else {
+      dp = it->second;
+    }
+    MF_CHECK(dp != nullptr, MF_EUNSUPPORTED,
+             "the fused system needs a cell-major discontinuous space and its matrix");
+    const int8_t* m0 = bc0 && bc0->nbc ? bc0->markers.ptr : nullptr;
+    const int8_t* m1 = bc1 && bc1->nbc ? bc1->markers.ptr : nullptr;
+    int kflags = (flags & 1) ? 1 : 0;               // 1: overwrite A and b
+    if (m0) kflags |= 2;                            // Dirichlet diagonal (as mf_dgown)
+    if (flags & 2) kflags |= 4;                     // 2: add the functional into d_value
+    mf::launch(sys->k, static_cast<uint32_t>(sys->grid), static_cast<uint32_t>(sys->block), 0, S(s),
+               static_cast<int32_t>(dp->ncells), dp->cell_ptr.ptr,
+               reinterpret_cast<const int4*>(dp->items.ptr), a->mesh->cells.ptr, a->mesh->x.ptr,
+               a->packed.ptr, a->consts.ptr, sys->L->consts.ptr,
+               sys->M ? sys->M->consts.ptr : a->consts.ptr, A->row_ptr.ptr, m0, m1, A->values,
+               diagonal, kflags, a->mesh->fperm.ptr, d_b, sys->partials.ptr, sys->counter.ptr,
+               sys->M ? d_value : static_cast<double*>(nullptr));
+  });
+}
+
+int mf_system_source(mf_system* sys, char* buf, int64_t* len) {
+  return guard([&] {
+    MF_CHECK(sys && len, MF_EINVAL, "null argument");
+    int64_t need = static_cast<int64_t>(sys->source.size()) + 1;
+    if (buf && *len >= need) std::memcpy(buf, sys->source.c_str(), need);
+    *len = need;
+  });
+}
+
+int mf_system_destroy(mf_system* sys) {
+  return guard([&] { delete sys; });
+}
+
 // ------------------------------------------------------------------ interpolation
 int mf_interpolator_create(mf_context* ctx, mf_dofmap* dm1, mf_dofmap* dm0, const double* im,
                            mf_interpolator** out) {

@@ -323,6 +323,10 @@ extern const char* kDGIntMatBody;
 extern const char* kDGCaseCellVec;
 extern const char* kDGCaseExtVec;
 extern const char* kDGIntVecBody;
+extern const char* kDGSystem;
+extern const char* kDGSysCell;
+extern const char* kDGSysExt;
+extern const char* kDGSysIntBody;
 
 bool dg_owner_emitted(const mf_form& f) {
   if (f.rank == 0) return false;
@@ -330,6 +334,72 @@ bool dg_owner_emitted(const mf_form& f) {
   if (f.rank == 2 && f.dm[0] != f.dm[1]) return false;
   const int ndb = f.dm[0]->nd * f.dm[0]->bs;
   return (f.rank == 1 || f.rank == 2) && ndb <= 10;
+}
+
+std::string generate_system_source(const mf_form& a, const mf_form& L, const mf_form* M,
+                                   int block, int wpe) {
+  std::ostringstream os;
+  os << kPrelude;
+  std::map<std::string, bool> seen;
+  for (const mf_form* f : {&a, &L, M}) {
+    if (!f) continue;
+    for (auto& I : f->ints) {
+      if (seen.count(I->tab_name)) continue;
+      seen[I->tab_name] = true;
+      std::string src = I->tab_src;
+      size_t p = src.find("void " + I->tab_name);
+      MF_CHECK(p != std::string::npos, MF_EINVAL,
+               "tabulate source does not define void " + I->tab_name);
+      src.insert(p, "static __device__ __forceinline__ ");
+      os << "\n" << src << "\n";
+    }
+  }
+  std::string interlace;
+  for (auto& c : a.coefs) {
+    std::ostringstream il;
+    il << "    for (int k = 0; k < " << c.width << "; ++k) { w[" << 2 * c.offset << " + k] = w0["
+       << c.offset << " + k]; w[" << 2 * c.offset + c.width << " + k] = w1[" << c.offset
+       << " + k]; }\n";
+    interlace += il.str();
+  }
+  std::string cases;
+  for (size_t k = 0; k < a.ints.size(); ++k) {
+    const auto& I = a.ints[k];
+    std::map<std::string, std::string> c{{"$K", std::to_string(I->index)},
+                                         {"$TABJ", I->tab_name},
+                                         {"$TABF", L.ints[k]->tab_name}};
+    const std::string tabE = M ? M->ints[k]->tab_name : "";
+    if (I->type == MF_CELL || I->type == MF_EXTERIOR_FACET) {
+      const bool cell = I->type == MF_CELL;
+      c["$E"] = M ? std::string("          { double e[1] = {0.0}; ") + tabE +
+                        (cell ? "(e, ws, cE, xs, nullptr, nullptr);" : "(e, ws, cE, xs, eli, qp);") +
+                        " esum += e[0]; }\n"
+                  : "";
+      cases += subst(cell ? kDGSysCell : kDGSysExt, c);
+    } else {
+      std::string body = "        case " + std::to_string(I->index) + ": {\n";
+      for (int side : {0, 1}) {
+        std::string e = (M && side == 0)
+                            ? "          { double e[1] = {0.0}; " + tabE +
+                                  "(e, w, cE, xc, eli, qp); esum += e[0]; }\n"
+                            : "";
+        std::string t = subst(kDGCaseIntSide, {{"$BODY", kDGSysIntBody}, {"$INTERLACE", interlace}});
+        body += subst(t, {{"$SIDE", std::to_string(side)}, {"$TABJ", I->tab_name},
+                          {"$TABF", L.ints[k]->tab_name}, {"$E", e}});
+      }
+      cases += body + "        } break;\n";
+    }
+  }
+  std::map<std::string, std::string> m;
+  m["$NV"] = std::to_string(a.mesh->nv);
+  m["$NF"] = std::to_string(std::max(1, a.mesh->nf));
+  m["$W"] = std::to_string(a.W);
+  m["$ND0"] = std::to_string(a.dm[0]->nd);
+  m["$BS0"] = std::to_string(a.dm[0]->bs);
+  m["$DBLOCK"] = std::to_string(block);
+  m["$DATTR"] = wpe > 0 ? "__attribute__((amdgpu_waves_per_eu(" + std::to_string(wpe) + ")))" : "";
+  os << subst(subst(kDGSystem, {{"$CASES", cases}}), m);
+  return os.str();
 }
 
 std::string generate_form_source(const mf_form& f) {

@@ -230,4 +230,205 @@ const char* kDGIntVecBody = R"(
           for (int i = 0; i < NDB; ++i) S[i] += A[$SIDE * NDB + i];
 )";
 
+// ---------------------------------------------------------------------------
+// Newton system: a Jacobian (rank 2), its residual (rank 1) and optionally a
+// functional (rank 0) with the same integrals, entities and coefficients,
+// assembled in ONE persistent launch.  Each thread walks cells
+// (c = global thread id, stride = grid size), gathers a cell's geometry and
+// coefficients once and, per item, calls the three element bodies on them:
+// the matrix part is the cell-owner scheme above, the vector entries of the
+// cell are summed in registers and stored once, and the functional adds the
+// cell integral, its exterior facets and each interior facet once (from the
+// facet's first cell).  The functional's per-thread sums are reduced per
+// workgroup (wave64 shuffles + LDS) into partial[blockIdx]; the last
+// workgroup to finish (device-scope counter) adds the partials in block
+// order, writes the value and resets the counter, so the launch needs no
+// second pass and no host-side clearing (graph-capturable), and the value is
+// the same on every run with the same grid.
+// Reference semantics: each form's kernels as above (CUDAFormIntegral.cpp:
+// matrix :1103-1226, 1667-1793, 1988-2103; vector :270-522; scalar :68-266),
+// which the reference runs as separate launches per form and integral.
+const char* kDGSystem = R"(
+extern "C" __global__ void __launch_bounds__($DBLOCK) $DATTR
+mf_dgsys(const int32_t ncells, const int32_t* __restrict__ cell_ptr, const int4* __restrict__ items,
+         const int32_t* __restrict__ geo, const double* __restrict__ xg,
+         const double* __restrict__ coeffs, const double* __restrict__ cJ,
+         const double* __restrict__ cF, const double* __restrict__ cE,
+         const int32_t* __restrict__ row_ptr, const int8_t* __restrict__ bc0,
+         const int8_t* __restrict__ bc1, double* __restrict__ values, const double diag,
+         const int flags, const uint8_t* __restrict__ fperm, double* __restrict__ b,
+         double* __restrict__ partial, unsigned int* __restrict__ counter,
+         double* __restrict__ evalue)
+{
+  constexpr int NDB = $ND0 * $BS0, N2 = 2 * NDB;
+  const bool zero = flags & 1;
+  double esum = 0.0;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x) {
+    const int i0 = cell_ptr[c], i1 = cell_ptr[c + 1];
+    double* bout = b + (int64_t)c * NDB;
+    if (i0 == i1) {
+      if (zero)
+        for (int i = 0; i < NDB; ++i) bout[i] = 0.0;
+      continue;
+    }
+    double xs[$NV * 3];
+    for (int v = 0; v < $NV; ++v) {
+      const int64_t gv = geo[(int64_t)c * $NV + v];
+      xs[3 * v] = xg[3 * gv]; xs[3 * v + 1] = xg[3 * gv + 1]; xs[3 * v + 2] = xg[3 * gv + 2];
+    }
+    const double* ws = coeffs + (int64_t)c * $W;
+    const int64_t r0 = (int64_t)c * NDB;
+    const int64_t rp0 = row_ptr[r0];
+    const int64_t L = row_ptr[r0 + 1] - rp0;
+    double S[NDB * NDB], SV[NDB];
+#pragma unroll
+    for (int k = 0; k < NDB * NDB; ++k) S[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NDB; ++k) SV[k] = 0.0;
+    int pself = 0;
+    auto put_block = [&](const double* blk, int64_t co, int pos, int stride) {
+      for (int i = 0; i < NDB; ++i) {
+        const int64_t row = r0 + i;
+        const bool rmask = bc0 && bc0[row];
+        double* out = values + rp0 + i * L + (int64_t)pos * NDB;
+        for (int j = 0; j < NDB; ++j) {
+          const bool m = rmask || (bc1 && bc1[co * NDB + j]);
+          const double v = m ? 0.0 : blk[i * stride + j];
+          if (zero) out[j] = v;
+          else if (!m) out[j] += v;
+        }
+      }
+    };
+    for (int t = i0; t < i1; ++t) {
+      const int4 it = items[t];
+      pself = (it.w >> 4) & 0xFFF;
+      switch (it.x) {
+$CASES
+        default: break;
+      }
+    }
+    for (int i = 0; i < NDB; ++i) {
+      const int64_t row = r0 + i;
+      const bool rmask = bc0 && bc0[row];
+      double* out = values + rp0 + i * L + (int64_t)pself * NDB;
+      for (int j = 0; j < NDB; ++j) {
+        const bool m = rmask || (bc1 && bc1[r0 + j]);
+        double v = m ? 0.0 : S[i * NDB + j];
+        if ((flags & 2) && rmask && j == i) {
+          out[j] = diag;
+          continue;
+        }
+        if (zero) out[j] = v;
+        else if (!m) out[j] += v;
+      }
+    }
+    if (zero) {
+      for (int i = 0; i < NDB; ++i) bout[i] = SV[i];
+    } else {
+      for (int i = 0; i < NDB; ++i) bout[i] += SV[i];
+    }
+  }
+  if (evalue) {
+    for (int o = 32; o > 0; o >>= 1) esum += __shfl_down(esum, o, 64);
+    __shared__ double red[$DBLOCK / 64];
+    __shared__ double fin[$DBLOCK / 64];
+    __shared__ int last;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = esum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int k = 0; k < $DBLOCK / 64; ++k) t += red[k];
+      partial[blockIdx.x] = t;
+      __threadfence();                                  // partial visible device-wide
+      last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last) {
+      __threadfence();                                  // see every workgroup's partial
+      double acc = 0.0;
+      for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x)
+        acc += __builtin_nontemporal_load(&partial[k]);
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+      if ((threadIdx.x & 63) == 0) fin[threadIdx.x >> 6] = acc;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < $DBLOCK / 64; ++k) t += fin[k];
+        evalue[0] = (flags & 4) ? evalue[0] + t : t;
+        counter[0] = 0u;                                // ready for the next launch
+      }
+    }
+  }
+}
+)";
+
+// Per-integral cases of mf_dgsys: $K index, $TABJ / $TABF / $TABE the three
+// forms' element bodies, $E the functional statement (or nothing).
+const char* kDGSysCell = R"(
+        case $K: {
+          {
+            double A[NDB * NDB];
+#pragma unroll
+            for (int k = 0; k < NDB * NDB; ++k) A[k] = 0.0;
+            $TABJ(A, ws, cJ, xs, nullptr, nullptr);
+#pragma unroll
+            for (int k = 0; k < NDB * NDB; ++k) S[k] += A[k];
+          }
+          {
+            double V[NDB];
+#pragma unroll
+            for (int k = 0; k < NDB; ++k) V[k] = 0.0;
+            $TABF(V, ws, cF, xs, nullptr, nullptr);
+#pragma unroll
+            for (int k = 0; k < NDB; ++k) SV[k] += V[k];
+          }
+$E        } break;
+)";
+
+const char* kDGSysExt = R"(
+        case $K: {
+          const int eli[1] = {it.z & 0xFF};
+          const uint8_t qp[1] = {fperm ? fperm[(int64_t)c * $NF + eli[0]] : (uint8_t)0};
+          {
+            double A[NDB * NDB];
+#pragma unroll
+            for (int k = 0; k < NDB * NDB; ++k) A[k] = 0.0;
+            $TABJ(A, ws, cJ, xs, eli, qp);
+#pragma unroll
+            for (int k = 0; k < NDB * NDB; ++k) S[k] += A[k];
+          }
+          {
+            double V[NDB];
+#pragma unroll
+            for (int k = 0; k < NDB; ++k) V[k] = 0.0;
+            $TABF(V, ws, cF, xs, eli, qp);
+#pragma unroll
+            for (int k = 0; k < NDB; ++k) SV[k] += V[k];
+          }
+$E        } break;
+)";
+
+// interior facet, one side ($SIDE), inside kDGCaseIntSide
+const char* kDGSysIntBody = R"(
+          {
+            double A[N2 * N2];
+#pragma unroll
+            for (int k = 0; k < N2 * N2; ++k) A[k] = 0.0;
+            $TABJ(A, w, cJ, xc, eli, qp);
+#pragma unroll
+            for (int i = 0; i < NDB; ++i)
+#pragma unroll
+              for (int j = 0; j < NDB; ++j) S[i * NDB + j] += A[($SIDE * NDB + i) * N2 + $SIDE * NDB + j];
+            put_block(&A[$SIDE * NDB * N2 + (1 - $SIDE) * NDB], co, (it.w >> 16) & 0xFFF, N2);
+          }
+          {
+            double V[N2];
+#pragma unroll
+            for (int k = 0; k < N2; ++k) V[k] = 0.0;
+            $TABF(V, w, cF, xc, eli, qp);
+#pragma unroll
+            for (int i = 0; i < NDB; ++i) SV[i] += V[$SIDE * NDB + i];
+          }
+$E)";
+
 }  // namespace mf

@@ -274,8 +274,6 @@ DevArray<int32_t>& get_table(mf_form* a, Integral& I, mf_matrix* A, hipStream_t 
 // Cell-owner schedule for discontinuous spaces (kernel: dg_kernel.cpp)
 // ---------------------------------------------------------------------------
 
-namespace {
-
 // The test space must number dofs cell by cell (node(c, i) = c * nd + i) so a
 // cell's CSR rows are its own.  With a matrix, each item also carries the
 // block positions of the owner and the neighbour inside the owner's rows.
@@ -357,8 +355,6 @@ std::shared_ptr<DGPlan> build_dg_plan(const mf_form& f, const mf_matrix* A) {
   plan->nitems = nitems;
   return plan;
 }
-
-}  // namespace
 
 namespace {
 

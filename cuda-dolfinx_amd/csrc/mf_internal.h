@@ -264,6 +264,20 @@ struct mf_interpolator {
   mf::DevArray<int32_t> own;          // per target node: c * n1 + i, or -1
 };
 
+struct mf_system {
+  mf_context* ctx = nullptr;
+  mf_form* a = nullptr;               // Jacobian (rank 2)
+  mf_form* L = nullptr;               // residual (rank 1)
+  mf_form* M = nullptr;               // functional (rank 0) or null
+  std::unique_ptr<mf::Module> mod;
+  hipFunction_t k = nullptr;
+  std::string source;
+  int block = 64;
+  int64_t grid = 0;                   // persistent grid (resident workgroups)
+  mf::DevArray<double> partials;      // one per workgroup
+  mf::DevArray<uint32_t> counter;     // finished workgroups (reset by the last)
+};
+
 struct mf_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -309,6 +323,12 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
                      double diagonal, int zero, hipStream_t s);
 void assemble_vector(mf_form* L, double* b, int64_t n, int zero, hipStream_t s);
 void assemble_scalar(mf_form* M, double* d_value, int zero, hipStream_t s);
+// cell-owner schedule of a form on a cell-major discontinuous space (null if
+// the form/matrix does not qualify); with A, block positions in A's rows
+std::shared_ptr<DGPlan> build_dg_plan(const mf_form& f, const mf_matrix* A);
+// fused Newton-system kernel source (Jacobian a, residual L, optional functional M)
+std::string generate_system_source(const mf_form& a, const mf_form& L, const mf_form* M,
+                                   int block, int wpe);
 // scalar-level CSR of a block of forms (row-major nr x nc, null: empty block)
 void build_block_pattern(const std::vector<mf_form*>& forms, int nr, int nc, int64_t nrows,
                          int64_t ncols, std::vector<int32_t>& row_ptr, std::vector<int32_t>& cols);

@@ -12,6 +12,7 @@ from .context import CUDAContext, DeviceArray, get_cuda_context
 from .form import BlockCUDAForm, CUDAForm, form
 from .graph import CUDAGraph
 from .la import CUDAMatrix, CUDAVector
+from .system import CUDASystem
 
 __all__ = ["CUDAAssembler", "CUDADirichletBC", "CUDAContext", "DeviceArray", "get_cuda_context",
-           "CUDAForm", "BlockCUDAForm", "form", "CUDAMatrix", "CUDAVector", "CUDAGraph", "Coefficient"]
+           "CUDAForm", "BlockCUDAForm", "form", "CUDAMatrix", "CUDAVector", "CUDAGraph", "Coefficient", "CUDASystem"]

@@ -38,6 +38,7 @@ typedef struct mf_matrix mf_matrix;
 typedef struct mf_bc mf_bc;
 typedef struct mf_graph mf_graph;
 typedef struct mf_interpolator mf_interpolator;
+typedef struct mf_system mf_system;
 typedef void* mf_stream;
 
 enum mf_status {
@@ -204,6 +205,31 @@ int mf_assemble_scalar(mf_form* M, double* value, mf_stream stream);
  * value, on `stream`, with no host synchronization (usable under graph
  * capture).  d_value is device memory. */
 int mf_assemble_scalar_device(mf_form* M, double* d_value, int zero, mf_stream stream);
+/* ---------------------------------------------------------------- fused Newton system */
+/* No counterpart in the reference, which launches every form and integral
+ * type separately.  For callers that reassemble a Jacobian a, its residual L
+ * and optionally a functional M (energy / merit function) every Newton or
+ * time step on a discontinuous space whose dofs are numbered cell by cell:
+ * the three forms must have the same integrals over the same entities and
+ * the same coefficient layout (the state u); the coefficients are read from
+ * a's packed array.  mf_system_create generates and compiles one persistent
+ * kernel over the cell, exterior-facet and interior-facet integrals of all
+ * three forms; block = threads per workgroup (0: 64). */
+int mf_system_create(mf_form* a, mf_form* L, mf_form* M, int block, mf_system** out);
+/* One launch on `stream`: A (from mf_matrix_create(a)) with rows in bc0 and
+ * columns in bc1 skipped and the Dirichlet diagonal set, b (length n) and,
+ * if the system has a functional, d_value[0] (device memory).  flags: 1 =
+ * overwrite A and b (else add), 2 = add the functional to d_value[0] (else
+ * overwrite).  No allocation after the first call on a matrix, no host
+ * synchronization (graph-capturable); the functional is summed in a fixed
+ * order, so it repeats exactly. */
+int mf_system_assemble(mf_system* sys, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc1,
+                       double diagonal, double* d_b, int64_t n, double* d_value, int flags,
+                       mf_stream stream);
+/* Generated HIP source of the system kernel; len includes the NUL. */
+int mf_system_source(mf_system* sys, char* buf, int64_t* len);
+int mf_system_destroy(mf_system* sys);
+
 /* ---------------------------------------------------------------- interpolation */
 /* Same-map interpolation between two spaces on one mesh: the device half of
  * CUDACoefficient::interpolate (cpp/cudolfinx/fem/CUDACoefficient.h:50-66,

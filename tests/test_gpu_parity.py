@@ -610,3 +610,97 @@ def test_edge_cases_one_cube_and_empty_integrals():
     asm.synchronize()
     rp, cols, vals = A.csr()
     assert A.nnz == 0 and cols.size == 0 and np.all(rp == 0)
+
+
+# ------------------------------------------------------------ fused Newton system (J + F + E)
+@pytest.mark.parametrize("k", [1, 2])
+def test_fused_system_matches_separate_forms_and_oracle(k):
+    _say(f"fused J + F + E launch, SIPG DG{k} n=4: vs separate launches, oracle, repeat, accumulate")
+    import torch
+    import cudolfinx as cufem
+    from mifem.dgforms import sipg_energy
+    P = problems.dg_plaplace(4, degree=k)
+    u = P.extras["u"]
+    rng = np.random.default_rng(51)
+    u.x[:] = rng.uniform(-1, 1, P.V.size)
+    u.version += 1
+    E_ = sipg_energy(P.V, u)
+    J, F, E = cufem.form(P.a), cufem.form(P.L), cufem.form(E_)
+    asm = cufem.CUDAAssembler()
+    A1 = asm.assemble_matrix(J)
+    b1 = asm.assemble_vector(F)
+    e1 = asm.assemble_scalar(E)
+    system = cufem.CUDASystem(J, F, E)
+    A2, b2 = asm.create_matrix(J), asm.create_vector(F)
+    ev = torch.zeros(1, dtype=torch.float64, device="cuda")
+    system.assemble(asm, A2, b2, ev)
+    asm.synchronize()
+    v1, v2 = A1.csr()[2], A2.csr()[2]
+    eA, eb = _rel(v2, v1), _rel(b2.to_host(), b1.to_host())
+    e2 = ev.item()
+    _say(f"  fused vs separate: A {eA:.2e}, b {eb:.2e}, E {abs(e2 - e1) / abs(e1):.2e}")
+    assert eA <= 1e-14 and eb <= 1e-14 and abs(e2 - e1) <= 1e-13 * abs(e1)
+    oa = cpu.OracleAssembler(P.a)
+    rp, cols, _, _ = oa.sparsity()
+    assert _rel(v2, oa.assemble_matrix(rp, cols)) <= TOL
+    assert _rel(b2.to_host(), cpu.OracleAssembler(P.L).assemble_vector()) <= TOL
+    ref = cpu.OracleAssembler(E_).assemble_scalar()
+    assert abs(e2 - ref) <= 1e-12 * abs(ref)
+    # repeat: identical; accumulate: doubled A, b and E
+    system.assemble(asm, A2, b2, ev)
+    asm.synchronize()
+    assert ev.item() == e2 and np.array_equal(A2.csr()[2], v2)
+    system.assemble(asm, A2, b2, ev, zero=False, accumulate_value=True)
+    asm.synchronize()
+    assert _rel(A2.csr()[2], 2 * v2) <= 1e-15 and abs(ev.item() - 2 * e2) <= 1e-14 * abs(e2)
+
+
+def test_fused_system_dirichlet_and_graph_replay():
+    _say("fused J + F + E with Dirichlet rows (diagonal 2.5) and HIP graph replays, DG1 n=3")
+    import torch
+    import cudolfinx as cufem
+    from mifem.dgforms import sipg_energy
+    P = problems.dg_plaplace(3)
+    n = P.V.size
+    rng = np.random.default_rng(61)
+    dofs = np.arange(0, n, 5, dtype=np.int32)
+    bc = dirichletbc(0.0, dofs, P.V)
+    s = torch.cuda.Stream()
+    E_ = sipg_energy(P.V, P.extras["u"])
+    J, F, E = cufem.form(P.a), cufem.form(P.L), cufem.form(E_)
+    asm = cufem.CUDAAssembler(stream=s)
+    ud = torch.zeros(n, dtype=torch.float64, device="cuda")
+    for f in (J, F, E):
+        f.set_coefficient_device(0, ud.data_ptr(), n)
+    system = cufem.CUDASystem(J, F, E)
+    A, b = asm.create_matrix(J), asm.create_vector(F)
+    dbc = asm.pack_bcs([bc])
+    ev = torch.zeros(1, dtype=torch.float64, device="cuda")
+
+    def step():
+        system.assemble(asm, A, b, ev, bcs=dbc, diagonal=2.5)
+
+    step()
+    asm.synchronize()
+    g = cufem.CUDAGraph(s)
+    with g:
+        step()
+    oa = cpu.OracleAssembler(P.a)
+    rp, cols, _, _ = oa.sparsity()
+    for rep in range(3):
+        u = rng.uniform(-1, 1, n)
+        ud.copy_(torch.from_numpy(u).cuda())
+        torch.cuda.synchronize()
+        g.launch()
+        asm.synchronize()
+        vg, bg, eg = A.csr()[2].copy(), b.to_host().copy(), ev.item()
+        step()
+        asm.synchronize()
+        assert np.array_equal(vg, A.csr()[2]) and np.array_equal(bg, b.to_host()) and eg == ev.item()
+        P.extras["u"].x[:] = u
+        oa = cpu.OracleAssembler(P.a)
+        ea = _rel(vg, oa.assemble_matrix(rp, cols, [bc], diagonal=2.5))
+        eb = _rel(bg, cpu.OracleAssembler(P.L).assemble_vector())
+        ref = cpu.OracleAssembler(E_).assemble_scalar()
+        _say(f"  replay {rep}: A {ea:.2e} b {eb:.2e} E {abs(eg - ref) / abs(ref):.2e}")
+        assert ea <= TOL and eb <= TOL and abs(eg - ref) <= 1e-12 * abs(ref)

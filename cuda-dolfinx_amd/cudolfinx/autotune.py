@@ -1,4 +1,5 @@
-"""Autotuner for the cooperative owner-computes matrix kernel.
+"""Autotuners: the cooperative owner-computes matrix kernel and the fused
+Newton-system kernel.
 
 Picks the kernel shape -- LDS value budget per workgroup ("chunk_values",
 i.e. how many CSR rows a workgroup owns), pairs per chunk ("owner_pairs"),
@@ -86,3 +87,36 @@ def autotune_matrix(form, asm, A, bcs, candidates=None, reps: int = 5, verbose: 
     except OSError:
         pass
     return best[1], best[2], timings
+
+
+SYSTEM_BLOCKS = (64, 128, 256)
+
+
+def autotune_system(a, L, M, asm, A, b, value, blocks=SYSTEM_BLOCKS, reps: int = 10,
+                    stream=None, verbose: bool = False):
+    """Cells per workgroup of the fused Newton-system kernel: times one
+    `CUDASystem` per candidate block size on the actual forms and matrix (HIP
+    events on `stream`, the torch stream the assembler runs on; default the
+    current stream) and returns (best system, best block, [(block, ms)])."""
+    import torch
+    from .system import CUDASystem
+
+    timings, best = [], None
+    for blk in blocks:
+        system = CUDASystem(a, L, M, block=blk)
+        system.assemble(asm, A, b, value)
+        asm.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s = stream if stream is not None else torch.cuda.current_stream()
+        e0.record(s)
+        for _ in range(reps):
+            system.assemble(asm, A, b, value, pack=False)
+        e1.record(s)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        timings.append((blk, ms))
+        if verbose:
+            print(f"[autotune] system block {blk}: {ms:.3f} ms", file=sys.stderr, flush=True)
+        if best is None or ms < best[1]:
+            best = (system, ms, blk)
+    return best[0], best[2], timings

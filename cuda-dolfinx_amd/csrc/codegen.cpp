@@ -336,6 +336,26 @@ bool dg_owner_emitted(const mf_form& f) {
   return (f.rank == 1 || f.rank == 2) && ndb <= 10;
 }
 
+// The integral's UFCx source with every tabulate_tensor_integral_* definition
+// in it (the named body and any body it calls, e.g. a diagonal-part wrapper
+// around a matrix body) made a device function.
+std::string device_tabulate_source(const Integral& I) {
+  std::string src = I.tab_src;
+  MF_CHECK(src.find("void " + I.tab_name) != std::string::npos, MF_EINVAL,
+           "tabulate source does not define void " + I.tab_name);
+  const std::string key = "void tabulate_tensor_integral_";
+  const std::string qual = "static __device__ __forceinline__ ";
+  size_t p = 0;
+  while ((p = src.find(key, p)) != std::string::npos) {
+    if (p == 0 || src[p - 1] == '\n') {
+      src.insert(p, qual);
+      p += qual.size();
+    }
+    p += key.size();
+  }
+  return src;
+}
+
 std::string generate_system_source(const mf_form& a, const mf_form& L, const mf_form* M,
                                    int block, int wpe) {
   std::ostringstream os;
@@ -346,12 +366,7 @@ std::string generate_system_source(const mf_form& a, const mf_form& L, const mf_
     for (auto& I : f->ints) {
       if (seen.count(I->tab_name)) continue;
       seen[I->tab_name] = true;
-      std::string src = I->tab_src;
-      size_t p = src.find("void " + I->tab_name);
-      MF_CHECK(p != std::string::npos, MF_EINVAL,
-               "tabulate source does not define void " + I->tab_name);
-      src.insert(p, "static __device__ __forceinline__ ");
-      os << "\n" << src << "\n";
+      os << "\n" << device_tabulate_source(*I) << "\n";
     }
   }
   std::string interlace;
@@ -409,12 +424,7 @@ std::string generate_form_source(const mf_form& f) {
   for (auto& I : f.ints) {
     if (seen.count(I->tab_name)) continue;
     seen[I->tab_name] = true;
-    std::string src = I->tab_src;
-    size_t p = src.find("void " + I->tab_name);
-    MF_CHECK(p != std::string::npos, MF_EINVAL,
-             "tabulate source does not define void " + I->tab_name);
-    src.insert(p, "static __device__ __forceinline__ ");
-    os << "\n" << src << "\n";
+    os << "\n" << device_tabulate_source(*I) << "\n";
   }
   const mf_dofmap* d0 = f.dm[0];
   const mf_dofmap* d1 = f.rank == 2 ? f.dm[1] : f.dm[0];

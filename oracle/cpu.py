@@ -49,7 +49,8 @@ class Kernel(C.Structure):
                 ("k2", C.c_double),
                 ("p", C.c_double), ("eps", C.c_double), ("alpha", C.c_double),
                 ("penalty", C.c_double), ("fval", C.c_double),
-                ("u_off", C.c_int), ("coeff_width", C.c_int), ("mu0", C.c_double)]
+                ("u_off", C.c_int), ("coeff_width", C.c_int), ("mu0", C.c_double),
+                ("diag_part", C.c_int), ("kappa_degree", C.c_int)]
 
 
 class Entities(C.Structure):
@@ -140,7 +141,7 @@ def basis(cell: int, k: int, X):
 # ------------------------------------------------------------- kernels
 _KINDS = {"stiffness": 0, "elasticity": 1, "source": 2, "facet_source": 3,
           "dg_cell": 4, "dg_ext": 5, "dg_int": 6, "fn_cell": 7, "fn_ext": 8, "fn_int": 9,
-          "mass": 10}
+          "mass": 10, "nl_cell": 11, "nl_ext": 12, "nl_int": 13}
 _TYPES = {"cell": 0, "exterior_facet": 1, "interior_facet": 2}
 
 
@@ -184,6 +185,8 @@ def kernel_from_spec(spec: dict, coeff_width: int = 0) -> Kernel:
         k.f_off, k.f_degree = int(spec["v_off"]), int(spec["v_degree"])
     if "trial_degree" in spec:
         k.f_degree = int(spec["trial_degree"])
+    k.diag_part = int(spec.get("diag_part", 0))
+    k.kappa_degree = int(spec.get("kappa_degree", spec["degree"]))
     k.coeff_width = coeff_width
     return k
 
@@ -192,7 +195,7 @@ def tabulate(spec: dict, w, c, coordinate_dofs, entity_local_index=None):
     """Element tensor of one entity, as a flat array (rank 2: nr x nc)."""
     k = kernel_from_spec(spec, len(w))
     nd = lib().or_ndofs(spec["cell"], spec["degree"]) * spec["bs"]
-    mult = 2 if spec["kind"] == "dg_int" else 1
+    mult = 2 if spec["kind"] in ("dg_int", "nl_int") else 1
     n = nd * mult
     m = (lib().or_ndofs(spec["cell"], spec["trial_degree"]) * spec["bs"] * mult
          if "trial_degree" in spec else n)

@@ -14,7 +14,9 @@
  *     e3=(0,3) e4=(0,2) e5=(0,1), faces opposite vertex i; hexahedra
  *     lexicographic);
  *   - the element integrals the engine generates (stiffness / Helmholtz,
- *     mass between two Lagrange spaces, linear elasticity, volume and facet sources, SIPG + p-Laplacian DG
+ *     mass between two Lagrange spaces, linear elasticity, the nonlinear
+ *     residuals / Jacobians of the reference's assembly tests and the
+ *     diagonal part of a bilinear form, volume and facet sources, SIPG + p-Laplacian DG
  *     with its energy functional, a functional over cells, exterior and
  *     interior facets),
  *     evaluated directly at quadrature points, in the UFCx argument layout;
@@ -320,7 +322,10 @@ enum {
   OR_FN_CELL = 7,     /* functional: exp(u) kappa dx              */
   OR_FN_EXT = 8,      /* functional: u kappa ds                   */
   OR_FN_INT = 9,      /* functional: avg(v^2) avg(kappa) dS       */
-  OR_MASS = 10        /* kappa u v, P_degree test x P_f_degree trial */
+  OR_MASS = 10,       /* kappa u v, P_degree test x P_f_degree trial */
+  OR_NL_CELL = 11,    /* (exp(u) kappa p + grad u . grad p), its Jacobian */
+  OR_NL_EXT = 12,     /* u kappa p (grad u . n), its Jacobian         */
+  OR_NL_INT = 13      /* avg(q) avg(kappa) avg(v^2), its Jacobian     */
 };
 
 typedef struct {
@@ -346,6 +351,8 @@ typedef struct {
   int u_off;       /* offset of the DG1 state u in w (per cell) */
   int coeff_width; /* per-cell packed width (for interior-facet interlacing) */
   double mu0;      /* constant part of the p-Laplacian coefficient */
+  int diag_part;   /* rank 1 from a bilinear kind: the element matrix's diagonal */
+  int kappa_degree;/* degree of kappa where it differs from the test space's */
 } or_kernel;
 
 static double param(int src, int off, double val, const double* w, const double* c)
@@ -671,10 +678,113 @@ static void tab_mass(const or_kernel* K, double* A, const double* w, const doubl
   }
 }
 
+/* ------------------------------------------------------------------ */
+/* the nonlinear forms of python/test/test_cuda_assembly.py:66-81       */
+/* (u at u_off, kappa at kappa_off, both P_degree; v in DG_f_degree)     */
+/* ------------------------------------------------------------------ */
+
+static void tab_nl_cell(const or_kernel* K, double* A, const double* w, const double* xc)
+{
+  static double pts[3 * OR_MAXQ], wts[OR_MAXQ];
+  const int nd = or_ndofs(0, K->degree);
+  const int nq = or_tet_rule(K->qm, pts, wts);
+  double phi[OR_MAXND], g[OR_MAXND][3], adet;
+  for (int q = 0; q < nq; ++q) {
+    phys_grad(K, xc, &pts[3 * q], phi, g, &adet);
+    double uq = 0.0, kq = 0.0, gu[3] = {0, 0, 0};
+    for (int i = 0; i < nd; ++i) {
+      uq += w[K->u_off + i] * phi[i];
+      kq += w[K->kappa_off + i] * phi[i];
+      for (int d = 0; d < 3; ++d) gu[d] += w[K->u_off + i] * g[i][d];
+    }
+    const double dx = wts[q] * adet, ek = exp(uq) * kq;
+    for (int i = 0; i < nd; ++i) {
+      if (K->rank == 1) {
+        A[i] += dx * (ek * phi[i] + gu[0] * g[i][0] + gu[1] * g[i][1] + gu[2] * g[i][2]);
+        continue;
+      }
+      for (int j = 0; j < nd; ++j)
+        A[i * nd + j] += dx * (ek * phi[j] * phi[i] + g[j][0] * g[i][0] + g[j][1] * g[i][1] +
+                               g[j][2] * g[i][2]);
+    }
+  }
+}
+
+static void tab_nl_ext(const or_kernel* K, double* A, const double* w, const double* xc, int lf)
+{
+  double pts[2 * OR_MAXND * OR_MAXND], wts[OR_MAXND * OR_MAXND], n[3];
+  const int nd = or_ndofs(0, K->degree);
+  const int nq = or_tri_rule(K->qm, pts, wts);
+  const double ds = facet_normal(xc, lf, n);
+  for (int q = 0; q < nq; ++q) {
+    double X[3], phi[OR_MAXND], g[OR_MAXND][3];
+    facet_point(lf, pts[2 * q], pts[2 * q + 1], X);
+    phys_grad(K, xc, X, phi, g, NULL);
+    double uq = 0.0, kq = 0.0, gun = 0.0;
+    for (int i = 0; i < nd; ++i) {
+      uq += w[K->u_off + i] * phi[i];
+      kq += w[K->kappa_off + i] * phi[i];
+      gun += w[K->u_off + i] * (g[i][0] * n[0] + g[i][1] * n[1] + g[i][2] * n[2]);
+    }
+    const double fw = wts[q] * ds;
+    for (int i = 0; i < nd; ++i) {
+      if (K->rank == 1) { A[i] += fw * uq * kq * phi[i] * gun; continue; }
+      for (int j = 0; j < nd; ++j) {
+        const double gjn = g[j][0] * n[0] + g[j][1] * n[1] + g[j][2] * n[2];
+        A[i * nd + j] += fw * kq * phi[i] * (phi[j] * gun + uq * gjn);
+      }
+    }
+  }
+}
+
+/* interlaced w: kappa (P_kappa_degree) at 2 kappa_off, v (DG_degree) at 2 f_off */
+static void tab_nl_int(const or_kernel* K, double* A, const double* w, const double* xc,
+                       const int* eli)
+{
+  double pts[2 * OR_MAXND * OR_MAXND], wts[OR_MAXND * OR_MAXND], n[3];
+  const int nd = or_ndofs(0, K->degree), n2 = 2 * nd;
+  const int nc = or_ndofs(0, K->kappa_degree);
+  const int nq = or_tri_rule(K->qm, pts, wts);
+  const double ds = facet_normal(xc, eli[0], n);
+  const double* k0 = w + 2 * K->kappa_off;
+  const double* v0 = w + 2 * K->f_off;
+  for (int q = 0; q < nq; ++q) {
+    double X0[3], X1[3], pa[OR_MAXND], pb[OR_MAXND], d[3 * OR_MAXND];
+    facet_point(eli[0], pts[2 * q], pts[2 * q + 1], X0);
+    facet_point(eli[1], pts[2 * q], pts[2 * q + 1], X1);
+    const double kav = 0.5 * (eval_at(K->kappa_degree, X0, k0) + eval_at(K->kappa_degree, X1, k0 + nc));
+    const double ua = eval_at(K->degree, X0, v0), ub = eval_at(K->degree, X1, v0 + nd);
+    tet_basis(K->degree, X0, pa, d);
+    tet_basis(K->degree, X1, pb, d);
+    const double fw = wts[q] * ds;
+    for (int I = 0; I < n2; ++I) {
+      const double pI = 0.5 * (I < nd ? pa[I] : pb[I - nd]);
+      if (K->rank == 1) { A[I] += fw * pI * kav * 0.5 * (ua * ua + ub * ub); continue; }
+      for (int J = 0; J < n2; ++J)
+        A[I * n2 + J] += fw * pI * kav * (J < nd ? ua * pa[J] : ub * pb[J - nd]);
+    }
+  }
+}
+
+void or_tabulate(const or_kernel* K, double* A, const double* w, const double* c,
+                 const double* coordinate_dofs, const int* entity_local_index);
+
 /* UFCx-shaped entry point: A += element tensor. */
 void or_tabulate(const or_kernel* K, double* A, const double* w, const double* c,
                  const double* coordinate_dofs, const int* entity_local_index)
 {
+  if (K->diag_part) {
+    /* the diagonal of the kind's element matrix */
+    static double T[4 * OR_MAXND * OR_MAXND];
+    or_kernel M = *K;
+    M.diag_part = 0;
+    M.rank = 2;
+    const int n = or_ndofs(K->cell, K->degree) * K->bs * (K->kind == OR_NL_INT || K->kind == OR_DG_INT ? 2 : 1);
+    memset(T, 0, sizeof(double) * n * n);
+    or_tabulate(&M, T, w, c, coordinate_dofs, entity_local_index);
+    for (int i = 0; i < n; ++i) A[i] += T[i * n + i];
+    return;
+  }
   switch (K->kind) {
   case OR_STIFFNESS:
   case OR_ELASTICITY:
@@ -685,6 +795,9 @@ void or_tabulate(const or_kernel* K, double* A, const double* w, const double* c
   case OR_DG_INT: tab_dg_int(K, A, w, coordinate_dofs, entity_local_index); break;
   case OR_FN_CELL: tab_fn_cell(K, A, w, coordinate_dofs); break;
   case OR_MASS: tab_mass(K, A, w, c, coordinate_dofs); break;
+  case OR_NL_CELL: tab_nl_cell(K, A, w, coordinate_dofs); break;
+  case OR_NL_EXT: tab_nl_ext(K, A, w, coordinate_dofs, entity_local_index[0]); break;
+  case OR_NL_INT: tab_nl_int(K, A, w, coordinate_dofs, entity_local_index); break;
   case OR_FN_EXT: tab_fn_ext(K, A, w, coordinate_dofs, entity_local_index[0]); break;
   case OR_FN_INT: tab_fn_int(K, A, w, coordinate_dofs, entity_local_index); break;
   default: break;

@@ -155,6 +155,17 @@ def test_generated_mixed_functional(tmp_path, k, kv):
     _check_form(mixed_functional(u, kap, v), tmp_path, ncells=4)
 
 
+@pytest.mark.parametrize("k", [1, 2])
+def test_generated_reference_test_forms(tmp_path, k):
+    """The reference test suite's residuals, Jacobians, functional and
+    diagonal parts (test_cuda_assembly.py:48-91, 200-209) on tetrahedra."""
+    from mifem.nonlinear import diagonal, reference_test_forms
+    mesh = create_unit_cube(2, jitter=0.2, seed=7)
+    F = reference_test_forms(mesh, k)
+    for form in F["vector"] + F["matrix"] + F["scalar"] + [diagonal(a) for a in F["matrix"]]:
+        _check_form(form, tmp_path, ncells=3)
+
+
 def test_staged_description_matches_oracle():
     P = problems.elasticity_p2(2)
     I = P.a.integrals[0]

@@ -704,3 +704,84 @@ def test_fused_system_dirichlet_and_graph_replay():
         ref = cpu.OracleAssembler(E_).assemble_scalar()
         _say(f"  replay {rep}: A {ea:.2e} b {eb:.2e} E {abs(eg - ref) / abs(ref):.2e}")
         assert ea <= TOL and eb <= TOL and abs(eg - ref) <= 1e-12 * abs(ref)
+
+
+# ------------------------------------------------------------ the reference test suite, on tetrahedra
+# (python/test/test_cuda_assembly.py: test_cuda_assembly, test_reassembly, test_lifting,
+# test_diagonal_assembly; its 2D unit square replaced by a jittered unit cube)
+def _ref_suite(k=1):
+    from mifem.nonlinear import reference_test_forms
+    return reference_test_forms(create_unit_cube(4, jitter=0.15, seed=3), k)
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_reference_suite_cuda_assembly(k):
+    _say(f"reference suite: functional, 3 residuals, 3 Jacobians with bcs, P{k}/DG{k}")
+    import cudolfinx as cufem
+    F = _ref_suite(k)
+    asm = cufem.CUDAAssembler()
+    for form in F["scalar"]:
+        v1 = cpu.OracleAssembler(form).assemble_scalar()
+        v2 = asm.assemble_scalar(cufem.form(form))
+        assert abs(v2 - v1) <= 1e-12 * abs(v1), (v1, v2)
+    for form in F["vector"]:
+        ref = cpu.OracleAssembler(form).assemble_vector()
+        got = asm.assemble_vector(cufem.form(form)).to_host()
+        assert _rel(got, ref) <= TOL, form.description
+    for form in F["matrix"]:
+        oa = cpu.OracleAssembler(form)
+        rp, cols, _, _ = oa.sparsity()
+        ref = oa.assemble_matrix(rp, cols, F["bcs"])
+        A = asm.assemble_matrix(cufem.form(form), bcs=F["bcs"])
+        A.assemble()
+        grp, gcols, vals = A.mat.getValuesCSR()
+        assert np.array_equal(grp, rp) and np.array_equal(gcols, cols), form.description
+        err = _rel(vals, ref)
+        _say(f"  {form.description}: nnz {cols.size} relerr {err:.2e}")
+        assert err <= TOL
+
+
+def test_reference_suite_reassembly():
+    _say("reference suite: reassembly after kappa.interpolate(x^d + y^d), d = 2, 3")
+    import cudolfinx as cufem
+    F = _ref_suite()
+    coeff = F["coeff"]
+    cform = cufem.form(F["vector"][0])
+    asm = cufem.CUDAAssembler()
+    vec = asm.assemble_vector(cform)
+    assert _rel(vec.to_host(), cpu.OracleAssembler(F["vector"][0]).assemble_vector()) <= TOL
+    for d in (2, 3):
+        coeff.interpolate(lambda x: x[0] ** d + x[1] ** d)
+        cform.to_device()
+        asm.assemble_vector(cform, vec)
+        assert _rel(vec.to_host(), cpu.OracleAssembler(F["vector"][0]).assemble_vector()) <= TOL
+
+
+def test_reference_suite_lifting():
+    _say("reference suite: set_bc then apply_lifting on the exterior residual / Jacobian")
+    import cudolfinx as cufem
+    F = _ref_suite()
+    L, a = F["vector"][1], F["matrix"][1]
+    asm = cufem.CUDAAssembler()
+    vec = asm.assemble_vector(cufem.form(L))
+    ref = cpu.OracleAssembler(L).assemble_vector()
+    assert _rel(vec.to_host(), ref) <= TOL
+    cpu.set_bc(ref, F["bcs"], L.function_spaces[0])
+    asm.set_bc(vec, F["bcs"], L.function_spaces[0])
+    assert _rel(vec.to_host(), ref) <= TOL
+    cpu.OracleAssembler(a).apply_lifting(ref, F["bcs"])
+    asm.apply_lifting(vec, [cufem.form(a)], [F["bcs"]])
+    assert _rel(vec.to_host(), ref) <= TOL
+
+
+def test_reference_suite_diagonal_assembly():
+    _say("reference suite: diagonal part of each Jacobian assembled as a vector")
+    import cudolfinx as cufem
+    from mifem.nonlinear import diagonal
+    F = _ref_suite()
+    asm = cufem.CUDAAssembler()
+    for a in F["matrix"]:
+        d = diagonal(a)
+        got = asm.assemble_vector(cufem.form(d)).to_host()
+        ref = cpu.OracleAssembler(d).assemble_vector()
+        assert _rel(got, ref) <= TOL, a.description

@@ -610,3 +610,71 @@ def test_mixed_mass_element_sums(kv, kw):
         np.testing.assert_allclose(A.sum(axis=1), vol / 4, rtol=1e-13)
     if kv == 1 and kw == 1:
         np.testing.assert_allclose(A, vol / 20 * (np.ones((4, 4)) + np.eye(4)), rtol=1e-13)
+
+
+# ------------------------------------------------------------ the reference test suite's forms
+def _ref_forms(n=2, k=1, seed=7):
+    from mifem.nonlinear import reference_test_forms
+    mesh = create_unit_cube(n, jitter=0.2, seed=seed)
+    return mesh, reference_test_forms(mesh, k)
+
+
+@pytest.mark.parametrize("which", [0, 1, 2])
+def test_nonlinear_jacobians_are_residual_derivatives(which):
+    """J = dF/du (cell, exterior) and dF/du_dg (interior) by central differences."""
+    import scipy.sparse as sp
+    mesh, F = _ref_forms(2, 1)
+    L, a = F["vector"][which], F["matrix"][which]
+    state = F["u_dg"] if which == 2 else F["u"]
+    rng = np.random.default_rng(9 + which)
+    state.x[:] = rng.uniform(-0.5, 0.5, state.x.size)
+    x0 = state.x.copy()
+    oa = cpu.OracleAssembler(a)
+    rp, cols, _, _ = oa.sparsity()
+    V = a.function_spaces[0]
+    J = sp.csr_matrix((oa.assemble_matrix(rp, cols), cols, rp), shape=(V.size, V.size))
+    d = rng.standard_normal(x0.size)
+    h = 1e-6
+    state.x[:] = x0 + h * d
+    fp = cpu.OracleAssembler(L).assemble_vector()
+    state.x[:] = x0 - h * d
+    fm = cpu.OracleAssembler(L).assemble_vector()
+    state.x[:] = x0
+    fd = (fp - fm) / (2 * h)
+    assert np.linalg.norm(fd - J @ d) <= 1e-7 * max(1.0, np.linalg.norm(J @ d))
+
+
+def test_nonlinear_residual_sums_closed_forms():
+    """Summed over all test functions (partition of unity): the cell residual is
+    int exp(u) kappa dx (u = 0.4, kappa = 1: e^0.4 x volume 1); the exterior
+    residual with u = y, kappa = z is int y z n_y ds = 1/2 (the face y = 1);
+    the interior residual equals the functional's interior term."""
+    mesh, F = _ref_forms(3, 1)
+    u, kap, udg = F["u"], F["coeff"], F["u_dg"]
+    u.x[:] = 0.4
+    kap.x[:] = 1.0
+    assert abs(cpu.OracleAssembler(F["vector"][0]).assemble_vector().sum() - math.exp(0.4)) <= 1e-13
+    u.interpolate(lambda x: x[1])
+    kap.interpolate(lambda x: x[2])
+    assert abs(cpu.OracleAssembler(F["vector"][1]).assemble_vector().sum() - 0.5) <= 1e-13
+    kap.interpolate(lambda x: x[0])
+    udg.interpolate(lambda x: 0.3 + 0.6 * x[0] - 0.2 * x[1] + 0.9 * x[2])
+    got = cpu.OracleAssembler(F["vector"][2]).assemble_vector().sum()
+    ref = cpu.OracleAssembler(_sub(F["scalar"][0], 2)).assemble_scalar()
+    assert abs(got - ref) <= 1e-13 * abs(ref)
+
+
+def test_diagonal_part_is_matrix_diagonal():
+    """Assembled diagonal part = diagonal of the assembled matrix (the element
+    diagonals are what lands on the global diagonal)."""
+    from mifem.nonlinear import diagonal
+    mesh, F = _ref_forms(2, 1)
+    for a in F["matrix"]:
+        oa = cpu.OracleAssembler(a)
+        rp, cols, _, _ = oa.sparsity()
+        vals = oa.assemble_matrix(rp, cols)
+        rows = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+        diag = np.zeros(rp.size - 1)
+        diag[rows[rows == cols]] = vals[rows == cols]
+        got = cpu.OracleAssembler(diagonal(a)).assemble_vector()
+        np.testing.assert_allclose(got, diag, atol=1e-14 * max(1.0, abs(diag).max()))

@@ -334,6 +334,11 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
         const int rbase = BS * BS * lo;
         const uint8_t* kk = &skk[pc * ND];
         const int j0 = part * jw, j1 = min(ND, j0 + jw);
+        // scalar trace forms: a few column iterations in flight at once (their
+        // LDS reads of R_ij overlap instead of waiting one by one)
+#ifdef SK_TRACE
+#pragma unroll 4
+#endif
         for (int j = j0; j < j1; ++j) {
           double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #ifdef SK_TRACE

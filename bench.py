@@ -63,6 +63,75 @@ def measured_traffic():
     return None
 
 
+def cpu_baseline_c3(n_sample: int = 16):
+    """Oracle (single core, FP64) Q3 Helmholtz matrix on n_sample^3 jittered hexes."""
+    from mifem import problems
+    from oracle import cpu
+    P = problems.helmholtz_q3(n_sample)
+    Oa = cpu.OracleAssembler(P.a)
+    rp, cols, _, _ = Oa.sparsity()
+    t0 = time.perf_counter()
+    Oa.assemble_matrix(rp, cols, P.bcs)
+    dt = time.perf_counter() - t0
+    nc = P.mesh.num_cells
+    return {"value": nc / dt, "unit": "cells/s (A)", "cores": 1, "kind": "port",
+            "est_full_ms": 110592 / (nc / dt) * 1e3,
+            "sample": f"helmholtz_q3 {n_sample}^3 hexes ({nc} cells): A in {dt:.2f} s, oracle (C, serial)",
+            "cpu": platform.processor() or platform.machine()}
+
+
+def cpu_baseline_c4(n_sample: int = 24):
+    """Oracle (single core) Jacobian + residual + energy of the SIPG DG1 step."""
+    from mifem import problems
+    from mifem.dgforms import sipg_energy
+    from oracle import cpu
+    P = problems.dg_plaplace(n_sample)
+    Oa = cpu.OracleAssembler(P.a)
+    rp, cols, _, _ = Oa.sparsity()
+    t0 = time.perf_counter()
+    Oa.assemble_matrix(rp, cols)
+    cpu.OracleAssembler(P.L).assemble_vector()
+    cpu.OracleAssembler(sipg_energy(P.V, P.extras["u"])).assemble_scalar()
+    dt = time.perf_counter() - t0
+    nc = P.mesh.num_cells
+    return {"value": dt * 663552 / nc * 1e3, "unit": "ms per Newton step at 48^3 (scaled by cells)",
+            "cores": 1, "kind": "port",
+            "sample": f"sipg_plaplace DG1 {n_sample}^3x6 tets ({nc} cells): J + F + E in {dt:.2f} s, "
+                      f"oracle (C, serial)",
+            "cpu": platform.processor() or platform.machine()}
+
+
+def cpu_baseline_c5(B, per_degree: int = 2):
+    """Oracle (single core) A + b for the first instances of each degree of the batch."""
+    from mifem import batch as mb
+    from mifem import forms as mf
+    from mifem.fem import Constant, Function, dirichletbc, functionspace, locate_dofs_topological
+    from oracle import cpu
+    picked = []
+    for g in B.groups:
+        picked += [(i, g.degree) for i in g.instances[:per_degree]]
+    total = 0.0
+    for i, k in picked:
+        mesh, V, kap = mb.poisson_instance(B.sizes[i], k)
+        D = functionspace(mesh, ("DG", 0))
+        kf = Function(D)
+        kf.set_cellwise(kap)
+        a, L = mf.stiffness(V, kf), mf.source(V, Constant([1.0]))
+        bc = dirichletbc(0.0, locate_dofs_topological(V, 2, mesh.exterior_facets), V)
+        Oa, OL = cpu.OracleAssembler(a), cpu.OracleAssembler(L)
+        rp, cols, _, _ = Oa.sparsity()
+        t0 = time.perf_counter()
+        Oa.assemble_matrix(rp, cols, [bc])
+        b = OL.assemble_vector()
+        Oa.apply_lifting(b, [bc])
+        cpu.set_bc(b, [bc], V)
+        total += time.perf_counter() - t0
+    return {"value": len(picked) / total, "unit": "assemblies/s", "cores": 1, "kind": "port",
+            "sample": f"instances {[i for i, _ in picked]} (first {per_degree} of each degree): "
+                      f"A + b in {total:.2f} s, oracle (C, serial)",
+            "cpu": platform.processor() or platform.machine()}
+
+
 def cpu_baseline(n_sample: int):
     """Oracle (single core, FP64) on the same workload at n_sample^3 x 6 cells."""
     from mifem import problems
@@ -196,7 +265,7 @@ def bench_c3(args):
                      "unit": "TFLOP/s", "frac": flops / (times[32] * 1e-3) / 1e12 / 157.3,
                      "traffic": None, "kernel": "matrix launch: mf_hprep_cell_0 + mf_howner32_cell_0",
                      "fp64_achieved": flops / (times[64] * 1e-3) / 1e12, "fp64_peak": 78.6},
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else cpu_baseline_c3(),
     }
     print(json.dumps(out), flush=True)
 
@@ -369,7 +438,7 @@ def bench_c4(args):
                      "traffic": None,
                      "kernel": "mf_dgsys (J + F + E over cells and facets, one launch)",
                      "algorithmic_bytes": alg},
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else cpu_baseline_c4(),
     }
     print(json.dumps(out), flush=True)
 
@@ -500,7 +569,7 @@ def bench_c5(args):
                      "unit": "GB/s", "frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "traffic": None, "kernel": "owner-computes matrix launches (3 groups)",
                      "algorithmic_bytes": alg},
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else cpu_baseline_c5(B),
     }
     print(json.dumps(out), flush=True)
 

@@ -332,11 +332,24 @@ int mf_form_set_restriction(mf_form* f, int axis, const int32_t* node_map, int64
   });
 }
 
+namespace {
+void compile_form(mf_form* f, int verbose);
+void pack_all(mf_form* f, hipStream_t s);
+}  // namespace
+
 int mf_form_compile(mf_form* f, int verbose) {
   return guard([&] {
     MF_CHECK(f, MF_EINVAL, "null form");
+    compile_form(f, verbose);
+  });
+}
+
+namespace {
+void compile_form(mf_form* f, int verbose) {
+  {
     MF_CHECK(!f->ints.empty(), MF_ESTATE, "form has no integrals");
     use_device(f->ctx);
+    for (auto& I : f->ints) I->owner_grid = 0;    // the kernels below are new
     if (!f->consts.ptr) {
       double z = 0.0;
       f->consts.upload(&z, 1);
@@ -418,8 +431,19 @@ int mf_form_compile(mf_form* f, int verbose) {
                    (unsigned long long)f->uid, f->ints.size(), code.size(),
                    cached ? " (cache)" : "");
     f->compiled = true;
-  });
+  }
 }
+
+// packs every coefficient that has values (all of them after set_coefficient)
+void pack_all(mf_form* f, hipStream_t s) {
+  for (auto& c : f->coefs) {
+    const double* v = c.ext ? c.ext : c.own.ptr;
+    if (!v) continue;
+    mf::k_pack(f->packed.ptr, f->mesh->num_cells, f->W, c.offset, v, c.dm->dev.ptr, c.dm->nd,
+               c.dm->bs, s);
+  }
+}
+}  // namespace
 
 int mf_form_source(mf_form* f, char* buf, int64_t* len) {
   return guard([&] {
@@ -805,6 +829,79 @@ int mf_assemble_scalar_device(mf_form* M, double* d_value, int zero, mf_stream s
     MF_CHECK(M && d_value, MF_EINVAL, "null argument");
     use_device(M->ctx);
     mf::assemble_scalar(M, d_value, zero, S(s));
+  });
+}
+
+int mf_autotune_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc, int reps, mf_stream s,
+                       char* chosen, int64_t len, double* best_ms) {
+  return guard([&] {
+    MF_CHECK(a && A, MF_EINVAL, "null argument");
+    MF_CHECK(a->rank == 2 && a->compiled, MF_ESTATE, "autotune needs a compiled bilinear form");
+    MF_CHECK(!A->block && !a->restricted, MF_EUNSUPPORTED, "block member forms");
+    use_device(a->ctx);
+    reps = std::max(1, reps);
+    // owner-computes shapes: values per chunk, pairs per chunk, threads per
+    // workgroup, optional register cap and LDS budget (as cudolfinx.autotune)
+    static const std::vector<std::vector<std::pair<const char*, int64_t>>> cands = {
+        {{"chunk_values", 4096}, {"owner_pairs", 256}, {"owner_block", 256}},
+        {{"chunk_values", 4096}, {"owner_pairs", 256}, {"owner_block", 256}, {"owner_wpe", 4}},
+        {{"chunk_values", 4096}, {"owner_pairs", 384}, {"owner_block", 256}},
+        {{"chunk_values", 5120}, {"owner_pairs", 384}, {"owner_block", 256}, {"owner_lds", 81920}},
+        {{"chunk_values", 3072}, {"owner_pairs", 256}, {"owner_block", 256}},
+        {{"chunk_values", 6144}, {"owner_pairs", 512}, {"owner_block", 256}, {"owner_lds", 98304}},
+        {{"chunk_values", 2048}, {"owner_pairs", 256}, {"owner_block", 256}},
+    };
+    const auto base = a->options;
+    auto drop_plans = [&] {
+      for (auto it = A->plans.begin(); it != A->plans.end();)
+        it = it->first.first == a->uid ? A->plans.erase(it) : std::next(it);
+    };
+    hipEvent_t e0, e1;
+    MF_HIP(hipEventCreate(&e0));
+    MF_HIP(hipEventCreate(&e1));
+    double best = 1e300;
+    int best_k = -1;
+    for (size_t k = 0; k < cands.size(); ++k) {
+      a->options = base;
+      for (auto& kv : cands[k]) a->options[kv.first] = kv.second;
+      a->options["matrix_kernel"] = 2;
+      try {
+        compile_form(a, 0);
+        drop_plans();
+        pack_all(a, S(s));
+        mf::assemble_matrix(a, A, bc, bc, 1.0, 1, S(s));          // builds the schedule
+      } catch (const mf::Error&) {
+        continue;                                                  // shape does not fit / apply
+      }
+      MF_HIP(hipEventRecord(e0, S(s)));
+      for (int r = 0; r < reps; ++r) mf::assemble_matrix(a, A, bc, bc, 1.0, 1, S(s));
+      MF_HIP(hipEventRecord(e1, S(s)));
+      MF_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      MF_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms / reps < best) {
+        best = ms / reps;
+        best_k = static_cast<int>(k);
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    a->options = base;
+    std::string desc;
+    if (best_k >= 0)
+      for (auto& kv : cands[best_k]) {
+        a->options[kv.first] = kv.second;
+        desc += (desc.empty() ? "" : ",") + std::string(kv.first) + "=" + std::to_string(kv.second);
+      }
+    compile_form(a, 0);
+    drop_plans();
+    pack_all(a, S(s));
+    MF_CHECK(best_k >= 0, MF_EUNSUPPORTED, "no owner-computes shape applies to this form");
+    if (chosen && len > 0) {
+      std::strncpy(chosen, desc.c_str(), static_cast<size_t>(len - 1));
+      chosen[len - 1] = 0;
+    }
+    if (best_ms) *best_ms = best;
   });
 }
 

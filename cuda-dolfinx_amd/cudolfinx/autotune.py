@@ -120,3 +120,20 @@ def autotune_system(a, L, M, asm, A, b, value, blocks=SYSTEM_BLOCKS, reps: int =
         if best is None or ms < best[1]:
             best = (system, ms, blk)
     return best[0], best[2], timings
+
+
+def autotune_matrix_native(a, asm, A, bcs=(), reps: int = 5):
+    """The C-ABI tuner (mf_autotune_matrix) on an existing CUDAForm: the form
+    is recompiled in place with the fastest owner-computes shape.  Returns
+    (description "key=value,...", ms per assembly)."""
+    import ctypes as C
+
+    from ._lib import check, lib
+    coll = asm._bc_collection(list(bcs) if isinstance(bcs, tuple) else bcs)
+    bc = coll._get_cpp_bcs(a.function_spaces[0])
+    a.to_device(force=False)
+    buf = C.create_string_buffer(256)
+    ms = C.c_double(0.0)
+    check(lib().mf_autotune_matrix(a.handle, A.handle, bc.handle, int(reps), asm._stream, buf,
+                                   256, C.byref(ms)))
+    return buf.value.decode(), ms.value

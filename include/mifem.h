@@ -194,6 +194,16 @@ int mf_assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* 
                        double diagonal, int zero, mf_stream stream);
 /* assemble_vector_on_device (fem.cpp:291-309, CUDAAssembler.h:369-424). */
 int mf_assemble_vector(mf_form* L, double* d_b, int64_t n, int zero, mf_stream stream);
+/* Autotuner of the owner-computes matrix kernel (the "mf_autotune" of the
+ * boundary; the reference has no tuner -- it launches one thread per cell):
+ * recompiles `a` for each candidate shape (values per workgroup chunk, pairs
+ * per chunk, threads per workgroup, register cap, LDS budget), assembles A
+ * `reps` times per candidate with HIP events on `stream`, and keeps the
+ * fastest.  chosen (len bytes, may be NULL) receives "key=value,..."; best_ms
+ * (may be NULL) its time per assembly.  The form's coefficients are
+ * re-packed from their current values. */
+int mf_autotune_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc, int reps, mf_stream stream,
+                       char* chosen, int64_t len, double* best_ms);
 /* assemble_scalar_on_device (python/cudolfinx/wrappers/fem.cpp:311-323,
  * CUDAAssembler.h:296-351): the value of a functional (rank-0 form), summed
  * over its cell, exterior-facet and interior-facet integrals.  Returns the

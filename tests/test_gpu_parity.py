@@ -785,3 +785,24 @@ def test_reference_suite_diagonal_assembly():
         got = asm.assemble_vector(cufem.form(d)).to_host()
         ref = cpu.OracleAssembler(d).assemble_vector()
         assert _rel(got, ref) <= TOL, a.description
+
+
+def test_native_autotuner_then_parity():
+    _say("C-ABI owner-kernel autotuner (mf_autotune_matrix) on elasticity P2 n=4, then parity")
+    import cudolfinx as cufem
+    from cudolfinx.autotune import autotune_matrix_native
+    P = problems.elasticity_p2(4)
+    a, L = cufem.form(P.a), cufem.form(P.L)
+    asm = cufem.CUDAAssembler()
+    A = asm.create_matrix(a)
+    desc, ms = autotune_matrix_native(a, asm, A, P.bcs, reps=2)
+    _say(f"  chosen {desc} ({ms:.3f} ms)")
+    assert "chunk_values=" in desc and ms > 0
+    asm.assemble_matrix(a, A, bcs=P.bcs)
+    b = asm.assemble_vector(L)
+    dbc = asm.pack_bcs(P.bcs)
+    asm.apply_lifting(b, [a], [dbc])
+    asm.set_bc(b, dbc, P.V)
+    asm.synchronize()
+    rp, cols, vals = A.csr()
+    _compare(P, (rp, cols, vals, b.to_host()), _oracle(P))

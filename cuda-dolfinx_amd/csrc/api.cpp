@@ -659,6 +659,73 @@ int mf_assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* 
   });
 }
 
+// ------------------------------------------------------------------ batched groups
+int mf_batch_create(mf_form* const* forms, int n, mf_batch** out) {
+  return guard([&] {
+    MF_CHECK(forms && n > 0 && out, MF_EINVAL, "bad arguments");
+    MF_CHECK(n <= 32, MF_EINVAL, "at most 32 forms per batch (88 bytes of kernel arguments each)");
+    auto B = std::make_unique<mf_batch>();
+    std::vector<const mf_form*> cf;
+    std::vector<const mf::Integral*> ci;
+    for (int g = 0; g < n; ++g) {
+      mf_form* f = forms[g];
+      MF_CHECK(f && f->rank == 2 && f->compiled, MF_EINVAL, "batch members are compiled bilinear forms");
+      MF_CHECK(!f->restricted, MF_EUNSUPPORTED, "block member forms");
+      MF_CHECK(g == 0 || f->ctx == forms[0]->ctx, MF_EINVAL, "forms of different contexts");
+      mf::Integral* I = nullptr;
+      for (auto& J : f->ints)
+        if (J->type == MF_CELL && J->k_owner) {
+          I = J.get();
+          break;
+        }
+      MF_CHECK(I && f->ints.size() == 1, MF_EUNSUPPORTED,
+               "batch members need one cell integral on the owner-computes path (affine simplices with a staged description)");
+      B->forms.push_back(f);
+      B->ints.push_back(I);
+      cf.push_back(f);
+      ci.push_back(I);
+    }
+    B->ctx = forms[0]->ctx;
+    use_device(B->ctx);
+    B->block = static_cast<int>(forms[0]->opt("owner_block", 256));
+    for (auto* f : B->forms)
+      MF_CHECK(f->opt("owner_block", 256) == B->block, MF_EINVAL, "batch members must share owner_block");
+    int64_t lds = 0;
+    B->source = mf::generate_batch_source(cf, ci, B->block, &lds);
+    auto code = mf::rtc_compile(B->source, "mifem_batch", rtc_options(B->ctx), B->ctx->cache_dir, nullptr);
+    B->mod = std::make_unique<mf::Module>();
+    B->mod->load(code);
+    B->k = B->mod->get("mf_owner_batch");
+    for (int g = 0; g < n; ++g) B->preps.push_back(B->mod->get("mf_prep_g" + std::to_string(g)));
+    int nblk = 0;
+    MF_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, B->k, B->block, 0));
+    B->grid = std::max<int64_t>(8, (int64_t)std::max(1, nblk) * B->ctx->num_cus / 8 * 8);
+    *out = B.release();
+  });
+}
+
+int mf_batch_assemble(mf_batch* B, mf_matrix* const* As, const mf_bc* const* bcs, double diagonal,
+                      int zero, mf_stream s) {
+  return guard([&] {
+    MF_CHECK(B && As, MF_EINVAL, "null argument");
+    use_device(B->ctx);
+    mf::assemble_batch(B, As, bcs, diagonal, zero, S(s));
+  });
+}
+
+int mf_batch_source(mf_batch* B, char* buf, int64_t* len) {
+  return guard([&] {
+    MF_CHECK(B && len, MF_EINVAL, "null argument");
+    int64_t need = static_cast<int64_t>(B->source.size()) + 1;
+    if (buf && *len >= need) std::memcpy(buf, B->source.c_str(), need);
+    *len = need;
+  });
+}
+
+int mf_batch_destroy(mf_batch* B) {
+  return guard([&] { delete B; });
+}
+
 // ------------------------------------------------------------------ fused Newton system
 namespace {
 bool same_integrals(const mf_form& x, const mf_form& y) {

@@ -306,6 +306,8 @@ std::string kernel_name(const char* what, int type, int index) {
 }
 
 extern const char* kOwnerTemplate;         // owner_kernel.cpp
+extern const char* kOwnerParams;
+extern const char* kOwnerBatchKernel;
 extern const char* kOwnerCommon;
 extern const char* kOwnerVectorTemplate;
 extern const char* kStagedLiftTemplate;
@@ -414,6 +416,106 @@ std::string generate_system_source(const mf_form& a, const mf_form& L, const mf_
   m["$DBLOCK"] = std::to_string(block);
   m["$DATTR"] = wpe > 0 ? "__attribute__((amdgpu_waves_per_eu(" + std::to_string(wpe) + ")))" : "";
   os << subst(subst(kDGSystem, {{"$CASES", cases}}), m);
+  return os.str();
+}
+
+// The MF_SHARED slices of one owner-computes body (kOwnerTemplate), in
+// declaration order: name, element type, bytes (rounded up to 16).  The
+// reference-tensor slice (sRS or sRT) is present only for those variants.
+struct LdsSlice {
+  const char* name;
+  const char* type;
+  int64_t bytes;
+};
+static std::vector<LdsSlice> owner_lds_slices(const mf_form& f, const Integral& I) {
+  auto r16 = [](int64_t b) { return (b + 15) / 16 * 16; };
+  const int64_t nd = staged_define(I.staged_src, "SK_ND"), bs = staged_define(I.staged_src, "SK_BS");
+  const int64_t cap = f.opt("chunk_values", 4096), maxr = f.opt("chunk_rows", 64);
+  const int64_t maxp = owner_shape(f, I).maxp;
+  std::vector<LdsSlice> v = {{"acc", "double", r16(8 * cap)},
+                             {"lrp", "int32_t", r16(4 * (maxr + 1))},
+                             {"snb", "uint8_t", r16(maxr)},
+                             {"lcols", "int32_t", r16(4 * (cap / (bs * bs) + 1))},
+                             {"spair", "int32_t", r16(4 * maxp)},
+                             {"skk", "uint8_t", r16((maxp * nd + 15) / 16 * 16)}};
+  if (staged_trace(I)) v.push_back({"sRS", "double", r16(8 * nd * nd * 6)});
+  else if (staged_fullrt(I)) v.push_back({"sRT", "double", r16(8 * nd * nd * 9)});
+  return v;
+}
+
+int64_t owner_lds_bytes(const mf_form& f, const Integral& I) {
+  int64_t b = 0;
+  for (const auto& sl : owner_lds_slices(f, I)) b += sl.bytes;
+  return b;
+}
+
+std::string generate_batch_source(const std::vector<const mf_form*>& forms,
+                                  const std::vector<const Integral*>& ints, int block,
+                                  int64_t* max_lds) {
+  std::ostringstream os;
+  os << kPrelude << kOwnerCommon;
+  std::string calls, params;
+  int64_t lds = 0;
+  for (size_t g = 0; g < forms.size(); ++g) {
+    const mf_form& f = *forms[g];
+    const Integral& I = *ints[g];
+    const std::string tag = "G" + std::to_string(g) + "I" + std::to_string(I.index);
+    const std::string pfx = tag + "_";
+    auto rename = [&](std::string t) {
+      for (const char* sym : {"SK_ND", "SK_BS", "SK_NQ", "SK_W", "SK_DPHI", "SK_TRACE", "SK_RS",
+                              "SK_FULLRT", "SK_RT", "sk_combine"})
+        replace_all(t, sym, pfx + sym);
+      return t;
+    };
+    std::map<std::string, std::string> m;
+    m["$NV"] = std::to_string(f.mesh->nv);
+    m["$W"] = std::to_string(f.W);
+    m["$WW"] = std::to_string(std::max(1, f.W));
+    m["$TAG"] = tag;
+    const OwnerShape sh = owner_shape(f, I);
+    m["$ONB"] = std::to_string(sh.nb);
+    m["$OMAXP"] = std::to_string(sh.maxp);
+    m["$OCAP"] = std::to_string(f.opt("chunk_values", 4096));
+    m["$OMAXR"] = std::to_string(f.opt("chunk_rows", 64));
+    m["$OBLOCK"] = std::to_string(block);
+    os << "\n" << rename(I.staged_src) << "\n";
+    os << subst(rename(kCellRecord), m);
+    m["$KNAME"] = "mf_prep_g" + std::to_string(g);
+    os << subst(rename(kPrepTemplate), m);
+    m["$KNAME"] = "mf_owner_body_g" + std::to_string(g);
+    m["$OHEAD"] = "static __device__ __forceinline__ void " + m["$KNAME"];
+    // each LDS slice arrives as its own __restrict__ parameter: slices carved
+    // from one buffer would otherwise be assumed to overlap, which orders
+    // every LDS load after every earlier LDS store
+    const auto slices = owner_lds_slices(f, I);
+    std::string sparams, sargs;
+    int64_t off = 0;
+    for (const auto& sl : slices) {
+      sparams += std::string(", ") + sl.type + "* __restrict__ mf_slice_" + sl.name;
+      sargs += std::string(", reinterpret_cast<") + sl.type + "*>(mf_lds + " + std::to_string(off) + ")";
+      off += sl.bytes;
+    }
+    m["$OPARAMS"] = std::string(kOwnerParams) + sparams;
+    os << "#undef MF_SHARED_BEGIN\n#undef MF_SHARED\n"
+          "#define MF_SHARED_BEGIN do {} while (0)\n"
+          "#define MF_SHARED(T, name, count) T* const name = mf_slice_##name\n";
+    os << subst(rename(kOwnerTemplate), m);
+    os << "#undef MF_SHARED_BEGIN\n#undef MF_SHARED\n"
+          "#define MF_SHARED_BEGIN do {} while (0)\n"
+          "#define MF_SHARED(T, name, count) __shared__ __attribute__((aligned(16))) T name[count]\n";
+    lds = std::max(lds, owner_lds_bytes(f, I));
+    {
+      const std::string a = "a" + std::to_string(g) + ".";
+      params += std::string(g ? ", " : "") + "const mf_batch_args a" + std::to_string(g);
+      calls += "  mf_owner_body_g" + std::to_string(g) + "(" + a + "nchunks, " + a + "chunk_meta, " +
+               a + "item_cell, " + a + "pair_list, " + a + "pair_kk, reinterpret_cast<const mf_rec_" +
+               tag + "*>(" + a + "rec), " + a + "consts, " + a + "rpb, " + a + "colsb, " + a +
+               "nb0, " + a + "values, diag, flags" + sargs + ");\n  __syncthreads();\n";
+    }
+  }
+  os << subst(kOwnerBatchKernel, {{"$CALLS", calls}, {"$PARAMS", params}, {"$OBLOCK", std::to_string(block)},
+                                  {"$MAXLDS", std::to_string(lds)}});
+  if (max_lds) *max_lds = lds;
   return os.str();
 }
 
@@ -549,7 +651,11 @@ std::string generate_form_source(const mf_form& f) {
           m["$OMAXP"] = std::to_string(sh.maxp);
           // optional register cap: at least owner_wpe waves per SIMD
           const int64_t wpe = f.opt("owner_wpe", 0);
-          m["$OATTR"] = wpe > 0 ? "__attribute__((amdgpu_waves_per_eu(" + std::to_string(wpe) + ")))" : "";
+          const std::string attr =
+              wpe > 0 ? "__attribute__((amdgpu_waves_per_eu(" + std::to_string(wpe) + ")))" : "";
+          m["$OHEAD"] = "extern \"C\" __global__ void __launch_bounds__($OBLOCK) " + attr + "\n" +
+                        m["$KNAME"];
+          m["$OPARAMS"] = kOwnerParams;
           os << subst(rename(kOwnerTemplate), m);
         }
         m["$KNAME"] = kernel_name("slift", I->type, I->index);

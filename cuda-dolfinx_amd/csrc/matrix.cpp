@@ -638,6 +638,63 @@ void assemble_vector(mf_form* L, double* b, int64_t n, int zero, hipStream_t s) 
   }
 }
 
+// Several bilinear forms (one element group each, e.g. the degree groups of
+// many independent meshes) in one persistent owner-computes launch, after
+// one per-cell record pass per group.  Each group's matrix gets exactly what
+// assemble_matrix's owner path writes (values overwritten when zero != 0;
+// Dirichlet rows/columns of bcs[g] masked and the diagonal set).
+void assemble_batch(mf_batch* B, mf_matrix* const* As, const mf_bc* const* bcs, double diag,
+                    int zero, hipStream_t s) {
+  struct Args {   // layout of mf_batch_args in kOwnerBatchKernel (passed by value)
+    const void* chunk_meta; const void* item_cell; const void* pair_list; const void* pair_kk;
+    const void* rec; const void* consts; const void* rpb; const void* colsb; const void* nb0;
+    void* values; int32_t nchunks; int32_t pad;
+  };
+  static_assert(sizeof(Args) == 88, "batch argument layout");
+  const size_t ng = B->forms.size();
+  std::vector<Args> h(ng);
+  for (size_t g = 0; g < ng; ++g) {
+    mf_form* a = B->forms[g];
+    Integral* I = B->ints[g];
+    mf_matrix* A = As[g];
+    const mf_bc* bc = bcs ? bcs[g] : nullptr;
+    MF_CHECK(A && A->form && !A->block && A->nrows == a->dm[0]->num_nodes * a->dm[0]->bs,
+             MF_ESHAPE, "batch matrix " + std::to_string(g) + " does not match its form");
+    if (bc) MF_CHECK(bc->ndofs == A->nrows, MF_ESHAPE, "batch bc size does not match the rows");
+    auto key = std::make_pair(a->uid, I->index);
+    std::shared_ptr<OwnerPlan> plan;
+    auto it = A->plans.find(key);
+    if (it == A->plans.end()) {
+      plan = build_owner_plan(*a, *I, *A, a->opt("chunk_values", 4096),
+                              static_cast<int>(a->opt("chunk_rows", 64)), owner_shape(*a, *I).maxp,
+                              true);
+      A->plans[key] = plan;
+    } else {
+      plan = it->second;
+    }
+    MF_CHECK(plan != nullptr, MF_EUNSUPPORTED, "owner-computes schedule unavailable for group " +
+                                                   std::to_string(g));
+    const uint8_t* n0 = bc && bc->nbc ? bc->bits_for(a->dm[0]->bs, s) : nullptr;
+    launch(B->preps[g], static_cast<uint32_t>((I->n + 255) / 256), 256, 0, s,
+           static_cast<int32_t>(I->n), I->d_ents.ptr, a->mesh->cells.ptr, a->mesh->x.ptr,
+           a->packed.ptr, a->dm[0]->dev.ptr, n0, I->records.ptr);
+    h[g] = Args{plan->chunk_meta.ptr, plan->item_cell.ptr, plan->pair_list.ptr, plan->pair_kk.ptr,
+                I->records.ptr, a->consts.ptr, A->rpb.ptr, A->colsb.ptr, n0, A->values,
+                static_cast<int32_t>(plan->nchunks), 0};
+  }
+  // kernel arguments: the groups' structs by value, then diag and flags
+  // (hipModuleLaunchKernel copies each from its pointer per the kernel's
+  // argument layout)
+  int flags = zero ? 1 : 0;
+  if (zero) flags |= 2;              // Dirichlet diagonal for the groups that have constraints
+  std::vector<void*> ptrs;
+  for (size_t g = 0; g < ng; ++g) ptrs.push_back(&h[g]);
+  ptrs.push_back(&diag);
+  ptrs.push_back(&flags);
+  MF_HIP(hipModuleLaunchKernel(B->k, static_cast<uint32_t>(B->grid), 1, 1,
+                               static_cast<uint32_t>(B->block), 1, 1, 0, s, ptrs.data(), nullptr));
+}
+
 // Rank-0 assembly (CUDAAssembler.h:296-351, CUDAFormIntegral.cpp:68-266):
 // per integral, in cell / exterior / interior order, one partial per
 // workgroup; then one fixed-order sum of all partials into d_value.

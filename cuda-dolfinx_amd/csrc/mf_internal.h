@@ -278,6 +278,18 @@ struct mf_system {
   mf::DevArray<uint32_t> counter;     // finished workgroups (reset by the last)
 };
 
+struct mf_batch {
+  mf_context* ctx = nullptr;
+  std::vector<mf_form*> forms;
+  std::vector<mf::Integral*> ints;    // each form's owner-computes cell integral
+  std::unique_ptr<mf::Module> mod;
+  hipFunction_t k = nullptr;
+  std::vector<hipFunction_t> preps;
+  std::string source;
+  int block = 256;
+  int64_t grid = 0;
+};
+
 struct mf_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -323,9 +335,16 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
                      double diagonal, int zero, hipStream_t s);
 void assemble_vector(mf_form* L, double* b, int64_t n, int zero, hipStream_t s);
 void assemble_scalar(mf_form* M, double* d_value, int zero, hipStream_t s);
+void assemble_batch(mf_batch* B, mf_matrix* const* As, const mf_bc* const* bcs, double diag,
+                    int zero, hipStream_t s);
 // cell-owner schedule of a form on a cell-major discontinuous space (null if
 // the form/matrix does not qualify); with A, block positions in A's rows
 std::shared_ptr<DGPlan> build_dg_plan(const mf_form& f, const mf_matrix* A);
+// one persistent launch over several forms' owner-computes bodies (the cell
+// integral ints[g] of forms[g]); *max_lds = LDS bytes of the largest body
+std::string generate_batch_source(const std::vector<const mf_form*>& forms,
+                                  const std::vector<const Integral*>& ints, int block,
+                                  int64_t* max_lds);
 // fused Newton-system kernel source (Jacobian a, residual L, optional functional M)
 std::string generate_system_source(const mf_form& a, const mf_form& L, const mf_form* M,
                                    int block, int wpe);

@@ -32,6 +32,9 @@ namespace mf {
 const char* kOwnerCommon = R"(
 #ifndef MF_OWNER_COMMON
 #define MF_OWNER_COMMON
+// LDS arrays of the owner-computes body (one form per launch)
+#define MF_SHARED_BEGIN do {} while (0)
+#define MF_SHARED(T, name, count) __shared__ __attribute__((aligned(16))) T name[count]
 typedef double mf_d2 __attribute__((ext_vector_type(2)));
 static __device__ __forceinline__ int mf_lds_lower_bound(const int32_t* a, int lo, int hi, int32_t key)
 {
@@ -53,6 +56,37 @@ static __device__ __forceinline__ int mf_xcd_chunk(int nchunks)
   return (slot >= per || k >= nchunks) ? -1 : k;
 }
 #endif
+)";
+
+// Parameters of one form's owner-computes body.
+const char* kOwnerParams =
+    "const int32_t nchunks, const int32_t* __restrict__ chunk_meta, "
+    "const int32_t* __restrict__ item_cell, const int32_t* __restrict__ pair_list, "
+    "const uint8_t* __restrict__ pair_kk, const mf_rec_$TAG* __restrict__ rec, "
+    "const double* __restrict__ consts, const int32_t* __restrict__ rpb, "
+    "const int32_t* __restrict__ colsb, const uint8_t* __restrict__ nb0, "
+    "double* __restrict__ values, const double diag, const int flags";
+
+// Several forms (one element group each) in ONE persistent launch: every
+// workgroup runs each group's chunk schedule in turn (the groups' chunks are
+// spread over the same resident grid, so a small group's tail overlaps the
+// next group's work), reusing one LDS buffer sized for the largest group.
+// Each group's arguments come by value ($PARAMS: mf_batch_args a0, a1, ...):
+// pointer fields of by-value kernel arguments are known to address global
+// memory, where pointers read from a table in memory would be generic (flat
+// loads and stores, which also wait on the LDS counter).  $CALLS: one call
+// per group.
+const char* kOwnerBatchKernel = R"(
+struct mf_batch_args {
+  const int32_t* chunk_meta; const int32_t* item_cell; const int32_t* pair_list;
+  const uint8_t* pair_kk; const void* rec; const double* consts; const int32_t* rpb;
+  const int32_t* colsb; const uint8_t* nb0; double* values; int32_t nchunks; int32_t pad;
+};
+extern "C" __global__ void __launch_bounds__($OBLOCK)
+mf_owner_batch($PARAMS, const double diag, const int flags)
+{
+  __shared__ __attribute__((aligned(16))) char mf_lds[$MAXLDS];
+$CALLS}
 )";
 
 // Per-cell record consumed by the owner kernel (square forms): geometry
@@ -111,15 +145,15 @@ $KNAME(const int32_t n, const int32_t* __restrict__ ents, const int32_t* __restr
 }
 )";
 
+// $OHEAD is either a kernel header (one form per launch) or a device-function
+// header (several forms' bodies inside one persistent launch, see
+// kOwnerBatchKernel); MF_SHARED declares the LDS arrays either as __shared__
+// arrays or, in a body, binds them to its `mf_slice_<name>` parameters (one
+// __restrict__ pointer per array, carved by the caller from one buffer).
 const char* kOwnerTemplate = R"(
-extern "C" __global__ void __launch_bounds__($OBLOCK) $OATTR
-$KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
-       const int32_t* __restrict__ item_cell, const int32_t* __restrict__ pair_list,
-       const uint8_t* __restrict__ pair_kk, const mf_rec_$TAG* __restrict__ rec,
-       const double* __restrict__ consts, const int32_t* __restrict__ rpb,
-       const int32_t* __restrict__ colsb, const uint8_t* __restrict__ nb0,
-       double* __restrict__ values, const double diag, const int flags)
+$OHEAD($OPARAMS)
 {
+  MF_SHARED_BEGIN;
   // chunk_meta[12k ..]: r0, r1, i0, i1, cb0 (= rpb[r0]), ncb, base (CSR offset),
   //   seglen, p0, p1 (pair range), kko (byte offset of the chunk's column
   //   offsets, 16-byte aligned), 0.
@@ -144,19 +178,19 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
   constexpr int PF_P = (MAXP + NT - 1) / NT, PF_K = (KK4 + NT - 1) / NT;
   constexpr int PF_R = (MAXR + 1 + NT - 1) / NT, PF_C = (CAPB + NT - 1) / NT;
   constexpr int PF_B = (MAXR + NT - 1) / NT;
-  __shared__ __attribute__((aligned(16))) double acc[CAP];
-  __shared__ int32_t lrp[MAXR + 1];
-  __shared__ uint8_t snb[MAXR];                      // Dirichlet bits of the chunk's rows
-  __shared__ int32_t lcols[CAPB];
-  __shared__ int32_t spair[MAXP];
-  __shared__ __attribute__((aligned(16))) uint8_t skk[KK4 * 16];
+  MF_SHARED(double, acc, CAP);
+  MF_SHARED(int32_t, lrp, MAXR + 1);
+  MF_SHARED(uint8_t, snb, MAXR);                     // Dirichlet bits of the chunk's rows
+  MF_SHARED(int32_t, lcols, CAPB);
+  MF_SHARED(int32_t, spair, MAXP);
+  MF_SHARED(uint8_t, skk, KK4 * 16);
 #ifdef SK_TRACE
   // reference tensors R_ij (symmetrised, 6 per entry), loaded once per workgroup
-  __shared__ double sRS[ND * ND * 6];
+  MF_SHARED(double, sRS, ND * ND * 6);
   for (int p = threadIdx.x; p < ND * ND * 6; p += NT) sRS[p] = (&SK_RS[0][0][0])[p];
 #elif defined(SK_FULLRT)
   // full reference tensors R_ij[3 r + s] (9 per entry), loaded once per workgroup
-  __shared__ double sRT[ND * ND * 9];
+  MF_SHARED(double, sRT, ND * ND * 9);
   for (int p = threadIdx.x; p < ND * ND * 9; p += NT) sRT[p] = (&SK_RT[0][0][0])[p];
 #endif
 

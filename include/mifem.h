@@ -39,6 +39,7 @@ typedef struct mf_bc mf_bc;
 typedef struct mf_graph mf_graph;
 typedef struct mf_interpolator mf_interpolator;
 typedef struct mf_system mf_system;
+typedef struct mf_batch mf_batch;
 typedef void* mf_stream;
 
 enum mf_status {
@@ -215,6 +216,24 @@ int mf_assemble_scalar(mf_form* M, double* value, mf_stream stream);
  * value, on `stream`, with no host synchronization (usable under graph
  * capture).  d_value is device memory. */
 int mf_assemble_scalar_device(mf_form* M, double* d_value, int zero, mf_stream stream);
+/* ---------------------------------------------------------------- batched element groups */
+/* No counterpart in the reference (one launch per form and integral type).
+ * For callers with many independent problems merged per element group (e.g.
+ * many meshes of uneven size and degree): n compiled bilinear forms, each
+ * with one cell integral on the owner-computes path, assembled by ONE
+ * persistent launch (after one per-cell record pass per group).  The forms
+ * keep their own compile-time shapes; they must share owner_block.  At most
+ * 32 forms (each passes 88 bytes of kernel arguments). */
+int mf_batch_create(mf_form* const* forms, int n, mf_batch** out);
+/* Assembles form g into As[g] (from mf_matrix_create(forms[g])) with rows
+ * and columns of bcs[g] (bcs or bcs[g] may be NULL) masked and, when zero !=
+ * 0, their diagonal set to `diagonal`; zero != 0 overwrites the values.  No
+ * allocation or host synchronization once the matrices' schedules exist. */
+int mf_batch_assemble(mf_batch* batch, mf_matrix* const* As, const mf_bc* const* bcs,
+                      double diagonal, int zero, mf_stream stream);
+int mf_batch_source(mf_batch* batch, char* buf, int64_t* len);
+int mf_batch_destroy(mf_batch* batch);
+
 /* ---------------------------------------------------------------- fused Newton system */
 /* No counterpart in the reference, which launches every form and integral
  * type separately.  For callers that reassemble a Jacobian a, its residual L

@@ -438,6 +438,52 @@ def test_many_instances_block_separate():
             assert _rel(v1, v2) <= TOL
 
 
+def test_many_instances_one_launch():
+    _say("batch of 12 Poisson instances (P1-P3, n=2..5), all groups in one launch")
+    from mifem import batch
+    import cudolfinx as cufem
+    B = batch.many_poisson(12, seed=3, nmin=2, nmax=5)
+    asm = cufem.CUDAAssembler()
+    forms_ = [cufem.form(g.a) for g in B.groups]
+    bcs = [asm.pack_bcs(g.bcs) for g in B.groups]
+    mats = [asm.create_matrix(a) for a in forms_]
+    sep = [asm.create_matrix(a) for a in forms_]
+    fused = cufem.CUDABatch(forms_)
+    assert "mf_owner_batch" in fused.source()
+    for rep in range(2):
+        fused.assemble(asm, mats, bcs, diagonal=1.0 + rep)
+        for a, S, bc in zip(forms_, sep, bcs):
+            asm.assemble_matrix(a, S, bcs=bc, diagonal=1.0 + rep)
+        asm.synchronize()
+        for g, A, S in zip(B.groups, mats, sep):
+            class _P:
+                pass
+            P = _P()
+            P.name, P.a, P.L, P.bcs, P.V = f"one-launch P{g.degree} call {rep}", g.a, g.L, g.bcs, g.V
+            rp, cols, vals = A.csr()
+            srp, scols, svals = S.csr()
+            assert np.array_equal(rp, srp) and np.array_equal(cols, scols)
+            orp, ocols, _, _ = cpu.OracleAssembler(P.a).sparsity()
+            ovals = cpu.OracleAssembler(P.a).assemble_matrix(orp, ocols, P.bcs, diagonal=1.0 + rep)
+            assert np.array_equal(rp, orp) and np.array_equal(cols, ocols)
+            es, eo = _rel(vals, svals), _rel(vals, ovals)
+            _say(f"{P.name}: nnz={cols.size} vs separate {es:.2e}, vs oracle {eo:.2e}")
+            assert es <= 1e-14 and eo <= TOL, (es, eo)
+    # accumulate (zero=False): a second pass doubles every entry off the constrained rows/columns
+    A0 = [m.csr()[2].copy() for m in mats]
+    fused.assemble(asm, mats, bcs, diagonal=2.0, zero=False)
+    asm.synchronize()
+    for g, A, v0 in zip(B.groups, mats, A0):
+        rp, cols, vals = A.csr()
+        bc = np.zeros(g.V.size, dtype=bool)
+        for d in g.bcs:
+            bc[d.dofs] = True
+        rows = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+        free = ~(bc[rows] | bc[cols])
+        assert np.allclose(vals[free], 2 * v0[free], rtol=1e-14, atol=0)
+        assert np.array_equal(vals[~free], v0[~free])
+
+
 # ------------------------------------------------------------ block / restricted assembly
 def _selector(V, keep):
     """Scalar-dof selection matrix of the kept nodes (in order), bs-unrolled."""

@@ -235,6 +235,7 @@ def bench_c3(args):
         log(f"precision {prec}: matrix {times[prec]:.3f} ms")
     # the reference's scheme in FP64: one thread per cell, full 64 x 64 tabulate_tensor,
     # FP64 atomics into the CSR (per-entity kernel)
+    a.set_option("precision", 64)
     a.set_option("matrix_kernel", 1)
     gbuf = torch.empty(A.nnz, dtype=torch.float64, device="cuda")
     _lib.check(lib.mf_matrix_borrow_values(A.handle, C.c_void_p(gbuf.data_ptr())))
@@ -510,7 +511,9 @@ def bench_c5(args):
     asm = cufem.CUDAAssembler()
     sys_ = []
     for g in B.groups:
-        a, L = cufem.form(g.a), cufem.form(g.L)
+        # --c5-fp32: degree >= 2 element tensors and LDS accumulation in FP32 (CSR values FP64)
+        a = cufem.form(g.a, options={"precision": 32} if g.degree >= 2 and args.c5_fp32 else None)
+        L = cufem.form(g.L)
         A, b = asm.create_matrix(a), asm.create_vector(L)
         bcs = asm.pack_bcs(g.bcs)
         sys_.append((g, a, L, A, b, bcs))
@@ -614,11 +617,12 @@ def bench_c5(args):
         "metric": "independent-instance assemblies/s (A+b, 512 Poisson meshes, P1-P3)",
         "value": B.num_instances / (ms * 1e-3), "unit": "assemblies/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (P2, P3) / f64 (P1)" if args.c5_fp32 else "f64",
         "data": "synthetic (seeded sizes, degrees and lognormal conductivity)",
         "config": {"workload": f"many_poisson {B.num_instances} instances n~U{{4..24}} P1-P3",
                    "cells": ncells, "dofs": ndofs,
-                   "groups": [{"degree": g.degree, "instances": len(g.instances), "nnz": A.nnz}
+                   "groups": [{"degree": g.degree, "instances": len(g.instances), "nnz": A.nnz,
+                               "precision": 32 if g.degree >= 2 and args.c5_fp32 else 64}
                               for g, a, L, A, b, bcs in sys_],
                    "dof_per_s": ndofs / (ms * 1e-3), "matrix_ms": kms, "matrix_ms_per_group_launches": kms_sep,
                    "sampled_errors": errs,
@@ -653,6 +657,8 @@ def main():
     ap.add_argument("--instances", type=int, default=512, help="instances for c5")
     ap.add_argument("--c5-sample", type=int, default=4, help="oracle-checked instances per group")
     ap.add_argument("--c5-oracle-n", type=int, default=10, help="largest n checked by the oracle")
+    ap.add_argument("--c5-fp32", action="store_true",
+                    help="c5: degree >= 2 groups accumulate in FP32 (slower on this GPU: FP32 LDS atomics)")
     ap.add_argument("--n-dg", type=int, default=48, help="cubes per direction for c4")
     ap.add_argument("--newton", type=int, default=10, help="Newton steps for c4")
     ap.add_argument("--n-hex", type=int, default=48, help="hexahedra per direction for c3")

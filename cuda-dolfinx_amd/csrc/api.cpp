@@ -349,7 +349,7 @@ void compile_form(mf_form* f, int verbose) {
   {
     MF_CHECK(!f->ints.empty(), MF_ESTATE, "form has no integrals");
     use_device(f->ctx);
-    for (auto& I : f->ints) I->owner_grid = 0;    // the kernels below are new
+    for (auto& I : f->ints) I->owner_grid = I->owner_grid32 = 0;    // the kernels below are new
     if (!f->consts.ptr) {
       double z = 0.0;
       f->consts.upload(&z, 1);
@@ -402,6 +402,9 @@ void compile_form(mf_form* f, int verbose) {
           I->k_slift = f->mod->get(mf::kernel_name("slift", I->type, I->index));
           if (mf::owner_supported(*f, *I)) {
             I->k_owner = f->mod->get(mf::kernel_name("owner", I->type, I->index));
+            I->k_owner32 = mf::staged_trace(*I)
+                               ? f->mod->get(mf::kernel_name("owner32", I->type, I->index))
+                               : nullptr;
             I->k_prep = f->mod->get(mf::kernel_name("prep", I->type, I->index));
             I->record_bytes = mf::record_bytes(*f, *I);
             I->records.alloc(f->mesh->num_cells * I->record_bytes / 8);

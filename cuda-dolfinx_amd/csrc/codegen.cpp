@@ -427,18 +427,25 @@ struct LdsSlice {
   const char* type;
   int64_t bytes;
 };
+static bool owner_f32(const mf_form& f, const Integral& I) {
+  return staged_trace(I) && f.opt("precision", 64) == 32;
+}
+
 static std::vector<LdsSlice> owner_lds_slices(const mf_form& f, const Integral& I) {
   auto r16 = [](int64_t b) { return (b + 15) / 16 * 16; };
   const int64_t nd = staged_define(I.staged_src, "SK_ND"), bs = staged_define(I.staged_src, "SK_BS");
   const int64_t cap = f.opt("chunk_values", 4096), maxr = f.opt("chunk_rows", 64);
   const int64_t maxp = owner_shape(f, I).maxp;
-  std::vector<LdsSlice> v = {{"acc", "double", r16(8 * cap)},
+  const bool f32 = owner_f32(f, I);
+  const char* at = f32 ? "float" : "double";
+  const int64_t as = f32 ? 4 : 8;
+  std::vector<LdsSlice> v = {{"acc", at, r16(as * cap)},
                              {"lrp", "int32_t", r16(4 * (maxr + 1))},
                              {"snb", "uint8_t", r16(maxr)},
                              {"lcols", "int32_t", r16(4 * (cap / (bs * bs) + 1))},
                              {"spair", "int32_t", r16(4 * maxp)},
                              {"skk", "uint8_t", r16((maxp * nd + 15) / 16 * 16)}};
-  if (staged_trace(I)) v.push_back({"sRS", "double", r16(8 * nd * nd * 6)});
+  if (staged_trace(I)) v.push_back({"sRS", at, r16(as * nd * nd * 6)});
   else if (staged_fullrt(I)) v.push_back({"sRT", "double", r16(8 * nd * nd * 9)});
   return v;
 }
@@ -496,13 +503,15 @@ std::string generate_batch_source(const std::vector<const mf_form*>& forms,
       off += sl.bytes;
     }
     m["$OPARAMS"] = std::string(kOwnerParams) + sparams;
+    os << "#undef MF_OWNER_F32\n#define MF_OWNER_F32 " << (owner_f32(f, I) ? 1 : 0) << "\n";
     os << "#undef MF_SHARED_BEGIN\n#undef MF_SHARED\n"
           "#define MF_SHARED_BEGIN do {} while (0)\n"
           "#define MF_SHARED(T, name, count) T* const name = mf_slice_##name\n";
     os << subst(rename(kOwnerTemplate), m);
     os << "#undef MF_SHARED_BEGIN\n#undef MF_SHARED\n"
           "#define MF_SHARED_BEGIN do {} while (0)\n"
-          "#define MF_SHARED(T, name, count) __shared__ __attribute__((aligned(16))) T name[count]\n";
+          "#define MF_SHARED(T, name, count) __shared__ __attribute__((aligned(16))) T name[count]\n"
+          "#undef MF_OWNER_F32\n#define MF_OWNER_F32 0\n";
     lds = std::max(lds, owner_lds_bytes(f, I));
     {
       const std::string a = "a" + std::to_string(g) + ".";
@@ -657,6 +666,14 @@ std::string generate_form_source(const mf_form& f) {
                         m["$KNAME"];
           m["$OPARAMS"] = kOwnerParams;
           os << subst(rename(kOwnerTemplate), m);
+          if (staged_trace(*I)) {
+            // FP32 accumulation variant (precision = 32)
+            m["$KNAME"] = kernel_name("owner32", I->type, I->index);
+            m["$OHEAD"] = "extern \"C\" __global__ void __launch_bounds__($OBLOCK) " + attr + "\n" +
+                          m["$KNAME"];
+            os << "#undef MF_OWNER_F32\n#define MF_OWNER_F32 1\n" << subst(rename(kOwnerTemplate), m)
+               << "#undef MF_OWNER_F32\n#define MF_OWNER_F32 0\n";
+          }
         }
         m["$KNAME"] = kernel_name("slift", I->type, I->index);
         os << subst(rename(kStagedLiftTemplate), m);

@@ -508,6 +508,9 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
            "owner-computes kernel requested but unavailable for this form/matrix");
   bool diag_done = false;
   if (owner && square && bc0 != bc1 && (m0 || m1)) owner = nullptr;   // records hold one bc
+  MF_CHECK(owner || a->opt("precision", 64) == 64, MF_EUNSUPPORTED,
+           "precision 32 / 16 runs on the owner-computes kernels only (same Dirichlet set on "
+           "rows and columns)");
   MF_CHECK(!(mode == 2 && !owner), MF_EUNSUPPORTED,
            "owner-computes kernel needs the same Dirichlet set on rows and columns");
   if (owner) {
@@ -537,16 +540,23 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
     // (multiple of 8, one group of chunks per XCD); owner_persistent=0 gives
     // one workgroup per chunk
     const uint32_t oblock = static_cast<uint32_t>(a->opt("owner_block", 256));
+    // precision 32: FP32 accumulation (scalar reference-tensor forms only)
+    const int64_t prec = a->opt("precision", 64);
+    MF_CHECK(prec == 64 || (prec == 32 && owner->k_owner32), MF_EUNSUPPORTED,
+             "precision " + std::to_string(prec) +
+                 " on simplices needs a scalar reference-tensor form (precision 32) or 64");
+    hipFunction_t kown = prec == 32 ? owner->k_owner32 : owner->k_owner;
+    int64_t& ogrid = prec == 32 ? owner->owner_grid32 : owner->owner_grid;
     int64_t grid = per * nx;
     if (a->opt("owner_persistent", 1)) {
-      if (owner->owner_grid == 0) {
+      if (ogrid == 0) {
         int nblk = 0;
-        MF_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, owner->k_owner, oblock, 0));
-        owner->owner_grid = std::max<int64_t>(nx, (int64_t)std::max(1, nblk) * a->ctx->num_cus / nx * nx);
+        MF_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, kown, oblock, 0));
+        ogrid = std::max<int64_t>(nx, (int64_t)std::max(1, nblk) * a->ctx->num_cus / nx * nx);
       }
-      grid = std::min<int64_t>(grid, owner->owner_grid * a->opt("owner_waves", 1));
+      grid = std::min<int64_t>(grid, ogrid * a->opt("owner_waves", 1));
     }
-    launch(owner->k_owner, static_cast<uint32_t>(grid), oblock, 0, s,
+    launch(kown, static_cast<uint32_t>(grid), oblock, 0, s,
            static_cast<int32_t>(plan->nchunks), plan->chunk_meta.ptr, plan->item_cell.ptr,
            plan->pair_list.ptr, plan->pair_kk.ptr, owner->records.ptr, a->consts.ptr, A->rpb.ptr,
            A->colsb.ptr, n0, A->values, diagonal, flags);

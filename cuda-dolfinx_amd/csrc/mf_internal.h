@@ -199,6 +199,8 @@ struct Integral {
   uint32_t scl_grid = 1;              // rank 0: workgroups (one partial each)
   DevArray<double> records;           // per-cell records of the owner-computes kernel
   int64_t owner_grid = 0;             // resident workgroups of the owner kernel (0: not queried)
+  hipFunction_t k_owner32 = nullptr;  // FP32-accumulation owner kernel (scalar reference-tensor forms)
+  int64_t owner_grid32 = 0;
   int64_t record_bytes = 0;
   // lifting entity lists per bc object uid
   std::map<uint64_t, std::pair<DevArray<int32_t>, int64_t>> lift_lists;

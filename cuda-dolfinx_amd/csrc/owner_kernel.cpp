@@ -36,6 +36,12 @@ const char* kOwnerCommon = R"(
 #define MF_SHARED_BEGIN do {} while (0)
 #define MF_SHARED(T, name, count) __shared__ __attribute__((aligned(16))) T name[count]
 typedef double mf_d2 __attribute__((ext_vector_type(2)));
+typedef float mf_f2 __attribute__((ext_vector_type(2)));
+// MF_OWNER_F32 (defined 0 or 1 before each body): 1 accumulates the element
+// contributions of a scalar reference-tensor form in FP32 (reference tensors,
+// element arithmetic and the LDS accumulator); geometry stays FP64 and the
+// CSR values are written as FP64
+#define MF_OWNER_F32 0
 static __device__ __forceinline__ int mf_lds_lower_bound(const int32_t* a, int lo, int hi, int32_t key)
 {
   while (lo < hi) {
@@ -178,7 +184,13 @@ $OHEAD($OPARAMS)
   constexpr int PF_P = (MAXP + NT - 1) / NT, PF_K = (KK4 + NT - 1) / NT;
   constexpr int PF_R = (MAXR + 1 + NT - 1) / NT, PF_C = (CAPB + NT - 1) / NT;
   constexpr int PF_B = (MAXR + NT - 1) / NT;
-  MF_SHARED(double, acc, CAP);
+#if MF_OWNER_F32
+  typedef float mf_acc_t;
+#else
+  typedef double mf_acc_t;
+#endif
+  static_assert(CAP * sizeof(mf_acc_t) % 16 == 0, "chunk_values: the accumulator is zeroed in 16-byte words");
+  MF_SHARED(mf_acc_t, acc, CAP);
   MF_SHARED(int32_t, lrp, MAXR + 1);
   MF_SHARED(uint8_t, snb, MAXR);                     // Dirichlet bits of the chunk's rows
   MF_SHARED(int32_t, lcols, CAPB);
@@ -186,8 +198,8 @@ $OHEAD($OPARAMS)
   MF_SHARED(uint8_t, skk, KK4 * 16);
 #ifdef SK_TRACE
   // reference tensors R_ij (symmetrised, 6 per entry), loaded once per workgroup
-  MF_SHARED(double, sRS, ND * ND * 6);
-  for (int p = threadIdx.x; p < ND * ND * 6; p += NT) sRS[p] = (&SK_RS[0][0][0])[p];
+  MF_SHARED(mf_acc_t, sRS, ND * ND * 6);
+  for (int p = threadIdx.x; p < ND * ND * 6; p += NT) sRS[p] = (mf_acc_t)(&SK_RS[0][0][0])[p];
 #elif defined(SK_FULLRT)
   // full reference tensors R_ij[3 r + s] (9 per entry), loaded once per workgroup
   MF_SHARED(double, sRT, ND * ND * 9);
@@ -214,9 +226,8 @@ $OHEAD($OPARAMS)
     for (int p = tid; p < n4; p += nt) reinterpret_cast<int4*>(skk)[p] = src[p];
   }
   {
-    mf_d2* a2 = reinterpret_cast<mf_d2*>(acc);
-    const mf_d2 z = {0.0, 0.0};
-    for (int p = tid; p < CAP / 2; p += nt) a2[p] = z;
+    int4* a4 = reinterpret_cast<int4*>(acc);
+    for (int p = tid; p < CAP * (int)sizeof(mf_acc_t) / 16; p += nt) a4[p] = make_int4(0, 0, 0, 0);
   }
   for (int p = tid; p <= m0.y - m0.x; p += nt) lrp[p] = rpb[m0.x + p] - m1.x;
   for (int p = tid; p < m0.y - m0.x; p += nt) snb[p] = nb0 ? nb0[m0.x + p] : 0;
@@ -346,6 +357,10 @@ $OHEAD($OPARAMS)
         const double M00 = dt * (K0 * K0 + K1 * K1 + K2 * K2), M11 = dt * (K3 * K3 + K4 * K4 + K5 * K5);
         const double M22 = dt * (K6 * K6 + K7 * K7 + K8 * K8), M01 = dt * (K0 * K3 + K1 * K4 + K2 * K5);
         const double M02 = dt * (K0 * K6 + K1 * K7 + K2 * K8), M12 = dt * (K3 * K6 + K4 * K7 + K5 * K8);
+#if MF_OWNER_F32
+        const float f00 = (float)M00, f11 = (float)M11, f22 = (float)M22;
+        const float f01 = (float)M01, f02 = (float)M02, f12 = (float)M12;
+#endif
 #elif defined(SK_FULLRT)
         // G_ij = |det J| K^T R_ij K; the left factor carries |det J|
         const double L0 = dt * K0, L1 = dt * K1, L2 = dt * K2, L3 = dt * K3, L4 = dt * K4;
@@ -377,8 +392,12 @@ $OHEAD($OPARAMS)
           double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #ifdef SK_TRACE
           {
-            const double* R = &sRS[(i * ND + j) * 6];
+            const mf_acc_t* R = &sRS[(i * ND + j) * 6];
+#if MF_OWNER_F32
+            G[0] = (double)(f00 * R[0] + f11 * R[1] + f22 * R[2] + f01 * R[3] + f02 * R[4] + f12 * R[5]);
+#else
             G[0] = M00 * R[0] + M11 * R[1] + M22 * R[2] + M01 * R[3] + M02 * R[4] + M12 * R[5];
+#endif
           }
 #elif defined(SK_FULLRT)
           {
@@ -416,13 +435,13 @@ $OHEAD($OPARAMS)
           const int rb = rbase + BS * kk[j];
           if ((rb0 | cb1) == 0) {
             for (int a = 0; a < BS; ++a)
-              for (int bb = 0; bb < BS; ++bb) atomicAdd(&acc[rb + a * rowlen + bb], blk[a * BS + bb]);
+              for (int bb = 0; bb < BS; ++bb) atomicAdd(&acc[rb + a * rowlen + bb], (mf_acc_t)blk[a * BS + bb]);
           } else {
             for (int a = 0; a < BS; ++a) {
               if (rb0 & (1u << a)) continue;
               for (int bb = 0; bb < BS; ++bb) {
                 if (cb1 & (1u << bb)) continue;
-                atomicAdd(&acc[rb + a * rowlen + bb], blk[a * BS + bb]);
+                atomicAdd(&acc[rb + a * rowlen + bb], (mf_acc_t)blk[a * BS + bb]);
               }
             }
           }
@@ -437,6 +456,33 @@ $OHEAD($OPARAMS)
 
     // write the segment once and re-zero what was read
     double* out = values + (int64_t)(uint32_t)base;
+#if MF_OWNER_F32
+    if ((flags & 1) && ((base & 1) == 0)) {
+      mf_d2* o2 = reinterpret_cast<mf_d2*>(out);
+      mf_f2* f2 = reinterpret_cast<mf_f2*>(acc);
+      const mf_f2 z = {0.0f, 0.0f};
+      for (int p = tid; p < seglen / 2; p += nt) {
+        const mf_f2 v = f2[p];
+        const mf_d2 d = {(double)v.x, (double)v.y};
+        __builtin_nontemporal_store(d, &o2[p]);
+        f2[p] = z;
+      }
+      if ((seglen & 1) && tid == 0) {
+        out[seglen - 1] = (double)acc[seglen - 1];
+        acc[seglen - 1] = 0.0f;
+      }
+    } else if (flags & 1) {
+      for (int p = tid; p < seglen; p += nt) {
+        __builtin_nontemporal_store((double)acc[p], &out[p]);
+        acc[p] = 0.0f;
+      }
+    } else {
+      for (int p = tid; p < seglen; p += nt) {
+        out[p] += (double)acc[p];
+        acc[p] = 0.0f;
+      }
+    }
+#else
     mf_d2* a2 = reinterpret_cast<mf_d2*>(acc);
     const mf_d2 z = {0.0, 0.0};
     if ((flags & 1) && ((base & 1) == 0)) {
@@ -460,6 +506,7 @@ $OHEAD($OPARAMS)
         acc[p] = 0.0;
       }
     }
+#endif
     if (!more) break;
     // the lists of this chunk were last read before the barrier above, so the
     // next chunk's lists can be stored without another barrier

@@ -132,7 +132,9 @@ int mf_form_compile(mf_form* form, int verbose);
 int mf_form_pack_coefficients(mf_form* form, const int* indices, int n, mf_stream stream);
 /* Tuning knobs: "matrix_kernel" (0 auto, 1 per-entity atomic scatter,
  * 2 cooperative owner-computes), "chunk_values" (owner-computes LDS budget
- * in doubles), "block" (threads per workgroup). */
+ * in values), "block" (threads per workgroup), "precision" (64; 32 / 16 on
+ * the hexahedral matrix-core kernel, 32 on the simplex owner-computes kernel
+ * of scalar reference-tensor forms: MF_EUNSUPPORTED at assembly otherwise). */
 int mf_form_set_option(mf_form* form, const char* key, int64_t value);
 /* Block / restricted assembly (CUDAForm::set_restriction, CUDAForm.h:201-231,
  * and the offset + restriction unrolling of CUDADofMap::update,

@@ -484,6 +484,54 @@ def test_many_instances_one_launch():
         assert np.array_equal(vals[~free], v0[~free])
 
 
+def test_reference_tensor_fp32():
+    _say("scalar reference-tensor owner kernel in FP32 (precision=32): P1-P3 Poisson groups, "
+         "alone and in one launch")
+    from mifem import batch
+    import cudolfinx as cufem
+    B = batch.many_poisson(6, seed=3, nmin=2, nmax=4)
+    asm = cufem.CUDAAssembler()
+    forms32, mats, bcs, refs = [], [], [], []
+    for g in B.groups:
+        a = cufem.form(g.a, options={"matrix_kernel": 2, "precision": 32})
+        dbc = asm.pack_bcs(g.bcs)
+        A = asm.assemble_matrix(a, bcs=dbc)
+        asm.synchronize()
+        Oa = cpu.OracleAssembler(g.a)
+        orp, ocols, _, _ = Oa.sparsity()
+        ovals = Oa.assemble_matrix(orp, ocols, g.bcs)
+        rp, cols, vals = A.csr()
+        assert np.array_equal(rp, orp) and np.array_equal(cols, ocols)
+        ea = _rel(vals, ovals)
+        _say(f"  P{g.degree} alone: nnz={cols.size} relerr A={ea:.3e}")
+        assert 0 < ea <= 1e-6, ea               # FP32 arithmetic, not the FP64 kernel
+        forms32.append(a)
+        mats.append(asm.create_matrix(a))
+        bcs.append(dbc)
+        refs.append(ovals)
+    fused = cufem.CUDABatch(forms32)
+    fused.assemble(asm, mats, bcs)
+    asm.synchronize()
+    for g, A, ovals in zip(B.groups, mats, refs):
+        ea = _rel(A.csr()[2], ovals)
+        _say(f"  P{g.degree} in one launch: relerr A={ea:.3e}")
+        assert 0 < ea <= 1e-6, ea
+
+
+def test_precision_errors():
+    _say("precision 32 outside the scalar reference-tensor owner path is rejected")
+    import cudolfinx as cufem
+    P = problems.elasticity_p2(2)
+    asm = cufem.CUDAAssembler()
+    a = cufem.form(P.a, options={"precision": 32})
+    with pytest.raises(NotImplementedError):
+        asm.assemble_matrix(a, bcs=P.bcs)
+    Q = problems.poisson_p1(3)
+    q = cufem.form(Q.a, options={"precision": 32, "matrix_kernel": 1})
+    with pytest.raises(NotImplementedError):
+        asm.assemble_matrix(q, bcs=Q.bcs)
+
+
 # ------------------------------------------------------------ block / restricted assembly
 def _selector(V, keep):
     """Scalar-dof selection matrix of the kept nodes (in order), bs-unrolled."""

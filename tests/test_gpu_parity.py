@@ -484,6 +484,42 @@ def test_many_instances_one_launch():
         assert np.array_equal(vals[~free], v0[~free])
 
 
+def test_one_launch_mixed_block_sizes():
+    _say("one launch for vector P2 elasticity (bs 3) and scalar P1 Poisson (bs 1)")
+    import cudolfinx as cufem
+    Ps = [problems.elasticity_p2(3), problems.poisson_p1(5)]
+    asm = cufem.CUDAAssembler()
+    fs = [cufem.form(P.a) for P in Ps]
+    bcs = [asm.pack_bcs(P.bcs) for P in Ps]
+    mats = [asm.create_matrix(a) for a in fs]
+    cufem.CUDABatch(fs).assemble(asm, mats, bcs)
+    asm.synchronize()
+    for P, A in zip(Ps, mats):
+        rp, cols, vals = A.csr()
+        orp, ocols, ovals, _ = _oracle(P)
+        assert np.array_equal(rp, orp) and np.array_equal(cols, ocols)
+        ea = _rel(vals, ovals)
+        _say(f"  {P.name}: nnz={cols.size} relerr A={ea:.3e}")
+        assert ea <= TOL, ea
+
+
+def test_one_launch_errors():
+    _say("one-launch batch: unsupported members and mismatched arguments are rejected")
+    import cudolfinx as cufem
+    H = problems.helmholtz_q3(2)
+    with pytest.raises(NotImplementedError):
+        cufem.CUDABatch([cufem.form(H.a)])
+    P = problems.poisson_p1(3)
+    a = cufem.form(P.a)
+    asm = cufem.CUDAAssembler()
+    fused = cufem.CUDABatch([a])
+    with pytest.raises(ValueError):
+        fused.assemble(asm, [asm.create_matrix(a), asm.create_matrix(a)])
+    Q = problems.poisson_p1(4)
+    with pytest.raises(ValueError):           # matrix of a different mesh
+        fused.assemble(asm, [asm.create_matrix(cufem.form(Q.a))])
+
+
 def test_reference_tensor_fp32():
     _say("scalar reference-tensor owner kernel in FP32 (precision=32): P1-P3 Poisson groups, "
          "alone and in one launch")

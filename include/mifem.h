@@ -225,7 +225,8 @@ int mf_assemble_scalar_device(mf_form* M, double* d_value, int zero, mf_stream s
  * with one cell integral on the owner-computes path, assembled by ONE
  * persistent launch (after one per-cell record pass per group).  The forms
  * keep their own compile-time shapes; they must share owner_block.  At most
- * 32 forms (each passes 88 bytes of kernel arguments). */
+ * 32 forms (each passes 88 bytes of kernel arguments).  The batch refers to
+ * the forms: destroy it before any of them. */
 int mf_batch_create(mf_form* const* forms, int n, mf_batch** out);
 /* Assembles form g into As[g] (from mf_matrix_create(forms[g])) with rows
  * and columns of bcs[g] (bcs or bcs[g] may be NULL) masked and, when zero !=

@@ -83,6 +83,16 @@ def _declare(L):
                               C.c_int, C.c_int64, C.POINTER(_ip), C.POINTER(_ip),
                               C.POINTER(C.c_int64), C.POINTER(_ip), C.POINTER(_ip)]
     L.or_free.argtypes = [C.c_void_p]
+    _i8 = C.POINTER(C.c_int8)
+    _lp = C.POINTER(C.c_int64)
+    L.or_colour_entities.argtypes = [C.POINTER(Problem), C.POINTER(Entities), C.c_int64, _i8]
+    L.or_assemble_matrix_mt.argtypes = [C.POINTER(Problem), C.c_int, C.POINTER(Kernel),
+                                        C.POINTER(Entities), C.POINTER(_lp), C.POINTER(_lp),
+                                        C.POINTER(C.c_int), C.c_int, _i8, _i8, _ip, _ip, _dp,
+                                        C.c_int, C.c_double, C.c_int64]
+    L.or_assemble_vector_mt.argtypes = [C.POINTER(Problem), C.c_int, C.POINTER(Kernel),
+                                        C.POINTER(Entities), C.POINTER(_lp), C.POINTER(_lp),
+                                        C.POINTER(C.c_int), C.c_int, _dp]
     L.or_assemble_matrix.argtypes = [C.POINTER(Problem), C.c_int, C.POINTER(Kernel),
                                      C.POINTER(Entities), _i8, _i8, _ip, _ip, _dp, C.c_int,
                                      C.c_double, C.c_int64]
@@ -91,6 +101,10 @@ def _declare(L):
                                      C.POINTER(Entities), _dp]
     L.or_apply_lifting.argtypes = [C.POINTER(Problem), C.c_int, C.POINTER(Kernel),
                                    C.POINTER(Entities), _i8, _dp, _dp, C.c_double, _dp]
+    L.or_apply_lifting_mt.argtypes = [C.POINTER(Problem), C.c_int, C.POINTER(Kernel),
+                                      C.POINTER(Entities), C.POINTER(_lp), C.POINTER(_lp),
+                                      C.POINTER(C.c_int), C.c_int, _i8, _dp, _dp, C.c_double,
+                                      _dp]
     L.or_assemble_scalar.argtypes = [C.POINTER(Problem), C.c_int, C.POINTER(Kernel),
                                      C.POINTER(Entities)]
     L.or_assemble_scalar.restype = C.c_double
@@ -286,6 +300,64 @@ class OracleAssembler:
             lib().or_free(ptr)
         return row_ptr, c, row_ptr_b, cb
 
+    # multi-core assembly (CPU baseline) -------------------------------
+    def colouring(self):
+        """Per integral: (entities ordered colour by colour, colour starts),
+        no two entities of a colour sharing a row dof.  Computed once."""
+        if getattr(self, "_colouring", None) is None:
+            out = []
+            nrows = self.V0.num_nodes * self.V0.bs
+            for k, I in enumerate(self.form.integrals):
+                col = np.zeros(max(I.num_entities, 1), dtype=np.int8)
+                nc = lib().or_colour_entities(C.byref(self.P), C.byref(self.ents[k]), nrows,
+                                              _p(col, C.c_int8))
+                if nc < 0:
+                    raise RuntimeError("entity colouring needs more than 64 colours")
+                col = col[:I.num_entities]
+                order = np.argsort(col, kind="stable").astype(np.int64)
+                starts = np.zeros(nc + 1, dtype=np.int64)
+                starts[1:] = np.cumsum(np.bincount(col, minlength=nc))
+                out.append((order, starts, nc))
+            self._colouring = out
+        return self._colouring
+
+    def _colour_args(self):
+        cl = self.colouring()
+        n = len(cl)
+        lp = C.POINTER(C.c_int64)
+        orders = (lp * n)(*[_p(o, C.c_int64) for o, _, _ in cl])
+        starts = (lp * n)(*[_p(st, C.c_int64) for _, st, _ in cl])
+        ncol = (C.c_int * n)(*[nc for _, _, nc in cl])
+        return orders, starts, ncol
+
+    def assemble_matrix_mt(self, row_ptr, cols, bcs=(), diagonal=1.0, threads=None):
+        """assemble_matrix on `threads` cores (default: all), entities in colours."""
+        threads = threads or os.cpu_count() or 1
+        n0 = self.V0.size
+        bc0 = _markers(bcs, self.V0)
+        bc1 = _markers(bcs, self.V1)
+        vals = np.zeros(cols.size)
+        row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        orders, starts, ncol = self._colour_args()
+        missing = lib().or_assemble_matrix_mt(
+            C.byref(self.P), len(self.form.integrals), self.kernels, self.ents, orders, starts,
+            ncol, int(threads), _p(bc0, C.c_int8), _p(bc1, C.c_int8), _p(row_ptr, C.c_int32),
+            _p(cols, C.c_int32), _p(vals, C.c_double),
+            int(self.V0 is self.V1 and bc0 is not None), diagonal, n0)
+        if missing:
+            raise RuntimeError(f"{missing} element entries missing from the sparsity pattern")
+        return vals
+
+    def assemble_vector_mt(self, threads=None):
+        threads = threads or os.cpu_count() or 1
+        b = np.zeros(self.V0.size)
+        orders, starts, ncol = self._colour_args()
+        lib().or_assemble_vector_mt(C.byref(self.P), len(self.form.integrals), self.kernels,
+                                    self.ents, orders, starts, ncol, int(threads),
+                                    _p(b, C.c_double))
+        return b
+
     # assembly ---------------------------------------------------------
     def assemble_matrix(self, row_ptr, cols, bcs=(), diagonal=1.0, values=None):
         n0 = self.V0.size
@@ -315,15 +387,23 @@ class OracleAssembler:
         return float(lib().or_assemble_scalar(C.byref(self.P), len(self.form.integrals),
                                               self.kernels, self.ents))
 
-    def apply_lifting(self, b, bcs, x0=None, scale=1.0):
+    def apply_lifting(self, b, bcs, x0=None, scale=1.0, threads=1):
+        """b -= scale A[:, bc](g - x0); threads > 1 runs the coloured multi-core loop."""
         bc1 = _markers(bcs, self.V1)
         if bc1 is None:
             return b
         g = _dense_values(bcs, self.V1)
         x0 = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+        x0p = _p(x0, C.c_double) if x0 is not None else None
+        if threads > 1:
+            orders, starts, ncol = self._colour_args()
+            lib().or_apply_lifting_mt(C.byref(self.P), len(self.form.integrals), self.kernels,
+                                      self.ents, orders, starts, ncol, int(threads),
+                                      _p(bc1, C.c_int8), _p(g, C.c_double), x0p, scale,
+                                      _p(b, C.c_double))
+            return b
         lib().or_apply_lifting(C.byref(self.P), len(self.form.integrals), self.kernels, self.ents,
-                               _p(bc1, C.c_int8), _p(g, C.c_double),
-                               _p(x0, C.c_double) if x0 is not None else None, scale,
+                               _p(bc1, C.c_int8), _p(g, C.c_double), x0p, scale,
                                _p(b, C.c_double))
         return b
 

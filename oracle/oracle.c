@@ -5,7 +5,8 @@
  * assembly engine: tests, the smoke check and the CPU-baseline leg of the
  * benchmark call it; the product never links or calls it.
  *
- * It restates, in plain serial C:
+ * It restates, in plain serial C (plus a coloured multi-threaded copy of
+ * the matrix and vector loops, timed as the multi-core CPU baseline):
  *   - quadrature (Gauss-Jacobi / Gauss-Legendre by Newton on the Jacobi
  *     recurrence, collapsed tetrahedron and triangle rules);
  *   - Lagrange bases (P0-P3 tetrahedra by closed-form barycentric
@@ -370,7 +371,7 @@ static int quad_cell(const or_kernel* K, double* pts, double* wts)
 static void tab_cell(const or_kernel* K, double* A, const double* w, const double* c,
                      const double* xc)
 {
-  static double pts[3 * OR_MAXQ], wts[OR_MAXQ];
+  static _Thread_local double pts[3 * OR_MAXQ], wts[OR_MAXQ];
   const int nd = or_ndofs(K->cell, K->degree), bs = K->bs, n = nd * bs;
   const int nq = quad_cell(K, pts, wts);
   double phi[OR_MAXND], dref[3 * OR_MAXND], g[OR_MAXND][3], Kinv[3][3];
@@ -503,7 +504,7 @@ static void phys_grad(const or_kernel* K, const double* xc, const double* X, dou
 
 static void tab_dg_cell(const or_kernel* K, double* A, const double* w, const double* xc)
 {
-  static double pts[3 * OR_MAXQ], wts[OR_MAXQ];
+  static _Thread_local double pts[3 * OR_MAXQ], wts[OR_MAXQ];
   const int nd = or_ndofs(0, K->degree);
   const int nq = or_tet_rule(K->qm, pts, wts);
   double phi[OR_MAXND], g[OR_MAXND][3], adet;
@@ -617,7 +618,7 @@ static double eval_at(int k, const double* X, const double* coef)
 
 static void tab_fn_cell(const or_kernel* K, double* A, const double* w, const double* xc)
 {
-  static double pts[3 * OR_MAXQ], wts[OR_MAXQ];
+  static _Thread_local double pts[3 * OR_MAXQ], wts[OR_MAXQ];
   const int nq = or_tet_rule(K->qm, pts, wts);
   double Kinv[3][3];
   for (int q = 0; q < nq; ++q) {
@@ -663,7 +664,7 @@ static void tab_fn_int(const or_kernel* K, double* A, const double* w, const dou
 static void tab_mass(const or_kernel* K, double* A, const double* w, const double* c,
                      const double* xc)
 {
-  static double pts[3 * OR_MAXQ], wts[OR_MAXQ];
+  static _Thread_local double pts[3 * OR_MAXQ], wts[OR_MAXQ];
   const int nq = or_tet_rule(K->qm, pts, wts);
   const int nv = or_ndofs(0, K->degree), nw = or_ndofs(0, K->f_degree);
   double Kinv[3][3], phi[OR_MAXND], psi[OR_MAXND], d[3 * OR_MAXND];
@@ -685,7 +686,7 @@ static void tab_mass(const or_kernel* K, double* A, const double* w, const doubl
 
 static void tab_nl_cell(const or_kernel* K, double* A, const double* w, const double* xc)
 {
-  static double pts[3 * OR_MAXQ], wts[OR_MAXQ];
+  static _Thread_local double pts[3 * OR_MAXQ], wts[OR_MAXQ];
   const int nd = or_ndofs(0, K->degree);
   const int nq = or_tet_rule(K->qm, pts, wts);
   double phi[OR_MAXND], g[OR_MAXND][3], adet;
@@ -775,7 +776,7 @@ void or_tabulate(const or_kernel* K, double* A, const double* w, const double* c
 {
   if (K->diag_part) {
     /* the diagonal of the kind's element matrix */
-    static double T[4 * OR_MAXND * OR_MAXND];
+    static _Thread_local double T[4 * OR_MAXND * OR_MAXND];
     or_kernel M = *K;
     M.diag_part = 0;
     M.rank = 2;
@@ -1076,6 +1077,161 @@ double or_assemble_scalar(const or_problem* P, int nint, const or_kernel* kernel
     }
   }
   return total;
+}
+
+/* ---------------------------------------------------------------- multi-threaded
+ * The same matrix and vector assembly on several cores, for the CPU
+ * baseline (DOLFINx itself assembles serially per MPI rank).  Entities are
+ * coloured so that no two entities of one colour share a row dof; each colour
+ * is one parallel loop, so every value is written by one thread at a time.
+ * Within a value, contributions arrive colour by colour instead of entity by
+ * entity, so results match the serial functions to rounding. */
+
+/* Greedy colouring of integral `it`'s entities by their row dofs (dofs0):
+ * colours[e] in [0, 64).  Returns the number of colours, -1 if 64 do not
+ * suffice, -2 if out of memory. */
+int or_colour_entities(const or_problem* P, const or_entities* ent, int64_t nrows,
+                       int8_t* colours)
+{
+  uint64_t* used = (uint64_t*)calloc((size_t)(nrows > 0 ? nrows : 1), sizeof(uint64_t));
+  if (!used) return -2;
+  double xc[2 * 27 * 3], w[1024];
+  int32_t dofs0[OR_MAXT], dofs1[OR_MAXT];
+  int eli[2], ncol = 0;
+  const int s = stride_of(ent->type);
+  for (int64_t e = 0; e < ent->n; ++e) {
+    int ncell = gather(P, ent->type, &ent->entities[e * s], xc, w, dofs0, dofs1, eli);
+    int nr = ncell * P->nd0 * P->bs0;
+    uint64_t mask = 0;
+    for (int i = 0; i < nr; ++i) mask |= used[dofs0[i]];
+    if (mask == ~(uint64_t)0) { free(used); return -1; }
+    int c = 0;
+    while (mask & ((uint64_t)1 << c)) ++c;
+    for (int i = 0; i < nr; ++i) used[dofs0[i]] |= (uint64_t)1 << c;
+    colours[e] = (int8_t)c;
+    if (c + 1 > ncol) ncol = c + 1;
+  }
+  free(used);
+  return ncol;
+}
+
+/* or_assemble_matrix with `nthreads` threads: order[it] lists integral it's
+ * entities colour by colour, cstart[it][c] .. cstart[it][c + 1] those of
+ * colour c (ncol[it] colours). */
+int or_assemble_matrix_mt(const or_problem* P, int nint, const or_kernel* kernels,
+                          const or_entities* ints, const int64_t* const* order,
+                          const int64_t* const* cstart, const int* ncol, int nthreads,
+                          const int8_t* bc0, const int8_t* bc1, const int32_t* row_ptr,
+                          const int32_t* cols, double* values, int insert_diag, double diagonal,
+                          int64_t nrows)
+{
+  int missing = 0;
+  for (int it = 0; it < nint; ++it) {
+    const int s = stride_of(ints[it].type);
+    for (int c = 0; c < ncol[it]; ++c) {
+#pragma omp parallel num_threads(nthreads) reduction(+ : missing)
+      {
+        double* A = (double*)malloc(sizeof(double) * OR_MAXT * OR_MAXT);
+        double xc[2 * 27 * 3], w[1024];
+        int32_t dofs0[OR_MAXT], dofs1[OR_MAXT];
+        int eli[2];
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t k = cstart[it][c]; k < cstart[it][c + 1]; ++k) {
+          const int32_t* ent = &ints[it].entities[order[it][k] * s];
+          int ncell = gather(P, ints[it].type, ent, xc, w, dofs0, dofs1, eli);
+          int nr = ncell * P->nd0 * P->bs0, nc = ncell * P->nd1 * P->bs1;
+          memset(A, 0, sizeof(double) * nr * nc);
+          or_tabulate(&kernels[it], A, w, P->constants, xc, eli);
+          for (int i = 0; i < nr; ++i) {
+            int32_t row = dofs0[i];
+            if (bc0 && bc0[row]) continue;
+            for (int j = 0; j < nc; ++j) {
+              int32_t col = dofs1[j];
+              if (bc1 && bc1[col]) continue;
+              int64_t p = find(row_ptr, cols, row, col);
+              if (p < 0) { ++missing; continue; }
+              values[p] += A[i * nc + j];
+            }
+          }
+        }
+        free(A);
+      }
+    }
+  }
+  if (insert_diag && bc0)
+    for (int64_t r = 0; r < nrows; ++r)
+      if (bc0[r]) {
+        int64_t p = find(row_ptr, cols, (int32_t)r, (int32_t)r);
+        if (p >= 0) values[p] = diagonal;
+      }
+  return missing;
+}
+
+/* or_assemble_vector with `nthreads` threads, colours as above. */
+int or_assemble_vector_mt(const or_problem* P, int nint, const or_kernel* kernels,
+                          const or_entities* ints, const int64_t* const* order,
+                          const int64_t* const* cstart, const int* ncol, int nthreads, double* b)
+{
+  for (int it = 0; it < nint; ++it) {
+    const int s = stride_of(ints[it].type);
+    for (int c = 0; c < ncol[it]; ++c) {
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 256)
+      for (int64_t k = cstart[it][c]; k < cstart[it][c + 1]; ++k) {
+        double be[OR_MAXT], xc[2 * 27 * 3], w[1024];
+        int32_t dofs0[OR_MAXT], dofs1[OR_MAXT];
+        int eli[2];
+        const int32_t* ent = &ints[it].entities[order[it][k] * s];
+        int ncell = gather(P, ints[it].type, ent, xc, w, dofs0, dofs1, eli);
+        int nr = ncell * P->nd0 * P->bs0;
+        memset(be, 0, sizeof(double) * nr);
+        or_tabulate(&kernels[it], be, w, P->constants, xc, eli);
+        for (int i = 0; i < nr; ++i) b[dofs0[i]] += be[i];
+      }
+    }
+  }
+  return 0;
+}
+
+/* or_apply_lifting with `nthreads` threads, colours as above. */
+int or_apply_lifting_mt(const or_problem* P, int nint, const or_kernel* kernels,
+                        const or_entities* ints, const int64_t* const* order,
+                        const int64_t* const* cstart, const int* ncol, int nthreads,
+                        const int8_t* bc1, const double* g, const double* x0, double scale,
+                        double* b)
+{
+  for (int it = 0; it < nint; ++it) {
+    const int s = stride_of(ints[it].type);
+    for (int c = 0; c < ncol[it]; ++c) {
+#pragma omp parallel num_threads(nthreads)
+      {
+        double* A = (double*)malloc(sizeof(double) * OR_MAXT * OR_MAXT);
+        double be[OR_MAXT], xc[2 * 27 * 3], w[1024];
+        int32_t dofs0[OR_MAXT], dofs1[OR_MAXT];
+        int eli[2];
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t k = cstart[it][c]; k < cstart[it][c + 1]; ++k) {
+          const int32_t* ent = &ints[it].entities[order[it][k] * s];
+          int ncell = gather(P, ints[it].type, ent, xc, w, dofs0, dofs1, eli);
+          int nr = ncell * P->nd0 * P->bs0, nc = ncell * P->nd1 * P->bs1;
+          int has = 0;
+          for (int j = 0; j < nc; ++j) has |= bc1[dofs1[j]] != 0;
+          if (!has) continue;
+          memset(A, 0, sizeof(double) * nr * nc);
+          memset(be, 0, sizeof(double) * nr);
+          or_tabulate(&kernels[it], A, w, P->constants, xc, eli);
+          for (int j = 0; j < nc; ++j) {
+            int32_t col = dofs1[j];
+            if (!bc1[col]) continue;
+            double v = scale * (g[col] - (x0 ? x0[col] : 0.0));
+            for (int i = 0; i < nr; ++i) be[i] -= A[i * nc + j] * v;
+          }
+          for (int i = 0; i < nr; ++i) b[dofs0[i]] += be[i];
+        }
+        free(A);
+      }
+    }
+  }
+  return 0;
 }
 
 /* b -= scale * A[:, bc] (g - x0) over every integral of the bilinear form. */

@@ -678,3 +678,34 @@ def test_diagonal_part_is_matrix_diagonal():
         diag[rows[rows == cols]] = vals[rows == cols]
         got = cpu.OracleAssembler(diagonal(a)).assemble_vector()
         np.testing.assert_allclose(got, diag, atol=1e-14 * max(1.0, abs(diag).max()))
+
+
+@pytest.mark.parametrize("which", ["elasticity", "dg"])
+def test_multithreaded_oracle_matches_serial(which):
+    """The coloured multi-core assembly (CPU baseline) equals the serial oracle
+    to rounding: same entries, contributions summed colour by colour."""
+    P = problems.elasticity_p2(3) if which == "elasticity" else problems.dg_plaplace(3)
+    Oa, OL = cpu.OracleAssembler(P.a), cpu.OracleAssembler(P.L)
+    rp, cols, _, _ = Oa.sparsity()
+    ref = Oa.assemble_matrix(rp, cols, P.bcs)
+    got = Oa.assemble_matrix_mt(rp, cols, P.bcs, threads=4)
+    assert np.linalg.norm(got - ref) <= 1e-14 * np.linalg.norm(ref)
+    bref, bgot = OL.assemble_vector(), OL.assemble_vector_mt(threads=4)
+    assert np.linalg.norm(bgot - bref) <= 1e-14 * np.linalg.norm(bref)
+    g = np.random.default_rng(5).uniform(-1, 1, P.V.size)
+    for bc in P.bcs:
+        bc.values[bc.dofs] = g[bc.dofs]           # nonzero Dirichlet data
+    lref = Oa.apply_lifting(bref.copy(), P.bcs, scale=0.7)
+    lgot = Oa.apply_lifting(bref.copy(), P.bcs, scale=0.7, threads=4)
+    assert np.linalg.norm(lgot - lref) <= 1e-14 * np.linalg.norm(lref)
+    assert (np.linalg.norm(lref - bref) > 0) == bool(P.bcs)   # DG: Robin terms only
+    for (order, starts, nc), I in zip(Oa.colouring(), P.a.integrals):
+        assert starts[-1] == I.num_entities and nc <= 64
+        assert np.array_equal(np.sort(order), np.arange(I.num_entities))
+    if which == "elasticity":
+        # no two cells of one colour share a row dof
+        dm = P.V.unrolled_dofmap()
+        order, starts, nc = Oa.colouring()[0]
+        for c in range(nc):
+            d = dm[order[starts[c]:starts[c + 1]]].ravel()
+            assert np.unique(d).size == d.size

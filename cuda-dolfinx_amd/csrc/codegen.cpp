@@ -514,12 +514,18 @@ std::string generate_batch_source(const std::vector<const mf_form*>& forms,
           "#undef MF_OWNER_F32\n#define MF_OWNER_F32 0\n";
     lds = std::max(lds, owner_lds_bytes(f, I));
     {
-      const std::string a = "a" + std::to_string(g) + ".";
-      params += std::string(g ? ", " : "") + "const mf_batch_args a" + std::to_string(g);
+      // parameters in the order of kOwnerParams (and of the host's argument list)
+      const std::string a = "g" + std::to_string(g) + "_";
+      params += std::string(g ? ",\n    " : "") + "const int32_t " + a + "nchunks, const int32_t* __restrict__ " +
+                a + "chunk_meta, const int32_t* __restrict__ " + a + "item_cell, const int32_t* __restrict__ " +
+                a + "pair_list, const uint8_t* __restrict__ " + a + "pair_kk, const mf_rec_" + tag +
+                "* __restrict__ " + a + "rec, const double* __restrict__ " + a +
+                "consts, const int32_t* __restrict__ " + a + "rpb, const int32_t* __restrict__ " + a +
+                "colsb, const uint8_t* __restrict__ " + a + "nb0, double* __restrict__ " + a + "values";
       calls += "  mf_owner_body_g" + std::to_string(g) + "(" + a + "nchunks, " + a + "chunk_meta, " +
-               a + "item_cell, " + a + "pair_list, " + a + "pair_kk, reinterpret_cast<const mf_rec_" +
-               tag + "*>(" + a + "rec), " + a + "consts, " + a + "rpb, " + a + "colsb, " + a +
-               "nb0, " + a + "values, diag, flags" + sargs + ");\n  __syncthreads();\n";
+               a + "item_cell, " + a + "pair_list, " + a + "pair_kk, " + a + "rec, " + a + "consts, " +
+               a + "rpb, " + a + "colsb, " + a + "nb0, " + a + "values, diag, flags" + sargs +
+               ");\n  __syncthreads();\n";
     }
   }
   os << subst(kOwnerBatchKernel, {{"$CALLS", calls}, {"$PARAMS", params}, {"$OBLOCK", std::to_string(block)},

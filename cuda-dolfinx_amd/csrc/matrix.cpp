@@ -655,7 +655,7 @@ void assemble_vector(mf_form* L, double* b, int64_t n, int zero, hipStream_t s) 
 // Dirichlet rows/columns of bcs[g] masked and the diagonal set).
 void assemble_batch(mf_batch* B, mf_matrix* const* As, const mf_bc* const* bcs, double diag,
                     int zero, hipStream_t s) {
-  struct Args {   // layout of mf_batch_args in kOwnerBatchKernel (passed by value)
+  struct Args {   // one group's kernel arguments (see the pointer list below)
     const void* chunk_meta; const void* item_cell; const void* pair_list; const void* pair_kk;
     const void* rec; const void* consts; const void* rpb; const void* colsb; const void* nb0;
     void* values; int32_t nchunks; int32_t pad;
@@ -692,13 +692,22 @@ void assemble_batch(mf_batch* B, mf_matrix* const* As, const mf_bc* const* bcs, 
                 I->records.ptr, a->consts.ptr, A->rpb.ptr, A->colsb.ptr, n0, A->values,
                 static_cast<int32_t>(plan->nchunks), 0};
   }
-  // kernel arguments: the groups' structs by value, then diag and flags
-  // (hipModuleLaunchKernel copies each from its pointer per the kernel's
-  // argument layout)
+  // kernel arguments, per group in the order of kOwnerParams (nchunks,
+  // chunk_meta, ..., values), then diag and flags; hipModuleLaunchKernel
+  // copies each from its pointer per the kernel's argument layout
   int flags = zero ? 1 : 0;
   if (zero) flags |= 2;              // Dirichlet diagonal for the groups that have constraints
   std::vector<void*> ptrs;
-  for (size_t g = 0; g < ng; ++g) ptrs.push_back(&h[g]);
+  for (size_t g = 0; g < ng; ++g) {
+    Args& x = h[g];
+    for (void* p : {static_cast<void*>(&x.nchunks), static_cast<void*>(&x.chunk_meta),
+                    static_cast<void*>(&x.item_cell), static_cast<void*>(&x.pair_list),
+                    static_cast<void*>(&x.pair_kk), static_cast<void*>(&x.rec),
+                    static_cast<void*>(&x.consts), static_cast<void*>(&x.rpb),
+                    static_cast<void*>(&x.colsb), static_cast<void*>(&x.nb0),
+                    static_cast<void*>(&x.values)})
+      ptrs.push_back(p);
+  }
   ptrs.push_back(&diag);
   ptrs.push_back(&flags);
   MF_HIP(hipModuleLaunchKernel(B->k, static_cast<uint32_t>(B->grid), 1, 1,

@@ -77,17 +77,12 @@ const char* kOwnerParams =
 // workgroup runs each group's chunk schedule in turn (the groups' chunks are
 // spread over the same resident grid, so a small group's tail overlaps the
 // next group's work), reusing one LDS buffer sized for the largest group.
-// Each group's arguments come by value ($PARAMS: mf_batch_args a0, a1, ...):
-// pointer fields of by-value kernel arguments are known to address global
-// memory, where pointers read from a table in memory would be generic (flat
-// loads and stores, which also wait on the LDS counter).  $CALLS: one call
-// per group.
+// Each group's arguments come as separate kernel parameters ($PARAMS: g<k>_nchunks,
+// g<k>_chunk_meta, ... g<k>_values): __restrict__ pointer parameters of the
+// kernel are known not to alias the values it writes, so wave-uniform reads
+// through them (chunk metadata) can use scalar loads; the same pointers read
+// from a table or a by-value struct could not.  $CALLS: one call per group.
 const char* kOwnerBatchKernel = R"(
-struct mf_batch_args {
-  const int32_t* chunk_meta; const int32_t* item_cell; const int32_t* pair_list;
-  const uint8_t* pair_kk; const void* rec; const double* consts; const int32_t* rpb;
-  const int32_t* colsb; const uint8_t* nb0; double* values; int32_t nchunks; int32_t pad;
-};
 extern "C" __global__ void __launch_bounds__($OBLOCK)
 mf_owner_batch($PARAMS, const double diag, const int flags)
 {

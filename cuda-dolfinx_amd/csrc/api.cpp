@@ -355,12 +355,16 @@ void compile_form(mf_form* f, int verbose) {
       f->consts.upload(&z, 1);
     }
     if (f->W > 0) f->packed.alloc(f->mesh->num_cells * (int64_t)f->W);
-    // owner-kernel shape: LDS budget (doubles) and cells per batch
-    // scalar forms on the reference-tensor path keep a (nd x nd x 6) table in
-    // LDS: give them a 2048-value chunk unless the caller chose one
-    bool trace = false;
-    for (auto& I : f->ints) trace |= mf::staged_trace(*I);
-    int64_t cap = f->opt("chunk_values", trace ? 2048 : 4096);
+    // owner-kernel shape: LDS budget (values) and cells per batch.  Scalar
+    // forms on the reference-tensor path keep a (nd x nd x 6) table in LDS:
+    // 2048 values; other simplex forms 4608 (the fastest of the tuned shapes
+    // for vector P2 elasticity); hexahedra 4096.  A caller's choice wins.
+    bool trace = false, hex = false;
+    for (auto& I : f->ints) {
+      trace |= mf::staged_trace(*I);
+      hex |= mf::is_hex_staged(*I);
+    }
+    int64_t cap = f->opt("chunk_values", trace ? 2048 : (hex ? 4096 : 4608));
     f->options["chunk_values"] = cap;
     if (!f->options.count("owner_nb")) f->options["owner_nb"] = 32;
     if (!f->options.count("chunk_rows")) f->options["chunk_rows"] = 64;
@@ -920,6 +924,7 @@ int mf_autotune_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc, int reps, mf_s
         {{"chunk_values", 3072}, {"owner_pairs", 256}, {"owner_block", 256}},
         {{"chunk_values", 6144}, {"owner_pairs", 512}, {"owner_block", 256}, {"owner_lds", 98304}},
         {{"chunk_values", 2048}, {"owner_pairs", 256}, {"owner_block", 256}},
+        {{"chunk_values", 4608}, {"owner_pairs", 384}, {"owner_block", 256}, {"owner_lds", 73728}},
     };
     const auto base = a->options;
     auto drop_plans = [&] {

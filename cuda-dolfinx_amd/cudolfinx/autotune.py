@@ -25,6 +25,7 @@ DEFAULT_CANDIDATES = [
     {"chunk_values": 4096, "owner_pairs": 256, "owner_block": 256},
     {"chunk_values": 4096, "owner_pairs": 256, "owner_block": 256, "owner_wpe": 4},
     {"chunk_values": 4096, "owner_pairs": 384, "owner_block": 256},
+    {"chunk_values": 4608, "owner_pairs": 384, "owner_block": 256, "owner_lds": 73728},
     {"chunk_values": 5120, "owner_pairs": 384, "owner_block": 256, "owner_lds": 81920},
     {"chunk_values": 3072, "owner_pairs": 256, "owner_block": 256},
     {"chunk_values": 6144, "owner_pairs": 512, "owner_block": 256, "owner_lds": 98304},

@@ -673,7 +673,7 @@ def main():
     ap.add_argument("--c5-fp32", action="store_true",
                     help="c5: degree >= 2 groups accumulate in FP32 (slower on this GPU: FP32 LDS atomics)")
     ap.add_argument("--n-dg", type=int, default=48, help="cubes per direction for c4")
-    ap.add_argument("--newton", type=int, default=10, help="Newton steps for c4")
+    ap.add_argument("--newton", type=int, default=4, help="Newton steps for c4 (after the timed region)")
     ap.add_argument("--n-hex", type=int, default=48, help="hexahedra per direction for c3")
     ap.add_argument("--cg-maxiter", type=int, default=4000)
     ap.add_argument("--copy-calib", type=int, default=0,

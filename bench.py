@@ -11,8 +11,11 @@ outside the timed region.  Inputs live in HBM before timing starts.
 
 Printed: one JSON line with DoF/s (value), the roofline of the dominant
 kernel (owner-computes matrix kernel, timed live with HIP events on the
-stream it runs on) and a CPU baseline (the oracle, one core, on a bounded
-sample of the same workload).
+stream it runs on, beside the copy bandwidth measured on the same GPU) and a
+CPU baseline (the oracle on all usable cores, at most 16, and on one core,
+on a bounded sample of the same workload).  --workload c3 / c4 / c5 run the
+Q3 Helmholtz precision comparison, the fused SIPG Newton step and the
+512-instance batch instead.
 """
 from __future__ import annotations
 
@@ -680,8 +683,8 @@ def main():
                     help="also run a plain device copy of this many MiB (PMC traffic calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
-                          "n_gpus": args.gpus}))
+        print(json.dumps({"error": "bench.py runs the assembly on one GPU; --gpus above 1 is "
+                                   "not supported", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.workload == "c3":
         bench_c3(args)

@@ -267,3 +267,30 @@ def test_library_exports_every_declared_symbol():
     rc = L.mf_context_create(0, None, C.byref(h))
     if rc != 0:
         assert L.mf_last_error()
+
+
+def test_null_arguments_rejected_without_a_device():
+    """Entry points validate their arguments before any device work: null or
+    empty arguments return MF_EINVAL with a message (no GPU needed)."""
+    lib_path = ROOT / "cuda-dolfinx_amd" / "lib" / "libmifem.so"
+    if not lib_path.exists():
+        pytest.skip("libmifem.so not built")
+    from cudolfinx import _lib
+    L = _lib.lib()
+    n = C.c_int64(0)
+    calls = {
+        "mf_batch_create": lambda: L.mf_batch_create(None, 0, None),
+        "mf_batch_assemble": lambda: L.mf_batch_assemble(None, None, None, 1.0, 1, None),
+        "mf_batch_source": lambda: L.mf_batch_source(None, None, C.byref(n)),
+        "mf_system_create": lambda: L.mf_system_create(None, None, None, 0, None),
+        "mf_form_set_option": lambda: L.mf_form_set_option(None, b"chunk_values", 1),
+        "mf_matrix_info": lambda: L.mf_matrix_info(None, None, None, None),
+        "mf_interpolator_create": lambda: L.mf_interpolator_create(None, None, None, None, None),
+        "mf_assemble_scalar": lambda: L.mf_assemble_scalar(None, None, None),
+        "mf_form_set_restriction": lambda: L.mf_form_set_restriction(None, 0, None, 0, 0),
+        "mf_matrix_create_block": lambda: L.mf_matrix_create_block(None, 0, 0, None),
+    }
+    for name, call in calls.items():
+        assert call() == _lib.MF_EINVAL, name
+        assert L.mf_last_error(), name
+    assert L.mf_batch_destroy(None) == _lib.MF_OK

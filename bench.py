@@ -214,7 +214,7 @@ def bench_c3(args):
     import torch
     import cudolfinx as cufem
     from cudolfinx import _lib
-    from cudolfinx.solve import csr_tensor, pcg
+    from cudolfinx.solve import pcg
     from mifem import problems
 
     t0 = time.perf_counter()
@@ -289,10 +289,16 @@ def bench_c3(args):
     # fixed-tolerance CG (Jacobi) with each matrix
     sol = {}
     for p in (64, 32, 16):
-        M = csr_tensor(A, vals[p])
-        x, it, res = pcg(M, bt, rtol=1e-12, maxiter=args.cg_maxiter)
-        sol[p] = (x, it, res)
-        log(f"CG precision {p}: {it} iterations, relative residual {res:.2e}")
+        # native CSR SpMV on the matrix's arrays, with this precision's values
+        _lib.check(lib.mf_matrix_borrow_values(A.handle, C.c_void_p(vals[p].data_ptr())))
+        torch.cuda.synchronize()
+        t_cg = time.perf_counter()
+        x, it, res = pcg(A, bt, rtol=1e-12, maxiter=args.cg_maxiter)
+        torch.cuda.synchronize()
+        t_cg = time.perf_counter() - t_cg
+        sol[p] = (x, it, res, t_cg / max(it, 1) * 1e3)
+        log(f"CG precision {p}: {it} iterations, relative residual {res:.2e}, "
+            f"{t_cg / max(it, 1) * 1e3:.3f} ms per iteration")
     xref = sol[64][0]
     xerr = {p: float(torch.linalg.norm(sol[p][0] - xref) / torch.linalg.norm(xref)) for p in (32, 16)}
     flops = 2.175e6 * P.mesh.num_cells
@@ -313,7 +319,8 @@ def bench_c3(args):
                    "speedup_fp32_vs_fp64_per_entity": generic_ms / times[32],
                    "rel_diff_per_entity_vs_owner_fp64": generic_err,
                    "rel_err_A": errs, "rel_err_b": 0.0, "rel_err_cg_solution": xerr,
-                   "cg_iterations": {str(p): sol[p][1] for p in sol}},
+                   "cg_iterations": {str(p): sol[p][1] for p in sol},
+                   "cg_ms_per_iteration": {str(p): sol[p][3] for p in sol}},
         "roofline": {"bound": "mfma", "achieved": flops / (times[32] * 1e-3) / 1e12, "peak": 157.3,
                      "unit": "TFLOP/s", "frac": flops / (times[32] * 1e-3) / 1e12 / 157.3,
                      "traffic": None, "kernel": "matrix launch: mf_hprep_cell_0 + mf_howner32_cell_0",
@@ -341,7 +348,7 @@ def bench_c4(args):
     import torch
     import cudolfinx as cufem
     from cudolfinx import _lib
-    from cudolfinx.solve import csr_tensor, pcg
+    from cudolfinx.solve import pcg
     from mifem import problems
     from mifem.dgforms import sipg_energy
 
@@ -446,15 +453,18 @@ def bench_c4(args):
 
     # Newton with an Armijo line search on the energy (F = grad E, J = Hessian,
     # J is SPD, so the Newton direction descends); one graph replay per step
-    history, energies = [], []
+    history, energies, cg_ms = [], [], []
     for it in range(args.newton):
         g.launch()
         s.synchronize()
         r0, e0 = float(torch.linalg.norm(bt)), float(ev.item())
         history.append(r0)
         energies.append(e0)
-        M = csr_tensor(A, vals)
-        du, cg_it, cg_res = pcg(M, -bt.clone(), rtol=1e-10, maxiter=args.cg_maxiter)
+        s.synchronize()
+        t_cg = time.perf_counter()
+        du, cg_it, cg_res = pcg(A, -bt.clone(), rtol=1e-10, maxiter=args.cg_maxiter)
+        torch.cuda.synchronize()
+        cg_ms.append((time.perf_counter() - t_cg) / max(cg_it, 1) * 1e3)
         slope = float(torch.dot(bt, du))
         u_old = ud.clone()
         alpha = 1.0
@@ -465,8 +475,8 @@ def bench_c4(args):
             if float(ev.item()) <= e0 + 1e-4 * alpha * slope:
                 break
             alpha *= 0.5
-        log(f"Newton {it}: |F| = {r0:.3e}, E = {e0:.12e}, CG {cg_it} its (res {cg_res:.1e}), "
-            f"step {alpha}")
+        log(f"Newton {it}: |F| = {r0:.3e}, E = {e0:.12e}, CG {cg_it} its (res {cg_res:.1e}, "
+            f"{cg_ms[-1]:.3f} ms/it), step {alpha}")
     g.launch()
     s.synchronize()
     history.append(float(torch.linalg.norm(bt)))
@@ -489,7 +499,8 @@ def bench_c4(args):
                    "system_autotune": [{"block": b_, "ms": m_} for b_, m_ in stune],
                    "separate_forms_eager_ms": sep_eager_ms, "separate_forms_graph_ms": sep_graph_ms,
                    "J_kernel_ms": jk_ms, "F_kernel_ms": fk_ms, "E_kernels_ms": ek_ms,
-                   "newton_residuals": history, "newton_energies": energies},
+                   "newton_residuals": history, "newton_energies": energies,
+                   "cg_ms_per_iteration": cg_ms},
         "roofline": {"bound": "hbm", "achieved": alg / (sys_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": alg / (sys_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "traffic": None,

@@ -578,6 +578,26 @@ int mf_matrix_device_arrays(mf_matrix* A, int32_t** rp, int32_t** cols, double**
   });
 }
 
+int mf_spmv(mf_matrix* A, const double* d_x, double* d_y, mf_stream s) {
+  return guard([&] {
+    MF_CHECK(A && d_x && d_y, MF_EINVAL, "null argument");
+    MF_CHECK(A->values, MF_ESTATE, "matrix has no values");
+    MF_CHECK(d_x != d_y, MF_EINVAL, "x and y must not alias");
+    if (A->form) use_device(A->form->ctx);
+    mf::k_spmv(A->nrows, A->nnz, A->row_ptr.ptr, A->cols.ptr, A->values, d_x, d_y, S(s));
+  });
+}
+
+int mf_matrix_diagonal(mf_matrix* A, double* d_diag, mf_stream s) {
+  return guard([&] {
+    MF_CHECK(A && d_diag, MF_EINVAL, "null argument");
+    MF_CHECK(A->values, MF_ESTATE, "matrix has no values");
+    MF_CHECK(A->nrows == A->ncols, MF_ESHAPE, "diagonal of a non-square matrix");
+    if (A->form) use_device(A->form->ctx);
+    mf::k_diagonal(A->nrows, A->row_ptr.ptr, A->cols.ptr, A->values, d_diag, S(s));
+  });
+}
+
 int mf_matrix_borrow_values(mf_matrix* A, double* d_values) {
   return guard([&] {
     MF_CHECK(A && d_values, MF_EINVAL, "null argument");

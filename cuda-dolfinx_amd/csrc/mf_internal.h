@@ -103,6 +103,11 @@ void launch(hipFunction_t f, uint32_t grid, uint32_t block, uint32_t shmem, hipS
 // ------------------------------------------------------------ util kernels
 // (precompiled for gfx950 in util_kernels.hip; launchers run on `s`)
 void k_fill(double* x, int64_t n, double v, hipStream_t s);
+// y = A x (CSR, int32 indices, FP64 values); d = diag(A)
+void k_spmv(int64_t nrows, int64_t nnz, const int32_t* rp, const int32_t* cols, const double* v,
+            const double* x, double* y, hipStream_t s);
+void k_diagonal(int64_t nrows, const int32_t* rp, const int32_t* cols, const double* v, double* d,
+                hipStream_t s);
 void k_fill_i32(int32_t* x, int64_t n, int32_t v, hipStream_t s);
 // out[0] (+)= sum of partials[0..n) in a fixed order (one workgroup)
 void k_sum_partials(const double* partials, int64_t n, double* out, int accumulate, hipStream_t s);

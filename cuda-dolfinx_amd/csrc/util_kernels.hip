@@ -235,6 +235,43 @@ __global__ void lift_list_kernel(int32_t* __restrict__ list, int32_t* __restrict
   }
 }
 
+// y = A x for a CSR matrix (int32 row_ptr / cols, FP64 values): G lanes per
+// row (G a power of two <= 64, so a wave holds 64 / G rows); each lane sums a
+// strided part of its row and the lanes of a row combine with shuffles, in a
+// fixed order (repeated products are bit-identical).
+template <int G>
+__global__ void __launch_bounds__(kBlock) spmv_kernel(int64_t nrows, const int32_t* __restrict__ rp,
+                                                      const int32_t* __restrict__ cols,
+                                                      const double* __restrict__ v,
+                                                      const double* __restrict__ x,
+                                                      double* __restrict__ y) {
+  const int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / G;
+  const int lane = threadIdx.x & (G - 1);
+  double acc = 0.0;
+  if (row < nrows) {
+    const int32_t e = rp[row + 1];
+    for (int32_t k = rp[row] + lane; k < e; k += G) acc += v[k] * x[cols[k]];
+  }
+  for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
+  if (row < nrows && lane == 0) y[row] = acc;
+}
+
+// d[r] = A[r][r] (0 where the pattern has no diagonal entry); columns sorted.
+__global__ void diag_kernel(int64_t nrows, const int32_t* __restrict__ rp,
+                            const int32_t* __restrict__ cols, const double* __restrict__ v,
+                            double* __restrict__ d) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < nrows; r += stride) {
+    int32_t lo = rp[r], hi = rp[r + 1];
+    while (lo < hi) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (cols[mid] < r) lo = mid + 1;
+      else hi = mid;
+    }
+    d[r] = (lo < rp[r + 1] && cols[lo] == r) ? v[lo] : 0.0;
+  }
+}
+
 }  // namespace
 
 void k_fill(double* x, int64_t n, double v, hipStream_t s) {
@@ -322,6 +359,35 @@ void k_lift_list(int32_t* list, int32_t* count, const int32_t* ents, int64_t nen
   if (nents <= 0) return;
   lift_list_kernel<<<grid_for(nents), kBlock, 0, s>>>(list, count, ents, nents, stride, ncell, dm1,
                                                       nd1, bs1, bc1);
+  MF_HIP(hipGetLastError());
+}
+
+
+void k_spmv(int64_t nrows, int64_t nnz, const int32_t* rp, const int32_t* cols, const double* v,
+            const double* x, double* y, hipStream_t s) {
+  if (nrows <= 0) return;
+  // lanes per row from the mean row length (about three entries per lane)
+  const double mean = (double)nnz / (double)nrows;
+  int g = 2;
+  while (g < 64 && g * 3 < mean) g *= 2;
+  const int64_t threads = nrows * g;
+  MF_CHECK(threads / kBlock < (1LL << 31), MF_ESHAPE, "matrix too large for one SpMV launch");
+  const uint32_t grid = static_cast<uint32_t>((threads + kBlock - 1) / kBlock);
+  switch (g) {
+    case 2: spmv_kernel<2><<<grid, kBlock, 0, s>>>(nrows, rp, cols, v, x, y); break;
+    case 4: spmv_kernel<4><<<grid, kBlock, 0, s>>>(nrows, rp, cols, v, x, y); break;
+    case 8: spmv_kernel<8><<<grid, kBlock, 0, s>>>(nrows, rp, cols, v, x, y); break;
+    case 16: spmv_kernel<16><<<grid, kBlock, 0, s>>>(nrows, rp, cols, v, x, y); break;
+    case 32: spmv_kernel<32><<<grid, kBlock, 0, s>>>(nrows, rp, cols, v, x, y); break;
+    default: spmv_kernel<64><<<grid, kBlock, 0, s>>>(nrows, rp, cols, v, x, y); break;
+  }
+  MF_HIP(hipGetLastError());
+}
+
+void k_diagonal(int64_t nrows, const int32_t* rp, const int32_t* cols, const double* v, double* d,
+                hipStream_t s) {
+  if (nrows <= 0) return;
+  diag_kernel<<<grid_for(nrows), kBlock, 0, s>>>(nrows, rp, cols, v, d);
   MF_HIP(hipGetLastError());
 }
 

@@ -49,6 +49,8 @@ SIGNATURES = {
     "mf_matrix_info": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)]),
     "mf_matrix_device_arrays": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     "mf_matrix_borrow_values": (C.c_int, [_vp, _vp]),
+    "mf_spmv": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mf_matrix_diagonal": (C.c_int, [_vp, _vp, _vp]),
     "mf_matrix_copy_csr": (C.c_int, [_vp, _i32p, _i32p, _dp]),
     "mf_matrix_destroy": (C.c_int, [_vp]),
     "mf_bc_create": (C.c_int, [_vp, _i64, _i32p, _i64, _i64, _dp, C.POINTER(_vp)]),

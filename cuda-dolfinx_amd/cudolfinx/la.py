@@ -83,6 +83,25 @@ class CUDAMatrix:
         """Handoff: make the device values final (CUDAMatrix::to_host + MatAssembly)."""
         self._ctx.synchronize()
 
+    @staticmethod
+    def _addr(v):
+        return C.c_void_p(v.data_ptr() if hasattr(v, "data_ptr") else int(v))
+
+    def mult(self, x, y, stream=None):
+        """y = A x on the device (x, y: float64 torch tensors or device addresses),
+        the MatMult of a PETSc solve on the handed-off matrix."""
+        if hasattr(x, "numel") and (x.numel() != self.ncols or y.numel() != self.nrows):
+            raise ValueError("vector sizes do not match the matrix")
+        check(lib().mf_spmv(self._handle, self._addr(x), self._addr(y), stream))
+        return y
+
+    def diagonal(self, d, stream=None):
+        """d = diag(A) on the device (the Jacobi preconditioner)."""
+        if hasattr(d, "numel") and d.numel() != self.nrows:
+            raise ValueError("vector size does not match the matrix")
+        check(lib().mf_matrix_diagonal(self._handle, self._addr(d), stream))
+        return d
+
     def csr(self):
         rp = np.empty(self.nrows + 1, dtype=np.int32)
         cols = np.empty(max(self.nnz, 1), dtype=np.int32)

@@ -2,10 +2,13 @@
 
 Stands in for the PETSc KSP (cg + jacobi) the reference's examples hand the
 assembled Mat/Vec to (python/examples/poisson.py:116-125) when petsc4py is
-absent.  The CSR arrays are wrapped as a torch sparse CSR tensor (device
-memory shared with the matrix for the values); the iteration runs in FP64.
+absent.  Given a CUDAMatrix, the products and the diagonal run on the
+engine's own CSR kernels (mf_spmv / mf_matrix_diagonal) on the matrix's device
+arrays; a torch sparse CSR tensor also works.  The iteration runs in FP64.
 """
 from __future__ import annotations
+
+import ctypes as C
 
 import numpy as np
 
@@ -29,16 +32,33 @@ def csr_tensor(A, values=None):
 
 
 def pcg(M, b, rtol: float = 1e-10, maxiter: int = 5000):
-    """Solve M x = b (torch sparse CSR, torch vector) with Jacobi-PCG.
+    """Solve M x = b with Jacobi-PCG.  M: a CUDAMatrix (native CSR SpMV and
+    diagonal, mf_spmv / mf_matrix_diagonal, on torch's current stream) or a
+    torch sparse CSR tensor; b: a float64 torch vector.
 
     Returns (x, iterations, final relative residual).
     """
     import torch
-    d = torch.zeros(M.shape[0], dtype=torch.float64, device=b.device)
-    crow, col, val = M.crow_indices(), M.col_indices(), M.values()
-    rows = torch.repeat_interleave(torch.arange(M.shape[0], device=b.device), crow[1:] - crow[:-1])
-    diag_mask = rows == col
-    d.index_put_((rows[diag_mask],), val[diag_mask], accumulate=True)
+    native = hasattr(M, "mult")
+    if native:
+        stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        d = torch.empty(M.nrows, dtype=torch.float64, device=b.device)
+        M.diagonal(d, stream)
+        Ap = torch.empty_like(b)
+
+        def matvec(p):
+            M.mult(p, Ap, stream)
+            return Ap
+    else:
+        d = torch.zeros(M.shape[0], dtype=torch.float64, device=b.device)
+        crow, col, val = M.crow_indices(), M.col_indices(), M.values()
+        rows = torch.repeat_interleave(torch.arange(M.shape[0], device=b.device),
+                                       crow[1:] - crow[:-1])
+        diag_mask = rows == col
+        d.index_put_((rows[diag_mask],), val[diag_mask], accumulate=True)
+
+        def matvec(p):
+            return torch.mv(M, p)
     dinv = torch.where(d != 0, 1.0 / d, torch.zeros_like(d))
     x = torch.zeros_like(b)
     r = b.clone()
@@ -49,7 +69,7 @@ def pcg(M, b, rtol: float = 1e-10, maxiter: int = 5000):
     it = 0
     res = 1.0
     for it in range(1, maxiter + 1):
-        Ap = torch.mv(M, p)
+        Ap = matvec(p)
         alpha = rz / torch.dot(p, Ap)
         x += alpha * p
         r -= alpha * Ap

@@ -171,6 +171,14 @@ int mf_matrix_info(mf_matrix* A, int64_t* nrows, int64_t* ncols, int64_t* nnz);
  * buffer, e.g. a PETSc-HIP MatSeqAIJHIPSPARSEGetArray pointer). */
 int mf_matrix_device_arrays(mf_matrix* A, int32_t** d_row_ptr, int32_t** d_cols, double** d_values);
 int mf_matrix_borrow_values(mf_matrix* A, double* d_values);
+/* y = A x on `stream` (device vectors of ncols / nrows doubles, not
+ * aliasing).  Stands in for PETSc's MatMult on the handed-off Mat, which the
+ * reference's examples use through KSP (python/examples/poisson.py:116-125;
+ * CUDASeqMatrix.cpp:459-533 hands the device arrays to PETSc). */
+int mf_spmv(mf_matrix* A, const double* d_x, double* d_y, mf_stream stream);
+/* d[r] = A[r][r] (0 where the pattern has no diagonal entry): the Jacobi
+ * preconditioner of that solve (PETSc MatGetDiagonal). */
+int mf_matrix_diagonal(mf_matrix* A, double* d_diag, mf_stream stream);
 /* Copies CSR arrays to host (any pointer may be NULL); the handoff of
  * CUDAMatrix::to_host (CUDASeqMatrix.cpp:459-533) when no PETSc-HIP exists. */
 int mf_matrix_copy_csr(mf_matrix* A, int32_t* row_ptr, int32_t* cols, double* values);

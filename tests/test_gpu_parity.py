@@ -568,6 +568,39 @@ def test_precision_errors():
         asm.assemble_matrix(q, bcs=Q.bcs)
 
 
+@pytest.mark.parametrize("which", ["poisson_p1", "elasticity_p2", "helmholtz_q3"])
+def test_native_spmv_diagonal_and_cg(which):
+    _say(f"native CSR SpMV / diagonal / Jacobi-CG on the assembled {which} matrix")
+    import torch
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from cudolfinx.solve import pcg
+    P = {"poisson_p1": lambda: problems.poisson_p1(6), "elasticity_p2": lambda: problems.elasticity_p2(3),
+         "helmholtz_q3": lambda: problems.helmholtz_q3(3)}[which]()
+    rp, cols, vals, b, (a, L, asm, A, dbc) = _gpu(P)
+    S = sp.csr_matrix((vals, cols, rp), shape=(A.nrows, A.ncols))
+    rng = np.random.default_rng(7)
+    x = rng.uniform(-1, 1, A.ncols)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.empty(A.nrows, dtype=torch.float64, device="cuda")
+    A.mult(xd, yd)
+    y1 = yd.cpu().numpy()
+    assert _rel(y1, S @ x) <= 1e-14
+    A.mult(xd, yd)
+    assert np.array_equal(yd.cpu().numpy(), y1)              # repeatable, fixed order
+    dd = torch.empty(A.nrows, dtype=torch.float64, device="cuda")
+    A.diagonal(dd)
+    assert np.array_equal(dd.cpu().numpy(), S.diagonal())
+    with pytest.raises(ValueError):
+        A.mult(xd, xd)
+    bd = torch.from_numpy(b).cuda()
+    u, it, res = pcg(A, bd, rtol=1e-12, maxiter=5000)
+    ref = spla.spsolve(S.tocsc(), b)
+    err = _rel(u.cpu().numpy(), ref)
+    _say(f"  nnz={S.nnz}: SpMV ok, CG {it} its, residual {res:.1e}, error vs direct solve {err:.1e}")
+    assert res < 1e-12 and err <= 1e-8, (res, err)
+
+
 # ------------------------------------------------------------ block / restricted assembly
 def _selector(V, keep):
     """Scalar-dof selection matrix of the kept nodes (in order), bs-unrolled."""

@@ -36,8 +36,7 @@ inline hipStream_t S(mf_stream s) { return reinterpret_cast<hipStream_t>(s); }
 void use_device(const mf_context* ctx) { MF_HIP(hipSetDevice(ctx->device)); }
 
 std::vector<std::string> rtc_options(const mf_context* ctx) {
-  return {"--offload-arch=" + ctx->arch, "-O3", "-munsafe-fp-atomics", "-std=c++17",
-          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  return mf::rtc_compile_options(ctx->arch);
 }
 
 }  // namespace
@@ -355,34 +354,7 @@ void compile_form(mf_form* f, int verbose) {
       f->consts.upload(&z, 1);
     }
     if (f->W > 0) f->packed.alloc(f->mesh->num_cells * (int64_t)f->W);
-    // owner-kernel shape: LDS budget (values) and cells per batch.  Scalar
-    // forms on the reference-tensor path keep a (nd x nd x 6) table in LDS:
-    // 2048 values; other simplex forms 4608 (the fastest of the tuned shapes
-    // for vector P2 elasticity); hexahedra 4096.  A caller's choice wins.
-    bool trace = false, hex = false;
-    for (auto& I : f->ints) {
-      trace |= mf::staged_trace(*I);
-      hex |= mf::is_hex_staged(*I);
-    }
-    int64_t cap = f->opt("chunk_values", trace ? 2048 : (hex ? 4096 : 4608));
-    f->options["chunk_values"] = cap;
-    if (!f->options.count("owner_nb")) f->options["owner_nb"] = 32;
-    if (!f->options.count("chunk_rows")) f->options["chunk_rows"] = 64;
-    std::string src = mf::generate_form_source(*f);
-    {
-      // owner-kernel compile-time shape
-      auto rep = [&](const std::string& k, int64_t v) {
-        size_t p = 0;
-        std::string t = std::to_string(v);
-        while ((p = src.find(k, p)) != std::string::npos) {
-          src.replace(p, k.size(), t);
-          p += t.size();
-        }
-      };
-      rep("$OBLOCK", f->opt("owner_block", 256));
-      rep("$OCAP", cap);
-      rep("$OMAXR", f->opt("chunk_rows", 64));
-    }
+    std::string src = mf::compile_source(*f);
     f->source = src;
     bool cached = false;
     auto code = mf::rtc_compile(src, "mifem_form", rtc_options(f->ctx), f->ctx->cache_dir, &cached);

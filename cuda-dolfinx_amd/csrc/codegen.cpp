@@ -534,6 +534,41 @@ std::string generate_batch_source(const std::vector<const mf_form*>& forms,
   return os.str();
 }
 
+std::vector<std::string> rtc_compile_options(const std::string& arch) {
+  return {"--offload-arch=" + arch, "-O3", "-munsafe-fp-atomics", "-std=c++17",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+}
+
+std::string compile_source(mf_form& f) {
+  // owner-kernel shape: LDS budget (values) and cells per batch.  Scalar
+  // forms on the reference-tensor path keep a (nd x nd x 6) table in LDS:
+  // 2048 values; other simplex forms 4608 (the fastest of the tuned shapes
+  // for vector P2 elasticity); hexahedra 4096.  A caller's choice wins.
+  bool trace = false, hex = false;
+  for (auto& I : f.ints) {
+    trace |= staged_trace(*I);
+    hex |= is_hex_staged(*I);
+  }
+  const int64_t cap = f.opt("chunk_values", trace ? 2048 : (hex ? 4096 : 4608));
+  f.options["chunk_values"] = cap;
+  if (!f.options.count("owner_nb")) f.options["owner_nb"] = 32;
+  if (!f.options.count("chunk_rows")) f.options["chunk_rows"] = 64;
+  std::string src = generate_form_source(f);
+  // owner-kernel compile-time shape
+  auto rep = [&](const std::string& k, int64_t v) {
+    size_t p = 0;
+    const std::string t = std::to_string(v);
+    while ((p = src.find(k, p)) != std::string::npos) {
+      src.replace(p, k.size(), t);
+      p += t.size();
+    }
+  };
+  rep("$OBLOCK", f.opt("owner_block", 256));
+  rep("$OCAP", cap);
+  rep("$OMAXR", f.opt("chunk_rows", 64));
+  return src;
+}
+
 std::string generate_form_source(const mf_form& f) {
   std::ostringstream os;
   os << kPrelude;

@@ -324,6 +324,10 @@ struct mf_matrix {
 namespace mf {
 // code generation (codegen.cpp)
 std::string generate_form_source(const mf_form& f);
+// generate_form_source with the owner-kernel shape options defaulted and
+// substituted: the translation unit mf_form_compile hands to hipRTC
+std::string compile_source(mf_form& f);
+std::vector<std::string> rtc_compile_options(const std::string& arch);
 bool owner_supported(const mf_form& f, const Integral& I);
 bool is_hex_staged(const Integral& I);
 bool staged_trace(const Integral& I);

@@ -294,3 +294,104 @@ def test_null_arguments_rejected_without_a_device():
         assert call() == _lib.MF_EINVAL, name
         assert L.mf_last_error(), name
     assert L.mf_batch_destroy(None) == _lib.MF_OK
+
+
+# ----------------------------------------------------------------- gfx950 compile check
+_ITYPES = {"cell": 0, "exterior_facet": 1, "interior_facet": 2}
+_READELF = Path("/opt/rocm/llvm/bin/llvm-readelf")
+
+
+def _describe(form, path):
+    """The engine-side view of a form for tests/codegen_check.cpp: rank, shapes,
+    coefficient layout, and each integral's UFCx body + staged description."""
+    if form.rank:
+        V0, V1 = form.function_spaces[0], form.function_spaces[-1]
+        sh = (V0.element.ndofs, V0.bs, V1.element.ndofs, V1.bs)
+    else:
+        sh = (1, 1, 1, 1)
+    off = form.coefficient_offsets
+    with open(path, "w") as o:
+        o.write(f"{form.rank} {form.mesh.nv_per_cell} {form.mesh.facet_table.shape[0]} {off[-1]} "
+                f"{sh[0]} {sh[1]} {sh[2]} {sh[3]} {len(form.integrals)} {len(form.coefficients)}\n")
+        for j in range(len(form.coefficients)):
+            o.write(f"{off[j]} {off[j + 1] - off[j]}\n")
+        for i, I in enumerate(form.integrals):
+            st = I.staged.source() if I.staged is not None else ""
+            o.write(f"{_ITYPES[I.type]} {i} {I.name} {len(I.source.encode())} {len(st.encode())}\n")
+            o.write(I.source)
+            o.write(st)
+
+
+def _kernels(co: Path):
+    """{kernel name: (vgprs, scratch bytes, LDS bytes)} from a code object's metadata."""
+    notes = subprocess.run([str(_READELF), "--notes", str(co)], capture_output=True, text=True,
+                           check=True).stdout
+    out = {}
+    for block in notes.split("  - .agpr_count")[1:]:
+        name = re.search(r"\.name:\s+(\S+)", block).group(1)
+        get = lambda key: int(re.search(rf"\.{key}:\s+(\d+)", block).group(1))
+        out[name] = (get("vgpr_count"), get("private_segment_fixed_size"),
+                     get("group_segment_fixed_size"))
+    return out
+
+
+def test_generated_kernels_compile_for_gfx950(tmp_path):
+    """Every kernel family's generated source compiles with hipRTC for gfx950
+    (no GPU needed): owner-computes FP64 + FP32 variants (scalar P1 / P3),
+    vector P2 elasticity, the Q3 matrix-core kernels, the SIPG cell-owner
+    kernel, and the one-launch batch of three element groups.  The owner
+    kernels use no scratch memory; the batch reserves its largest group's LDS."""
+    import shutil
+    from mifem import dgforms  # noqa: F401
+    from mifem.fem import Function as Fn
+    build = ROOT / "cuda-dolfinx_amd" / "_build"
+    objs = [build / f"{n}.o" for n in ("codegen", "runtime", "owner_kernel", "hex_kernel",
+                                        "dg_kernel", "sparsity", "util_kernels", "matrix")]
+    if not shutil.which("hipcc") or not _READELF.exists() or not all(o.exists() for o in objs):
+        pytest.skip("hipcc / llvm-readelf / library objects not available")
+    drv = tmp_path / "codegen_check"
+    subprocess.run(["hipcc", "-c", "--offload-arch=gfx950", "-fPIC", "-std=c++17", "-O1",
+                    f"-I{ROOT / 'cuda-dolfinx_amd' / 'csrc'}", f"-I{ROOT / 'include'}",
+                    str(ROOT / "tests" / "codegen_check.cpp"), "-o", str(tmp_path / "check.o")],
+                   check=True)
+    subprocess.run(["hipcc", str(tmp_path / "check.o"), *map(str, objs), "-lhiprtc", "-o", str(drv)],
+                   check=True)
+    m = create_unit_cube(2)
+    D0 = functionspace(m, ("DG", 0))
+    kap = Fn(D0)
+    forms_ = {
+        "p1": forms.stiffness(functionspace(m, ("P", 1)), kap),
+        "p2": forms.stiffness(functionspace(m, ("P", 2)), kap),
+        "p3": forms.stiffness(functionspace(m, ("P", 3)), kap),
+        "elasticity": problems.elasticity_p2(2).a,
+        "q3": problems.helmholtz_q3(2).a,
+        "sipg": problems.dg_plaplace(2).a,
+    }
+    for k, f in forms_.items():
+        _describe(f, tmp_path / f"{k}.txt")
+    runs = {k: [str(drv), "form", str(tmp_path / f"{k}.txt"), str(tmp_path / f"{k}.co")]
+            for k in ("p1", "p3", "elasticity", "q3", "sipg")}
+    runs["batch"] = [str(drv), "batch", str(tmp_path / "batch.co")] + \
+        [str(tmp_path / f"{k}.txt") for k in ("p1", "p2", "p3")]
+    procs = {k: subprocess.Popen(c, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
+             for k, c in runs.items()}
+    outs = {k: p.communicate(timeout=600)[0] for k, p in procs.items()}
+    for k, o in outs.items():
+        assert o.startswith("OK"), (k, o[-2000:])
+    K = {k: _kernels(tmp_path / f"{k}.co") for k in runs}
+    for k in ("p1", "p3"):
+        assert {"mf_prep_cell_0", "mf_owner_cell_0", "mf_owner32_cell_0", "mf_mat_cell_0"} <= set(K[k])
+        assert K[k]["mf_owner_cell_0"][1] == 0 and K[k]["mf_owner32_cell_0"][1] == 0
+        assert K[k]["mf_owner32_cell_0"][2] < K[k]["mf_owner_cell_0"][2]    # FP32 tables
+    assert "mf_owner32_cell_0" not in K["elasticity"] and K["elasticity"]["mf_owner_cell_0"][1] == 0
+    assert {"mf_hprep_cell_0", "mf_howner64_cell_0", "mf_howner32_cell_0",
+            "mf_howner16_cell_0"} <= set(K["q3"])
+    # matrix-core FP32 / BF16 kernels: no scratch; the FP64 one keeps 48 B (the per-entity
+    # kernels hold the whole 64 x 64 element tensor per thread in scratch, as the reference does)
+    assert K["q3"]["mf_howner32_cell_0"][1] == 0 and K["q3"]["mf_howner16_cell_0"][1] == 0
+    assert K["q3"]["mf_howner64_cell_0"][1] <= 64
+    assert "mf_dgown" in K["sipg"]
+    lds = int(outs["batch"].split()[-1])
+    assert {"mf_owner_batch", "mf_prep_g0", "mf_prep_g1", "mf_prep_g2"} <= set(K["batch"])
+    assert K["batch"]["mf_owner_batch"][1] == 0 and K["batch"]["mf_owner_batch"][2] == lds
+    assert lds == K["p3"]["mf_owner_cell_0"][2]                # the largest group's LDS

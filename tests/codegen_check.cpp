@@ -6,6 +6,7 @@
 //
 //   codegen_check form  <description> <out.co>        one form's translation unit
 //   codegen_check batch <out.co> <description>...      the one-launch batch of several forms
+//   codegen_check system <out.co> <J> <F> [<E>]        the fused Jacobian/residual/functional launch
 //
 // Prints "OK <bytes>" or "FAIL <message>"; the test reads the kernels'
 // resource use from the code object's metadata.
@@ -98,7 +99,19 @@ int main(int argc, char** argv) {
       std::cout << "OK " << code.size() << " lds " << lds << "\n";
       return 0;
     }
-    std::cout << "FAIL usage: codegen_check form <description> <out.co> | batch <out.co> <description>...\n";
+    if (mode == "system" && (argc == 5 || argc == 6)) {
+      auto J = load(argv[3]);
+      auto F = load(argv[4]);
+      std::unique_ptr<Loaded> E = argc == 6 ? load(argv[5]) : nullptr;
+      const std::string src =
+          mf::generate_system_source(J->form, F->form, E ? &E->form : nullptr, 256, 0);
+      auto code = mf::rtc_compile(src, "codegen_check", opts, "", nullptr);
+      write(argv[2], code);
+      std::cout << "OK " << code.size() << "\n";
+      return 0;
+    }
+    std::cout << "FAIL usage: codegen_check form <description> <out.co> | batch <out.co> "
+                 "<description>... | system <out.co> <J> <F> [<E>]\n";
     return 2;
   } catch (const std::exception& e) {
     std::cout << "FAIL " << e.what() << "\n";

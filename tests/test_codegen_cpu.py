@@ -339,7 +339,8 @@ def test_generated_kernels_compile_for_gfx950(tmp_path):
     """Every kernel family's generated source compiles with hipRTC for gfx950
     (no GPU needed): owner-computes FP64 + FP32 variants (scalar P1 / P3),
     vector P2 elasticity, the Q3 matrix-core kernels, the SIPG cell-owner
-    kernel, and the one-launch batch of three element groups.  The owner
+    kernel, the fused Newton-system kernel and the one-launch batch of three
+    element groups.  The owner
     kernels use no scratch memory; the batch reserves its largest group's LDS."""
     import shutil
     from mifem import dgforms  # noqa: F401
@@ -365,14 +366,19 @@ def test_generated_kernels_compile_for_gfx950(tmp_path):
         "p3": forms.stiffness(functionspace(m, ("P", 3)), kap),
         "elasticity": problems.elasticity_p2(2).a,
         "q3": problems.helmholtz_q3(2).a,
-        "sipg": problems.dg_plaplace(2).a,
-    }
+        }
+    Pdg = problems.dg_plaplace(2)
+    forms_["sipg"] = Pdg.a
+    forms_["sipg_F"] = Pdg.L
+    forms_["sipg_E"] = dgforms.sipg_energy(Pdg.V, Pdg.extras["u"])
     for k, f in forms_.items():
         _describe(f, tmp_path / f"{k}.txt")
     runs = {k: [str(drv), "form", str(tmp_path / f"{k}.txt"), str(tmp_path / f"{k}.co")]
             for k in ("p1", "p3", "elasticity", "q3", "sipg")}
     runs["batch"] = [str(drv), "batch", str(tmp_path / "batch.co")] + \
         [str(tmp_path / f"{k}.txt") for k in ("p1", "p2", "p3")]
+    runs["system"] = [str(drv), "system", str(tmp_path / "system.co")] + \
+        [str(tmp_path / f"{k}.txt") for k in ("sipg", "sipg_F", "sipg_E")]
     procs = {k: subprocess.Popen(c, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
              for k, c in runs.items()}
     outs = {k: p.communicate(timeout=600)[0] for k, p in procs.items()}
@@ -391,6 +397,7 @@ def test_generated_kernels_compile_for_gfx950(tmp_path):
     assert K["q3"]["mf_howner32_cell_0"][1] == 0 and K["q3"]["mf_howner16_cell_0"][1] == 0
     assert K["q3"]["mf_howner64_cell_0"][1] <= 64
     assert "mf_dgown" in K["sipg"]
+    assert K["system"]["mf_dgsys"][1] == 0                      # fused J + F + E, no scratch
     lds = int(outs["batch"].split()[-1])
     assert {"mf_owner_batch", "mf_prep_g0", "mf_prep_g1", "mf_prep_g2"} <= set(K["batch"])
     assert K["batch"]["mf_owner_batch"][1] == 0 and K["batch"]["mf_owner_batch"][2] == lds

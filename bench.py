@@ -90,9 +90,9 @@ def algorithmic_bytes(P, A):
     return 8 * nnz + 4 * nnz_b + 4 * (nb + 1) + 24 * nv + 16 * nc + 16 * nc + 4 * nd * nc
 
 
-def measured_traffic():
-    """HBM bytes per matrix launch from the committed rocprofv3 FETCH/WRITE summary."""
-    p = ROOT / "profiles" / "c2_elasticity_p2_64_matrix_traffic.md"
+def measured_traffic(name: str = "c2_elasticity_p2_64_matrix_traffic.md"):
+    """HBM bytes per launch from a committed rocprofv3 FETCH/WRITE summary."""
+    p = ROOT / "profiles" / name
     try:
         for line in p.read_text().splitlines():
             if line.startswith("traffic_bytes_per_launch:"):
@@ -525,7 +525,7 @@ def bench_c4(args):
                    "cg_ms_per_iteration": cg_ms},
         "roofline": {"bound": "hbm", "achieved": alg / (sys_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": alg / (sys_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": None,
+                     "traffic": measured_traffic("c4_sipg_dg1_48_traffic.md") if args.n_dg == 48 else None,
                      "kernel": "mf_dgsys (J + F + E over cells and facets, one launch)",
                      "algorithmic_bytes": alg},
         "cpu_baseline": None if args.no_cpu else cpu_baseline_c4(),
@@ -679,7 +679,8 @@ def bench_c5(args):
                    "max_rel_err_b": max((e["rel_err_b"] for e in errs), default=None)},
         "roofline": {"bound": "hbm", "achieved": alg / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "mf_owner_batch (all groups in one launch) + one record pass per group",
+                     "traffic": (measured_traffic("c5_many_poisson_512_traffic.md")
+                                 if args.instances == 512 and not args.c5_fp32 else None), "kernel": "mf_owner_batch (all groups in one launch) + one record pass per group",
                      "algorithmic_bytes": alg},
         "cpu_baseline": None if args.no_cpu else cpu_baseline_c5(B),
     }

@@ -166,6 +166,7 @@ int mf_dofmap_create(mf_context* ctx, const int32_t* dofs, int64_t ncells, int n
     d->num_nodes = nnodes;
     d->nd = nd;
     d->bs = bs;
+    d->uid = g_uid++;
     *out = d.release();
   });
 }

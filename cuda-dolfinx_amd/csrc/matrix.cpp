@@ -533,9 +533,10 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
              plan->pair_list.ptr, plan->pair_pos.ptr, owner->records.ptr, A->rpb.ptr,
              plan->diag_pos.ptr, n0, A->values, diagonal, flags);
     } else {
+    const uint8_t* cf = n0 ? bc0->cell_flags_for(a->dm[0], s) : nullptr;
     launch(owner->k_prep, static_cast<uint32_t>((owner->n + 255) / 256), 256, 0, s,
            static_cast<int32_t>(owner->n), owner->d_ents.ptr, a->mesh->cells.ptr, a->mesh->x.ptr,
-           a->packed.ptr, a->dm[0]->dev.ptr, n0, owner->records.ptr);
+           a->packed.ptr, a->dm[0]->dev.ptr, n0, cf, owner->records.ptr);
     // persistent grid: as many workgroups as fit on the device at once
     // (multiple of 8, one group of chunks per XCD); owner_persistent=0 gives
     // one workgroup per chunk
@@ -685,9 +686,10 @@ void assemble_batch(mf_batch* B, mf_matrix* const* As, const mf_bc* const* bcs, 
     MF_CHECK(plan != nullptr, MF_EUNSUPPORTED, "owner-computes schedule unavailable for group " +
                                                    std::to_string(g));
     const uint8_t* n0 = bc && bc->nbc ? bc->bits_for(a->dm[0]->bs, s) : nullptr;
+    const uint8_t* cf = n0 ? bc->cell_flags_for(a->dm[0], s) : nullptr;
     launch(B->preps[g], static_cast<uint32_t>((I->n + 255) / 256), 256, 0, s,
            static_cast<int32_t>(I->n), I->d_ents.ptr, a->mesh->cells.ptr, a->mesh->x.ptr,
-           a->packed.ptr, a->dm[0]->dev.ptr, n0, I->records.ptr);
+           a->packed.ptr, a->dm[0]->dev.ptr, n0, cf, I->records.ptr);
     h[g] = Args{plan->chunk_meta.ptr, plan->item_cell.ptr, plan->pair_list.ptr, plan->pair_kk.ptr,
                 I->records.ptr, a->consts.ptr, A->rpb.ptr, A->colsb.ptr, n0, A->values,
                 static_cast<int32_t>(plan->nchunks), 0};
@@ -728,6 +730,16 @@ void assemble_scalar(mf_form* M, double* d_value, int zero, hipStream_t s) {
 }
 
 }  // namespace mf
+
+const uint8_t* mf_bc::cell_flags_for(const mf_dofmap* dm, hipStream_t s) const {
+  auto it = cell_flags.find(dm->uid);
+  if (it != cell_flags.end()) return it->second.ptr;
+  const uint8_t* bits = bits_for(dm->bs, s);
+  mf::DevArray<uint8_t> f;
+  f.alloc(std::max<int64_t>(1, dm->num_cells));
+  mf::k_cell_flags(f.ptr, dm->dev.ptr, dm->num_cells, dm->nd, bits, s);
+  return cell_flags.emplace(dm->uid, std::move(f)).first->second.ptr;
+}
 
 const uint8_t* mf_bc::bits_for(int bs, hipStream_t s) const {
   auto it = node_bits.find(bs);

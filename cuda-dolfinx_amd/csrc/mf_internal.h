@@ -125,6 +125,9 @@ void k_build_block_table(int32_t* table, const int32_t* ents, int64_t nents, int
                          hipStream_t s);
 void k_mark(int8_t* markers, int64_t ndofs, const int32_t* dofs, int64_t nbc, hipStream_t s);
 void k_node_bits(uint8_t* bits, const int8_t* markers, int64_t nnodes, int bs, hipStream_t s);
+// flags[c] = 1 if any node of cell c (dofmap dm, nd nodes per cell) has bits
+void k_cell_flags(uint8_t* flags, const int32_t* dm, int64_t ncells, int nd, const uint8_t* bits,
+                  hipStream_t s);
 void k_pack(double* packed, int64_t ncells, int W, int off, const double* values,
             const int32_t* dofmap, int nd, int bs, hipStream_t s);
 void k_set_bc(double* b, int64_t n, const int32_t* dofs, int64_t nbc, const double* g,
@@ -172,6 +175,7 @@ struct mf_dofmap {
   mf::DevArray<int32_t> dev;
   int64_t num_cells = 0, num_nodes = 0;
   int nd = 0, bs = 1;
+  uint64_t uid = 0;
 };
 
 struct mf_bc {
@@ -186,6 +190,10 @@ struct mf_bc {
   // per-node component bits (bit a: scalar dof bs*node + a constrained), by block size
   mutable std::map<int, mf::DevArray<uint8_t>> node_bits;
   const uint8_t* bits_for(int bs, hipStream_t s) const;
+  // per cell of a dofmap: 1 if any of its nodes has a constrained component
+  // (the record passes gather node bits only for those cells), by dofmap uid
+  mutable std::map<uint64_t, mf::DevArray<uint8_t>> cell_flags;
+  const uint8_t* cell_flags_for(const mf_dofmap* dm, hipStream_t s) const;
 };
 
 namespace mf {

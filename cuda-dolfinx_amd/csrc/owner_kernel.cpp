@@ -112,8 +112,10 @@ extern "C" __global__ void __launch_bounds__(256)
 $KNAME(const int32_t n, const int32_t* __restrict__ ents, const int32_t* __restrict__ geo,
        const double* __restrict__ xg, const double* __restrict__ coeffs,
        const int32_t* __restrict__ dm0, const uint8_t* __restrict__ nb0,
-       mf_rec_$TAG* __restrict__ rec)
+       const uint8_t* __restrict__ cflag, mf_rec_$TAG* __restrict__ rec)
 {
+  // nb0: Dirichlet bits per node (null: none); cflag[c] != 0 where cell c has
+  // a constrained node (set with nb0), so other cells skip the bit gather
   constexpr int ND = SK_ND, BS = SK_BS, NV = $NV, W = $W;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
@@ -136,10 +138,11 @@ $KNAME(const int32_t n, const int32_t* __restrict__ ents, const int32_t* __restr
   r.det = fabs(det);
   for (int l = 0; l < $WW; ++l) r.w[l] = l < W ? coeffs[c * W + l] : 0.0;
   uint32_t bits = 0;
+  const bool gather = nb0 && cflag[c];
   for (int l = 0; l < ND; ++l) {
     const int32_t node = dm0[c * ND + l];
     r.d[l] = node;
-    if (nb0) bits |= (uint32_t)nb0[node] << (BS * l);
+    if (gather) bits |= (uint32_t)nb0[node] << (BS * l);
   }
   r.bits = bits;
   rec[c] = r;

@@ -74,6 +74,16 @@ __global__ void mark_kernel(int8_t* __restrict__ m, int64_t ndofs, const int32_t
   }
 }
 
+__global__ void cell_flags_kernel(uint8_t* __restrict__ flags, const int32_t* __restrict__ dm,
+                                  int64_t ncells, int nd, const uint8_t* __restrict__ bits) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += stride) {
+    uint8_t any = 0;
+    for (int l = 0; l < nd; ++l) any |= bits[dm[c * nd + l]];
+    flags[c] = any ? 1 : 0;
+  }
+}
+
 __global__ void node_bits_kernel(uint8_t* __restrict__ bits, const int8_t* __restrict__ m,
                                  int64_t nnodes, int bs) {
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -310,6 +320,12 @@ void k_build_block_table(int32_t* table, const int32_t* ents, int64_t nents, int
 void k_mark(int8_t* m, int64_t ndofs, const int32_t* dofs, int64_t nbc, hipStream_t s) {
   if (nbc <= 0) return;
   mark_kernel<<<grid_for(nbc), kBlock, 0, s>>>(m, ndofs, dofs, nbc);
+  MF_HIP(hipGetLastError());
+}
+void k_cell_flags(uint8_t* flags, const int32_t* dm, int64_t ncells, int nd, const uint8_t* bits,
+                  hipStream_t s) {
+  if (ncells <= 0) return;
+  cell_flags_kernel<<<grid_for(ncells), kBlock, 0, s>>>(flags, dm, ncells, nd, bits);
   MF_HIP(hipGetLastError());
 }
 void k_node_bits(uint8_t* bits, const int8_t* m, int64_t nnodes, int bs, hipStream_t s) {

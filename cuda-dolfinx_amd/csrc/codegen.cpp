@@ -230,8 +230,11 @@ int staged_define(const std::string& src, const std::string& name) {
 }
 
 int64_t record_bytes(const mf_form& f, const Integral& I) {
+  // layout of kCellRecord: {K[9], det} or, for scalar reference-tensor forms,
+  // the metric M[6]; then w[max(1, W)], d[nd], bits; 16-byte aligned
   const int nd = staged_define(I.staged_src, "SK_ND");
-  const int64_t raw = 72 + 8 + 8LL * std::max(1, f.W) + 4LL * nd + 4;
+  const int64_t geo = staged_trace(I) ? 48 : 72 + 8;
+  const int64_t raw = geo + 8LL * std::max(1, f.W) + 4LL * nd + 4;
   return (raw + 15) / 16 * 16;
 }
 

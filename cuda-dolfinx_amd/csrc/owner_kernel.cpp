@@ -98,8 +98,12 @@ const char* kCellRecord = R"(
 #ifndef MF_REC_$TAG
 #define MF_REC_$TAG
 struct __attribute__((aligned(16))) mf_rec_$TAG {
+#ifdef SK_TRACE
+  double M[6];                         // |det J| K K^T: M00 M11 M22 M01 M02 M12
+#else
   double K[9];
   double det;
+#endif
   double w[$WW];
   int32_t d[SK_ND];
   uint32_t bits;
@@ -132,10 +136,23 @@ $KNAME(const int32_t n, const int32_t* __restrict__ ents, const int32_t* __restr
   const double det = J00 * m00 + J10 * m01 + J20 * m02;
   const double id = 1.0 / det;
   mf_rec_$TAG r;
-  r.K[0] = m00 * id; r.K[1] = m01 * id; r.K[2] = m02 * id;
-  r.K[3] = (J12 * J20 - J10 * J22) * id; r.K[4] = (J00 * J22 - J02 * J20) * id; r.K[5] = (J02 * J10 - J00 * J12) * id;
-  r.K[6] = (J10 * J21 - J11 * J20) * id; r.K[7] = (J01 * J20 - J00 * J21) * id; r.K[8] = (J00 * J11 - J01 * J10) * id;
+  double K[9];
+  K[0] = m00 * id; K[1] = m01 * id; K[2] = m02 * id;
+  K[3] = (J12 * J20 - J10 * J22) * id; K[4] = (J00 * J22 - J02 * J20) * id; K[5] = (J02 * J10 - J00 * J12) * id;
+  K[6] = (J10 * J21 - J11 * J20) * id; K[7] = (J01 * J20 - J00 * J21) * id; K[8] = (J00 * J11 - J01 * J10) * id;
+#ifdef SK_TRACE
+  // scalar reference-tensor forms only need the metric (K[3 r + c] = dX_r / dx_c)
+  const double dt = fabs(det);
+  r.M[0] = dt * (K[0] * K[0] + K[1] * K[1] + K[2] * K[2]);
+  r.M[1] = dt * (K[3] * K[3] + K[4] * K[4] + K[5] * K[5]);
+  r.M[2] = dt * (K[6] * K[6] + K[7] * K[7] + K[8] * K[8]);
+  r.M[3] = dt * (K[0] * K[3] + K[1] * K[4] + K[2] * K[5]);
+  r.M[4] = dt * (K[0] * K[6] + K[1] * K[7] + K[2] * K[8]);
+  r.M[5] = dt * (K[3] * K[6] + K[4] * K[7] + K[5] * K[8]);
+#else
+  for (int x = 0; x < 9; ++x) r.K[x] = K[x];
   r.det = fabs(det);
+#endif
   for (int l = 0; l < $WW; ++l) r.w[l] = l < W ? coeffs[c * W + l] : 0.0;
   uint32_t bits = 0;
   const bool gather = nb0 && cflag[c];
@@ -248,8 +265,12 @@ $OHEAD($OPARAMS)
   for (int x = 0; x < $WW; ++x) u_w[x] = 0.0;
   auto mf_load_unit = [&](uint32_t pv) {
     const mf_rec_$TAG& r = rec[pv >> 6];
+#ifdef SK_TRACE
+    for (int x = 0; x < 6; ++x) u_K[x] = r.M[x];       // the metric, see the record
+#else
     for (int x = 0; x < 9; ++x) u_K[x] = r.K[x];
     u_det = r.det;
+#endif
     for (int x = 0; x < $WW; ++x) u_w[x] = r.w[x];
     u_di = r.d[pv & 63];
     u_bits = r.bits;
@@ -351,10 +372,9 @@ $OHEAD($OPARAMS)
         const double K0 = u_K[0], K1 = u_K[1], K2 = u_K[2], K3 = u_K[3], K4 = u_K[4], K5 = u_K[5];
         const double K6 = u_K[6], K7 = u_K[7], K8 = u_K[8], dt = u_det;
 #ifdef SK_TRACE
-        // metric M = |det J| K K^T (K[3 r + c] = dX_r / dx_c): tr G_ij = sum_rs M_rs R_ij[r][s]
-        const double M00 = dt * (K0 * K0 + K1 * K1 + K2 * K2), M11 = dt * (K3 * K3 + K4 * K4 + K5 * K5);
-        const double M22 = dt * (K6 * K6 + K7 * K7 + K8 * K8), M01 = dt * (K0 * K3 + K1 * K4 + K2 * K5);
-        const double M02 = dt * (K0 * K6 + K1 * K7 + K2 * K8), M12 = dt * (K3 * K6 + K4 * K7 + K5 * K8);
+        // metric M = |det J| K K^T from the record: tr G_ij = sum_rs M_rs R_ij[r][s]
+        const double M00 = K0, M11 = K1, M22 = K2, M01 = K3, M02 = K4, M12 = K5;
+        (void)K6; (void)K7; (void)K8; (void)dt;
 #if MF_OWNER_F32
         const float f00 = (float)M00, f11 = (float)M11, f22 = (float)M22;
         const float f01 = (float)M01, f02 = (float)M02, f12 = (float)M12;

@@ -345,7 +345,9 @@ def bench_c3(args):
                    "cg_ms_per_iteration": {str(p): sol[p][3] for p in sol}},
         "roofline": {"bound": "mfma", "achieved": flops / (times[32] * 1e-3) / 1e12, "peak": 157.3,
                      "unit": "TFLOP/s", "frac": flops / (times[32] * 1e-3) / 1e12 / 157.3,
-                     "traffic": None, "kernel": "matrix launch: mf_hprep_cell_0 + mf_howner32_cell_0",
+                     "traffic": (measured_traffic("c3_helmholtz_q3_48_traffic.md")
+                                 if args.n_hex == 48 else None),
+                     "kernel": "matrix launch: mf_hprep_cell_0 + mf_howner32_cell_0",
                      "fp64_achieved": flops / (times[64] * 1e-3) / 1e12, "fp64_peak": 78.6},
         "cpu_baseline": None if args.no_cpu else cpu_baseline_c3(),
     }

@@ -709,3 +709,26 @@ def test_multithreaded_oracle_matches_serial(which):
         for c in range(nc):
             d = dm[order[starts[c]:starts[c + 1]]].ravel()
             assert np.unique(d).size == d.size
+
+
+def test_jacobi_cg_matches_direct_solve():
+    """cudolfinx.solve.pcg (its torch sparse branch, on CPU tensors) solves the
+    oracle's P1 Poisson system with Dirichlet rows to a direct solve."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    import torch
+    from cudolfinx.solve import pcg
+    P = problems.poisson_p1(6)
+    Oa, OL = cpu.OracleAssembler(P.a), cpu.OracleAssembler(P.L)
+    rp, cols, _, _ = Oa.sparsity()
+    vals = Oa.assemble_matrix(rp, cols, P.bcs)
+    b = OL.assemble_vector()
+    Oa.apply_lifting(b, P.bcs)
+    cpu.set_bc(b, P.bcs, P.V)
+    S = sp.csr_matrix((vals, cols, rp), shape=(P.V.size, P.V.size))
+    M = torch.sparse_csr_tensor(torch.from_numpy(rp.astype(np.int64)), torch.from_numpy(cols.astype(np.int64)),
+                                torch.from_numpy(vals), size=S.shape)
+    x, it, res = pcg(M, torch.from_numpy(b), rtol=1e-12, maxiter=2000)
+    ref = spla.spsolve(S.tocsc(), b)
+    assert res < 1e-12 and 0 < it < 2000
+    assert np.linalg.norm(x.numpy() - ref) <= 1e-9 * np.linalg.norm(ref)

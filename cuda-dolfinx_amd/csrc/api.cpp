@@ -263,6 +263,7 @@ int mf_form_set_coefficient(mf_form* f, int idx, const double* values, int64_t n
       MF_HIP(hipMemcpy(c.own.ptr, values, sizeof(double) * n, hipMemcpyHostToDevice));
       c.ext = nullptr;
     }
+    c.dirty = true;
   });
 }
 
@@ -355,6 +356,7 @@ void compile_form(mf_form* f, int verbose) {
       f->consts.upload(&z, 1);
     }
     if (f->W > 0) f->packed.alloc(f->mesh->num_cells * (int64_t)f->W);
+    for (auto& c : f->coefs) c.dirty = true;       // the packed array is new
     std::string src = mf::compile_source(*f);
     f->source = src;
     bool cached = false;
@@ -442,13 +444,21 @@ int mf_form_pack_coefficients(mf_form* f, const int* idx, int n, mf_stream s) {
     if (idx) sel.assign(idx, idx + n);
     else
       for (int j = 0; j < (int)f->coefs.size(); ++j) sel.push_back(j);
+    // values uploaded through mf_form_set_coefficient are packed again only
+    // after they change; under graph capture every pack is recorded, since the
+    // graph replays after later uploads
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    MF_HIP(hipStreamIsCapturing(S(s), &cap));
+    const bool capturing = cap != hipStreamCaptureStatusNone;
     for (int j : sel) {
       MF_CHECK(j >= 0 && j < (int)f->coefs.size(), MF_EINVAL, "coefficient index out of range");
       auto& c = f->coefs[j];
       const double* v = c.ext ? c.ext : c.own.ptr;
       MF_CHECK(v, MF_ESTATE, "coefficient " + std::to_string(j) + " has no values");
+      if (!c.ext && !c.dirty && !capturing) continue;
       mf::k_pack(f->packed.ptr, f->mesh->num_cells, f->W, c.offset, v, c.dm->dev.ptr, c.dm->nd,
                  c.dm->bs, S(s));
+      if (!c.ext && !capturing) c.dirty = false;
     }
   });
 }

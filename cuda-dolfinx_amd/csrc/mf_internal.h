@@ -224,6 +224,9 @@ struct Coef {
   DevArray<double> own;
   const double* ext = nullptr;
   int offset = 0, width = 0;
+  // own values changed since they were last packed (a coefficient bound to
+  // caller device memory, ext, is always packed: it can change unseen)
+  bool dirty = true;
 };
 struct OwnerPlan;   // cooperative owner-computes schedule (matrix.cpp)
 // Cell-owner schedule for forms on discontinuous spaces (matrix.cpp)

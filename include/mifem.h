@@ -128,7 +128,11 @@ int mf_form_set_constants(mf_form* form, const double* values, int n);
 int mf_form_compile(mf_form* form, int verbose);
 /* Gathers coefficient values into the per-cell packed array on the device
  * (pack_coefficients, CUDAAssembler.h:115-283 and CUDAAssembler.cpp:59-96).
- * indices NULL: all coefficients. */
+ * indices NULL: all coefficients.  Values uploaded with
+ * mf_form_set_coefficient(..., on_device = 0) are packed again only after an
+ * upload or a compile (the reference repacks on every call); coefficients
+ * bound to caller device memory are always packed, and under stream capture
+ * every pack is recorded. */
 int mf_form_pack_coefficients(mf_form* form, const int* indices, int n, mf_stream stream);
 /* Tuning knobs: "matrix_kernel" (0 auto, 1 per-entity atomic scatter,
  * 2 cooperative owner-computes), "chunk_values" (owner-computes LDS budget

@@ -601,6 +601,35 @@ def test_native_spmv_diagonal_and_cg(which):
     assert res < 1e-12 and err <= 1e-8, (res, err)
 
 
+def test_coefficient_updates_repack():
+    _say("host coefficient changed between assemblies: repacked, matrices follow the new values")
+    import cudolfinx as cufem
+    from mifem import batch
+    mesh, V, kap = batch.poisson_instance(4, 2)
+    D = functionspace(mesh, ("DG", 0))
+    kf = Function(D)
+    kf.set_cellwise(kap)
+    bc = dirichletbc(0.0, locate_dofs_topological(V, 2, mesh.exterior_facets), V)
+    form = forms.stiffness(V, kf)
+    a = cufem.form(form)
+    asm = cufem.CUDAAssembler()
+    A = asm.create_matrix(a)
+    Oa = cpu.OracleAssembler(form)
+    orp, ocols, _, _ = Oa.sparsity()
+    rng = np.random.default_rng(3)
+    for rep in range(3):
+        # 0: first pack; 1: new host values, packed again; 2: unchanged, the pack is skipped
+        if rep == 1:
+            kf.x[:] = rng.uniform(0.5, 2.0, kf.x.size)
+            kf.version += 1
+        asm.assemble_matrix(a, A, bcs=[bc])
+        asm.synchronize()
+        ref = cpu.OracleAssembler(form).assemble_matrix(orp, ocols, [bc])
+        err = _rel(A.csr()[2], ref)
+        _say(f"  assembly {rep}: relerr A={err:.2e}")
+        assert err <= TOL, (rep, err)
+
+
 # ------------------------------------------------------------ block / restricted assembly
 def _selector(V, keep):
     """Scalar-dof selection matrix of the kept nodes (in order), bs-unrolled."""

@@ -562,8 +562,9 @@ def bench_c5(args):
     asm = cufem.CUDAAssembler()
     sys_ = []
     for g in B.groups:
-        # --c5-fp32: degree >= 2 element tensors and LDS accumulation in FP32 (CSR values FP64)
-        a = cufem.form(g.a, options={"precision": 32} if g.degree >= 2 and args.c5_fp32 else None)
+        # degree >= 2: element tensors in FP32 (FP64 geometry, LDS accumulation and CSR
+        # values); --c5-fp64 keeps every group in FP64
+        a = cufem.form(g.a, options={"precision": 32} if g.degree >= 2 and not args.c5_fp64 else None)
         L = cufem.form(g.L)
         A, b = asm.create_matrix(a), asm.create_vector(L)
         bcs = asm.pack_bcs(g.bcs)
@@ -668,12 +669,12 @@ def bench_c5(args):
         "metric": "independent-instance assemblies/s (A+b, 512 Poisson meshes, P1-P3)",
         "value": B.num_instances / (ms * 1e-3), "unit": "assemblies/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (P2, P3) / f64 (P1)" if args.c5_fp32 else "f64",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.c5_fp64 else "f32 element tensors (P2, P3) / f64 (P1)",
         "data": "synthetic (seeded sizes, degrees and lognormal conductivity)",
         "config": {"workload": f"many_poisson {B.num_instances} instances n~U{{4..24}} P1-P3",
                    "cells": ncells, "dofs": ndofs,
                    "groups": [{"degree": g.degree, "instances": len(g.instances), "nnz": A.nnz,
-                               "precision": 32 if g.degree >= 2 and args.c5_fp32 else 64}
+                               "precision": 32 if g.degree >= 2 and not args.c5_fp64 else 64}
                               for g, a, L, A, b, bcs in sys_],
                    "dof_per_s": ndofs / (ms * 1e-3), "matrix_ms": kms, "matrix_ms_per_group_launches": kms_sep,
                    "sampled_errors": errs,
@@ -682,7 +683,7 @@ def bench_c5(args):
         "roofline": {"bound": "hbm", "achieved": alg / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "traffic": (measured_traffic("c5_many_poisson_512_traffic.md")
-                                 if args.instances == 512 and not args.c5_fp32 else None), "kernel": "mf_owner_batch (all groups in one launch) + one record pass per group",
+                                 if args.instances == 512 else None), "kernel": "mf_owner_batch (all groups in one launch) + one record pass per group",
                      "algorithmic_bytes": alg},
         "cpu_baseline": None if args.no_cpu else cpu_baseline_c5(B),
     }
@@ -709,8 +710,8 @@ def main():
     ap.add_argument("--instances", type=int, default=512, help="instances for c5")
     ap.add_argument("--c5-sample", type=int, default=4, help="oracle-checked instances per group")
     ap.add_argument("--c5-oracle-n", type=int, default=10, help="largest n checked by the oracle")
-    ap.add_argument("--c5-fp32", action="store_true",
-                    help="c5: degree >= 2 groups accumulate in FP32 (slower on this GPU: FP32 LDS atomics)")
+    ap.add_argument("--c5-fp64", action="store_true",
+                    help="c5: every group in FP64 (default: degree >= 2 element tensors in FP32)")
     ap.add_argument("--n-dg", type=int, default=48, help="cubes per direction for c4")
     ap.add_argument("--newton", type=int, default=4, help="Newton steps for c4 (after the timed region)")
     ap.add_argument("--n-hex", type=int, default=48, help="hexahedra per direction for c3")

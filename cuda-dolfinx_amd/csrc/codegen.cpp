@@ -439,10 +439,10 @@ static std::vector<LdsSlice> owner_lds_slices(const mf_form& f, const Integral& 
   const int64_t nd = staged_define(I.staged_src, "SK_ND"), bs = staged_define(I.staged_src, "SK_BS");
   const int64_t cap = f.opt("chunk_values", 4096), maxr = f.opt("chunk_rows", 64);
   const int64_t maxp = owner_shape(f, I).maxp;
-  const bool f32 = owner_f32(f, I);
+  const bool f32 = owner_f32(f, I), f32acc = f32 && f.opt("owner_f32acc", 0);
   const char* at = f32 ? "float" : "double";
   const int64_t as = f32 ? 4 : 8;
-  std::vector<LdsSlice> v = {{"acc", at, r16(as * cap)},
+  std::vector<LdsSlice> v = {{"acc", f32acc ? "float" : "double", r16((f32acc ? 4 : 8) * cap)},
                              {"lrp", "int32_t", r16(4 * (maxr + 1))},
                              {"snb", "uint8_t", r16(maxr)},
                              {"lcols", "int32_t", r16(4 * (cap / (bs * bs) + 1))},
@@ -506,7 +506,8 @@ std::string generate_batch_source(const std::vector<const mf_form*>& forms,
       off += sl.bytes;
     }
     m["$OPARAMS"] = std::string(kOwnerParams) + sparams;
-    os << "#undef MF_OWNER_F32\n#define MF_OWNER_F32 " << (owner_f32(f, I) ? 1 : 0) << "\n";
+    os << "#undef MF_OWNER_F32\n#define MF_OWNER_F32 "
+       << (owner_f32(f, I) ? (f.opt("owner_f32acc", 0) ? 1 : 2) : 0) << "\n";
     os << "#undef MF_SHARED_BEGIN\n#undef MF_SHARED\n"
           "#define MF_SHARED_BEGIN do {} while (0)\n"
           "#define MF_SHARED(T, name, count) T* const name = mf_slice_##name\n";
@@ -715,7 +716,8 @@ std::string generate_form_source(const mf_form& f) {
             m["$KNAME"] = kernel_name("owner32", I->type, I->index);
             m["$OHEAD"] = "extern \"C\" __global__ void __launch_bounds__($OBLOCK) " + attr + "\n" +
                           m["$KNAME"];
-            os << "#undef MF_OWNER_F32\n#define MF_OWNER_F32 1\n" << subst(rename(kOwnerTemplate), m)
+            os << "#undef MF_OWNER_F32\n#define MF_OWNER_F32 " << (f.opt("owner_f32acc", 0) ? 1 : 2)
+               << "\n" << subst(rename(kOwnerTemplate), m)
                << "#undef MF_OWNER_F32\n#define MF_OWNER_F32 0\n";
           }
         }

@@ -37,10 +37,10 @@ const char* kOwnerCommon = R"(
 #define MF_SHARED(T, name, count) __shared__ __attribute__((aligned(16))) T name[count]
 typedef double mf_d2 __attribute__((ext_vector_type(2)));
 typedef float mf_f2 __attribute__((ext_vector_type(2)));
-// MF_OWNER_F32 (defined 0 or 1 before each body): 1 accumulates the element
-// contributions of a scalar reference-tensor form in FP32 (reference tensors,
-// element arithmetic and the LDS accumulator); geometry stays FP64 and the
-// CSR values are written as FP64
+// MF_OWNER_F32 (defined 0, 1 or 2 before each body), for scalar
+// reference-tensor forms: 1 and 2 keep the reference tensors in FP32 and form
+// each entry with FP32 arithmetic; 1 also accumulates in an FP32 LDS array, 2
+// in the FP64 one.  Geometry stays FP64 and the CSR values are written as FP64
 #define MF_OWNER_F32 0
 static __device__ __forceinline__ int mf_lds_lower_bound(const int32_t* a, int lo, int hi, int32_t key)
 {
@@ -199,10 +199,15 @@ $OHEAD($OPARAMS)
   constexpr int PF_P = (MAXP + NT - 1) / NT, PF_K = (KK4 + NT - 1) / NT;
   constexpr int PF_R = (MAXR + 1 + NT - 1) / NT, PF_C = (CAPB + NT - 1) / NT;
   constexpr int PF_B = (MAXR + NT - 1) / NT;
-#if MF_OWNER_F32
+#if MF_OWNER_F32 == 1
   typedef float mf_acc_t;
 #else
   typedef double mf_acc_t;
+#endif
+#if MF_OWNER_F32
+  typedef float mf_rt_t;
+#else
+  typedef double mf_rt_t;
 #endif
   static_assert(CAP * sizeof(mf_acc_t) % 16 == 0, "chunk_values: the accumulator is zeroed in 16-byte words");
   MF_SHARED(mf_acc_t, acc, CAP);
@@ -213,8 +218,8 @@ $OHEAD($OPARAMS)
   MF_SHARED(uint8_t, skk, KK4 * 16);
 #ifdef SK_TRACE
   // reference tensors R_ij (symmetrised, 6 per entry), loaded once per workgroup
-  MF_SHARED(mf_acc_t, sRS, ND * ND * 6);
-  for (int p = threadIdx.x; p < ND * ND * 6; p += NT) sRS[p] = (mf_acc_t)(&SK_RS[0][0][0])[p];
+  MF_SHARED(mf_rt_t, sRS, ND * ND * 6);
+  for (int p = threadIdx.x; p < ND * ND * 6; p += NT) sRS[p] = (mf_rt_t)(&SK_RS[0][0][0])[p];
 #elif defined(SK_FULLRT)
   // full reference tensors R_ij[3 r + s] (9 per entry), loaded once per workgroup
   MF_SHARED(double, sRT, ND * ND * 9);
@@ -410,7 +415,7 @@ $OHEAD($OPARAMS)
           double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #ifdef SK_TRACE
           {
-            const mf_acc_t* R = &sRS[(i * ND + j) * 6];
+            const mf_rt_t* R = &sRS[(i * ND + j) * 6];
 #if MF_OWNER_F32
             G[0] = (double)(f00 * R[0] + f11 * R[1] + f22 * R[2] + f01 * R[3] + f02 * R[4] + f12 * R[5]);
 #else
@@ -474,7 +479,7 @@ $OHEAD($OPARAMS)
 
     // write the segment once and re-zero what was read
     double* out = values + (int64_t)(uint32_t)base;
-#if MF_OWNER_F32
+#if MF_OWNER_F32 == 1
     if ((flags & 1) && ((base & 1) == 0)) {
       mf_d2* o2 = reinterpret_cast<mf_d2*>(out);
       mf_f2* f2 = reinterpret_cast<mf_f2*>(acc);

@@ -138,7 +138,9 @@ int mf_form_pack_coefficients(mf_form* form, const int* indices, int n, mf_strea
  * 2 cooperative owner-computes), "chunk_values" (owner-computes LDS budget
  * in values), "block" (threads per workgroup), "precision" (64; 32 / 16 on
  * the hexahedral matrix-core kernel, 32 on the simplex owner-computes kernel
- * of scalar reference-tensor forms: MF_EUNSUPPORTED at assembly otherwise). */
+ * of scalar reference-tensor forms, FP32 element arithmetic with FP64
+ * accumulation unless "owner_f32acc" is 1: MF_EUNSUPPORTED at assembly
+ * otherwise). */
 int mf_form_set_option(mf_form* form, const char* key, int64_t value);
 /* Block / restricted assembly (CUDAForm::set_restriction, CUDAForm.h:201-231,
  * and the offset + restriction unrolling of CUDADofMap::update,

@@ -122,6 +122,98 @@ def test_p2_elasticity_element_rigid_modes_and_psd():
     np.testing.assert_allclose(A2, 2 * A, rtol=1e-12, atol=1e-13 * abs(A).max())
 
 
+def _p2_nodes(x):
+    """Physical coordinates of the P2 nodes of the tetrahedron x (4 x 3)."""
+    ref = E.Element("P", "tetrahedron", 2).nodes
+    lam = np.concatenate([1 - ref.sum(1, keepdims=True), ref], axis=1)
+    return lam @ x
+
+
+# Linear displacement fields and their elastic energy densities
+# 2 mu eps(u):eps(u) + lambda (div u)^2 (constant for a linear field):
+#   u = (x, 0, 0): eps = diag(1, 0, 0)          -> 2 mu + lambda
+#   u = (y, 0, 0): eps_xy = eps_yx = 1/2, div 0 -> mu
+#   u = (x, y, z): eps = I, div u = 3            -> 6 mu + 9 lambda
+_ENERGY_FIELDS = [
+    (lambda p: np.array([p[0], 0.0, 0.0]), lambda mu, lam: 2 * mu + lam),
+    (lambda p: np.array([p[1], 0.0, 0.0]), lambda mu, lam: mu),
+    (lambda p: np.array([p[0], p[1], p[2]]), lambda mu, lam: 6 * mu + 9 * lam),
+]
+
+
+@pytest.mark.parametrize("mu,lam", [(2.0, 3.0), (3.0, 2.0), (1.0, 7.5)])
+def test_p2_elasticity_element_closed_form_energies(mu, lam):
+    """u^T A u of linear fields on one random tetrahedron equals the closed-form
+    energy times |T|: the three fields tell mu from lambda, so a swapped pair,
+    a dropped lambda div-div term or a dropped mu trace term fails."""
+    rng = np.random.default_rng(7)
+    x = _tet(rng)
+    vol = abs(np.linalg.det((x[1:] - x[0]).T)) / 6
+    spec = dict(kind="elasticity", cell=0, degree=2, bs=3, rank=2, qdeg=2,
+                mu=("w", 0), lam=("w", 1))
+    A = cpu.tabulate(spec, [mu, lam], [], x)
+    X = _p2_nodes(x)
+    for field, energy in _ENERGY_FIELDS:
+        u = np.concatenate([field(p) for p in X])
+        assert abs(u @ A @ u - energy(mu, lam) * vol) < 1e-12 * energy(mu, lam) * vol
+
+
+def test_elasticity_global_energy_cellwise_lame():
+    """Global oracle assembly with the seeded per-cell Lame fields: u^T A u of
+    each linear field is sum_c energy(mu_c, lambda_c) |T_c| (pins that each
+    cell reads its own coefficients, in the right slots)."""
+    P = problems.elasticity_p2(2)
+    O = cpu.OracleAssembler(P.a)
+    rp, cols, _, _ = O.sparsity()
+    A = _dense(rp, cols, O.assemble_matrix(rp, cols), P.V.size)
+    mesh = P.mesh
+    dg = P.extras["mu"].function_space.dofmap[:, 0]
+    mu_c, lam_c = P.extras["mu"].x[dg], P.extras["lambda"].x[dg]
+    assert np.ptp(mu_c) > 1 and np.ptp(lam_c) > 1
+    xc = mesh.x[mesh.cells]
+    vol = np.abs(np.linalg.det(np.transpose(xc[:, 1:] - xc[:, :1], (0, 2, 1)))) / 6
+    X = P.V.node_coordinates
+    for field, energy in _ENERGY_FIELDS:
+        u = np.concatenate([field(p) for p in X])
+        exact = float(np.sum(energy(mu_c, lam_c) * vol))
+        assert abs(u @ A @ u - exact) < 1e-12 * exact
+
+
+def _hex_dense(P, spec):
+    """Dense global matrix of a hexahedral form from per-cell oracle tabulation."""
+    mesh, V = P.mesh, P.V
+    A = np.zeros((V.size, V.size))
+    for c in range(mesh.num_cells):
+        d = V.dofmap[c]
+        A[np.ix_(d, d)] += cpu.tabulate(spec, [], [], mesh.x[mesh.cells[c]])
+    return A
+
+
+@pytest.mark.parametrize("degree", [2, 3])
+def test_jittered_hex_linear_patch(degree):
+    """Trilinear (non-affine) hexahedra: for u = a . x, which Q_k holds exactly,
+    the stiffness rows of interior nodes vanish and u^T K u = |a|^2 vol(=1);
+    the mass gives u^T M u = int_(unit cube) (a . x)^2.  Every integrand is a
+    polynomial the Gauss rule integrates exactly, so these hold to round-off;
+    a transposed Jacobian or a wrong inverse on non-affine cells breaks them."""
+    P = problems.helmholtz_q3(3, degree=degree)
+    spec = dict(P.a.integrals[0].spec)
+    a = np.array([0.3, -0.7, 0.5])
+    X = P.V.node_coordinates
+    u = X @ a
+    spec.update(kappa=("val", 1.0), k2=0.0)
+    K = _hex_dense(P, spec)
+    interior = np.all((X > 1e-9) & (X < 1 - 1e-9), axis=1)
+    assert interior.sum() > 0
+    assert np.abs((K @ u)[interior]).max() < 1e-13
+    assert abs(u @ K @ u - a @ a) < 1e-13
+    spec.update(kappa=("val", 0.0), k2=-1.0)
+    M = _hex_dense(P, spec)
+    # int (a . x)^2 over [0,1]^3 = sum_i a_i^2 / 3 + sum_{i != j} a_i a_j / 4
+    exact = (a @ a) / 3 + (a.sum() ** 2 - a @ a) / 4
+    assert abs(u @ M @ u - exact) < 1e-13
+
+
 def _gll1d_matrices(k, m):
     t = E.gll_points(k)
     V = np.vander(t, k + 1, increasing=True)

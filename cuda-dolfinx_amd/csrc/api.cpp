@@ -287,7 +287,8 @@ int mf_form_set_option(mf_form* f, const char* key, int64_t value) {
                                   "owner_block", "owner_nb", "owner_lds", "vector_kernel",
                                   "vector_block", "vector_chunk_values", "lift_kernel",
                                   "owner_pairs", "precision", "dg_block", "owner_persistent",
-                                  "owner_waves", "owner_wpe", "hex_block", "dg_wpe", "owner_f32acc"};
+                                  "owner_waves", "owner_wpe", "hex_block", "dg_wpe", "owner_f32acc",
+                                  "owner_pipe", "owner_cwaves"};
     bool ok = false;
     for (auto k : known) ok |= std::strcmp(k, key) == 0;
     MF_CHECK(ok, MF_EINVAL, std::string("unknown option ") + key);
@@ -296,7 +297,8 @@ int mf_form_set_option(mf_form* f, const char* key, int64_t value) {
                              std::strcmp(key, "vector_kernel") == 0 ||
                              std::strcmp(key, "lift_kernel") == 0 ||
                              std::strcmp(key, "owner_persistent") == 0 ||
-                             std::strcmp(key, "owner_waves") == 0;
+                             std::strcmp(key, "owner_waves") == 0 ||
+                             std::strcmp(key, "owner_pipe") == 0;
     MF_CHECK(!f->compiled || runtime_key, MF_ESTATE,
              "kernel shape options must be set before compile");
     f->options[key] = value;
@@ -350,7 +352,10 @@ void compile_form(mf_form* f, int verbose) {
   {
     MF_CHECK(!f->ints.empty(), MF_ESTATE, "form has no integrals");
     use_device(f->ctx);
-    for (auto& I : f->ints) I->owner_grid = I->owner_grid32 = 0;    // the kernels below are new
+    for (auto& I : f->ints)                                           // the kernels below are new
+      I->owner_grid = I->owner_grid32 = I->owner_gridp = I->owner_gridp32 = 0;
+    ++f->gen;                          // schedules built for an earlier compile are stale
+    f->vplans.clear();
     if (!f->consts.ptr) {
       double z = 0.0;
       f->consts.upload(&z, 1);
@@ -384,6 +389,11 @@ void compile_form(mf_form* f, int verbose) {
             I->k_owner32 = mf::staged_trace(*I)
                                ? f->mod->get(mf::kernel_name("owner32", I->type, I->index))
                                : nullptr;
+            const bool pipe = mf::owner_pipe_emitted(*f, *I);
+            I->k_ownerp = pipe ? f->mod->get(mf::kernel_name("ownerp", I->type, I->index)) : nullptr;
+            I->k_ownerp32 = pipe && mf::staged_trace(*I)
+                                ? f->mod->get(mf::kernel_name("ownerp32", I->type, I->index))
+                                : nullptr;
             I->k_prep = f->mod->get(mf::kernel_name("prep", I->type, I->index));
             I->record_bytes = mf::record_bytes(*f, *I);
             I->records.alloc(f->mesh->num_cells * I->record_bytes / 8);
@@ -468,10 +478,12 @@ int mf_form_destroy(mf_form* f) {
 }
 
 // ------------------------------------------------------------------ matrix
-int mf_matrix_create(mf_form* a, mf_matrix** out) {
-  return guard([&] {
-    MF_CHECK(a && out, MF_EINVAL, "null argument");
-    MF_CHECK(a->rank == 2, MF_EINVAL, "create_matrix needs a bilinear form");
+}  // extern "C"
+
+namespace {
+// The matrix of a bilinear form on its own pattern (every integral's entries,
+// sorted unique blocked columns per blocked row, bs-unrolled).
+std::unique_ptr<mf_matrix> create_form_matrix(mf_form* a) {
     use_device(a->ctx);
     auto A = std::make_unique<mf_matrix>();
     A->form = a;
@@ -499,6 +511,129 @@ int mf_matrix_create(mf_form* a, mf_matrix** out) {
     A->own_values.alloc(std::max<int64_t>(1, A->nnz));
     A->values = A->own_values.ptr;
     mf::k_fill(A->values, A->nnz, 0.0, nullptr);
+    MF_HIP(hipDeviceSynchronize());
+    return A;
+}
+}  // namespace
+
+extern "C" {
+
+int mf_matrix_create(mf_form* a, mf_matrix** out) {
+  return guard([&] {
+    MF_CHECK(a && out, MF_EINVAL, "null argument");
+    MF_CHECK(a->rank == 2, MF_EINVAL, "create_matrix needs a bilinear form");
+    *out = create_form_matrix(a).release();
+  });
+}
+
+int mf_matrix_create_from_csr(mf_form* a, const int32_t* row_ptr, const int32_t* cols,
+                              int64_t n_rows, int64_t nnz, double* d_values, mf_matrix** out) {
+  return guard([&] {
+    MF_CHECK(a && row_ptr && (cols || nnz == 0) && out, MF_EINVAL, "null argument");
+    MF_CHECK(a->rank == 2, MF_EINVAL, "create_matrix needs a bilinear form");
+    const int bs0 = a->dm[0]->bs, bs1 = a->dm[1]->bs;
+    const int64_t nrows = a->dm[0]->num_nodes * bs0, ncols = a->dm[1]->num_nodes * bs1;
+    MF_CHECK(n_rows == nrows, MF_ESHAPE,
+             "CSR has " + std::to_string(n_rows) + " rows, the test space " + std::to_string(nrows));
+    MF_CHECK(nnz >= 0 && nnz < INT32_MAX, MF_EUNSUPPORTED, "nnz outside int32 CSR indexing");
+    MF_CHECK(row_ptr[0] == 0 && row_ptr[n_rows] == nnz, MF_EINVAL, "row_ptr must run from 0 to nnz");
+    for (int64_t r = 0; r < n_rows; ++r) {
+      MF_CHECK(row_ptr[r] <= row_ptr[r + 1], MF_EINVAL, "row_ptr decreases at row " + std::to_string(r));
+      for (int32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p) {
+        MF_CHECK(cols[p] >= 0 && cols[p] < ncols, MF_EINVAL, "column out of range in row " + std::to_string(r));
+        MF_CHECK(p == row_ptr[r] || cols[p - 1] < cols[p], MF_EINVAL,
+                 "columns must be sorted and unique in row " + std::to_string(r));
+      }
+    }
+    use_device(a->ctx);
+    // every entry of the form's pattern must be present
+    std::vector<int32_t> frpb, fcolsb;
+    mf::build_pattern(*a, frpb, fcolsb);
+    int64_t missing = 0;
+    for (int64_t R = 0; R < a->dm[0]->num_nodes; ++R)
+      for (int i = 0; i < bs0; ++i) {
+        const int32_t* rb = cols + row_ptr[bs0 * R + i];
+        const int32_t* re = cols + row_ptr[bs0 * R + i + 1];
+        const int32_t* q = rb;
+        for (int32_t k = frpb[R]; k < frpb[R + 1]; ++k)
+          for (int j = 0; j < bs1; ++j) {
+            const int32_t c = bs1 * fcolsb[k] + j;
+            q = std::lower_bound(q, re, c);
+            if (q == re || *q != c) ++missing;
+          }
+      }
+    MF_CHECK(missing == 0, MF_ENOTFOUND,
+             std::to_string(missing) + " entries of the form's pattern are missing from the CSR");
+    // bs-unrolled block structure: blocked row R's bs0 scalar rows are equal
+    // and hold whole groups bs1*C .. bs1*C + bs1 - 1
+    bool blocked = true;
+    std::vector<int32_t> rpb(a->dm[0]->num_nodes + 1, 0), colsb;
+    for (int64_t R = 0; R < a->dm[0]->num_nodes && blocked; ++R) {
+      const int32_t p0 = row_ptr[bs0 * R], len = row_ptr[bs0 * R + 1] - p0;
+      blocked = len % bs1 == 0;
+      for (int i = 1; i < bs0 && blocked; ++i) {
+        const int32_t pi = row_ptr[bs0 * R + i];
+        blocked = row_ptr[bs0 * R + i + 1] - pi == len &&
+                  std::equal(cols + p0, cols + p0 + len, cols + pi);
+      }
+      for (int32_t g = 0; g < len / bs1 && blocked; ++g) {
+        const int32_t c0 = cols[p0 + g * bs1];
+        blocked = c0 % bs1 == 0;
+        for (int j = 1; j < bs1 && blocked; ++j) blocked = cols[p0 + g * bs1 + j] == c0 + j;
+        if (blocked) colsb.push_back(c0 / bs1);
+      }
+      rpb[R + 1] = static_cast<int32_t>(colsb.size());
+    }
+    auto A = std::make_unique<mf_matrix>();
+    A->form = a;
+    A->nrows = nrows;
+    A->ncols = ncols;
+    A->nnz = nnz;
+    if (blocked) {
+      A->bs0 = bs0;
+      A->bs1 = bs1;
+      A->nrows_b = a->dm[0]->num_nodes;
+      A->h_rpb = std::move(rpb);
+      A->h_colsb = std::move(colsb);
+      A->nnz_b = A->h_rpb.back();
+      A->rpb.upload(A->h_rpb);
+      A->colsb.upload(A->h_colsb);
+    } else {
+      // the form assembles on its own pattern; its values are added into
+      // this one through a position map (every form entry is present)
+      A->inner = create_form_matrix(a);
+      const mf_matrix& In = *A->inner;
+      std::vector<int32_t> map(std::max<int64_t>(1, In.nnz));
+      int64_t k = 0;
+      for (int64_t R = 0; R < In.nrows_b; ++R)
+        for (int i = 0; i < In.bs0; ++i) {
+          const int64_t r = In.bs0 * R + i;
+          const int32_t* rb = cols + row_ptr[r];
+          const int32_t* re = cols + row_ptr[r + 1];
+          const int32_t* q = rb;
+          for (int32_t kb = In.h_rpb[R]; kb < In.h_rpb[R + 1]; ++kb)
+            for (int j = 0; j < In.bs1; ++j) {
+              q = std::lower_bound(q, re, In.bs1 * In.h_colsb[kb] + j);
+              map[k++] = static_cast<int32_t>(q - cols);
+            }
+        }
+      MF_CHECK(k == In.nnz, MF_ESTATE, "pattern map size mismatch");
+      A->inner_map.upload(map);
+      A->bs0 = A->bs1 = 1;
+      A->nrows_b = nrows;
+      A->nnz_b = nnz;
+    }
+    A->row_ptr.upload(row_ptr, n_rows + 1);
+    A->cols.alloc(std::max<int64_t>(1, nnz));
+    if (nnz)
+      MF_HIP(hipMemcpy(A->cols.ptr, cols, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+    if (d_values) {
+      A->values = d_values;
+    } else {
+      A->own_values.alloc(std::max<int64_t>(1, nnz));
+      A->values = A->own_values.ptr;
+      mf::k_fill(A->values, nnz, 0.0, nullptr);
+    }
     MF_HIP(hipDeviceSynchronize());
     *out = A.release();
   });
@@ -622,6 +757,10 @@ int mf_bc_create(mf_context* ctx, int64_t ndofs, const int32_t* dofs, int64_t nb
     b->nbc = nbc;
     b->n_owned = n_owned;
     b->uid = g_uid++;
+    {
+      std::string k(reinterpret_cast<const char*>(dofs), nbc ? sizeof(int32_t) * nbc : 0);
+      b->key = mf::fnv1a(k, mf::fnv1a(std::to_string(ndofs) + "/" + std::to_string(n_owned)));
+    }
     if (nbc) b->dofs.upload(dofs, nbc);
     b->g.upload(g, ndofs);
     b->h_dofs.assign(dofs, dofs + nbc);
@@ -1007,12 +1146,22 @@ int mf_apply_lifting(mf_form* const* a, int nforms, double* b, int64_t n,
                "bc size does not match the trial space");
       use_device(f->ctx);
       const double* xk = x0 ? x0[k] : nullptr;
-      if (!xk && bc->g_zero) continue;   // b -= scale A[:, bc] (g - x0) with g - x0 = 0
+      // Under graph capture the launches are recorded even when g - x0 = 0
+      // now: the constrained values may change before a replay (mf_bc_update).
+      // The entity lists are built eagerly (the count needs a host read), so
+      // a captured call finds them from an earlier uncaptured call.
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      MF_HIP(hipStreamIsCapturing(S(s), &cs));
+      const bool capturing = cs != hipStreamCaptureStatusNone;
+      const bool nothing = !xk && bc->g_zero && !capturing;   // b -= scale A[:, bc] (g - x0) = 0
       const uint32_t block = static_cast<uint32_t>(f->opt("block", 128));
       for (auto& I : f->ints) {
         if (I->n == 0) continue;
-        auto it = I->lift_lists.find(bc->uid);
+        auto it = I->lift_lists.find(bc->key);
         if (it == I->lift_lists.end()) {
+          MF_CHECK(!capturing, MF_ESTATE,
+                   "apply_lifting under graph capture needs one uncaptured call with this form "
+                   "and Dirichlet set first (it builds the lifting entity lists)");
           mf::DevArray<int32_t> list, cnt;
           list.alloc(std::max<int64_t>(1, I->n));
           cnt.alloc(1);
@@ -1023,10 +1172,10 @@ int mf_apply_lifting(mf_form* const* a, int nforms, double* b, int64_t n,
           int32_t h = 0;
           MF_HIP(hipMemcpyAsync(&h, cnt.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, S(s)));
           MF_HIP(hipStreamSynchronize(S(s)));
-          it = I->lift_lists.emplace(bc->uid, std::make_pair(std::move(list), (int64_t)h)).first;
+          it = I->lift_lists.emplace(bc->key, std::make_pair(std::move(list), (int64_t)h)).first;
         }
         const int64_t nl = it->second.second;
-        if (nl == 0) continue;
+        if (nl == 0 || nothing) continue;
         if (I->k_slift && f->opt("lift_kernel", 0) != 1) {
           const int64_t nt = nl * f->dm[0]->nd;
           mf::launch(I->k_slift, static_cast<uint32_t>((nt + 127) / 128), 128, 0, S(s),

@@ -303,6 +303,17 @@ OwnerShape owner_shape(const mf_form& f, const Integral& I) {
   return sh;
 }
 
+bool owner_pipe_emitted(const mf_form& f, const Integral& I) {
+  // the warp-specialised kernel holds two accumulators and two sets of
+  // chunk lists in LDS; it is emitted when they fit the 160 KB of a CU
+  const int64_t nd = staged_define(I.staged_src, "SK_ND");
+  const int64_t cap = f.opt("chunk_values", 4096), maxr = f.opt("chunk_rows", 64);
+  const int64_t maxp = owner_shape(f, I).maxp;
+  const int64_t lists = 4 * maxp + (maxp * nd + 15) / 16 * 16 + 4 * (maxr + 1) + maxr;
+  const int64_t table = staged_trace(I) ? 8 * nd * nd * 6 : (staged_fullrt(I) ? 8 * nd * nd * 9 : 0);
+  return f.opt("owner_f32acc", 0) == 0 && 16 * cap + 2 * lists + table + 256 <= 160 * 1024;
+}
+
 std::string kernel_name(const char* what, int type, int index) {
   static const char* tn[] = {"cell", "ext", "int"};
   return std::string("mf_") + what + "_" + tn[type] + "_" + std::to_string(index);
@@ -310,6 +321,9 @@ std::string kernel_name(const char* what, int type, int index) {
 
 extern const char* kOwnerTemplate;         // owner_kernel.cpp
 extern const char* kOwnerParams;
+extern const char* kOwnerPipeTemplate;
+extern const char* kOwnerPipeParams;
+extern const char* kOwnerUnit;
 extern const char* kOwnerBatchKernel;
 extern const char* kOwnerCommon;
 extern const char* kOwnerVectorTemplate;
@@ -332,6 +346,13 @@ extern const char* kDGSystem;
 extern const char* kDGSysCell;
 extern const char* kDGSysExt;
 extern const char* kDGSysIntBody;
+
+// An owner-computes template with the shared per-unit body ($UNIT) inserted.
+static std::string owner_template(const char* t) {
+  std::string s = t;
+  replace_all(s, "$UNIT", kOwnerUnit);
+  return s;
+}
 
 bool dg_owner_emitted(const mf_form& f) {
   if (f.rank == 0) return false;
@@ -511,7 +532,7 @@ std::string generate_batch_source(const std::vector<const mf_form*>& forms,
     os << "#undef MF_SHARED_BEGIN\n#undef MF_SHARED\n"
           "#define MF_SHARED_BEGIN do {} while (0)\n"
           "#define MF_SHARED(T, name, count) T* const name = mf_slice_##name\n";
-    os << subst(rename(kOwnerTemplate), m);
+    os << subst(rename(owner_template(kOwnerTemplate)), m);
     os << "#undef MF_SHARED_BEGIN\n#undef MF_SHARED\n"
           "#define MF_SHARED_BEGIN do {} while (0)\n"
           "#define MF_SHARED(T, name, count) __shared__ __attribute__((aligned(16))) T name[count]\n"
@@ -710,15 +731,34 @@ std::string generate_form_source(const mf_form& f) {
           m["$OHEAD"] = "extern \"C\" __global__ void __launch_bounds__($OBLOCK) " + attr + "\n" +
                         m["$KNAME"];
           m["$OPARAMS"] = kOwnerParams;
-          os << subst(rename(kOwnerTemplate), m);
+          os << subst(rename(owner_template(kOwnerTemplate)), m);
+          // warp-specialised variant: owner_cwaves compute waves + one store wave
+          const bool pipe = owner_pipe_emitted(f, *I);
+          const int64_t ncw = f.opt("owner_cwaves", 4);
+          m["$ONCW"] = std::to_string(ncw);
+          m["$KNAME"] = kernel_name("ownerp", I->type, I->index);
+          m["$OHEAD"] = "extern \"C\" __global__ void __launch_bounds__(" + std::to_string(64 * (ncw + 1)) +
+                        ") " + attr + "\n" + m["$KNAME"];
+          m["$OPARAMS"] = kOwnerPipeParams;
+          if (pipe) os << subst(rename(owner_template(kOwnerPipeTemplate)), m);
+          m["$OPARAMS"] = kOwnerParams;
           if (staged_trace(*I)) {
             // FP32 accumulation variant (precision = 32)
             m["$KNAME"] = kernel_name("owner32", I->type, I->index);
             m["$OHEAD"] = "extern \"C\" __global__ void __launch_bounds__($OBLOCK) " + attr + "\n" +
                           m["$KNAME"];
             os << "#undef MF_OWNER_F32\n#define MF_OWNER_F32 " << (f.opt("owner_f32acc", 0) ? 1 : 2)
-               << "\n" << subst(rename(kOwnerTemplate), m)
-               << "#undef MF_OWNER_F32\n#define MF_OWNER_F32 0\n";
+               << "\n" << subst(rename(owner_template(kOwnerTemplate)), m);
+            if (pipe) {
+              // FP32 element arithmetic, FP64 accumulator: warp-specialised too
+              m["$KNAME"] = kernel_name("ownerp32", I->type, I->index);
+              m["$OHEAD"] = "extern \"C\" __global__ void __launch_bounds__(" +
+                            std::to_string(64 * (ncw + 1)) + ") " + attr + "\n" + m["$KNAME"];
+              m["$OPARAMS"] = kOwnerPipeParams;
+              os << subst(rename(owner_template(kOwnerPipeTemplate)), m);
+              m["$OPARAMS"] = kOwnerParams;
+            }
+            os << "#undef MF_OWNER_F32\n#define MF_OWNER_F32 0\n";
           }
         }
         m["$KNAME"] = kernel_name("slift", I->type, I->index);

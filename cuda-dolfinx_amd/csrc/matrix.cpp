@@ -210,7 +210,9 @@ std::shared_ptr<OwnerPlan> build_owner_plan(const mf_form& a, const Integral& I,
   plan->pair_list.upload(plist);
   plan->pair_kk.upload(pkk);
   plan->pair_pos.upload(ppos);
-  if (!with_kk) {
+  {
+    // position of the diagonal block in each blocked row (-1: none): the
+    // Dirichlet diagonal of the hexahedral and warp-specialised kernels
     std::vector<int32_t> dp(std::max<int64_t>(1, A.nrows_b), -1);
     for (int64_t R = 0; R < A.nrows_b; ++R) {
       const int32_t* rb = A.h_colsb.data() + A.h_rpb[R];
@@ -415,7 +417,7 @@ void assemble_block_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_
   const bool diag_block = a->dm[0] == a->dm[1] && a->h_rmap[0] == a->h_rmap[1] &&
                           a->roff[0] == a->roff[1];
   if (diag_block && m0 && bc0->n_owned) {
-    auto it = a->block_diag_rows.find(bc0->uid);
+    auto it = a->block_diag_rows.find(bc0->key);
     if (it == a->block_diag_rows.end()) {
       const int bs = a->dm[0]->bs;
       std::vector<int32_t> rows;
@@ -427,7 +429,7 @@ void assemble_block_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_
       DevArray<int32_t> dr;
       dr.upload(rows.empty() ? std::vector<int32_t>{0} : rows);
       dr.n = static_cast<int64_t>(rows.size());
-      it = a->block_diag_rows.emplace(bc0->uid, std::move(dr)).first;
+      it = a->block_diag_rows.emplace(bc0->key, std::move(dr)).first;
     }
     if (it->second.n)
       k_set_diagonal(A->values, A->row_ptr.ptr, A->cols.ptr, it->second.ptr, it->second.n,
@@ -441,6 +443,16 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
                      double diagonal, int zero, hipStream_t s) {
   MF_CHECK(a->rank == 2, MF_EINVAL, "assemble_matrix needs a bilinear form");
   MF_CHECK(a->compiled, MF_ESTATE, "form is not compiled");
+  if (A->inner) {
+    // adopted pattern without block structure: assemble on the form's own
+    // pattern, then add into the caller's positions (extra entries stay 0)
+    MF_CHECK(a == A->inner->form, MF_EUNSUPPORTED,
+             "this adopted matrix assembles the form it was created for");
+    assemble_matrix(a, A->inner.get(), bc0, bc1, diagonal, 1, s);
+    if (zero) k_fill(A->values, A->nnz, 0.0, s);
+    k_scatter_add(A->values, A->inner->values, A->inner_map.ptr, A->inner->nnz, s);
+    return;
+  }
   if (A->block || a->restricted) {
     assemble_block_matrix(a, A, bc0, bc1, diagonal, zero, s);
     return;
@@ -491,16 +503,16 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
   if (owner) {
     auto key = std::make_pair(a->uid, owner->index);
     auto it = A->plans.find(key);
-    if (it == A->plans.end()) {
+    if (it == A->plans.end() || it->second.first != a->gen) {
       const bool hex = owner->k_hprep != nullptr;
       plan = build_owner_plan(*a, *owner, *A, a->opt("chunk_values", 4096),
                               static_cast<int>(a->opt("chunk_rows", 64)),
                               hex ? static_cast<int>(a->opt("owner_pairs", 384))
                                   : owner_shape(*a, *owner).maxp,
                               !hex);
-      A->plans[key] = plan;
+      A->plans[key] = std::make_pair(a->gen, plan);
     } else {
-      plan = it->second;
+      plan = it->second.second;
     }
     if (!plan) owner = nullptr;
   }
@@ -546,6 +558,22 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
     MF_CHECK(prec == 64 || (prec == 32 && owner->k_owner32), MF_EUNSUPPORTED,
              "precision " + std::to_string(prec) +
                  " on simplices needs a scalar reference-tensor form (precision 32) or 64");
+    // warp-specialised kernel (owner_pipe = 1): compute waves plus one store
+    // wave, double-buffered accumulators
+    hipFunction_t kpipe = prec == 32 ? owner->k_ownerp32 : owner->k_ownerp;
+    if (kpipe && a->opt("owner_pipe", 0)) {
+      int64_t& pgrid = prec == 32 ? owner->owner_gridp32 : owner->owner_gridp;
+      const uint32_t pblock = static_cast<uint32_t>(64 * (a->opt("owner_cwaves", 4) + 1));
+      if (pgrid == 0) {
+        int nblk = 0;
+        MF_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, kpipe, pblock, 0));
+        pgrid = std::max<int64_t>(nx, (int64_t)std::max(1, nblk) * a->ctx->num_cus / nx * nx);
+      }
+      launch(kpipe, static_cast<uint32_t>(std::min<int64_t>(per * nx, pgrid)), pblock, 0, s,
+             static_cast<int32_t>(plan->nchunks), plan->chunk_meta.ptr, plan->pair_list.ptr,
+             plan->pair_kk.ptr, owner->records.ptr, a->consts.ptr, A->rpb.ptr, plan->diag_pos.ptr,
+             n0, A->values, diagonal, flags);
+    } else {
     hipFunction_t kown = prec == 32 ? owner->k_owner32 : owner->k_owner;
     int64_t& ogrid = prec == 32 ? owner->owner_grid32 : owner->owner_grid;
     int64_t grid = per * nx;
@@ -561,6 +589,7 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
            static_cast<int32_t>(plan->nchunks), plan->chunk_meta.ptr, plan->item_cell.ptr,
            plan->pair_list.ptr, plan->pair_kk.ptr, owner->records.ptr, a->consts.ptr, A->rpb.ptr,
            A->colsb.ptr, n0, A->values, diagonal, flags);
+    }
     }
   } else if (zero) {
     k_fill(A->values, A->nnz, 0.0, s);
@@ -675,13 +704,13 @@ void assemble_batch(mf_batch* B, mf_matrix* const* As, const mf_bc* const* bcs, 
     auto key = std::make_pair(a->uid, I->index);
     std::shared_ptr<OwnerPlan> plan;
     auto it = A->plans.find(key);
-    if (it == A->plans.end()) {
+    if (it == A->plans.end() || it->second.first != a->gen) {
       plan = build_owner_plan(*a, *I, *A, a->opt("chunk_values", 4096),
                               static_cast<int>(a->opt("chunk_rows", 64)), owner_shape(*a, *I).maxp,
                               true);
-      A->plans[key] = plan;
+      A->plans[key] = std::make_pair(a->gen, plan);
     } else {
-      plan = it->second;
+      plan = it->second.second;
     }
     MF_CHECK(plan != nullptr, MF_EUNSUPPORTED, "owner-computes schedule unavailable for group " +
                                                    std::to_string(g));

@@ -103,6 +103,7 @@ void launch(hipFunction_t f, uint32_t grid, uint32_t block, uint32_t shmem, hipS
 // ------------------------------------------------------------ util kernels
 // (precompiled for gfx950 in util_kernels.hip; launchers run on `s`)
 void k_fill(double* x, int64_t n, double v, hipStream_t s);
+void k_scatter_add(double* dst, const double* src, const int32_t* map, int64_t n, hipStream_t s);
 // y = A x (CSR, int32 indices, FP64 values); d = diag(A)
 void k_spmv(int64_t nrows, int64_t nnz, const int32_t* rp, const int32_t* cols, const double* v,
             const double* x, double* y, hipStream_t s);
@@ -186,6 +187,10 @@ struct mf_bc {
   mf::DevArray<int8_t> markers;
   mf::DevArray<double> g;
   uint64_t uid = 0;
+  // content key: hash of (ndofs, n_owned, constrained dofs).  Per-form caches
+  // derived from the constrained set alone (lifting entity lists, block
+  // diagonal rows) are keyed by it, so equal sets rebuilt per call share them
+  uint64_t key = 0;
   bool g_zero = false;                // every constrained value is 0 (lifting adds nothing)
   // per-node component bits (bit a: scalar dof bs*node + a constrained), by block size
   mutable std::map<int, mf::DevArray<uint8_t>> node_bits;
@@ -214,8 +219,12 @@ struct Integral {
   int64_t owner_grid = 0;             // resident workgroups of the owner kernel (0: not queried)
   hipFunction_t k_owner32 = nullptr;  // FP32-accumulation owner kernel (scalar reference-tensor forms)
   int64_t owner_grid32 = 0;
+  // warp-specialised owner kernels (compute waves + one store wave), FP64 and
+  // FP32 element arithmetic, and their resident workgroups (0: not queried)
+  hipFunction_t k_ownerp = nullptr, k_ownerp32 = nullptr;
+  int64_t owner_gridp = 0, owner_gridp32 = 0;
   int64_t record_bytes = 0;
-  // lifting entity lists per bc object uid
+  // lifting entity lists per constrained set (mf_bc::key)
   std::map<uint64_t, std::pair<DevArray<int32_t>, int64_t>> lift_lists;
 };
 struct Coef {
@@ -264,10 +273,11 @@ struct mf_form {
   int64_t roff[2] = {0, 0};
   int64_t rcount[2] = {0, 0};         // nodes kept
   bool restricted = false;
-  std::map<uint64_t, mf::DevArray<int32_t>> block_diag_rows;   // bc uid -> global owned bc rows
+  std::map<uint64_t, mf::DevArray<int32_t>> block_diag_rows;   // bc key -> global owned bc rows
   mf::DevArray<double> partials;      // rank 0: per-workgroup partial sums of every integral
   mf::DevArray<double> scalar_out;    // rank 0: device result of the host-returning entry
   uint64_t uid = 0;
+  uint64_t gen = 0;                   // compile generation (bumped by every compile)
   int64_t opt(const std::string& k, int64_t dflt) const {
     auto it = options.find(k);
     return it == options.end() ? dflt : it->second;
@@ -328,8 +338,16 @@ struct mf_matrix {
   double* values = nullptr;
   // per (form uid, integral index) offset tables
   std::map<std::pair<uint64_t, int>, mf::DevArray<int32_t>> tables;
-  std::map<std::pair<uint64_t, int>, std::shared_ptr<mf::OwnerPlan>> plans;
+  // owner-computes schedules by (form uid, integral), with the form's compile
+  // generation they were built for (a recompile with other shape options
+  // makes them stale; they are rebuilt on the next assembly)
+  std::map<std::pair<uint64_t, int>, std::pair<uint64_t, std::shared_ptr<mf::OwnerPlan>>> plans;
   std::map<uint64_t, std::shared_ptr<mf::DGPlan>> dgplans;   // per form uid (null: unsupported)
+  // adopted caller pattern without the bs-unrolled block structure: the form
+  // assembles into `inner` (its own pattern), whose value k is then added at
+  // position inner_map[k] of this matrix's values
+  std::unique_ptr<mf_matrix> inner;
+  mf::DevArray<int32_t> inner_map;
 };
 
 namespace mf {
@@ -347,6 +365,8 @@ int hex_batch(const mf_form& f, const Integral& I, int prec);
 int hex_block(const mf_form& f, int prec);
 struct OwnerShape { int nb = 0, maxp = 0; };
 OwnerShape owner_shape(const mf_form& f, const Integral& I);
+// the warp-specialised owner kernels (mf_ownerp_*, mf_ownerp32_*) are emitted
+bool owner_pipe_emitted(const mf_form& f, const Integral& I);
 int64_t record_bytes(const mf_form& f, const Integral& I);
 std::string kernel_name(const char* what, int type, int index);
 bool dg_owner_emitted(const mf_form& f);

@@ -372,103 +372,7 @@ $OHEAD($OPARAMS)
           u_pv = (uint32_t)spair[pc];
           mf_load_unit(u_pv);
         }
-        const uint32_t pv = u_pv;
-        const int i = pv & 63;
-        const double K0 = u_K[0], K1 = u_K[1], K2 = u_K[2], K3 = u_K[3], K4 = u_K[4], K5 = u_K[5];
-        const double K6 = u_K[6], K7 = u_K[7], K8 = u_K[8], dt = u_det;
-#ifdef SK_TRACE
-        // metric M = |det J| K K^T from the record: tr G_ij = sum_rs M_rs R_ij[r][s]
-        const double M00 = K0, M11 = K1, M22 = K2, M01 = K3, M02 = K4, M12 = K5;
-        (void)K6; (void)K7; (void)K8; (void)dt;
-#if MF_OWNER_F32
-        const float f00 = (float)M00, f11 = (float)M11, f22 = (float)M22;
-        const float f01 = (float)M01, f02 = (float)M02, f12 = (float)M12;
-#endif
-#elif defined(SK_FULLRT)
-        // G_ij = |det J| K^T R_ij K; the left factor carries |det J|
-        const double L0 = dt * K0, L1 = dt * K1, L2 = dt * K2, L3 = dt * K3, L4 = dt * K4;
-        const double L5 = dt * K5, L6 = dt * K6, L7 = dt * K7, L8 = dt * K8;
-#else
-        double gi[NQ][3];
-        for (int q = 0; q < NQ; ++q) {
-          const double s = SK_W[q] * dt;
-          const double d0 = SK_DPHI[q][i][0], d1v = SK_DPHI[q][i][1], d2 = SK_DPHI[q][i][2];
-          gi[q][0] = s * (K0 * d0 + K3 * d1v + K6 * d2);
-          gi[q][1] = s * (K1 * d0 + K4 * d1v + K7 * d2);
-          gi[q][2] = s * (K2 * d0 + K5 * d1v + K8 * d2);
-        }
-#endif
-        const uint32_t bits = u_bits;
-        const uint32_t rb0 = (bits >> (BS * i)) & ((1u << BS) - 1);
-        const int lr = u_di - r0;
-        const int lo = lrp[lr], hi = lrp[lr + 1];
-        const int rowlen = BS * (hi - lo);
-        const int rbase = BS * BS * lo;
-        const uint8_t* kk = &skk[pc * ND];
-        const int j0 = part * jw, j1 = min(ND, j0 + jw);
-        // scalar trace forms: a few column iterations in flight at once (their
-        // LDS reads of R_ij overlap instead of waiting one by one)
-#ifdef SK_TRACE
-#pragma unroll 4
-#endif
-        for (int j = j0; j < j1; ++j) {
-          double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-#ifdef SK_TRACE
-          {
-            const mf_rt_t* R = &sRS[(i * ND + j) * 6];
-#if MF_OWNER_F32
-            G[0] = (double)(f00 * R[0] + f11 * R[1] + f22 * R[2] + f01 * R[3] + f02 * R[4] + f12 * R[5]);
-#else
-            G[0] = M00 * R[0] + M11 * R[1] + M22 * R[2] + M01 * R[3] + M02 * R[4] + M12 * R[5];
-#endif
-          }
-#elif defined(SK_FULLRT)
-          {
-            const double* R = &sRT[(i * ND + j) * 9];
-            // T = R K (T[3 r + c] = sum_s R[3 r + s] K[3 s + c]), G = L^T T
-            double T[9];
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-              const double a0 = R[3 * r], a1 = R[3 * r + 1], a2 = R[3 * r + 2];
-              T[3 * r] = a0 * K0 + a1 * K3 + a2 * K6;
-              T[3 * r + 1] = a0 * K1 + a1 * K4 + a2 * K7;
-              T[3 * r + 2] = a0 * K2 + a1 * K5 + a2 * K8;
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              G[c] = L0 * T[c] + L3 * T[3 + c] + L6 * T[6 + c];
-              G[3 + c] = L1 * T[c] + L4 * T[3 + c] + L7 * T[6 + c];
-              G[6 + c] = L2 * T[c] + L5 * T[3 + c] + L8 * T[6 + c];
-            }
-          }
-#else
-          for (int q = 0; q < NQ; ++q) {
-            const double d0 = SK_DPHI[q][j][0], d1v = SK_DPHI[q][j][1], d2 = SK_DPHI[q][j][2];
-            const double c0 = K0 * d0 + K3 * d1v + K6 * d2;
-            const double c1 = K1 * d0 + K4 * d1v + K7 * d2;
-            const double c2 = K2 * d0 + K5 * d1v + K8 * d2;
-            G[0] += gi[q][0] * c0; G[1] += gi[q][0] * c1; G[2] += gi[q][0] * c2;
-            G[3] += gi[q][1] * c0; G[4] += gi[q][1] * c1; G[5] += gi[q][1] * c2;
-            G[6] += gi[q][2] * c0; G[7] += gi[q][2] * c1; G[8] += gi[q][2] * c2;
-          }
-#endif
-          double blk[BS * BS];
-          sk_combine(G, u_w, consts, blk);
-          const uint32_t cb1 = (bits >> (BS * j)) & ((1u << BS) - 1);
-          const int rb = rbase + BS * kk[j];
-          if ((rb0 | cb1) == 0) {
-            for (int a = 0; a < BS; ++a)
-              for (int bb = 0; bb < BS; ++bb) atomicAdd(&acc[rb + a * rowlen + bb], (mf_acc_t)blk[a * BS + bb]);
-          } else {
-            for (int a = 0; a < BS; ++a) {
-              if (rb0 & (1u << a)) continue;
-              for (int bb = 0; bb < BS; ++bb) {
-                if (cb1 & (1u << bb)) continue;
-                atomicAdd(&acc[rb + a * rowlen + bb], (mf_acc_t)blk[a * BS + bb]);
-              }
-            }
-          }
-        }
+$UNIT
       }
     }
     __syncthreads();                                   // all contributions in acc
@@ -683,6 +587,379 @@ $KNAME(const int32_t nlist, const int32_t* __restrict__ list, const int32_t* __r
   const int64_t n0 = dm0[c * ND + i];
   for (int a = 0; a < BS; ++a) atomicAdd(&b[BS * n0 + a], be[a]);
 }
+)";
+
+// Parameters of the warp-specialised owner-computes kernel.
+const char* kOwnerPipeParams =
+    "const int32_t nchunks, const int32_t* __restrict__ chunk_meta, "
+    "const int32_t* __restrict__ pair_list, const uint8_t* __restrict__ pair_kk, "
+    "const mf_rec_$TAG* __restrict__ rec, const double* __restrict__ consts, "
+    "const int32_t* __restrict__ rpb, const int32_t* __restrict__ dpos, "
+    "const uint8_t* __restrict__ nb0, double* __restrict__ values, const double diag, "
+    "const int flags";
+
+// Warp-specialised owner-computes kernel (same chunks, pairs and element
+// arithmetic as kOwnerTemplate).  Waves 0 .. NCW-1 compute chunk k into one
+// of two LDS accumulators while wave NCW, the store wave, streams the
+// previous chunk's accumulator to the CSR values with non-temporal stores
+// and re-zeroes it.  A wave's vector loads and stores share one counter
+// (vmcnt), so a wave that waits for a load also waits for every store it
+// issued earlier: with all value stores in a wave that loads nothing, the
+// compute waves' record and list loads return without waiting for the
+// write-out, and the write-out of chunk k-1 overlaps the arithmetic and LDS
+// atomics of chunk k.  One barrier per chunk: at it the compute waves have
+// finished chunk k and staged chunk k+1's lists in the other list buffer,
+// and the store wave has drained and zeroed chunk k-1's accumulator.
+// dpos[R]: position of the diagonal block in blocked row R (-1: none).
+const char* kOwnerPipeTemplate = R"(
+$OHEAD($OPARAMS)
+{
+  constexpr int ND = SK_ND, BS = SK_BS, NQ = SK_NQ;
+  constexpr int CAP = $OCAP, MAXR = $OMAXR, MAXP = $OMAXP, NCW = $ONCW, NT = NCW * 64;
+  constexpr int KK4 = (MAXP * ND + 15) / 16;         // column-offset words (16 bytes)
+  constexpr int PF_P = (MAXP + NT - 1) / NT, PF_K = (KK4 + NT - 1) / NT;
+  constexpr int PF_R = (MAXR + 1 + NT - 1) / NT, PF_B = (MAXR + NT - 1) / NT;
+  typedef double mf_acc_t;
+#if MF_OWNER_F32
+  typedef float mf_rt_t;
+#else
+  typedef double mf_rt_t;
+#endif
+  static_assert(CAP % 2 == 0, "chunk_values must be even");
+  __shared__ __attribute__((aligned(16))) double acc2[2][CAP];
+  __shared__ __attribute__((aligned(16))) int32_t spair2[2][MAXP];
+  __shared__ __attribute__((aligned(16))) uint8_t skk2[2][KK4 * 16];
+  __shared__ __attribute__((aligned(16))) int32_t lrp2[2][MAXR + 1];
+  __shared__ __attribute__((aligned(16))) uint8_t snb2[2][MAXR];
+#ifdef SK_TRACE
+  __shared__ __attribute__((aligned(16))) mf_rt_t sRS[ND * ND * 6];
+  for (int p = threadIdx.x; p < ND * ND * 6; p += NT + 64) sRS[p] = (mf_rt_t)(&SK_RS[0][0][0])[p];
+#elif defined(SK_FULLRT)
+  __shared__ __attribute__((aligned(16))) double sRT[ND * ND * 9];
+  for (int p = threadIdx.x; p < ND * ND * 9; p += NT + 64) sRT[p] = (&SK_RT[0][0][0])[p];
+#endif
+  const int tid = threadIdx.x, nt = NT;
+  const int nx = 8, gx = gridDim.x / nx;
+  const int per = (nchunks + nx - 1) / nx;
+  const int xcd = blockIdx.x % nx;
+  int jj = blockIdx.x / nx;
+  int k = xcd * per + jj;
+  if (jj >= per || k >= nchunks) return;
+  const bool storer = __builtin_amdgcn_readfirstlane(tid >> 6) == NCW;
+  {
+    int4* a4 = reinterpret_cast<int4*>(&acc2[0][0]);
+    for (int p = tid; p < CAP; p += NT + 64) a4[p] = make_int4(0, 0, 0, 0);   // 2 CAP doubles
+  }
+  const int4* meta4 = reinterpret_cast<const int4*>(chunk_meta);
+  int4 m0 = {0, 0, 0, 0}, m1 = {0, 0, 0, 0}, m2 = {0, 0, 0, 0};
+  uint32_t u_pv = 0, u_bits = 0;
+  int32_t u_di = 0;
+  double u_K[9], u_det = 0.0, u_w[$WW];
+  for (int x = 0; x < 9; ++x) u_K[x] = 0.0;
+  for (int x = 0; x < $WW; ++x) u_w[x] = 0.0;
+  auto mf_load_unit = [&](uint32_t pv) {
+    const mf_rec_$TAG& r = rec[pv >> 6];
+#ifdef SK_TRACE
+    for (int x = 0; x < 6; ++x) u_K[x] = r.M[x];       // the metric, see the record
+#else
+    for (int x = 0; x < 9; ++x) u_K[x] = r.K[x];
+    u_det = r.det;
+#endif
+    for (int x = 0; x < $WW; ++x) u_w[x] = r.w[x];
+    u_di = r.d[pv & 63];
+    u_bits = r.bits;
+  };
+  // pair index of this thread's first (pair, column range) unit in a chunk
+  // of np pairs (same mapping as the unit loop), -1 if it has none
+  auto mf_first_pair = [&](int np) -> int {
+    int js = ((NT >> 6) * 64) / max(np, 1);
+    js = js < 1 ? 1 : (js > ND ? ND : js);
+    const int pr = ((tid >> 6) / js) * 64 + (tid & 63);
+    return pr < np ? pr : -1;
+  };
+  // store wave: write chunk kc's segment from accumulator A, re-zero A
+  auto mf_drain = [&](double* A, int kc) {
+    const int4 d1 = meta4[3 * kc + 1];
+    const int32_t base = d1.z, seglen = d1.w;
+    const int lane = tid & 63;
+    double* out = values + (int64_t)(uint32_t)base;
+    if (flags & 1) {
+      // peel one value when the segment starts on an odd double, so the
+      // rest goes out as aligned 16-byte non-temporal stores
+      const int h = base & 1;
+      if (h && lane == 0) {
+        __builtin_nontemporal_store(A[0], &out[0]);
+        A[0] = 0.0;
+      }
+      mf_d2* o2 = reinterpret_cast<mf_d2*>(out + h);
+      const int n2 = (seglen - h) >> 1;
+      for (int p = lane; p < n2; p += 64) {
+        const double v0 = A[h + 2 * p], v1 = A[h + 2 * p + 1];
+        A[h + 2 * p] = 0.0;
+        A[h + 2 * p + 1] = 0.0;
+        const mf_d2 v = {v0, v1};
+        __builtin_nontemporal_store(v, &o2[p]);
+      }
+      if (((seglen - h) & 1) && lane == 0) {
+        __builtin_nontemporal_store(A[seglen - 1], &out[seglen - 1]);
+        A[seglen - 1] = 0.0;
+      }
+    } else {
+      for (int p = lane; p < seglen; p += 64) {
+        out[p] += A[p];
+        A[p] = 0.0;
+      }
+    }
+  };
+  if (!storer) {
+    m0 = meta4[3 * k];
+    m1 = meta4[3 * k + 1];
+    m2 = meta4[3 * k + 2];
+    for (int p = tid; p < m2.y - m2.x; p += nt) spair2[0][p] = pair_list[m2.x + p];
+    {
+      const int n4 = ((m2.y - m2.x) * ND + 15) / 16;
+      const int4* src = reinterpret_cast<const int4*>(pair_kk + (uint32_t)m2.z);
+      for (int p = tid; p < n4; p += nt) reinterpret_cast<int4*>(skk2[0])[p] = src[p];
+    }
+    for (int p = tid; p <= m0.y - m0.x; p += nt) lrp2[0][p] = rpb[m0.x + p] - m1.x;
+    for (int p = tid; p < m0.y - m0.x; p += nt) snb2[0][p] = nb0 ? nb0[m0.x + p] : 0;
+    const int pr0 = mf_first_pair(m2.y - m2.x);
+    if (pr0 >= 0) {
+      u_pv = (uint32_t)pair_list[m2.x + pr0];
+      mf_load_unit(u_pv);
+    }
+  }
+  int it = 0, kp = -1;
+  __syncthreads();
+  while (true) {
+    const int jn = jj + gx, kn = xcd * per + jn;
+    const bool more = jn < per && kn < nchunks;
+    const int b = it & 1;
+    int4 n0 = {0, 0, 0, 0}, n1 = {0, 0, 0, 0}, n2 = {0, 0, 0, 0};
+    if (!storer) {
+      double* const acc = acc2[b];
+      const int32_t* const spair = spair2[b];
+      const uint8_t* const skk = skk2[b];
+      const int32_t* const lrp = lrp2[b];
+      const uint8_t* const snb = snb2[b];
+      const int32_t r0 = m0.x, r1 = m0.y;
+      const int nr = r1 - r0;
+      const int npc = m2.y - m2.x;
+      if ((flags & 2) && nb0) {
+        // Dirichlet rows: the diagonal entries of constrained components get
+        // no contributions (their rows are masked), so they are set here
+        for (int lr = tid; lr < nr; lr += nt) {
+          const uint32_t rbits = snb[lr];
+          if (!rbits) continue;
+          const int lo = lrp[lr], hi = lrp[lr + 1];
+          const int p = dpos[r0 + lr];
+          if (p >= 0)
+            for (int a = 0; a < BS; ++a)
+              if (rbits & (1u << a)) acc[BS * BS * lo + a * BS * (hi - lo) + BS * p + a] = diag;
+        }
+      }
+      // the next chunk's lists into registers (latency hidden by the units)
+      uint32_t pvn = 0xFFFFFFFFu;
+      int pf_p[PF_P], pf_r[PF_R], pf_b[PF_B];
+      int4 pf_k[PF_K];
+      if (more) {
+        n0 = meta4[3 * kn];
+        n1 = meta4[3 * kn + 1];
+        n2 = meta4[3 * kn + 2];
+        const int nnp = n2.y - n2.x, nk4 = (nnp * ND + 15) / 16;
+        const int4* ksrc = reinterpret_cast<const int4*>(pair_kk + (uint32_t)n2.z);
+#pragma unroll
+        for (int t = 0; t < PF_P; ++t) {
+          const int p = tid + t * NT;
+          pf_p[t] = p < nnp ? pair_list[n2.x + p] : 0;
+        }
+#pragma unroll
+        for (int t = 0; t < PF_K; ++t) {
+          const int p = tid + t * NT;
+          pf_k[t] = p < nk4 ? ksrc[p] : make_int4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < PF_R; ++t) {
+          const int p = tid + t * NT;
+          pf_r[t] = p <= n0.y - n0.x ? rpb[n0.x + p] - n1.x : 0;
+        }
+#pragma unroll
+        for (int t = 0; t < PF_B; ++t) {
+          const int p = tid + t * NT;
+          pf_b[t] = p < n0.y - n0.x && nb0 ? nb0[n0.x + p] : 0;
+        }
+        const int prn = mf_first_pair(nnp);
+        if (prn >= 0) pvn = (uint32_t)pair_list[n2.x + prn];
+      }
+      {
+        // one thread per (pair, column range); lanes of a wave share the
+        // column range, so the column basis tables are wave-uniform
+        const int np = npc;
+        int js = (NCW * 64) / max(np, 1);
+        js = js < 1 ? 1 : (js > ND ? ND : js);
+        const int jw = (ND + js - 1) / js;
+        const int units = ((np + 63) >> 6) * 64 * js;
+        for (int u = tid; u < units; u += nt) {
+          const int lane = u & 63, wv = u >> 6;
+          const int part = __builtin_amdgcn_readfirstlane(wv % js);
+          const int pr = (wv / js) * 64 + lane;
+          if (pr >= np) continue;
+          const int pc = pr;                           // pair index within the chunk
+          if (u != tid) {                              // later units: load directly
+            u_pv = (uint32_t)spair[pc];
+            mf_load_unit(u_pv);
+          }
+$UNIT
+        }
+      }
+      if (more) {
+        // the other list buffer was last read in the previous chunk, before
+        // the previous barrier
+        const int nnp = n2.y - n2.x, nk4 = (nnp * ND + 15) / 16;
+#pragma unroll
+        for (int t = 0; t < PF_P; ++t) {
+          const int p = tid + t * NT;
+          if (p < nnp) spair2[b ^ 1][p] = pf_p[t];
+        }
+#pragma unroll
+        for (int t = 0; t < PF_K; ++t) {
+          const int p = tid + t * NT;
+          if (p < nk4) reinterpret_cast<int4*>(skk2[b ^ 1])[p] = pf_k[t];
+        }
+#pragma unroll
+        for (int t = 0; t < PF_R; ++t) {
+          const int p = tid + t * NT;
+          if (p <= n0.y - n0.x) lrp2[b ^ 1][p] = pf_r[t];
+        }
+#pragma unroll
+        for (int t = 0; t < PF_B; ++t) {
+          const int p = tid + t * NT;
+          if (p < n0.y - n0.x) snb2[b ^ 1][p] = (uint8_t)pf_b[t];
+        }
+        if (pvn != 0xFFFFFFFFu) {
+          u_pv = pvn;
+          mf_load_unit(pvn);
+        }
+      }
+    } else if (kp >= 0) {
+      mf_drain(acc2[b ^ 1], kp);
+    }
+    __syncthreads();
+    if (!more) break;
+    kp = k;
+    k = kn;
+    jj = jn;
+    ++it;
+    m0 = n0; m1 = n1; m2 = n2;
+  }
+  if (storer) mf_drain(acc2[it & 1], k);
+}
+)";
+
+
+// Body of one (pair, column range) unit of the owner-computes kernels,
+// shared by kOwnerTemplate and kOwnerPipeTemplate ($UNIT): the element row
+// blocks of pair pv = u_pv for columns [part * jw, ...), masked by the
+// Dirichlet bits and added into the LDS accumulator `acc` at the positions
+// the chunk's column offsets `skk` and row pointers `lrp` give.
+const char* kOwnerUnit = R"(
+        const uint32_t pv = u_pv;
+        const int i = pv & 63;
+        const double K0 = u_K[0], K1 = u_K[1], K2 = u_K[2], K3 = u_K[3], K4 = u_K[4], K5 = u_K[5];
+        const double K6 = u_K[6], K7 = u_K[7], K8 = u_K[8], dt = u_det;
+#ifdef SK_TRACE
+        // metric M = |det J| K K^T from the record: tr G_ij = sum_rs M_rs R_ij[r][s]
+        const double M00 = K0, M11 = K1, M22 = K2, M01 = K3, M02 = K4, M12 = K5;
+        (void)K6; (void)K7; (void)K8; (void)dt;
+#if MF_OWNER_F32
+        const float f00 = (float)M00, f11 = (float)M11, f22 = (float)M22;
+        const float f01 = (float)M01, f02 = (float)M02, f12 = (float)M12;
+#endif
+#elif defined(SK_FULLRT)
+        // G_ij = |det J| K^T R_ij K; the left factor carries |det J|
+        const double L0 = dt * K0, L1 = dt * K1, L2 = dt * K2, L3 = dt * K3, L4 = dt * K4;
+        const double L5 = dt * K5, L6 = dt * K6, L7 = dt * K7, L8 = dt * K8;
+#else
+        double gi[NQ][3];
+        for (int q = 0; q < NQ; ++q) {
+          const double s = SK_W[q] * dt;
+          const double d0 = SK_DPHI[q][i][0], d1v = SK_DPHI[q][i][1], d2 = SK_DPHI[q][i][2];
+          gi[q][0] = s * (K0 * d0 + K3 * d1v + K6 * d2);
+          gi[q][1] = s * (K1 * d0 + K4 * d1v + K7 * d2);
+          gi[q][2] = s * (K2 * d0 + K5 * d1v + K8 * d2);
+        }
+#endif
+        const uint32_t bits = u_bits;
+        const uint32_t rb0 = (bits >> (BS * i)) & ((1u << BS) - 1);
+        const int lr = u_di - r0;
+        const int lo = lrp[lr], hi = lrp[lr + 1];
+        const int rowlen = BS * (hi - lo);
+        const int rbase = BS * BS * lo;
+        const uint8_t* kk = &skk[pc * ND];
+        const int j0 = part * jw, j1 = min(ND, j0 + jw);
+        // scalar trace forms: a few column iterations in flight at once (their
+        // LDS reads of R_ij overlap instead of waiting one by one)
+#ifdef SK_TRACE
+#pragma unroll 4
+#endif
+        for (int j = j0; j < j1; ++j) {
+          double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#ifdef SK_TRACE
+          {
+            const mf_rt_t* R = &sRS[(i * ND + j) * 6];
+#if MF_OWNER_F32
+            G[0] = (double)(f00 * R[0] + f11 * R[1] + f22 * R[2] + f01 * R[3] + f02 * R[4] + f12 * R[5]);
+#else
+            G[0] = M00 * R[0] + M11 * R[1] + M22 * R[2] + M01 * R[3] + M02 * R[4] + M12 * R[5];
+#endif
+          }
+#elif defined(SK_FULLRT)
+          {
+            const double* R = &sRT[(i * ND + j) * 9];
+            // T = R K (T[3 r + c] = sum_s R[3 r + s] K[3 s + c]), G = L^T T
+            double T[9];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+              const double a0 = R[3 * r], a1 = R[3 * r + 1], a2 = R[3 * r + 2];
+              T[3 * r] = a0 * K0 + a1 * K3 + a2 * K6;
+              T[3 * r + 1] = a0 * K1 + a1 * K4 + a2 * K7;
+              T[3 * r + 2] = a0 * K2 + a1 * K5 + a2 * K8;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              G[c] = L0 * T[c] + L3 * T[3 + c] + L6 * T[6 + c];
+              G[3 + c] = L1 * T[c] + L4 * T[3 + c] + L7 * T[6 + c];
+              G[6 + c] = L2 * T[c] + L5 * T[3 + c] + L8 * T[6 + c];
+            }
+          }
+#else
+          for (int q = 0; q < NQ; ++q) {
+            const double d0 = SK_DPHI[q][j][0], d1v = SK_DPHI[q][j][1], d2 = SK_DPHI[q][j][2];
+            const double c0 = K0 * d0 + K3 * d1v + K6 * d2;
+            const double c1 = K1 * d0 + K4 * d1v + K7 * d2;
+            const double c2 = K2 * d0 + K5 * d1v + K8 * d2;
+            G[0] += gi[q][0] * c0; G[1] += gi[q][0] * c1; G[2] += gi[q][0] * c2;
+            G[3] += gi[q][1] * c0; G[4] += gi[q][1] * c1; G[5] += gi[q][1] * c2;
+            G[6] += gi[q][2] * c0; G[7] += gi[q][2] * c1; G[8] += gi[q][2] * c2;
+          }
+#endif
+          double blk[BS * BS];
+          sk_combine(G, u_w, consts, blk);
+          const uint32_t cb1 = (bits >> (BS * j)) & ((1u << BS) - 1);
+          const int rb = rbase + BS * kk[j];
+          if ((rb0 | cb1) == 0) {
+            for (int a = 0; a < BS; ++a)
+              for (int bb = 0; bb < BS; ++bb) atomicAdd(&acc[rb + a * rowlen + bb], (mf_acc_t)blk[a * BS + bb]);
+          } else {
+            for (int a = 0; a < BS; ++a) {
+              if (rb0 & (1u << a)) continue;
+              for (int bb = 0; bb < BS; ++bb) {
+                if (cb1 & (1u << bb)) continue;
+                atomicAdd(&acc[rb + a * rowlen + bb], (mf_acc_t)blk[a * BS + bb]);
+              }
+            }
+          }
+        }
 )";
 
 }  // namespace mf

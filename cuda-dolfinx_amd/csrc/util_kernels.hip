@@ -21,6 +21,16 @@ __global__ void fill_kernel(double* __restrict__ x, int64_t n, double v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = v;
 }
 
+// dst[map[k]] += src[k]: values of a matrix on the form's own pattern added
+// into a caller's CSR whose pattern is a superset without the bs-unrolled
+// block structure (map is injective, so no two threads share a target)
+__global__ void scatter_add_kernel(double* __restrict__ dst, const double* __restrict__ src,
+                                   const int32_t* __restrict__ map, int64_t n) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+    dst[map[k]] += src[k];
+}
+
 __global__ void fill_i32_kernel(int32_t* __restrict__ x, int64_t n, int32_t v) {
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = v;
@@ -287,6 +297,11 @@ __global__ void diag_kernel(int64_t nrows, const int32_t* __restrict__ rp,
 void k_fill(double* x, int64_t n, double v, hipStream_t s) {
   if (n <= 0) return;
   fill_kernel<<<grid_for(n, 4), kBlock, 0, s>>>(x, n, v);
+  MF_HIP(hipGetLastError());
+}
+void k_scatter_add(double* dst, const double* src, const int32_t* map, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  scatter_add_kernel<<<grid_for(n, 4), kBlock, 0, s>>>(dst, src, map, n);
   MF_HIP(hipGetLastError());
 }
 void k_fill_i32(int32_t* x, int64_t n, int32_t v, hipStream_t s) {

@@ -62,6 +62,7 @@ SIGNATURES = {
     "mf_assemble_scalar": (C.c_int, [_vp, _vp, _vp]),
     "mf_form_set_restriction": (C.c_int, [_vp, C.c_int, _vp, _i64, _i64]),
     "mf_matrix_create_block": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
+    "mf_matrix_create_from_csr": (C.c_int, [_vp, _i32p, _i32p, _i64, _i64, _vp, C.POINTER(_vp)]),
     "mf_autotune_matrix": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, _i64, _vp]),
     "mf_batch_create": (C.c_int, [_vp, C.c_int, _vp]),
     "mf_batch_assemble": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int, _vp]),

@@ -93,6 +93,7 @@ class CUDAAssembler:
             raise TypeError(f"Expected CUDAForm, got '{type(a)}'")
         if mat is None:
             mat = self.create_matrix(a)
+        mat._bind(a)                       # adopted caller matrices: handle on first use
         bc = self._bc_collection(list(bcs) if isinstance(bcs, tuple) else bcs)
         V0, V1 = a.function_spaces[0], a.function_spaces[1]
         b0, b1 = bc._get_cpp_bcs(V0), bc._get_cpp_bcs(V1)

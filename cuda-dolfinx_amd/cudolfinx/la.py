@@ -33,24 +33,93 @@ class DeviceCSRMatrix:
 
 
 class CUDAMatrix:
-    """Device CSR matrix created from a bilinear form (la/CUDAMatrix.cpp:36-225)."""
+    """Device CSR matrix (la/CUDAMatrix.cpp:36-225).
 
-    def __init__(self, ctx, form):
+    CUDAMatrix(ctx, form) builds the pattern of a bilinear form's integrals.
+    CUDAMatrix(ctx, mat) adopts the pattern of a caller's matrix, as the
+    reference's CUDAMatrix(ctx, petsc_mat) does (python/cudolfinx/la.py:45-51):
+
+    * a torch sparse CSR tensor on the GPU (float64 values): its values are
+      assembled in place (borrowed device memory);
+    * a scipy.sparse CSR matrix, or an object with petsc4py's
+      getValuesCSR() / setValuesCSR(): values live on the device and are
+      written back to the caller's matrix at the handoff (assemble()).
+
+    The device handle is created at the first assembly, against that form's
+    spaces (the reference builds its offsets at assembly time too); every
+    entry the form touches must be in the pattern (MF_ENOTFOUND otherwise).
+    """
+
+    def __init__(self, ctx, form_or_mat):
         self._ctx = ctx
-        self._form = form            # keeps the form (and its dofmaps) alive
-        h = C.c_void_p()
-        check(lib().mf_matrix_create(form.handle, C.byref(h)))
+        self._handle = None
+        self._foreign = None
+        if hasattr(form_or_mat, "handle") and hasattr(form_or_mat, "function_spaces"):
+            form = form_or_mat
+            self._form = form            # keeps the form (and its dofmaps) alive
+            h = C.c_void_p()
+            check(lib().mf_matrix_create(form.handle, C.byref(h)))
+            self._set_handle(h)
+        else:
+            self._adopt(form_or_mat)
+        self._mat = DeviceCSRMatrix(self) if self._foreign is None else form_or_mat
+
+    def _set_handle(self, h):
         self._handle = h
         nr, nc, nnz = C.c_int64(), C.c_int64(), C.c_int64()
         check(lib().mf_matrix_info(h, C.byref(nr), C.byref(nc), C.byref(nnz)))
         self.nrows, self.ncols, self.nnz = nr.value, nc.value, nnz.value
-        self._mat = DeviceCSRMatrix(self)
+
+    def _adopt(self, mat):
+        """Record a caller matrix's CSR pattern (and device values if it has them)."""
+        dev_vals = None
+        if hasattr(mat, "crow_indices") and getattr(mat, "layout", None) is not None:
+            import torch
+            if mat.layout != torch.sparse_csr or mat.dtype != torch.float64 or not mat.is_cuda:
+                raise TypeError("a torch matrix must be a float64 sparse CSR tensor on the GPU")
+            rp = mat.crow_indices().cpu().numpy()
+            cols = mat.col_indices().cpu().numpy()
+            dev_vals = mat.values()
+            if not dev_vals.is_contiguous():
+                raise ValueError("the CSR values must be contiguous")
+            kind, shape = "torch", tuple(mat.shape)
+        elif hasattr(mat, "indptr") and hasattr(mat, "indices"):
+            if getattr(mat, "format", "csr") != "csr":
+                raise TypeError("expected a CSR matrix")
+            mat.sort_indices()
+            rp, cols, kind, shape = mat.indptr, mat.indices, "scipy", mat.shape
+        elif hasattr(mat, "getValuesCSR"):
+            rp, cols, _ = mat.getValuesCSR()
+            kind, shape = "petsc", tuple(mat.getSize())
+        else:
+            raise TypeError(f"Expected a CUDAForm or a CSR matrix, got '{type(mat)}'")
+        self._foreign = dict(kind=kind, mat=mat, values=dev_vals, shape=shape,
+                             rp=np.ascontiguousarray(rp, dtype=np.int32),
+                             cols=np.ascontiguousarray(cols, dtype=np.int32))
+        self.nrows, self.ncols = int(shape[0]), int(shape[1])
+        self.nnz = int(self._foreign["rp"][-1])
+
+    def _bind(self, form):
+        """Create the device handle against `form` (first assembly of an adopted matrix)."""
+        if self._handle is not None:
+            return
+        f = self._foreign
+        if (self.nrows, self.ncols) != (form.function_spaces[0].size, form.function_spaces[1].size):
+            raise ValueError(f"matrix shape {(self.nrows, self.ncols)} does not match the form's spaces")
+        self._form = form
+        vals = C.c_void_p(f["values"].data_ptr()) if f["values"] is not None else None
+        h = C.c_void_p()
+        check(lib().mf_matrix_create_from_csr(form.handle, f["rp"].ctypes.data_as(C.POINTER(C.c_int32)),
+                                              f["cols"].ctypes.data_as(C.POINTER(C.c_int32)),
+                                              self.nrows, self.nnz, vals, C.byref(h)))
+        self._set_handle(h)
 
     @classmethod
     def block(cls, ctx, block_form):
         """Monolithic matrix of a BlockCUDAForm (create_cuda_matrix_block, fem/petsc.h:39-107)."""
         self = cls.__new__(cls)
         self._ctx = ctx
+        self._foreign = None
         self._form = block_form
         rows = block_form.forms
         nr, nc = len(rows), len(rows[0])
@@ -67,6 +136,8 @@ class CUDAMatrix:
 
     @property
     def handle(self):
+        if self._handle is None:
+            raise RuntimeError("an adopted matrix gets its device handle at its first assembly")
         return self._handle
 
     @property
@@ -80,8 +151,18 @@ class CUDAMatrix:
         return rp.value, cols.value, vals.value
 
     def assemble(self):
-        """Handoff: make the device values final (CUDAMatrix::to_host + MatAssembly)."""
+        """Handoff: make the device values final (CUDAMatrix::to_host + MatAssembly).
+        An adopted host matrix (scipy, petsc4py-like) receives the values."""
         self._ctx.synchronize()
+        f = self._foreign
+        if f is None or self._handle is None or f["kind"] == "torch":
+            return
+        _, _, vals = self.csr()
+        if f["kind"] == "scipy":
+            f["mat"].data[:] = vals
+        else:
+            f["mat"].setValuesCSR(f["rp"], f["cols"], vals)
+            f["mat"].assemble()
 
     @staticmethod
     def _addr(v):

@@ -138,14 +138,74 @@ def functionspace(mesh: Mesh, element, shape: Optional[Sequence[int]] = None) ->
     return FunctionSpace(mesh, el, shape)
 
 
+_WRITING_FUNCS = {np.copyto, np.put, np.place, np.putmask, np.fill_diagonal}
+
+
+class _TrackedArray(np.ndarray):
+    """The values of a Function: a float64 array that bumps its Function's
+    `version` whenever it, or a view of it, is written through numpy
+    (item assignment, in-place arithmetic, ufuncs with out=, fill, copyto,
+    put, place, putmask).  The device copy of a coefficient is refreshed
+    when the version changed, so host writes such as `f.x[:] = v` reach the
+    next assembly.  Writes through raw pointers (ctypes, buffers) are not
+    seen: bump `version` by hand after those."""
+
+    _owner = None
+
+    def __array_finalize__(self, obj):
+        self._owner = getattr(obj, "_owner", None)
+
+    def _touch(self):
+        if self._owner is not None:
+            self._owner.version += 1
+
+    def __setitem__(self, key, value):
+        super().__setitem__(key, value)
+        self._touch()
+
+    def fill(self, value):
+        super().fill(value)
+        self._touch()
+
+    def __array_ufunc__(self, ufunc, method, *inputs, out=None, **kwargs):
+        args = [x.view(np.ndarray) if isinstance(x, _TrackedArray) else x for x in inputs]
+        touched = []
+        if out is not None:
+            touched = [o for o in out if isinstance(o, _TrackedArray)]
+            kwargs["out"] = tuple(o.view(np.ndarray) if isinstance(o, _TrackedArray) else o
+                                  for o in out)
+        res = getattr(ufunc, method)(*args, **kwargs)
+        for o in touched:
+            o._touch()
+        if out is not None and len(out) == 1:
+            return out[0]
+        return res
+
+    def __array_function__(self, func, types, args, kwargs):
+        res = super().__array_function__(func, types, args, kwargs)
+        if func in _WRITING_FUNCS and args and isinstance(args[0], _TrackedArray):
+            args[0]._touch()
+        return res
+
+
 class Function:
     """Coefficient vector on a function space (scalar dofs, bs-unrolled)."""
 
     def __init__(self, V: FunctionSpace, name: str = "f"):
         self.function_space = V
         self.name = name
-        self.x = np.zeros(V.size, dtype=np.float64)
-        self.version = 0            # bumped on every host-side change
+        self.version = 0            # bumped on every host-side change of x
+        x = np.zeros(V.size, dtype=np.float64).view(_TrackedArray)
+        x._owner = self
+        self._x = x
+
+    @property
+    def x(self) -> np.ndarray:
+        return self._x
+
+    @x.setter
+    def x(self, values) -> None:
+        self._x[:] = values
 
     @property
     def array(self) -> np.ndarray:

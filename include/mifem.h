@@ -172,6 +172,20 @@ int mf_matrix_create(mf_form* a, mf_matrix** out);
  * built per scalar row (sorted, unique).  Assemble each block with
  * mf_assemble_matrix(form_ij, A, ..., zero = 0) after one zeroing. */
 int mf_matrix_create_block(mf_form* const* forms, int nr, int nc, mf_matrix** out);
+/* Adopts a caller-built scalar CSR pattern (host row_ptr[n_rows + 1] and
+ * cols[nnz], sorted unique columns per row, as MatGetRowIJ returns it) with
+ * the values in caller device memory d_values[nnz] (borrowed, not freed; NULL:
+ * the library allocates them).  Replaces CUDAMatrix(ctx, petsc_mat)
+ * (python/cudolfinx/la.py:45-51 -> cpp/cudolfinx/la/CUDASeqMatrix.cpp:40-321,
+ * which reads the pattern of any PETSc AIJ matrix).  Every entry the form's
+ * integrals touch must be present (MF_ENOTFOUND otherwise); extra entries are
+ * allowed and assemble to zero.  A pattern with the bs-unrolled block
+ * structure of the form's spaces (each blocked row's bs0 scalar rows hold the
+ * same bs1-column groups) is assembled in place by the owner-computes
+ * kernels; any other is assembled on the form's own pattern (a library-held
+ * copy) whose values are then added at their positions in the given one. */
+int mf_matrix_create_from_csr(mf_form* a, const int32_t* row_ptr, const int32_t* cols,
+                              int64_t n_rows, int64_t nnz, double* d_values, mf_matrix** out);
 int mf_matrix_info(mf_matrix* A, int64_t* nrows, int64_t* ncols, int64_t* nnz);
 /* Device pointers of the CSR arrays (values may be replaced by a borrowed
  * buffer, e.g. a PETSc-HIP MatSeqAIJHIPSPARSEGetArray pointer). */

@@ -104,6 +104,39 @@ def test_poisson_p1_full_size_c1():
     _compare(P, _gpu(P), _oracle(P))
 
 
+@pytest.mark.timeout(900)
+def test_elasticity_p2_full_size_c2():
+    """Vector P2 elasticity on the 64^3 x 6 mesh (1,572,864 cells, 6,440,067
+    dofs, 547,739,145 nonzeros) with the owner-kernel shape the benchmark runs
+    (4608-value chunks, 384 pairs, 256 threads): CSR bit-exact and A, b within
+    1e-12 of the oracle (its coloured multi-core loops, which equal the serial
+    ones to round-off)."""
+    import os
+    import time
+    _say("elasticity_p2 n=64 (1,572,864 cells), owner shape 4608/384/256")
+    t0 = time.perf_counter()
+    P = problems.elasticity_p2(64)
+    shape = {"matrix_kernel": 2, "chunk_values": 4608, "owner_pairs": 384, "owner_block": 256,
+             "owner_lds": 73728}
+    gpu = _gpu(P, shape)
+    _say(f"  GPU A+b done ({time.perf_counter() - t0:.1f} s)")
+    gpu = gpu[:4]                       # release the device objects
+    threads = max(1, min(32, len(os.sched_getaffinity(0))))
+    Oa, OL = cpu.OracleAssembler(P.a), cpu.OracleAssembler(P.L)
+    orp, ocols, _, _ = Oa.sparsity()
+    assert np.array_equal(gpu[0], orp), "row pointers differ"
+    assert np.array_equal(gpu[1], ocols), "column indices differ"
+    Oa.colouring()
+    OL.colouring()
+    ovals = Oa.assemble_matrix_mt(orp, ocols, P.bcs, threads=threads)
+    ob = OL.assemble_vector_mt(threads=threads)
+    Oa.apply_lifting(ob, P.bcs, threads=threads)
+    cpu.set_bc(ob, P.bcs, P.V)
+    _say(f"  oracle A+b on {threads} threads done ({time.perf_counter() - t0:.1f} s)")
+    assert gpu[1].size == 547739145 and gpu[0].size == 6440068
+    _compare(P, gpu, (orp, ocols, ovals, ob))
+
+
 @pytest.mark.parametrize("k", [2, 3])
 def test_pk_poisson_dg0_kappa(k):
     _say(f"P{k} Poisson with DG0 kappa, jittered mesh")
@@ -998,3 +1031,221 @@ def test_native_autotuner_then_parity():
     asm.synchronize()
     rp, cols, vals = A.csr()
     _compare(P, (rp, cols, vals, b.to_host()), _oracle(P))
+
+
+def test_graph_lifting_recorded_while_g_is_zero():
+    """apply_lifting captured while every constrained value is 0 is still in
+    the graph: after CUDADirichletBC.update sets g != 0, each replay equals
+    the eager sequence and the oracle."""
+    _say("HIP graph: lifting captured with g = 0, replayed after bc update to g != 0")
+    import torch
+    import cudolfinx as cufem
+    P = problems.poisson_p1(6)
+    s = torch.cuda.Stream()
+    a, L = cufem.form(P.a), cufem.form(P.L)
+    asm = cufem.CUDAAssembler(stream=s)
+    g = Function(P.V)
+    bc = dirichletbc(g, locate_dofs_topological(P.V, 2, P.mesh.exterior_facets), P.V)
+    dbc = asm.pack_bcs([bc])
+    A = asm.assemble_matrix(a, bcs=dbc)
+    b = asm.create_vector(L)
+
+    def seq():
+        asm.assemble_vector(L, b)
+        asm.apply_lifting(b, [a], [dbc])
+        asm.set_bc(b, dbc, P.V)
+
+    seq()                                   # g = 0: builds the lifting lists, adds nothing
+    asm.synchronize()
+    graph = cufem.CUDAGraph(s)
+    with graph:
+        seq()
+    rng = np.random.default_rng(5)
+    for rep in range(3):
+        g.x[:] = rng.uniform(-1, 1, P.V.size)
+        dbc.update([bc])
+        graph.launch()
+        asm.synchronize()
+        bg = b.to_host().copy()
+        seq()
+        asm.synchronize()
+        # the vector and lifting kernels add with atomics: equal up to summation order
+        assert _rel(bg, b.to_host()) <= 1e-14, f"replay {rep} differs from eager"
+        _, _, _, ob = _oracle(P, [bc])
+        eb = _rel(bg, ob)
+        _say(f"  replay {rep}: relerr b={eb:.3e}")
+        assert eb <= TOL
+
+
+def test_lifting_with_plain_bc_lists_keeps_device_memory_flat():
+    """apply_lifting called with a plain list (a new device condition per call,
+    as the reference's own test does) reuses one lifting entity list per
+    constrained set: device memory does not grow with the number of calls."""
+    _say("apply_lifting with plain bc lists, 30 calls: device memory flat")
+    import gc
+    import torch
+    import cudolfinx as cufem
+    P = problems.poisson_p1(24)
+    g = Function(P.V)
+    g.x[:] = 1.0
+    bc = dirichletbc(g, locate_dofs_topological(P.V, 2, P.mesh.exterior_facets), P.V)
+    a, L = cufem.form(P.a), cufem.form(P.L)
+    asm = cufem.CUDAAssembler()
+    b = asm.assemble_vector(L)
+    for _ in range(2):
+        asm.apply_lifting(b, [a], [[bc]])
+    asm.synchronize()
+    gc.collect()
+    free0 = torch.cuda.mem_get_info()[0]
+    for _ in range(30):
+        asm.apply_lifting(b, [a], [[bc]])
+    asm.synchronize()
+    gc.collect()
+    grown = free0 - torch.cuda.mem_get_info()[0]
+    _say(f"  device memory grown by {grown / 1e6:.2f} MB over 30 calls "
+         f"(a list per call would be {30 * 4 * P.mesh.num_cells / 1e6:.1f} MB)")
+    assert grown < 4 * P.mesh.num_cells * 10
+
+
+def test_host_coefficient_writes_reach_the_next_assembly():
+    """Writing a coefficient's values in place (f.x[:] = ..., f.x *= ...)
+    without touching anything else is seen by the next assembly."""
+    _say("coefficient written through f.x only, then reassembled")
+    import cudolfinx as cufem
+    P = problems.elasticity_p2(3)
+    a = cufem.form(P.a)
+    asm = cufem.CUDAAssembler()
+    dbc = asm.pack_bcs(P.bcs)
+    A = asm.assemble_matrix(a, bcs=dbc)
+    asm.synchronize()
+    mu = P.extras["mu"]
+    for step, write in enumerate((lambda: mu.x.__setitem__(slice(None), mu.x * 1.5),
+                                  lambda: mu.x.__imul__(0.25))):
+        write()
+        asm.assemble_matrix(a, A, bcs=dbc)
+        asm.synchronize()
+        _, _, vals = A.csr()
+        orp, ocols, ovals, _ = _oracle(P)
+        ea = _rel(vals, ovals)
+        _say(f"  write {step}: relerr A={ea:.3e}")
+        assert ea <= TOL
+
+
+def test_autotune_on_one_matrix_keeps_other_matrices_correct():
+    """Two matrices of one form: tuning (recompiling) the form on one leaves
+    the schedule of the other stale; its next assembly rebuilds it and still
+    matches the oracle."""
+    _say("autotune with a second matrix of the same form, then reassemble that matrix")
+    import cudolfinx as cufem
+    from cudolfinx.autotune import autotune_matrix_native
+    P = problems.elasticity_p2(4)
+    a = cufem.form(P.a, options={"chunk_values": 6144, "owner_pairs": 512, "owner_lds": 98304})
+    asm = cufem.CUDAAssembler()
+    dbc = asm.pack_bcs(P.bcs)
+    A1, A2 = asm.create_matrix(a), asm.create_matrix(a)
+    asm.assemble_matrix(a, A1, bcs=dbc)
+    asm.assemble_matrix(a, A2, bcs=dbc)
+    asm.synchronize()
+    desc, ms = autotune_matrix_native(a, asm, A1, P.bcs, reps=1)
+    _say(f"  tuned on A1: {desc}")
+    asm.assemble_matrix(a, A2, bcs=dbc)
+    asm.synchronize()
+    orp, ocols, ovals, _ = _oracle(P)
+    rp, cols, vals = A2.csr()
+    assert np.array_equal(rp, orp) and np.array_equal(cols, ocols)
+    assert _rel(vals, ovals) <= TOL
+
+
+def _oracle_csr(P):
+    import scipy.sparse as sp
+    rp, cols, vals, b = _oracle(P)
+    n = P.V.size
+    return sp.csr_matrix((vals, cols, rp), shape=(n, n)), b
+
+
+def test_adopted_csr_scipy_matches_oracle():
+    """CUDAMatrix(ctx, scipy CSR) with the oracle's pattern (bs-unrolled block
+    structure: owner-computes path): after the handoff the caller's matrix
+    holds the oracle's values."""
+    _say("adopted scipy CSR (oracle pattern, vector P2 elasticity n=3)")
+    import cudolfinx as cufem
+    from cudolfinx.context import get_cuda_context
+    P = problems.elasticity_p2(3)
+    ref, _ = _oracle_csr(P)
+    mine = ref.copy()
+    mine.data[:] = -7.0
+    a = cufem.form(P.a, options={"matrix_kernel": 2})
+    asm = cufem.CUDAAssembler()
+    M = cufem.CUDAMatrix(get_cuda_context(), mine)
+    asm.assemble_matrix(a, M, bcs=P.bcs)
+    M.assemble()
+    assert M.mat is mine
+    ea = _rel(mine.data, ref.data)
+    _say(f"  relerr A={ea:.3e}")
+    assert ea <= TOL
+
+
+def test_adopted_csr_torch_superset_borrowed_values():
+    """A torch sparse CSR on the GPU whose pattern is the oracle's plus extra
+    scalar entries (no longer bs-unrolled blocks: per-entity path): values are
+    assembled in place in the tensor's own memory; pattern entries equal the
+    oracle, extra entries are zero."""
+    _say("adopted torch CSR (superset pattern, borrowed device values)")
+    import scipy.sparse as sp
+    import torch
+    import cudolfinx as cufem
+    from cudolfinx.context import get_cuda_context
+    P = problems.elasticity_p2(3)
+    ref, _ = _oracle_csr(P)
+    n = P.V.size
+    rng = np.random.default_rng(9)
+    refpat = sp.csr_matrix((np.ones(ref.nnz), ref.indices, ref.indptr), shape=(n, n))
+    extra = sp.csr_matrix((np.ones(40), (rng.integers(0, n, 40), rng.integers(0, n, 40))), shape=(n, n))
+    pat = (refpat + extra).tocsr()          # all entries positive: nothing is pruned
+    pat.sort_indices()
+    assert pat.nnz > ref.nnz
+    t = torch.sparse_csr_tensor(torch.from_numpy(pat.indptr.astype(np.int64)),
+                                torch.from_numpy(pat.indices.astype(np.int64)),
+                                torch.full((pat.nnz,), 5.0, dtype=torch.float64), size=(n, n)).cuda()
+    vptr = t.values().data_ptr()
+    a = cufem.form(P.a)
+    asm = cufem.CUDAAssembler()
+    M = cufem.CUDAMatrix(get_cuda_context(), t)
+    asm.assemble_matrix(a, M, bcs=P.bcs)
+    M.assemble()
+    assert t.values().data_ptr() == vptr
+    vals = t.values().cpu().numpy()                  # in the pattern's CSR order
+    rows = np.repeat(np.arange(n), np.diff(pat.indptr))
+    cols = pat.indices
+    mask = np.asarray(refpat[rows, cols]).ravel() != 0
+    expect = np.asarray(ref[rows, cols]).ravel()
+    ea = _rel(vals[mask], expect[mask])
+    _say(f"  relerr A={ea:.3e}, {int((~mask).sum())} extra entries, max |extra| "
+         f"{np.abs(vals[~mask]).max():.1e}")
+    assert ea <= TOL and np.all(vals[~mask] == 0.0)
+
+
+def test_adopted_csr_missing_entry_is_rejected():
+    """A pattern lacking an entry the form touches is rejected with
+    MF_ENOTFOUND at the first assembly; a wrong shape with ValueError."""
+    _say("adopted CSR with a missing entry / wrong shape")
+    import scipy.sparse as sp
+    import cudolfinx as cufem
+    from cudolfinx._lib import MifemError
+    from cudolfinx.context import get_cuda_context
+    P = problems.poisson_p1(3)
+    ref, _ = _oracle_csr(P)
+    ones = sp.csr_matrix((np.ones(ref.nnz), ref.indices, ref.indptr), shape=ref.shape)
+    lil = ones.tolil()
+    lil[5, 5] = 0.0
+    holed = lil.tocsr()
+    holed.eliminate_zeros()
+    assert holed.nnz == ref.nnz - 1
+    a = cufem.form(P.a)
+    asm = cufem.CUDAAssembler()
+    M = cufem.CUDAMatrix(get_cuda_context(), holed)
+    with pytest.raises(MifemError, match="missing"):
+        asm.assemble_matrix(a, M)
+    small = sp.identity(4, format="csr")
+    with pytest.raises(ValueError):
+        asm.assemble_matrix(a, cufem.CUDAMatrix(get_cuda_context(), small))

@@ -11,11 +11,14 @@ outside the timed region.  Inputs live in HBM before timing starts.
 
 Printed: one JSON line with DoF/s (value), the roofline of the dominant
 kernel (owner-computes matrix kernel, timed live with HIP events on the
-stream it runs on, beside the copy bandwidth measured on the same GPU) and a
-CPU baseline (the oracle on all usable cores, at most 16, and on one core,
-on a bounded sample of the same workload).  --workload c3 / c4 / c5 run the
-Q3 Helmholtz precision comparison, the fused SIPG Newton step and the
-512-instance batch instead.
+stream it runs on, beside the copy bandwidth measured on the same GPU), a
+CPU baseline (the oracle on the full workload on every core the process may
+use, and on one core on the 32^3 sample), and the metric's mixed-precision
+half as the `c3` sub-record: Q3 Helmholtz on 48^3 jittered hexes, FP64 GPU
+path against the FP32 / BF16 matrix-core paths, errors of A, b and a
+fixed-tolerance CG solution against the FP64 oracle.  --workload c3 / c4 /
+c5 run the Q3 precision comparison, the fused SIPG Newton step and the
+512-instance batch as their own lines.
 """
 from __future__ import annotations
 
@@ -102,8 +105,40 @@ def measured_traffic(name: str = "c2_elasticity_p2_64_matrix_traffic.md"):
     return None
 
 
+def _cgroup_cpus():
+    """CPUs the cgroup quota allows this process (None: no quota)."""
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            return max(1, -(-int(quota) // int(period)))
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def _cpu_threads() -> int:
-    return max(1, min(16, len(os.sched_getaffinity(0))))
+    """Every core this process may run on: its CPU affinity, capped by the
+    cgroup CPU quota when one is set (more threads than the quota would only
+    time-slice)."""
+    n = len(os.sched_getaffinity(0))
+    q = _cgroup_cpus()
+    return max(1, min(n, q) if q else n)
+
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def _cpu_fields() -> dict:
+    """CPU model and core counts recorded with every CPU baseline."""
+    return {"cpu": _cpu_model(), "affinity_cores": len(os.sched_getaffinity(0)),
+            "cgroup_cpus": _cgroup_cpus(), "host_cpus": os.cpu_count()}
 
 
 def cpu_baseline_c3(n_sample: int = 16):
@@ -127,7 +162,7 @@ def cpu_baseline_c3(n_sample: int = 16):
             "est_full_ms": 110592 / (nc / dt) * 1e3, "serial_value": nc / dt1,
             "sample": f"helmholtz_q3 {n_sample}^3 hexes ({nc} cells): A in {dt:.2f} s on {threads} "
                       f"threads (oracle, C + OpenMP, coloured cells), {dt1:.2f} s serial",
-            "cpu": platform.processor() or platform.machine()}
+            **_cpu_fields()}
 
 
 def cpu_baseline_c4(n_sample: int = 24):
@@ -159,7 +194,7 @@ def cpu_baseline_c4(n_sample: int = 24):
             "sample": f"sipg_plaplace DG1 {n_sample}^3x6 tets ({nc} cells): J + F + E in {dt:.2f} s "
                       f"on {threads} threads (oracle, C + OpenMP, coloured entities; E serial), "
                       f"{dt1:.2f} s serial",
-            "cpu": platform.processor() or platform.machine()}
+            **_cpu_fields()}
 
 
 def cpu_baseline_c5(B, per_degree: int = 2):
@@ -190,47 +225,61 @@ def cpu_baseline_c5(B, per_degree: int = 2):
     return {"value": len(picked) / total, "unit": "assemblies/s", "cores": 1, "kind": "port",
             "sample": f"instances {[i for i, _ in picked]} (first {per_degree} of each degree): "
                       f"A + b in {total:.2f} s, oracle (C, serial)",
-            "cpu": platform.processor() or platform.machine()}
+            **_cpu_fields()}
 
 
-def cpu_baseline(n_sample: int):
-    """Oracle (FP64) on the same workload at n_sample^3 x 6 cells: serial, and
-    on all usable cores (at most 16) with the entities coloured so that no two
-    of a colour share a row (colouring and sparsity are set-up, not timed)."""
+def cpu_baseline(n_full: int, n_serial: int):
+    """Oracle (FP64) on the same workload: A + b (matrix, vector, lifting,
+    set_bc) at n_full^3 x 6 cells on every core this process may use, the
+    entities coloured so that no two of a colour share a row (colouring and
+    sparsity are set-up, not timed); and serially on n_serial^3 x 6 cells."""
     from mifem import problems
     from oracle import cpu
-    P = problems.elasticity_p2(n_sample)
-    Oa, OL = cpu.OracleAssembler(P.a), cpu.OracleAssembler(P.L)
-    rp, cols, _, _ = Oa.sparsity()
-    t0 = time.perf_counter()
-    Oa.assemble_matrix(rp, cols, P.bcs)
-    b = OL.assemble_vector()
-    Oa.apply_lifting(b, P.bcs)
-    cpu.set_bc(b, P.bcs, P.V)
-    dt1 = time.perf_counter() - t0
+
+    def run(n, threads):
+        P = problems.elasticity_p2(n)
+        Oa, OL = cpu.OracleAssembler(P.a), cpu.OracleAssembler(P.L)
+        rp, cols, _, _ = Oa.sparsity()
+        if threads > 1:
+            Oa.colouring()
+            OL.colouring()
+        t0 = time.perf_counter()
+        if threads > 1:
+            Oa.assemble_matrix_mt(rp, cols, P.bcs, threads=threads)
+            b = OL.assemble_vector_mt(threads=threads)
+            Oa.apply_lifting(b, P.bcs, threads=threads)
+        else:
+            Oa.assemble_matrix(rp, cols, P.bcs)
+            b = OL.assemble_vector()
+            Oa.apply_lifting(b, P.bcs)
+        cpu.set_bc(b, P.bcs, P.V)
+        return P, time.perf_counter() - t0
+
     threads = _cpu_threads()
-    Oa.colouring()
-    OL.colouring()
-    t0 = time.perf_counter()
-    Oa.assemble_matrix_mt(rp, cols, P.bcs, threads=threads)
-    b = OL.assemble_vector_mt(threads=threads)
-    Oa.apply_lifting(b, P.bcs, threads=threads)
-    cpu.set_bc(b, P.bcs, P.V)
-    dtn = time.perf_counter() - t0
+    P, dtn = run(n_full, threads)
+    Ps, dt1 = run(n_serial, 1)
     return {"value": P.V.size / dtn, "unit": "DoF/s", "cores": threads, "kind": "port",
-            "sample": f"elasticity_p2 {n_sample}^3x6 tets ({P.mesh.num_cells} cells, "
+            "sample": f"elasticity_p2 {n_full}^3x6 tets ({P.mesh.num_cells} cells, "
                       f"{P.V.size} dofs): A+b in {dtn:.2f} s on {threads} threads (oracle, C + "
-                      f"OpenMP, coloured cells; set_bc serial), {dt1:.2f} s serial",
-            "serial_value": P.V.size / dt1,
-            "cpu": platform.processor() or platform.machine()}
+                      f"OpenMP, coloured cells; set_bc serial); serial figure: {n_serial}^3x6 "
+                      f"({Ps.V.size} dofs) in {dt1:.2f} s on one core",
+            "serial_value": Ps.V.size / dt1,
+            **_cpu_fields()}
 
 
-def bench_c3(args):
+def c3_record(args, oracle: bool = True) -> dict:
     """Q3 Helmholtz (k=5) on 48^3 jittered hexes: FP64 vs FP32 vs BF16 matrix-core paths.
 
-    value = FP64-path time / FP32-path time for A (matrix-core owner kernel,
-    including its per-cell prep), with the FP32 path's relative errors vs the
-    FP64 path (A Frobenius, b l2, CG solution) reported beside it.
+    speedup = FP64 GPU path time / FP32 matrix-core time for A (each a whole
+    matrix launch: per-cell record pass + owner kernel).  Errors are taken
+    against the oracle's FP64 assembly of the same A and b (its coloured
+    multi-core loops, timed here as the CPU baseline): relative Frobenius
+    error of A, l2 error of b, and the relative error of the solution of
+    Jacobi-CG at a fixed tolerance with each matrix against the solution with
+    the oracle's matrix.  The <= 1e-6 gate applies to the assembled A (and b);
+    the CG solution error is A's error amplified by the condition number of
+    this near-resonant Helmholtz system (k^2 = 25 against the first Dirichlet
+    eigenvalue 3 pi^2), and is reported beside it.
     """
     import numpy as np
     import torch
@@ -251,26 +300,27 @@ def bench_c3(args):
     stream = torch.cuda.current_stream()
     lib = _lib.lib()
     b0 = bcs._get_cpp_bcs(P.V)
-    # FP64 reference values live in a torch buffer borrowed by the matrix
-    vals = {}
-    times = {}
-    for prec in (64, 32, 16):
-        a.set_option("precision", prec)
-        buf = torch.empty(A.nnz, dtype=torch.float64, device="cuda")
-        _lib.check(lib.mf_matrix_borrow_values(A.handle, C.c_void_p(buf.data_ptr())))
-        asm.assemble_matrix(a, A, bcs=bcs)
-        asm.synchronize()
+
+    def launch_ms(reps):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        reps = max(3, args.steps // 2)
         e0.record(stream)
         for _ in range(reps):
             _lib.check(lib.mf_assemble_matrix(a.handle, A.handle, b0.handle, b0.handle, 1.0, 1,
                                               C.c_void_p(stream.cuda_stream)))
         e1.record(stream)
         e1.synchronize()
-        times[prec] = e0.elapsed_time(e1) / reps
+        return e0.elapsed_time(e1) / reps
+
+    vals, times = {}, {}
+    for prec in (64, 32, 16):
+        a.set_option("precision", prec)
+        buf = torch.empty(A.nnz, dtype=torch.float64, device="cuda")
+        _lib.check(lib.mf_matrix_borrow_values(A.handle, C.c_void_p(buf.data_ptr())))
+        asm.assemble_matrix(a, A, bcs=bcs)
+        asm.synchronize()
+        times[prec] = launch_ms(max(3, args.steps // 2))
         vals[prec] = buf
-        log(f"precision {prec}: matrix {times[prec]:.3f} ms")
+        log(f"C3 precision {prec}: matrix {times[prec]:.3f} ms")
     # the reference's scheme in FP64: one thread per cell, full 64 x 64 tabulate_tensor,
     # FP64 atomics into the CSR (per-entity kernel)
     a.set_option("precision", 64)
@@ -279,17 +329,10 @@ def bench_c3(args):
     _lib.check(lib.mf_matrix_borrow_values(A.handle, C.c_void_p(gbuf.data_ptr())))
     asm.assemble_matrix(a, A, bcs=bcs)
     asm.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(2):
-        _lib.check(lib.mf_assemble_matrix(a.handle, A.handle, b0.handle, b0.handle, 1.0, 1,
-                                          C.c_void_p(stream.cuda_stream)))
-    e1.record(stream)
-    e1.synchronize()
-    generic_ms = e0.elapsed_time(e1) / 2
-    generic_err = float(torch.linalg.norm(gbuf - vals[64]) / torch.linalg.norm(vals[64]))
+    generic_ms = launch_ms(2)
+    generic_diff = float(torch.linalg.norm(gbuf - vals[64]) / torch.linalg.norm(vals[64]))
     del gbuf
-    log(f"per-entity FP64 kernel: matrix {generic_ms:.3f} ms (rel diff to owner FP64 {generic_err:.1e})")
+    log(f"C3 per-entity FP64 kernel: matrix {generic_ms:.3f} ms")
     a.set_option("matrix_kernel", 2)
     asm.assemble_vector(L, b)
     asm.apply_lifting(b, [a], [bcs])
@@ -305,51 +348,98 @@ def bench_c3(args):
     e1.synchronize()
     bms = e0.elapsed_time(e1) / 3
     bt = torch.from_numpy(b.to_host()).cuda()
-    ref = vals[64]
+    rec = {"workload": f"helmholtz_q3 {args.n_hex}^3 jittered hexes, k=5, 64 Gauss points",
+           "cells": P.mesh.num_cells, "dofs": P.V.size, "nnz": A.nnz,
+           "matrix_ms": {str(p): times[p] for p in times}, "b_region_ms": bms,
+           "speedup_fp32_vs_fp64": times[64] / times[32],
+           "speedup_bf16_vs_fp64": times[64] / times[16],
+           "fp64_per_entity_matrix_ms": generic_ms,
+           "speedup_fp32_vs_fp64_per_entity": generic_ms / times[32],
+           "rel_diff_per_entity_vs_owner_fp64": generic_diff,
+           "note": "FP64 = vector-ALU FP64 contraction, FP32 / BF16 = matrix cores (FP32 "
+                   "accumulation), geometry and CSR accumulation FP64 in all; b on the FP64 "
+                   "vector path in every variant"}
+    flops = 2.175e6 * P.mesh.num_cells
+    rec["fp32_tflops"] = flops / (times[32] * 1e-3) / 1e12
+    rec["fp64_tflops"] = flops / (times[64] * 1e-3) / 1e12
+    rec["bf16_tflops"] = flops / (times[16] * 1e-3) / 1e12
+    if not oracle:
+        return rec
+    from oracle import cpu
+    threads = _cpu_threads()
+    Oa, OL = cpu.OracleAssembler(P.a), cpu.OracleAssembler(P.L)
+    orp, ocols, _, _ = Oa.sparsity()
+    rp, cols, _ = A.csr()
+    rec["csr_bit_exact"] = bool(np.array_equal(rp, orp) and np.array_equal(cols, ocols))
+    del rp, cols
+    Oa.colouring()
+    OL.colouring()
+    t1 = time.perf_counter()
+    ovals = Oa.assemble_matrix_mt(orp, ocols, P.bcs, threads=threads)
+    t_oracle = time.perf_counter() - t1
+    ob = OL.assemble_vector_mt(threads=threads)
+    Oa.apply_lifting(ob, P.bcs, threads=threads)
+    cpu.set_bc(ob, P.bcs, P.V)
+    log(f"C3 oracle A on {threads} threads: {t_oracle:.1f} s")
+    ref = torch.from_numpy(ovals).cuda()
+    del ovals
     rnorm = torch.linalg.norm(ref)
-    errs = {p: float(torch.linalg.norm(vals[p] - ref) / rnorm) for p in (32, 16)}
-    # fixed-tolerance CG (Jacobi) with each matrix
+    rec["rel_err_A"] = {str(p): float(torch.linalg.norm(vals[p] - ref) / rnorm) for p in vals}
+    obt = torch.from_numpy(ob).cuda()
+    rec["rel_err_b"] = float(torch.linalg.norm(bt - obt) / torch.linalg.norm(obt))
+    # fixed-tolerance Jacobi-CG with each matrix (native CSR SpMV), against the
+    # solution with the oracle's matrix and vector
     sol = {}
-    for p in (64, 32, 16):
-        # native CSR SpMV on the matrix's arrays, with this precision's values
-        _lib.check(lib.mf_matrix_borrow_values(A.handle, C.c_void_p(vals[p].data_ptr())))
+    for p, v, rhs in (("oracle", ref, obt), ("64", vals[64], bt), ("32", vals[32], bt),
+                      ("16", vals[16], bt)):
+        _lib.check(lib.mf_matrix_borrow_values(A.handle, C.c_void_p(v.data_ptr())))
         torch.cuda.synchronize()
         t_cg = time.perf_counter()
-        x, it, res = pcg(A, bt, rtol=1e-12, maxiter=args.cg_maxiter)
+        x, it, res = pcg(A, rhs, rtol=1e-12, maxiter=args.cg_maxiter)
         torch.cuda.synchronize()
-        t_cg = time.perf_counter() - t_cg
-        sol[p] = (x, it, res, t_cg / max(it, 1) * 1e3)
-        log(f"CG precision {p}: {it} iterations, relative residual {res:.2e}, "
-            f"{t_cg / max(it, 1) * 1e3:.3f} ms per iteration")
-    xref = sol[64][0]
-    xerr = {p: float(torch.linalg.norm(sol[p][0] - xref) / torch.linalg.norm(xref)) for p in (32, 16)}
-    flops = 2.175e6 * P.mesh.num_cells
+        sol[p] = (x, it, res, (time.perf_counter() - t_cg) / max(it, 1) * 1e3)
+        log(f"C3 CG with the {p} matrix: {it} iterations, relative residual {res:.2e}")
+    xref = sol["oracle"][0]
+    xn = torch.linalg.norm(xref)
+    rec["rel_err_cg_solution"] = {p: float(torch.linalg.norm(sol[p][0] - xref) / xn)
+                                  for p in ("64", "32", "16")}
+    rec["cg"] = {"rtol": 1e-12, "iterations": {p: sol[p][1] for p in sol},
+                 "ms_per_iteration": {p: sol[p][3] for p in sol}}
+    rec["gate"] = {"bound": 1e-6, "on": "rel_err_A and rel_err_b (assembled system)",
+                   "fp32_meets": rec["rel_err_A"]["32"] <= 1e-6 and rec["rel_err_b"] <= 1e-6,
+                   "fp32_cg_solution_meets": rec["rel_err_cg_solution"]["32"] <= 1e-6}
+    rec["cpu_baseline"] = {"value": P.mesh.num_cells / t_oracle, "unit": "cells/s (A)",
+                           "cores": threads, "kind": "port",
+                           "sample": f"the full workload: oracle FP64 A in {t_oracle:.2f} s on "
+                                     f"{threads} threads (C + OpenMP, coloured cells)",
+                           **_cpu_fields()}
+    del ref
+    return rec
+
+
+def bench_c3(args):
+    """Q3 Helmholtz precision comparison as its own JSON line: value = FP64-path
+    time / FP32-path time for A, with the errors of c3_record."""
+    rec = c3_record(args, oracle=not args.no_cpu)
+    flops = 2.175e6 * rec["cells"]
     out = {
         "metric": "mixed-prec vs FP64 speedup at rel err<=1e-6 (Q3 Helmholtz matrix assembly)",
-        "value": times[64] / times[32],
+        "value": rec["speedup_fp32_vs_fp64"],
         "unit": "x (FP64 time / FP32 time)",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": times[32] + bms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 (MFMA contraction; FP64 geometry and accumulation)",
-        "note": "b is assembled on the FP64 vector path in every variant (rel_err_b = 0 by construction); "
-                "FP64 variant = vector-ALU FP64 contraction, FP32/BF16 = matrix cores",
+        "ms_per_step": rec["matrix_ms"]["32"] + rec["b_region_ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (MFMA contraction; FP64 geometry and accumulation)",
         "data": "synthetic (seeded jitter)",
-        "config": {"workload": f"helmholtz_q3 {args.n_hex}^3 jittered hexes, k=5, 64 Gauss points",
-                   "cells": P.mesh.num_cells, "dofs": P.V.size, "nnz": A.nnz,
-                   "matrix_ms": {str(p): times[p] for p in times}, "b_region_ms": bms,
-                   "fp64_per_entity_matrix_ms": generic_ms,
-                   "speedup_fp32_vs_fp64_per_entity": generic_ms / times[32],
-                   "rel_diff_per_entity_vs_owner_fp64": generic_err,
-                   "rel_err_A": errs, "rel_err_b": 0.0, "rel_err_cg_solution": xerr,
-                   "cg_iterations": {str(p): sol[p][1] for p in sol},
-                   "cg_ms_per_iteration": {str(p): sol[p][3] for p in sol}},
-        "roofline": {"bound": "mfma", "achieved": flops / (times[32] * 1e-3) / 1e12, "peak": 157.3,
-                     "unit": "TFLOP/s", "frac": flops / (times[32] * 1e-3) / 1e12 / 157.3,
+        "config": {k: v for k, v in rec.items() if k != "cpu_baseline"},
+        "roofline": {"bound": "mfma", "achieved": rec["fp32_tflops"], "peak": 157.3,
+                     "unit": "TFLOP/s", "frac": rec["fp32_tflops"] / 157.3,
                      "traffic": (measured_traffic("c3_helmholtz_q3_48_traffic.md")
                                  if args.n_hex == 48 else None),
                      "kernel": "matrix launch: mf_hprep_cell_0 + mf_howner32_cell_0",
-                     "fp64_achieved": flops / (times[64] * 1e-3) / 1e12, "fp64_peak": 78.6},
-        "cpu_baseline": None if args.no_cpu else cpu_baseline_c3(),
+                     "fp64_achieved": rec["fp64_tflops"], "fp64_peak": 78.6,
+                     "algorithmic_flops": flops},
+        "cpu_baseline": rec.get("cpu_baseline"),
     }
     m = measured_peaks("mfma")
     m["frac_of_sgemm"] = out["roofline"]["achieved"] / m["sgemm_tflops"]
@@ -700,7 +790,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=64, help="cubes per direction (64: 1.57M cells)")
     ap.add_argument("--kernel", type=int, default=0, help="matrix kernel: 0 auto, 1 atomic, 2 owner")
-    ap.add_argument("--cpu-sample", type=int, default=32, help="oracle sample size (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=64,
+                    help="oracle size for the multi-core CPU baseline (0: skip)")
+    ap.add_argument("--cpu-serial", type=int, default=32, help="oracle size for the one-core figure")
+    ap.add_argument("--no-c3", action="store_true",
+                    help="c2: skip the Q3 mixed-precision sub-record")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the owner-kernel autotuner")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
@@ -854,7 +948,18 @@ def main():
         log(f"traffic calibration: fill + sum of {args.copy_calib} MiB (sum={float(tot):.0f})")
     if not args.no_cpu and args.cpu_sample > 0:
         log("cpu baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.cpu_serial)
+    if not args.no_c3:
+        # the metric's mixed-precision half: Q3 FP32 / BF16 matrix cores against
+        # the FP64 GPU path, errors against the FP64 oracle
+        log("c3: Q3 Helmholtz mixed precision ...")
+        c3 = c3_record(args, oracle=not args.no_cpu)
+        m = measured_peaks("mfma")
+        c3["mfma_roofline"] = {"bound": "mfma", "achieved": c3["fp32_tflops"], "peak": 157.3,
+                               "unit": "TFLOP/s", "frac": c3["fp32_tflops"] / 157.3,
+                               "measured": m, "frac_of_sgemm": c3["fp32_tflops"] / m["sgemm_tflops"],
+                               "fp64_frac_of_dgemm": c3["fp64_tflops"] / m["dgemm_tflops"]}
+        out["c3"] = c3
     print(json.dumps(out), flush=True)
 
 

@@ -64,6 +64,7 @@ int mf_context_create(int device, const char* cache_dir, mf_context** out) {
     c->cache_dir = cache_dir ? std::string(cache_dir) : mf::default_cache_dir();
     c->num_cus = p.multiProcessorCount;
     c->lds_per_block = p.sharedMemPerBlock;
+    mf::lift_scratch_init(device);     // device address of the allocation-free lifting scratch
     *out = c;
   });
 }
@@ -1179,16 +1180,75 @@ int mf_apply_lifting(mf_form* const* a, int nforms, double* b, int64_t n,
         if (I->k_slift && f->opt("lift_kernel", 0) != 1) {
           const int64_t nt = nl * f->dm[0]->nd;
           mf::launch(I->k_slift, static_cast<uint32_t>((nt + 127) / 128), 128, 0, S(s),
-                     static_cast<int32_t>(nl), it->second.first.ptr, I->d_ents.ptr,
+                     static_cast<int32_t>(nl), static_cast<const int32_t*>(nullptr),
+                     it->second.first.ptr, I->d_ents.ptr,
                      f->mesh->cells.ptr, f->mesh->x.ptr, f->packed.ptr, f->consts.ptr,
                      f->dm[0]->dev.ptr, f->dm[1]->dev.ptr, bc->markers.ptr, bc->g.ptr, xk, scale,
                      b);
         } else {
           mf::launch(I->k_lift, static_cast<uint32_t>((nl + block - 1) / block), block, 0, S(s),
-                     static_cast<int32_t>(nl), it->second.first.ptr, I->d_ents.ptr,
+                     static_cast<int32_t>(nl), static_cast<const int32_t*>(nullptr),
+                     it->second.first.ptr, I->d_ents.ptr,
                      f->mesh->cells.ptr, f->mesh->x.ptr, f->packed.ptr, f->consts.ptr,
                      f->dm[0]->dev.ptr, f->dm[1]->dev.ptr, bc->markers.ptr, bc->g.ptr, xk, scale,
                      b, f->mesh->fperm.ptr);
+        }
+      }
+    }
+  });
+}
+
+int mf_apply_lifting_noalloc(mf_form* const* a, int nforms, double* b, int64_t n,
+                             const mf_bc* const* bcs1, const double* const* x0, double scale,
+                             mf_stream s) {
+  return guard([&] {
+    MF_CHECK(nforms >= 0 && (nforms == 0 || (a && bcs1)) && b, MF_EINVAL, "bad arguments");
+    for (int k = 0; k < nforms; ++k) {
+      mf_form* f = a[k];
+      const mf_bc* bc = bcs1[k];
+      MF_CHECK(f && f->rank == 2 && f->compiled, MF_EINVAL, "lifting needs compiled bilinear forms");
+      MF_CHECK(!f->restricted, MF_EUNSUPPORTED, "lifting of block/restricted forms is not supported");
+      MF_CHECK(n == f->dm[0]->num_nodes * f->dm[0]->bs, MF_ESHAPE, "vector length mismatch");
+      if (!bc || bc->nbc == 0) continue;
+      MF_CHECK(bc->ndofs == f->dm[1]->num_nodes * f->dm[1]->bs, MF_ESHAPE,
+               "bc size does not match the trial space");
+      for (auto& I : f->ints)
+        MF_CHECK(I->n <= mf::kLiftScratchEntities, MF_EUNSUPPORTED,
+                 "integral with " + std::to_string(I->n) + " entities exceeds the allocation-free "
+                 "lifting limit of " + std::to_string(mf::kLiftScratchEntities));
+    }
+    for (int k = 0; k < nforms; ++k) {
+      mf_form* f = a[k];
+      const mf_bc* bc = bcs1[k];
+      if (!bc || bc->nbc == 0) continue;
+      use_device(f->ctx);
+      int32_t* ptr = mf::lift_scratch(f->ctx->device);   // [0] count, [64 ..] list
+      int32_t* count = ptr;
+      int32_t* list = ptr + 64;
+      const double* xk = x0 ? x0[k] : nullptr;
+      const uint32_t block = static_cast<uint32_t>(f->opt("block", 128));
+      for (auto& I : f->ints) {
+        if (I->n == 0) continue;
+        // the list of entities touching a constrained column, built on the
+        // device each call; the lift kernel reads its length from `count`
+        // and its grid covers every entity of the integral
+        mf::k_fill_i32(count, 1, 0, S(s));
+        mf::k_lift_list(list, count, I->d_ents.ptr, I->n, mf::entity_stride(I->type),
+                        mf::entity_ncell(I->type), f->dm[1]->dev.ptr, f->dm[1]->nd,
+                        f->dm[1]->bs, bc->markers.ptr, S(s));
+        if (I->k_slift && f->opt("lift_kernel", 0) != 1) {
+          const int64_t nt = I->n * f->dm[0]->nd;
+          mf::launch(I->k_slift, static_cast<uint32_t>((nt + 127) / 128), 128, 0, S(s),
+                     static_cast<int32_t>(I->n), static_cast<const int32_t*>(count), list,
+                     I->d_ents.ptr, f->mesh->cells.ptr, f->mesh->x.ptr, f->packed.ptr,
+                     f->consts.ptr, f->dm[0]->dev.ptr, f->dm[1]->dev.ptr, bc->markers.ptr,
+                     bc->g.ptr, xk, scale, b);
+        } else {
+          mf::launch(I->k_lift, static_cast<uint32_t>((I->n + block - 1) / block), block, 0, S(s),
+                     static_cast<int32_t>(I->n), static_cast<const int32_t*>(count), list,
+                     I->d_ents.ptr, f->mesh->cells.ptr, f->mesh->x.ptr, f->packed.ptr,
+                     f->consts.ptr, f->dm[0]->dev.ptr, f->dm[1]->dev.ptr, bc->markers.ptr,
+                     bc->g.ptr, xk, scale, b, f->mesh->fperm.ptr);
         }
       }
     }

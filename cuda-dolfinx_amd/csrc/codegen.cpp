@@ -170,7 +170,8 @@ $GATHER
 
 const char* kLift = R"(
 extern "C" __global__ void __launch_bounds__($BLOCK)
-$KNAME(const int32_t nlist, const int32_t* __restrict__ list, const int32_t* __restrict__ ents,
+$KNAME(const int32_t nlist, const int32_t* __restrict__ nlist_dev, const int32_t* __restrict__ list,
+       const int32_t* __restrict__ ents,
        const int32_t* __restrict__ geo, const double* __restrict__ xg,
        const double* __restrict__ coeffs, const double* __restrict__ consts,
        const int32_t* __restrict__ dm0, const int32_t* __restrict__ dm1,
@@ -178,8 +179,10 @@ $KNAME(const int32_t nlist, const int32_t* __restrict__ list, const int32_t* __r
        const double* __restrict__ x0, const double scale, double* __restrict__ b,
        const uint8_t* __restrict__ fperm)
 {
+  // entities in the list: nlist, or *nlist_dev when the list was built on the
+  // device in the same stream (the grid then covers the largest possible list)
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nlist) return;
+  if (t >= (nlist_dev ? *nlist_dev : nlist)) return;
   const int e = list[t];
 $GATHER
   double A[$NR * $NC];

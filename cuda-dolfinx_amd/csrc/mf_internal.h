@@ -103,6 +103,12 @@ void launch(hipFunction_t f, uint32_t grid, uint32_t block, uint32_t shmem, hipS
 // ------------------------------------------------------------ util kernels
 // (precompiled for gfx950 in util_kernels.hip; launchers run on `s`)
 void k_fill(double* x, int64_t n, double v, hipStream_t s);
+// allocation-free lifting (mf_apply_lifting_noalloc): a static device array of
+// 64 + kLiftScratchEntities int32 per device (count at [0], list from [64])
+constexpr int64_t kLiftScratchEntities = int64_t(1) << 22;
+constexpr int kMaxScratchDevices = 64;
+void lift_scratch_init(int device);          // once per device, with that device current
+int32_t* lift_scratch(int device);           // its device address
 void k_scatter_add(double* dst, const double* src, const int32_t* map, int64_t n, hipStream_t s);
 // y = A x (CSR, int32 indices, FP64 values); d = diag(A)
 void k_spmv(int64_t nrows, int64_t nnz, const int32_t* rp, const int32_t* cols, const double* v,

@@ -523,7 +523,8 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_rows,
 
 const char* kStagedLiftTemplate = R"(
 extern "C" __global__ void __launch_bounds__(128)
-$KNAME(const int32_t nlist, const int32_t* __restrict__ list, const int32_t* __restrict__ ents,
+$KNAME(const int32_t nlist, const int32_t* __restrict__ nlist_dev, const int32_t* __restrict__ list,
+       const int32_t* __restrict__ ents,
        const int32_t* __restrict__ geo, const double* __restrict__ xg,
        const double* __restrict__ coeffs, const double* __restrict__ consts,
        const int32_t* __restrict__ dm0, const int32_t* __restrict__ dm1,
@@ -531,8 +532,10 @@ $KNAME(const int32_t nlist, const int32_t* __restrict__ list, const int32_t* __r
        const double* __restrict__ x0, const double scale, double* __restrict__ b)
 {
   constexpr int ND = SK_ND, BS = SK_BS, NQ = SK_NQ, NV = $NV, W = $W;
+  // cells in the list: nlist, or *nlist_dev when the list was built on the
+  // device in the same stream (the grid then covers the largest possible list)
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nlist * ND) return;
+  if (t >= (nlist_dev ? *nlist_dev : nlist) * ND) return;
   const int e = list[t / ND], i = t % ND;
   const int64_t c = ents[e];
   double x[NV][3];

@@ -3,6 +3,8 @@
 // compiled with --device-debug per fem.cpp:210): zero fill, coefficient
 // packing, set_bc, set_diagonal, plus the setup kernels of this engine
 // (bs-unrolling of the CSR columns, offset tables, lifting entity lists).
+#include <mutex>
+
 #include "mf_internal.h"
 
 namespace mf {
@@ -384,6 +386,39 @@ void k_build_table(int32_t* table, const int32_t* ents, int64_t nents, int strid
                                                      missing);
   MF_HIP(hipGetLastError());
 }
+// Scratch of the allocation-free lifting entry point (mf_apply_lifting_noalloc):
+// scratch[0] counts the entities of the list in scratch[64 ..].  One copy per
+// device, loaded with the library's code object and held until the process
+// ends.  Host code never passes `scratch` itself to a kernel: its device
+// address comes from hipGetSymbolAddress, looked up once per device.
+static __device__ __align__(256) int32_t scratch[kLiftScratchEntities + 64];
+
+namespace {
+std::mutex g_scratch_mutex;
+int32_t* g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+void lift_scratch_init(int device) {
+  MF_CHECK(device >= 0 && device < kMaxScratchDevices, MF_EUNSUPPORTED, "device index too large");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[device]) return;
+  int cur = -1;
+  MF_HIP(hipGetDevice(&cur));
+  MF_CHECK(cur == device, MF_ESTATE, "the scratch lookup needs its device current");
+  void* ptr = nullptr;
+  MF_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  MF_CHECK(ptr != nullptr, MF_EHIP, "no device address for the lifting scratch");
+  g_scratch_ptr[device] = static_cast<int32_t*>(ptr);
+}
+
+int32_t* lift_scratch(int device) {
+  MF_CHECK(device >= 0 && device < kMaxScratchDevices, MF_EUNSUPPORTED, "device index too large");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  MF_CHECK(g_scratch_ptr[device] != nullptr, MF_ESTATE,
+           "lifting scratch not set up on this device (create a context first)");
+  return g_scratch_ptr[device];
+}
+
 void k_lift_list(int32_t* list, int32_t* count, const int32_t* ents, int64_t nents, int stride,
                  int ncell, const int32_t* dm1, int nd1, int bs1, const int8_t* bc1,
                  hipStream_t s) {

@@ -78,6 +78,8 @@ SIGNATURES = {
     "mf_assemble_scalar_device": (C.c_int, [_vp, _vp, C.c_int, _vp]),
     "mf_apply_lifting": (C.c_int, [C.POINTER(_vp), C.c_int, _vp, _i64, C.POINTER(_vp),
                                    C.POINTER(_vp), C.c_double, _vp]),
+    "mf_apply_lifting_noalloc": (C.c_int, [C.POINTER(_vp), C.c_int, _vp, _i64, C.POINTER(_vp),
+                                           C.POINTER(_vp), C.c_double, _vp]),
     "mf_set_bc": (C.c_int, [_vp, _vp, _i64, _vp, _vp, C.c_double, _vp]),
     "mf_graph_begin": (C.c_int, [_vp, _vp]),
     "mf_graph_end": (C.c_int, [_vp, _vp, _vp]),

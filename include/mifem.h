@@ -310,11 +310,26 @@ int mf_interpolator_destroy(mf_interpolator* ip);
 
 /* apply_lifting_on_device (fem.cpp:327-355, CUDAAssembler.h:564-603):
  * b -= scale * A_j[:, bc_j] (g_j - x0_j) over every integral of every form.
- * A form whose constrained values are all zero and has no x0 adds exactly
- * nothing and launches no kernel. */
+ * Outside graph capture, a form whose constrained values are all zero and has
+ * no x0 adds exactly nothing and launches no kernel; under capture its
+ * launches are recorded (the values may change before a replay).  The first
+ * call per (form, constrained set) builds the entity lists with a host
+ * synchronisation, so it must not be captured. */
 int mf_apply_lifting(mf_form* const* a, int nforms, double* d_b, int64_t n,
                      const mf_bc* const* bcs1, const double* const* d_x0, double scale,
                      mf_stream stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory or synchronise (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc): mf_apply_lifting with the same arguments and
+ * semantics that calls no allocating routine and no host synchronisation.
+ * The entity list of each integral is rebuilt on the device every call, in a
+ * static device array of 4,194,304 entities (16 MiB per GPU, held until the
+ * process ends); integrals with more entities are rejected with
+ * MF_EUNSUPPORTED.  The array is shared by every call on a GPU: calls that
+ * overlap on one GPU (different streams, threads) race. */
+int mf_apply_lifting_noalloc(mf_form* const* a, int nforms, double* d_b, int64_t n,
+                             const mf_bc* const* bcs1, const double* const* d_x0, double scale,
+                             mf_stream stream);
 /* set_bc_on_device (fem.cpp:357-383, CUDAAssembler.cpp:99-126):
  * b[d] = scale * (g[d] - x0[d]) for the constrained dofs. */
 int mf_set_bc(mf_context* ctx, double* d_b, int64_t n, const mf_bc* bc, const double* d_x0,

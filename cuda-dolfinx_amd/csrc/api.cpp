@@ -140,7 +140,10 @@ int mf_mesh_create(mf_context* ctx, const double* x, int64_t nvert, const int32_
     m->nf = nf;
     m->x.upload(x, nvert * 3);
     m->cells.upload(cells, ncells * nv);
-    if (fperm && nf > 0) m->fperm.upload(fperm, ncells * nf);
+    if (fperm && nf > 0) {
+      m->fperm.upload(fperm, ncells * nf);
+      for (int64_t i = 0; i < ncells * nf && !m->has_perms; ++i) m->has_perms = fperm[i] != 0;
+    }
     *out = m.release();
   });
 }

@@ -61,8 +61,13 @@ const char* kGatherExt = R"(
 const char* kGatherInt = R"(
   const int32_t cl[2] = {ents[4 * e], ents[4 * e + 2]};
   const int eli[2] = {ents[4 * e + 1], ents[4 * e + 3]};
-  const uint8_t qperm[2] = {fperm ? fperm[(int64_t)cl[0] * $NF + eli[0]] : (uint8_t)0,
-                            fperm ? fperm[(int64_t)cl[1] * $NF + eli[1]] : (uint8_t)0};
+#if MF_MESH_PERMS
+  const uint8_t qperm_[2] = {fperm ? fperm[(int64_t)cl[0] * $NF + eli[0]] : (uint8_t)0,
+                             fperm ? fperm[(int64_t)cl[1] * $NF + eli[1]] : (uint8_t)0};
+  const uint8_t* qperm = qperm_;
+#else
+  const uint8_t* qperm = nullptr;                 // the mesh has no non-zero permutation
+#endif
   double xc[2 * $NV * 3];
   for (int s = 0; s < 2; ++s)
     for (int v = 0; v < $NV; ++v) {
@@ -388,7 +393,7 @@ std::string device_tabulate_source(const Integral& I) {
 std::string generate_system_source(const mf_form& a, const mf_form& L, const mf_form* M,
                                    int block, int wpe) {
   std::ostringstream os;
-  os << kPrelude;
+  os << kPrelude << "#define MF_MESH_PERMS " << (a.mesh->has_perms ? 1 : 0) << "\n";
   std::map<std::string, bool> seen;
   for (const mf_form* f : {&a, &L, M}) {
     if (!f) continue;
@@ -487,7 +492,7 @@ std::string generate_batch_source(const std::vector<const mf_form*>& forms,
                                   const std::vector<const Integral*>& ints, int block,
                                   int64_t* max_lds) {
   std::ostringstream os;
-  os << kPrelude << kOwnerCommon;
+  os << kPrelude << "#define MF_MESH_PERMS 0\n" << kOwnerCommon;
   std::string calls, params;
   int64_t lds = 0;
   for (size_t g = 0; g < forms.size(); ++g) {
@@ -599,7 +604,7 @@ std::string compile_source(mf_form& f) {
 
 std::string generate_form_source(const mf_form& f) {
   std::ostringstream os;
-  os << kPrelude;
+  os << kPrelude << "#define MF_MESH_PERMS " << (f.mesh->has_perms ? 1 : 0) << "\n";
   std::map<std::string, bool> seen;
   for (auto& I : f.ints) {
     if (seen.count(I->tab_name)) continue;

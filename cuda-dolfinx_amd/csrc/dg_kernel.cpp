@@ -179,8 +179,14 @@ const char* kDGCaseIntSide = R"(
           double w[2 * $W + 1];
 $INTERLACE          const int eli[2] = {$SIDE == 0 ? lfs : lfo, $SIDE == 0 ? lfo : lfs};
           const int64_t c0 = $SIDE == 0 ? (int64_t)c : co, c1 = $SIDE == 0 ? co : (int64_t)c;
-          const uint8_t qp[2] = {fperm ? fperm[c0 * $NF + eli[0]] : (uint8_t)0,
-                                 fperm ? fperm[c1 * $NF + eli[1]] : (uint8_t)0};
+#if MF_MESH_PERMS
+          const uint8_t qp_[2] = {fperm ? fperm[c0 * $NF + eli[0]] : (uint8_t)0,
+                                  fperm ? fperm[c1 * $NF + eli[1]] : (uint8_t)0};
+          const uint8_t* qp = qp_;
+#else
+          const uint8_t* qp = nullptr;            // the mesh has no non-zero permutation
+          (void)c0; (void)c1;
+#endif
           (void)w0; (void)w1;
 $BODY        }
 )";

@@ -174,6 +174,10 @@ struct mf_mesh {
   mf::DevArray<uint8_t> fperm;
   int64_t num_vertices = 0, num_cells = 0;
   int nv = 0, nf = 0;
+  // some facet permutation is non-zero: interior-facet kernels pass each
+  // side's permutation to the tabulate body; otherwise they pass none (a null
+  // pointer the generated bodies fold to permutation 0 at compile time)
+  bool has_perms = false;
 };
 
 struct mf_dofmap {

@@ -14,6 +14,7 @@ from .context import CUDAContext, DeviceArray, get_cuda_context
 from .form import BlockCUDAForm, CUDAForm, form
 from .graph import CUDAGraph
 from .la import CUDAMatrix, CUDAVector
+from . import jit
 from .system import CUDASystem
 
 __all__ = ["CUDAAssembler", "CUDADirichletBC", "CUDAContext", "DeviceArray", "get_cuda_context",

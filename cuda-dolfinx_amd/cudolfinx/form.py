@@ -85,7 +85,13 @@ def _create_dofmap_on_device(V):
 class CUDAForm:
     """Device wrapper of a form (CUDAForm.h:29-268)."""
 
-    def __init__(self, form, jit_args: typing.Optional[dict] = None, options=None):
+    def __init__(self, form, jit_args: typing.Optional[dict] = None, options=None,
+                 ffcx_source=None):
+        """ffcx_source: FFCx-generated C for this form (a .c path, the C text,
+        or a DOLFINx-style form whose module's .c file holds it); its
+        tabulate_tensor bodies then replace the form's own, matched to the
+        form's integrals by the file's form_integrals order (the reference's
+        names / sources / integral_tensor_indices, form.py:38-39)."""
         self._ctx = get_cuda_context()
         self._form = form
         self._cuda_mesh = _create_mesh_on_device(form.mesh)
@@ -99,12 +105,20 @@ class CUDAForm:
                                    dm0, dm1, C.byref(h)))
         self._handle = h
         self._entities = []
-        for I in form.integrals:
+        sources = [(I.name, I.source) for I in form.integrals]
+        if ffcx_source is not None:
+            from .jit import get_wrapped_tabulate_tensors
+            wrapped, order = get_wrapped_tabulate_tensors(ffcx_source)
+            if len(order) != len(form.integrals):
+                raise ValueError(f"the FFCx source has {len(order)} integrals, the form "
+                                 f"{len(form.integrals)}")
+            sources = [wrapped[i] for i in order]
+        for I, (tname, tsrc) in zip(form.integrals, sources):
             ents = np.ascontiguousarray(I.entities, dtype=np.int32)
             self._entities.append(ents)
             staged = I.staged.source().encode() if I.staged is not None else None
-            check(lib().mf_form_add_integral(h, _ITYPE[I.type], I.id, I.name.encode(),
-                                             I.source.encode(),
+            check(lib().mf_form_add_integral(h, _ITYPE[I.type], I.id, tname.encode(),
+                                             tsrc.encode(),
                                              ents.ctypes.data_as(C.POINTER(C.c_int32)),
                                              I.num_entities, staged))
         self._coef_dofmaps = []
@@ -310,15 +324,16 @@ class BlockCUDAForm:
 
 
 def form(form, restriction=None, cuda_jit_args: typing.Optional[dict] = None,
-         options: typing.Optional[dict] = None, **kwargs):
+         options: typing.Optional[dict] = None, ffcx_source=None, **kwargs):
     """CUDAForm of a mifem form; a list (of lists) of forms gives a BlockCUDAForm
-    with the given restrictions (form.py:227-249)."""
+    with the given restrictions (form.py:227-249).  ffcx_source (single forms):
+    FFCx-generated C whose tabulate_tensor bodies the form uses (CUDAForm)."""
     def _create(f):
         if f is None:
             return None
         if isinstance(f, collections.abc.Iterable) and not hasattr(f, "integrals"):
             return [_create(g) for g in f]
-        return CUDAForm(f, jit_args=cuda_jit_args, options=options)
+        return CUDAForm(f, jit_args=cuda_jit_args, options=options, ffcx_source=ffcx_source)
 
     cuda_form = _create(form)
     if isinstance(cuda_form, list):

@@ -23,8 +23,10 @@ Interior-facet tensors follow the UFCx layout the reference assembles
 (cpp/cudolfinx/fem/CUDAFormIntegral.cpp:1988-2103): rows and columns
 [cell0 dofs, cell1 dofs], coefficients interlaced (w_cell0, w_cell1)
 (:1855-1874), coordinate_dofs [2][4][3], entity_local_index {lf0, lf1}.
-Facet quadrature points of the two cells coincide because tetrahedron
-vertices are stored in ascending global order (no quadrature permutation).
+Interior-facet bodies read their facet tables through the facet quadrature
+permutations (quadrature_permutation = {perm of (cell 0, lf0), perm of
+(cell 1, lf1)}, mesh.Mesh.facet_permutations), so the two cells evaluate at
+the same physical points whatever order they list the facet's vertices in.
 """
 from __future__ import annotations
 
@@ -86,6 +88,12 @@ _FACET = """
 """
 
 
+# interior facets: the facet permutation of each side (0 without an array)
+_QPERM = """  const int qp0 = quadrature_permutation ? quadrature_permutation[0] : 0;
+  const int qp1 = quadrature_permutation ? quadrature_permutation[1] : 0;
+"""
+
+
 def _grad(s: str, tab: str, nd: int) -> str:
     """g{s}[i][d] = sum_r tab[i][r] K{s}[r][d] (physical gradients of the basis)."""
     return f"""    double g{s}[{nd}][3];
@@ -144,6 +152,7 @@ def _sipg_parts(V, u, f, alpha, penalty, p, eps, mu0):
         phif.append(t[0])
         dphif.append(np.stack([t[1], t[2], t[3]], axis=-1))
     phif, dphif = np.stack(phif), np.stack(dphif)                   # (4, nqf, nd[, 3])
+    phip, dphip = E.facet_tables_permuted(el, qf.points)            # (4, 6, nqf, nd[, 3])
     nqc, nqf = qc.weights.size, qf.weights.size
     e2 = repr(float(eps) ** 2)
     e_mu, e_dmu = repr((p - 2.0) / 2.0), repr((p - 4.0) / 2.0)
@@ -221,15 +230,22 @@ def _sipg_parts(V, u, f, alpha, penalty, p, eps, mu0):
             else:
                 loop += "    A[0] += fw * (0.5 * sig * ju * ju - du * ju);\n"
         body = (_carr("FE_w", qf.weights) + _carr("FE_phi", phif) + _carr("FE_dphi", dphif)
-                + _FACET + "  const int lf1 = entity_local_index[1];\n"
+                + _carr("FE_phiP", phip) + _carr("FE_dphiP", dphip)
+                + _FACET + "  const int lf1 = entity_local_index[1];\n" + _QPERM
                 + _geom("0", "coordinate_dofs") + _geom("1", "(coordinate_dofs + 12)") + f"""
   (void)adet0; (void)adet1;
   const double sig = {float(penalty)!r} / (0.5 * (h0 + h1));
   for (int iq = 0; iq < {nqf}; ++iq) {{
-{_grad("0", "FE_dphi[lf0][iq]", nd)}{_grad("1", "FE_dphi[lf1][iq]", nd)}    double jmp[{n2}], dn[{n2}];
+    /* each side's tables at the facet points under its permutation (the
+       identity tables when there is none) */
+    const double* ph0 = qp0 ? FE_phiP[lf0][qp0][iq] : FE_phi[lf0][iq];
+    const double* ph1 = qp1 ? FE_phiP[lf1][qp1][iq] : FE_phi[lf1][iq];
+    const double (*dp0)[3] = qp0 ? FE_dphiP[lf0][qp0][iq] : FE_dphi[lf0][iq];
+    const double (*dp1)[3] = qp1 ? FE_dphiP[lf1][qp1][iq] : FE_dphi[lf1][iq];
+{_grad("0", "dp0", nd)}{_grad("1", "dp1", nd)}    double jmp[{n2}], dn[{n2}];
     for (int i = 0; i < {nd}; ++i) {{
-      jmp[i] = FE_phi[lf0][iq][i];
-      jmp[{nd} + i] = -FE_phi[lf1][iq][i];
+      jmp[i] = ph0[i];
+      jmp[{nd} + i] = -ph1[i];
       dn[i] = 0.5 * (g0[i][0] * nx + g0[i][1] * ny + g0[i][2] * nz);
       dn[{nd} + i] = 0.5 * (g1[i][0] * nx + g1[i][1] * ny + g1[i][2] * nz);
     }}

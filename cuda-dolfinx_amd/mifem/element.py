@@ -284,8 +284,35 @@ def quad_gauss(m: int) -> Quadrature:
     return Quadrature(pts, wts)
 
 
-def tet_facet_points(lf: int, pts2d: np.ndarray) -> np.ndarray:
-    """Map facet reference points (s, t) onto facet lf of the reference tetrahedron."""
+# Facet quadrature permutations of a triangle facet: FACET_PERMS[q][k] is the
+# rank, among the facet's three global vertex numbers, of the facet's k-th
+# vertex as the cell lists it (TET_FACES order).  q = 0 is the identity
+# (vertices stored in ascending global order).
+FACET_PERMS = list(itertools.permutations(range(3)))
+
+
+def tet_facet_points(lf: int, pts2d: np.ndarray, perm: int = 0) -> np.ndarray:
+    """Map facet reference points (s, t) onto facet lf of the reference tetrahedron.
+
+    (s, t) are coordinates on the facet's vertices in ascending global order
+    (barycentric weights (1 - s - t, s, t)); `perm` (FACET_PERMS) says where
+    the cell's own facet vertices fall in that order, so both cells of an
+    interior facet evaluate at the same physical points."""
     f = TET_FACES[lf]
-    v0, v1, v2 = TET_VERTICES[f[0]], TET_VERTICES[f[1]], TET_VERTICES[f[2]]
-    return v0[None] + pts2d[:, :1] * (v1 - v0)[None] + pts2d[:, 1:2] * (v2 - v0)[None]
+    lam = np.stack([1.0 - pts2d[:, 0] - pts2d[:, 1], pts2d[:, 0], pts2d[:, 1]], axis=1)
+    r = FACET_PERMS[perm]
+    return sum(lam[:, r[k]:r[k] + 1] * TET_VERTICES[f[k]][None] for k in range(3))
+
+
+def facet_tables_permuted(el, pts2d: np.ndarray):
+    """Basis values (4, 6, nq, nd) and reference gradients (4, 6, nq, nd, 3) at
+    the facet points of every local facet under every facet permutation (the
+    tables of interior-facet bodies, indexed [lf][perm][iq])."""
+    phi = np.empty((4, len(FACET_PERMS), len(pts2d), el.ndofs))
+    dphi = np.empty(phi.shape + (3,))
+    for lf in range(4):
+        for q in range(len(FACET_PERMS)):
+            t = el.tabulate(tet_facet_points(lf, pts2d, q))
+            phi[lf, q] = t[0]
+            dphi[lf, q] = np.stack([t[1], t[2], t[3]], axis=-1)
+    return phi, dphi

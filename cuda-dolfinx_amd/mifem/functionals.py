@@ -20,7 +20,7 @@ import numpy as np
 from . import element as E
 from .fem import Function
 from .forms import CELL, EXTERIOR_FACET, INTERIOR_FACET, Form, Integral, _SIG, _carr, _uid
-from .dgforms import _FACET, _geom
+from .dgforms import _FACET, _QPERM, _geom
 
 
 def mixed_functional(u: Function, kappa: Function, v: Function) -> Form:
@@ -74,18 +74,23 @@ def mixed_functional(u: Function, kappa: Function, v: Function) -> Form:
     # interior facets: avg(v^2) avg(kappa); interlaced w: u0 u1 | k0 k1 | v0 v1
     name = f"tabulate_tensor_integral_{_uid('fn_int', k, kv)}"
     ko, vo = 2 * np_, 4 * np_
-    body = (_carr("FE_w", qi.weights) + _carr("FE_phi", facet_tab(Vp.element, qi))
-            + _carr("FE_psi", facet_tab(Vd.element, qi)) + _FACET
-            + "  const int lf1 = entity_local_index[1];\n" + f"""
+    body = (_carr("FE_w", qi.weights)
+            + _carr("FE_phi", E.facet_tables_permuted(Vp.element, qi.points)[0])
+            + _carr("FE_psi", E.facet_tables_permuted(Vd.element, qi.points)[0]) + _FACET
+            + "  const int lf1 = entity_local_index[1];\n" + _QPERM + f"""
   for (int iq = 0; iq < {qi.weights.size}; ++iq) {{
+    const double* pa = FE_phi[lf0][qp0][iq];
+    const double* pb = FE_phi[lf1][qp1][iq];
+    const double* qa = FE_psi[lf0][qp0][iq];
+    const double* qb = FE_psi[lf1][qp1][iq];
     double ka = 0.0, kb = 0.0, va = 0.0, vb = 0.0;
     for (int i = 0; i < {np_}; ++i) {{
-      ka += w[{ko} + i] * FE_phi[lf0][iq][i];
-      kb += w[{ko + np_} + i] * FE_phi[lf1][iq][i];
+      ka += w[{ko} + i] * pa[i];
+      kb += w[{ko + np_} + i] * pb[i];
     }}
     for (int i = 0; i < {nv}; ++i) {{
-      va += w[{vo} + i] * FE_psi[lf0][iq][i];
-      vb += w[{vo + nv} + i] * FE_psi[lf1][iq][i];
+      va += w[{vo} + i] * qa[i];
+      vb += w[{vo + nv} + i] * qb[i];
     }}
     A[0] += FE_w[iq] * ds * 0.5 * (va * va + vb * vb) * 0.5 * (ka + kb);
   }}

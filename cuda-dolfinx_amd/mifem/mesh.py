@@ -158,12 +158,24 @@ class Mesh:
     def facet_permutations(self) -> np.ndarray:
         """(Nc, nf) uint8 quadrature permutation of every cell facet.
 
-        Tetrahedron vertices are stored in ascending global order, so both
-        cells of an interior facet see its vertices in the same order and no
-        reflection/rotation is needed; for the structured hexahedra the
-        lexicographic vertex order gives the same guarantee.  All zeros.
+        Tetrahedra: index into element.FACET_PERMS of the ranks, in ascending
+        global order, of the facet's vertices as the cell lists them; 0 when a
+        cell stores its vertices in ascending order (the default meshes).
+        Generated interior-facet bodies evaluate their facet points through it,
+        so both cells of a facet use the same physical points.  Hexahedra keep
+        the lexicographic order of the structured lattice: all zeros.
         """
-        return np.zeros((self.num_cells, self.facet_table.shape[0]), dtype=np.uint8)
+        from .element import FACET_PERMS
+        nf = self.facet_table.shape[0]
+        if self.cell_type != "tetrahedron":
+            return np.zeros((self.num_cells, nf), dtype=np.uint8)
+        fv = self.cells[:, self.facet_table]                      # (Nc, 4, 3)
+        ranks = np.argsort(np.argsort(fv, axis=2), axis=2)       # rank of each listed vertex
+        code = ranks[..., 0] * 9 + ranks[..., 1] * 3 + ranks[..., 2]
+        lut = np.zeros(27, dtype=np.uint8)
+        for q, r in enumerate(FACET_PERMS):
+            lut[r[0] * 9 + r[1] * 3 + r[2]] = q
+        return lut[code]
 
 
 def _cube_order(n: int) -> np.ndarray:
@@ -176,7 +188,7 @@ def _cube_order(n: int) -> np.ndarray:
 
 
 def create_unit_cube(n: int, cell_type: str = "tetrahedron", jitter: float = 0.0,
-                     seed: int = 0) -> Mesh:
+                     seed: int = 0, shuffle_vertices: int | None = None) -> Mesh:
     """Unit cube split into n^3 cubes (6 Kuhn tetrahedra or 1 hexahedron each).
 
     jitter > 0 moves every *interior* vertex (i, j, k) by
@@ -184,6 +196,11 @@ def create_unit_cube(n: int, cell_type: str = "tetrahedron", jitter: float = 0.0
     a hash of the lattice index, so the perturbation does not depend on how
     the vertices are numbered.  Boundary vertices stay on the cube surface,
     so the mesh volume stays exactly 1.
+
+    shuffle_vertices (tetrahedra): a seed; each cell then lists its four
+    vertices in a random order instead of ascending global order, so the two
+    cells of an interior facet see its vertices in different orders (non-zero
+    facet quadrature permutations).
     """
     if cell_type not in ("tetrahedron", "hexahedron"):
         raise ValueError(f"unsupported cell type {cell_type!r}")
@@ -214,6 +231,9 @@ def create_unit_cube(n: int, cell_type: str = "tetrahedron", jitter: float = 0.0
     if cell_type == "tetrahedron":
         cells = cv[:, KUHN_TETS].reshape(-1, 4)
         cells = np.sort(cells, axis=1)
+        if shuffle_vertices is not None:
+            keys = np.random.default_rng(shuffle_vertices).random(cells.shape)
+            cells = np.take_along_axis(cells, np.argsort(keys, axis=1), axis=1)
         cell_cube = np.repeat(cubes, 6, axis=0)
         cell_sub = np.tile(np.arange(6, dtype=np.int64), len(cubes))
     else:

@@ -25,7 +25,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import element as E
-from .dgforms import _FACET, _geom, _grad
+from .dgforms import _FACET, _QPERM, _geom, _grad
 from .fem import Function, FunctionSpace, dirichletbc, functionspace
 from .forms import CELL, EXTERIOR_FACET, INTERIOR_FACET, Form, Integral, _SIG, _carr, _uid
 
@@ -119,15 +119,15 @@ def _int_forms(Vc: FunctionSpace, Vd: FunctionSpace, rank: int):
     nc, ndg = Vc.element.ndofs, Vd.element.ndofs
     m = (3 * kd + kc + 2) // 2
     q = E.tri_collapsed(m)
-    phc, _ = _facet_tables(Vc.element, q)
-    phd, _ = _facet_tables(Vd.element, q)
+    phc = E.facet_tables_permuted(Vc.element, q.points)[0]
+    phd = E.facet_tables_permuted(Vd.element, q.points)[0]
     ko, uo = 0, 2 * nc                                 # interlaced: k0 k1 | u0 u1
     n2 = 2 * ndg
     if rank == 2:
         loop = f"""    for (int I = 0; I < {n2}; ++I) {{
-      const double pI = 0.5 * (I < {ndg} ? FE_psi[lf0][iq][I] : FE_psi[lf1][iq][I - {ndg}]);
+      const double pI = 0.5 * (I < {ndg} ? FE_psi[lf0][qp0][iq][I] : FE_psi[lf1][qp1][iq][I - {ndg}]);
       for (int J = 0; J < {n2}; ++J) {{
-        const double uJ = J < {ndg} ? ua * FE_psi[lf0][iq][J] : ub * FE_psi[lf1][iq][J - {ndg}];
+        const double uJ = J < {ndg} ? ua * FE_psi[lf0][qp0][iq][J] : ub * FE_psi[lf1][qp1][iq][J - {ndg}];
         A[I * {n2} + J] += fw * pI * kav * uJ;
       }}
     }}
@@ -135,22 +135,22 @@ def _int_forms(Vc: FunctionSpace, Vd: FunctionSpace, rank: int):
     else:
         loop = f"""    const double uav = 0.5 * (ua * ua + ub * ub);
     for (int I = 0; I < {n2}; ++I) {{
-      const double pI = 0.5 * (I < {ndg} ? FE_psi[lf0][iq][I] : FE_psi[lf1][iq][I - {ndg}]);
+      const double pI = 0.5 * (I < {ndg} ? FE_psi[lf0][qp0][iq][I] : FE_psi[lf1][qp1][iq][I - {ndg}]);
       A[I] += fw * pI * kav * uav;
     }}
 """
     name = f"tabulate_tensor_integral_{_uid('nl_int', rank, kc, kd)}"
     body = (_carr("FE_w", q.weights) + _carr("FE_phi", phc) + _carr("FE_psi", phd) + _FACET
-            + "  const int lf1 = entity_local_index[1];\n" + f"""
+            + "  const int lf1 = entity_local_index[1];\n" + _QPERM + f"""
   for (int iq = 0; iq < {q.weights.size}; ++iq) {{
     double ka = 0.0, kb = 0.0, ua = 0.0, ub = 0.0;
     for (int i = 0; i < {nc}; ++i) {{
-      ka += w[{ko} + i] * FE_phi[lf0][iq][i];
-      kb += w[{ko + nc} + i] * FE_phi[lf1][iq][i];
+      ka += w[{ko} + i] * FE_phi[lf0][qp0][iq][i];
+      kb += w[{ko + nc} + i] * FE_phi[lf1][qp1][iq][i];
     }}
     for (int i = 0; i < {ndg}; ++i) {{
-      ua += w[{uo} + i] * FE_psi[lf0][iq][i];
-      ub += w[{uo + ndg} + i] * FE_psi[lf1][iq][i];
+      ua += w[{uo} + i] * FE_psi[lf0][qp0][iq][i];
+      ub += w[{uo + ndg} + i] * FE_psi[lf1][qp1][iq][i];
     }}
     const double kav = 0.5 * (ka + kb);
     const double fw = FE_w[iq] * ds;

@@ -84,9 +84,13 @@ def helmholtz_q3(n: int = 48, seed: int = 1, k: float = 5.0, degree: int = 3) ->
     return Problem("helmholtz_q3", mesh, V, a, L, [_boundary_bc(mesh, V)], {"k": k})
 
 
-def dg_plaplace(n: int = 48, seed: int = 2, degree: int = 1, p: float = 3.0) -> Problem:
+def dg_plaplace(n: int = 48, seed: int = 2, degree: int = 1, p: float = 3.0,
+                shuffle_vertices=None) -> Problem:
+    """SIPG p-Laplacian on jittered tetrahedra; shuffle_vertices (a seed) lists
+    each cell's vertices in random order (non-zero facet permutations)."""
     from .dgforms import sipg_plaplace
-    mesh = create_unit_cube(n, "tetrahedron", jitter=0.2, seed=seed)
+    mesh = create_unit_cube(n, "tetrahedron", jitter=0.2, seed=seed,
+                            shuffle_vertices=shuffle_vertices)
     V = functionspace(mesh, ("DG", degree))
     u = Function(V, "u")
     u.interpolate(lambda x: np.sin(np.pi * x[0]) * np.sin(np.pi * x[1]) * np.sin(np.pi * x[2]))

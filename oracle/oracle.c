@@ -488,6 +488,27 @@ static void facet_point(int lf, double s, double t, double* X)
     X[d] = TET_V[f[0]][d] + s * (TET_V[f[1]][d] - TET_V[f[0]][d]) + t * (TET_V[f[2]][d] - TET_V[f[0]][d]);
 }
 
+/* Interior facets: the reference point of the second cell (vertices xc1) at
+ * the physical position of reference point X0 of the first cell (xc0), by the
+ * affine maps x = x0 + J X: X1 = J1^{-1} (x - x1_0).  The two sides of a
+ * facet are paired through physical space, whatever order either cell lists
+ * the facet's vertices in (the engine's bodies pair them through facet
+ * quadrature permutations instead; CUDAFormIntegral.cpp:1913-1920 gathers
+ * those per facet). */
+static void other_side_point(const double* xc0, const double* X0, const double* xc1, double* X1)
+{
+  double x[3], Kinv[3][3];
+  for (int i = 0; i < 3; ++i) {
+    x[i] = xc0[i];
+    for (int j = 0; j < 3; ++j) x[i] += (xc0[3 * (j + 1) + i] - xc0[i]) * X0[j];
+  }
+  jacobian(0, xc1, X0, Kinv);
+  for (int r = 0; r < 3; ++r) {
+    X1[r] = 0.0;
+    for (int c = 0; c < 3; ++c) X1[r] += Kinv[r][c] * (x[c] - xc1[c]);
+  }
+}
+
 /* physical basis gradients of a P_k tetrahedron at X */
 static void phys_grad(const or_kernel* K, const double* xc, const double* X, double* phi,
                       double g[][3], double* adet)
@@ -571,7 +592,7 @@ static void tab_dg_int(const or_kernel* K, double* A, const double* w, const dou
     double X0[3], X1[3], phi0[OR_MAXND], phi1[OR_MAXND], g0[OR_MAXND][3], g1[OR_MAXND][3];
     double jmp[2 * OR_MAXND], dn[2 * OR_MAXND];
     facet_point(eli[0], pts[2 * q], pts[2 * q + 1], X0);
-    facet_point(eli[1], pts[2 * q], pts[2 * q + 1], X1);
+    other_side_point(xc, X0, xc + 12, X1);
     phys_grad(K, xc, X0, phi0, g0, NULL);
     phys_grad(K, xc1, X1, phi1, g1, NULL);
     for (int i = 0; i < nd; ++i) {
@@ -653,7 +674,7 @@ static void tab_fn_int(const or_kernel* K, double* A, const double* w, const dou
   for (int q = 0; q < nq; ++q) {
     double X0[3], X1[3];
     facet_point(eli[0], pts[2 * q], pts[2 * q + 1], X0);
-    facet_point(eli[1], pts[2 * q], pts[2 * q + 1], X1);
+    other_side_point(xc, X0, xc + 12, X1);
     double va = eval_at(K->f_degree, X0, v0), vb = eval_at(K->f_degree, X1, v0 + nv);
     double ka = eval_at(K->degree, X0, k0), kb = eval_at(K->degree, X1, k0 + np);
     A[0] += wts[q] * ds * 0.5 * (va * va + vb * vb) * 0.5 * (ka + kb);
@@ -752,7 +773,7 @@ static void tab_nl_int(const or_kernel* K, double* A, const double* w, const dou
   for (int q = 0; q < nq; ++q) {
     double X0[3], X1[3], pa[OR_MAXND], pb[OR_MAXND], d[3 * OR_MAXND];
     facet_point(eli[0], pts[2 * q], pts[2 * q + 1], X0);
-    facet_point(eli[1], pts[2 * q], pts[2 * q + 1], X1);
+    other_side_point(xc, X0, xc + 12, X1);
     const double kav = 0.5 * (eval_at(K->kappa_degree, X0, k0) + eval_at(K->kappa_degree, X1, k0 + nc));
     const double ua = eval_at(K->degree, X0, v0), ub = eval_at(K->degree, X1, v0 + nd);
     tet_basis(K->degree, X0, pa, d);

@@ -32,21 +32,23 @@ def _compile(src: str, tmp_path: Path, name: str):
     return C.CDLL(str(so))
 
 
-def _call(lib, name, n, w, c, xc, eli=None, nA=None):
+def _call(lib, name, n, w, c, xc, eli=None, nA=None, qperm=None):
     fn = getattr(lib, name)
     A = np.zeros(nA)
     w = np.ascontiguousarray(w if len(w) else [0.0], dtype=np.float64)
     c = np.ascontiguousarray(c if len(c) else [0.0], dtype=np.float64)
     xc = np.ascontiguousarray(xc, dtype=np.float64)
     e = (C.c_int * 2)(*(eli or [0, 0]))
+    q = (C.c_uint8 * 2)(*qperm) if qperm is not None else None
     dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
-    fn(dp(A), dp(w), dp(c), dp(xc), e, None)
+    fn(dp(A), dp(w), dp(c), dp(xc), e, q)
     return A
 
 
 def _check_form(form, tmp_path, ncells=5, seed=0):
     rng = np.random.default_rng(seed)
     mesh = form.mesh
+    perms = mesh.facet_permutations()
     packed = form.pack_coefficients()
     consts = form.constant_values()
     for k, I in enumerate(form.integrals):
@@ -63,6 +65,7 @@ def _check_form(form, tmp_path, ncells=5, seed=0):
             xc = mesh.x[mesh.cells[c]]
             w = packed[c]
             eli = [int(ent[1])] if I.type == "exterior_facet" else None
+            qperm = None
             n, n1 = nd, nd1
             if I.type == "interior_facet":
                 c1 = ent[2]
@@ -72,12 +75,15 @@ def _check_form(form, tmp_path, ncells=5, seed=0):
                                                     packed[c1][off[j]:off[j + 1]]])
                                     for j in range(len(off) - 1)]) if len(off) > 1 else w
                 eli = [int(ent[1]), int(ent[3])]
+                qperm = [int(perms[c, ent[1]]), int(perms[c1, ent[3]])]
                 n = 2 * nd if form.rank else 1
                 n1 = 2 * nd1
             nA = n * n1 if form.rank == 2 else n
-            got = _call(lib, I.name, nd, w, consts, xc, eli, nA)
+            got = _call(lib, I.name, nd, w, consts, xc, eli, nA, qperm)
             ref = cpu.tabulate(I.spec, w, consts, xc, eli).reshape(-1)
-            scale = max(np.abs(ref).max(), 1e-300)
+            # tensors that vanish up to round-off (e.g. a jump of a continuous
+            # field) are compared on an absolute floor
+            scale = max(np.abs(ref).max(), 1e-3)
             assert np.abs(got - ref).max() <= 1e-13 * scale, (I.name, e)
 
 
@@ -120,6 +126,34 @@ def test_generated_sipg_plaplace(tmp_path, k):
     P.extras["u"].x[:] = rng.uniform(-1, 1, P.V.size)
     _check_form(P.a, tmp_path, ncells=4)
     _check_form(P.L, tmp_path, ncells=4)
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_generated_interior_facet_bodies_with_facet_permutations(tmp_path, k):
+    """Cells listing their vertices in random order (every facet permutation
+    occurs): the generated interior-facet bodies, given each side's
+    permutation, equal the oracle, which pairs the two sides through physical
+    space instead (SIPG J, F and energy; the mixed functional; the reference
+    suite's interior residual and Jacobian)."""
+    from mifem.dgforms import sipg_energy
+    from mifem.functionals import mixed_functional
+    from mifem.nonlinear import reference_test_forms
+    P = problems.dg_plaplace(2, degree=k, shuffle_vertices=11)
+    assert len(np.unique(P.mesh.facet_permutations())) == 6
+    rng = np.random.default_rng(3)
+    P.extras["u"].x[:] = rng.uniform(-1, 1, P.V.size)
+    for f in (P.a, P.L, sipg_energy(P.V, P.extras["u"])):
+        _check_form(f, tmp_path, ncells=24)
+    mesh = P.mesh
+    V, D = functionspace(mesh, ("P", k)), functionspace(mesh, ("DG", k))
+    u, kap, v = Function(V), Function(V), Function(D)
+    u.x[:] = rng.uniform(-1, 1, V.size)
+    kap.x[:] = rng.uniform(1, 2, V.size)
+    v.x[:] = rng.uniform(-1, 1, D.size)
+    _check_form(mixed_functional(u, kap, v), tmp_path, ncells=24)
+    F = reference_test_forms(mesh, k)
+    for form in F["vector"] + F["matrix"]:
+        _check_form(form, tmp_path, ncells=12)
 
 
 @pytest.mark.parametrize("kv,kw", [(1, 1), (2, 1), (1, 3)])
@@ -398,6 +432,7 @@ def test_generated_kernels_compile_for_gfx950(tmp_path):
     assert K["q3"]["mf_howner64_cell_0"][1] <= 64
     assert "mf_dgown" in K["sipg"]
     assert K["system"]["mf_dgsys"][1] == 0                      # fused J + F + E, no scratch
+    assert K["sipg"]["mf_dgown"][1] == 0
     lds = int(outs["batch"].split()[-1])
     assert {"mf_owner_batch", "mf_prep_g0", "mf_prep_g1", "mf_prep_g2"} <= set(K["batch"])
     assert K["batch"]["mf_owner_batch"][1] == 0 and K["batch"]["mf_owner_batch"][2] == lds

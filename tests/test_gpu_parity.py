@@ -1249,3 +1249,95 @@ def test_adopted_csr_missing_entry_is_rejected():
     small = sp.identity(4, format="csr")
     with pytest.raises(ValueError):
         asm.assemble_matrix(a, cufem.CUDAMatrix(get_cuda_context(), small))
+
+
+@pytest.mark.parametrize("which", ["elasticity_p2", "dg_plaplace"])
+def test_forms_from_ffcx_laid_out_source(which, tmp_path):
+    """Forms built from an FFCx-laid-out C file (cudolfinx.jit extraction,
+    integrals matched by the file's form_integrals order) assemble to the
+    oracle's A and b on the per-entity kernels, which run the extracted
+    tabulate_tensor bodies."""
+    _say(f"{which}: tabulate_tensor bodies extracted from an FFCx-laid-out .c file")
+    import cudolfinx as cufem
+    from ffcx_layout import body_of, ffcx_c_file, integral_id
+    P = getattr(problems, which)(3)
+    text = ffcx_c_file({"a": [(integral_id(I.name), body_of(I.source)) for I in P.a.integrals],
+                        "L": [(integral_id(I.name), body_of(I.source)) for I in P.L.integrals]})
+    path = tmp_path / "form_module.c"
+    path.write_text(text)
+    from cudolfinx import jit
+    wa, oa = jit.get_wrapped_tabulate_tensors(str(path), form_name="a")
+    a = cufem.CUDAForm(P.a, options={"matrix_kernel": 1, "lift_kernel": 1},
+                       ffcx_source=_OneForm(path, "a"))
+    L = cufem.CUDAForm(P.L, options={"vector_kernel": 1}, ffcx_source=_OneForm(path, "L"))
+    assert a.source().count("tabulate_tensor_integral_") >= len(P.a.integrals)
+    asm = cufem.CUDAAssembler()
+    dbc = asm.pack_bcs(P.bcs)
+    A = asm.assemble_matrix(a, bcs=dbc)
+    b = asm.assemble_vector(L)
+    asm.apply_lifting(b, [a], [dbc])
+    asm.set_bc(b, dbc, P.V)
+    asm.synchronize()
+    rp, cols, vals = A.csr()
+    _compare(P, (rp, cols, vals, b.to_host()), _oracle(P))
+
+
+class _OneForm:
+    """C text of one form of a multi-form FFCx file: the file with only that
+    form's order array (what a single-form FFCx module holds)."""
+
+    def __init__(self, path, name):
+        import re
+        text = path.read_text()
+        keep = f"form_integrals_form_{name}["
+        self.text = "\n".join(l for l in text.splitlines()
+                              if not re.match(r"\s*ufcx_integral\* form_integrals_form_", l)
+                              or keep in l)
+
+    def __str__(self):
+        return self.text
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_interior_facets_with_facet_permutations(k):
+    """Cells list their vertices in random order, so the two cells of an
+    interior facet see it in different orders (all six facet permutations
+    occur): the per-entity kernels, the cell-owner kernel and the fused
+    J + F + E launch gather each side's permutation (CUDAFormIntegral.cpp:
+    1913-1920) and match the oracle, which pairs the sides through physical
+    space; the mixed functional's interior term likewise."""
+    _say(f"SIPG DG{k} n=3 with shuffled cell vertices (non-zero facet permutations)")
+    import torch
+    import cudolfinx as cufem
+    from mifem.dgforms import sipg_energy
+    from mifem.functionals import mixed_functional
+    P = problems.dg_plaplace(3, degree=k, shuffle_vertices=7)
+    assert len(np.unique(P.mesh.facet_permutations())) == 6
+    u = P.extras["u"]
+    u.x[:] = np.random.default_rng(61).uniform(-1, 1, P.V.size)
+    ref = _oracle(P)
+    for kernel in (1, 3):
+        _compare(P, _gpu(P, {"matrix_kernel": kernel}, loptions={"vector_kernel": kernel}), ref)
+    E_ = sipg_energy(P.V, u)
+    J, F, E = cufem.form(P.a), cufem.form(P.L), cufem.form(E_)
+    asm = cufem.CUDAAssembler()
+    system = cufem.CUDASystem(J, F, E)
+    A, b = asm.create_matrix(J), asm.create_vector(F)
+    ev = torch.zeros(1, dtype=torch.float64, device="cuda")
+    system.assemble(asm, A, b, ev)
+    asm.synchronize()
+    _compare(P, (*A.csr(), b.to_host()), ref)
+    eref = cpu.OracleAssembler(E_).assemble_scalar()
+    _say(f"  fused energy rel err {abs(ev.item() - eref) / abs(eref):.2e}")
+    assert abs(ev.item() - eref) <= 1e-12 * abs(eref)
+    V, D = functionspace(P.mesh, ("P", k)), functionspace(P.mesh, ("DG", k))
+    uu, kap, v = Function(V), Function(V), Function(D)
+    rng = np.random.default_rng(62)
+    uu.x[:] = rng.uniform(-1, 1, V.size)
+    kap.x[:] = rng.uniform(1, 2, V.size)
+    v.x[:] = rng.uniform(-1, 1, D.size)
+    M = mixed_functional(uu, kap, v)
+    got = asm.assemble_scalar(cufem.form(M))
+    mref = cpu.OracleAssembler(M).assemble_scalar()
+    _say(f"  mixed functional rel err {abs(got - mref) / abs(mref):.2e}")
+    assert abs(got - mref) <= 1e-12 * abs(mref)

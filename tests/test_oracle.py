@@ -824,3 +824,47 @@ def test_jacobi_cg_matches_direct_solve():
     ref = spla.spsolve(S.tocsc(), b)
     assert res < 1e-12 and 0 < it < 2000
     assert np.linalg.norm(x.numpy() - ref) <= 1e-9 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_sipg_invariant_under_cell_vertex_order(k):
+    """The oracle's SIPG energy and residual do not depend on the order in
+    which cells list their vertices: the same physical state on the ascending
+    mesh and on a mesh with every cell's vertices shuffled (all six facet
+    permutations occur) gives the same energy and, node by node (matched by
+    cell and physical position), the same residual.  A side of an interior
+    facet evaluated at the wrong points would change both."""
+    from mifem.dgforms import sipg_energy
+    # p = 2: every integrand is a polynomial the rules integrate exactly, so
+    # moving the (non-symmetric) cell quadrature points by reordering a cell's
+    # vertices changes nothing but round-off
+    P0 = problems.dg_plaplace(2, degree=k, p=2.0)
+    P1 = problems.dg_plaplace(2, degree=k, p=2.0, shuffle_vertices=13)
+    assert len(np.unique(P1.mesh.facet_permutations())) == 6
+    # node k of cell c on the shuffled mesh <-> node match[c][k] on the ascending
+    # one (same cell, same physical position)
+    X0, X1 = P0.V.node_coordinates, P1.V.node_coordinates
+    d0, d1 = P0.V.dofmap, P1.V.dofmap
+    perm = np.empty(P0.V.size, dtype=np.int64)          # dof of mesh 1 -> dof of mesh 0
+    for c in range(d0.shape[0]):
+        a, b = X0[d0[c]], X1[d1[c]]
+        match = np.argmin(np.linalg.norm(b[:, None] - a[None], axis=2), axis=1)
+        assert np.allclose(b, a[match], atol=1e-13)
+        perm[d1[c]] = d0[c][match]
+    # a discontinuous state (random per cell and node): interior jumps are non-zero
+    u0 = np.random.default_rng(17).uniform(-1, 1, P0.V.size)
+    P0.extras["u"].x[:] = u0
+    P1.extras["u"].x[:] = u0[perm]
+    E = [cpu.OracleAssembler(sipg_energy(P.V, P.extras["u"], p=2.0)).assemble_scalar()
+         for P in (P0, P1)]
+    F0 = cpu.OracleAssembler(P0.L).assemble_vector()
+    F1 = cpu.OracleAssembler(P1.L).assemble_vector()
+    assert abs(E[0] - E[1]) <= 1e-12 * abs(E[0])
+    assert np.abs(F1 - F0[perm]).max() <= 1e-12 * np.abs(F0).max()
+    # the interior-facet part is not trivially zero for this state
+    inter = [I for I in sipg_energy(P0.V, P0.extras["u"], p=2.0).integrals
+             if I.type == "interior_facet"][0]
+    from mifem.forms import Form
+    Ei = cpu.OracleAssembler(Form(0, [], [inter], [P0.extras["u"]], [], "int",
+                                  domain=P0.mesh)).assemble_scalar()
+    assert abs(Ei) > 1e-3 * abs(E[0])

@@ -288,48 +288,70 @@ def c3_record(args, oracle: bool = True) -> dict:
     from cudolfinx.solve import pcg
     from mifem import problems
 
+    from cudolfinx.autotune import HEX_CANDIDATES
+
     t0 = time.perf_counter()
     P = problems.helmholtz_q3(args.n_hex)
     log(f"C3 problem: {P.mesh.num_cells} cells, {P.V.size} dofs ({time.perf_counter() - t0:.1f} s)")
-    a = cufem.form(P.a, options={"matrix_kernel": 2, "precision": 64})
     L = cufem.form(P.L)
     asm = cufem.CUDAAssembler()
-    A = asm.create_matrix(a)
     b = asm.create_vector(L)
     bcs = asm.pack_bcs(P.bcs)
     stream = torch.cuda.current_stream()
     lib = _lib.lib()
     b0 = bcs._get_cpp_bcs(P.V)
+    A = None
 
-    def launch_ms(reps):
+    def launch_ms(form, reps):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for _ in range(reps):
-            _lib.check(lib.mf_assemble_matrix(a.handle, A.handle, b0.handle, b0.handle, 1.0, 1,
+            _lib.check(lib.mf_assemble_matrix(form.handle, A.handle, b0.handle, b0.handle, 1.0, 1,
                                               C.c_void_p(stream.cuda_stream)))
         e1.record(stream)
         e1.synchronize()
         return e0.elapsed_time(e1) / reps
 
-    vals, times = {}, {}
+    # autotuned shape per precision (one hipRTC build per candidate holds the
+    # FP64, FP32 and BF16 kernels): chunk values, threads per workgroup, cells
+    # per record batch
+    cands = [{}] + [c for c in HEX_CANDIDATES if c != {"chunk_values": 4096}] if args.c3_tune else [{}]
+    best, tuning = {}, []
+    for cand in cands:
+        a_c = cufem.form(P.a, options={"matrix_kernel": 2, "precision": 64, **cand})
+        if A is None:
+            A = asm.create_matrix(a_c)
+        row = {"options": cand}
+        for prec in (64, 32, 16):
+            a_c.set_option("precision", prec)
+            asm.assemble_matrix(a_c, A, bcs=bcs)
+            asm.synchronize()
+            row[str(prec)] = launch_ms(a_c, 3)
+            if prec not in best or row[str(prec)] < best[prec][0]:
+                best[prec] = (row[str(prec)], cand)
+        tuning.append(row)
+        log(f"C3 shape {cand}: FP64 {row['64']:.3f} ms, FP32 {row['32']:.3f} ms, BF16 {row['16']:.3f} ms")
+    vals, times, shapes, forms_ = {}, {}, {}, {}
     for prec in (64, 32, 16):
-        a.set_option("precision", prec)
+        a = forms_[prec] = cufem.form(P.a, options={"matrix_kernel": 2, "precision": prec,
+                                                    **best[prec][1]})
         buf = torch.empty(A.nnz, dtype=torch.float64, device="cuda")
         _lib.check(lib.mf_matrix_borrow_values(A.handle, C.c_void_p(buf.data_ptr())))
         asm.assemble_matrix(a, A, bcs=bcs)
         asm.synchronize()
-        times[prec] = launch_ms(max(3, args.steps // 2))
+        times[prec] = launch_ms(a, max(3, args.steps // 2))
         vals[prec] = buf
-        log(f"C3 precision {prec}: matrix {times[prec]:.3f} ms")
+        shapes[str(prec)] = best[prec][1]
+        log(f"C3 precision {prec} (shape {best[prec][1]}): matrix {times[prec]:.3f} ms")
     # the reference's scheme in FP64: one thread per cell, full 64 x 64 tabulate_tensor,
     # FP64 atomics into the CSR (per-entity kernel)
-    a.set_option("precision", 64)
+    a = forms_[64]
     a.set_option("matrix_kernel", 1)
     gbuf = torch.empty(A.nnz, dtype=torch.float64, device="cuda")
     _lib.check(lib.mf_matrix_borrow_values(A.handle, C.c_void_p(gbuf.data_ptr())))
     asm.assemble_matrix(a, A, bcs=bcs)
     asm.synchronize()
-    generic_ms = launch_ms(2)
+    generic_ms = launch_ms(a, 2)
     generic_diff = float(torch.linalg.norm(gbuf - vals[64]) / torch.linalg.norm(vals[64]))
     del gbuf
     log(f"C3 per-entity FP64 kernel: matrix {generic_ms:.3f} ms")
@@ -351,6 +373,7 @@ def c3_record(args, oracle: bool = True) -> dict:
     rec = {"workload": f"helmholtz_q3 {args.n_hex}^3 jittered hexes, k=5, 64 Gauss points",
            "cells": P.mesh.num_cells, "dofs": P.V.size, "nnz": A.nnz,
            "matrix_ms": {str(p): times[p] for p in times}, "b_region_ms": bms,
+           "shapes": shapes, "autotune": tuning,
            "speedup_fp32_vs_fp64": times[64] / times[32],
            "speedup_bf16_vs_fp64": times[64] / times[16],
            "fp64_per_entity_matrix_ms": generic_ms,
@@ -807,8 +830,10 @@ def main():
     ap.add_argument("--c5-fp64", action="store_true",
                     help="c5: every group in FP64 (default: degree >= 2 element tensors in FP32)")
     ap.add_argument("--n-dg", type=int, default=48, help="cubes per direction for c4")
-    ap.add_argument("--newton", type=int, default=4, help="Newton steps for c4 (after the timed region)")
+    ap.add_argument("--newton", type=int, default=10, help="Newton steps for c4 (after the timed region)")
     ap.add_argument("--n-hex", type=int, default=48, help="hexahedra per direction for c3")
+    ap.add_argument("--no-c3-tune", dest="c3_tune", action="store_false",
+                    help="c3: the default shape only (no autotuning)")
     ap.add_argument("--cg-maxiter", type=int, default=4000)
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (PMC traffic calibration)")

@@ -3,8 +3,9 @@ Newton-system kernel.
 
 Picks the kernel shape -- LDS value budget per workgroup ("chunk_values",
 i.e. how many CSR rows a workgroup owns), pairs per chunk ("owner_pairs"),
-threads per workgroup ("owner_block"), an optional register cap
-("owner_wpe": waves per SIMD) -- by timing each
+threads per workgroup ("owner_block", "hex_block" for hexahedra), cells per
+record batch ("owner_nb"), an optional register cap ("owner_wpe": waves per
+SIMD), the warp-specialised variant ("owner_pipe") -- by timing each
 candidate on the actual form, matrix and boundary conditions.  Every
 candidate is a separate hipRTC build of the same form; the matrix (sparsity)
 is shared.  Results are cached in a JSON table keyed by the form description
@@ -29,7 +30,42 @@ DEFAULT_CANDIDATES = [
     {"chunk_values": 5120, "owner_pairs": 384, "owner_block": 256, "owner_lds": 81920},
     {"chunk_values": 3072, "owner_pairs": 256, "owner_block": 256},
     {"chunk_values": 6144, "owner_pairs": 512, "owner_block": 256, "owner_lds": 98304},
+    # warp-specialised variant: three compute waves and one store wave,
+    # double-buffered accumulators
+    {"chunk_values": 2304, "owner_pairs": 192, "owner_pipe": 1, "owner_cwaves": 3,
+     "owner_lds": 98304},
 ]
+
+# scalar reference-tensor forms (kappa-weighted stiffness, any degree): the
+# (nd x nd x 6) table shares LDS with the accumulator
+TRACE_CANDIDATES = [
+    {"chunk_values": 2048, "owner_pairs": 384, "owner_block": 256},
+    {"chunk_values": 1536, "owner_pairs": 384, "owner_block": 256},
+    {"chunk_values": 3072, "owner_pairs": 384, "owner_block": 256, "owner_lds": 98304},
+    {"chunk_values": 2048, "owner_pairs": 256, "owner_block": 128},
+    {"chunk_values": 2048, "owner_pairs": 384, "owner_pipe": 1, "owner_cwaves": 3,
+     "owner_lds": 98304},
+]
+
+# Q_k hexahedra (matrix-core / FP64 owner kernel): value chunk, threads per
+# workgroup and cells per record batch
+HEX_CANDIDATES = [
+    {"chunk_values": 4096},
+    {"chunk_values": 4096, "hex_block": 256},
+    {"chunk_values": 4096, "owner_nb": 8},
+    {"chunk_values": 6144, "owner_lds": 110000},
+    {"chunk_values": 3072, "owner_lds": 70000},
+]
+
+
+def candidates_for(form):
+    """The candidate shapes for `form`'s owner-computes kernel."""
+    spec = form.integrals[0].staged.source() if form.integrals and form.integrals[0].staged else ""
+    if "HX_NQ1" in spec:
+        return HEX_CANDIDATES
+    if "SK_TRACE" in spec:
+        return TRACE_CANDIDATES
+    return DEFAULT_CANDIDATES
 
 
 def _key(form, A) -> str:
@@ -43,10 +79,16 @@ def _table_path() -> Path:
     return Path(base) / "autotune.json"
 
 
-def autotune_matrix(form, asm, A, bcs, candidates=None, reps: int = 5, verbose: bool = False):
-    """Return (best options, best CUDAForm, timings[list]) for assembling `form` into A."""
+def autotune_matrix(form, asm, A, bcs, candidates=None, reps: int = 5, verbose: bool = False,
+                    base_options=None):
+    """Return (best options, best CUDAForm, timings[list]) for assembling `form` into A.
+
+    candidates: shapes to time (default: by the form's kind, candidates_for);
+    base_options: options every candidate carries (e.g. {"precision": 32}).
+    """
     table_path = _table_path()
-    key = _key(form, A)
+    key = _key(form, A) + ("" if not base_options else
+                          "|" + ",".join(f"{k}={v}" for k, v in sorted(base_options.items())))
     table = {}
     if table_path.exists():
         try:
@@ -58,8 +100,9 @@ def autotune_matrix(form, asm, A, bcs, candidates=None, reps: int = 5, verbose: 
         return opts, CUDAForm(form, options=opts), table[key].get("timings", [])
     timings = []
     best = None
-    for cand in candidates or DEFAULT_CANDIDATES:
-        opts = dict(cand)
+    for cand in candidates or candidates_for(form):
+        opts = dict(base_options or {})
+        opts.update(cand)
         opts["matrix_kernel"] = 2
         try:
             a = CUDAForm(form, options=opts)

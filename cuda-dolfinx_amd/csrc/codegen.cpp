@@ -354,6 +354,7 @@ extern const char* kDGSystem;
 extern const char* kDGSysCell;
 extern const char* kDGSysExt;
 extern const char* kDGSysIntBody;
+extern const char* kDGSysCaseIntSideP;
 
 // An owner-computes template with the shared per-unit body ($UNIT) inserted.
 static std::string owner_template(const char* t) {
@@ -394,6 +395,20 @@ std::string generate_system_source(const mf_form& a, const mf_form& L, const mf_
                                    int block, int wpe) {
   std::ostringstream os;
   os << kPrelude << "#define MF_MESH_PERMS " << (a.mesh->has_perms ? 1 : 0) << "\n";
+  // item / neighbour prefetch: a queue of the cell's first $MAXI items
+  const bool pf = a.opt("dg_prefetch", 1) != 0;
+  // items per cell: one per cell integral, and per facet of the cell one per
+  // integral of its kind (exterior or interior)
+  int n_cell = 0, n_ext = 0, n_int = 0;
+  for (auto& I : a.ints)
+    (I->type == MF_CELL ? n_cell : I->type == MF_EXTERIOR_FACET ? n_ext : n_int) += 1;
+  const int max_items = n_cell + std::max(1, a.mesh->nf) * std::max(n_ext, n_int);
+  // overwrite mode stages a block of cells' values in LDS when the segment
+  // (at most block x nd x nd x (1 + facets) values) fits in 64 KB
+  const int64_t ndb = static_cast<int64_t>(a.dm[0]->nd) * a.dm[0]->bs;
+  const int64_t segcap = static_cast<int64_t>(block) * ndb * ndb * (1 + std::max(1, a.mesh->nf));
+  const bool stage = a.opt("dg_stage", 1) != 0 && 8 * segcap + 64 <= 64 * 1024;
+  os << "#define MF_DG_PF " << (pf ? 1 : 0) << "\n#define MF_DG_STAGE " << (stage ? 1 : 0) << "\n";
   std::map<std::string, bool> seen;
   for (const mf_form* f : {&a, &L, M}) {
     if (!f) continue;
@@ -432,7 +447,8 @@ std::string generate_system_source(const mf_form& a, const mf_form& L, const mf_
                             ? "          { double e[1] = {0.0}; " + tabE +
                                   "(e, w, cE, xc, eli, qp); esum += e[0]; }\n"
                             : "";
-        std::string t = subst(kDGCaseIntSide, {{"$BODY", kDGSysIntBody}, {"$INTERLACE", interlace}});
+        std::string t = subst(pf ? kDGSysCaseIntSideP : kDGCaseIntSide,
+                              {{"$BODY", kDGSysIntBody}, {"$INTERLACE", interlace}});
         body += subst(t, {{"$SIDE", std::to_string(side)}, {"$TABJ", I->tab_name},
                           {"$TABF", L.ints[k]->tab_name}, {"$E", e}});
       }
@@ -447,6 +463,8 @@ std::string generate_system_source(const mf_form& a, const mf_form& L, const mf_
   m["$BS0"] = std::to_string(a.dm[0]->bs);
   m["$DBLOCK"] = std::to_string(block);
   m["$DATTR"] = wpe > 0 ? "__attribute__((amdgpu_waves_per_eu(" + std::to_string(wpe) + ")))" : "";
+  m["$MAXI"] = std::to_string(std::max(1, std::min(max_items, 6)));
+  m["$SEGCAP"] = std::to_string(std::max<int64_t>(2, segcap));
   os << subst(subst(kDGSystem, {{"$CASES", cases}}), m);
   return os.str();
 }
@@ -514,6 +532,7 @@ std::string generate_batch_source(const std::vector<const mf_form*>& forms,
     const OwnerShape sh = owner_shape(f, I);
     m["$ONB"] = std::to_string(sh.nb);
     m["$OMAXP"] = std::to_string(sh.maxp);
+    m["$OBAL"] = std::to_string(f.opt("owner_balance", 1) ? 1 : 0);
     m["$OCAP"] = std::to_string(f.opt("chunk_values", 4096));
     m["$OMAXR"] = std::to_string(f.opt("chunk_rows", 64));
     m["$OBLOCK"] = std::to_string(block);
@@ -732,6 +751,7 @@ std::string generate_form_source(const mf_form& f) {
           const OwnerShape sh = owner_shape(f, *I);
           m["$ONB"] = std::to_string(sh.nb);
           m["$OMAXP"] = std::to_string(sh.maxp);
+          m["$OBAL"] = std::to_string(f.opt("owner_balance", 1) ? 1 : 0);
           // optional register cap: at least owner_wpe waves per SIMD
           const int64_t wpe = f.opt("owner_wpe", 0);
           const std::string attr =

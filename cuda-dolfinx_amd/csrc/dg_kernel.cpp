@@ -255,6 +255,7 @@ const char* kDGIntVecBody = R"(
 // matrix :1103-1226, 1667-1793, 1988-2103; vector :270-522; scalar :68-266),
 // which the reference runs as separate launches per form and integral.
 const char* kDGSystem = R"(
+typedef double mf_dsd2 __attribute__((ext_vector_type(2)));
 extern "C" __global__ void __launch_bounds__($DBLOCK) $DATTR
 mf_dgsys(const int32_t ncells, const int32_t* __restrict__ cell_ptr, const int4* __restrict__ items,
          const int32_t* __restrict__ geo, const double* __restrict__ xg,
@@ -269,13 +270,24 @@ mf_dgsys(const int32_t ncells, const int32_t* __restrict__ cell_ptr, const int4*
   constexpr int NDB = $ND0 * $BS0, N2 = 2 * NDB;
   const bool zero = flags & 1;
   double esum = 0.0;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x) {
+#if MF_DG_STAGE
+  // Overwrite mode: the values of a block of $DBLOCK consecutive cells (their
+  // rows are one contiguous CSR segment, the dofs being cell-major) are
+  // assembled in LDS and written out once, in order, with 16-byte stores,
+  // instead of 8-byte stores scattered over every row of every cell.  The
+  // flush re-zeroes what it read, so the segment is zero at each block start.
+  __shared__ __attribute__((aligned(16))) double seg[$SEGCAP];
+  for (int p = threadIdx.x; p < $SEGCAP; p += $DBLOCK) seg[p] = 0.0;
+  __syncthreads();
+  int64_t segb = 0;
+#endif
+  auto cell_body = [&](const int c) {
     const int i0 = cell_ptr[c], i1 = cell_ptr[c + 1];
     double* bout = b + (int64_t)c * NDB;
     if (i0 == i1) {
       if (zero)
         for (int i = 0; i < NDB; ++i) bout[i] = 0.0;
-      continue;
+      return;
     }
     double xs[$NV * 3];
     for (int v = 0; v < $NV; ++v) {
@@ -292,21 +304,83 @@ mf_dgsys(const int32_t ncells, const int32_t* __restrict__ cell_ptr, const int4*
 #pragma unroll
     for (int k = 0; k < NDB; ++k) SV[k] = 0.0;
     int pself = 0;
+#if MF_DG_PF
+    // The cell's first MI items and their neighbours' geometry dofs are loaded
+    // up front (two waits per cell), and each item's neighbour coordinates and
+    // coefficients are loaded while the previous item computes: the per-facet
+    // chain item -> geometry dofs -> coordinates no longer waits in series,
+    // and the loads are issued before the previous item's stores (which the
+    // same memory counter tracks).
+    constexpr int MI = $MAXI;
+    int4 itq[MI];
+    int32_t gq[MI][$NV];
+#pragma unroll
+    for (int k = 0; k < MI; ++k) itq[k] = i0 + k < i1 ? items[i0 + k] : make_int4(-1, -1, 0, 0);
+#pragma unroll
+    for (int k = 0; k < MI; ++k)
+#pragma unroll
+      for (int v = 0; v < $NV; ++v) gq[k][v] = itq[k].y >= 0 ? geo[(int64_t)itq[k].y * $NV + v] : 0;
+    double nxq[$NV * 3], nwq[$W + 1];
+    auto load_nb = [&](const int4 q, const int32_t* g) {
+      if (q.y >= 0) {
+#pragma unroll
+        for (int v = 0; v < $NV; ++v) {
+          nxq[3 * v] = xg[3 * (int64_t)g[v]];
+          nxq[3 * v + 1] = xg[3 * (int64_t)g[v] + 1];
+          nxq[3 * v + 2] = xg[3 * (int64_t)g[v] + 2];
+        }
+#pragma unroll
+        for (int k = 0; k < $W; ++k) nwq[k] = coeffs[(int64_t)q.y * $W + k];
+      }
+    };
+    load_nb(itq[0], gq[0]);
+#endif
     auto put_block = [&](const double* blk, int64_t co, int pos, int stride) {
       for (int i = 0; i < NDB; ++i) {
         const int64_t row = r0 + i;
         const bool rmask = bc0 && bc0[row];
         double* out = values + rp0 + i * L + (int64_t)pos * NDB;
+#if MF_DG_STAGE
+        double* so = seg + (rp0 - segb) + i * L + (int64_t)pos * NDB;
+#endif
         for (int j = 0; j < NDB; ++j) {
           const bool m = rmask || (bc1 && bc1[co * NDB + j]);
           const double v = m ? 0.0 : blk[i * stride + j];
+#if MF_DG_STAGE
+          if (zero) so[j] = v;
+#else
           if (zero) out[j] = v;
+#endif
           else if (!m) out[j] += v;
         }
       }
     };
     for (int t = i0; t < i1; ++t) {
+#if MF_DG_PF
+      const int4 it = itq[0];
+      double nx[$NV * 3], nw[$W + 1];
+#pragma unroll
+      for (int k = 0; k < $NV * 3; ++k) nx[k] = nxq[k];
+#pragma unroll
+      for (int k = 0; k < $W; ++k) nw[k] = nwq[k];
+#pragma unroll
+      for (int k = 0; k + 1 < MI; ++k) {
+        itq[k] = itq[k + 1];
+#pragma unroll
+        for (int v = 0; v < $NV; ++v) gq[k][v] = gq[k + 1][v];
+      }
+      if (t + MI < i1) {                         // more items than the queue holds
+        itq[MI - 1] = items[t + MI];
+#pragma unroll
+        for (int v = 0; v < $NV; ++v)
+          gq[MI - 1][v] = itq[MI - 1].y >= 0 ? geo[(int64_t)itq[MI - 1].y * $NV + v] : 0;
+      } else {
+        itq[MI - 1] = make_int4(-1, -1, 0, 0);
+      }
+      if (t + 1 < i1) load_nb(itq[0], gq[0]);    // the next item's neighbour
+#else
       const int4 it = items[t];
+#endif
       pself = (it.w >> 4) & 0xFFF;
       switch (it.x) {
 $CASES
@@ -317,14 +391,26 @@ $CASES
       const int64_t row = r0 + i;
       const bool rmask = bc0 && bc0[row];
       double* out = values + rp0 + i * L + (int64_t)pself * NDB;
+#if MF_DG_STAGE
+      double* so = seg + (rp0 - segb) + i * L + (int64_t)pself * NDB;
+#endif
       for (int j = 0; j < NDB; ++j) {
         const bool m = rmask || (bc1 && bc1[r0 + j]);
         double v = m ? 0.0 : S[i * NDB + j];
         if ((flags & 2) && rmask && j == i) {
+#if MF_DG_STAGE
+          if (zero) so[j] = diag;
+          else out[j] = diag;
+#else
           out[j] = diag;
+#endif
           continue;
         }
+#if MF_DG_STAGE
+        if (zero) so[j] = v;
+#else
         if (zero) out[j] = v;
+#endif
         else if (!m) out[j] += v;
       }
     }
@@ -333,12 +419,54 @@ $CASES
     } else {
       for (int i = 0; i < NDB; ++i) bout[i] += SV[i];
     }
+  };
+#if MF_DG_STAGE
+  const int nblk = (ncells + $DBLOCK - 1) / $DBLOCK;
+  for (int cb = blockIdx.x; cb < nblk; cb += gridDim.x) {
+    const int c = cb * $DBLOCK + threadIdx.x;
+    const int cend = min(ncells, (cb + 1) * $DBLOCK);
+    segb = row_ptr[(int64_t)cb * $DBLOCK * NDB];
+    const int64_t sege = row_ptr[(int64_t)cend * NDB];
+    if (c < ncells) cell_body(c);
+    if (zero) {
+      __syncthreads();                                  // the block's values are in seg
+      const int n = (int)(sege - segb);
+      double* out = values + segb;
+      const int h = (int)(segb & 1);                    // peel to a 16-byte boundary
+      if (h && threadIdx.x == 0) {
+        __builtin_nontemporal_store(seg[0], &out[0]);
+        seg[0] = 0.0;
+      }
+      mf_dsd2* o2 = reinterpret_cast<mf_dsd2*>(out + h);
+      for (int p = threadIdx.x; p < (n - h) >> 1; p += $DBLOCK) {
+        const mf_dsd2 v = {seg[h + 2 * p], seg[h + 2 * p + 1]};
+        __builtin_nontemporal_store(v, &o2[p]);
+        seg[h + 2 * p] = 0.0;
+        seg[h + 2 * p + 1] = 0.0;
+      }
+      if (((n - h) & 1) && threadIdx.x == 0) {
+        __builtin_nontemporal_store(seg[n - 1], &out[n - 1]);
+        seg[n - 1] = 0.0;
+      }
+      __syncthreads();                                  // seg zero again
+    }
   }
+#else
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x)
+    cell_body(c);
+#endif
   if (evalue) {
     for (int o = 32; o > 0; o >>= 1) esum += __shfl_down(esum, o, 64);
+#if MF_DG_STAGE
+    // the staging segment is free here (zero, after the last barrier)
+    double* red = seg;
+    double* fin = seg + $DBLOCK / 64;
+    int& last = *reinterpret_cast<int*>(seg + 2 * ($DBLOCK / 64));
+#else
     __shared__ double red[$DBLOCK / 64];
     __shared__ double fin[$DBLOCK / 64];
     __shared__ int last;
+#endif
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = esum;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -412,6 +540,35 @@ const char* kDGSysExt = R"(
             for (int k = 0; k < NDB; ++k) SV[k] += V[k];
           }
 $E        } break;
+)";
+
+// interior facet, one side ($SIDE), of the system kernel with the neighbour's
+// coordinates (nx) and coefficients (nw) already in registers
+const char* kDGSysCaseIntSideP = R"(
+        if ((it.w & 1) == $SIDE) {
+          const int64_t co = it.y;
+          const int lfs = it.z & 0xFF, lfo = (it.z >> 8) & 0xFF;
+          double xc[2 * $NV * 3];
+#pragma unroll
+          for (int k = 0; k < $NV * 3; ++k) {
+            xc[$SIDE * $NV * 3 + k] = xs[k];
+            xc[(1 - $SIDE) * $NV * 3 + k] = nx[k];
+          }
+          const double* w0 = $SIDE == 0 ? ws : nw;
+          const double* w1 = $SIDE == 0 ? nw : ws;
+          double w[2 * $W + 1];
+$INTERLACE          const int eli[2] = {$SIDE == 0 ? lfs : lfo, $SIDE == 0 ? lfo : lfs};
+          const int64_t c0 = $SIDE == 0 ? (int64_t)c : co, c1 = $SIDE == 0 ? co : (int64_t)c;
+#if MF_MESH_PERMS
+          const uint8_t qp_[2] = {fperm ? fperm[c0 * $NF + eli[0]] : (uint8_t)0,
+                                  fperm ? fperm[c1 * $NF + eli[1]] : (uint8_t)0};
+          const uint8_t* qp = qp_;
+#else
+          const uint8_t* qp = nullptr;            // the mesh has no non-zero permutation
+          (void)c0; (void)c1;
+#endif
+          (void)w0; (void)w1;
+$BODY        }
 )";
 
 // interior facet, one side ($SIDE), inside kDGCaseIntSide

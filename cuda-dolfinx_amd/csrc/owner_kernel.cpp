@@ -256,9 +256,16 @@ $OHEAD($OPARAMS)
   // pair index of this thread's first (pair, column range) unit in a chunk
   // of np pairs (same mapping as the unit loop below), -1 if it has none
   auto mf_first_pair = [&](int np) -> int {
+#if $OBAL
+    const int items = ((np + 63) >> 6) * ND, waves = NT >> 6, wave = tid >> 6;
+    const int xa = (wave * items) / waves, xe = ((wave + 1) * items) / waves;
+    if (xa >= xe) return -1;
+    const int pr = (xa / ND) * 64 + (tid & 63);
+#else
     int js = ((NT >> 6) * 64) / max(np, 1);
     js = js < 1 ? 1 : (js > ND ? ND : js);
     const int pr = ((tid >> 6) / js) * 64 + (tid & 63);
+#endif
     return pr < np ? pr : -1;
   };
   // this thread's first unit: pair value and the record fields it needs,
@@ -354,9 +361,37 @@ $OHEAD($OPARAMS)
       if (prn >= 0) pvn = (uint32_t)pair_list[n2.x + prn];
     }
     {
+      const int np = npc;
+#if $OBAL
+      // work items (group of 64 pairs, column j): each wave takes a
+      // contiguous range of nearly equal length, cut into segments of one
+      // group and a column range.  The lanes of a wave share each segment's
+      // columns, so the column basis tables are wave-uniform, and no wave
+      // idles while another computes all columns of a group.
+      const int waves = nt >> 6, lane = tid & 63;
+      const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+      const int items = ((np + 63) >> 6) * ND;
+      const int xa = (wave * items) / waves, xe = ((wave + 1) * items) / waves;
+      bool first = true;
+      for (int x = xa; x < xe;) {
+        const int g = x / ND, jb = x - g * ND;
+        const int je = min(ND, jb + (xe - x));
+        x += je - jb;
+        const bool reload = !first;
+        first = false;
+        const int pr = g * 64 + lane;
+        if (pr >= np) continue;
+        const int pc = pr;                             // pair index within the chunk
+        if (reload) {                                  // later segments: load directly
+          u_pv = (uint32_t)spair[pc];
+          mf_load_unit(u_pv);
+        }
+        const int u_j0 = jb, u_j1 = je;
+$UNIT
+      }
+#else
       // one thread per (pair, column range); lanes of a wave share the
       // column range, so the column basis tables are wave-uniform.
-      const int np = npc;
       const int waves = nt >> 6;
       int js = (waves * 64) / max(np, 1);
       js = js < 1 ? 1 : (js > ND ? ND : js);
@@ -372,8 +407,10 @@ $OHEAD($OPARAMS)
           u_pv = (uint32_t)spair[pc];
           mf_load_unit(u_pv);
         }
+        const int u_j0 = part * jw, u_j1 = min(ND, u_j0 + jw);
 $UNIT
       }
+#endif
     }
     __syncthreads();                                   // all contributions in acc
     if (pvn != 0xFFFFFFFFu) {
@@ -812,6 +849,7 @@ $OHEAD($OPARAMS)
             u_pv = (uint32_t)spair[pc];
             mf_load_unit(u_pv);
           }
+          const int u_j0 = part * jw, u_j1 = min(ND, u_j0 + jw);
 $UNIT
         }
       }
@@ -899,7 +937,7 @@ const char* kOwnerUnit = R"(
         const int rowlen = BS * (hi - lo);
         const int rbase = BS * BS * lo;
         const uint8_t* kk = &skk[pc * ND];
-        const int j0 = part * jw, j1 = min(ND, j0 + jw);
+        const int j0 = u_j0, j1 = u_j1;
         // scalar trace forms: a few column iterations in flight at once (their
         // LDS reads of R_ij overlap instead of waiting one by one)
 #ifdef SK_TRACE

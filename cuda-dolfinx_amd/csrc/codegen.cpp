@@ -670,6 +670,12 @@ std::string generate_form_source(const mf_form& f) {
       }
     }
     m["$KNAME"] = f.rank == 2 ? "mf_dgown" : "mf_dgvown";
+    // overwrite mode stages the workgroup's values in LDS when they fit in 64 KB
+    const int64_t ndb = static_cast<int64_t>(d0->nd) * d0->bs;
+    const int64_t segcap = f.opt("dg_block", 64) * ndb * ndb * (1 + std::max(1, f.mesh->nf));
+    const bool stage = f.rank == 2 && f.opt("dg_stage", 1) != 0 && 8 * segcap + 64 <= 64 * 1024;
+    m["$SEGCAP"] = std::to_string(std::max<int64_t>(2, segcap));
+    os << "#undef MF_DG_STAGE\n#define MF_DG_STAGE " << (stage ? 1 : 0) << "\n";
     os << subst(subst(f.rank == 2 ? kDGOwnerMatrix : kDGOwnerVector, {{"$CASES", cases}}), m);
   }
   if (f.rank == 0) {
@@ -726,8 +732,9 @@ std::string generate_form_source(const mf_form& f) {
             m["$HNB"] = std::to_string(hex_batch(f, *I, prec));
             m["$HBLOCK"] = std::to_string(hex_block(f, prec));
             m["$OMAXP"] = std::to_string(f.opt("owner_pairs", 384));
-            os << "\n#define MF_HX_PREC " << prec << "\n" << subst(kHexOwnerTemplate, m)
-               << "\n#undef MF_HX_PREC\n";
+            os << "\n#define MF_HX_PREC " << prec << "\n#define MF_HX_SPLIT "
+               << (f.opt("hex_split", 1) ? 1 : 0) << "\n" << subst(kHexOwnerTemplate, m)
+               << "\n#undef MF_HX_PREC\n#undef MF_HX_SPLIT\n";
           }
         }
       } else if (I->type == MF_CELL && !I->staged_src.empty()) {

@@ -31,6 +31,7 @@
 namespace mf {
 
 const char* kDGOwnerMatrix = R"(
+typedef double mf_dod2 __attribute__((ext_vector_type(2)));
 extern "C" __global__ void __launch_bounds__($DBLOCK) $DATTR
 $KNAME(const int32_t ncells, const int32_t* __restrict__ cell_ptr, const int4* __restrict__ items,
        const int32_t* __restrict__ geo, const double* __restrict__ xg,
@@ -40,62 +41,104 @@ $KNAME(const int32_t ncells, const int32_t* __restrict__ cell_ptr, const int4* _
        const int flags, const uint8_t* __restrict__ fperm)
 {
   constexpr int NDB = $ND0 * $BS0, N2 = 2 * NDB;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
-  const int i0 = cell_ptr[c], i1 = cell_ptr[c + 1];
-  if (i0 == i1) return;
-  double xs[$NV * 3];
-  for (int v = 0; v < $NV; ++v) {
-    const int64_t gv = geo[(int64_t)c * $NV + v];
-    xs[3 * v] = xg[3 * gv]; xs[3 * v + 1] = xg[3 * gv + 1]; xs[3 * v + 2] = xg[3 * gv + 2];
-  }
-  const double* ws = coeffs + (int64_t)c * $W;
-  const int64_t r0 = (int64_t)c * NDB;
-  const int64_t rp0 = row_ptr[r0];
-  const int64_t L = row_ptr[r0 + 1] - rp0;
   const bool zero = flags & 1;
-  double S[NDB * NDB];
+#if MF_DG_STAGE
+  // overwrite mode: the workgroup's cells' rows (one contiguous CSR segment)
+  // are assembled in LDS and written out once, in order, with 16-byte stores
+  __shared__ __attribute__((aligned(16))) double seg[$SEGCAP];
+  const int cend = min(ncells, ((int)blockIdx.x + 1) * $DBLOCK);
+  const int64_t segb = row_ptr[(int64_t)blockIdx.x * $DBLOCK * NDB];
+  const int64_t sege = row_ptr[(int64_t)cend * NDB];
+  if (zero) {
+    for (int p = threadIdx.x; p < (int)(sege - segb); p += $DBLOCK) seg[p] = 0.0;
+    __syncthreads();
+  }
+#endif
+  auto cell_body = [&](const int c) {
+    const int i0 = cell_ptr[c], i1 = cell_ptr[c + 1];
+    if (i0 == i1) return;
+    double xs[$NV * 3];
+    for (int v = 0; v < $NV; ++v) {
+      const int64_t gv = geo[(int64_t)c * $NV + v];
+      xs[3 * v] = xg[3 * gv]; xs[3 * v + 1] = xg[3 * gv + 1]; xs[3 * v + 2] = xg[3 * gv + 2];
+    }
+    const double* ws = coeffs + (int64_t)c * $W;
+    const int64_t r0 = (int64_t)c * NDB;
+    const int64_t rp0 = row_ptr[r0];
+    const int64_t L = row_ptr[r0 + 1] - rp0;
+    double S[NDB * NDB];
 #pragma unroll
-  for (int k = 0; k < NDB * NDB; ++k) S[k] = 0.0;
-  int pself = 0;
-  // store one neighbour block (rows of this cell, columns of cell co)
-  auto put_block = [&](const double* blk, int64_t co, int pos, int stride) {
+    for (int k = 0; k < NDB * NDB; ++k) S[k] = 0.0;
+    int pself = 0;
+    // store one neighbour block (rows of this cell, columns of cell co)
+    auto put_block = [&](const double* blk, int64_t co, int pos, int stride) {
+      for (int i = 0; i < NDB; ++i) {
+        const int64_t row = r0 + i;
+        const bool rmask = bc0 && bc0[row];
+        double* out = values + rp0 + i * L + (int64_t)pos * NDB;
+#if MF_DG_STAGE
+        double* so = seg + (rp0 - segb) + i * L + (int64_t)pos * NDB;
+#endif
+        for (int j = 0; j < NDB; ++j) {
+          const bool m = rmask || (bc1 && bc1[co * NDB + j]);
+          const double v = m ? 0.0 : blk[i * stride + j];
+#if MF_DG_STAGE
+          if (zero) so[j] = v;
+#else
+          if (zero) out[j] = v;
+#endif
+          else if (!m) out[j] += v;
+        }
+      }
+    };
+    for (int t = i0; t < i1; ++t) {
+      const int4 it = items[t];
+      pself = (it.w >> 4) & 0xFFF;
+      switch (it.x) {
+$CASES
+        default: break;
+      }
+    }
+    // self block, with the Dirichlet diagonal
     for (int i = 0; i < NDB; ++i) {
       const int64_t row = r0 + i;
       const bool rmask = bc0 && bc0[row];
-      double* out = values + rp0 + i * L + (int64_t)pos * NDB;
+      double* out = values + rp0 + i * L + (int64_t)pself * NDB;
+#if MF_DG_STAGE
+      double* so = seg + (rp0 - segb) + i * L + (int64_t)pself * NDB;
+#endif
       for (int j = 0; j < NDB; ++j) {
-        const bool m = rmask || (bc1 && bc1[co * NDB + j]);
-        const double v = m ? 0.0 : blk[i * stride + j];
+        const bool m = rmask || (bc1 && bc1[r0 + j]);
+        double v = m ? 0.0 : S[i * NDB + j];
+        if ((flags & 2) && rmask && j == i) v = diag;
+        else if (!zero && m) continue;
+#if MF_DG_STAGE
+        if (zero) so[j] = v;
+#else
         if (zero) out[j] = v;
-        else if (!m) out[j] += v;
+#endif
+        else if ((flags & 2) && rmask && j == i) out[j] = v;
+        else out[j] += v;
       }
     }
   };
-  for (int t = i0; t < i1; ++t) {
-    const int4 it = items[t];
-    pself = (it.w >> 4) & 0xFFF;
-    switch (it.x) {
-$CASES
-      default: break;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncells) cell_body(c);
+#if MF_DG_STAGE
+  if (zero) {
+    __syncthreads();                                  // the workgroup's values are in seg
+    const int n = (int)(sege - segb);
+    double* out = values + segb;
+    const int h = (int)(segb & 1);                    // peel to a 16-byte boundary
+    if (h && threadIdx.x == 0) __builtin_nontemporal_store(seg[0], &out[0]);
+    mf_dod2* o2 = reinterpret_cast<mf_dod2*>(out + h);
+    for (int p = threadIdx.x; p < (n - h) >> 1; p += $DBLOCK) {
+      const mf_dod2 v = {seg[h + 2 * p], seg[h + 2 * p + 1]};
+      __builtin_nontemporal_store(v, &o2[p]);
     }
+    if (((n - h) & 1) && threadIdx.x == 0) __builtin_nontemporal_store(seg[n - 1], &out[n - 1]);
   }
-  // self block, with the Dirichlet diagonal
-  for (int i = 0; i < NDB; ++i) {
-    const int64_t row = r0 + i;
-    const bool rmask = bc0 && bc0[row];
-    double* out = values + rp0 + i * L + (int64_t)pself * NDB;
-    for (int j = 0; j < NDB; ++j) {
-      const bool m = rmask || (bc1 && bc1[r0 + j]);
-      double v = m ? 0.0 : S[i * NDB + j];
-      if ((flags & 2) && rmask && j == i) {
-        out[j] = diag;
-        continue;
-      }
-      if (zero) out[j] = v;
-      else if (!m) out[j] += v;
-    }
-  }
+#endif
 }
 )";
 

@@ -732,9 +732,8 @@ std::string generate_form_source(const mf_form& f) {
             m["$HNB"] = std::to_string(hex_batch(f, *I, prec));
             m["$HBLOCK"] = std::to_string(hex_block(f, prec));
             m["$OMAXP"] = std::to_string(f.opt("owner_pairs", 384));
-            os << "\n#define MF_HX_PREC " << prec << "\n#define MF_HX_SPLIT "
-               << (f.opt("hex_split", 1) ? 1 : 0) << "\n" << subst(kHexOwnerTemplate, m)
-               << "\n#undef MF_HX_PREC\n#undef MF_HX_SPLIT\n";
+            os << "\n#define MF_HX_PREC " << prec << "\n" << subst(kHexOwnerTemplate, m)
+               << "\n#undef MF_HX_PREC\n";
           }
         }
       } else if (I->type == MF_CELL && !I->staged_src.empty()) {

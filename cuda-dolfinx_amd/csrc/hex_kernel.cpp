@@ -225,17 +225,7 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
     const int np = pe - pb;
     const int ntile = (np + 15) / 16;
     const int r = lane & 15, slot = lane >> 4;      // A row / K slot of this lane
-#if MF_HX_PREC != 64 && MF_HX_SPLIT
-    // (row tile, 16-column tile) items: the four column tiles of a row tile go
-    // to different waves, so a batch with fewer row tiles than waves still
-    // keeps every wave busy (each builds the row tile's A operand itself)
-    constexpr int NJT = 1;
-    for (int u = wave; u < ntile * 4; u += nwave) {
-      const int t = u >> 2, JT0 = u & 3;
-#else
-    constexpr int NJT = 4, JT0 = 0;
     for (int t = wave; t < ntile; t += nwave) {
-#endif
       const int pr = t * 16 + r;
       const bool valid = pr < np;
       const int pc = pb + (valid ? pr : 0);
@@ -306,8 +296,8 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
       // column factors of this lane: j = 16 * tile + (lane % 16), tile = jc
       const int ja = (lane & 15) % K1, jb = (lane & 15) / K1;
       typedef float acc4 __attribute__((ext_vector_type(4)));
-      acc4 C[NJT];
-      for (int jj = 0; jj < NJT; ++jj) C[jj] = (acc4){0, 0, 0, 0};
+      acc4 C[4];
+      for (int jt = 0; jt < 4; ++jt) C[jt] = (acc4){0, 0, 0, 0};
 #if MF_HX_PREC == 16
       // K slot block of 16 = 4 quadrature points; lane holds points q0 + slot, b = 0..3
       for (int qb = 0; qb < NQ; qb += 4) {
@@ -326,15 +316,14 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
         for (int h = 0; h < 4; ++h) av[h] = valid ? mf_to_bf16(f[h]) : (short)0;
         const double cLx = sTL[(q1) * K1 + (ja)], cLy = sTL[(q2) * K1 + (jb)];
         const double cGx = sTD[(q1) * K1 + (ja)], cGy = sTD[(q2) * K1 + (jb)];
-        for (int jj = 0; jj < NJT; ++jj) {
-          const int jt = JT0 + jj;
+        for (int jt = 0; jt < 4; ++jt) {
           const double cLz = sTL[(q3) * K1 + (jt)], cGz = sTD[(q3) * K1 + (jt)];
           mf_bf4 bv;
           bv[0] = mf_to_bf16((float)(cGx * cLy * cLz));
           bv[1] = mf_to_bf16((float)(cLx * cGy * cLz));
           bv[2] = mf_to_bf16((float)(cLx * cLy * cGz));
           bv[3] = mf_to_bf16((float)(cLx * cLy * cLz));
-          C[jj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, C[jj], 0, 0, 0);
+          C[jt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, C[jt], 0, 0, 0);
         }
       }
 #else
@@ -366,10 +355,9 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
 #pragma unroll 1
       for (int q3 = 0; q3 < M1; ++q3) {
         const float Lz = sTL[(q3) * K1 + (ic)], Gz = sTD[(q3) * K1 + (ic)];
-        float cz[NJT];
+        float cz[4];
 #pragma unroll
-        for (int jj = 0; jj < NJT; ++jj)
-          cz[jj] = slot == 2 ? sTD[(q3) * K1 + (JT0 + jj)] : sTL[(q3) * K1 + (JT0 + jj)];
+        for (int jt = 0; jt < 4; ++jt) cz[jt] = slot == 2 ? sTD[(q3) * K1 + (jt)] : sTL[(q3) * K1 + (jt)];
 #pragma unroll
         for (int q2 = 0; q2 < 4; ++q2) {
           const float Ly = sTL[(q2) * K1 + (ib2)], Gy = sTD[(q2) * K1 + (ib2)];
@@ -387,8 +375,8 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
             const float fa = P[q1] * LyLz * d0[q1] + Q[q1] * (GyLz * d1[q1] + LyGz * d2[q1]);
             const float cxy = cx[q1] * cy[q2];
 #pragma unroll
-            for (int jj = 0; jj < NJT; ++jj)
-              C[jj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa, cxy * cz[jj], C[jj], 0, 0, 0);
+            for (int jt = 0; jt < 4; ++jt)
+              C[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa, cxy * cz[jt], C[jt], 0, 0, 0);
           }
         }
       }
@@ -396,8 +384,8 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
       // scatter the 16 x 64 tile into the LDS segment: all 16 positions of
       // the lane first (independent searches), then the adds
       {
-        int pos[4][NJT];
-        bool ok[4][NJT];
+        int pos[4][4];
+        bool ok[4][4];
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
           const int prr = t * 16 + slot * 4 + rr;
@@ -410,17 +398,17 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
           const int lo = lrp[lr];
           const uint16_t* pp = pair_pos + ((int64_t)p0 + pcc) * ND;   // precomputed positions
 #pragma unroll
-          for (int jj = 0; jj < NJT; ++jj) {
-            const int j = (JT0 + jj) * 16 + (lane & 15);
-            ok[rr][jj] = rowok && !((R.bits[j >> 5] >> (j & 31)) & 1u);     // Dirichlet column
-            pos[rr][jj] = lo + (int)pp[j];
+          for (int jt = 0; jt < 4; ++jt) {
+            const int j = jt * 16 + (lane & 15);
+            ok[rr][jt] = rowok && !((R.bits[j >> 5] >> (j & 31)) & 1u);     // Dirichlet column
+            pos[rr][jt] = lo + (int)pp[j];
           }
         }
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr)
 #pragma unroll
-          for (int jj = 0; jj < NJT; ++jj)
-            if (ok[rr][jj]) atomicAdd(&acc[pos[rr][jj]], (double)C[jj][rr]);
+          for (int jt = 0; jt < 4; ++jt)
+            if (ok[rr][jt]) atomicAdd(&acc[pos[rr][jt]], (double)C[jt][rr]);
       }
 #endif
     }

@@ -880,6 +880,48 @@ def test_fused_system_matches_separate_forms_and_oracle(k):
     assert _rel(A2.csr()[2], 2 * v2) <= 1e-15 and abs(ev.item() - 2 * e2) <= 1e-14 * abs(e2)
 
 
+@pytest.mark.parametrize("opts", [{"owner_balance": 0},
+                                  {"owner_balance": 1, "chunk_values": 2048, "owner_pairs": 128}])
+def test_owner_unit_mappings(opts):
+    _say(f"elasticity_p2 n=6 owner kernel {opts}")
+    P = problems.elasticity_p2(6)
+    _compare(P, _gpu(P, {"matrix_kernel": 2, **opts}), _oracle(P))
+
+
+@pytest.mark.parametrize("stage,prefetch", [(0, 0), (0, 1), (1, 0)])
+def test_dg_kernel_variants_match_default(stage, prefetch):
+    """Direct stores (dg_stage 0) and no item prefetch (dg_prefetch 0) give the
+    values of the default DG kernels (staged, prefetching): the standalone
+    matrix kernel bit for bit, the fused system's A and b bit for bit and its
+    energy to round-off (the workgroup count, which orders the energy's
+    partial sums, depends on the kernel's resources)."""
+    _say(f"SIPG DG1 n=4: dg_stage={stage} dg_prefetch={prefetch} against the defaults")
+    import torch
+    import cudolfinx as cufem
+    from mifem.dgforms import sipg_energy
+    P = problems.dg_plaplace(4)
+    u = P.extras["u"]
+    u.x[:] = np.random.default_rng(73).uniform(-1, 1, P.V.size)
+    E_ = sipg_energy(P.V, u)
+    out = []
+    for o in (None, {"dg_stage": stage, "dg_prefetch": prefetch}):
+        J, F, E = cufem.form(P.a, options=o), cufem.form(P.L, options=o), cufem.form(E_, options=o)
+        asm = cufem.CUDAAssembler()
+        A1 = asm.assemble_matrix(J)
+        system = cufem.CUDASystem(J, F, E)
+        A2, b2 = asm.create_matrix(J), asm.create_vector(F)
+        ev = torch.zeros(1, dtype=torch.float64, device="cuda")
+        system.assemble(asm, A2, b2, ev)
+        asm.synchronize()
+        out.append((A1.csr()[2].copy(), A2.csr()[2].copy(), b2.to_host().copy(), ev.item()))
+    (a0, s0, b0, e0), (a1, s1, b1, e1) = out
+    assert np.array_equal(a0, a1) and np.array_equal(s0, s1) and np.array_equal(b0, b1)
+    assert abs(e1 - e0) <= 1e-14 * abs(e0)
+    oa = cpu.OracleAssembler(P.a)
+    rp, cols, _, _ = oa.sparsity()
+    assert _rel(a1, oa.assemble_matrix(rp, cols)) <= TOL
+
+
 def test_fused_system_dirichlet_and_graph_replay():
     _say("fused J + F + E with Dirichlet rows (diagonal 2.5) and HIP graph replays, DG1 n=3")
     import torch

@@ -689,13 +689,17 @@ def bench_c5(args):
     fused = cufem.CUDABatch([a for g, a, L, A, b, bcs in sys_])
     mats = [A for g, a, L, A, b, bcs in sys_]
     gbcs = [bcs for g, a, L, A, b, bcs in sys_]
+    # every group's right-hand side (cell integrals) in one launch, the
+    # Dirichlet values of all groups in one launch
+    rhs = cufem.CUDABatch([L for g, a, L, A, b, bcs in sys_])
+    vecs = [b for g, a, L, A, b, bcs in sys_]
 
     def step():
         fused.assemble(asm, mats, gbcs)
+        rhs.assemble_vectors(asm, vecs)
         for g, a, L, A, b, bcs in sys_:
-            asm.assemble_vector(L, b)
             asm.apply_lifting(b, [a], [bcs])
-            asm.set_bc(b, bcs, g.V)
+        rhs.set_bc(asm, vecs, gbcs)
 
     step()
     asm.synchronize()

@@ -820,6 +820,40 @@ int mf_batch_create(mf_form* const* forms, int n, mf_batch** out) {
     auto B = std::make_unique<mf_batch>();
     std::vector<const mf_form*> cf;
     std::vector<const mf::Integral*> ci;
+    if (forms[0] && forms[0]->rank == 1) {
+      // right-hand sides: every form's cell integral in one vector launch
+      B->rank = 1;
+      for (int g = 0; g < n; ++g) {
+        mf_form* f = forms[g];
+        MF_CHECK(f && f->rank == 1 && f->compiled, MF_EINVAL,
+                 "vector batch members are compiled linear forms");
+        MF_CHECK(!f->restricted, MF_EUNSUPPORTED, "block member forms");
+        MF_CHECK(g == 0 || f->ctx == forms[0]->ctx, MF_EINVAL, "forms of different contexts");
+        mf::Integral* I = nullptr;
+        for (auto& J : f->ints)
+          if (J->type == MF_CELL && J->k_vown) {
+            I = J.get();
+            break;
+          }
+        MF_CHECK(I, MF_EUNSUPPORTED, "vector batch members need a cell integral on the owner vector path");
+        B->forms.push_back(f);
+        B->ints.push_back(I);
+        cf.push_back(f);
+        ci.push_back(I);
+      }
+      B->ctx = forms[0]->ctx;
+      use_device(B->ctx);
+      B->block = static_cast<int>(forms[0]->opt("vector_block", 256));
+      for (auto* f : B->forms)
+        MF_CHECK(f->opt("vector_block", 256) == B->block, MF_EINVAL, "batch members must share vector_block");
+      B->source = mf::generate_vbatch_source(cf, ci, B->block);
+      auto code = mf::rtc_compile(B->source, "mifem_vbatch", rtc_options(B->ctx), B->ctx->cache_dir, nullptr);
+      B->mod = std::make_unique<mf::Module>();
+      B->mod->load(code);
+      B->k = B->mod->get("mf_vown_batch");
+      *out = B.release();
+      return;
+    }
     for (int g = 0; g < n; ++g) {
       mf_form* f = forms[g];
       MF_CHECK(f && f->rank == 2 && f->compiled, MF_EINVAL, "batch members are compiled bilinear forms");
@@ -857,10 +891,40 @@ int mf_batch_create(mf_form* const* forms, int n, mf_batch** out) {
   });
 }
 
+int mf_batch_assemble_vector(mf_batch* B, double* const* d_b, const int64_t* n_b, int zero,
+                             mf_stream s) {
+  return guard([&] {
+    MF_CHECK(B && d_b && n_b, MF_EINVAL, "null argument");
+    MF_CHECK(B->rank == 1, MF_EINVAL, "not a batch of linear forms");
+    for (auto* f : B->forms) MF_CHECK(f->compiled, MF_ESTATE, "forms are not compiled");
+    use_device(B->ctx);
+    mf::assemble_vbatch(B, d_b, n_b, zero, S(s));
+  });
+}
+
+int mf_set_bc_batch(mf_context* ctx, int n, double* const* d_b, const int64_t* n_b,
+                    const mf_bc* const* bcs, double scale, mf_stream s) {
+  return guard([&] {
+    MF_CHECK(ctx && n >= 0 && n <= 32 && (n == 0 || (d_b && n_b && bcs)), MF_EINVAL,
+             "bad arguments (at most 32 vectors)");
+    use_device(ctx);
+    std::vector<mf::SetBcGroup> gr;
+    for (int g = 0; g < n; ++g) {
+      const mf_bc* bc = bcs[g];
+      if (!bc) continue;
+      MF_CHECK(d_b[g], MF_EINVAL, "null vector " + std::to_string(g));
+      MF_CHECK(bc->ndofs == n_b[g], MF_ESHAPE, "bc " + std::to_string(g) + " does not match its vector");
+      gr.push_back(mf::SetBcGroup{d_b[g], bc->dofs.ptr, bc->g.ptr, n_b[g], bc->n_owned});
+    }
+    mf::k_set_bc_batch(gr.data(), static_cast<int>(gr.size()), scale, S(s));
+  });
+}
+
 int mf_batch_assemble(mf_batch* B, mf_matrix* const* As, const mf_bc* const* bcs, double diagonal,
                       int zero, mf_stream s) {
   return guard([&] {
     MF_CHECK(B && As, MF_EINVAL, "null argument");
+    MF_CHECK(B->rank == 2, MF_EINVAL, "not a batch of bilinear forms");
     use_device(B->ctx);
     mf::assemble_batch(B, As, bcs, diagonal, zero, S(s));
   });

@@ -335,6 +335,7 @@ extern const char* kOwnerUnit;
 extern const char* kOwnerBatchKernel;
 extern const char* kOwnerCommon;
 extern const char* kOwnerVectorTemplate;
+extern const char* kOwnerVectorParams;
 extern const char* kStagedLiftTemplate;
 extern const char* kCellRecord;
 extern const char* kPrepTemplate;
@@ -504,6 +505,60 @@ int64_t owner_lds_bytes(const mf_form& f, const Integral& I) {
   int64_t b = 0;
   for (const auto& sl : owner_lds_slices(f, I)) b += sl.bytes;
   return b;
+}
+
+// The cell-integral vectors of several linear forms (one element group
+// each) in ONE launch: each form's owner vector body becomes a device
+// function taking its chunk index and the LDS accumulator; workgroup k runs
+// chunk k of the first form whose chunks it falls in.
+std::string generate_vbatch_source(const std::vector<const mf_form*>& forms,
+                                   const std::vector<const Integral*>& ints, int block) {
+  std::ostringstream os;
+  os << kPrelude << "#define MF_MESH_PERMS 0\n" << kOwnerCommon;
+  std::map<std::string, bool> seen;
+  std::string calls, params;
+  int64_t cap = 2;
+  for (size_t g = 0; g < forms.size(); ++g) {
+    const mf_form& f = *forms[g];
+    const Integral& I = *ints[g];
+    if (!seen.count(I.tab_name)) {
+      seen[I.tab_name] = true;
+      os << "\n" << device_tabulate_source(I) << "\n";
+    }
+    const mf_dofmap* d0 = f.dm[0];
+    std::map<std::string, std::string> m;
+    m["$NV"] = std::to_string(f.mesh->nv);
+    m["$W"] = std::to_string(f.W);
+    m["$ND0"] = std::to_string(d0->nd);
+    m["$BS0"] = std::to_string(d0->bs);
+    m["$NR"] = std::to_string(d0->nd * d0->bs);
+    m["$TAB"] = I.tab_name;
+    const int64_t vcap = f.opt("vector_chunk_values", 2048);
+    cap = std::max(cap, vcap);
+    m["$VCAP"] = std::to_string(vcap);
+    const std::string fn = "mf_vbody_g" + std::to_string(g);
+    m["$VHEAD"] = "static __device__ __forceinline__ void " + fn;
+    m["$VPARAMS"] = std::string("const int k, double* __restrict__ acc, ") + kOwnerVectorParams;
+    m["$VK"] = "";
+    os << subst(kOwnerVectorTemplate, m);
+    const std::string a = "g" + std::to_string(g) + "_";
+    std::string pl = kOwnerVectorParams, args;
+    // prefix every parameter name of the list with this group's tag
+    for (const char* nm : {"nchunks", "chunk_rows", "chunk_items", "item_cell", "item_mask", "geo",
+                           "xg", "coeffs", "consts", "dm0", "b", "flags"}) {
+      const std::string key = std::string(" ") + nm;
+      const size_t at = pl.rfind(key);
+      pl.replace(at, key.size(), " " + a + nm);
+      args += (args.empty() ? "" : ", ") + a + nm;
+    }
+    params += (params.empty() ? "" : ",\n    ") + pl;
+    calls += "  if (k < " + a + "nchunks) {\n    " + fn + "(k, mf_vacc, " + args +
+             ");\n    return;\n  }\n  k -= " + a + "nchunks;\n";
+  }
+  os << "\nextern \"C\" __global__ void __launch_bounds__(" << block << ")\nmf_vown_batch(" << params
+     << ")\n{\n  __shared__ __attribute__((aligned(16))) double mf_vacc[" << cap
+     << "];\n  int k = blockIdx.x;\n" << calls << "}\n";
+  return os.str();
 }
 
 std::string generate_batch_source(const std::vector<const mf_form*>& forms,
@@ -802,8 +857,13 @@ std::string generate_form_source(const mf_form& f) {
       m["$KNAME"] = kernel_name("vec", I->type, I->index);
       os << subst(subst(kVector, {{"$GATHER", gather}}), m);
       if (I->type == MF_CELL && d0->nd <= 32) {
-        m["$KNAME"] = kernel_name("vown", I->type, I->index);
-        m["$VBLOCK"] = std::to_string(f.opt("vector_block", 256));
+        m["$VHEAD"] = "extern \"C\" __global__ void __launch_bounds__(" +
+                      std::to_string(f.opt("vector_block", 256)) + ")\n" +
+                      kernel_name("vown", I->type, I->index);
+        m["$VPARAMS"] = kOwnerVectorParams;
+        m["$VK"] = "  __shared__ double acc[CAPV];\n"
+                   "  const int k = mf_xcd_chunk(nchunks);\n"
+                   "  if (k < 0) return;";
         m["$VCAP"] = std::to_string(f.opt("vector_chunk_values", 2048));
         os << subst(kOwnerVectorTemplate, m);
       }

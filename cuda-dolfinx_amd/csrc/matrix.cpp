@@ -678,6 +678,62 @@ void assemble_vector(mf_form* L, double* b, int64_t n, int zero, hipStream_t s) 
   }
 }
 
+// The right-hand sides of several linear forms: every form's cell integral in
+// one launch of mf_vown_batch (workgroup k runs chunk k of the form whose
+// chunks it falls in), then each form's other integrals with their own
+// kernels, accumulating, as assemble_vector does for one form.
+void assemble_vbatch(mf_batch* B, double* const* d_b, const int64_t* n_b, int zero, hipStream_t s) {
+  struct Args {   // one group's kernel arguments, in the order of kOwnerVectorParams
+    int32_t nchunks; int32_t pad; const void* chunk_rows; const void* chunk_items;
+    const void* item_cell; const void* item_mask; const void* geo; const void* xg;
+    const void* coeffs; const void* consts; const void* dm0; void* b; int32_t flags; int32_t pad2;
+  };
+  const size_t ng = B->forms.size();
+  std::vector<Args> h(ng);
+  int64_t total = 0;
+  for (size_t g = 0; g < ng; ++g) {
+    mf_form* L = B->forms[g];
+    Integral* I = B->ints[g];
+    MF_CHECK(d_b[g], MF_EINVAL, "batch vector " + std::to_string(g) + " is null");
+    MF_CHECK(n_b[g] == L->dm[0]->num_nodes * L->dm[0]->bs, MF_ESHAPE,
+             "batch vector " + std::to_string(g) + " does not match its form");
+    auto plan = vector_plan(L, *I);
+    MF_CHECK(plan != nullptr, MF_EUNSUPPORTED,
+             "owner vector schedule unavailable for group " + std::to_string(g));
+    h[g] = Args{static_cast<int32_t>(plan->nchunks), 0, plan->chunk_rows.ptr, plan->chunk_items.ptr,
+                plan->item_cell.ptr, plan->item_mask.ptr, L->mesh->cells.ptr, L->mesh->x.ptr,
+                L->packed.ptr, L->consts.ptr, L->dm[0]->dev.ptr, d_b[g], zero ? 1 : 0, 0};
+    total += plan->nchunks;
+  }
+  MF_CHECK(total < INT32_MAX, MF_EUNSUPPORTED, "too many vector chunks for one launch");
+  if (total > 0) {
+    std::vector<void*> ptrs;
+    for (size_t g = 0; g < ng; ++g) {
+      Args& x = h[g];
+      for (void* p : {static_cast<void*>(&x.nchunks), static_cast<void*>(&x.chunk_rows),
+                      static_cast<void*>(&x.chunk_items), static_cast<void*>(&x.item_cell),
+                      static_cast<void*>(&x.item_mask), static_cast<void*>(&x.geo),
+                      static_cast<void*>(&x.xg), static_cast<void*>(&x.coeffs),
+                      static_cast<void*>(&x.consts), static_cast<void*>(&x.dm0),
+                      static_cast<void*>(&x.b), static_cast<void*>(&x.flags)})
+        ptrs.push_back(p);
+    }
+    MF_HIP(hipModuleLaunchKernel(B->k, static_cast<uint32_t>(total), 1, 1,
+                                 static_cast<uint32_t>(B->block), 1, 1, 0, s, ptrs.data(), nullptr));
+  }
+  const uint32_t block = 128;
+  for (size_t g = 0; g < ng; ++g) {
+    mf_form* L = B->forms[g];
+    for (auto& I : L->ints) {
+      if (I.get() == B->ints[g] || I->n == 0) continue;
+      launch(I->k_vec, static_cast<uint32_t>((I->n + block - 1) / block), block, 0, s,
+             static_cast<int32_t>(I->n), I->d_ents.ptr, L->mesh->cells.ptr, L->mesh->x.ptr,
+             L->packed.ptr, L->consts.ptr, L->dm[0]->dev.ptr, d_b[g], L->mesh->fperm.ptr,
+             static_cast<const int32_t*>(nullptr), static_cast<int64_t>(0));
+    }
+  }
+}
+
 // Several bilinear forms (one element group each, e.g. the degree groups of
 // many independent meshes) in one persistent owner-computes launch, after
 // one per-cell record pass per group.  Each group's matrix gets exactly what

@@ -137,6 +137,14 @@ void k_cell_flags(uint8_t* flags, const int32_t* dm, int64_t ncells, int nd, con
                   hipStream_t s);
 void k_pack(double* packed, int64_t ncells, int W, int off, const double* values,
             const int32_t* dofmap, int nd, int bs, hipStream_t s);
+struct SetBcGroup {
+  double* b;
+  const int32_t* dofs;
+  const double* g;
+  int64_t n;       // vector length
+  int64_t nbc;     // constrained dofs to set
+};
+void k_set_bc_batch(const SetBcGroup* groups, int ngroups, double scale, hipStream_t s);
 void k_set_bc(double* b, int64_t n, const int32_t* dofs, int64_t nbc, const double* g,
               const double* x0, double scale, hipStream_t s);
 void k_set_diagonal(double* values, const int32_t* row_ptr, const int32_t* cols,
@@ -318,6 +326,7 @@ struct mf_system {
 
 struct mf_batch {
   mf_context* ctx = nullptr;
+  int rank = 2;                        // 2: matrices, 1: right-hand sides
   std::vector<mf_form*> forms;
   std::vector<mf::Integral*> ints;    // each form's owner-computes cell integral
   std::unique_ptr<mf::Module> mod;
@@ -387,6 +396,7 @@ void assemble_matrix(mf_form* a, mf_matrix* A, const mf_bc* bc0, const mf_bc* bc
                      double diagonal, int zero, hipStream_t s);
 void assemble_vector(mf_form* L, double* b, int64_t n, int zero, hipStream_t s);
 void assemble_scalar(mf_form* M, double* d_value, int zero, hipStream_t s);
+void assemble_vbatch(mf_batch* B, double* const* d_b, const int64_t* n_b, int zero, hipStream_t s);
 void assemble_batch(mf_batch* B, mf_matrix* const* As, const mf_bc* const* bcs, double diag,
                     int zero, hipStream_t s);
 // cell-owner schedule of a form on a cell-major discontinuous space (null if
@@ -398,6 +408,8 @@ std::string generate_batch_source(const std::vector<const mf_form*>& forms,
                                   const std::vector<const Integral*>& ints, int block,
                                   int64_t* max_lds);
 // fused Newton-system kernel source (Jacobian a, residual L, optional functional M)
+std::string generate_vbatch_source(const std::vector<const mf_form*>& forms,
+                                   const std::vector<const Integral*>& ints, int block);
 std::string generate_system_source(const mf_form& a, const mf_form& L, const mf_form* M,
                                    int block, int wpe);
 // scalar-level CSR of a block of forms (row-major nr x nc, null: empty block)

@@ -510,19 +510,23 @@ $UNIT
 }
 )";
 
+// Parameters of one form's owner vector body.
+const char* kOwnerVectorParams =
+    "const int32_t nchunks, const int32_t* __restrict__ chunk_rows, "
+    "const int32_t* __restrict__ chunk_items, const int32_t* __restrict__ item_cell, "
+    "const uint32_t* __restrict__ item_mask, const int32_t* __restrict__ geo, "
+    "const double* __restrict__ xg, const double* __restrict__ coeffs, "
+    "const double* __restrict__ consts, const int32_t* __restrict__ dm0, "
+    "double* __restrict__ b, const int flags";
+
+// $VHEAD: the kernel head (one form: its chunks in XCD-aware order, $VK) or a
+// device function of a many-form launch (chunk k and the LDS accumulator as
+// arguments, $VK empty).
 const char* kOwnerVectorTemplate = R"(
-extern "C" __global__ void __launch_bounds__($VBLOCK)
-$KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_rows,
-       const int32_t* __restrict__ chunk_items, const int32_t* __restrict__ item_cell,
-       const uint32_t* __restrict__ item_mask, const int32_t* __restrict__ geo,
-       const double* __restrict__ xg, const double* __restrict__ coeffs,
-       const double* __restrict__ consts, const int32_t* __restrict__ dm0,
-       double* __restrict__ b, const int flags)
+$VHEAD($VPARAMS)
 {
   constexpr int CAPV = $VCAP;
-  __shared__ double acc[CAPV];
-  const int k = mf_xcd_chunk(nchunks);
-  if (k < 0) return;
+$VK
   const int tid = threadIdx.x, nt = blockDim.x;
   const int32_t r0 = chunk_rows[k], r1 = chunk_rows[k + 1];
   const int len = (r1 - r0) * $BS0;

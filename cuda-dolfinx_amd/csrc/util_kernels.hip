@@ -131,6 +131,30 @@ __global__ void set_bc_kernel(double* __restrict__ b, int64_t n, const int32_t* 
   }
 }
 
+// set_bc for up to 32 vectors in one launch: the groups' constrained dofs
+// are numbered consecutively (start[g] .. start[g + 1]), every thread finds
+// its group by a scan of the (at most 32) starts
+struct SetBcBatchArgs {
+  double* b[32];
+  const int32_t* dofs[32];
+  const double* g[32];
+  int64_t n[32];
+  int64_t start[33];
+  int32_t ngroups;
+};
+
+__global__ void set_bc_batch_kernel(const SetBcBatchArgs a, double scale) {
+  const int64_t total = a.start[a.ngroups];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    int grp = 0;
+    while (grp + 1 < a.ngroups && a.start[grp + 1] <= i) ++grp;
+    const int32_t d = a.dofs[grp][i - a.start[grp]];
+    if (d < 0 || d >= a.n[grp]) continue;
+    a.b[grp][d] = scale * a.g[grp][d];
+  }
+}
+
 __device__ __forceinline__ int64_t search(const int32_t* __restrict__ cols, int64_t lo, int64_t hi,
                                           int32_t key) {
   // first position in [lo, hi) with cols[p] >= key
@@ -355,6 +379,22 @@ void k_pack(double* packed, int64_t ncells, int W, int off, const double* values
   if (ncells <= 0) return;
   pack_kernel<<<grid_for(ncells * nd * bs), kBlock, 0, s>>>(packed, ncells, W, off, values, dofmap,
                                                              nd, bs);
+  MF_HIP(hipGetLastError());
+}
+void k_set_bc_batch(const SetBcGroup* groups, int ngroups, double scale, hipStream_t s) {
+  MF_CHECK(ngroups >= 0 && ngroups <= 32, MF_EINVAL, "at most 32 vectors per set_bc batch");
+  SetBcBatchArgs a{};
+  a.ngroups = ngroups;
+  a.start[0] = 0;
+  for (int g = 0; g < ngroups; ++g) {
+    a.b[g] = groups[g].b;
+    a.dofs[g] = groups[g].dofs;
+    a.g[g] = groups[g].g;
+    a.n[g] = groups[g].n;
+    a.start[g + 1] = a.start[g] + std::max<int64_t>(0, groups[g].nbc);
+  }
+  if (ngroups == 0 || a.start[ngroups] == 0) return;
+  set_bc_batch_kernel<<<grid_for(a.start[ngroups]), kBlock, 0, s>>>(a, scale);
   MF_HIP(hipGetLastError());
 }
 void k_set_bc(double* b, int64_t n, const int32_t* dofs, int64_t nbc, const double* g,

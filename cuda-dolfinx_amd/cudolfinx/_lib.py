@@ -66,6 +66,9 @@ SIGNATURES = {
     "mf_autotune_matrix": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, _i64, _vp]),
     "mf_batch_create": (C.c_int, [_vp, C.c_int, _vp]),
     "mf_batch_assemble": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int, _vp]),
+    "mf_batch_assemble_vector": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_i64), C.c_int, _vp]),
+    "mf_set_bc_batch": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_i64), C.POINTER(_vp),
+                                  C.c_double, _vp]),
     "mf_batch_source": (C.c_int, [_vp, _vp, _vp]),
     "mf_batch_destroy": (C.c_int, [_vp]),
     "mf_system_create": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),

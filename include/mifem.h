@@ -262,6 +262,20 @@ int mf_batch_create(mf_form* const* forms, int n, mf_batch** out);
  * allocation or host synchronization once the matrices' schedules exist. */
 int mf_batch_assemble(mf_batch* batch, mf_matrix* const* As, const mf_bc* const* bcs,
                       double diagonal, int zero, mf_stream stream);
+/* Right-hand sides of such groups.  mf_batch_create with n compiled LINEAR
+ * forms (each with a cell integral on the owner vector path, nd <= 32)
+ * makes a vector batch: mf_batch_assemble_vector assembles every form's
+ * cell integral into d_b[g] (length n_b[g]) in ONE launch (overwriting when
+ * zero != 0, else adding), then each form's other integrals with their own
+ * kernels, as mf_assemble_vector does per form (fem.cpp:281-309).  The forms'
+ * coefficients must be packed.  No counterpart in the reference. */
+int mf_batch_assemble_vector(mf_batch* batch, double* const* d_b, const int64_t* n_b, int zero,
+                             mf_stream stream);
+/* mf_set_bc for n vectors in ONE launch: d_b[g][dof] = scale * g_g[dof] for
+ * the owned constrained dofs of bcs[g] (set_bc_on_device, fem.cpp:357-383,
+ * without x0).  At most 32 vectors; bcs[g] may be NULL (nothing to set). */
+int mf_set_bc_batch(mf_context* ctx, int n, double* const* d_b, const int64_t* n_b,
+                    const mf_bc* const* bcs, double scale, mf_stream stream);
 int mf_batch_source(mf_batch* batch, char* buf, int64_t* len);
 int mf_batch_destroy(mf_batch* batch);
 

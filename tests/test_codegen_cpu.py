@@ -316,6 +316,8 @@ def test_null_arguments_rejected_without_a_device():
         "mf_batch_create": lambda: L.mf_batch_create(None, 0, None),
         "mf_batch_assemble": lambda: L.mf_batch_assemble(None, None, None, 1.0, 1, None),
         "mf_batch_source": lambda: L.mf_batch_source(None, None, C.byref(n)),
+        "mf_batch_assemble_vector": lambda: L.mf_batch_assemble_vector(None, None, None, 1, None),
+        "mf_set_bc_batch": lambda: L.mf_set_bc_batch(None, 1, None, None, None, 1.0, None),
         "mf_system_create": lambda: L.mf_system_create(None, None, None, 0, None),
         "mf_form_set_option": lambda: L.mf_form_set_option(None, b"chunk_values", 1),
         "mf_matrix_info": lambda: L.mf_matrix_info(None, None, None, None),

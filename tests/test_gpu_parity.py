@@ -517,6 +517,48 @@ def test_many_instances_one_launch():
         assert np.array_equal(vals[~free], v0[~free])
 
 
+def test_many_right_hand_sides_one_launch():
+    _say("right-hand sides of 12 Poisson instances (P1-P3) + elasticity (body force and traction): "
+         "cell integrals in one launch, set_bc in one launch")
+    from mifem import batch
+    import cudolfinx as cufem
+    B = batch.many_poisson(12, seed=5, nmin=2, nmax=5)
+    E = problems.elasticity_p2(3)                      # a cell and an exterior-facet integral
+    asm = cufem.CUDAAssembler()
+    Ls = [cufem.form(g.L) for g in B.groups] + [cufem.form(E.L)]
+    spaces = [g.V for g in B.groups] + [E.V]
+    bclists = [g.bcs for g in B.groups] + [E.bcs]
+    bcs = [asm.pack_bcs(b) for b in bclists]
+    vb = cufem.CUDABatch(Ls)
+    assert "mf_vown_batch" in vb.source()
+    vecs = [asm.create_vector(L) for L in Ls]
+    for rep in range(2):
+        vb.assemble_vectors(asm, vecs)
+        vb.set_bc(asm, vecs, bcs, scale=1.0 + rep)
+        asm.synchronize()
+        for L, V, bl, v in zip(Ls, spaces, bclists, vecs):
+            ref = asm.assemble_vector(L)
+            asm.set_bc(ref, asm.pack_bcs(bl), V, scale=1.0 + rep)
+            asm.synchronize()
+            e = _rel(v.to_host(), ref.to_host())
+            _say(f"  call {rep}, {V.element.family}{V.element.degree} bs {V.bs}: vs separate {e:.2e}")
+            assert e <= 1e-14
+    # accumulate: the cell and facet parts added on top
+    before = [v.to_host().copy() for v in vecs]
+    vb.assemble_vectors(asm, vecs, zero=False)
+    asm.synchronize()
+    for L, v, v0 in zip(Ls, vecs, before):
+        ref = asm.assemble_vector(L)
+        asm.synchronize()
+        assert _rel(v.to_host(), v0 + ref.to_host()) <= 1e-14
+    # against the oracle (without the Dirichlet values)
+    vb.assemble_vectors(asm, vecs)
+    asm.synchronize()
+    mforms = [g.L for g in B.groups] + [E.L]
+    for mf_, v in zip(mforms, vecs):
+        assert _rel(v.to_host(), cpu.OracleAssembler(mf_).assemble_vector()) <= TOL
+
+
 def test_one_launch_mixed_block_sizes():
     _say("one launch for vector P2 elasticity (bs 3) and scalar P1 Poisson (bs 1)")
     import cudolfinx as cufem

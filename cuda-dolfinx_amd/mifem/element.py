@@ -9,6 +9,10 @@ Elements
 * ``DG(tetrahedron, k)``: same nodes, no inter-cell sharing.
 * ``Lagrange(hexahedron, k)``, k = 1..3, Gauss-Lobatto-Legendre (GLL) nodes
   in lexicographic tensor order (index a + (k+1) b + (k+1)^2 c).
+* ``Lagrange(triangle, k)`` / ``DG(triangle, k)``, k = 1..3, equispaced
+  nodes in the Basix order: 3 vertices, edges e0=(1,2), e1=(0,2), e2=(0,1)
+  (nodes from the first to the second vertex), then interior.  Tables carry
+  two reference derivatives.
 
 Each node also carries an *integer* position: barycentric multi-indices
 (tetrahedra, sum = k) or tensor indices (hexahedra).  The function-space
@@ -38,6 +42,9 @@ import numpy as np
 from scipy.special import roots_jacobi, roots_legendre
 
 TET_VERTICES = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=np.float64)
+TRI_VERTICES = np.array([[0, 0], [1, 0], [0, 1]], dtype=np.float64)
+# Triangle facets (edges): facet i is opposite vertex i.
+TRI_EDGES = [(1, 2), (0, 2), (0, 1)]
 TET_EDGES = [(2, 3), (1, 3), (1, 2), (0, 3), (0, 2), (0, 1)]
 TET_FACES = [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)]
 # Outward unit normals of the reference tetrahedron facets (facet i opposite vertex i).
@@ -69,28 +76,35 @@ class Element:
         k = self.degree
         if self.cell == "tetrahedron":
             return (k + 1) * (k + 2) * (k + 3) // 6
+        if self.cell == "triangle":
+            return (k + 1) * (k + 2) // 2
         return (k + 1) ** 3
+
+    @property
+    def tdim(self) -> int:
+        return 2 if self.cell == "triangle" else 3
 
     # ------------------------------------------------------------------ nodes
     @property
     def node_indices(self) -> np.ndarray:
         """Integer node positions.
 
-        tetrahedron: (nd, 4) barycentric multi-indices m (sum k);
-        node X = sum_v m_v / k * vertex_v.
+        tetrahedron / triangle: (nd, 4) / (nd, 3) barycentric multi-indices m
+        (sum k); node X = sum_v m_v / k * vertex_v.
         hexahedron: (nd, 3) tensor indices (a, b, c) in 0..k.
         """
         return _node_indices(self.cell, self.degree)
 
     @property
     def nodes(self) -> np.ndarray:
-        """(nd, 3) reference coordinates of the nodes."""
+        """(nd, tdim) reference coordinates of the nodes."""
         idx = self.node_indices
         k = self.degree
-        if self.cell == "tetrahedron":
+        if self.cell in ("tetrahedron", "triangle"):
+            verts = TET_VERTICES if self.cell == "tetrahedron" else TRI_VERTICES
             if k == 0:
-                return np.array([[0.25, 0.25, 0.25]])
-            return (idx / k) @ TET_VERTICES
+                return verts.mean(axis=0, keepdims=True)
+            return (idx / k) @ verts
         p = gll_points(k)
         return p[idx]
 
@@ -99,6 +113,8 @@ class Element:
         k = self.degree
         if self.cell == "tetrahedron":
             return [e for e in itertools.product(range(k + 1), repeat=3) if sum(e) <= k]
+        if self.cell == "triangle":
+            return [e for e in itertools.product(range(k + 1), repeat=2) if sum(e) <= k]
         return list(itertools.product(range(k + 1), repeat=3))
 
     @property
@@ -108,26 +124,34 @@ class Element:
     def tabulate(self, points: np.ndarray, nderiv: int = 1) -> np.ndarray:
         """Values and first derivatives at points.
 
-        Returns (1 + 3*nderiv, npts, nd): [phi, dphi/dX, dphi/dY, dphi/dZ].
+        Returns (1 + tdim*nderiv, npts, nd): [phi, dphi/dX, dphi/dY(, dphi/dZ)].
         """
         pts = np.atleast_2d(np.asarray(points, dtype=np.float64))
         if self.cell == "hexahedron":
             return self._tabulate_tensor(pts, nderiv)
         mons = self._monomials()
+        dim = len(mons[0])
+
+        def mono(e):
+            v = np.ones(pts.shape[0])
+            for d in range(dim):
+                v = v * pts[:, d] ** e[d]
+            return v
+
         out = []
         V = np.empty((pts.shape[0], len(mons)))
         for j, e in enumerate(mons):
-            V[:, j] = pts[:, 0] ** e[0] * pts[:, 1] ** e[1] * pts[:, 2] ** e[2]
+            V[:, j] = mono(e)
         out.append(V @ self._coeffs)
         if nderiv:
-            for d in range(3):
+            for d in range(dim):
                 D = np.zeros_like(V)
                 for j, e in enumerate(mons):
                     if e[d] == 0:
                         continue
                     ee = list(e)
                     ee[d] -= 1
-                    D[:, j] = e[d] * pts[:, 0] ** ee[0] * pts[:, 1] ** ee[1] * pts[:, 2] ** ee[2]
+                    D[:, j] = e[d] * mono(ee)
                 out.append(D @ self._coeffs)
         return np.stack(out)
 
@@ -157,6 +181,23 @@ class Element:
 
 @lru_cache(maxsize=None)
 def _node_indices(cell: str, k: int) -> np.ndarray:
+    if cell == "triangle":
+        if k == 0:
+            return np.array([[1, 1, 1]], dtype=np.int64)
+        nodes = []
+        for v in range(3):
+            m = [0, 0, 0]
+            m[v] = k
+            nodes.append(m)
+        for a, b in TRI_EDGES:
+            for s_ in range(1, k):
+                m = [0, 0, 0]
+                m[a], m[b] = k - s_, s_
+                nodes.append(m)
+        for t in range(1, k):
+            for s_ in range(1, k - t):
+                nodes.append([k - s_ - t, s_, t])
+        return np.array(nodes, dtype=np.int64)
     if cell == "hexahedron":
         return np.array([(a, b, c) for c in range(k + 1) for b in range(k + 1)
                          for a in range(k + 1)], dtype=np.int64)
@@ -191,9 +232,10 @@ def _basis_coeffs(cell: str, k: int) -> np.ndarray:
     el = Element("P", cell, k)
     X = el.nodes
     mons = el._monomials()
-    V = np.empty((X.shape[0], len(mons)))
+    V = np.ones((X.shape[0], len(mons)))
     for j, e in enumerate(mons):
-        V[:, j] = X[:, 0] ** e[0] * X[:, 1] ** e[1] * X[:, 2] ** e[2]
+        for d in range(len(e)):
+            V[:, j] *= X[:, d] ** e[d]
     return np.linalg.inv(V)
 
 
@@ -302,6 +344,42 @@ def tet_facet_points(lf: int, pts2d: np.ndarray, perm: int = 0) -> np.ndarray:
     lam = np.stack([1.0 - pts2d[:, 0] - pts2d[:, 1], pts2d[:, 0], pts2d[:, 1]], axis=1)
     r = FACET_PERMS[perm]
     return sum(lam[:, r[k]:r[k] + 1] * TET_VERTICES[f[k]][None] for k in range(3))
+
+
+@lru_cache(maxsize=None)
+def line_gauss(m: int) -> Quadrature:
+    """m-point Gauss-Legendre rule on [0, 1] (triangle facets), points (m, 1)."""
+    x, w = _gauss_legendre01(m)
+    return Quadrature(x[:, None], w)
+
+
+def tri_facet_points(lf: int, pts1d: np.ndarray, perm: int = 0) -> np.ndarray:
+    """Map edge reference points s onto facet (edge) lf of the reference triangle.
+
+    s runs from the edge's lower to its higher global vertex; perm 1 means
+    the cell lists the edge's vertices the other way round (perm 0: in
+    ascending global order), so both cells of an interior edge evaluate at
+    the same physical points."""
+    a, b = TRI_EDGES[lf]
+    s = np.asarray(pts1d, dtype=np.float64).reshape(-1)
+    if perm:
+        s = 1.0 - s
+    return (1.0 - s)[:, None] * TRI_VERTICES[a][None] + s[:, None] * TRI_VERTICES[b][None]
+
+
+def tri_facet_tables_permuted(el, pts1d: np.ndarray):
+    """Basis values (3, 2, nq, nd) and reference gradients (3, 2, nq, nd, 2) at
+    the edge points of every local edge under both orientations (the tables
+    of interior-facet bodies on triangles, indexed [lf][perm][iq])."""
+    nq = len(np.asarray(pts1d).reshape(-1))
+    phi = np.empty((3, 2, nq, el.ndofs))
+    dphi = np.empty(phi.shape + (2,))
+    for lf in range(3):
+        for q in range(2):
+            t = el.tabulate(tri_facet_points(lf, pts1d, q))
+            phi[lf, q] = t[0]
+            dphi[lf, q] = np.stack([t[1], t[2]], axis=-1)
+    return phi, dphi
 
 
 def facet_tables_permuted(el, pts2d: np.ndarray):

@@ -58,7 +58,7 @@ class FunctionSpace:
         k = el.degree
         idx = el.node_indices
         lat = mesh.lattice[mesh.cells]                          # (Nc, nv, 3)
-        if mesh.cell_type == "tetrahedron":
+        if mesh.cell_type in ("tetrahedron", "triangle"):
             keys = np.einsum("dv,cvx->cdx", idx, lat)           # (Nc, nd, 3), refined by k
             code = morton3(keys[..., 0], keys[..., 1], keys[..., 2])
         else:
@@ -82,8 +82,8 @@ class FunctionSpace:
         mesh, el = self.mesh, self.element
         X = el.nodes
         xc = mesh.x[mesh.cells]                                  # (Nc, nv, 3)
-        if mesh.cell_type == "tetrahedron":
-            lam = np.concatenate([1.0 - X.sum(axis=1, keepdims=True), X], axis=1)  # (nd, 4)
+        if mesh.cell_type in ("tetrahedron", "triangle"):
+            lam = np.concatenate([1.0 - X.sum(axis=1, keepdims=True), X], axis=1)  # (nd, nv)
             xp = np.einsum("dv,cvx->cdx", lam, xc)
         else:
             a, b, c = X[:, 0], X[:, 1], X[:, 2]
@@ -116,7 +116,7 @@ class FunctionSpace:
             cells = facets[facets[:, 1] == lf, 0]
             if cells.size == 0:
                 continue
-            if self.mesh.cell_type == "tetrahedron":
+            if self.mesh.cell_type in ("tetrahedron", "triangle"):
                 on = idx[:, lf] == 0 if k > 0 else np.zeros(len(idx), bool)
             else:
                 axis, val = {0: (2, 0), 1: (1, 0), 2: (0, 0), 3: (0, k), 4: (1, k), 5: (2, k)}[lf]

@@ -36,6 +36,8 @@ CUBE_CORNERS = np.array([(a, b, c) for c in (0, 1) for b in (0, 1) for a in (0, 
 
 # Facet -> local vertices.  Tetrahedron facet i is opposite vertex i.
 TET_FACETS = np.array([(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)], dtype=np.int64)
+# Triangle facet (edge) i is opposite vertex i.
+TRI_FACETS = np.array([(1, 2), (0, 2), (0, 1)], dtype=np.int64)
 # Basix reference hexahedron facets: z=0, y=0, x=0, x=1, y=1, z=1.
 HEX_FACETS = np.array([(0, 1, 2, 3), (0, 1, 4, 5), (0, 2, 4, 6),
                        (1, 3, 5, 7), (2, 3, 6, 7), (4, 5, 6, 7)], dtype=np.int64)
@@ -100,7 +102,11 @@ class Mesh:
 
     @property
     def facet_table(self) -> np.ndarray:
-        return TET_FACETS if self.cell_type == "tetrahedron" else HEX_FACETS
+        return {"tetrahedron": TET_FACETS, "triangle": TRI_FACETS}.get(self.cell_type, HEX_FACETS)
+
+    @property
+    def tdim(self) -> int:
+        return 2 if self.cell_type == "triangle" else 3
 
     # ------------------------------------------------------------------ facets
     @cached_property
@@ -167,6 +173,10 @@ class Mesh:
         """
         from .element import FACET_PERMS
         nf = self.facet_table.shape[0]
+        if self.cell_type == "triangle":
+            # an edge: 1 when the cell lists its two vertices in descending order
+            fv = self.cells[:, self.facet_table]                  # (Nc, 3, 2)
+            return (fv[..., 0] > fv[..., 1]).astype(np.uint8)
         if self.cell_type != "tetrahedron":
             return np.zeros((self.num_cells, nf), dtype=np.uint8)
         fv = self.cells[:, self.facet_table]                      # (Nc, 4, 3)
@@ -243,6 +253,44 @@ def create_unit_cube(n: int, cell_type: str = "tetrahedron", jitter: float = 0.0
     return Mesh(x=np.ascontiguousarray(x), cells=np.ascontiguousarray(cells.astype(np.int32)),
                 cell_type=cell_type, n=n, lattice=lat, cell_cube=cell_cube,
                 cell_sub=cell_sub, jitter=float(jitter), seed=int(seed))
+
+
+def create_unit_square(nx: int, ny: int, cell_type: str = "triangle",
+                       shuffle_vertices: int | None = None) -> Mesh:
+    """`dolfinx.mesh.create_unit_square` look-alike: nx x ny squares, each split
+    into two triangles along the diagonal from (i, j) to (i + 1, j + 1)
+    (DOLFINx's default "right" diagonal).  Squares in Morton order, vertices
+    by the Morton code of their lattice index (z = 0), cell vertices in
+    ascending global order (or shuffled with the seed shuffle_vertices, so
+    interior edges get non-zero facet permutations)."""
+    if cell_type != "triangle":
+        raise ValueError(f"unsupported cell type {cell_type!r}")
+    nx, ny = int(nx), int(ny)
+    if nx < 1 or ny < 1:
+        raise ValueError("nx, ny must be >= 1")
+    gx, gy = np.arange(nx + 1, dtype=np.int64), np.arange(ny + 1, dtype=np.int64)
+    vx, vy = np.meshgrid(gx, gy, indexing="ij")
+    lat = np.stack([vx.ravel(), vy.ravel(), np.zeros(vx.size, dtype=np.int64)], axis=1)
+    lat = lat[np.argsort(morton3(lat[:, 0], lat[:, 1], lat[:, 2]), kind="stable")]
+    lex_to_vertex = np.empty((nx + 1) * (ny + 1), dtype=np.int64)
+    lex_to_vertex[lat[:, 0] + (nx + 1) * lat[:, 1]] = np.arange(lat.shape[0])
+    x = np.stack([lat[:, 0] / nx, lat[:, 1] / ny, np.zeros(lat.shape[0])], axis=1)
+    sq = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij"), axis=-1).reshape(-1, 2)
+    sq = sq[np.argsort(morton3(sq[:, 0], sq[:, 1], np.zeros(len(sq), dtype=np.int64)), kind="stable")]
+    def vid(i, j):
+        return lex_to_vertex[i + (nx + 1) * j]
+    v0, v1 = vid(sq[:, 0], sq[:, 1]), vid(sq[:, 0] + 1, sq[:, 1])
+    v2, v3 = vid(sq[:, 0], sq[:, 1] + 1), vid(sq[:, 0] + 1, sq[:, 1] + 1)
+    cells = np.stack([np.stack([v0, v1, v3], axis=1), np.stack([v0, v2, v3], axis=1)],
+                     axis=1).reshape(-1, 3)
+    cells = np.sort(cells, axis=1)
+    if shuffle_vertices is not None:
+        keys = np.random.default_rng(shuffle_vertices).random(cells.shape)
+        cells = np.take_along_axis(cells, np.argsort(keys, axis=1), axis=1)
+    cube = np.concatenate([np.repeat(sq, 2, axis=0), np.zeros((2 * len(sq), 1), dtype=np.int64)], axis=1)
+    return Mesh(x=np.ascontiguousarray(x), cells=np.ascontiguousarray(cells.astype(np.int32)),
+                cell_type="triangle", n=max(nx, ny), lattice=lat, cell_cube=cube,
+                cell_sub=np.tile(np.arange(2, dtype=np.int64), len(sq)))
 
 
 def cell_random_uniform(mesh: Mesh, seed: int, field_id: int, lo: float, hi: float) -> np.ndarray:

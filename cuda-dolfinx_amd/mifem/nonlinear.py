@@ -1,5 +1,6 @@
 """The nonlinear residual / Jacobian forms the reference's assembly tests use
-(python/test/test_cuda_assembly.py:48-91), on affine tetrahedra, and the
+(python/test/test_cuda_assembly.py:48-91), on affine tetrahedra or triangles
+(the reference's own domain, a unit square of triangles), and the
 "diagonal part" of a bilinear form (FFCx form_compiler_options
 {"part": "diagonal"}, test_cuda_assembly.py:200-209).
 
@@ -11,7 +12,8 @@ unit normal:
   interior residual  F_i(q)  = int_dS avg(q) avg(kappa) avg(u_dg^2)      (q in DG_k)
   Jacobians          J_c = dF_c/du, J_e = dF_e/du, J_i = dF_i/du_dg
 
-Quadrature: collapsed Gauss-Jacobi rules with m points per direction,
+Quadrature: collapsed Gauss-Jacobi rules with m points per direction
+(Gauss-Legendre with m points on triangle edges),
 cells m = k + 2 (the exp term is not polynomial), exterior facets
 m = (4k + 1) // 2 (exact for the degree 4k - 1 integrand), interior facets
 m = (3 k_dg + k + 2) // 2 (degree 3 k_dg + k) -- the rules the oracle
@@ -24,90 +26,85 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import element as E
-from .dgforms import _FACET, _QPERM, _geom, _grad
+from .dgforms import _QPERM
 from .fem import Function, FunctionSpace, dirichletbc, functionspace
 from .forms import CELL, EXTERIOR_FACET, INTERIOR_FACET, Form, Integral, _SIG, _carr, _uid
-
-
-def _facet_tables(el, q):
-    """(4, nq, nd) values and (4, nq, nd, 3) reference gradients on the 4 facets."""
-    phi, dphi = [], []
-    for lf in range(4):
-        t = el.tabulate(E.tet_facet_points(lf, q.points))
-        phi.append(t[0])
-        dphi.append(np.stack([t[1], t[2], t[3]], axis=-1))
-    return np.stack(phi), np.stack(dphi)
+from .simplex import Kit
 
 
 def _cell_forms(V: FunctionSpace, u: Function, kappa: Function, rank: int):
     el, nd, k = V.element, V.element.ndofs, V.element.degree
+    kit = Kit(V.mesh.cell_type)
+    D = kit.tdim
     m = k + 2
-    q = E.tet_collapsed(m)
-    t = el.tabulate(q.points)
-    phi, dphi = t[0], np.stack([t[1], t[2], t[3]], axis=-1)
+    q = kit.cell_rule(m)
+    phi, dphi = kit.cell_tables(el, q)
     if rank == 2:
         loop = f"""    for (int i = 0; i < {nd}; ++i)
       for (int j = 0; j < {nd}; ++j)
         A[i * {nd} + j] += fw * (ek * FE_phi[iq][j] * FE_phi[iq][i]
-                                 + g[j][0] * g[i][0] + g[j][1] * g[i][1] + g[j][2] * g[i][2]);
+                                 + {kit.dot("g[j][{}]", "g[i][{}]")});
 """
     else:
         loop = f"""    for (int i = 0; i < {nd}; ++i)
-      A[i] += fw * (ek * FE_phi[iq][i] + gu[0] * g[i][0] + gu[1] * g[i][1] + gu[2] * g[i][2]);
+      A[i] += fw * (ek * FE_phi[iq][i] + {kit.dot("gu[{}]", "g[i][{}]")});
 """
-    name = f"tabulate_tensor_integral_{_uid('nl_cell', rank, k)}"
+    name = f"tabulate_tensor_integral_{_uid('nl_cell', rank, k, D)}"
     body = (_carr("FE_w", q.weights) + _carr("FE_phi", phi) + _carr("FE_dphi", dphi)
-            + _geom("", "coordinate_dofs") + f"""
+            + kit.geom("", "coordinate_dofs") + f"""
   (void)h;
   for (int iq = 0; iq < {q.weights.size}; ++iq) {{
-{_grad("", "FE_dphi[iq]", nd)}    double uq = 0.0, kq = 0.0, gu[3] = {{0.0, 0.0, 0.0}};
+{kit.grad("", "FE_dphi[iq]", nd)}    double uq = 0.0, kq = 0.0, gu[{D}] = {{0.0}};
     for (int i = 0; i < {nd}; ++i) {{
       uq += w[i] * FE_phi[iq][i];
       kq += w[{nd} + i] * FE_phi[iq][i];
-      for (int d = 0; d < 3; ++d) gu[d] += w[i] * g[i][d];
+      for (int d = 0; d < {D}; ++d) gu[d] += w[i] * g[i][d];
     }}
     const double ek = exp(uq) * kq;
     const double fw = FE_w[iq] * adet;
 {loop}  }}
 """)
-    spec = dict(kind="nl_cell", cell=0, degree=k, bs=1, rank=rank, qm=m, u_off=0, kappa_off=nd)
+    spec = dict(kind="nl_cell", cell=kit.oracle_cell, degree=k, bs=1, rank=rank, qm=m, u_off=0,
+                kappa_off=nd)
     return Integral(CELL, -1, np.arange(V.mesh.num_cells, dtype=np.int32), name,
                     _SIG.format(name=name) + "\n{\n" + body + "}\n", None, spec)
 
 
 def _ext_forms(V: FunctionSpace, rank: int):
     el, nd, k = V.element, V.element.ndofs, V.element.degree
+    kit = Kit(V.mesh.cell_type)
+    D = kit.tdim
     m = (4 * k + 1) // 2
-    q = E.tri_collapsed(m)
-    phi, dphi = _facet_tables(el, q)
+    q = kit.facet_rule(m)
+    phi, dphi = kit.facet_tables(el, q)
     if rank == 2:
         loop = f"""    for (int i = 0; i < {nd}; ++i)
       for (int j = 0; j < {nd}; ++j) {{
-        const double gjn = g[j][0] * nx + g[j][1] * ny + g[j][2] * nz;
+        const double gjn = {kit.normal_dot("g[j][{}]")};
         A[i * {nd} + j] += fw * kq * ph[i] * (ph[j] * gun + uq * gjn);
       }}
 """
     else:
         loop = f"""    for (int i = 0; i < {nd}; ++i) A[i] += fw * uq * kq * ph[i] * gun;
 """
-    name = f"tabulate_tensor_integral_{_uid('nl_ext', rank, k)}"
-    body = (_carr("FE_w", q.weights) + _carr("FE_phi", phi) + _carr("FE_dphi", dphi) + _FACET
-            + _geom("", "coordinate_dofs") + f"""
+    name = f"tabulate_tensor_integral_{_uid('nl_ext', rank, k, D)}"
+    body = (_carr("FE_w", q.weights) + _carr("FE_phi", phi) + _carr("FE_dphi", dphi) + kit.FACET
+            + kit.geom("", "coordinate_dofs") + f"""
   (void)h; (void)adet;
   for (int iq = 0; iq < {q.weights.size}; ++iq) {{
     const double* ph = FE_phi[lf0][iq];
-{_grad("", "FE_dphi[lf0][iq]", nd)}    double uq = 0.0, kq = 0.0, gu[3] = {{0.0, 0.0, 0.0}};
+{kit.grad("", "FE_dphi[lf0][iq]", nd)}    double uq = 0.0, kq = 0.0, gu[{D}] = {{0.0}};
     for (int i = 0; i < {nd}; ++i) {{
       uq += w[i] * ph[i];
       kq += w[{nd} + i] * ph[i];
-      for (int d = 0; d < 3; ++d) gu[d] += w[i] * g[i][d];
+      for (int d = 0; d < {D}; ++d) gu[d] += w[i] * g[i][d];
     }}
-    const double gun = gu[0] * nx + gu[1] * ny + gu[2] * nz;
+    const double gun = {kit.normal_dot("gu[{}]")};
     const double fw = FE_w[iq] * ds;
 {loop}  }}
 """)
-    spec = dict(kind="nl_ext", cell=0, degree=k, bs=1, rank=rank, qm=m, u_off=0, kappa_off=nd)
+    spec = dict(kind="nl_ext", cell=kit.oracle_cell, degree=k, bs=1, rank=rank, qm=m, u_off=0,
+                kappa_off=nd)
     ent = np.ascontiguousarray(V.mesh.exterior_facets.reshape(-1), dtype=np.int32)
     return Integral(EXTERIOR_FACET, -1, ent, name, _SIG.format(name=name) + "\n{\n" + body + "}\n",
                     None, spec)
@@ -117,10 +114,11 @@ def _int_forms(Vc: FunctionSpace, Vd: FunctionSpace, rank: int):
     """Interior residual / Jacobian on the DG space Vd; kappa on the continuous Vc."""
     kc, kd = Vc.element.degree, Vd.element.degree
     nc, ndg = Vc.element.ndofs, Vd.element.ndofs
+    kit = Kit(Vd.mesh.cell_type)
     m = (3 * kd + kc + 2) // 2
-    q = E.tri_collapsed(m)
-    phc = E.facet_tables_permuted(Vc.element, q.points)[0]
-    phd = E.facet_tables_permuted(Vd.element, q.points)[0]
+    q = kit.facet_rule(m)
+    phc = kit.facet_tables_permuted(Vc.element, q)[0]
+    phd = kit.facet_tables_permuted(Vd.element, q)[0]
     ko, uo = 0, 2 * nc                                 # interlaced: k0 k1 | u0 u1
     n2 = 2 * ndg
     if rank == 2:
@@ -139,9 +137,10 @@ def _int_forms(Vc: FunctionSpace, Vd: FunctionSpace, rank: int):
       A[I] += fw * pI * kav * uav;
     }}
 """
-    name = f"tabulate_tensor_integral_{_uid('nl_int', rank, kc, kd)}"
-    body = (_carr("FE_w", q.weights) + _carr("FE_phi", phc) + _carr("FE_psi", phd) + _FACET
+    name = f"tabulate_tensor_integral_{_uid('nl_int', rank, kc, kd, kit.tdim)}"
+    body = (_carr("FE_w", q.weights) + _carr("FE_phi", phc) + _carr("FE_psi", phd) + kit.FACET
             + "  const int lf1 = entity_local_index[1];\n" + _QPERM + f"""
+  (void)nx; (void)ny;
   for (int iq = 0; iq < {q.weights.size}; ++iq) {{
     double ka = 0.0, kb = 0.0, ua = 0.0, ub = 0.0;
     for (int i = 0; i < {nc}; ++i) {{
@@ -156,7 +155,7 @@ def _int_forms(Vc: FunctionSpace, Vd: FunctionSpace, rank: int):
     const double fw = FE_w[iq] * ds;
 {loop}  }}
 """)
-    spec = dict(kind="nl_int", cell=0, degree=kd, bs=1, rank=rank, qm=m, kappa_off=0,
+    spec = dict(kind="nl_int", cell=kit.oracle_cell, degree=kd, bs=1, rank=rank, qm=m, kappa_off=0,
                 v_off=nc, v_degree=kd, kappa_degree=kc)
     ent = np.ascontiguousarray(Vd.mesh.interior_facets.reshape(-1), dtype=np.int32)
     return Integral(INTERIOR_FACET, -1, ent, name, _SIG.format(name=name) + "\n{\n" + body + "}\n",
@@ -164,8 +163,9 @@ def _int_forms(Vc: FunctionSpace, Vd: FunctionSpace, rank: int):
 
 
 def reference_test_forms(mesh, k: int = 1):
-    """The reference test's forms (test_cuda_assembly.py:48-91) on a tet mesh:
-    a dict with "coeff", "u", "u_dg", "bcs", "scalar", "vector", "matrix"."""
+    """The reference test's forms (test_cuda_assembly.py:48-91) on a tetrahedral
+    or triangular mesh: a dict with "coeff", "u", "u_dg", "bcs", "scalar",
+    "vector", "matrix"."""
     from .functionals import mixed_functional
     V = functionspace(mesh, ("P", k))
     Vd = functionspace(mesh, ("DG", k))

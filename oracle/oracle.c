@@ -164,10 +164,76 @@ static const int TET_F[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
 static const double TET_V[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
 static const double DLAM[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
 
+/* cell types: 0 tetrahedron, 1 hexahedron, 2 triangle (facet i opposite vertex i) */
+static const int TRI_E[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+static const double TRI_V[3][2] = {{0, 0}, {1, 0}, {0, 1}};
+static const double DLAM2[3][2] = {{-1, -1}, {1, 0}, {0, 1}};
+
 int or_ndofs(int cell, int k)
 {
   if (cell == 0) return (k + 1) * (k + 2) * (k + 3) / 6;
+  if (cell == 2) return (k + 1) * (k + 2) / 2;
   return (k + 1) * (k + 1) * (k + 1);
+}
+
+/* P_k on the reference triangle (Basix node order: vertices, edges (1,2),
+ * (0,2), (0,1) from the first to the second vertex, interior): values
+ * phi[nd], gradients dphi[nd][3] (third component 0). */
+static void tri_basis(int k, const double* X, double* phi, double* dphi)
+{
+  double l[3] = {1.0 - X[0] - X[1], X[0], X[1]};
+  int n = 0;
+#define SETG2(idx, c0, v0, c1, v1)                                                     \
+  do {                                                                                 \
+    for (int d = 0; d < 2; ++d) dphi[3 * (idx) + d] = (c0) * DLAM2[v0][d] + (c1) * DLAM2[v1][d]; \
+    dphi[3 * (idx) + 2] = 0.0;                                                         \
+  } while (0)
+  if (k == 0) {
+    phi[0] = 1.0;
+    dphi[0] = dphi[1] = dphi[2] = 0.0;
+    return;
+  }
+  if (k == 1) {
+    for (int v = 0; v < 3; ++v) {
+      phi[v] = l[v];
+      SETG2(v, 1.0, v, 0.0, v);
+    }
+    return;
+  }
+  if (k == 2) {
+    for (int v = 0; v < 3; ++v, ++n) {
+      phi[n] = l[v] * (2.0 * l[v] - 1.0);
+      SETG2(n, 4.0 * l[v] - 1.0, v, 0.0, v);
+    }
+    for (int e = 0; e < 3; ++e, ++n) {
+      int a = TRI_E[e][0], b = TRI_E[e][1];
+      phi[n] = 4.0 * l[a] * l[b];
+      SETG2(n, 4.0 * l[b], a, 4.0 * l[a], b);
+    }
+    return;
+  }
+  /* k == 3, equispaced */
+  for (int v = 0; v < 3; ++v, ++n) {
+    double L = l[v];
+    phi[n] = 0.5 * L * (3.0 * L - 1.0) * (3.0 * L - 2.0);
+    SETG2(n, 0.5 * (27.0 * L * L - 18.0 * L + 2.0), v, 0.0, v);
+  }
+  for (int e = 0; e < 3; ++e) {
+    int a = TRI_E[e][0], b = TRI_E[e][1];
+    double la = l[a], lb = l[b];
+    phi[n] = 4.5 * la * lb * (3.0 * la - 1.0);
+    SETG2(n, 4.5 * lb * (6.0 * la - 1.0), a, 4.5 * la * (3.0 * la - 1.0), b);
+    ++n;
+    phi[n] = 4.5 * la * lb * (3.0 * lb - 1.0);
+    SETG2(n, 4.5 * lb * (3.0 * lb - 1.0), a, 4.5 * la * (6.0 * lb - 1.0), b);
+    ++n;
+  }
+  phi[n] = 27.0 * l[0] * l[1] * l[2];
+  for (int d = 0; d < 2; ++d)
+    dphi[3 * n + d] = 27.0 * (l[1] * l[2] * DLAM2[0][d] + l[0] * l[2] * DLAM2[1][d] +
+                              l[0] * l[1] * DLAM2[2][d]);
+  dphi[3 * n + 2] = 0.0;
+#undef SETG2
 }
 
 /* P_k on the reference tetrahedron: values phi[nd], gradients dphi[nd][3]. */
@@ -275,12 +341,15 @@ static void hex_basis(int k, const double* X, double* phi, double* dphi)
 void or_basis(int cell, int k, const double* X, double* phi, double* dphi)
 {
   if (cell == 0) tet_basis(k, X, phi, dphi);
+  else if (cell == 2) tri_basis(k, X, phi, dphi);
   else hex_basis(k, X, phi, dphi);
 }
 
 /* ------------------------------------------------------------------ */
 /* geometry                                                            */
 /* ------------------------------------------------------------------ */
+
+static void facet_point(int lf, double s, double t, double* X);
 
 /* Jacobian J[i][j] = dx_i/dX_j at X; returns det J, fills K = J^{-1}. */
 static double jacobian(int cell, const double* xc, const double* X, double K[3][3])
@@ -289,6 +358,12 @@ static double jacobian(int cell, const double* xc, const double* X, double K[3][
   if (cell == 0) {
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) J[i][j] = xc[3 * (j + 1) + i] - xc[i];
+  } else if (cell == 2) {
+    /* the 2 x 2 map in the (x, y) plane, with J[2][2] = 1: det J is the 2D
+     * determinant and K = J^{-1} keeps the third direction unchanged */
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) J[i][j] = xc[3 * (j + 1) + i] - xc[i];
+    J[2][2] = 1.0;
   } else {
     double p[8], dp[24];
     hex_basis(1, X, p, dp);
@@ -461,9 +536,22 @@ static double diameter(const double* xc)
   return h;
 }
 
-/* 2 x area of facet lf and its unit normal pointing away from the opposite vertex. */
-static double facet_normal(const double* xc, int lf, double* n)
+/* Size of facet lf (twice the triangle's area on a tetrahedron, the edge's
+ * length on a triangle) and its unit normal pointing away from the opposite
+ * vertex. */
+static double facet_normal_c(int cell, const double* xc, int lf, double* n)
 {
+  if (cell == 2) {
+    const int a = TRI_E[lf][0], b = TRI_E[lf][1];
+    const double tx = xc[3 * b] - xc[3 * a], ty = xc[3 * b + 1] - xc[3 * a + 1];
+    const double ox = xc[3 * lf] - xc[3 * a], oy = xc[3 * lf + 1] - xc[3 * a + 1];
+    const double ds = sqrt(tx * tx + ty * ty);
+    double sgn = (ty * ox - tx * oy) > 0.0 ? -1.0 : 1.0;
+    n[0] = sgn * ty / ds;
+    n[1] = -sgn * tx / ds;
+    n[2] = 0.0;
+    return ds;
+  }
   const int* f = TET_F[lf];
   double e1[3], e2[3], o[3];
   for (int d = 0; d < 3; ++d) {
@@ -478,6 +566,21 @@ static double facet_normal(const double* xc, int lf, double* n)
   double sgn = (n[0] * o[0] + n[1] * o[1] + n[2] * o[2]) > 0.0 ? -1.0 : 1.0;
   for (int d = 0; d < 3; ++d) n[d] *= sgn / ds;
   return ds;
+}
+
+static double facet_normal(const double* xc, int lf, double* n) { return facet_normal_c(0, xc, lf, n); }
+
+/* reference point of facet lf at facet coordinates (s, t) (t unused on a
+ * triangle's edge, which runs from its first to its second vertex) */
+static void facet_point_c(int cell, int lf, double s, double t, double* X)
+{
+  if (cell == 2) {
+    const int a = TRI_E[lf][0], b = TRI_E[lf][1];
+    for (int d = 0; d < 2; ++d) X[d] = TRI_V[a][d] + s * (TRI_V[b][d] - TRI_V[a][d]);
+    X[2] = 0.0;
+    return;
+  }
+  facet_point(lf, s, t, X);
 }
 
 /* reference point of facet lf at triangle coordinates (s, t) */
@@ -495,18 +598,26 @@ static void facet_point(int lf, double s, double t, double* X)
  * the facet's vertices in (the engine's bodies pair them through facet
  * quadrature permutations instead; CUDAFormIntegral.cpp:1913-1920 gathers
  * those per facet). */
-static void other_side_point(const double* xc0, const double* X0, const double* xc1, double* X1)
+static void other_side_point_c(int cell, const double* xc0, const double* X0, const double* xc1,
+                               double* X1)
 {
+  const int dim = cell == 2 ? 2 : 3;
   double x[3], Kinv[3][3];
   for (int i = 0; i < 3; ++i) {
     x[i] = xc0[i];
-    for (int j = 0; j < 3; ++j) x[i] += (xc0[3 * (j + 1) + i] - xc0[i]) * X0[j];
+    for (int j = 0; j < dim; ++j) x[i] += (xc0[3 * (j + 1) + i] - xc0[i]) * X0[j];
   }
-  jacobian(0, xc1, X0, Kinv);
+  jacobian(cell, xc1, X0, Kinv);
   for (int r = 0; r < 3; ++r) {
     X1[r] = 0.0;
     for (int c = 0; c < 3; ++c) X1[r] += Kinv[r][c] * (x[c] - xc1[c]);
   }
+  if (dim == 2) X1[2] = 0.0;
+}
+
+static void other_side_point(const double* xc0, const double* X0, const double* xc1, double* X1)
+{
+  other_side_point_c(0, xc0, X0, xc1, X1);
 }
 
 /* physical basis gradients of a P_k tetrahedron at X */
@@ -514,10 +625,10 @@ static void phys_grad(const or_kernel* K, const double* xc, const double* X, dou
                       double g[][3], double* adet)
 {
   double dref[3 * OR_MAXND], Kinv[3][3];
-  double det = jacobian(0, xc, X, Kinv);
+  double det = jacobian(K->cell, xc, X, Kinv);
   if (adet) *adet = fabs(det);
-  tet_basis(K->degree, X, phi, dref);
-  int nd = or_ndofs(0, K->degree);
+  or_basis(K->cell, K->degree, X, phi, dref);
+  int nd = or_ndofs(K->cell, K->degree);
   for (int i = 0; i < nd; ++i)
     for (int d = 0; d < 3; ++d)
       g[i][d] = dref[3 * i] * Kinv[0][d] + dref[3 * i + 1] * Kinv[1][d] + dref[3 * i + 2] * Kinv[2][d];
@@ -628,36 +739,65 @@ static void tab_dg_int(const or_kernel* K, double* A, const double* w, const dou
 /* (offsets u_off, kappa_off), v in DG_f_degree (offset f_off)          */
 /* ------------------------------------------------------------------ */
 
-static double eval_at(int k, const double* X, const double* coef)
+static double eval_at_c(int cell, int k, const double* X, const double* coef)
 {
   double phi[OR_MAXND], dref[3 * OR_MAXND];
-  tet_basis(k, X, phi, dref);
+  or_basis(cell, k, X, phi, dref);
   double v = 0.0;
-  for (int i = 0; i < or_ndofs(0, k); ++i) v += coef[i] * phi[i];
+  for (int i = 0; i < or_ndofs(cell, k); ++i) v += coef[i] * phi[i];
   return v;
 }
+
+/* cell rule (points at stride 3) and facet rule (points (s, t) at stride 2)
+ * of a simplex cell: tetrahedron, or triangle with Gauss-Legendre on edges */
+static int cell_rule(int cell, int m, double* pts, double* wts)
+{
+  if (cell != 2) return or_tet_rule(m, pts, wts);
+  double p2[2 * OR_MAXQ];
+  const int n = or_tri_rule(m, p2, wts);
+  for (int q = n - 1; q >= 0; --q) {
+    pts[3 * q] = p2[2 * q];
+    pts[3 * q + 1] = p2[2 * q + 1];
+    pts[3 * q + 2] = 0.0;
+  }
+  return n;
+}
+
+static int facet_rule(int cell, int m, double* pts, double* wts)
+{
+  if (cell != 2) return or_tri_rule(m, pts, wts);
+  double t[OR_MAXND];
+  or_gauss_jacobi(m, 0.0, t, wts);
+  for (int q = 0; q < m; ++q) {
+    pts[2 * q] = t[q];
+    pts[2 * q + 1] = 0.0;
+  }
+  return m;
+}
+
+static int cell_nv(int cell) { return cell == 2 ? 3 : (cell == 0 ? 4 : 8); }
 
 static void tab_fn_cell(const or_kernel* K, double* A, const double* w, const double* xc)
 {
   static _Thread_local double pts[3 * OR_MAXQ], wts[OR_MAXQ];
-  const int nq = or_tet_rule(K->qm, pts, wts);
+  const int nq = cell_rule(K->cell, K->qm, pts, wts);
   double Kinv[3][3];
   for (int q = 0; q < nq; ++q) {
     const double* X = &pts[3 * q];
-    double dx = wts[q] * fabs(jacobian(0, xc, X, Kinv));
-    A[0] += dx * exp(eval_at(K->degree, X, w + K->u_off)) * eval_at(K->degree, X, w + K->kappa_off);
+    double dx = wts[q] * fabs(jacobian(K->cell, xc, X, Kinv));
+    A[0] += dx * exp(eval_at_c(K->cell, K->degree, X, w + K->u_off)) * eval_at_c(K->cell, K->degree, X, w + K->kappa_off);
   }
 }
 
 static void tab_fn_ext(const or_kernel* K, double* A, const double* w, const double* xc, int lf)
 {
   double pts[2 * OR_MAXND * OR_MAXND], wts[OR_MAXND * OR_MAXND], nrm[3];
-  const int nq = or_tri_rule(K->qm, pts, wts);
-  double ds = facet_normal(xc, lf, nrm);
+  const int nq = facet_rule(K->cell, K->qm, pts, wts);
+  double ds = facet_normal_c(K->cell, xc, lf, nrm);
   for (int q = 0; q < nq; ++q) {
     double X[3];
-    facet_point(lf, pts[2 * q], pts[2 * q + 1], X);
-    A[0] += wts[q] * ds * eval_at(K->degree, X, w + K->u_off) * eval_at(K->degree, X, w + K->kappa_off);
+    facet_point_c(K->cell, lf, pts[2 * q], pts[2 * q + 1], X);
+    A[0] += wts[q] * ds * eval_at_c(K->cell, K->degree, X, w + K->u_off) * eval_at_c(K->cell, K->degree, X, w + K->kappa_off);
   }
 }
 
@@ -666,17 +806,17 @@ static void tab_fn_int(const or_kernel* K, double* A, const double* w, const dou
                        const int* eli)
 {
   double pts[2 * OR_MAXND * OR_MAXND], wts[OR_MAXND * OR_MAXND], nrm[3];
-  const int nq = or_tri_rule(K->qm, pts, wts);
-  const int np = or_ndofs(0, K->degree), nv = or_ndofs(0, K->f_degree);
-  double ds = facet_normal(xc, eli[0], nrm);
+  const int nq = facet_rule(K->cell, K->qm, pts, wts);
+  const int np = or_ndofs(K->cell, K->degree), nv = or_ndofs(K->cell, K->f_degree);
+  double ds = facet_normal_c(K->cell, xc, eli[0], nrm);
   const double* k0 = w + 2 * K->kappa_off;
   const double* v0 = w + 2 * K->f_off;
   for (int q = 0; q < nq; ++q) {
     double X0[3], X1[3];
-    facet_point(eli[0], pts[2 * q], pts[2 * q + 1], X0);
-    other_side_point(xc, X0, xc + 12, X1);
-    double va = eval_at(K->f_degree, X0, v0), vb = eval_at(K->f_degree, X1, v0 + nv);
-    double ka = eval_at(K->degree, X0, k0), kb = eval_at(K->degree, X1, k0 + np);
+    facet_point_c(K->cell, eli[0], pts[2 * q], pts[2 * q + 1], X0);
+    other_side_point_c(K->cell, xc, X0, xc + 3 * cell_nv(K->cell), X1);
+    double va = eval_at_c(K->cell, K->f_degree, X0, v0), vb = eval_at_c(K->cell, K->f_degree, X1, v0 + nv);
+    double ka = eval_at_c(K->cell, K->degree, X0, k0), kb = eval_at_c(K->cell, K->degree, X1, k0 + np);
     A[0] += wts[q] * ds * 0.5 * (va * va + vb * vb) * 0.5 * (ka + kb);
   }
 }
@@ -708,8 +848,8 @@ static void tab_mass(const or_kernel* K, double* A, const double* w, const doubl
 static void tab_nl_cell(const or_kernel* K, double* A, const double* w, const double* xc)
 {
   static _Thread_local double pts[3 * OR_MAXQ], wts[OR_MAXQ];
-  const int nd = or_ndofs(0, K->degree);
-  const int nq = or_tet_rule(K->qm, pts, wts);
+  const int nd = or_ndofs(K->cell, K->degree);
+  const int nq = cell_rule(K->cell, K->qm, pts, wts);
   double phi[OR_MAXND], g[OR_MAXND][3], adet;
   for (int q = 0; q < nq; ++q) {
     phys_grad(K, xc, &pts[3 * q], phi, g, &adet);
@@ -735,12 +875,12 @@ static void tab_nl_cell(const or_kernel* K, double* A, const double* w, const do
 static void tab_nl_ext(const or_kernel* K, double* A, const double* w, const double* xc, int lf)
 {
   double pts[2 * OR_MAXND * OR_MAXND], wts[OR_MAXND * OR_MAXND], n[3];
-  const int nd = or_ndofs(0, K->degree);
-  const int nq = or_tri_rule(K->qm, pts, wts);
-  const double ds = facet_normal(xc, lf, n);
+  const int nd = or_ndofs(K->cell, K->degree);
+  const int nq = facet_rule(K->cell, K->qm, pts, wts);
+  const double ds = facet_normal_c(K->cell, xc, lf, n);
   for (int q = 0; q < nq; ++q) {
     double X[3], phi[OR_MAXND], g[OR_MAXND][3];
-    facet_point(lf, pts[2 * q], pts[2 * q + 1], X);
+    facet_point_c(K->cell, lf, pts[2 * q], pts[2 * q + 1], X);
     phys_grad(K, xc, X, phi, g, NULL);
     double uq = 0.0, kq = 0.0, gun = 0.0;
     for (int i = 0; i < nd; ++i) {
@@ -764,20 +904,20 @@ static void tab_nl_int(const or_kernel* K, double* A, const double* w, const dou
                        const int* eli)
 {
   double pts[2 * OR_MAXND * OR_MAXND], wts[OR_MAXND * OR_MAXND], n[3];
-  const int nd = or_ndofs(0, K->degree), n2 = 2 * nd;
-  const int nc = or_ndofs(0, K->kappa_degree);
-  const int nq = or_tri_rule(K->qm, pts, wts);
-  const double ds = facet_normal(xc, eli[0], n);
+  const int nd = or_ndofs(K->cell, K->degree), n2 = 2 * nd;
+  const int nc = or_ndofs(K->cell, K->kappa_degree);
+  const int nq = facet_rule(K->cell, K->qm, pts, wts);
+  const double ds = facet_normal_c(K->cell, xc, eli[0], n);
   const double* k0 = w + 2 * K->kappa_off;
   const double* v0 = w + 2 * K->f_off;
   for (int q = 0; q < nq; ++q) {
     double X0[3], X1[3], pa[OR_MAXND], pb[OR_MAXND], d[3 * OR_MAXND];
-    facet_point(eli[0], pts[2 * q], pts[2 * q + 1], X0);
-    other_side_point(xc, X0, xc + 12, X1);
-    const double kav = 0.5 * (eval_at(K->kappa_degree, X0, k0) + eval_at(K->kappa_degree, X1, k0 + nc));
-    const double ua = eval_at(K->degree, X0, v0), ub = eval_at(K->degree, X1, v0 + nd);
-    tet_basis(K->degree, X0, pa, d);
-    tet_basis(K->degree, X1, pb, d);
+    facet_point_c(K->cell, eli[0], pts[2 * q], pts[2 * q + 1], X0);
+    other_side_point_c(K->cell, xc, X0, xc + 3 * cell_nv(K->cell), X1);
+    const double kav = 0.5 * (eval_at_c(K->cell, K->kappa_degree, X0, k0) + eval_at_c(K->cell, K->kappa_degree, X1, k0 + nc));
+    const double ua = eval_at_c(K->cell, K->degree, X0, v0), ub = eval_at_c(K->cell, K->degree, X1, v0 + nd);
+    or_basis(K->cell, K->degree, X0, pa, d);
+    or_basis(K->cell, K->degree, X1, pb, d);
     const double fw = wts[q] * ds;
     for (int I = 0; I < n2; ++I) {
       const double pI = 0.5 * (I < nd ? pa[I] : pb[I - nd]);

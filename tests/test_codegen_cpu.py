@@ -200,6 +200,22 @@ def test_generated_reference_test_forms(tmp_path, k):
         _check_form(form, tmp_path, ncells=3)
 
 
+@pytest.mark.parametrize("k", [1, 2])
+@pytest.mark.parametrize("shuffle", [None, 11])
+def test_generated_reference_test_forms_on_triangles(tmp_path, k, shuffle):
+    """The same forms on the reference suite's own domain, a unit square of
+    triangles; with shuffle, cells list their vertices in random order, so
+    interior edges are seen in opposite directions (edge permutation 1)."""
+    from mifem.mesh import create_unit_square
+    from mifem.nonlinear import diagonal, reference_test_forms
+    mesh = create_unit_square(4, 3, shuffle_vertices=shuffle)
+    if shuffle is not None:
+        assert mesh.facet_permutations().max() == 1
+    F = reference_test_forms(mesh, k)
+    for form in F["vector"] + F["matrix"] + F["scalar"] + [diagonal(a) for a in F["matrix"]]:
+        _check_form(form, tmp_path, ncells=6)
+
+
 def test_staged_description_matches_oracle():
     P = problems.elasticity_p2(2)
     I = P.a.integrals[0]

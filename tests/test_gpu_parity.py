@@ -1018,16 +1018,23 @@ def test_fused_system_dirichlet_and_graph_replay():
 # ------------------------------------------------------------ the reference test suite, on tetrahedra
 # (python/test/test_cuda_assembly.py: test_cuda_assembly, test_reassembly, test_lifting,
 # test_diagonal_assembly; its 2D unit square replaced by a jittered unit cube)
-def _ref_suite(k=1):
+def _ref_suite(k=1, domain="tet"):
+    """The reference suite's forms on a jittered tetrahedral cube, or on its
+    own domain (test_cuda_assembly.py:40-46): the 19 x 27 unit square of
+    triangles."""
+    from mifem.mesh import create_unit_square
     from mifem.nonlinear import reference_test_forms
-    return reference_test_forms(create_unit_cube(4, jitter=0.15, seed=3), k)
+    mesh = (create_unit_cube(4, jitter=0.15, seed=3) if domain == "tet"
+            else create_unit_square(19, 27))
+    return reference_test_forms(mesh, k)
 
 
+@pytest.mark.parametrize("domain", ["tet", "tri"])
 @pytest.mark.parametrize("k", [1, 2])
-def test_reference_suite_cuda_assembly(k):
-    _say(f"reference suite: functional, 3 residuals, 3 Jacobians with bcs, P{k}/DG{k}")
+def test_reference_suite_cuda_assembly(k, domain):
+    _say(f"reference suite ({domain}): functional, 3 residuals, 3 Jacobians with bcs, P{k}/DG{k}")
     import cudolfinx as cufem
-    F = _ref_suite(k)
+    F = _ref_suite(k, domain)
     asm = cufem.CUDAAssembler()
     for form in F["scalar"]:
         v1 = cpu.OracleAssembler(form).assemble_scalar()
@@ -1050,10 +1057,11 @@ def test_reference_suite_cuda_assembly(k):
         assert err <= TOL
 
 
-def test_reference_suite_reassembly():
-    _say("reference suite: reassembly after kappa.interpolate(x^d + y^d), d = 2, 3")
+@pytest.mark.parametrize("domain", ["tet", "tri"])
+def test_reference_suite_reassembly(domain):
+    _say(f"reference suite: reassembly after kappa.interpolate(x^d + y^d), d = 2, 3 ({domain})")
     import cudolfinx as cufem
-    F = _ref_suite()
+    F = _ref_suite(1, domain)
     coeff = F["coeff"]
     cform = cufem.form(F["vector"][0])
     asm = cufem.CUDAAssembler()
@@ -1066,10 +1074,11 @@ def test_reference_suite_reassembly():
         assert _rel(vec.to_host(), cpu.OracleAssembler(F["vector"][0]).assemble_vector()) <= TOL
 
 
-def test_reference_suite_lifting():
-    _say("reference suite: set_bc then apply_lifting on the exterior residual / Jacobian")
+@pytest.mark.parametrize("domain", ["tet", "tri"])
+def test_reference_suite_lifting(domain):
+    _say(f"reference suite: set_bc then apply_lifting on the exterior residual / Jacobian ({domain})")
     import cudolfinx as cufem
-    F = _ref_suite()
+    F = _ref_suite(1, domain)
     L, a = F["vector"][1], F["matrix"][1]
     asm = cufem.CUDAAssembler()
     vec = asm.assemble_vector(cufem.form(L))
@@ -1083,11 +1092,12 @@ def test_reference_suite_lifting():
     assert _rel(vec.to_host(), ref) <= TOL
 
 
-def test_reference_suite_diagonal_assembly():
-    _say("reference suite: diagonal part of each Jacobian assembled as a vector")
+@pytest.mark.parametrize("domain", ["tet", "tri"])
+def test_reference_suite_diagonal_assembly(domain):
+    _say(f"reference suite: diagonal part of each Jacobian assembled as a vector ({domain})")
     import cudolfinx as cufem
     from mifem.nonlinear import diagonal
-    F = _ref_suite()
+    F = _ref_suite(1, domain)
     asm = cufem.CUDAAssembler()
     for a in F["matrix"]:
         d = diagonal(a)

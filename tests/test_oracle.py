@@ -868,3 +868,124 @@ def test_sipg_invariant_under_cell_vertex_order(k):
     Ei = cpu.OracleAssembler(Form(0, [], [inter], [P0.extras["u"]], [], "int",
                                   domain=P0.mesh)).assemble_scalar()
     assert abs(Ei) > 1e-3 * abs(E[0])
+
+
+# ------------------------------------------------------------ triangles (the reference suite's domain)
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_triangle_basis_nodal_partition_and_gradients(k):
+    """P_k on the reference triangle: nodal at the Basix-ordered nodes of the
+    engine's element, a partition of unity, gradients summing to zero, and
+    the interpolant of a degree-k polynomial reproducing it and its gradient."""
+    el = E.Element("P", "triangle", k)
+    X = np.hstack([el.nodes, np.zeros((el.ndofs, 1))])
+    V = np.array([cpu.basis(2, k, x)[0] for x in X])
+    np.testing.assert_allclose(V, np.eye(el.ndofs), atol=1e-14)
+    f = lambda x: x[..., 0] ** k - 2 * x[..., 0] * x[..., 1] ** (k - 1) + 0.5 * x[..., 1]
+    vals = f(X)
+    for p in np.random.default_rng(4).random((7, 2)) * 0.45:
+        p3 = np.array([p[0], p[1], 0.0])
+        phi, dphi = cpu.basis(2, k, p3)
+        assert abs(phi.sum() - 1) < 1e-14
+        np.testing.assert_allclose(dphi.sum(axis=0), 0, atol=1e-12)
+        gf = np.array([k * p[0] ** (k - 1) - 2 * p[1] ** (k - 1),
+                       -2 * (k - 1) * p[0] * p[1] ** max(k - 2, 0) + 0.5, 0.0])
+        np.testing.assert_allclose(vals @ dphi, gf, atol=1e-11)
+        assert abs(vals @ phi - f(p3)) < 1e-12
+
+
+def test_line_rule_exactness():
+    """The m-point Gauss rule on [0, 1] (triangle edges) integrates t^a exactly
+    for a <= 2m - 1."""
+    for m in range(1, 6):
+        t, w = cpu.gauss_jacobi(m, 0.0)
+        for a in range(2 * m):
+            assert abs(np.sum(w * t ** a) - 1.0 / (a + 1)) < 1e-14
+
+
+def _tri_forms(nx=5, ny=7, k=1):
+    from mifem.mesh import create_unit_square
+    from mifem.nonlinear import reference_test_forms
+    mesh = create_unit_square(nx, ny)
+    return mesh, reference_test_forms(mesh, k)
+
+
+def test_triangle_p1_stiffness_closed_form():
+    """The cell Jacobian with kappa = 0 is the P1 stiffness matrix |T| G G^T
+    (G: barycentric gradients of the physical triangle)."""
+    rng = np.random.default_rng(3)
+    x = np.hstack([rng.random((3, 2)), np.zeros((3, 1))])
+    area = 0.5 * abs(np.linalg.det(x[1:, :2] - x[0, :2]))
+    G = np.linalg.inv(np.hstack([np.ones((3, 1)), x[:, :2]]))[1:].T      # (3, 2)
+    spec = dict(kind="nl_cell", cell=2, degree=1, bs=1, rank=2, qm=3, u_off=0, kappa_off=3)
+    A = cpu.tabulate(spec, np.zeros(6), [], x)
+    np.testing.assert_allclose(A, area * G @ G.T, rtol=1e-12, atol=1e-14)
+
+
+def test_triangle_reference_forms_closed_forms():
+    """On the unit square of triangles, summed over all test functions (a
+    partition of unity): cell residual = e^0.4 (u = 0.4, kappa = 1); exterior
+    residual = int u kappa (grad u . n) ds = 1/2 (u = y, kappa = x: the edge
+    y = 1 only); interior residual = the functional's interior term.  The
+    functional: e^0.4 for constant u; int_dOmega y ds = 2; int avg(v^2)
+    avg(kappa) over interior edges with v = kappa = 1 = their total length."""
+    mesh, F = _tri_forms()
+    u, kap, udg = F["u"], F["coeff"], F["u_dg"]
+    u.x[:] = 0.4
+    kap.x[:] = 1.0
+    assert abs(cpu.OracleAssembler(F["vector"][0]).assemble_vector().sum() - math.exp(0.4)) <= 1e-13
+    assert abs(cpu.OracleAssembler(_sub(F["scalar"][0], 0)).assemble_scalar() - math.exp(0.4)) <= 1e-13
+    u.interpolate(lambda x: x[1])
+    assert abs(cpu.OracleAssembler(_sub(F["scalar"][0], 1)).assemble_scalar() - 2.0) <= 1e-13
+    kap.interpolate(lambda x: x[0])
+    assert abs(cpu.OracleAssembler(F["vector"][1]).assemble_vector().sum() - 0.5) <= 1e-13
+    udg.interpolate(lambda x: 0.3 + 0.6 * x[0] - 0.2 * x[1])
+    got = cpu.OracleAssembler(F["vector"][2]).assemble_vector().sum()
+    ref = cpu.OracleAssembler(_sub(F["scalar"][0], 2)).assemble_scalar()
+    assert abs(got - ref) <= 1e-13 * abs(ref)
+    kap.x[:] = 1.0
+    udg.x[:] = 1.0
+    ft = mesh.facet_table
+    length = sum(np.linalg.norm(np.subtract(*mesh.x[mesh.cells[c0][list(ft[l0])]]))
+                 for c0, l0, _, _ in mesh.interior_facets)
+    assert abs(cpu.OracleAssembler(_sub(F["scalar"][0], 2)).assemble_scalar() - length) <= 1e-12 * length
+
+
+def test_triangle_functional_cell_term_closed_form():
+    """int_T e^l for linear l: 2 |T| sum_i e^{l_i} / prod_{j != i} (l_i - l_j)."""
+    mesh, F = _tri_forms(4, 3)
+    a = np.array([0.3, -0.5, 0.0])
+    F["u"].interpolate(lambda x: 0.1 + a @ x)
+    F["coeff"].x[:] = 1.0
+    got = cpu.OracleAssembler(_sub(F["scalar"][0], 0)).assemble_scalar()
+    ref = 0.0
+    for c in mesh.cells:
+        X = mesh.x[c]
+        area = 0.5 * abs(np.linalg.det(X[1:, :2] - X[0, :2]))
+        l = 0.1 + X @ a
+        ref += 2 * area * sum(np.exp(l[i]) / np.prod([l[i] - l[j] for j in range(3) if j != i])
+                              for i in range(3))
+    assert abs(got - ref) <= 1e-7 * abs(ref), (got, ref)
+
+
+@pytest.mark.parametrize("which", [0, 1, 2])
+def test_triangle_jacobians_are_residual_derivatives(which):
+    """J = dF/du (cell, exterior) and dF/du_dg (interior) by central differences, on triangles."""
+    import scipy.sparse as sp
+    mesh, F = _tri_forms(4, 5)
+    L, a = F["vector"][which], F["matrix"][which]
+    state = F["u_dg"] if which == 2 else F["u"]
+    rng = np.random.default_rng(19 + which)
+    state.x[:] = rng.uniform(-0.5, 0.5, state.x.size)
+    x0 = state.x.copy()
+    oa = cpu.OracleAssembler(a)
+    rp, cols, _, _ = oa.sparsity()
+    V = a.function_spaces[0]
+    J = sp.csr_matrix((oa.assemble_matrix(rp, cols), cols, rp), shape=(V.size, V.size))
+    d = rng.standard_normal(x0.size)
+    h = 1e-6
+    state.x[:] = x0 + h * d
+    fp = cpu.OracleAssembler(L).assemble_vector()
+    state.x[:] = x0 - h * d
+    fm = cpu.OracleAssembler(L).assemble_vector()
+    state.x[:] = x0
+    assert np.linalg.norm((fp - fm) / (2 * h) - J @ d) <= 1e-7 * max(1.0, np.linalg.norm(J @ d))

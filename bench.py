@@ -686,9 +686,18 @@ def bench_c5(args):
             f"{g.V.size} dofs, nnz {A.nnz} ({time.perf_counter() - t0:.1f} s)")
 
     # every group's matrix from one persistent launch (after one record pass per group)
-    fused = cufem.CUDABatch([a for g, a, L, A, b, bcs in sys_])
     mats = [A for g, a, L, A, b, bcs in sys_]
     gbcs = [bcs for g, a, L, A, b, bcs in sys_]
+    batch_tune = None
+    if args.no_tune:
+        fused = cufem.CUDABatch([a for g, a, L, A, b, bcs in sys_])
+    else:
+        # member shapes (rows per workgroup, pairs per chunk) tuned together
+        from cudolfinx.autotune import autotune_batch
+        base = [{"precision": 32} if g.degree >= 2 and not args.c5_fp64 else {} for g in B.groups]
+        _, fused, _, batch_tune = autotune_batch([g.a for g in B.groups], asm, mats, gbcs,
+                                                 base_options=base, verbose=True)
+        log(f"batch autotuned ({time.perf_counter() - t0:.1f} s)")
     # every group's right-hand side (cell integrals) in one launch, the
     # Dirichlet values of all groups in one launch
     rhs = cufem.CUDABatch([L for g, a, L, A, b, bcs in sys_])
@@ -794,6 +803,7 @@ def bench_c5(args):
                                "precision": 32 if g.degree >= 2 and not args.c5_fp64 else 64}
                               for g, a, L, A, b, bcs in sys_],
                    "dof_per_s": ndofs / (ms * 1e-3), "matrix_ms": kms, "matrix_ms_per_group_launches": kms_sep,
+                   "batch_autotune": batch_tune,
                    "sampled_errors": errs,
                    "max_rel_err_A": max((e["rel_err_A"] for e in errs), default=None),
                    "max_rel_err_b": max((e["rel_err_b"] for e in errs), default=None)},

@@ -121,48 +121,69 @@ $KNAME(const int32_t n, const int32_t* __restrict__ ents, const int32_t* __restr
   // nb0: Dirichlet bits per node (null: none); cflag[c] != 0 where cell c has
   // a constrained node (set with nb0), so other cells skip the bit gather
   constexpr int ND = SK_ND, BS = SK_BS, NV = $NV, W = $W;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  const int64_t c = ents[e];
-  double x[NV][3];
-  for (int v = 0; v < NV; ++v) {
-    const int64_t gv = geo[c * NV + v];
-    x[v][0] = xg[3 * gv]; x[v][1] = xg[3 * gv + 1]; x[v][2] = xg[3 * gv + 2];
-  }
-  const double J00 = x[1][0] - x[0][0], J01 = x[2][0] - x[0][0], J02 = x[3][0] - x[0][0];
-  const double J10 = x[1][1] - x[0][1], J11 = x[2][1] - x[0][1], J12 = x[3][1] - x[0][1];
-  const double J20 = x[1][2] - x[0][2], J21 = x[2][2] - x[0][2], J22 = x[3][2] - x[0][2];
-  const double m00 = J11 * J22 - J12 * J21, m01 = J02 * J21 - J01 * J22, m02 = J01 * J12 - J02 * J11;
-  const double det = J00 * m00 + J10 * m01 + J20 * m02;
-  const double id = 1.0 / det;
-  mf_rec_$TAG r;
-  double K[9];
-  K[0] = m00 * id; K[1] = m01 * id; K[2] = m02 * id;
-  K[3] = (J12 * J20 - J10 * J22) * id; K[4] = (J00 * J22 - J02 * J20) * id; K[5] = (J02 * J10 - J00 * J12) * id;
-  K[6] = (J10 * J21 - J11 * J20) * id; K[7] = (J01 * J20 - J00 * J21) * id; K[8] = (J00 * J11 - J01 * J10) * id;
+  constexpr int RW = sizeof(mf_rec_$TAG) / 16;
+  // When the workgroup's cells are consecutive (a cell integral over every
+  // cell), its records are staged in LDS and written out as one contiguous
+  // block with 16-byte stores at consecutive addresses, instead of each lane
+  // storing its record at a stride of sizeof(record) bytes.
+  __shared__ int4 srec[256 * RW];
+  const int e0 = blockIdx.x * 256;
+  const int cnt = min(256, n - e0);
+  const int e = e0 + threadIdx.x;
+  // block-uniform, so the one barrier below sits in uniform control flow
+  // (lanes past n skip only the record computation)
+  const bool staged = __syncthreads_and(e >= n || ents[e] == ents[e0] + (int32_t)threadIdx.x);
+  if (e < n) {
+    const int64_t c = ents[e];
+    double x[NV][3];
+    for (int v = 0; v < NV; ++v) {
+      const int64_t gv = geo[c * NV + v];
+      x[v][0] = xg[3 * gv]; x[v][1] = xg[3 * gv + 1]; x[v][2] = xg[3 * gv + 2];
+    }
+    const double J00 = x[1][0] - x[0][0], J01 = x[2][0] - x[0][0], J02 = x[3][0] - x[0][0];
+    const double J10 = x[1][1] - x[0][1], J11 = x[2][1] - x[0][1], J12 = x[3][1] - x[0][1];
+    const double J20 = x[1][2] - x[0][2], J21 = x[2][2] - x[0][2], J22 = x[3][2] - x[0][2];
+    const double m00 = J11 * J22 - J12 * J21, m01 = J02 * J21 - J01 * J22, m02 = J01 * J12 - J02 * J11;
+    const double det = J00 * m00 + J10 * m01 + J20 * m02;
+    const double id = 1.0 / det;
+    mf_rec_$TAG r;
+    double K[9];
+    K[0] = m00 * id; K[1] = m01 * id; K[2] = m02 * id;
+    K[3] = (J12 * J20 - J10 * J22) * id; K[4] = (J00 * J22 - J02 * J20) * id; K[5] = (J02 * J10 - J00 * J12) * id;
+    K[6] = (J10 * J21 - J11 * J20) * id; K[7] = (J01 * J20 - J00 * J21) * id; K[8] = (J00 * J11 - J01 * J10) * id;
 #ifdef SK_TRACE
-  // scalar reference-tensor forms only need the metric (K[3 r + c] = dX_r / dx_c)
-  const double dt = fabs(det);
-  r.M[0] = dt * (K[0] * K[0] + K[1] * K[1] + K[2] * K[2]);
-  r.M[1] = dt * (K[3] * K[3] + K[4] * K[4] + K[5] * K[5]);
-  r.M[2] = dt * (K[6] * K[6] + K[7] * K[7] + K[8] * K[8]);
-  r.M[3] = dt * (K[0] * K[3] + K[1] * K[4] + K[2] * K[5]);
-  r.M[4] = dt * (K[0] * K[6] + K[1] * K[7] + K[2] * K[8]);
-  r.M[5] = dt * (K[3] * K[6] + K[4] * K[7] + K[5] * K[8]);
+    // scalar reference-tensor forms only need the metric (K[3 r + c] = dX_r / dx_c)
+    const double dt = fabs(det);
+    r.M[0] = dt * (K[0] * K[0] + K[1] * K[1] + K[2] * K[2]);
+    r.M[1] = dt * (K[3] * K[3] + K[4] * K[4] + K[5] * K[5]);
+    r.M[2] = dt * (K[6] * K[6] + K[7] * K[7] + K[8] * K[8]);
+    r.M[3] = dt * (K[0] * K[3] + K[1] * K[4] + K[2] * K[5]);
+    r.M[4] = dt * (K[0] * K[6] + K[1] * K[7] + K[2] * K[8]);
+    r.M[5] = dt * (K[3] * K[6] + K[4] * K[7] + K[5] * K[8]);
 #else
-  for (int x = 0; x < 9; ++x) r.K[x] = K[x];
-  r.det = fabs(det);
+    for (int x = 0; x < 9; ++x) r.K[x] = K[x];
+    r.det = fabs(det);
 #endif
-  for (int l = 0; l < $WW; ++l) r.w[l] = l < W ? coeffs[c * W + l] : 0.0;
-  uint32_t bits = 0;
-  const bool gather = nb0 && cflag[c];
-  for (int l = 0; l < ND; ++l) {
-    const int32_t node = dm0[c * ND + l];
-    r.d[l] = node;
-    if (gather) bits |= (uint32_t)nb0[node] << (BS * l);
+    for (int l = 0; l < $WW; ++l) r.w[l] = l < W ? coeffs[c * W + l] : 0.0;
+    uint32_t bits = 0;
+    const bool gather = nb0 && cflag[c];
+    for (int l = 0; l < ND; ++l) {
+      const int32_t node = dm0[c * ND + l];
+      r.d[l] = node;
+      if (gather) bits |= (uint32_t)nb0[node] << (BS * l);
+    }
+    r.bits = bits;
+    if (staged)
+      reinterpret_cast<mf_rec_$TAG*>(srec)[threadIdx.x] = r;
+    else
+      rec[c] = r;
   }
-  r.bits = bits;
-  rec[c] = r;
+  if (staged) {
+    __syncthreads();
+    const int64_t c0 = ents[e0];
+    int4* dst = reinterpret_cast<int4*>(rec + c0);
+    for (int p = threadIdx.x; p < cnt * RW; p += 256) dst[p] = srec[p];
+  }
 }
 )";
 

@@ -1,4 +1,5 @@
-"""Autotuners: the cooperative owner-computes matrix kernel and the fused
+"""Autotuners: the cooperative owner-computes matrix kernel, the batched
+(several element groups in one launch) matrix kernel and the fused
 Newton-system kernel.
 
 Picks the kernel shape -- LDS value budget per workgroup ("chunk_values",
@@ -181,3 +182,83 @@ def autotune_matrix_native(a, asm, A, bcs=(), reps: int = 5):
     check(lib().mf_autotune_matrix(a.handle, A.handle, bc.handle, int(reps), asm._stream, buf,
                                    256, C.byref(ms)))
     return buf.value.decode(), ms.value
+
+
+# member shapes of a CUDABatch: value chunk (rows per workgroup) and pairs
+# per chunk; the batch kernel's threads per workgroup are shared by all members
+BATCH_CANDIDATES = [
+    {"chunk_values": 2048, "owner_pairs": 384},
+    {"chunk_values": 1536, "owner_pairs": 256},
+    {"chunk_values": 3072, "owner_pairs": 384, "owner_lds": 98304},
+    {"chunk_values": 4096, "owner_pairs": 384, "owner_lds": 98304},
+]
+
+
+def autotune_batch(forms, asm, mats, bcs, base_options=None, candidates=None, reps: int = 5,
+                   sweeps: int = 1, verbose: bool = False):
+    """Member shapes of a CUDABatch of bilinear forms, tuned together.
+
+    The batch kernel's occupancy follows its largest member's LDS, so one
+    member's shape changes the speed of all: the tuner walks the members one
+    at a time (coordinate descent, `sweeps` passes), timing each candidate as
+    the whole batch launch on the actual matrices and boundary conditions
+    while the other members keep their best shape so far.  forms: the
+    members' (mifem) bilinear forms; base_options[g]: options member g always
+    carries (e.g. {"precision": 32}).
+
+    Returns (options per member, the CUDABatch, its CUDAForms, timings).
+    """
+    from .batch import CUDABatch
+
+    cands = list(candidates or BATCH_CANDIDATES)
+    n = len(forms)
+    base = [dict(o or {}) for o in (base_options or [None] * n)]
+    compiled = {}
+
+    def member(g, c):
+        key = (g, c)
+        if key not in compiled:
+            opts = dict(base[g])
+            opts.update(cands[c])
+            opts["matrix_kernel"] = 2
+            compiled[key] = CUDAForm(forms[g], options=opts)
+        return compiled[key]
+
+    def time_batch(choice):
+        fs = [member(g, c) for g, c in enumerate(choice)]
+        batch = CUDABatch(fs)
+        batch.assemble(asm, mats, bcs)                # warm-up (and coefficient upload)
+        asm.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            batch.assemble(asm, mats, bcs, pack=False)
+        asm.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3, batch, fs
+
+    choice = [0] * n
+    best_ms, best_batch, best_fs = time_batch(choice)
+    timings = [{"members": list(choice), "ms": best_ms}]
+    if verbose:
+        print(f"[autotune] batch {choice}: {best_ms:.3f} ms", file=sys.stderr, flush=True)
+    for _ in range(sweeps):
+        for g in range(n):
+            for c in range(1, len(cands)) if n else ():
+                if c == choice[g]:
+                    continue
+                trial = list(choice)
+                trial[g] = c
+                try:
+                    ms, batch, fs = time_batch(trial)
+                except (RuntimeError, ValueError) as e:
+                    if verbose:
+                        print(f"[autotune] batch {trial}: unavailable ({e})", file=sys.stderr, flush=True)
+                    continue
+                timings.append({"members": trial, "ms": ms})
+                if verbose:
+                    print(f"[autotune] batch {trial}: {ms:.3f} ms", file=sys.stderr, flush=True)
+                if ms < best_ms:
+                    best_ms, best_batch, best_fs, choice = ms, batch, fs, trial
+    opts = [dict(base[g], **cands[c]) for g, c in enumerate(choice)]
+    for t in timings:
+        t["members"] = [cands[c] for c in t["members"]]
+    return opts, best_batch, best_fs, timings

@@ -517,6 +517,39 @@ def test_many_instances_one_launch():
         assert np.array_equal(vals[~free], v0[~free])
 
 
+def test_batch_autotuner_shapes_agree():
+    _say("batch autotuner on 12 Poisson instances (P1-P3): every member shape it times and the "
+         "winner give the matrices of separate assembly")
+    from mifem import batch
+    import cudolfinx as cufem
+    from cudolfinx.autotune import autotune_batch
+    B = batch.many_poisson(12, seed=7, nmin=2, nmax=5)
+    asm = cufem.CUDAAssembler()
+    forms_ = [cufem.form(g.a) for g in B.groups]
+    bcs = [asm.pack_bcs(g.bcs) for g in B.groups]
+    mats = [asm.create_matrix(a) for a in forms_]
+    sep = [asm.create_matrix(a) for a in forms_]
+    for a, S, bc in zip(forms_, sep, bcs):
+        asm.assemble_matrix(a, S, bcs=bc)
+    cands = [{"chunk_values": 1024, "owner_pairs": 256}, {"chunk_values": 2048, "owner_pairs": 384}]
+    opts, fused, members, timings = autotune_batch([g.a for g in B.groups], asm, mats, bcs,
+                                                   candidates=cands, reps=2)
+    # the first choice plus one trial per member and other candidate
+    assert len(timings) == 1 + len(B.groups) * (len(cands) - 1)
+    assert len(opts) == len(members) == len(B.groups)
+    for o in opts:
+        assert {k: o[k] for k in ("chunk_values", "owner_pairs")} in cands
+    fused.assemble(asm, mats, bcs)
+    asm.synchronize()
+    for g, A, S in zip(B.groups, mats, sep):
+        rp, cols, vals = A.csr()
+        srp, scols, svals = S.csr()
+        assert np.array_equal(rp, srp) and np.array_equal(cols, scols)
+        e = _rel(vals, svals)
+        _say(f"P{g.degree}: nnz={cols.size}, tuned {opts[B.groups.index(g)]}, vs separate {e:.2e}")
+        assert e <= 1e-14
+
+
 def test_many_right_hand_sides_one_launch():
     _say("right-hand sides of 12 Poisson instances (P1-P3) + elasticity (body force and traction): "
          "cell integrals in one launch, set_bc in one launch")

@@ -303,7 +303,9 @@ OwnerShape owner_shape(const mf_form& f, const Integral& I) {
   OwnerShape sh;
   sh.nb = static_cast<int>(std::min<int64_t>(f.opt("owner_nb", 64), 1023));
   sh.maxp = static_cast<int>(f.opt("owner_pairs", 384));
-  const int64_t trace = staged_trace(I) ? 8LL * nd * nd * 6 : (staged_fullrt(I) ? 8LL * nd * nd * 9 : 0);
+  const int64_t nq = staged_define(I.staged_src, "SK_NQ");
+  const int64_t trace = staged_trace(I) ? 8LL * nd * nd * 6
+                                        : (staged_fullrt(I) ? 8LL * nd * nd * 9 : 8LL * nq * nd * 3);
   const int64_t used = fixed + trace + (int64_t)sh.maxp * (4 + nd);
   MF_CHECK(used <= budget, MF_EUNSUPPORTED,
            "owner-computes kernel shape exceeds the LDS budget (" + std::to_string(used) + " > " +
@@ -318,7 +320,9 @@ bool owner_pipe_emitted(const mf_form& f, const Integral& I) {
   const int64_t cap = f.opt("chunk_values", 4096), maxr = f.opt("chunk_rows", 64);
   const int64_t maxp = owner_shape(f, I).maxp;
   const int64_t lists = 4 * maxp + (maxp * nd + 15) / 16 * 16 + 4 * (maxr + 1) + maxr;
-  const int64_t table = staged_trace(I) ? 8 * nd * nd * 6 : (staged_fullrt(I) ? 8 * nd * nd * 9 : 0);
+  const int64_t nq = staged_define(I.staged_src, "SK_NQ");
+  const int64_t table = staged_trace(I) ? 8 * nd * nd * 6
+                                        : (staged_fullrt(I) ? 8 * nd * nd * 9 : 8 * nq * nd * 3);
   return f.opt("owner_f32acc", 0) == 0 && 16 * cap + 2 * lists + table + 256 <= 160 * 1024;
 }
 
@@ -472,7 +476,8 @@ std::string generate_system_source(const mf_form& a, const mf_form& L, const mf_
 
 // The MF_SHARED slices of one owner-computes body (kOwnerTemplate), in
 // declaration order: name, element type, bytes (rounded up to 16).  The
-// reference-tensor slice (sRS or sRT) is present only for those variants.
+// last slice holds the reference tensors (sRS or sRT) of those variants, the
+// reference gradients (sDPHI) otherwise.
 struct LdsSlice {
   const char* name;
   const char* type;
@@ -498,6 +503,7 @@ static std::vector<LdsSlice> owner_lds_slices(const mf_form& f, const Integral& 
                              {"skk", "uint8_t", r16((maxp * nd + 15) / 16 * 16)}};
   if (staged_trace(I)) v.push_back({"sRS", at, r16(as * nd * nd * 6)});
   else if (staged_fullrt(I)) v.push_back({"sRT", "double", r16(8 * nd * nd * 9)});
+  else v.push_back({"sDPHI", "double", r16(8 * staged_define(I.staged_src, "SK_NQ") * nd * 3)});
   return v;
 }
 

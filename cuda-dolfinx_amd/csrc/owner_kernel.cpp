@@ -245,6 +245,11 @@ $OHEAD($OPARAMS)
   // full reference tensors R_ij[3 r + s] (9 per entry), loaded once per workgroup
   MF_SHARED(double, sRT, ND * ND * 9);
   for (int p = threadIdx.x; p < ND * ND * 9; p += NT) sRT[p] = (&SK_RT[0][0][0])[p];
+#else
+  // reference gradients, read with the lane's own row index: from LDS these
+  // reads do not share the vector-memory counter with the value stores
+  MF_SHARED(double, sDPHI, NQ * ND * 3);
+  for (int p = threadIdx.x; p < NQ * ND * 3; p += NT) sDPHI[p] = (&SK_DPHI[0][0][0])[p];
 #endif
 
   const int tid = threadIdx.x, nt = NT;
@@ -364,7 +369,7 @@ $OHEAD($OPARAMS)
 #pragma unroll
       for (int t = 0; t < PF_R; ++t) {
         const int p = tid + t * NT;
-        pf_r[t] = p <= n0.y - n0.x ? rpb[n0.x + p] - n1.x : 0;
+        pf_r[t] = p <= n0.y - n0.x ? rpb[n0.x + p] : 0;
       }
 #pragma unroll
       for (int t = 0; t < PF_C; ++t) {
@@ -510,7 +515,7 @@ $UNIT
 #pragma unroll
       for (int t = 0; t < PF_R; ++t) {
         const int p = tid + t * NT;
-        if (p <= n0.y - n0.x) lrp[p] = pf_r[t];
+        if (p <= n0.y - n0.x) lrp[p] = pf_r[t] - n1.x;
       }
 #pragma unroll
       for (int t = 0; t < PF_C; ++t) {
@@ -702,6 +707,9 @@ $OHEAD($OPARAMS)
 #elif defined(SK_FULLRT)
   __shared__ __attribute__((aligned(16))) double sRT[ND * ND * 9];
   for (int p = threadIdx.x; p < ND * ND * 9; p += NT + 64) sRT[p] = (&SK_RT[0][0][0])[p];
+#else
+  __shared__ __attribute__((aligned(16))) double sDPHI[NQ * ND * 3];
+  for (int p = threadIdx.x; p < NQ * ND * 3; p += NT + 64) sDPHI[p] = (&SK_DPHI[0][0][0])[p];
 #endif
   const int tid = threadIdx.x, nt = NT;
   const int nx = 8, gx = gridDim.x / nx;
@@ -846,7 +854,7 @@ $OHEAD($OPARAMS)
 #pragma unroll
         for (int t = 0; t < PF_R; ++t) {
           const int p = tid + t * NT;
-          pf_r[t] = p <= n0.y - n0.x ? rpb[n0.x + p] - n1.x : 0;
+          pf_r[t] = p <= n0.y - n0.x ? rpb[n0.x + p] : 0;
         }
 #pragma unroll
         for (int t = 0; t < PF_B; ++t) {
@@ -895,7 +903,7 @@ $UNIT
 #pragma unroll
         for (int t = 0; t < PF_R; ++t) {
           const int p = tid + t * NT;
-          if (p <= n0.y - n0.x) lrp2[b ^ 1][p] = pf_r[t];
+          if (p <= n0.y - n0.x) lrp2[b ^ 1][p] = pf_r[t] - n1.x;
         }
 #pragma unroll
         for (int t = 0; t < PF_B; ++t) {
@@ -949,7 +957,8 @@ const char* kOwnerUnit = R"(
         double gi[NQ][3];
         for (int q = 0; q < NQ; ++q) {
           const double s = SK_W[q] * dt;
-          const double d0 = SK_DPHI[q][i][0], d1v = SK_DPHI[q][i][1], d2 = SK_DPHI[q][i][2];
+          const double* dq = &sDPHI[(q * ND + i) * 3];
+          const double d0 = dq[0], d1v = dq[1], d2 = dq[2];
           gi[q][0] = s * (K0 * d0 + K3 * d1v + K6 * d2);
           gi[q][1] = s * (K1 * d0 + K4 * d1v + K7 * d2);
           gi[q][2] = s * (K2 * d0 + K5 * d1v + K8 * d2);

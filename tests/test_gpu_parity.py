@@ -92,6 +92,42 @@ def test_elasticity_p2_reference_tensor_owner():
     _compare(P, _gpu(P, {"matrix_kernel": 2}), _oracle(P))
 
 
+@pytest.mark.parametrize("form", ["elasticity", "stiffness_p2"])
+@pytest.mark.parametrize("subset", ["mixed", "prefix"])
+def test_owner_cell_integral_on_a_subset_of_cells(form, subset):
+    """A cell integral over part of the mesh (a subdomain) on the owner path.
+    The record pass stages a workgroup's records in LDS when its 256 cells are
+    consecutive and stores them one by one otherwise: "mixed" keeps cells
+    0..699 (two staged blocks, then a block that is not) and a random third of
+    the rest (a partial last block, not staged); "prefix" keeps cells 0..599
+    (a partial last block that is staged)."""
+    import dataclasses
+    _say(f"owner-computes cell integral on a subset of cells: {form}, {subset}")
+    if form == "elasticity":
+        P = problems.elasticity_p2(6)
+    else:
+        P = problems.poisson_p1(6)
+        P.V = functionspace(P.mesh, ("P", 2))
+        kap = Function(functionspace(P.mesh, ("DG", 0)), "kappa")
+        kap.x[:] = np.random.default_rng(3).lognormal(size=kap.x.size)
+        f = Function(P.V, "f")
+        f.interpolate(lambda x: x[0] * x[1] + x[2])
+        P.a, P.L = forms.stiffness(P.V, kap), forms.source(P.V, f)
+        P.bcs = [dirichletbc(0.0, locate_dofs_topological(P.V, 2, P.mesh.exterior_facets), P.V)]
+        P.name = "stiffness_p2"
+    nc = P.mesh.num_cells
+    if subset == "mixed":
+        rng = np.random.default_rng(5)
+        keep = np.concatenate([np.arange(0, 700), rng.choice(np.arange(700, nc), size=(nc - 700) // 3,
+                                                             replace=False)])
+    else:
+        keep = np.arange(0, 600)
+    keep = np.sort(keep).astype(np.int32)
+    I = P.a.integrals[0]
+    P.a = dataclasses.replace(P.a, integrals=[dataclasses.replace(I, entities=keep)])
+    _compare(P, _gpu(P, {"matrix_kernel": 2}), _oracle(P))
+
+
 def test_elasticity_p2_medium():
     _say("elasticity_p2 n=16 owner-computes")
     P = problems.elasticity_p2(16)

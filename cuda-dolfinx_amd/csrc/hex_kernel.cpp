@@ -18,12 +18,12 @@
 // accumulation (v_mfma_f32_16x16x4f32), 16 = BF16 operands with FP32
 // accumulation (v_mfma_f32_16x16x16bf16_1k), 64 = FP64 on the vector ALUs
 // (same tiles and pair packing; on MI355X the FP64 vector and matrix peaks
-// are equal, and chained v_mfma_f64_16x16x4f64 accumulations gave wrong
-// results with this toolchain).  Geometry (J, det, K), D_q, and the CSR
-// accumulation are FP64 in all three.
+// are equal, 78.6 TFLOP/s, so FP64 matrix cores offer no higher ceiling).
+// Geometry (J, det, K), D_q, and the CSR accumulation are FP64 in all three.
 //
 // Operand layouts (16x16xK, lane l): A[row l%16][k-slot l/16], B[k-slot
-// l/16][col l%16], result rows 4*(l/16)+r, column l%16.  The K index is
+// l/16][col l%16], result rows 4*(l/16)+r, column l%16 (the FP32-result
+// instructions; v_mfma_f64_16x16x4f64 places its result rows at l/16 + 4r).  The K index is
 // (q, b) with b fastest, so one K-slot block covers one quadrature point.
 #include "mf_internal.h"
 

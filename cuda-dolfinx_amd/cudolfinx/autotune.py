@@ -242,7 +242,7 @@ def autotune_batch(forms, asm, mats, bcs, base_options=None, candidates=None, re
         print(f"[autotune] batch {choice}: {best_ms:.3f} ms", file=sys.stderr, flush=True)
     for _ in range(sweeps):
         for g in range(n):
-            for c in range(1, len(cands)) if n else ():
+            for c in range(len(cands)):
                 if c == choice[g]:
                     continue
                 trial = list(choice)

@@ -83,18 +83,20 @@ class CUDABatch:
         on the space of linear form g."""
         if len(vecs) != len(self._forms) or len(bcs) != len(self._forms):
             raise ValueError(f"expected {len(self._forms)} vectors and bc sets")
-        hs = []
-        for g, L in enumerate(self._forms):
+        hs, colls = [], []          # colls: a list of DirichletBC packs a new collection,
+        for g, L in enumerate(self._forms):       # which must outlive the call
             if bcs[g] is None:
                 hs.append(None)
                 continue
             coll = asm._bc_collection(bcs[g])
+            colls.append(coll)
             hs.append(coll._get_cpp_bcs(L.function_spaces[0]).handle.value)
         n = len(vecs)
         ptrs = (C.c_void_p * n)(*[v.ptr.value for v in vecs])
         sizes = (C.c_int64 * n)(*[v.size for v in vecs])
         B = (C.c_void_p * n)(*hs)
         check(lib().mf_set_bc_batch(asm._ctx.handle, n, ptrs, sizes, B, float(scale), asm._stream))
+        del colls
         return vecs
 
     def assemble(self, asm, mats, bcs=None, diagonal: float = 1.0, zero: bool = True,
@@ -105,7 +107,7 @@ class CUDABatch:
             raise ValueError(f"expected {len(self._forms)} matrices, got {len(mats)}")
         if bcs is not None and len(bcs) != len(self._forms):
             raise ValueError(f"expected {len(self._forms)} bc sets, got {len(bcs)}")
-        hs = []
+        hs, colls = [], []          # the collections must outlive the call (see set_bc)
         for g, a in enumerate(self._forms):
             if pack:
                 a.to_device(force=False)
@@ -114,12 +116,14 @@ class CUDABatch:
                 hs.append(None)
                 continue
             coll = asm._bc_collection(list(bcs[g]) if isinstance(bcs[g], tuple) else bcs[g])
+            colls.append(coll)
             hs.append(coll._get_cpp_bcs(a.function_spaces[0]).handle.value)
         n = len(self._forms)
         A = (C.c_void_p * n)(*[m.handle.value for m in mats])
         B = (C.c_void_p * n)(*hs)
         check(lib().mf_batch_assemble(self._handle, A, B, float(diagonal), int(bool(zero)),
                                       asm._stream))
+        del colls
         return mats
 
     def __del__(self):

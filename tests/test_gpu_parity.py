@@ -584,6 +584,11 @@ def test_batch_autotuner_shapes_agree():
         e = _rel(vals, svals)
         _say(f"P{g.degree}: nnz={cols.size}, tuned {opts[B.groups.index(g)]}, vs separate {e:.2e}")
         assert e <= 1e-14
+    # plain lists of DirichletBC: each call packs its own collections
+    fused.assemble(asm, mats, [list(g.bcs) for g in B.groups])
+    asm.synchronize()
+    for A, S in zip(mats, sep):
+        assert _rel(A.csr()[2], S.csr()[2]) <= 1e-14
 
 
 def test_many_right_hand_sides_one_launch():
@@ -603,7 +608,8 @@ def test_many_right_hand_sides_one_launch():
     vecs = [asm.create_vector(L) for L in Ls]
     for rep in range(2):
         vb.assemble_vectors(asm, vecs)
-        vb.set_bc(asm, vecs, bcs, scale=1.0 + rep)
+        # packed collections, then plain lists of DirichletBC (packed per call)
+        vb.set_bc(asm, vecs, bcs if rep == 0 else bclists, scale=1.0 + rep)
         asm.synchronize()
         for L, V, bl, v in zip(Ls, spaces, bclists, vecs):
             ref = asm.assemble_vector(L)

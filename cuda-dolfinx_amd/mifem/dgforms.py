@@ -160,6 +160,18 @@ def _sipg_parts(V, u, f, alpha, penalty, p, eps, mu0):
                   penalty=float(penalty), fval=float(f), u_off=0, mu0=float(mu0))
 
     def cell(rank):
+        def power(base: str, e: float) -> str:
+            # half-integer exponents (p an integer) without pow(): sqrt is one
+            # instruction sequence, FP64 pow a long call
+            t = 2.0 * e
+            if t == int(t) and abs(t) <= 6:
+                t = int(t)
+                whole = "*".join([base] * (abs(t) // 2))
+                half = f"sqrt({base})" if abs(t) % 2 else ""
+                prod = " * ".join(x for x in (whole.replace("*", " * "), half) if x) or "1.0"
+                return prod if t >= 0 else f"(1.0 / ({prod}))"
+            return f"pow({base}, {e!r})"
+
         name = f"tabulate_tensor_integral_{_uid('dg_cell', rank, k, p, eps, f, mu0)}"
         out = (f"        A[i * {nd} + j] += fw * (mu * (g[i][0] * g[j][0] + g[i][1] * g[j][1] + g[i][2] * g[j][2])\n"
                f"                               + dmu * gui[i] * gui[j]);\n" if rank == 2 else "")
@@ -170,7 +182,7 @@ def _sipg_parts(V, u, f, alpha, penalty, p, eps, mu0):
                     f"      A[i] += fw * (mu * gui[i] - {float(f)!r} * FE_phi[iq][i]);\n")
         else:
             loop = (f"    double uq = 0.0;\n    for (int i = 0; i < {nd}; ++i) uq += w[i] * FE_phi[iq][i];\n"
-                    f"    A[0] += fw * (pow(s2, {float(p) / 2.0!r}) / {float(p)!r}"
+                    f"    A[0] += fw * ({power('s2', float(p) / 2.0)} / {float(p)!r}"
                     f" + {0.5 * float(mu0)!r} * (gu[0] * gu[0] + gu[1] * gu[1] + gu[2] * gu[2])"
                     f" - {float(f)!r} * uq);\n")
         body = (_carr("FE_w", qc.weights) + _carr("FE_phi", phic) + _carr("FE_dphi", dphic)
@@ -183,8 +195,8 @@ def _sipg_parts(V, u, f, alpha, penalty, p, eps, mu0):
     double gui[{nd}];
     for (int i = 0; i < {nd}; ++i) gui[i] = gu[0] * g[i][0] + gu[1] * g[i][1] + gu[2] * g[i][2];
     const double s2 = {e2} + gu[0] * gu[0] + gu[1] * gu[1] + gu[2] * gu[2];
-    const double mu = {float(mu0)!r} + pow(s2, {e_mu});
-    const double dmu = {float(p - 2.0)!r} * pow(s2, {e_dmu});
+    const double mu = {float(mu0)!r} + {power('s2', float(e_mu))};
+    const double dmu = {float(p - 2.0)!r} * {power('s2', float(e_dmu))};
     (void)dmu;
     const double fw = FE_w[iq] * adet;
 {loop}  }}

@@ -52,13 +52,16 @@ def _geom(s: str, xd: str) -> str:
     {{(J12{s} * J20{s} - J10{s} * J22{s}) * id{s}, (J00{s} * J22{s} - J02{s} * J20{s}) * id{s}, (J02{s} * J10{s} - J00{s} * J12{s}) * id{s}}},
     {{(J10{s} * J21{s} - J11{s} * J20{s}) * id{s}, (J01{s} * J20{s} - J00{s} * J21{s}) * id{s}, (J00{s} * J11{s} - J01{s} * J10{s}) * id{s}}}}};
   const double adet{s} = fabs(det{s});
-  double h{s} = 0.0;
+  /* diameter = longest edge: one sqrt of the largest squared length (sqrt is
+     monotonic, so this equals the largest of the six edge lengths) */
+  double hh{s} = 0.0;
 """ + "".join(f"""  {{
     const double dx = {xd}[{3 * a}] - {xd}[{3 * b}], dy = {xd}[{3 * a + 1}] - {xd}[{3 * b + 1}], dz = {xd}[{3 * a + 2}] - {xd}[{3 * b + 2}];
-    const double dd = sqrt(dx * dx + dy * dy + dz * dz);
-    h{s} = dd > h{s} ? dd : h{s};
+    const double dd = dx * dx + dy * dy + dz * dz;
+    hh{s} = dd > hh{s} ? dd : hh{s};
   }}
-""" for a in range(4) for b in range(a + 1, 4))
+""" for a in range(4) for b in range(a + 1, 4)) + f"""  const double h{s} = sqrt(hh{s});
+"""
 
 
 _FACET = """
@@ -79,7 +82,10 @@ _FACET = """
   const double e2x = P2[0] - P0[0], e2y = P2[1] - P0[1], e2z = P2[2] - P0[2];
   double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
   const double ds = sqrt(nx * nx + ny * ny + nz * nz);
-  nx /= ds; ny /= ds; nz /= ds;
+  {
+    const double ids = 1.0 / ds;          /* one division for the three components */
+    nx *= ids; ny *= ids; nz *= ids;
+  }
   {
     /* outward from the first cell: away from the vertex opposite the facet */
     const double ox = PO[0] - P0[0], oy = PO[1] - P0[1], oz = PO[2] - P0[2];

@@ -396,6 +396,89 @@ std::string device_tabulate_source(const Integral& I) {
   return src;
 }
 
+// The function-local constant tables of tabulate bodies ("static const double
+// NAME[..]... = {...};", the form FFCx emits them in) moved to file scope as
+// __constant__ arrays named by their content, each defined once in `tables`.
+// NAME is aliased to the shared array by a #define around each body that
+// declared it, so bodies inlined one after another (the Jacobian, residual and
+// energy of one integral) read the same arrays and the compiler can share the
+// loads; as separate function-local statics they stay separate objects.
+static std::string hoist_tables(const std::string& src, std::map<std::string, std::string>& seen,
+                                std::string& tables) {
+  const std::string fkey = "void tabulate_tensor_integral_";
+  const std::string tkey = "static const double ";
+  std::string out;
+  size_t pos = 0;
+  while (true) {
+    const size_t f = src.find(fkey, pos);
+    if (f == std::string::npos) break;
+    size_t ls = src.rfind('\n', f);
+    ls = ls == std::string::npos ? 0 : ls + 1;     // start of the definition's line
+    const size_t open = src.find('{', f);
+    if (open == std::string::npos) break;
+    // the body's closing brace (table initialisers are balanced; comments skipped)
+    int depth = 0;
+    size_t close = std::string::npos;
+    for (size_t i = open; i < src.size(); ++i) {
+      if (src.compare(i, 2, "/*") == 0) {
+        const size_t e = src.find("*/", i + 2);
+        if (e == std::string::npos) break;
+        i = e + 1;
+      } else if (src.compare(i, 2, "//") == 0) {
+        const size_t e = src.find('\n', i);
+        if (e == std::string::npos) break;
+        i = e;
+      } else if (src[i] == '{') {
+        ++depth;
+      } else if (src[i] == '}' && --depth == 0) {
+        close = i;
+        break;
+      }
+    }
+    if (close == std::string::npos) break;
+    std::string body = src.substr(open, close + 1 - open);
+    std::string defs, undefs;
+    size_t t = 0;
+    while ((t = body.find(tkey, t)) != std::string::npos) {
+      size_t i = t + tkey.size();
+      size_t n = i;
+      while (n < body.size() && (std::isalnum(static_cast<unsigned char>(body[n])) || body[n] == '_')) ++n;
+      const std::string name = body.substr(i, n - i);
+      size_t d = n;
+      while (d < body.size() && body[d] == '[') {
+        const size_t e = body.find(']', d);
+        if (e == std::string::npos) break;
+        d = e + 1;
+      }
+      const std::string dims = body.substr(n, d - n);
+      size_t eq = d;
+      while (eq < body.size() && body[eq] == ' ') ++eq;
+      if (name.empty() || dims.empty() || eq >= body.size() || body[eq] != '=') {
+        t = n;                                       // scalars and anything else stay
+        continue;
+      }
+      const size_t semi = body.find(';', eq);
+      if (semi == std::string::npos) break;
+      const std::string init = body.substr(eq + 1, semi - eq - 1);
+      const std::string content = dims + "=" + init;
+      auto it = seen.find(content);
+      if (it == seen.end()) {
+        char h[32];
+        std::snprintf(h, sizeof h, "mf_tab_%zu", seen.size());
+        it = seen.emplace(content, h).first;
+        tables += "static __constant__ const double " + it->second + dims + " =" + init + ";\n";
+      }
+      defs += "#define " + name + " " + it->second + "\n";
+      undefs += "#undef " + name + "\n";
+      body.erase(t, semi + 1 - t);
+    }
+    out += src.substr(pos, ls - pos) + defs + src.substr(ls, open - ls) + body + "\n" + undefs;
+    pos = close + 1;
+  }
+  out += src.substr(pos);
+  return out;
+}
+
 std::string generate_system_source(const mf_form& a, const mf_form& L, const mf_form* M,
                                    int block, int wpe) {
   std::ostringstream os;
@@ -415,14 +498,17 @@ std::string generate_system_source(const mf_form& a, const mf_form& L, const mf_
   const bool stage = a.opt("dg_stage", 1) != 0 && 8 * segcap + 64 <= 64 * 1024;
   os << "#define MF_DG_PF " << (pf ? 1 : 0) << "\n#define MF_DG_STAGE " << (stage ? 1 : 0) << "\n";
   std::map<std::string, bool> seen;
+  std::map<std::string, std::string> shared;      // table content -> file-scope name
+  std::string tables, bodies;
   for (const mf_form* f : {&a, &L, M}) {
     if (!f) continue;
     for (auto& I : f->ints) {
       if (seen.count(I->tab_name)) continue;
       seen[I->tab_name] = true;
-      os << "\n" << device_tabulate_source(*I) << "\n";
+      bodies += "\n" + hoist_tables(device_tabulate_source(*I), shared, tables) + "\n";
     }
   }
+  os << tables << bodies;
   std::string interlace;
   for (auto& c : a.coefs) {
     std::ostringstream il;

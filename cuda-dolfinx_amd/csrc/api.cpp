@@ -292,7 +292,7 @@ int mf_form_set_option(mf_form* f, const char* key, int64_t value) {
                                   "vector_block", "vector_chunk_values", "lift_kernel",
                                   "owner_pairs", "precision", "dg_block", "owner_persistent",
                                   "owner_waves", "owner_wpe", "hex_block", "dg_wpe", "owner_f32acc",
-                                  "owner_pipe", "owner_cwaves", "owner_balance", "dg_prefetch", "dg_stage"};
+                                  "owner_pipe", "owner_cwaves", "owner_balance", "vector_prefetch", "dg_prefetch", "dg_stage"};
     bool ok = false;
     for (auto k : known) ok |= std::strcmp(k, key) == 0;
     MF_CHECK(ok, MF_EINVAL, std::string("unknown option ") + key);

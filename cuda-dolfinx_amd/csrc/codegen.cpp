@@ -628,6 +628,7 @@ std::string generate_vbatch_source(const std::vector<const mf_form*>& forms,
     const int64_t vcap = f.opt("vector_chunk_values", 2048);
     cap = std::max(cap, vcap);
     m["$VCAP"] = std::to_string(vcap);
+    m["$VPF"] = f.opt("vector_prefetch", 1) ? "1" : "0";
     const std::string fn = "mf_vbody_g" + std::to_string(g);
     m["$VHEAD"] = "static __device__ __forceinline__ void " + fn;
     m["$VPARAMS"] = std::string("const int k, double* __restrict__ acc, ") + kOwnerVectorParams;
@@ -957,6 +958,7 @@ std::string generate_form_source(const mf_form& f) {
                    "  const int k = mf_xcd_chunk(nchunks);\n"
                    "  if (k < 0) return;";
         m["$VCAP"] = std::to_string(f.opt("vector_chunk_values", 2048));
+        m["$VPF"] = f.opt("vector_prefetch", 1) ? "1" : "0";
         os << subst(kOwnerVectorTemplate, m);
       }
     }

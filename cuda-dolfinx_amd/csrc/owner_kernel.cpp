@@ -559,6 +559,42 @@ $VK
   for (int p = tid; p < len; p += nt) acc[p] = 0.0;
   __syncthreads();
   const int i0 = chunk_items[k], i1 = chunk_items[k + 1];
+#if $VPF
+  // the next item's cell, row mask, vertex coordinates and row dofs are
+  // loaded while the current item's body runs (the chain item -> cell ->
+  // vertices -> coordinates no longer waits in series per item)
+  int en = 0;
+  uint32_t mn = 0;
+  double xn[$NV * 3];
+  int32_t dn[$ND0];
+  auto fetch = [&](const int it) {
+    en = item_cell[it];
+    mn = item_mask[it];
+    for (int v = 0; v < $NV; ++v) {
+      const int64_t gv = geo[(int64_t)en * $NV + v];
+      xn[3 * v] = xg[3 * gv]; xn[3 * v + 1] = xg[3 * gv + 1]; xn[3 * v + 2] = xg[3 * gv + 2];
+    }
+    for (int i = 0; i < $ND0; ++i) dn[i] = (mn >> i) & 1u ? dm0[(int64_t)en * $ND0 + i] : 0;
+  };
+  if (i0 + tid < i1) fetch(i0 + tid);
+  for (int it = i0 + tid; it < i1; it += nt) {
+    const int e = en;
+    const uint32_t mask = mn;
+    double xc[$NV * 3];
+    for (int q = 0; q < $NV * 3; ++q) xc[q] = xn[q];
+    int32_t dofs[$ND0];
+    for (int i = 0; i < $ND0; ++i) dofs[i] = dn[i];
+    if (it + nt < i1) fetch(it + nt);
+    const double* w = coeffs + (int64_t)e * $W;
+    double be[$NR];
+    for (int i = 0; i < $NR; ++i) be[i] = 0.0;
+    $TAB(be, w, consts, xc, nullptr, nullptr);
+    for (int i = 0; i < $ND0; ++i) {
+      if (!(mask & (1u << i))) continue;
+      for (int a = 0; a < $BS0; ++a) atomicAdd(&acc[(dofs[i] - r0) * $BS0 + a], be[i * $BS0 + a]);
+    }
+  }
+#else
   for (int it = i0 + tid; it < i1; it += nt) {
     const int e = item_cell[it];            // entity index within the integral (cell)
     const uint32_t mask = item_mask[it];
@@ -578,6 +614,7 @@ $VK
       for (int a = 0; a < $BS0; ++a) atomicAdd(&acc[(node - r0) * $BS0 + a], be[i * $BS0 + a]);
     }
   }
+#endif
   __syncthreads();
   double* out = b + (int64_t)r0 * $BS0;
   if (flags & 1) {

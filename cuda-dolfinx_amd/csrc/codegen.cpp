@@ -416,6 +416,12 @@ static std::string hoist_tables(const std::string& src, std::map<std::string, st
     ls = ls == std::string::npos ? 0 : ls + 1;     // start of the definition's line
     const size_t open = src.find('{', f);
     if (open == std::string::npos) break;
+    const size_t semi0 = src.find(';', f);
+    if (semi0 < open) {                              // a prototype: no body, nothing to hoist
+      out += src.substr(pos, semi0 + 1 - pos);
+      pos = semi0 + 1;
+      continue;
+    }
     // the body's closing brace (table initialisers are balanced; comments skipped)
     int depth = 0;
     size_t close = std::string::npos;

@@ -591,12 +591,16 @@ def bench_c4(args):
     # Newton with an Armijo line search on the energy (F = grad E, J = Hessian,
     # J is SPD, so the Newton direction descends); one graph replay per step
     history, energies, cg_ms = [], [], []
+    converged = False
     for it in range(args.newton):
         g.launch()
         s.synchronize()
         r0, e0 = float(torch.linalg.norm(bt)), float(ev.item())
         history.append(r0)
         energies.append(e0)
+        if r0 <= 1e-10 * history[0]:
+            converged = True
+            break                          # converged: the energy no longer resolves a step
         s.synchronize()
         t_cg = time.perf_counter()
         du, cg_it, cg_res = pcg(A, -bt.clone(), rtol=1e-10, maxiter=args.cg_maxiter)
@@ -614,10 +618,11 @@ def bench_c4(args):
             alpha *= 0.5
         log(f"Newton {it}: |F| = {r0:.3e}, E = {e0:.12e}, CG {cg_it} its (res {cg_res:.1e}, "
             f"{cg_ms[-1]:.3f} ms/it), step {alpha}")
-    g.launch()
-    s.synchronize()
-    history.append(float(torch.linalg.norm(bt)))
-    energies.append(float(ev.item()))
+    if not converged:
+        g.launch()
+        s.synchronize()
+        history.append(float(torch.linalg.norm(bt)))
+        energies.append(float(ev.item()))
 
     nc = mesh.num_cells
     # values written once, b written once (8 n), items (16 B per cell + 2 per interior facet +

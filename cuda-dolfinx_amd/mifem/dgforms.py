@@ -11,7 +11,7 @@ Residual F(u; v) and its Jacobian J(u; du, v) for a DG_k state u:
 
 with mu = mu0 + (eps^2 + |grad u|^2)^((p-2)/2), mu' = (p-2)(eps^2 + |grad u|^2)^((p-4)/2),
 n the unit normal pointing out of the first cell of the facet, [w] = w+ - w-,
-{q} = (q+ + q-)/2, sigma = penalty (10 k^2 by default), h_F the mean of the
+{q} = (q+ + q-)/2, sigma = penalty (30 k^2 by default), h_F the mean of the
 two cells' diameters (largest vertex distance).  The facet terms are those
 of the linear Laplacian, so J is exactly the derivative of F; the constant
 part mu0 (default 1) of the coefficient keeps the cell term at least as
@@ -145,7 +145,10 @@ def _sipg_parts(V, u, f, alpha, penalty, p, eps, mu0):
         raise ValueError("the state u must live on V")
     k, nd = el.degree, el.ndofs
     if penalty is None:
-        penalty = 10.0 * k * k
+        # 30 k^2: with 10 k^2 the SIPG part is not coercive on the jittered
+        # tetrahedra of the benchmark mesh, the energy's Hessian turns
+        # indefinite where the p-term vanishes and Newton stalls
+        penalty = 30.0 * k * k
     mc = max(1, k)                       # cell rule: exact for degree 2k - 1 (grad.grad and f v)
     mf = k + 1                           # facet rule: exact for [u][v] (degree 2k)
     qc, qf = E.tet_collapsed(mc), E.tri_collapsed(mf)

@@ -107,7 +107,20 @@ int main(int argc, char** argv) {
           mf::generate_system_source(J->form, F->form, E ? &E->form : nullptr, 256, 0);
       auto code = mf::rtc_compile(src, "codegen_check", opts, "", nullptr);
       write(argv[2], code);
-      std::cout << "OK " << code.size() << "\n";
+      // constant tables: declared inside the bodies, shared file-scope arrays
+      // in the kernel source, function-local ones left in it
+      auto count = [](const std::string& t, const std::string& key) {
+        size_t n = 0;
+        for (size_t p = t.find(key); p != std::string::npos; p = t.find(key, p + key.size())) ++n;
+        return n;
+      };
+      size_t declared = 0;
+      for (const mf_form* f : {&J->form, &F->form, E ? &E->form : nullptr})
+        if (f)
+          for (auto& I : f->ints) declared += count(I->tab_src, "static const double ");
+      std::cout << "OK " << code.size() << " tables " << declared << " "
+                << count(src, "static __constant__ const double mf_tab_") << " "
+                << count(src, "static const double ") << "\n";
       return 0;
     }
     std::cout << "FAIL usage: codegen_check form <description> <out.co> | batch <out.co> "

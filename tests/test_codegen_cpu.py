@@ -450,6 +450,10 @@ def test_generated_kernels_compile_for_gfx950(tmp_path):
     assert K["q3"]["mf_howner64_cell_0"][1] <= 64
     assert "mf_dgown" in K["sipg"]
     assert K["system"]["mf_dgsys"][1] == 0                      # fused J + F + E, no scratch
+    # the J, F and E bodies' constant tables become shared file-scope arrays
+    # (identical tables once each); none stays function-local
+    declared, shared, local = map(int, outs["system"].split()[-3:])
+    assert 0 < shared < declared and local == 0, (declared, shared, local)
     assert K["sipg"]["mf_dgown"][1] == 0
     lds = int(outs["batch"].split()[-1])
     assert {"mf_owner_batch", "mf_prep_g0", "mf_prep_g1", "mf_prep_g2"} <= set(K["batch"])

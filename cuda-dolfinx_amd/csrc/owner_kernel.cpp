@@ -1009,12 +1009,14 @@ const char* kOwnerUnit = R"(
         const int rbase = BS * BS * lo;
         const uint8_t* kk = &skk[pc * ND];
         const int j0 = u_j0, j1 = u_j1;
-        // scalar trace forms: a few column iterations in flight at once (their
-        // LDS reads of R_ij overlap instead of waiting one by one)
 #ifdef SK_TRACE
-#pragma unroll 4
-#endif
+        // one column j of the unit (kj: its position in the row, from kk)
+        auto column = [&](const int j, const int kj) {
+#define MF_KJ kj
+#else
         for (int j = j0; j < j1; ++j) {
+#define MF_KJ kk[j]
+#endif
           double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #ifdef SK_TRACE
           {
@@ -1058,7 +1060,7 @@ const char* kOwnerUnit = R"(
           double blk[BS * BS];
           sk_combine(G, u_w, consts, blk);
           const uint32_t cb1 = (bits >> (BS * j)) & ((1u << BS) - 1);
-          const int rb = rbase + BS * kk[j];
+          const int rb = rbase + BS * MF_KJ;
           if ((rb0 | cb1) == 0) {
             for (int a = 0; a < BS; ++a)
               for (int bb = 0; bb < BS; ++bb) atomicAdd(&acc[rb + a * rowlen + bb], (mf_acc_t)blk[a * BS + bb]);
@@ -1071,7 +1073,30 @@ const char* kOwnerUnit = R"(
               }
             }
           }
+#undef MF_KJ
+#ifdef SK_TRACE
+        };
+        if (ND % 4 == 0) {
+          // scalar trace forms, ND a multiple of 4: a pair's column positions
+          // start 4-byte aligned, so one LDS word read gives four of them
+          const uint32_t* kw = reinterpret_cast<const uint32_t*>(kk);
+          for (int jw = j0 & ~3; jw < j1; jw += 4) {
+            const uint32_t w4 = kw[jw >> 2];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+              const int j = jw + t;
+              if (j >= j0 && j < j1) column(j, (int)((w4 >> (8 * t)) & 0xFFu));
+            }
+          }
+        } else {
+          // a few column iterations in flight at once (their LDS reads of
+          // R_ij overlap instead of waiting one by one)
+#pragma unroll 4
+          for (int j = j0; j < j1; ++j) column(j, kk[j]);
         }
+#else
+        }
+#endif
 )";
 
 }  // namespace mf

@@ -240,7 +240,16 @@ $OHEAD($OPARAMS)
 #ifdef SK_TRACE
   // reference tensors R_ij (symmetrised, 6 per entry), loaded once per workgroup
   MF_SHARED(mf_rt_t, sRS, ND * ND * 6);
+#if (SK_ND % 4 == 0) && MF_OWNER_F32
+  // FP32, ND a multiple of 4: row i's tensors transposed, sRS[(i * 6 + k) * ND + j],
+  // so one 16-byte LDS read gives component k of four consecutive columns
+  for (int p = threadIdx.x; p < ND * ND * 6; p += NT) {
+    const int i = p / (6 * ND), k = (p / ND) % 6, j = p % ND;
+    sRS[p] = (mf_rt_t)SK_RS[i][j][k];
+  }
+#else
   for (int p = threadIdx.x; p < ND * ND * 6; p += NT) sRS[p] = (mf_rt_t)(&SK_RS[0][0][0])[p];
+#endif
 #elif defined(SK_FULLRT)
   // full reference tensors R_ij[3 r + s] (9 per entry), loaded once per workgroup
   MF_SHARED(double, sRT, ND * ND * 9);
@@ -740,7 +749,14 @@ $OHEAD($OPARAMS)
   __shared__ __attribute__((aligned(16))) uint8_t snb2[2][MAXR];
 #ifdef SK_TRACE
   __shared__ __attribute__((aligned(16))) mf_rt_t sRS[ND * ND * 6];
+#if (SK_ND % 4 == 0) && MF_OWNER_F32
+  for (int p = threadIdx.x; p < ND * ND * 6; p += NT + 64) {
+    const int i = p / (6 * ND), k = (p / ND) % 6, j = p % ND;
+    sRS[p] = (mf_rt_t)SK_RS[i][j][k];
+  }
+#else
   for (int p = threadIdx.x; p < ND * ND * 6; p += NT + 64) sRS[p] = (mf_rt_t)(&SK_RS[0][0][0])[p];
+#endif
 #elif defined(SK_FULLRT)
   __shared__ __attribute__((aligned(16))) double sRT[ND * ND * 9];
   for (int p = threadIdx.x; p < ND * ND * 9; p += NT + 64) sRT[p] = (&SK_RT[0][0][0])[p];
@@ -1010,8 +1026,9 @@ const char* kOwnerUnit = R"(
         const uint8_t* kk = &skk[pc * ND];
         const int j0 = u_j0, j1 = u_j1;
 #ifdef SK_TRACE
-        // one column j of the unit (kj: its position in the row, from kk)
-        auto column = [&](const int j, const int kj) {
+        // one column j of the unit (kj: its position in the row, from kk;
+        // g0: tr G_ij = sum_rs M_rs R_ij[r][s])
+        auto column = [&](const int j, const int kj, const double g0) {
 #define MF_KJ kj
 #else
         for (int j = j0; j < j1; ++j) {
@@ -1019,14 +1036,8 @@ const char* kOwnerUnit = R"(
 #endif
           double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #ifdef SK_TRACE
-          {
-            const mf_rt_t* R = &sRS[(i * ND + j) * 6];
-#if MF_OWNER_F32
-            G[0] = (double)(f00 * R[0] + f11 * R[1] + f22 * R[2] + f01 * R[3] + f02 * R[4] + f12 * R[5]);
-#else
-            G[0] = M00 * R[0] + M11 * R[1] + M22 * R[2] + M01 * R[3] + M02 * R[4] + M12 * R[5];
-#endif
-          }
+          G[0] = g0;
+          (void)j;
 #elif defined(SK_FULLRT)
           {
             const double* R = &sRT[(i * ND + j) * 9];
@@ -1076,23 +1087,44 @@ const char* kOwnerUnit = R"(
 #undef MF_KJ
 #ifdef SK_TRACE
         };
+        // tr G_ij from the tensors in their row-major layout
+        auto trace = [&](const int j) {
+          const mf_rt_t* R = &sRS[(i * ND + j) * 6];
+#if MF_OWNER_F32
+          return (double)(f00 * R[0] + f11 * R[1] + f22 * R[2] + f01 * R[3] + f02 * R[4] + f12 * R[5]);
+#else
+          return M00 * R[0] + M11 * R[1] + M22 * R[2] + M01 * R[3] + M02 * R[4] + M12 * R[5];
+#endif
+        };
         if (ND % 4 == 0) {
           // scalar trace forms, ND a multiple of 4: a pair's column positions
           // start 4-byte aligned, so one LDS word read gives four of them
           const uint32_t* kw = reinterpret_cast<const uint32_t*>(kk);
           for (int jw = j0 & ~3; jw < j1; jw += 4) {
             const uint32_t w4 = kw[jw >> 2];
+#if (SK_ND % 4 == 0) && MF_OWNER_F32
+            // the transposed FP32 tensors: six 16-byte reads for four columns
+            typedef float mf_f4 __attribute__((ext_vector_type(4)));
+            const mf_f4* R4 = reinterpret_cast<const mf_f4*>(&sRS[i * 6 * ND + jw]);
+            const mf_f4 r0 = R4[0], r1 = R4[ND / 4], r2 = R4[2 * ND / 4];
+            const mf_f4 r3 = R4[3 * ND / 4], r4 = R4[4 * ND / 4], r5 = R4[5 * ND / 4];
+            const mf_f4 tg = f00 * r0 + f11 * r1 + f22 * r2 + f01 * r3 + f02 * r4 + f12 * r5;
+#endif
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
               const int j = jw + t;
-              if (j >= j0 && j < j1) column(j, (int)((w4 >> (8 * t)) & 0xFFu));
+#if (SK_ND % 4 == 0) && MF_OWNER_F32
+              if (j >= j0 && j < j1) column(j, (int)((w4 >> (8 * t)) & 0xFFu), (double)tg[t]);
+#else
+              if (j >= j0 && j < j1) column(j, (int)((w4 >> (8 * t)) & 0xFFu), trace(j));
+#endif
             }
           }
         } else {
           // a few column iterations in flight at once (their LDS reads of
           // R_ij overlap instead of waiting one by one)
 #pragma unroll 4
-          for (int j = j0; j < j1; ++j) column(j, kk[j]);
+          for (int j = j0; j < j1; ++j) column(j, kk[j], trace(j));
         }
 #else
         }

@@ -15,6 +15,8 @@
 //  * cell integrals with a staged description also get the cooperative
 //    owner-computes kernel (owner_kernel.inc), which writes each CSR row
 //    segment exactly once from LDS.
+#include <cctype>
+#include <cstdio>
 #include <sstream>
 
 #include "mf_internal.h"

@@ -1036,7 +1036,12 @@ const char* kOwnerUnit = R"(
 #endif
           double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #ifdef SK_TRACE
+          // the combine of these forms reads G[0] + G[4] + G[8]: with the
+          // other two at -0.0 both additions are exact identities the
+          // compiler drops (adding +0.0 is not: -0.0 + 0.0 = +0.0)
           G[0] = g0;
+          G[4] = -0.0;
+          G[8] = -0.0;
           (void)j;
 #elif defined(SK_FULLRT)
           {

@@ -989,3 +989,32 @@ def test_triangle_jacobians_are_residual_derivatives(which):
     fm = cpu.OracleAssembler(L).assemble_vector()
     state.x[:] = x0
     assert np.linalg.norm((fp - fm) / (2 * h) - J @ d) <= 1e-7 * max(1.0, np.linalg.norm(J @ d))
+
+
+def test_sipg_default_penalty_is_coercive():
+    """The Jacobian of the SIPG p-Laplacian at u = 0 (the linear SIPG operator
+    with the Robin term) is positive definite with the default penalty 30 k^2
+    on the jittered 12^3 x 6 mesh, and indefinite with 10 k^2 -- the penalty
+    with which Newton stalled once the iterate's gradients became small."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from mifem.dgforms import sipg_plaplace
+    P = problems.dg_plaplace(12)
+    u = P.extras["u"]
+    u.x[:] = 0.0
+    negative = {}
+    for pen in (None, 10.0):
+        a, _ = sipg_plaplace(P.V, u, f=1.0, alpha=2.0, p=3.0, eps=1e-3, penalty=pen)
+        O = cpu.OracleAssembler(a)
+        rp, cols, _, _ = O.sparsity()
+        n = P.V.size
+        A = sp.csr_matrix((O.assemble_matrix(rp, cols), cols, rp), shape=(n, n))
+        assert abs(A - A.T).max() <= 1e-13 * abs(A).max()          # symmetric
+        # inertia (Sylvester): a symmetric permutation P A P^T = L D L^T, diagonal
+        # pivots only; the number of negative pivots is the number of negative
+        # eigenvalues
+        lu = spla.splu(A.tocsc(), permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0,
+                       options=dict(SymmetricMode=True))
+        assert np.array_equal(lu.perm_r, lu.perm_c)
+        negative[pen] = int((lu.U.diagonal() < 0).sum())
+    assert negative[None] == 0 and negative[10.0] > 0, negative

@@ -235,8 +235,17 @@ def autotune_batch(forms, asm, mats, bcs, base_options=None, candidates=None, re
         asm.synchronize()
         return (time.perf_counter() - t0) / reps * 1e3, batch, fs
 
-    choice = [0] * n
-    best_ms, best_batch, best_fs = time_batch(choice)
+    # start from the first candidate every member can take
+    for c0 in range(len(cands)):
+        choice = [c0] * n
+        try:
+            best_ms, best_batch, best_fs = time_batch(choice)
+            break
+        except (RuntimeError, ValueError) as e:
+            if verbose:
+                print(f"[autotune] batch {choice}: unavailable ({e})", file=sys.stderr, flush=True)
+    else:
+        raise RuntimeError("no batch shape could run")
     timings = [{"members": list(choice), "ms": best_ms}]
     if verbose:
         print(f"[autotune] batch {choice}: {best_ms:.3f} ms", file=sys.stderr, flush=True)

@@ -395,6 +395,7 @@ $OHEAD($OPARAMS)
       const int prn = mf_first_pair(n2.y - n2.x);
       if (prn >= 0) pvn = (uint32_t)pair_list[n2.x + prn];
     }
+    __builtin_amdgcn_s_setprio(0);
     {
       const int np = npc;
 #if $OBAL
@@ -453,6 +454,11 @@ $UNIT
       mf_load_unit(pvn);
     }
 
+    // From the write-out to the next compute phase (stores, the next chunk's
+    // list staging and prefetch loads) the wave issues at a raised priority,
+    // so the memory traffic of one workgroup's skeleton goes out ahead of
+    // the LDS atomics of the workgroups that compute beside it on the CU.
+    __builtin_amdgcn_s_setprio(2);
     // write the segment once and re-zero what was read
     double* out = values + (int64_t)(uint32_t)base;
 #if MF_OWNER_F32 == 1

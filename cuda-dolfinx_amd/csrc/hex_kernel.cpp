@@ -150,6 +150,10 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
   for (int ib = i0; ib < i1; ib += NB) {
     const int nb = min(NB, i1 - ib);
     __syncthreads();                                 // previous batch done with srec / sD
+    // the batch's serial part (record staging, pair range, FP64 geometry)
+    // holds every wave of this workgroup at the next barrier, so it issues at
+    // a raised priority over the other workgroup's matrix-core contraction
+    __builtin_amdgcn_s_setprio(2);
     {
       int4* dst = reinterpret_cast<int4*>(srec);
 #pragma clang loop unroll(full)
@@ -212,6 +216,7 @@ $KNAME(const int32_t nchunks, const int32_t* __restrict__ chunk_meta,
       D[7] = (real)0;
     }
     __syncthreads();
+    __builtin_amdgcn_s_setprio(0);
     if (ib + NB < i1) {
       const int ibx = ib + NB, nbx = min(NB, i1 - ib - NB);
 #pragma clang loop unroll(full)
